@@ -1,0 +1,122 @@
+/* mwgreen: MI355X-native multiwavelet Green's-function apply, C ABI.
+ *
+ * Drop-in for the multiwavelet convolution that MRChem calls through
+ *   mrcpp::apply(prec, out, ConvolutionOperator&, inp, maxIter, absPrec)
+ *     src/qmoperators/two_electron/CoulombPotential.cpp:131,156
+ *     src/scf_solver/HelmholtzVector.cpp:149,154
+ *   mrcpp::apply(prec, out, oper, inp, precTrees, maxIter, absPrec)
+ *     src/qmoperators/two_electron/ExchangePotential.cpp:200,204
+ * together with the operator constructors
+ *   PoissonOperator(MRA, build_prec)        src/driver.cpp:1131,1283
+ *   HelmholtzOperator(MRA, mu, build_prec)  src/scf_solver/HelmholtzVector.cpp:144
+ * and the FunctionTree<3> node data MRChem reads (src/mrdft/Functional.cpp:212-232).
+ *
+ * Conventions
+ *   - All functions return MWG_OK (0) or a negative error code; the message of
+ *     the last failure on the calling thread is returned by mwg_last_error().
+ *     No call aborts the process (upstream MRCPP aborts through MSG_ABORT).
+ *   - Node keys are (depth, lx, ly, lz): depth 0 is the root scale, and
+ *     translations are absolute (root boxes have corner..corner+boxes-1).
+ *   - Node blocks hold 8 (k+1)^3 doubles in compressed form: component t
+ *     (bit d set = wavelet along direction d), then z, y, x with x fastest.
+ *   - Handles are opaque pointers; none may be shared between threads while a
+ *     call on it runs.  An apply takes `inp` and the operator read-only.
+ */
+#ifndef MWGREEN_H
+#define MWGREEN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MWG_OK 0
+#define MWG_ERR_ARG (-1)
+#define MWG_ERR_DEVICE (-2)
+#define MWG_ERR_RUNTIME (-3)
+
+#define MWG_BASIS_INTERPOLATING 0
+#define MWG_BASIS_LEGENDRE 1
+
+/* Arithmetic of the per-node contractions. */
+#define MWG_MODE_FP64 0
+
+typedef struct mwg_context_s *mwg_context;
+typedef struct mwg_mra_s *mwg_mra;
+typedef struct mwg_operator_s *mwg_operator;
+typedef struct mwg_tree_s *mwg_tree;
+
+const char *mwg_last_error(void);
+
+/* Device context: one HIP device, one stream, cached operator tables.
+ * Replaces the OpenMP/MPI environment mrcpp::apply runs in (src/mrenv.cpp:129-135). */
+int mwg_context_create(int device, mwg_context *ctx);
+int mwg_context_destroy(mwg_context ctx);
+
+/* MultiResolutionAnalysis<3>: BoundingBox(min_scale, corner, boxes) with a
+ * scaling factor, basis of the given order and type, and max depth.
+ * Mirrors mrenv::init_mra (src/mrenv.cpp:99-127). */
+int mwg_mra_create(int order, int basis_type, int root_scale, const int *corner, const int *boxes,
+                   double scaling_factor, int max_depth, mwg_mra *mra);
+int mwg_mra_destroy(mwg_mra mra);
+/* Two-scale filter [H0 H1; G0 G1], 2(k+1) x 2(k+1) row-major. */
+int mwg_mra_filter(mwg_mra mra, double *F);
+
+/* PoissonOperator(MRA, prec): kernel 1/r (src/driver.cpp:1131). */
+int mwg_poisson_create(mwg_mra mra, double build_prec, mwg_operator *op);
+/* HelmholtzOperator(MRA, mu, prec): kernel exp(-mu r)/r
+ * (src/scf_solver/HelmholtzVector.cpp:144). */
+int mwg_helmholtz_create(mwg_mra mra, double mu, double build_prec, mwg_operator *op);
+/* Operator from explicit Gaussian terms sum_m beta_m exp(-alpha_m r^2) (no
+ * counterpart in MRChem's calls; used for verification). */
+int mwg_gaussian_operator_create(mwg_mra mra, int n_terms, const double *alpha, const double *beta,
+                                 int max_depth, mwg_operator *op);
+int mwg_operator_destroy(mwg_operator op);
+int mwg_operator_info(mwg_operator op, int *n_terms, int *n_depth, int *kp1, double *r_min, double *r_max,
+                      double *fit_err);
+int mwg_operator_terms(mwg_operator op, double *alpha, double *beta);
+/* Flattened tables: per (term m, depth d) at index m*n_depth+d the half-width
+ * L (-1: empty), the offset of block (c=0, l=-L) in `blocks` and of its norm
+ * in `norms`; blocks are [c][l+L][i][j] with c = 2*out_wavelet + in_wavelet. */
+int mwg_operator_table_sizes(mwg_operator op, long long *n_block_values, long long *n_norm_values);
+int mwg_operator_tables(mwg_operator op, int *L, long long *boff, long long *noff, double *blocks,
+                        double *norms);
+
+/* FunctionTree<3>. */
+int mwg_tree_create(mwg_mra mra, mwg_tree *tree);
+int mwg_tree_destroy(mwg_tree tree);
+/* Adaptive projection of sum_i c_i (a_i/pi)^{3/2} exp(-a_i |r - R_i|^2). */
+int mwg_tree_project_gaussians(mwg_tree tree, double prec, int abs_prec, int max_depth, int n, const double *coef,
+                               const double *exponent, const double *center);
+int mwg_tree_size(mwg_tree tree, int *n_nodes);
+/* Nodes in tree order (parents before children); any array may be NULL. */
+int mwg_tree_get(mwg_tree tree, int *depth, long long *l, double *coefs);
+/* Replace the tree by the given complete set of nodes (every parent present). */
+int mwg_tree_set(mwg_tree tree, int n_nodes, const int *depth, const long long *l, const double *coefs);
+int mwg_tree_square_norm(mwg_tree tree, double *sq_norm);
+int mwg_tree_dot(mwg_tree a, mwg_tree b, double *dot);
+int mwg_tree_eval(mwg_tree tree, int n_points, const double *r, double *values);
+int mwg_tree_rescale(mwg_tree tree, double factor);
+/* Screening threshold of every node used by the last apply into this tree. */
+int mwg_tree_thresholds(mwg_tree tree, double *thr);
+
+/* mrcpp::apply(prec, out, oper, inp, max_iter, abs_prec): adaptive output
+ * grid grown from the roots, compressed output with norm set. */
+int mwg_apply(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp, int max_iter,
+              int abs_prec);
+/* mrcpp::apply(prec, out, oper, inp, precTrees, max_iter, abs_prec). */
+int mwg_apply_prec_trees(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp,
+                         const mwg_tree *prec_trees, int n_prec, int max_iter, int abs_prec);
+/* Frozen-grid apply: out's current grid is kept, thr[g] (tree order) is the
+ * screening threshold of node g; out receives the compressed result. */
+int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode);
+
+/* Statistics of the last apply on this context: [0] kernel milliseconds,
+ * [1] output nodes computed, [2] input slots, [3] generated slots,
+ * [4] host plan milliseconds, [5] kernel launches. */
+int mwg_last_apply_stats(mwg_context ctx, double *stats, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MWGREEN_H */

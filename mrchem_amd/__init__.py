@@ -1,0 +1,330 @@
+"""mrchem_amd: MI355X-native multiwavelet Green's-function apply.
+
+Host-side mirror of the MRCPP interface MRChem uses for its Green's-function
+applications, over the C ABI in include/mwgreen.h (libmwgreen.so):
+
+    MultiResolutionAnalysis   mrcpp::MultiResolutionAnalysis<3>   (src/mrenv.cpp:99-127)
+    PoissonOperator           mrcpp::PoissonOperator(MRA, prec)    (src/driver.cpp:1131)
+    HelmholtzOperator         mrcpp::HelmholtzOperator(MRA, mu, prec)
+                                                   (src/scf_solver/HelmholtzVector.cpp:144)
+    FunctionTree              mrcpp::FunctionTree<3>
+    apply(prec, out, oper, inp, max_iter=-1, abs_prec=False, prec_trees=None)
+                              mrcpp::apply            (CoulombPotential.cpp:131,
+                                                       ExchangePotential.cpp:200,
+                                                       HelmholtzVector.cpp:149)
+
+Every compute call runs the HIP kernels; there is no CPU fallback.  Importing
+this module fails when libmwgreen.so has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["MultiResolutionAnalysis", "PoissonOperator", "HelmholtzOperator", "GaussianOperator",
+           "FunctionTree", "Context", "apply", "apply_fixed", "MWGError", "lib", "LIB_PATH"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmwgreen.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libmwgreen.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+lib = ctypes.CDLL(LIB_PATH)
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_ll = ctypes.c_longlong
+_p = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_llp = ctypes.POINTER(ctypes.c_longlong)
+
+
+def _sig(name, *args):
+    f = getattr(lib, name)
+    f.argtypes = list(args)
+    f.restype = ctypes.c_int
+    return f
+
+
+lib.mwg_last_error.restype = ctypes.c_char_p
+lib.mwg_last_error.argtypes = []
+_sig("mwg_context_create", _i, ctypes.POINTER(_p))
+_sig("mwg_context_destroy", _p)
+_sig("mwg_mra_create", _i, _i, _i, _ip, _ip, _d, _i, ctypes.POINTER(_p))
+_sig("mwg_mra_destroy", _p)
+_sig("mwg_mra_filter", _p, _dp)
+_sig("mwg_poisson_create", _p, _d, ctypes.POINTER(_p))
+_sig("mwg_helmholtz_create", _p, _d, _d, ctypes.POINTER(_p))
+_sig("mwg_gaussian_operator_create", _p, _i, _dp, _dp, _i, ctypes.POINTER(_p))
+_sig("mwg_operator_destroy", _p)
+_sig("mwg_operator_info", _p, _ip, _ip, _ip, _dp, _dp, _dp)
+_sig("mwg_operator_terms", _p, _dp, _dp)
+_sig("mwg_operator_table_sizes", _p, _llp, _llp)
+_sig("mwg_operator_tables", _p, _ip, _llp, _llp, _dp, _dp)
+_sig("mwg_tree_create", _p, ctypes.POINTER(_p))
+_sig("mwg_tree_destroy", _p)
+_sig("mwg_tree_project_gaussians", _p, _d, _i, _i, _i, _dp, _dp, _dp)
+_sig("mwg_tree_size", _p, _ip)
+_sig("mwg_tree_get", _p, _ip, _llp, _dp)
+_sig("mwg_tree_set", _p, _i, _ip, _llp, _dp)
+_sig("mwg_tree_square_norm", _p, _dp)
+_sig("mwg_tree_dot", _p, _p, _dp)
+_sig("mwg_tree_eval", _p, _i, _dp, _dp)
+_sig("mwg_tree_rescale", _p, _d)
+_sig("mwg_tree_thresholds", _p, _dp)
+_sig("mwg_apply", _p, _d, _p, _p, _p, _i, _i)
+_sig("mwg_apply_prec_trees", _p, _d, _p, _p, _p, ctypes.POINTER(_p), _i, _i, _i)
+_sig("mwg_apply_fixed", _p, _p, _p, _p, _dp, _i)
+_sig("mwg_last_apply_stats", _p, _dp, _i)
+
+MODES = {"fp64": 0}
+
+
+class MWGError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise MWGError("mwgreen error %d: %s" % (rc, lib.mwg_last_error().decode()))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class Context:
+    """One HIP device and stream (the OpenMP environment of the reference apply)."""
+
+    _default = None
+
+    def __init__(self, device=0):
+        h = _p()
+        _check(lib.mwg_context_create(device, ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context(0)
+        return cls._default
+
+    def last_stats(self):
+        s = np.zeros(8)
+        _check(lib.mwg_last_apply_stats(self._h, _ptr(s, _d), 8))
+        return {"kernel_ms": s[0], "out_nodes": int(s[1]), "in_slots": int(s[2]), "generated": int(s[3]),
+                "plan_ms": s[4], "launches": int(s[5])}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mwg_context_destroy(h)
+            self._h = None
+
+
+class MultiResolutionAnalysis:
+    """MRA: BoundingBox(min_scale, corner, boxes) x scaling factor, basis order/type, max depth."""
+
+    def __init__(self, order, basis="interpolating", min_scale=-4, corner=(-1, -1, -1), boxes=(2, 2, 2),
+                 scaling_factor=1.0, max_depth=20):
+        btype = {"interpolating": 0, "legendre": 1}[basis]
+        c = np.ascontiguousarray(corner, dtype=np.int32)
+        b = np.ascontiguousarray(boxes, dtype=np.int32)
+        h = _p()
+        _check(lib.mwg_mra_create(order, btype, min_scale, _ptr(c, _i), _ptr(b, _i), scaling_factor, max_depth,
+                                  ctypes.byref(h)))
+        self._h = h
+        self.order, self.basis, self.min_scale = order, basis, min_scale
+        self.corner, self.boxes = tuple(corner), tuple(boxes)
+        self.scaling_factor, self.max_depth = scaling_factor, max_depth
+        self.kp1 = order + 1
+
+    @property
+    def block(self):
+        return 8 * self.kp1 ** 3
+
+    def filter(self):
+        F = np.zeros((2 * self.kp1, 2 * self.kp1))
+        _check(lib.mwg_mra_filter(self._h, _ptr(F, _d)))
+        return F
+
+    def box_size(self, depth):
+        return self.scaling_factor * 2.0 ** (-(self.min_scale + depth))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mwg_mra_destroy(h)
+            self._h = None
+
+
+class _Operator:
+    def __init__(self, mra, h):
+        self.mra = mra
+        self._h = h
+
+    def info(self):
+        n, nd, kp1 = _i(), _i(), _i()
+        rmin, rmax, err = _d(), _d(), _d()
+        _check(lib.mwg_operator_info(self._h, ctypes.byref(n), ctypes.byref(nd), ctypes.byref(kp1),
+                                     ctypes.byref(rmin), ctypes.byref(rmax), ctypes.byref(err)))
+        return {"n_terms": n.value, "n_depth": nd.value, "kp1": kp1.value, "r_min": rmin.value,
+                "r_max": rmax.value, "fit_err": err.value}
+
+    def size(self):
+        return self.info()["n_terms"]
+
+    def terms(self):
+        n = self.size()
+        a, b = np.zeros(n), np.zeros(n)
+        _check(lib.mwg_operator_terms(self._h, _ptr(a, _d), _ptr(b, _d)))
+        return a, b
+
+    def tables(self):
+        info = self.info()
+        nb, nn = _ll(), _ll()
+        _check(lib.mwg_operator_table_sizes(self._h, ctypes.byref(nb), ctypes.byref(nn)))
+        k = info["n_terms"] * info["n_depth"]
+        L = np.zeros(k, dtype=np.int32)
+        boff = np.zeros(k, dtype=np.int64)
+        noff = np.zeros(k, dtype=np.int64)
+        blocks = np.zeros(max(nb.value, 1))
+        norms = np.zeros(max(nn.value, 1))
+        _check(lib.mwg_operator_tables(self._h, _ptr(L, _i), _ptr(boff, _ll), _ptr(noff, _ll), _ptr(blocks, _d),
+                                       _ptr(norms, _d)))
+        return {"M": info["n_terms"], "ndepth": info["n_depth"], "L": L, "boff": boff, "noff": noff,
+                "blocks": blocks, "norms": norms}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mwg_operator_destroy(h)
+            self._h = None
+
+
+class PoissonOperator(_Operator):
+    """Separated-Gaussian NS operator of the kernel 1/r."""
+
+    def __init__(self, mra, prec):
+        h = _p()
+        _check(lib.mwg_poisson_create(mra._h, prec, ctypes.byref(h)))
+        super().__init__(mra, h)
+        self.prec = prec
+
+
+class HelmholtzOperator(_Operator):
+    """Separated-Gaussian NS operator of the bound-state kernel exp(-mu r)/r."""
+
+    def __init__(self, mra, mu, prec):
+        h = _p()
+        _check(lib.mwg_helmholtz_create(mra._h, mu, prec, ctypes.byref(h)))
+        super().__init__(mra, h)
+        self.mu, self.prec = mu, prec
+
+
+class GaussianOperator(_Operator):
+    """NS operator of an explicit kernel sum_m beta_m exp(-alpha_m r^2)."""
+
+    def __init__(self, mra, alpha, beta, max_depth=-1):
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        h = _p()
+        _check(lib.mwg_gaussian_operator_create(mra._h, len(a), _ptr(a, _d), _ptr(b, _d), max_depth,
+                                                ctypes.byref(h)))
+        super().__init__(mra, h)
+
+
+class FunctionTree:
+    """Adaptive 2^3-tree of compressed 8(k+1)^3 blocks (mrcpp::FunctionTree<3>)."""
+
+    def __init__(self, mra):
+        h = _p()
+        _check(lib.mwg_tree_create(mra._h, ctypes.byref(h)))
+        self._h = h
+        self.mra = mra
+
+    def project_gaussians(self, prec, coef, exponent, center, abs_prec=False, max_depth=-1):
+        c = np.ascontiguousarray(coef, dtype=np.float64).ravel()
+        e = np.ascontiguousarray(exponent, dtype=np.float64).ravel()
+        r = np.ascontiguousarray(center, dtype=np.float64).reshape(-1, 3)
+        _check(lib.mwg_tree_project_gaussians(self._h, prec, int(abs_prec), max_depth, len(c), _ptr(c, _d),
+                                              _ptr(e, _d), _ptr(r, _d)))
+        return self
+
+    def size(self):
+        n = _i()
+        _check(lib.mwg_tree_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def nodes(self):
+        n = self.size()
+        depth = np.zeros(n, dtype=np.int32)
+        l = np.zeros((n, 3), dtype=np.int64)
+        coefs = np.zeros((n, self.mra.block))
+        _check(lib.mwg_tree_get(self._h, _ptr(depth, _i), _ptr(l, _ll), _ptr(coefs, _d)))
+        return depth, l, coefs
+
+    def set_nodes(self, depth, l, coefs=None):
+        depth = np.ascontiguousarray(depth, dtype=np.int32)
+        l = np.ascontiguousarray(l, dtype=np.int64).reshape(-1, 3)
+        n = len(depth)
+        if coefs is None:
+            coefs = np.zeros((n, self.mra.block))
+        coefs = np.ascontiguousarray(coefs, dtype=np.float64).reshape(n, self.mra.block)
+        _check(lib.mwg_tree_set(self._h, n, _ptr(depth, _i), _ptr(l, _ll), _ptr(coefs, _d)))
+        return self
+
+    def square_norm(self):
+        s = _d()
+        _check(lib.mwg_tree_square_norm(self._h, ctypes.byref(s)))
+        return s.value
+
+    def norm(self):
+        return float(np.sqrt(max(self.square_norm(), 0.0)))
+
+    def dot(self, other):
+        s = _d()
+        _check(lib.mwg_tree_dot(self._h, other._h, ctypes.byref(s)))
+        return s.value
+
+    def evaluate(self, points):
+        r = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        v = np.zeros(len(r))
+        _check(lib.mwg_tree_eval(self._h, len(r), _ptr(r, _d), _ptr(v, _d)))
+        return v
+
+    def rescale(self, f):
+        _check(lib.mwg_tree_rescale(self._h, f))
+        return self
+
+    def thresholds(self):
+        thr = np.zeros(self.size())
+        _check(lib.mwg_tree_thresholds(self._h, _ptr(thr, _d)))
+        return thr
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mwg_tree_destroy(h)
+            self._h = None
+
+
+def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False, prec_trees=None, ctx=None):
+    """mrcpp::apply(prec, out, oper, inp[, precTrees], maxIter, absPrec) on the GPU."""
+    ctx = ctx or Context.default()
+    if prec_trees:
+        arr = (_p * len(prec_trees))(*[t._h for t in prec_trees])
+        _check(lib.mwg_apply_prec_trees(ctx._h, prec, out._h, oper._h, inp._h, arr, len(prec_trees), max_iter,
+                                        int(abs_prec)))
+    else:
+        _check(lib.mwg_apply(ctx._h, prec, out._h, oper._h, inp._h, max_iter, int(abs_prec)))
+    return out
+
+
+def apply_fixed(out, oper, inp, thr, mode="fp64", ctx=None):
+    """Frozen-grid apply: out keeps its grid; thr[g] is node g's screening threshold."""
+    ctx = ctx or Context.default()
+    t = np.ascontiguousarray(thr, dtype=np.float64)
+    if t.size != out.size():
+        raise ValueError("one threshold per output node is required")
+    _check(lib.mwg_apply_fixed(ctx._h, out._h, oper._h, inp._h, _ptr(t, _d), MODES[mode]))
+    return out

@@ -1,0 +1,224 @@
+// Host-side multiwavelet machinery shared by the C ABI and the HIP apply.
+//
+// Everything here is FP64 C++17.  It restates the published MRCPP design
+// (Alpert multiwavelets, Beylkin-Coifman-Rokhlin non-standard operator form,
+// separated Gaussian expansions of 1/r and exp(-mu r)/r) that MRChem calls
+// through mrcpp::apply (src/qmoperators/two_electron/CoulombPotential.cpp:131,
+// src/qmoperators/two_electron/ExchangePotential.cpp:200,
+// src/scf_solver/HelmholtzVector.cpp:149).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace mwg {
+
+// ---------------------------------------------------------------- math utils
+// n-point Gauss-Legendre rule mapped to [0,1] (nodes ascending, weights sum to 1).
+void gauss_legendre01(int n, double *x, double *w);
+// Normalised Legendre functions on [0,1]: P~_p(x) = sqrt(2p+1) P_p(2x-1), p < n.
+void legendre01(int n, double x, double *out);
+// Moments M_p = int_0^1 P~_p(v) exp(-A (v-c)^2) dv for p < n, accurate to ~1e-16
+// relative to int_0^1 |P~_p| exp(...) for any A >= 0 and real c.
+void gauss_moments01(int n, double A, double c, double *out);
+
+// ------------------------------------------------------------------ basis
+enum BasisType { BASIS_INTERPOLATING = 0, BASIS_LEGENDRE = 1 };
+
+struct Basis {
+    int k = 0;    // polynomial order
+    int kp1 = 0;  // functions per direction
+    int type = BASIS_INTERPOLATING;
+    // phi_i(x) = sum_p L2B[i*kp1+p] P~_p(x)  (orthogonal matrix)
+    std::vector<double> L2B;
+    // Two-scale filter, 2kp1 x 2kp1 row-major: rows [H0 H1 ; G0 G1].
+    // [s^n ; d^n] = F [s^{n+1}_{2l} ; s^{n+1}_{2l+1}]   (F orthogonal)
+    std::vector<double> F;
+    // Cross-correlation: psi_ij(u) = int phi_i(x) phi_j(x-u) dx restricted to
+    // u in [0,1] equals sum_p C[(i*kp1+j)*2kp1 + p] P~_p(u), p < 2kp1.
+    std::vector<double> C;
+    void eval(double x, double *out) const;  // phi_i(x), x in [0,1]
+};
+Basis make_basis(int k, int type);
+
+// ------------------------------------------------------------------- MRA
+struct MRA {
+    Basis basis;
+    int n0 = 0;                // root scale
+    int corner[3] = {-1, -1, -1};
+    int boxes[3] = {2, 2, 2};
+    double sfac = 1.0;         // physical side of a scale-0 box
+    int max_depth = 20;        // deepest depth (scale n0 + max_depth)
+    int kp1() const { return basis.kp1; }
+    int kp1_3() const { return basis.kp1 * basis.kp1 * basis.kp1; }
+    int block() const { return 8 * kp1_3(); }
+    double box_size(int depth) const;        // physical side at depth
+    double world_lo(int d) const;            // physical lower bound along d
+    double world_hi(int d) const;
+    long long tlo(int depth, int d) const { return (long long)corner[d] << depth; }
+    long long thi(int depth, int d) const { return (long long)(corner[d] + boxes[d]) << depth; }  // exclusive
+    bool in_world(int depth, const long long *l) const;
+};
+
+// --------------------------------------------------------- kernel fitting
+struct GaussTerm {
+    double alpha;  // exponent (physical units)
+    double beta;   // 3D coefficient
+};
+// Separated expansion K(r) ~= sum_m beta_m exp(-alpha_m r^2) with
+// |K_fit(r) - K(r)| * r <= eps on [r_min, r_max].  mu == 0 gives 1/r,
+// mu > 0 gives exp(-mu r)/r.
+std::vector<GaussTerm> fit_kernel(double mu, double eps, double r_min, double r_max,
+                                  double *achieved_err = nullptr);
+double kernel_fit_error(const std::vector<GaussTerm> &t, double mu, double r_min, double r_max,
+                        int npts);
+
+// ----------------------------------------------------- operator tables
+// One separated term: per depth, the four non-standard 1D components
+// c = 2*gt + ft (0: s<-s, 1: s<-d, 2: d<-s, 3: d<-d) for translations
+// l in [-L, L], each a kp1 x kp1 block O[i][j] = <out_i | K_m | in_j>.
+struct OperDepth {
+    int L = -1;                  // -1: nothing stored at this depth
+    std::vector<double> blocks;  // [c][l+L][i][j]
+    std::vector<double> norms;   // [c][l+L] Frobenius norms
+};
+struct OperTerm {
+    double alpha = 0, beta = 0, coef1d = 0;
+    std::vector<OperDepth> depth;
+};
+struct Operator {
+    const MRA *mra = nullptr;
+    double mu = 0.0;          // 0 for Poisson
+    double build_prec = 0.0;
+    double r_min = 0, r_max = 0, fit_err = 0;
+    int max_depth = 0;
+    std::vector<OperTerm> terms;
+    int size() const { return (int)terms.size(); }
+};
+// Build the operator for kernel exp(-mu r)/r (mu = 0: 1/r).
+void build_operator(Operator &op, const MRA &mra, double mu, double build_prec, int max_depth);
+// Tables for explicitly given Gaussian terms (tests and custom kernels).
+// Blocks below tau relative to the term's largest block at that depth are not
+// stored (see build_term_depth).
+void build_operator_terms(Operator &op, const MRA &mra, const std::vector<GaussTerm> &g, int max_depth,
+                          double tau = 1e-14);
+// Direct T-block at depth for translation l (used by tests and construction).
+void t_block(const MRA &mra, double alpha, double coef1d, int depth, long long l, double *out);
+
+// ----------------------------------------------------------------- trees
+struct NodeKey {
+    int depth;
+    long long l[3];
+};
+uint64_t morton3(uint64_t x, uint64_t y, uint64_t z);  // 21 bits per dimension
+struct KeyHash {
+    size_t operator()(const std::pair<int, uint64_t> &k) const {
+        return std::hash<uint64_t>()(k.second * 0x9E3779B97F4A7C15ULL ^ (uint64_t)k.first);
+    }
+};
+
+struct Tree {
+    const MRA *mra = nullptr;
+    std::vector<NodeKey> keys;
+    std::vector<int> parent;   // -1 for roots
+    std::vector<int> child0;   // first of 8 contiguous children, -1 for leaves
+    std::vector<double> coefs; // nodes x block; compressed [s, d1..d7] at every node
+    std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> index;
+    double sqnorm = 0.0;
+
+    int size() const { return (int)keys.size(); }
+    uint64_t morton(const NodeKey &k) const;
+    int find(int depth, const long long *l) const;
+    int add_node(const NodeKey &k, int par);   // coefs zero-initialised
+    void make_roots();
+    int split(int node);                       // create 8 children, returns child0
+    double *node(int i) { return coefs.data() + (size_t)i * mra->block(); }
+    const double *node(int i) const { return coefs.data() + (size_t)i * mra->block(); }
+    void clear();
+    int max_depth() const;
+    int n_leaves() const;
+};
+
+// 3D two-scale transforms of one node block (8 components of kp1^3, x fastest).
+// compress: in = 8 child scaling blocks (child t = bx + 2 by + 4 bz),
+//           out = [s, d1..d7] (component bit d set = wavelet along d).
+void mw_compress(const Basis &b, const double *in, double *out);
+void mw_reconstruct(const Basis &b, const double *in, double *out);
+void component_norms(const MRA &mra, const double *blk, double *norms8);
+double tree_square_norm(const Tree &t);
+double tree_dot(const Tree &a, const Tree &b);
+// bottom-up: recompute every branch node's [s,d] from its children's s parts
+void tree_compress(Tree &t);
+// top-down: add each branch node's reconstruction into its children's s part
+void tree_topdown_add(Tree &t);
+
+// Gaussian sums: f(r) = sum_i c_i (a_i/pi)^{3/2} exp(-a_i |r - R_i|^2)
+struct GaussSum {
+    std::vector<double> coef, exp, pos;  // pos: 3 per term
+    int size() const { return (int)coef.size(); }
+    double eval(const double *r) const;
+    double square_norm() const;
+};
+// Scaling coefficients of f on node (depth, l), kp1^3 block (exact integrals).
+void project_scaling(const MRA &mra, const GaussSum &f, int depth, const long long *l, double *out);
+// Adaptive projection: split while ||d|| > prec * tnorm * 2^{-(n+1)/2}.
+void project_adaptive(Tree &t, const GaussSum &f, double prec, bool abs_prec, int max_depth);
+// Evaluate a compressed tree at a physical point.
+double tree_eval(const Tree &t, const double *r);
+
+// Split test used by projection and by the adaptive apply.
+bool split_check(const MRA &mra, int depth, double wnorm, double prec, double tnorm);
+
+// ----------------------------------------------------------- apply plan
+// Everything the per-node calculator needs, laid out flat: the output nodes
+// (one refinement level or a whole fixed grid), the input nodes of their
+// bands (real or generated by reconstruction), the dense band table, and the
+// screening thresholds.  Built on the host; consumed by the HIP kernels.
+struct ApplyPlan {
+    int kp1 = 0, ndepth = 0, M = 0;
+    // output nodes (in the order of out_idx)
+    std::vector<int> out_node;      // node index in output tree
+    std::vector<int> out_depth;
+    std::vector<double> out_thr;    // screening threshold per output node
+    std::vector<long long> band_off;  // offset of this node's band table entries
+    std::vector<int> band_w;        // band half-width W (band has (2W+1)^3 entries)
+    std::vector<int> band;          // input slot or -1, order dz,dy,dx with dx fastest
+    // input slots (real or generated nodes)
+    std::vector<long long> slot_off;  // offset of the slot's block in in_coefs
+    std::vector<double> in_coefs;   // pool: full blocks (real) or scaling part (generated)
+    std::vector<double> in_norms;   // slots x 8 (wavelet norms 0 for generated slots)
+    std::vector<int> in_depth;
+    // screening-derived bandwidths: w[(d*M + m)*4 + c]
+    std::vector<int> bw;
+    std::vector<int> active_off, active_terms;  // per depth, terms with any w >= 0
+    long long n_generated = 0;
+};
+
+struct ApplyStats {
+    double term_apps = 0;      // term-applications passing screening
+    double candidates = 0;     // (f, ft, gt, m) candidates examined
+    double out_nodes = 0;
+    double in_slots = 0;
+    double generated = 0;
+};
+
+// Build the plan for output nodes `nodes` of `out` with screening parameter
+// thr_param (thr_g = thr_param * prec_fac[g]).
+void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
+                const std::vector<int> &nodes, const std::vector<double> &thr,
+                int max_band_w);
+double oper_norm(const Operator &op, int m, int d, int c, long long l);
+
+// Operator tables flattened for the device: per (m, d) the half-width L, the
+// offset of block (c=0, l=-L) in `blocks` (doubles) and of its norm in `norms`.
+struct OperFlat {
+    int M = 0, ndepth = 0, kp1 = 0;
+    std::vector<int> L;             // [m*ndepth + d]
+    std::vector<long long> boff, noff;
+    std::vector<double> blocks, norms;
+};
+void flatten_operator(const Operator &op, OperFlat &f);
+
+}  // namespace mwg

@@ -1,0 +1,231 @@
+// Apply plan: bands, generated input nodes and screening-derived bandwidths.
+//
+// Upstream MRCPP's OperApplicationCalculator::makeOperBand gathers, for each
+// output node g at scale n, the input nodes f with |l_f - l_g|_inf <= w(n),
+// generating missing ones by reconstruction from their deepest ancestor
+// (MWTree::getNode with node generation; inp.deleteGenerated() afterwards in
+// treebuilders/apply.cpp).  Here the same gathering happens once on the host
+// into flat arrays the HIP kernels read: a coefficient pool (full blocks for
+// real nodes, scaling part only for generated ones), per-slot component norms
+// and a dense (2W+1)^3 band table per output node.
+//
+// The band half-width is derived from the screening rule itself, so it never
+// changes which terms pass: along one direction, a translation l can pass
+//     N_x(l) N_y N_z ||f_ft|| > thr
+// only if N(l) * Nmax^2 * Fmax > thr_min, where Nmax bounds every block norm
+// of the term at that depth and Fmax bounds every input component norm there.
+#include "mw_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace mwg {
+
+namespace {
+struct GenKey {
+    int d;
+    long long x, y, z;
+    bool operator==(const GenKey &o) const { return d == o.d && x == o.x && y == o.y && z == o.z; }
+};
+struct GenKeyH {
+    size_t operator()(const GenKey &k) const {
+        uint64_t h = (uint64_t)k.d * 0x9E3779B97F4A7C15ULL;
+        h ^= (uint64_t)k.x * 0xC2B2AE3D27D4EB4FULL + (h << 6) + (h >> 2);
+        h ^= (uint64_t)k.y * 0x165667B19E3779F9ULL + (h << 6) + (h >> 2);
+        h ^= (uint64_t)k.z * 0x27D4EB2F165667C5ULL + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+}  // namespace
+
+double oper_norm(const Operator &op, int m, int d, int c, long long l) {
+    const OperDepth &od = op.terms[m].depth[d];
+    if (od.L < 0 || l < -od.L || l > od.L) return 0.0;
+    return od.norms[(size_t)c * (2 * od.L + 1) + (size_t)(l + od.L)];
+}
+
+void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
+                const std::vector<int> &nodes, const std::vector<double> &thr, int max_band_w) {
+    const MRA &mra = *out.mra;
+    const int n3 = mra.kp1_3(), blk = mra.block();
+    const int M = op.size(), ndepth = op.max_depth + 1;
+    if (inp.mra != out.mra || op.mra != out.mra) throw std::invalid_argument("apply: trees and operator must share one MRA");
+    if (thr.size() != nodes.size()) throw std::invalid_argument("apply: one threshold per output node");
+    plan = ApplyPlan();
+    plan.kp1 = mra.kp1();
+    plan.ndepth = ndepth;
+    plan.M = M;
+
+    int maxd = 0;
+    for (int g : nodes) maxd = std::max(maxd, out.keys[g].depth);
+    if (maxd >= ndepth) throw std::runtime_error("apply: output deeper than the operator tables");
+
+    std::vector<double> thr_min(maxd + 1, 1e300), fmax(maxd + 1, 0.0);
+    std::vector<char> has_depth(maxd + 1, 0);
+    for (size_t q = 0; q < nodes.size(); q++) {
+        int d = out.keys[nodes[q]].depth;
+        thr_min[d] = std::min(thr_min[d], thr[q]);
+        has_depth[d] = 1;
+    }
+    for (int i = 0; i < inp.size(); i++) {
+        int d = inp.keys[i].depth;
+        double nrm[8], tot = 0.0;
+        component_norms(mra, inp.node(i), nrm);
+        for (int t = 0; t < 8; t++) {
+            tot += nrm[t] * nrm[t];
+            if (d <= maxd) fmax[d] = std::max(fmax[d], nrm[t]);
+        }
+        if (inp.child0[i] < 0)
+            for (int dd = d + 1; dd <= maxd; dd++) fmax[dd] = std::max(fmax[dd], std::sqrt(tot));
+    }
+
+    plan.bw.assign((size_t)(maxd + 1) * M * 4, -1);
+    plan.active_off.assign(maxd + 2, 0);
+    std::vector<int> W(maxd + 1, 0);
+    for (int d = 0; d <= maxd; d++) {
+        plan.active_off[d] = (int)plan.active_terms.size();
+        if (!has_depth[d]) continue;
+        for (int m = 0; m < M; m++) {
+            const OperDepth &od = op.terms[m].depth[d];
+            if (od.L < 0) continue;
+            // nmax bounds every block of the term at this depth, wmax every
+            // wavelet-component block.  Off the root each candidate pairs at
+            // least one wavelet component with the others, so an s<-s block
+            // contributes at most N * nmax * wmax * Fmax.
+            double nmax = 0.0, wmax = 0.0;
+            for (int c = 0; c < 4; c++)
+                for (long long l = -od.L; l <= od.L; l++) {
+                    double nb = oper_norm(op, m, d, c, l);
+                    nmax = std::max(nmax, nb);
+                    if (c > 0) wmax = std::max(wmax, nb);
+                }
+            bool any = false;
+            for (int c = 0; c < 4; c++) {
+                double other = (d > 0 && c == 0) ? nmax * wmax : nmax * nmax;
+                int w = -1;
+                for (long long l = -od.L; l <= od.L; l++) {
+                    double nb = oper_norm(op, m, d, c, l);
+                    if (nb * other * fmax[d] * (1.0 + 1e-12) > thr_min[d]) w = std::max(w, (int)std::llabs(l));
+                }
+                plan.bw[((size_t)d * M + m) * 4 + c] = w;
+                if (w >= 0) {
+                    any = true;
+                    W[d] = std::max(W[d], w);
+                }
+            }
+            if (any) plan.active_terms.push_back(m);
+        }
+        if (W[d] > max_band_w)
+            throw std::runtime_error("apply: band half-width " + std::to_string(W[d]) + " exceeds limit " +
+                                     std::to_string(max_band_w));
+    }
+    plan.active_off[maxd + 1] = (int)plan.active_terms.size();
+
+    // Output node metadata and band tables; slots are created on first use.
+    std::unordered_map<GenKey, int, GenKeyH> slotmap;
+    std::vector<GenKey> slot_keys;
+    std::vector<int> slot_real;  // input-tree node or -1
+    plan.out_node = nodes;
+    plan.out_depth.resize(nodes.size());
+    plan.out_thr = thr;
+    plan.band_off.resize(nodes.size());
+    plan.band_w.resize(nodes.size());
+    long long total = 0;
+    for (size_t q = 0; q < nodes.size(); q++) {
+        int d = out.keys[nodes[q]].depth;
+        int w = W[d];
+        plan.out_depth[q] = d;
+        plan.band_w[q] = w;
+        plan.band_off[q] = total;
+        total += (long long)(2 * w + 1) * (2 * w + 1) * (2 * w + 1);
+    }
+    plan.band.assign(total, -1);
+    for (size_t q = 0; q < nodes.size(); q++) {
+        const NodeKey &gk = out.keys[nodes[q]];
+        int w = plan.band_w[q];
+        long long p = plan.band_off[q];
+        for (int dz = -w; dz <= w; dz++)
+            for (int dy = -w; dy <= w; dy++)
+                for (int dx = -w; dx <= w; dx++, p++) {
+                    long long l[3] = {gk.l[0] + dx, gk.l[1] + dy, gk.l[2] + dz};
+                    if (!mra.in_world(gk.depth, l)) continue;
+                    GenKey k{gk.depth, l[0], l[1], l[2]};
+                    auto ins = slotmap.emplace(k, (int)slot_keys.size());
+                    if (ins.second) {
+                        slot_keys.push_back(k);
+                        slot_real.push_back(inp.find(gk.depth, l));
+                    }
+                    plan.band[p] = ins.first->second;
+                }
+    }
+
+    // Coefficient pool: real slots keep the full block, generated slots the
+    // scaling part only.  Generated blocks are memoised along their paths.
+    const int ns = (int)slot_keys.size();
+    plan.slot_off.resize(ns);
+    long long pool = 0;
+    for (int s = 0; s < ns; s++) {
+        plan.slot_off[s] = pool;
+        pool += (slot_real[s] >= 0) ? blk : n3;
+    }
+    plan.in_coefs.assign(pool, 0.0);
+    plan.in_norms.assign((size_t)ns * 8, 0.0);
+    plan.in_depth.resize(ns);
+    std::unordered_map<GenKey, std::vector<double>, GenKeyH> gen;  // s-part of generated nodes
+    std::vector<double> tmp(blk), rec(blk);
+    for (int s = 0; s < ns; s++) {
+        const GenKey k = slot_keys[s];
+        plan.in_depth[s] = k.d;
+        double *dst = plan.in_coefs.data() + plan.slot_off[s];
+        if (slot_real[s] >= 0) {
+            std::memcpy(dst, inp.node(slot_real[s]), sizeof(double) * blk);
+            component_norms(mra, dst, &plan.in_norms[(size_t)s * 8]);
+            continue;
+        }
+        // walk up to the nearest memoised or real ancestor
+        std::vector<GenKey> path;
+        GenKey a = k;
+        const double *src_full = nullptr;
+        const std::vector<double> *src_gen = nullptr;
+        while (true) {
+            auto gi = gen.find(a);
+            if (gi != gen.end()) {
+                src_gen = &gi->second;
+                break;
+            }
+            long long l[3] = {a.x, a.y, a.z};
+            int r = inp.find(a.d, l);
+            if (r >= 0) {
+                src_full = inp.node(r);
+                break;
+            }
+            if (a.d == 0) break;
+            path.push_back(a);
+            a = GenKey{a.d - 1, a.x >> 1, a.y >> 1, a.z >> 1};
+        }
+        if (!src_full && !src_gen) throw std::runtime_error("apply: input tree has no root covering a band node");
+        if (src_full) std::memcpy(tmp.data(), src_full, sizeof(double) * blk);
+        else {
+            std::fill(tmp.begin(), tmp.end(), 0.0);
+            std::memcpy(tmp.data(), src_gen->data(), sizeof(double) * n3);
+        }
+        for (int p = (int)path.size() - 1; p >= 0; p--) {
+            const GenKey &ch = path[p];
+            int t = (int)((ch.x & 1) | ((ch.y & 1) << 1) | ((ch.z & 1) << 2));
+            mw_reconstruct(mra.basis, tmp.data(), rec.data());
+            std::fill(tmp.begin(), tmp.end(), 0.0);
+            std::memcpy(tmp.data(), rec.data() + (size_t)t * n3, sizeof(double) * n3);
+            gen.emplace(ch, std::vector<double>(tmp.begin(), tmp.begin() + n3));
+        }
+        std::memcpy(dst, tmp.data(), sizeof(double) * n3);
+        double q = 0.0;
+        for (int e = 0; e < n3; e++) q += dst[e] * dst[e];
+        plan.in_norms[(size_t)s * 8] = std::sqrt(q);
+        plan.n_generated++;
+    }
+}
+
+}  // namespace mwg

@@ -1,0 +1,559 @@
+// C ABI of the multiwavelet Green's-function apply and its host drivers.
+//
+// mwg_apply restates mrcpp::apply (treebuilders/apply.cpp upstream, fetched by
+// external/upstream/fetch_mrcpp.cmake:36-43; callers
+// src/qmoperators/two_electron/CoulombPotential.cpp:131,
+// src/qmoperators/two_electron/ExchangePotential.cpp:200,
+// src/scf_solver/HelmholtzVector.cpp:149): the output grid grows level by
+// level on the host (TreeBuilder + WaveletAdaptor), each level's nodes are
+// computed by one HIP launch, and the result is finished by the TopDown(add)
+// and BottomUp transforms.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/mwgreen.h"
+#include "mw_device.hpp"
+#include "mw_internal.hpp"
+
+using namespace mwg;
+
+struct mwg_context_s {
+    DeviceCtx *dev = nullptr;
+    std::vector<double> last_stats = std::vector<double>(8, 0.0);
+};
+struct mwg_mra_s {
+    MRA mra;
+};
+struct mwg_operator_s {
+    Operator op;
+    OperFlat flat;
+    std::shared_ptr<MRA> mra_keepalive;
+};
+struct mwg_tree_s {
+    Tree tree;
+    std::vector<double> apply_thr;
+};
+
+namespace {
+thread_local std::string g_err;
+std::mutex g_ctx_mtx;
+std::set<mwg_context_s *> g_contexts;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::invalid_argument &e) {
+        return fail(MWG_ERR_ARG, e.what());
+    } catch (const std::exception &e) {
+        return fail(MWG_ERR_RUNTIME, e.what());
+    }
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Norm of the node at key in a prec tree, or of its deepest ancestor scaled
+// by the volume ratio when the tree is coarser there.
+double prec_tree_norm(const Tree &t, const NodeKey &k) {
+    NodeKey a = k;
+    int up = 0;
+    while (true) {
+        int i = t.find(a.depth, a.l);
+        if (i >= 0) {
+            const double *b = t.node(i);
+            double s = 0.0;
+            for (int e = 0; e < t.mra->block(); e++) s += b[e] * b[e];
+            return std::sqrt(s) * std::pow(2.0, -1.5 * up);
+        }
+        if (a.depth == 0) return 0.0;
+        a = NodeKey{a.depth - 1, {a.l[0] >> 1, a.l[1] >> 1, a.l[2] >> 1}};
+        up++;
+    }
+}
+
+double prec_factor(const std::vector<const Tree *> &pt, const NodeKey &k) {
+    if (pt.empty()) return 1.0;
+    double pn = 1.0;
+    for (const Tree *t : pt) pn = std::max(pn, prec_tree_norm(*t, k));
+    return 1.0 / pn;
+}
+
+void check_compatible(const Tree &out, const Operator &op, const Tree &inp) {
+    if (out.mra != op.mra || inp.mra != op.mra)
+        throw std::invalid_argument("apply: trees and operator were built on different MRAs");
+    if (inp.size() == 0) throw std::invalid_argument("apply: input tree is empty");
+    for (double v : inp.coefs)
+        if (!std::isfinite(v)) throw std::invalid_argument("apply: input tree holds NaN or Inf");
+}
+
+int run_plan(mwg_context ctx, mwg_operator_s *op, const Tree &inp, Tree &out, const std::vector<int> &nodes,
+             const std::vector<double> &thr, std::vector<double> &ns, double &kms, double &plan_ms, double *slots,
+             double *gen) {
+    double t0 = now_ms();
+    ApplyPlan plan;
+    build_plan(plan, op->op, inp, out, nodes, thr, 64);
+    plan_ms += now_ms() - t0;
+    *slots += (double)plan.slot_off.size();
+    *gen += (double)plan.n_generated;
+    ns.assign(nodes.size() * (size_t)out.mra->block(), 0.0);
+    std::string err;
+    double ms = 0.0;
+    int rc = dev_apply_plan(ctx->dev, op, op->flat, plan, ns.data(), &ms, err);
+    if (rc) return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
+    kms += ms;
+    return MWG_OK;
+}
+
+void finish_output(Tree &out) {
+    tree_topdown_add(out);
+    tree_compress(out);
+    out.sqnorm = tree_square_norm(out);
+}
+
+int apply_adaptive(mwg_context ctx, double prec, mwg_tree_s *out, mwg_operator_s *op, mwg_tree_s *inp,
+                   const std::vector<const Tree *> &prec_trees, int max_iter, bool abs_prec) {
+    if (!ctx || !ctx->dev) return fail(MWG_ERR_ARG, "apply: null context");
+    if (!(prec >= 0.0) || !std::isfinite(prec)) return fail(MWG_ERR_ARG, "apply: precision must be finite and >= 0");
+    Tree &o = out->tree;
+    const Tree &in = inp->tree;
+    check_compatible(o, op->op, in);
+    const MRA &mra = *o.mra;
+    const int n3 = mra.kp1_3(), blk = mra.block();
+    const int M = op->op.size();
+    const int max_depth = std::min(mra.max_depth, op->op.max_depth);
+    o.clear();
+    o.make_roots();
+    std::vector<int> work;
+    for (int i = 0; i < o.size(); i++) work.push_back(i);
+    std::vector<double> thr_all(o.size(), 0.0);
+    double s_norm = 0.0, w_norm = 0.0, kms = 0.0, pms = 0.0, slots = 0.0, gen = 0.0;
+    int iter = 0, launches = 0;
+    double computed = 0.0;
+    std::vector<double> ns;
+    while (!work.empty()) {
+        const double Nsq = s_norm + w_norm;
+        std::vector<double> thr(work.size());
+        for (size_t q = 0; q < work.size(); q++)
+            thr[q] = prec * prec_factor(prec_trees, o.keys[work[q]]) * std::sqrt(Nsq / M);
+        int rc = run_plan(ctx, op, in, o, work, thr, ns, kms, pms, &slots, &gen);
+        if (rc) return rc;
+        launches++;
+        computed += (double)work.size();
+        thr_all.resize(o.size(), 0.0);
+        std::vector<double> wn(work.size());
+        for (size_t q = 0; q < work.size(); q++) {
+            const int i = work[q];
+            std::memcpy(o.node(i), ns.data() + q * blk, sizeof(double) * blk);
+            thr_all[i] = thr[q];
+            double w = 0.0, s = 0.0;
+            const double *b = ns.data() + q * blk;
+            for (int e = 0; e < n3; e++) s += b[e] * b[e];
+            for (int e = n3; e < blk; e++) w += b[e] * b[e];
+            wn[q] = std::sqrt(w);
+            w_norm += w;
+            if (o.keys[i].depth == 0) s_norm += s;
+        }
+        const double tnorm = abs_prec ? 1.0 : std::sqrt(s_norm + w_norm);
+        std::vector<int> next;
+        if (max_iter < 0 || iter < max_iter) {
+            for (size_t q = 0; q < work.size(); q++) {
+                const int i = work[q];
+                const NodeKey k = o.keys[i];
+                if (k.depth >= max_depth) continue;
+                double pf = prec_factor(prec_trees, k);
+                if (!split_check(mra, k.depth, wn[q], prec * pf, tnorm)) continue;
+                int c0 = o.split(i);
+                for (int c = 0; c < 8; c++) next.push_back(c0 + c);
+            }
+        }
+        work.swap(next);
+        iter++;
+    }
+    thr_all.resize(o.size(), 0.0);
+    finish_output(o);
+    out->apply_thr = thr_all;
+    auto &st = ctx->last_stats;
+    st.assign(8, 0.0);
+    st[0] = kms;
+    st[1] = computed;
+    st[2] = slots;
+    st[3] = gen;
+    st[4] = pms;
+    st[5] = launches;
+    return MWG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mwg_last_error(void) { return g_err.c_str(); }
+
+int mwg_context_create(int device, mwg_context *ctx) {
+    return guarded([&] {
+        if (!ctx) return fail(MWG_ERR_ARG, "null output pointer");
+        std::string err;
+        DeviceCtx *d = nullptr;
+        int rc = dev_create(device, &d, err);
+        if (rc) return fail(MWG_ERR_DEVICE, err);
+        auto *c = new mwg_context_s;
+        c->dev = d;
+        {
+            std::lock_guard<std::mutex> g(g_ctx_mtx);
+            g_contexts.insert(c);
+        }
+        *ctx = c;
+        return MWG_OK;
+    });
+}
+
+int mwg_context_destroy(mwg_context ctx) {
+    if (!ctx) return MWG_OK;
+    {
+        std::lock_guard<std::mutex> g(g_ctx_mtx);
+        g_contexts.erase(ctx);
+    }
+    dev_destroy(ctx->dev);
+    delete ctx;
+    return MWG_OK;
+}
+
+int mwg_mra_create(int order, int basis_type, int root_scale, const int *corner, const int *boxes,
+                   double scaling_factor, int max_depth, mwg_mra *out) {
+    return guarded([&] {
+        if (!out || !corner || !boxes) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (!(scaling_factor > 0.0)) return fail(MWG_ERR_ARG, "scaling factor must be > 0");
+        if (max_depth < 1 || max_depth > 20) return fail(MWG_ERR_ARG, "max_depth must be in [1, 20]");
+        for (int d = 0; d < 3; d++)
+            if (boxes[d] < 1 || boxes[d] > 2) return fail(MWG_ERR_ARG, "boxes per direction must be 1 or 2");
+        auto *m = new mwg_mra_s;
+        try {
+            m->mra.basis = make_basis(order, basis_type);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        m->mra.n0 = root_scale;
+        for (int d = 0; d < 3; d++) {
+            m->mra.corner[d] = corner[d];
+            m->mra.boxes[d] = boxes[d];
+        }
+        m->mra.sfac = scaling_factor;
+        m->mra.max_depth = max_depth;
+        *out = m;
+        return MWG_OK;
+    });
+}
+
+int mwg_mra_destroy(mwg_mra mra) {
+    delete mra;
+    return MWG_OK;
+}
+
+int mwg_mra_filter(mwg_mra mra, double *F) {
+    if (!mra || !F) return fail(MWG_ERR_ARG, "null pointer argument");
+    std::memcpy(F, mra->mra.basis.F.data(), sizeof(double) * mra->mra.basis.F.size());
+    return MWG_OK;
+}
+
+static int make_op(mwg_mra mra, double mu, double prec, mwg_operator *out) {
+    return guarded([&] {
+        if (!mra || !out) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (!(prec > 0.0 && prec < 1.0)) return fail(MWG_ERR_ARG, "build precision must be in (0, 1)");
+        if (!(mu >= 0.0) || !std::isfinite(mu)) return fail(MWG_ERR_ARG, "mu must be finite and >= 0");
+        auto *o = new mwg_operator_s;
+        try {
+            build_operator(o->op, mra->mra, mu, prec, mra->mra.max_depth);
+            flatten_operator(o->op, o->flat);
+        } catch (...) {
+            delete o;
+            throw;
+        }
+        *out = o;
+        return MWG_OK;
+    });
+}
+
+int mwg_poisson_create(mwg_mra mra, double build_prec, mwg_operator *op) { return make_op(mra, 0.0, build_prec, op); }
+
+int mwg_helmholtz_create(mwg_mra mra, double mu, double build_prec, mwg_operator *op) {
+    if (!(mu > 0.0)) return fail(MWG_ERR_ARG, "Helmholtz mu must be > 0");
+    return make_op(mra, mu, build_prec, op);
+}
+
+int mwg_gaussian_operator_create(mwg_mra mra, int n_terms, const double *alpha, const double *beta, int max_depth,
+                                 mwg_operator *out) {
+    return guarded([&] {
+        if (!mra || !out || !alpha || !beta || n_terms < 1) return fail(MWG_ERR_ARG, "bad arguments");
+        std::vector<GaussTerm> g(n_terms);
+        for (int i = 0; i < n_terms; i++) {
+            if (!(alpha[i] > 0.0) || !std::isfinite(beta[i])) return fail(MWG_ERR_ARG, "terms need alpha > 0");
+            g[i] = {alpha[i], beta[i]};
+        }
+        auto *o = new mwg_operator_s;
+        try {
+            build_operator_terms(o->op, mra->mra, g, max_depth < 0 ? mra->mra.max_depth : max_depth);
+            o->op.build_prec = 1e-14;
+            flatten_operator(o->op, o->flat);
+        } catch (...) {
+            delete o;
+            throw;
+        }
+        *out = o;
+        return MWG_OK;
+    });
+}
+
+int mwg_operator_destroy(mwg_operator op) {
+    if (!op) return MWG_OK;
+    {
+        std::lock_guard<std::mutex> g(g_ctx_mtx);
+        for (auto *c : g_contexts) dev_forget_operator(c->dev, op);
+    }
+    delete op;
+    return MWG_OK;
+}
+
+int mwg_operator_info(mwg_operator op, int *n_terms, int *n_depth, int *kp1, double *r_min, double *r_max,
+                      double *fit_err) {
+    if (!op) return fail(MWG_ERR_ARG, "null operator");
+    if (n_terms) *n_terms = op->op.size();
+    if (n_depth) *n_depth = op->op.max_depth + 1;
+    if (kp1) *kp1 = op->op.mra->kp1();
+    if (r_min) *r_min = op->op.r_min;
+    if (r_max) *r_max = op->op.r_max;
+    if (fit_err) *fit_err = op->op.fit_err;
+    return MWG_OK;
+}
+
+int mwg_operator_terms(mwg_operator op, double *alpha, double *beta) {
+    if (!op) return fail(MWG_ERR_ARG, "null operator");
+    for (int m = 0; m < op->op.size(); m++) {
+        if (alpha) alpha[m] = op->op.terms[m].alpha;
+        if (beta) beta[m] = op->op.terms[m].beta;
+    }
+    return MWG_OK;
+}
+
+int mwg_operator_table_sizes(mwg_operator op, long long *nb, long long *nn) {
+    if (!op) return fail(MWG_ERR_ARG, "null operator");
+    if (nb) *nb = (long long)op->flat.blocks.size();
+    if (nn) *nn = (long long)op->flat.norms.size();
+    return MWG_OK;
+}
+
+int mwg_operator_tables(mwg_operator op, int *L, long long *boff, long long *noff, double *blocks, double *norms) {
+    if (!op) return fail(MWG_ERR_ARG, "null operator");
+    const OperFlat &f = op->flat;
+    if (L) std::memcpy(L, f.L.data(), sizeof(int) * f.L.size());
+    if (boff) std::memcpy(boff, f.boff.data(), sizeof(long long) * f.boff.size());
+    if (noff) std::memcpy(noff, f.noff.data(), sizeof(long long) * f.noff.size());
+    if (blocks) std::memcpy(blocks, f.blocks.data(), sizeof(double) * f.blocks.size());
+    if (norms) std::memcpy(norms, f.norms.data(), sizeof(double) * f.norms.size());
+    return MWG_OK;
+}
+
+int mwg_tree_create(mwg_mra mra, mwg_tree *out) {
+    if (!mra || !out) return fail(MWG_ERR_ARG, "null pointer argument");
+    auto *t = new mwg_tree_s;
+    t->tree.mra = &mra->mra;
+    t->tree.make_roots();
+    *out = t;
+    return MWG_OK;
+}
+
+int mwg_tree_destroy(mwg_tree t) {
+    delete t;
+    return MWG_OK;
+}
+
+int mwg_tree_project_gaussians(mwg_tree t, double prec, int abs_prec, int max_depth, int n, const double *coef,
+                               const double *exponent, const double *center) {
+    return guarded([&] {
+        if (!t || n < 0 || (n > 0 && (!coef || !exponent || !center))) return fail(MWG_ERR_ARG, "bad arguments");
+        GaussSum f;
+        f.coef.assign(coef, coef + n);
+        f.exp.assign(exponent, exponent + n);
+        f.pos.assign(center, center + 3 * n);
+        for (int i = 0; i < n; i++)
+            if (!(exponent[i] > 0.0)) return fail(MWG_ERR_ARG, "Gaussian exponents must be > 0");
+        project_adaptive(t->tree, f, prec, abs_prec != 0, max_depth);
+        t->apply_thr.clear();
+        return MWG_OK;
+    });
+}
+
+int mwg_tree_size(mwg_tree t, int *n) {
+    if (!t || !n) return fail(MWG_ERR_ARG, "null pointer argument");
+    *n = t->tree.size();
+    return MWG_OK;
+}
+
+int mwg_tree_get(mwg_tree t, int *depth, long long *l, double *coefs) {
+    if (!t) return fail(MWG_ERR_ARG, "null tree");
+    const Tree &tr = t->tree;
+    for (int i = 0; i < tr.size(); i++) {
+        if (depth) depth[i] = tr.keys[i].depth;
+        if (l)
+            for (int d = 0; d < 3; d++) l[3 * i + d] = tr.keys[i].l[d];
+    }
+    if (coefs) std::memcpy(coefs, tr.coefs.data(), sizeof(double) * tr.coefs.size());
+    return MWG_OK;
+}
+
+int mwg_tree_set(mwg_tree t, int n, const int *depth, const long long *l, const double *coefs) {
+    return guarded([&] {
+        if (!t || n < 1 || !depth || !l) return fail(MWG_ERR_ARG, "bad arguments");
+        const MRA &mra = *t->tree.mra;
+        const int blk = mra.block();
+        Tree src;
+        src.mra = &mra;
+        for (int i = 0; i < n; i++) {
+            NodeKey k{depth[i], {l[3 * i], l[3 * i + 1], l[3 * i + 2]}};
+            if (k.depth < 0 || k.depth > mra.max_depth || !mra.in_world(k.depth, k.l))
+                return fail(MWG_ERR_ARG, "node key outside the world box");
+            if (src.find(k.depth, k.l) >= 0) return fail(MWG_ERR_ARG, "duplicate node key");
+            src.add_node(k, -1);
+            if (coefs) std::memcpy(src.node(i), coefs + (size_t)i * blk, sizeof(double) * blk);
+        }
+        Tree dst;
+        dst.mra = &mra;
+        dst.make_roots();
+        for (int i = 0; i < dst.size(); i++) {
+            int j = src.find(dst.keys[i].depth, dst.keys[i].l);
+            if (j < 0) return fail(MWG_ERR_ARG, "root node missing");
+            std::memcpy(dst.node(i), src.node(j), sizeof(double) * blk);
+        }
+        int placed = dst.size();
+        for (int i = 0; i < dst.size(); i++) {
+            const NodeKey k = dst.keys[i];
+            long long c0[3] = {2 * k.l[0], 2 * k.l[1], 2 * k.l[2]};
+            if (src.find(k.depth + 1, c0) < 0) continue;
+            int first = dst.split(i);
+            for (int c = 0; c < 8; c++) {
+                int j = src.find(dst.keys[first + c].depth, dst.keys[first + c].l);
+                if (j < 0) return fail(MWG_ERR_ARG, "incomplete node family (a child is missing)");
+                std::memcpy(dst.node(first + c), src.node(j), sizeof(double) * blk);
+            }
+            placed += 8;
+        }
+        if (placed != n) return fail(MWG_ERR_ARG, "nodes unreachable from the roots (missing parent)");
+        dst.sqnorm = tree_square_norm(dst);
+        t->tree = std::move(dst);
+        t->tree.mra = &mra;
+        t->apply_thr.clear();
+        return MWG_OK;
+    });
+}
+
+int mwg_tree_square_norm(mwg_tree t, double *s) {
+    if (!t || !s) return fail(MWG_ERR_ARG, "null pointer argument");
+    *s = tree_square_norm(t->tree);
+    return MWG_OK;
+}
+
+int mwg_tree_dot(mwg_tree a, mwg_tree b, double *s) {
+    if (!a || !b || !s) return fail(MWG_ERR_ARG, "null pointer argument");
+    if (a->tree.mra != b->tree.mra) return fail(MWG_ERR_ARG, "trees on different MRAs");
+    *s = tree_dot(a->tree, b->tree);
+    return MWG_OK;
+}
+
+int mwg_tree_eval(mwg_tree t, int n, const double *r, double *v) {
+    if (!t || n < 0 || (n > 0 && (!r || !v))) return fail(MWG_ERR_ARG, "bad arguments");
+    for (int i = 0; i < n; i++) v[i] = tree_eval(t->tree, r + 3 * i);
+    return MWG_OK;
+}
+
+int mwg_tree_rescale(mwg_tree t, double f) {
+    if (!t) return fail(MWG_ERR_ARG, "null tree");
+    for (double &x : t->tree.coefs) x *= f;
+    t->tree.sqnorm *= f * f;
+    return MWG_OK;
+}
+
+int mwg_tree_thresholds(mwg_tree t, double *thr) {
+    if (!t || !thr) return fail(MWG_ERR_ARG, "null pointer argument");
+    if (t->apply_thr.size() != (size_t)t->tree.size()) return fail(MWG_ERR_ARG, "tree holds no apply result");
+    std::memcpy(thr, t->apply_thr.data(), sizeof(double) * t->apply_thr.size());
+    return MWG_OK;
+}
+
+int mwg_apply(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp, int max_iter, int abs_prec) {
+    return guarded([&] {
+        if (!out || !op || !inp) return fail(MWG_ERR_ARG, "null handle");
+        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
+        return apply_adaptive(ctx, prec, out, op, inp, {}, max_iter, abs_prec != 0);
+    });
+}
+
+int mwg_apply_prec_trees(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp,
+                         const mwg_tree *prec_trees, int n_prec, int max_iter, int abs_prec) {
+    return guarded([&] {
+        if (!out || !op || !inp) return fail(MWG_ERR_ARG, "null handle");
+        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
+        if (n_prec < 0 || (n_prec > 0 && !prec_trees)) return fail(MWG_ERR_ARG, "bad prec tree list");
+        std::vector<const Tree *> pt;
+        for (int i = 0; i < n_prec; i++) {
+            if (!prec_trees[i]) return fail(MWG_ERR_ARG, "null prec tree");
+            if (prec_trees[i]->tree.mra != op->op.mra) return fail(MWG_ERR_ARG, "prec tree on another MRA");
+            pt.push_back(&prec_trees[i]->tree);
+        }
+        return apply_adaptive(ctx, prec, out, op, inp, pt, max_iter, abs_prec != 0);
+    });
+}
+
+int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode) {
+    return guarded([&] {
+        if (!ctx || !out || !op || !inp || !thr) return fail(MWG_ERR_ARG, "null argument");
+        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
+        if (mode != MWG_MODE_FP64) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+        Tree &o = out->tree;
+        check_compatible(o, op->op, inp->tree);
+        const int n = o.size(), blk = o.mra->block();
+        std::vector<int> nodes(n);
+        std::vector<double> th(thr, thr + n);
+        for (int i = 0; i < n; i++) {
+            nodes[i] = i;
+            if (!(th[i] >= 0.0) || !std::isfinite(th[i])) return fail(MWG_ERR_ARG, "thresholds must be finite and >= 0");
+        }
+        double kms = 0, pms = 0, slots = 0, gen = 0;
+        std::vector<double> ns;
+        int rc = run_plan(ctx, op, inp->tree, o, nodes, th, ns, kms, pms, &slots, &gen);
+        if (rc) return rc;
+        std::memcpy(o.coefs.data(), ns.data(), sizeof(double) * (size_t)n * blk);
+        finish_output(o);
+        out->apply_thr = th;
+        auto &st = ctx->last_stats;
+        st.assign(8, 0.0);
+        st[0] = kms;
+        st[1] = n;
+        st[2] = slots;
+        st[3] = gen;
+        st[4] = pms;
+        st[5] = 1;
+        return MWG_OK;
+    });
+}
+
+int mwg_last_apply_stats(mwg_context ctx, double *stats, int n) {
+    if (!ctx || !stats) return fail(MWG_ERR_ARG, "null pointer argument");
+    for (int i = 0; i < n && i < (int)ctx->last_stats.size(); i++) stats[i] = ctx->last_stats[i];
+    return MWG_OK;
+}
+
+}  // extern "C"

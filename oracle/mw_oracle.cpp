@@ -1,0 +1,589 @@
+// ORACLE - test infrastructure only.  Nothing in the product links this file;
+// only tests/, the smoke check and bench.py's cpu_baseline leg load it, as the
+// checker of the HIP apply.
+//
+// A plain FP64 OpenMP restatement of the multiwavelet Green's-function apply,
+// mrcpp::apply(prec, out, ConvolutionOperator&, inp[, precTrees], maxIter,
+// absPrec) as called at src/qmoperators/two_electron/CoulombPotential.cpp:131,
+// src/qmoperators/two_electron/ExchangePotential.cpp:200 and
+// src/scf_solver/HelmholtzVector.cpp:149.  The apply itself lives in upstream
+// MRCPP (external/upstream/fetch_mrcpp.cmake:36-43, not present), so this
+// follows its published design:
+//   * OperApplicationCalculator::calcNode - band of input nodes within the
+//     bandwidth, generated nodes below input leaves, and for every band node f,
+//     input component ft, output component gt and separated term m the bound
+//     ||O_x|| ||O_y|| ||O_z|| ||f_ft|| tested against the node threshold
+//     prec * precFac * sqrt(||out||^2 / M); passing terms add
+//     (O_x (x) O_y (x) O_z) f_ft into g_gt.  The s<-s combination (gt = ft = 0)
+//     is applied at the root scale only.
+//   * TreeBuilder::build with WaveletAdaptor - level-by-level refinement; a
+//     computed node splits when its wavelet norm exceeds
+//     prec * t_norm * 2^{-(n+1)/2}.
+//   * apply.cpp post-processing - TopDown transform adding the coarse scaling
+//     parts into the children, then BottomUp compression.
+//
+// Parity status:
+//   * pinned by closed-form Coulomb and Yukawa energies of Gaussian densities,
+//     by point values of the potential, and by linearity/symmetry invariants
+//     (tests/test_oracle.py);
+//   * node generation: pinned by reconstruction round-trips;
+//   * bit-level parity with upstream MRCPP: parity unpinned (MRCPP absent).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <unordered_map>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct Key {
+    int d;
+    long long x, y, z;
+    bool operator==(const Key &o) const { return d == o.d && x == o.x && y == o.y && z == o.z; }
+};
+struct KeyH {
+    size_t operator()(const Key &k) const {
+        uint64_t h = (uint64_t)k.d * 0x9E3779B97F4A7C15ULL;
+        h ^= (uint64_t)k.x * 0xC2B2AE3D27D4EB4FULL + (h << 6) + (h >> 2);
+        h ^= (uint64_t)k.y * 0x165667B19E3779F9ULL + (h << 6) + (h >> 2);
+        h ^= (uint64_t)k.z * 0x27D4EB2F165667C5ULL + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+using KeyMap = std::unordered_map<Key, int, KeyH>;
+
+struct Geo {
+    int kp1, n3, blk, n0;
+    int corner[3], boxes[3];
+    const double *F;  // 2kp1 x 2kp1 [H0 H1; G0 G1]
+    bool in_world(const Key &k) const {
+        long long l[3] = {k.x, k.y, k.z};
+        for (int d = 0; d < 3; d++) {
+            long long lo = (long long)corner[d] << k.d, hi = (long long)(corner[d] + boxes[d]) << k.d;
+            if (l[d] < lo || l[d] >= hi) return false;
+        }
+        return true;
+    }
+};
+
+// One direction of the two-scale transform on a pair of blocks.
+void pair_tf(const Geo &c, double *lo, double *hi, int dim, bool transpose) {
+    const int n = c.kp1, n2 = 2 * n;
+    int stride = dim == 0 ? 1 : (dim == 1 ? n : n * n);
+    double v[64], u[64];
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) {
+            int base = dim == 0 ? a * n + b * n * n : (dim == 1 ? a + b * n * n : a + b * n);
+            for (int i = 0; i < n; i++) {
+                v[i] = lo[base + i * stride];
+                v[n + i] = hi[base + i * stride];
+            }
+            for (int i = 0; i < n2; i++) {
+                double s = 0.0;
+                for (int j = 0; j < n2; j++) s += (transpose ? c.F[j * n2 + i] : c.F[i * n2 + j]) * v[j];
+                u[i] = s;
+            }
+            for (int i = 0; i < n; i++) {
+                lo[base + i * stride] = u[i];
+                hi[base + i * stride] = u[n + i];
+            }
+        }
+}
+void compress8(const Geo &c, double *b) {
+    for (int dim = 0; dim < 3; dim++)
+        for (int t = 0; t < 8; t++)
+            if (!((t >> dim) & 1)) pair_tf(c, b + t * c.n3, b + (t | (1 << dim)) * c.n3, dim, false);
+}
+void reconstruct8(const Geo &c, double *b) {
+    for (int dim = 2; dim >= 0; dim--)
+        for (int t = 0; t < 8; t++)
+            if (!((t >> dim) & 1)) pair_tf(c, b + t * c.n3, b + (t | (1 << dim)) * c.n3, dim, true);
+}
+
+struct OperView {
+    int M, ndepth, kp1;
+    const int *L;           // [m*ndepth + d]
+    const long long *boff;  // offset of block (c=0,l=-L) in doubles
+    const long long *noff;  // offset of norm (c=0,l=-L)
+    const double *blocks, *norms;
+    double norm(int m, int d, int c, long long l) const {
+        int Lm = L[m * ndepth + d];
+        if (Lm < 0 || l < -Lm || l > Lm) return 0.0;
+        return norms[noff[m * ndepth + d] + (long long)c * (2 * Lm + 1) + (l + Lm)];
+    }
+    const double *block(int m, int d, int c, long long l) const {
+        int Lm = L[m * ndepth + d];
+        return blocks + boff[m * ndepth + d] + ((long long)c * (2 * Lm + 1) + (l + Lm)) * kp1 * kp1;
+    }
+};
+
+// g += (Ox (x) Oy (x) Oz) f  for one (kp1)^3 block, x fastest.
+void contract(int n, const double *Ox, const double *Oy, const double *Oz, const double *f, double *g, double *t1,
+              double *t2) {
+    const int n2 = n * n, n3 = n2 * n;
+    for (int e = 0; e < n3; e++) t1[e] = 0.0;
+    for (int r = 0; r < n2; r++)
+        for (int jx = 0; jx < n; jx++) {
+            double fv = f[r * n + jx];
+            for (int ix = 0; ix < n; ix++) t1[r * n + ix] += Ox[ix * n + jx] * fv;
+        }
+    for (int e = 0; e < n3; e++) t2[e] = 0.0;
+    for (int jz = 0; jz < n; jz++)
+        for (int iy = 0; iy < n; iy++)
+            for (int jy = 0; jy < n; jy++) {
+                double o = Oy[iy * n + jy];
+                const double *src = t1 + (jz * n + jy) * n;
+                double *dst = t2 + (jz * n + iy) * n;
+                for (int ix = 0; ix < n; ix++) dst[ix] += o * src[ix];
+            }
+    for (int iz = 0; iz < n; iz++)
+        for (int jz = 0; jz < n; jz++) {
+            double o = Oz[iz * n + jz];
+            const double *src = t2 + jz * n2;
+            double *dst = g + iz * n2;
+            for (int e = 0; e < n2; e++) dst[e] += o * src[e];
+        }
+}
+
+// Input side: the input tree plus generated nodes, created on demand.
+struct Input {
+    const Geo *geo;
+    KeyMap map;  // input tree node -> index
+    const double *coefs;
+    std::vector<char> leaf;
+    KeyMap slotmap;
+    std::vector<Key> slots;
+    std::deque<std::vector<double>> sc;  // full block per slot
+    std::vector<double> snorm;           // 8 per slot
+    long long ngen = 0;
+    std::vector<double> fmax;            // per depth bound of any component norm
+
+    void init(int n_in, const int *dep, const long long *l, const double *c, int maxdepth) {
+        coefs = c;
+        map.reserve(n_in * 2);
+        for (int i = 0; i < n_in; i++) map[{dep[i], l[3 * i], l[3 * i + 1], l[3 * i + 2]}] = i;
+        leaf.assign(n_in, 1);
+        for (int i = 0; i < n_in; i++)
+            if (map.count({dep[i] + 1, 2 * l[3 * i], 2 * l[3 * i + 1], 2 * l[3 * i + 2]})) leaf[i] = 0;
+        fmax.assign(maxdepth + 1, 0.0);
+        const int n3 = geo->n3, blk = geo->blk;
+        for (int i = 0; i < n_in; i++) {
+            const double *b = c + (size_t)i * blk;
+            int d = dep[i];
+            double tot = 0.0;
+            for (int t = 0; t < 8; t++) {
+                double s = 0.0;
+                for (int e = 0; e < n3; e++) s += b[t * n3 + e] * b[t * n3 + e];
+                tot += s;
+                if (d <= maxdepth) fmax[d] = std::max(fmax[d], std::sqrt(s));
+            }
+            if (leaf[i])
+                for (int dd = d + 1; dd <= maxdepth; dd++) fmax[dd] = std::max(fmax[dd], std::sqrt(tot));
+        }
+    }
+    // ensure slots exist for the keys (serial registration, parallel fill)
+    void ensure(const std::vector<Key> &keys) {
+        const int n3 = geo->n3, blk = geo->blk;
+        int first = (int)slots.size();
+        for (const Key &k : keys)
+            if (slotmap.emplace(k, (int)slots.size()).second) slots.push_back(k);
+        int last = (int)slots.size();
+        for (int s = first; s < last; s++) sc.emplace_back(blk, 0.0);
+        snorm.resize((size_t)last * 8, 0.0);
+        long long g = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : g)
+        for (int s = first; s < last; s++) {
+            Key k = slots[s];
+            double *dst = sc[s].data();
+            auto it = map.find(k);
+            if (it != map.end()) {
+                std::memcpy(dst, coefs + (size_t)it->second * blk, sizeof(double) * blk);
+            } else {
+                g++;
+                std::vector<Key> path;
+                Key a = k;
+                int src = -1;
+                while (a.d > 0) {
+                    path.push_back(a);
+                    a = Key{a.d - 1, a.x >> 1, a.y >> 1, a.z >> 1};  // floor division by 2
+                    auto jt = map.find(a);
+                    if (jt != map.end()) {
+                        src = jt->second;
+                        break;
+                    }
+                }
+                if (src >= 0) {
+                    std::vector<double> cur(coefs + (size_t)src * blk, coefs + (size_t)(src + 1) * blk);
+                    for (int p = (int)path.size() - 1; p >= 0; p--) {
+                        const Key &ch = path[p];
+                        int t = (int)((ch.x & 1) | ((ch.y & 1) << 1) | ((ch.z & 1) << 2));
+                        reconstruct8(*geo, cur.data());
+                        if (t) std::memmove(cur.data(), cur.data() + (size_t)t * n3, sizeof(double) * n3);
+                        std::fill(cur.begin() + n3, cur.end(), 0.0);
+                    }
+                    std::memcpy(dst, cur.data(), sizeof(double) * blk);
+                }
+            }
+            for (int t = 0; t < 8; t++) {
+                double q = 0.0;
+                for (int e = 0; e < n3; e++) q += dst[t * n3 + e] * dst[t * n3 + e];
+                snorm[(size_t)s * 8 + t] = std::sqrt(q);
+            }
+        }
+        ngen += g;
+    }
+};
+
+// Conservative band half-width per depth from the screening rule.
+int band_width(const OperView &op, int d, double fmax, double thr_min) {
+    int w = 0;
+    for (int m = 0; m < op.M; m++) {
+        int Lm = op.L[m * op.ndepth + d];
+        if (Lm < 0) continue;
+        // nmax bounds every block, wmax every wavelet-component block; off the
+        // root an s<-s block always meets a wavelet block in another direction.
+        double nmax = 0.0, wmax = 0.0;
+        for (int c = 0; c < 4; c++)
+            for (long long l = -Lm; l <= Lm; l++) {
+                double nb = op.norm(m, d, c, l);
+                nmax = std::max(nmax, nb);
+                if (c > 0) wmax = std::max(wmax, nb);
+            }
+        for (int c = 0; c < 4; c++) {
+            double other = (d > 0 && c == 0) ? nmax * wmax : nmax * nmax;
+            for (long long l = -Lm; l <= Lm; l++)
+                if (op.norm(m, d, c, l) * other * fmax * (1.0 + 1e-12) > thr_min) w = std::max(w, (int)std::llabs(l));
+        }
+    }
+    return w;
+}
+
+struct Stats {
+    double apps = 0, cand = 0;
+    int wmax = 0;
+};
+
+// Non-standard contributions of the given output nodes (one block each).
+void compute_nodes(const Geo &geo, const OperView &op, Input &in, const std::vector<Key> &nodes,
+                   const std::vector<double> &thr, int band_override, double *ns_out, Stats &st) {
+    const int n = geo.kp1, n3 = geo.n3, blk = geo.blk;
+    int maxd = 0;
+    for (auto &k : nodes) maxd = std::max(maxd, k.d);
+    std::vector<double> thr_min(maxd + 1, 1e300);
+    for (size_t g = 0; g < nodes.size(); g++) thr_min[nodes[g].d] = std::min(thr_min[nodes[g].d], thr[g]);
+    std::vector<int> W(maxd + 1, 0);
+    for (int d = 0; d <= maxd; d++) {
+        if (thr_min[d] >= 1e300) continue;
+        W[d] = band_override >= 0 ? band_override : band_width(op, d, in.fmax[d], thr_min[d]);
+        st.wmax = std::max(st.wmax, W[d]);
+    }
+    // active terms per depth (anything stored for the relevant components)
+    std::vector<std::vector<int>> act(maxd + 1);
+    for (int d = 0; d <= maxd; d++)
+        for (int m = 0; m < op.M; m++)
+            if (op.L[m * op.ndepth + d] >= 0) act[d].push_back(m);
+    // register band slots
+    {
+        std::vector<Key> need;
+        KeyMap seen;
+        for (auto &g : nodes) {
+            int w = W[g.d];
+            for (int dz = -w; dz <= w; dz++)
+                for (int dy = -w; dy <= w; dy++)
+                    for (int dx = -w; dx <= w; dx++) {
+                        Key k{g.d, g.x + dx, g.y + dy, g.z + dz};
+                        if (!geo.in_world(k) || in.slotmap.count(k)) continue;
+                        if (seen.emplace(k, 0).second) need.push_back(k);
+                    }
+        }
+        in.ensure(need);
+    }
+    double napps = 0.0, ncand = 0.0;
+#pragma omp parallel reduction(+ : napps, ncand)
+    {
+        std::vector<double> t1(n3), t2(n3);
+#pragma omp for schedule(guided)
+        for (int gi = 0; gi < (int)nodes.size(); gi++) {
+            const Key gk = nodes[gi];
+            const int d = gk.d, w = W[d];
+            double *acc = ns_out + (size_t)gi * blk;
+            std::fill(acc, acc + blk, 0.0);
+            const double th = thr[gi];
+            for (int dz = -w; dz <= w; dz++)
+                for (int dy = -w; dy <= w; dy++)
+                    for (int dx = -w; dx <= w; dx++) {
+                        auto it = in.slotmap.find(Key{d, gk.x + dx, gk.y + dy, gk.z + dz});
+                        if (it == in.slotmap.end()) continue;
+                        const int s = it->second;
+                        const double *fb = in.sc[s].data();
+                        const double *fn = &in.snorm[(size_t)s * 8];
+                        double fnmax = 0.0;
+                        for (int t = 0; t < 8; t++) fnmax = std::max(fnmax, fn[t]);
+                        if (!(fnmax > 0.0)) continue;
+                        const long long l[3] = {dx, dy, dz};
+                        for (int m : act[d]) {
+                            // cheap conservative pre-test over all (ft, gt)
+                            double mx[3];
+                            for (int q = 0; q < 3; q++) {
+                                mx[q] = 0.0;
+                                for (int c = 0; c < 4; c++) mx[q] = std::max(mx[q], op.norm(m, d, c, l[q]));
+                            }
+                            if (!(mx[0] * mx[1] * mx[2] * fnmax * (1.0 + 1e-12) > th)) continue;
+                            for (int ft = 0; ft < 8; ft++) {
+                                if (!(fn[ft] > 0.0)) continue;
+                                for (int gt = 0; gt < 8; gt++) {
+                                    if (d > 0 && ft == 0 && gt == 0) continue;
+                                    ncand += 1.0;
+                                    int cx = ((gt & 1) << 1) | (ft & 1);
+                                    int cy = (((gt >> 1) & 1) << 1) | ((ft >> 1) & 1);
+                                    int cz = (((gt >> 2) & 1) << 1) | ((ft >> 2) & 1);
+                                    double nx = op.norm(m, d, cx, l[0]);
+                                    double ny = op.norm(m, d, cy, l[1]);
+                                    double nz = op.norm(m, d, cz, l[2]);
+                                    double bound = ((nx * ny) * nz) * fn[ft];
+                                    if (!(bound > th)) continue;
+                                    napps += 1.0;
+                                    contract(n, op.block(m, d, cx, l[0]), op.block(m, d, cy, l[1]),
+                                             op.block(m, d, cz, l[2]), fb + ft * n3, acc + gt * n3, t1.data(),
+                                             t2.data());
+                                }
+                            }
+                        }
+                    }
+        }
+    }
+    st.apps += napps;
+    st.cand += ncand;
+}
+
+// TopDown(add) then BottomUp on a complete output tree held as key list + blocks.
+int post_transform(const Geo &geo, const std::vector<Key> &keys, double *blocks) {
+    const int n3 = geo.n3, blk = geo.blk;
+    KeyMap map;
+    map.reserve(keys.size() * 2);
+    int maxd = 0;
+    for (size_t g = 0; g < keys.size(); g++) {
+        map[keys[g]] = (int)g;
+        maxd = std::max(maxd, keys[g].d);
+    }
+    auto child = [&](const Key &k, int t) {
+        auto it = map.find(Key{k.d + 1, 2 * k.x + (t & 1), 2 * k.y + ((t >> 1) & 1), 2 * k.z + ((t >> 2) & 1)});
+        return it == map.end() ? -1 : it->second;
+    };
+    std::vector<std::vector<int>> branches(maxd + 1);
+    for (size_t g = 0; g < keys.size(); g++)
+        if (child(keys[g], 0) >= 0) branches[keys[g].d].push_back((int)g);
+    int bad = 0;
+    for (int d = 0; d <= maxd; d++) {
+        auto &v = branches[d];
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+        for (int q = 0; q < (int)v.size(); q++) {
+            int g = v[q];
+            std::vector<double> rec(blocks + (size_t)g * blk, blocks + (size_t)(g + 1) * blk);
+            reconstruct8(geo, rec.data());
+            for (int t = 0; t < 8; t++) {
+                int ch = child(keys[g], t);
+                if (ch < 0) {
+                    bad++;
+                    continue;
+                }
+                double *dst = blocks + (size_t)ch * blk;
+                for (int e = 0; e < n3; e++) dst[e] += rec[t * n3 + e];
+            }
+        }
+    }
+    for (int d = maxd; d >= 0; d--) {
+        auto &v = branches[d];
+#pragma omp parallel for schedule(static)
+        for (int q = 0; q < (int)v.size(); q++) {
+            int g = v[q];
+            std::vector<double> in(blk, 0.0);
+            for (int t = 0; t < 8; t++) {
+                int ch = child(keys[g], t);
+                if (ch >= 0) std::memcpy(in.data() + t * n3, blocks + (size_t)ch * blk, sizeof(double) * n3);
+            }
+            compress8(geo, in.data());
+            std::memcpy(blocks + (size_t)g * blk, in.data(), sizeof(double) * blk);
+        }
+    }
+    return bad;
+}
+
+Geo make_geo(int kp1, int n0, const int *corner, const int *boxes, const double *F) {
+    Geo g;
+    g.kp1 = kp1;
+    g.n3 = kp1 * kp1 * kp1;
+    g.blk = 8 * g.n3;
+    g.n0 = n0;
+    for (int d = 0; d < 3; d++) {
+        g.corner[d] = corner[d];
+        g.boxes[d] = boxes[d];
+    }
+    g.F = F;
+    return g;
+}
+
+struct AdaptiveResult {
+    std::vector<int> depth;
+    std::vector<long long> l;
+    std::vector<double> thr, coefs;
+};
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#endif
+}
+
+// Frozen-grid apply.  Returns 0 on success, negative on error.
+//   Input tree: n_in nodes with keys (depth, lx, ly, lz) and compressed
+//   blocks (scaling part at every node); complete (all parents present).
+//   Output grid: n_out keys of a complete tree; thr[g] is the screening
+//   threshold of output node g.
+//   out_ns (optional): raw non-standard contributions; out: final blocks.
+//   stats[0] term-applications passing screening, [1] (f,ft,gt,m) candidates
+//   examined, [2] input slots, [3] generated slots, [4] max band half-width.
+int oracle_apply(int kp1, int n0, const int *corner, const int *boxes, const double *F, int M, int ndepth,
+                 const int *L, const long long *boff, const long long *noff, const double *blocks,
+                 const double *norms, int n_in, const int *in_depth, const long long *in_l, const double *in_coefs,
+                 int n_out, const int *out_depth, const long long *out_l, const double *thr, int band_w_override,
+                 double *out_ns, double *out, double *stats) {
+    Geo geo = make_geo(kp1, n0, corner, boxes, F);
+    OperView op{M, ndepth, kp1, L, boff, noff, blocks, norms};
+    int maxd = 0;
+    for (int g = 0; g < n_out; g++) maxd = std::max(maxd, out_depth[g]);
+    if (maxd >= ndepth) return -2;
+    Input in;
+    in.geo = &geo;
+    in.init(n_in, in_depth, in_l, in_coefs, maxd);
+    std::vector<Key> keys(n_out);
+    for (int g = 0; g < n_out; g++) keys[g] = Key{out_depth[g], out_l[3 * g], out_l[3 * g + 1], out_l[3 * g + 2]};
+    std::vector<double> th(thr, thr + n_out);
+    Stats st;
+    compute_nodes(geo, op, in, keys, th, band_w_override, out, st);
+    if (out_ns) std::memcpy(out_ns, out, sizeof(double) * (size_t)n_out * geo.blk);
+    int bad = post_transform(geo, keys, out);
+    if (stats) {
+        stats[0] = st.apps;
+        stats[1] = st.cand;
+        stats[2] = (double)in.slots.size();
+        stats[3] = (double)in.ngen;
+        stats[4] = st.wmax;
+    }
+    return bad ? -3 : 0;
+}
+
+// Adaptive apply (TreeBuilder + WaveletAdaptor restatement).  Grows the output
+// tree from the roots; node thresholds are prec * sqrt(Nsq / M) with Nsq the
+// output norm estimate after the previous level (0 before the first), and a
+// computed node splits when ||d|| > max(2 eps, prec * t_norm * 2^{-(n+1)/2}),
+// t_norm = 1 for absolute precision, sqrt(Nsq) otherwise.  max_iter < 0:
+// unlimited refinement.  Results are kept in a handle read back with
+// oracle_result_* and released with oracle_result_free.
+void *oracle_apply_adaptive(int kp1, int n0, const int *corner, const int *boxes, const double *F, int M,
+                            int ndepth, const int *L, const long long *boff, const long long *noff,
+                            const double *blocks, const double *norms, int n_in, const int *in_depth,
+                            const long long *in_l, const double *in_coefs, double prec, int abs_prec,
+                            int max_iter, int max_depth, double *stats) {
+    Geo geo = make_geo(kp1, n0, corner, boxes, F);
+    OperView op{M, ndepth, kp1, L, boff, noff, blocks, norms};
+    if (max_depth < 0 || max_depth >= ndepth) max_depth = ndepth - 1;
+    Input in;
+    in.geo = &geo;
+    in.init(n_in, in_depth, in_l, in_coefs, max_depth);
+    const int n3 = geo.n3, blk = geo.blk;
+    std::vector<Key> keys;
+    for (int z = corner[2]; z < corner[2] + boxes[2]; z++)
+        for (int y = corner[1]; y < corner[1] + boxes[1]; y++)
+            for (int x = corner[0]; x < corner[0] + boxes[0]; x++) keys.push_back(Key{0, x, y, z});
+    std::vector<double> coefs, thr;
+    std::vector<int> work;
+    for (int i = 0; i < (int)keys.size(); i++) work.push_back(i);
+    double s_norm = 0.0, w_norm = 0.0;
+    Stats st;
+    int iter = 0;
+    while (!work.empty()) {
+        double Nsq = s_norm + w_norm;
+        double th = prec * std::sqrt(Nsq / M);
+        std::vector<Key> lvl;
+        for (int i : work) lvl.push_back(keys[i]);
+        std::vector<double> lthr(lvl.size(), th);
+        std::vector<double> ns(lvl.size() * (size_t)blk);
+        compute_nodes(geo, op, in, lvl, lthr, -1, ns.data(), st);
+        coefs.resize(keys.size() * (size_t)blk, 0.0);
+        thr.resize(keys.size(), 0.0);
+        std::vector<double> wn(lvl.size());
+        for (size_t q = 0; q < lvl.size(); q++) {
+            int i = work[q];
+            std::memcpy(coefs.data() + (size_t)i * blk, ns.data() + q * blk, sizeof(double) * blk);
+            thr[i] = th;
+            double w = 0.0, s = 0.0;
+            for (int e = 0; e < n3; e++) s += ns[q * blk + e] * ns[q * blk + e];
+            for (int e = n3; e < blk; e++) w += ns[q * blk + e] * ns[q * blk + e];
+            wn[q] = std::sqrt(w);
+            w_norm += w;
+            if (lvl[q].d == 0) s_norm += s;
+        }
+        Nsq = s_norm + w_norm;
+        double tnorm = abs_prec ? 1.0 : std::sqrt(Nsq);
+        std::vector<int> next;
+        if (max_iter < 0 || iter < max_iter) {
+            for (size_t q = 0; q < lvl.size(); q++) {
+                const Key k = lvl[q];
+                if (k.d >= max_depth) continue;
+                double scale = geo.n0 + k.d;
+                double wthr = std::max(2.0 * 2.220446049250313e-16, prec * tnorm * std::pow(2.0, -0.5 * (scale + 1)));
+                if (!(wn[q] > wthr)) continue;
+                for (int t = 0; t < 8; t++) {
+                    next.push_back((int)keys.size());
+                    keys.push_back(Key{k.d + 1, 2 * k.x + (t & 1), 2 * k.y + ((t >> 1) & 1), 2 * k.z + ((t >> 2) & 1)});
+                }
+            }
+        }
+        work.swap(next);
+        iter++;
+    }
+    coefs.resize(keys.size() * (size_t)blk, 0.0);
+    thr.resize(keys.size(), 0.0);
+    post_transform(geo, keys, coefs.data());
+    auto *res = new AdaptiveResult;
+    for (auto &k : keys) {
+        res->depth.push_back(k.d);
+        res->l.push_back(k.x);
+        res->l.push_back(k.y);
+        res->l.push_back(k.z);
+    }
+    res->thr = thr;
+    res->coefs.swap(coefs);
+    if (stats) {
+        stats[0] = st.apps;
+        stats[1] = st.cand;
+        stats[2] = (double)in.slots.size();
+        stats[3] = (double)in.ngen;
+        stats[4] = st.wmax;
+    }
+    return res;
+}
+
+int oracle_result_size(void *h) { return (int)static_cast<AdaptiveResult *>(h)->depth.size(); }
+void oracle_result_get(void *h, int *depth, long long *l, double *thr, double *coefs) {
+    auto *r = static_cast<AdaptiveResult *>(h);
+    if (depth) std::memcpy(depth, r->depth.data(), sizeof(int) * r->depth.size());
+    if (l) std::memcpy(l, r->l.data(), sizeof(long long) * r->l.size());
+    if (thr) std::memcpy(thr, r->thr.data(), sizeof(double) * r->thr.size());
+    if (coefs) std::memcpy(coefs, r->coefs.data(), sizeof(double) * r->coefs.size());
+}
+void oracle_result_free(void *h) { delete static_cast<AdaptiveResult *>(h); }
+
+}  // extern "C"
