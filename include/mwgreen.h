@@ -44,6 +44,7 @@ typedef struct mwg_context_s *mwg_context;
 typedef struct mwg_mra_s *mwg_mra;
 typedef struct mwg_operator_s *mwg_operator;
 typedef struct mwg_tree_s *mwg_tree;
+typedef struct mwg_plan_s *mwg_plan;
 
 const char *mwg_last_error(void);
 
@@ -112,8 +113,27 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
 
 /* Statistics of the last apply on this context: [0] kernel milliseconds,
  * [1] output nodes computed, [2] input slots, [3] generated slots,
- * [4] host plan milliseconds, [5] kernel launches. */
+ * [4] host plan milliseconds, [5] kernel launches, [6] term-applications
+ * that passed screening (each 6(k+1)^4 flops of algorithmic work). */
 int mwg_last_apply_stats(mwg_context ctx, double *stats, int n);
+
+/* Frozen-grid apply split into a resident device plan (bands, input slots,
+ * operator tables and output blocks in HBM) and repeated executions on it.
+ * mwg_apply_fixed = create + execute + finish + destroy.  No counterpart in
+ * MRChem's calls; it serves repeated applies on one grid and measurement. */
+int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode,
+                    mwg_plan *plan);
+/* One execution of the per-node calculator on resident data; kernel_ms is the
+ * device time, term_apps the term-applications that passed screening. */
+int mwg_plan_execute(mwg_plan plan, double *kernel_ms, double *term_apps);
+/* Copy the result into `out` and run the TopDown/BottomUp transforms. */
+int mwg_plan_finish(mwg_plan plan);
+/* Raw non-standard blocks of the last execution (n_out x 8(k+1)^3, tree
+ * order), before the TopDown/BottomUp transforms. */
+int mwg_plan_download_ns(mwg_plan plan, double *ns);
+/* [0] output nodes, [1] input slots, [2] generated slots, [3] host plan ms. */
+int mwg_plan_info(mwg_plan plan, double *info, int n);
+int mwg_plan_destroy(mwg_plan plan);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ import os
 import numpy as np
 
 __all__ = ["MultiResolutionAnalysis", "PoissonOperator", "HelmholtzOperator", "GaussianOperator",
-           "FunctionTree", "Context", "apply", "apply_fixed", "MWGError", "lib", "LIB_PATH"]
+           "FunctionTree", "Context", "ApplyPlan", "apply", "apply_fixed", "MWGError", "lib", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmwgreen.so")
 if not os.path.exists(LIB_PATH):
@@ -75,6 +75,12 @@ _sig("mwg_apply", _p, _d, _p, _p, _p, _i, _i)
 _sig("mwg_apply_prec_trees", _p, _d, _p, _p, _p, ctypes.POINTER(_p), _i, _i, _i)
 _sig("mwg_apply_fixed", _p, _p, _p, _p, _dp, _i)
 _sig("mwg_last_apply_stats", _p, _dp, _i)
+_sig("mwg_plan_create", _p, _p, _p, _p, _dp, _i, ctypes.POINTER(_p))
+_sig("mwg_plan_execute", _p, _dp, _dp)
+_sig("mwg_plan_finish", _p)
+_sig("mwg_plan_info", _p, _dp, _i)
+_sig("mwg_plan_download_ns", _p, _dp)
+_sig("mwg_plan_destroy", _p)
 
 MODES = {"fp64": 0}
 
@@ -112,7 +118,7 @@ class Context:
         s = np.zeros(8)
         _check(lib.mwg_last_apply_stats(self._h, _ptr(s, _d), 8))
         return {"kernel_ms": s[0], "out_nodes": int(s[1]), "in_slots": int(s[2]), "generated": int(s[3]),
-                "plan_ms": s[4], "launches": int(s[5])}
+                "plan_ms": s[4], "launches": int(s[5]), "term_apps": int(s[6])}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -328,3 +334,43 @@ def apply_fixed(out, oper, inp, thr, mode="fp64", ctx=None):
         raise ValueError("one threshold per output node is required")
     _check(lib.mwg_apply_fixed(ctx._h, out._h, oper._h, inp._h, _ptr(t, _d), MODES[mode]))
     return out
+
+
+class ApplyPlan:
+    """Frozen-grid apply resident on the device; execute() reruns the kernel."""
+
+    def __init__(self, out, oper, inp, thr, mode="fp64", ctx=None):
+        self.ctx = ctx or Context.default()
+        t = np.ascontiguousarray(thr, dtype=np.float64)
+        if t.size != out.size():
+            raise ValueError("one threshold per output node is required")
+        h = _p()
+        _check(lib.mwg_plan_create(self.ctx._h, out._h, oper._h, inp._h, _ptr(t, _d), MODES[mode], ctypes.byref(h)))
+        self._h = h
+        self._keep = (out, oper, inp)
+
+    def execute(self):
+        ms, apps = _d(), _d()
+        _check(lib.mwg_plan_execute(self._h, ctypes.byref(ms), ctypes.byref(apps)))
+        return ms.value, int(apps.value)
+
+    def finish(self):
+        _check(lib.mwg_plan_finish(self._h))
+        return self._keep[0]
+
+    def download_ns(self):
+        n = self._keep[0].size()
+        ns = np.zeros((n, self._keep[0].mra.block))
+        _check(lib.mwg_plan_download_ns(self._h, _ptr(ns, _d)))
+        return ns
+
+    def info(self):
+        v = np.zeros(4)
+        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 4))
+        return {"out_nodes": int(v[0]), "in_slots": int(v[1]), "generated": int(v[2]), "plan_ms": v[3]}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mwg_plan_destroy(h)
+            self._h = None

@@ -54,6 +54,8 @@ struct KArgs {
     // screening tables
     const int *bw;
     const int *act_off, *act_terms;
+    // [0] term-applications passing screening (algorithmic work)
+    unsigned long long *counters;
 };
 
 __device__ __forceinline__ double op_norm(const KArgs &a, int m, int d, int c, int l) {
@@ -85,6 +87,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_fp64(KArgs a) {
     for (int t = 0; t < 8; t++)
 #pragma unroll
         for (int q = 0; q < NQ; q++) acc[t][q] = 0.0;
+    unsigned long long napps = 0;
 
     const int nband = side * side * side;
     for (int p = 0; p < nband; p++) {
@@ -110,7 +113,10 @@ __global__ __launch_bounds__(kThreads) void k_apply_fp64(KArgs a) {
             const double ny = op_norm(a, m, d, cy, ly);
             const double nz = op_norm(a, m, d, cz, lz);
             const double bound = ((nx * ny) * nz) * fn[ft];
-            if (bound > thr) atomicOr(&mask_s[ai * 2 + (ft >> 2)], 1u << ((ft & 3) * 8 + gt));
+            if (bound > thr) {
+                atomicOr(&mask_s[ai * 2 + (ft >> 2)], 1u << ((ft & 3) * 8 + gt));
+                napps++;
+            }
         }
         __syncthreads();
         const double *fsrc = a.pool + a.slot_off[s];
@@ -196,6 +202,9 @@ __global__ __launch_bounds__(kThreads) void k_apply_fp64(KArgs a) {
             if (loaded) __syncthreads();
         }
     }
+    // one atomic per wavefront for the work counter
+    for (int off = 32; off > 0; off >>= 1) napps += __shfl_down(napps, off);
+    if ((tid & 63) == 0 && napps) atomicAdd(&a.counters[0], napps);
     double *dst = a.out_ns + (long long)g * 8 * N3;
 #pragma unroll
     for (int t = 0; t < 8; t++)
@@ -206,12 +215,12 @@ __global__ __launch_bounds__(kThreads) void k_apply_fp64(KArgs a) {
         }
 }
 
-// ------------------------------------------------------------------ host side
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
 
+// ------------------------------------------------------------------ host side
 struct DevOper {
     int *L = nullptr;
     long long *boff = nullptr, *noff = nullptr;
@@ -223,33 +232,19 @@ struct DeviceCtx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::map<const void *, DevOper> opers;
-    std::vector<DevBuf> scratch;  // reused plan buffers
 };
 
-static int ensure(DeviceCtx *c, int slot, size_t bytes, void **out, std::string &err) {
-    if ((int)c->scratch.size() <= slot) c->scratch.resize(slot + 1);
-    DevBuf &b = c->scratch[slot];
-    if (b.bytes < bytes) {
-        if (b.p) HIP_TRY(hipFree(b.p));
-        b.p = nullptr;
-        b.bytes = 0;
-        size_t want = std::max<size_t>(bytes, 256);
-        HIP_TRY(hipMalloc(&b.p, want));
-        b.bytes = want;
+struct DevicePlan {
+    DeviceCtx *ctx = nullptr;
+    int n_out = 0, kp1 = 0;
+    KArgs args{};
+    std::vector<void *> bufs;
+    size_t out_bytes = 0;
+    ~DevicePlan() {
+        for (void *p : bufs)
+            if (p) (void)hipFree(p);
     }
-    *out = b.p;
-    return 0;
-}
-
-template <typename T>
-static int upload(DeviceCtx *c, int slot, const std::vector<T> &v, T **out, std::string &err) {
-    void *p = nullptr;
-    int rc = ensure(c, slot, sizeof(T) * std::max<size_t>(v.size(), 1), &p, err);
-    if (rc) return rc;
-    if (!v.empty()) HIP_TRY(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
-    *out = static_cast<T *>(p);
-    return 0;
-}
+};
 
 int dev_create(int device, DeviceCtx **out, std::string &err) {
     int n = 0;
@@ -287,8 +282,6 @@ void dev_destroy(DeviceCtx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->opers) free_oper(kv.second);
-    for (auto &b : c->scratch)
-        if (b.p) (void)hipFree(b.p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -332,6 +325,89 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
     return 0;
 }
 
+template <typename T>
+static int plan_copy(DevicePlan *p, const std::vector<T> &v, const T **out, std::string &err) {
+    T *d = nullptr;
+    int rc = alloc_copy(v, &d, p->ctx->stream, err);
+    p->bufs.push_back(d);
+    if (rc) return rc;
+    *out = d;
+    return 0;
+}
+
+int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, DevicePlan **out,
+                    std::string &err) {
+    HIP_TRY(hipSetDevice(c->device));
+    for (size_t d = 0; d + 1 < plan.active_off.size(); d++)
+        if (plan.active_off[d + 1] - plan.active_off[d] > kMaxActive) {
+            err = "more than " + std::to_string(kMaxActive) + " active terms at one depth";
+            return -1;
+        }
+    if (plan.kp1 < 2 || plan.kp1 > 12) {
+        err = "unsupported basis size k+1 = " + std::to_string(plan.kp1);
+        return -1;
+    }
+    DevOper *o = nullptr;
+    int rc = get_oper(c, op_key, f, &o, err);
+    if (rc) return rc;
+    auto *p = new DevicePlan;
+    p->ctx = c;
+    p->n_out = (int)plan.out_node.size();
+    p->kp1 = plan.kp1;
+    KArgs &a = p->args;
+    rc = plan_copy(p, plan.out_depth, &a.out_depth, err);
+    if (!rc) rc = plan_copy(p, plan.out_thr, &a.out_thr, err);
+    if (!rc) rc = plan_copy(p, plan.band_off, &a.band_off, err);
+    if (!rc) rc = plan_copy(p, plan.band_w, &a.band_w, err);
+    if (!rc) rc = plan_copy(p, plan.band, &a.band, err);
+    if (!rc) rc = plan_copy(p, plan.slot_off, &a.slot_off, err);
+    if (!rc) rc = plan_copy(p, plan.in_coefs, &a.pool, err);
+    if (!rc) rc = plan_copy(p, plan.in_norms, &a.slot_norm, err);
+    if (!rc) rc = plan_copy(p, plan.bw, &a.bw, err);
+    if (!rc) rc = plan_copy(p, plan.active_off, &a.act_off, err);
+    if (!rc) rc = plan_copy(p, plan.active_terms, &a.act_terms, err);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    const int N = plan.kp1;
+    p->out_bytes = sizeof(double) * (size_t)std::max(p->n_out, 1) * 8 * N * N * N;
+    void *d_out = nullptr, *d_cnt = nullptr;
+    hipError_t e = hipMalloc(&d_out, p->out_bytes);
+    if (e == hipSuccess) p->bufs.push_back(d_out);
+    if (e == hipSuccess) e = hipMalloc(&d_cnt, 64);
+    if (e == hipSuccess) p->bufs.push_back(d_cnt);
+    if (e != hipSuccess) {
+        err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        delete p;
+        return -2;
+    }
+    a.out_ns = static_cast<double *>(d_out);
+    a.counters = static_cast<unsigned long long *>(d_cnt);
+    a.opL = o->L;
+    a.op_boff = o->boff;
+    a.op_noff = o->noff;
+    a.op_blocks = o->blocks;
+    a.op_norms = o->norms;
+    a.M = f.M;
+    a.ndepth = f.ndepth;
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        err = std::string("upload: ") + hipGetErrorString(e);
+        delete p;
+        return -2;
+    }
+    *out = p;
+    return 0;
+}
+
+void dev_plan_destroy(DevicePlan *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    delete p;
+}
+
 template <int N>
 static int launch_n(const KArgs &a, int n_out, hipStream_t s, std::string &err) {
     constexpr int N2 = N * N, N3 = N2 * N;
@@ -342,83 +418,65 @@ static int launch_n(const KArgs &a, int n_out, hipStream_t s, std::string &err) 
     return 0;
 }
 
-int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, double *out_ns,
-                   double *kernel_ms, std::string &err) {
+int dev_plan_run(DevicePlan *p, double *kernel_ms, unsigned long long *term_apps, std::string &err) {
+    DeviceCtx *c = p->ctx;
     HIP_TRY(hipSetDevice(c->device));
-    const int n_out = (int)plan.out_node.size();
-    if (n_out == 0) {
+    if (p->n_out == 0) {
         if (kernel_ms) *kernel_ms = 0.0;
+        if (term_apps) *term_apps = 0;
         return 0;
     }
-    for (size_t d = 0; d + 1 < plan.active_off.size(); d++)
-        if (plan.active_off[d + 1] - plan.active_off[d] > kMaxActive) {
-            err = "more than " + std::to_string(kMaxActive) + " active terms at one depth";
-            return -1;
-        }
-    DevOper *o = nullptr;
-    int rc = get_oper(c, op_key, f, &o, err);
-    if (rc) return rc;
-    KArgs a{};
-    int *i_depth, *i_bw, *i_band, *i_bwt, *i_aoff, *i_aterm;
-    double *d_thr, *d_pool, *d_norm;
-    long long *l_boff, *l_soff;
-    if ((rc = upload(c, 0, plan.out_depth, &i_depth, err))) return rc;
-    if ((rc = upload(c, 1, plan.out_thr, &d_thr, err))) return rc;
-    if ((rc = upload(c, 2, plan.band_off, &l_boff, err))) return rc;
-    if ((rc = upload(c, 3, plan.band_w, &i_bw, err))) return rc;
-    if ((rc = upload(c, 4, plan.band, &i_band, err))) return rc;
-    if ((rc = upload(c, 5, plan.slot_off, &l_soff, err))) return rc;
-    if ((rc = upload(c, 6, plan.in_coefs, &d_pool, err))) return rc;
-    if ((rc = upload(c, 7, plan.in_norms, &d_norm, err))) return rc;
-    if ((rc = upload(c, 8, plan.bw, &i_bwt, err))) return rc;
-    if ((rc = upload(c, 9, plan.active_off, &i_aoff, err))) return rc;
-    if ((rc = upload(c, 10, plan.active_terms, &i_aterm, err))) return rc;
-    const int N = plan.kp1;
-    size_t out_bytes = sizeof(double) * (size_t)n_out * 8 * N * N * N;
-    void *d_out = nullptr;
-    if ((rc = ensure(c, 11, out_bytes, &d_out, err))) return rc;
-    a.out_depth = i_depth;
-    a.out_thr = d_thr;
-    a.band_off = l_boff;
-    a.band_w = i_bw;
-    a.band = i_band;
-    a.out_ns = static_cast<double *>(d_out);
-    a.slot_off = l_soff;
-    a.pool = d_pool;
-    a.slot_norm = d_norm;
-    a.opL = o->L;
-    a.op_boff = o->boff;
-    a.op_noff = o->noff;
-    a.op_blocks = o->blocks;
-    a.op_norms = o->norms;
-    a.M = f.M;
-    a.ndepth = f.ndepth;
-    a.bw = i_bwt;
-    a.act_off = i_aoff;
-    a.act_terms = i_aterm;
+    // counters cleared by a kernel (not a memset node, so the launch sequence
+    // can be captured into a graph)
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, p->args.counters, 8);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    switch (N) {
-        case 2: rc = launch_n<2>(a, n_out, c->stream, err); break;
-        case 3: rc = launch_n<3>(a, n_out, c->stream, err); break;
-        case 4: rc = launch_n<4>(a, n_out, c->stream, err); break;
-        case 5: rc = launch_n<5>(a, n_out, c->stream, err); break;
-        case 6: rc = launch_n<6>(a, n_out, c->stream, err); break;
-        case 7: rc = launch_n<7>(a, n_out, c->stream, err); break;
-        case 8: rc = launch_n<8>(a, n_out, c->stream, err); break;
-        case 9: rc = launch_n<9>(a, n_out, c->stream, err); break;
-        case 10: rc = launch_n<10>(a, n_out, c->stream, err); break;
-        case 11: rc = launch_n<11>(a, n_out, c->stream, err); break;
-        case 12: rc = launch_n<12>(a, n_out, c->stream, err); break;
-        default: err = "unsupported basis size k+1 = " + std::to_string(N); return -1;
+    int rc = 0;
+    switch (p->kp1) {
+        case 2: rc = launch_n<2>(p->args, p->n_out, c->stream, err); break;
+        case 3: rc = launch_n<3>(p->args, p->n_out, c->stream, err); break;
+        case 4: rc = launch_n<4>(p->args, p->n_out, c->stream, err); break;
+        case 5: rc = launch_n<5>(p->args, p->n_out, c->stream, err); break;
+        case 6: rc = launch_n<6>(p->args, p->n_out, c->stream, err); break;
+        case 7: rc = launch_n<7>(p->args, p->n_out, c->stream, err); break;
+        case 8: rc = launch_n<8>(p->args, p->n_out, c->stream, err); break;
+        case 9: rc = launch_n<9>(p->args, p->n_out, c->stream, err); break;
+        case 10: rc = launch_n<10>(p->args, p->n_out, c->stream, err); break;
+        case 11: rc = launch_n<11>(p->args, p->n_out, c->stream, err); break;
+        case 12: rc = launch_n<12>(p->args, p->n_out, c->stream, err); break;
+        default: err = "unsupported basis size"; return -1;
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    HIP_TRY(hipMemcpyAsync(out_ns, d_out, out_bytes, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long cnt[8];
+    HIP_TRY(hipMemcpyAsync(cnt, p->args.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (kernel_ms) *kernel_ms = ms;
+    if (term_apps) *term_apps = cnt[0];
     return 0;
+}
+
+int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    if (p->n_out == 0) return 0;
+    HIP_TRY(hipMemcpyAsync(out_ns, p->args.out_ns, sizeof(double) * (size_t)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, double *out_ns,
+                   double *kernel_ms, unsigned long long *term_apps, std::string &err) {
+    DevicePlan *p = nullptr;
+    int rc = dev_plan_create(c, op_key, f, plan, &p, err);
+    if (rc) return rc;
+    rc = dev_plan_run(p, kernel_ms, term_apps, err);
+    if (!rc) rc = dev_plan_download(p, out_ns, err);
+    dev_plan_destroy(p);
+    return rc;
 }
 
 }  // namespace mwg

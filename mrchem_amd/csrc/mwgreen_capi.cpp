@@ -27,6 +27,15 @@ using namespace mwg;
 struct mwg_context_s {
     DeviceCtx *dev = nullptr;
     std::vector<double> last_stats = std::vector<double>(8, 0.0);
+    double term_apps = 0.0;  // accumulated by run_plan within one apply
+};
+struct mwg_plan_s {
+    mwg_context ctx = nullptr;
+    DevicePlan *dev = nullptr;
+    mwg_tree out = nullptr;
+    std::vector<double> thr;
+    int n_out = 0;
+    double plan_ms = 0.0, slots = 0.0, gen = 0.0;
 };
 struct mwg_mra_s {
     MRA mra;
@@ -112,9 +121,11 @@ int run_plan(mwg_context ctx, mwg_operator_s *op, const Tree &inp, Tree &out, co
     ns.assign(nodes.size() * (size_t)out.mra->block(), 0.0);
     std::string err;
     double ms = 0.0;
-    int rc = dev_apply_plan(ctx->dev, op, op->flat, plan, ns.data(), &ms, err);
+    unsigned long long apps = 0;
+    int rc = dev_apply_plan(ctx->dev, op, op->flat, plan, ns.data(), &ms, &apps, err);
     if (rc) return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
     kms += ms;
+    ctx->term_apps += (double)apps;
     return MWG_OK;
 }
 
@@ -137,6 +148,7 @@ int apply_adaptive(mwg_context ctx, double prec, mwg_tree_s *out, mwg_operator_s
     const int max_depth = std::min(mra.max_depth, op->op.max_depth);
     o.clear();
     o.make_roots();
+    ctx->term_apps = 0.0;
     std::vector<int> work;
     for (int i = 0; i < o.size(); i++) work.push_back(i);
     std::vector<double> thr_all(o.size(), 0.0);
@@ -194,6 +206,7 @@ int apply_adaptive(mwg_context ctx, double prec, mwg_tree_s *out, mwg_operator_s
     st[3] = gen;
     st[4] = pms;
     st[5] = launches;
+    st[6] = ctx->term_apps;
     return MWG_OK;
 }
 
@@ -533,6 +546,7 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
         }
         double kms = 0, pms = 0, slots = 0, gen = 0;
         std::vector<double> ns;
+        ctx->term_apps = 0.0;
         int rc = run_plan(ctx, op, inp->tree, o, nodes, th, ns, kms, pms, &slots, &gen);
         if (rc) return rc;
         std::memcpy(o.coefs.data(), ns.data(), sizeof(double) * (size_t)n * blk);
@@ -546,8 +560,98 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
         st[3] = gen;
         st[4] = pms;
         st[5] = 1;
+        st[6] = ctx->term_apps;
         return MWG_OK;
     });
+}
+
+int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode,
+                    mwg_plan *plan) {
+    return guarded([&] {
+        if (!ctx || !ctx->dev || !out || !op || !inp || !thr || !plan) return fail(MWG_ERR_ARG, "null argument");
+        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
+        if (mode != MWG_MODE_FP64) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+        Tree &o = out->tree;
+        check_compatible(o, op->op, inp->tree);
+        const int n = o.size();
+        std::vector<int> nodes(n);
+        std::vector<double> th(thr, thr + n);
+        for (int i = 0; i < n; i++) {
+            nodes[i] = i;
+            if (!(th[i] >= 0.0) || !std::isfinite(th[i])) return fail(MWG_ERR_ARG, "thresholds must be finite and >= 0");
+        }
+        double t0 = now_ms();
+        ApplyPlan ap;
+        build_plan(ap, op->op, inp->tree, o, nodes, th, 64);
+        auto *p = new mwg_plan_s;
+        p->plan_ms = now_ms() - t0;
+        p->slots = (double)ap.slot_off.size();
+        p->gen = (double)ap.n_generated;
+        std::string err;
+        int rc = dev_plan_create(ctx->dev, op, op->flat, ap, &p->dev, err);
+        if (rc) {
+            delete p;
+            return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
+        }
+        p->ctx = ctx;
+        p->out = out;
+        p->thr = th;
+        p->n_out = n;
+        *plan = p;
+        return MWG_OK;
+    });
+}
+
+int mwg_plan_execute(mwg_plan plan, double *kernel_ms, double *term_apps) {
+    return guarded([&] {
+        if (!plan) return fail(MWG_ERR_ARG, "null plan");
+        std::string err;
+        double ms = 0.0;
+        unsigned long long apps = 0;
+        int rc = dev_plan_run(plan->dev, &ms, &apps, err);
+        if (rc) return fail(MWG_ERR_DEVICE, err);
+        if (kernel_ms) *kernel_ms = ms;
+        if (term_apps) *term_apps = (double)apps;
+        return MWG_OK;
+    });
+}
+
+int mwg_plan_finish(mwg_plan plan) {
+    return guarded([&] {
+        if (!plan) return fail(MWG_ERR_ARG, "null plan");
+        Tree &o = plan->out->tree;
+        if (o.size() != plan->n_out) return fail(MWG_ERR_ARG, "output tree changed since the plan was made");
+        std::string err;
+        int rc = dev_plan_download(plan->dev, o.coefs.data(), err);
+        if (rc) return fail(MWG_ERR_DEVICE, err);
+        finish_output(o);
+        plan->out->apply_thr = plan->thr;
+        return MWG_OK;
+    });
+}
+
+int mwg_plan_download_ns(mwg_plan plan, double *ns) {
+    return guarded([&] {
+        if (!plan || !ns) return fail(MWG_ERR_ARG, "null pointer argument");
+        std::string err;
+        int rc = dev_plan_download(plan->dev, ns, err);
+        if (rc) return fail(MWG_ERR_DEVICE, err);
+        return MWG_OK;
+    });
+}
+
+int mwg_plan_info(mwg_plan plan, double *info, int n) {
+    if (!plan || !info) return fail(MWG_ERR_ARG, "null pointer argument");
+    double v[4] = {(double)plan->n_out, plan->slots, plan->gen, plan->plan_ms};
+    for (int i = 0; i < n && i < 4; i++) info[i] = v[i];
+    return MWG_OK;
+}
+
+int mwg_plan_destroy(mwg_plan plan) {
+    if (!plan) return MWG_OK;
+    dev_plan_destroy(plan->dev);
+    delete plan;
+    return MWG_OK;
 }
 
 int mwg_last_apply_stats(mwg_context ctx, double *stats, int n) {

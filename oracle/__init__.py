@@ -60,7 +60,7 @@ def _declare(lib):
     lib.oracle_set_threads.argtypes = [_i]
     lib.oracle_set_threads.restype = None
     lib.oracle_apply.argtypes = [_i, _i, _ip, _ip, _dp, _i, _i, _ip, _llp, _llp, _dp, _dp, _i, _ip, _llp, _dp, _i, _ip,
-                                 _llp, _dp, _i, _dp, _dp, _dp]
+                                 _llp, _dp, _i, _i, _dp, _dp, _dp]
     lib.oracle_apply.restype = _i
     lib.oracle_apply_adaptive.argtypes = [_i, _i, _ip, _ip, _dp, _i, _i, _ip, _llp, _llp, _dp, _dp, _i, _ip, _llp,
                                           _dp, _d, _i, _i, _i, _dp]
@@ -95,9 +95,11 @@ def _common(mra, tables, in_tree):
             _c(coefs, np.float64))
 
 
-def apply_fixed(mra, tables, in_tree, out_depth, out_l, thr, band_w=-1, want_ns=False):
+def apply_fixed(mra, tables, in_tree, out_depth, out_l, thr, band_w=-1, want_ns=False, post=True):
     """Frozen-grid FP64 apply.  in_tree = (depth, l, coefs) of a complete
-    compressed input tree; returns (out_blocks, stats[, ns_blocks])."""
+    compressed input tree; returns (out_blocks, stats[, ns_blocks]).  With
+    post=False the output nodes may be any sample (no post transforms; the
+    returned blocks are the raw non-standard contributions)."""
     corner, boxes, F, L, boff, noff, blocks, norms, dep, l, coefs = _common(mra, tables, in_tree)
     od = _c(out_depth, np.int32)
     ol = _c(out_l, np.int64)
@@ -111,7 +113,7 @@ def apply_fixed(mra, tables, in_tree, out_depth, out_l, thr, band_w=-1, want_ns=
                           tables["ndepth"], _ptr(L, _i), _ptr(boff, ctypes.c_longlong),
                           _ptr(noff, ctypes.c_longlong), _ptr(blocks, _d), _ptr(norms, _d), len(dep), _ptr(dep, _i),
                           _ptr(l, ctypes.c_longlong), _ptr(coefs, _d), n_out, _ptr(od, _i),
-                          _ptr(ol, ctypes.c_longlong), _ptr(th, _d), band_w,
+                          _ptr(ol, ctypes.c_longlong), _ptr(th, _d), band_w, 0 if post else 1,
                           _ptr(ns, _d) if want_ns else None, _ptr(out, _d), _ptr(stats, _d))
     if rc != 0:
         raise RuntimeError("oracle_apply failed with %d" % rc)

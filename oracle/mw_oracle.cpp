@@ -452,13 +452,15 @@ void oracle_set_threads(int n) {
 //   Output grid: n_out keys of a complete tree; thr[g] is the screening
 //   threshold of output node g.
 //   out_ns (optional): raw non-standard contributions; out: final blocks.
+//   flags bit 0: skip the post transforms (out = raw contributions), which
+//   lets a sample of output nodes that is not a complete tree be timed.
 //   stats[0] term-applications passing screening, [1] (f,ft,gt,m) candidates
 //   examined, [2] input slots, [3] generated slots, [4] max band half-width.
 int oracle_apply(int kp1, int n0, const int *corner, const int *boxes, const double *F, int M, int ndepth,
                  const int *L, const long long *boff, const long long *noff, const double *blocks,
                  const double *norms, int n_in, const int *in_depth, const long long *in_l, const double *in_coefs,
                  int n_out, const int *out_depth, const long long *out_l, const double *thr, int band_w_override,
-                 double *out_ns, double *out, double *stats) {
+                 int flags, double *out_ns, double *out, double *stats) {
     Geo geo = make_geo(kp1, n0, corner, boxes, F);
     OperView op{M, ndepth, kp1, L, boff, noff, blocks, norms};
     int maxd = 0;
@@ -473,7 +475,7 @@ int oracle_apply(int kp1, int n0, const int *corner, const int *boxes, const dou
     Stats st;
     compute_nodes(geo, op, in, keys, th, band_w_override, out, st);
     if (out_ns) std::memcpy(out_ns, out, sizeof(double) * (size_t)n_out * geo.blk);
-    int bad = post_transform(geo, keys, out);
+    int bad = (flags & 1) ? 0 : post_transform(geo, keys, out);
     if (stats) {
         stats[0] = st.apps;
         stats[1] = st.cand;
