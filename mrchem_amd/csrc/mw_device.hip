@@ -2,14 +2,9 @@
 //
 // The per-node calculator of upstream MRCPP's OperApplicationCalculator
 // (called from mrcpp::apply, src/qmoperators/two_electron/CoulombPotential.cpp:131)
-// runs here: one workgroup per output node g.  For every band node f the
-// workgroup screens all (term, ft, gt) candidates in parallel with the FP64
-// bound ||O_x|| ||O_y|| ||O_z|| ||f_ft|| > thr_g (same operation order as the
-// host oracle, so the surviving set is identical), then applies the surviving
-// term-applications as three passes of (k+1)x(k+1) contractions through LDS,
-// sharing the x pass between the output components that agree in x and the y
-// pass between those that agree in x and y.  Accumulation is FP64 in
-// registers; each thread owns a fixed strided slice of the 8(k+1)^3 output.
+// runs here.  Screening uses the FP64 bound ||O_x|| ||O_y|| ||O_z|| ||f_ft||
+// > thr_g evaluated in the same operation order as the host oracle, so the
+// set of surviving term-applications is identical.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,8 +26,7 @@ namespace mwg {
         }                                                                                  \
     } while (0)
 
-constexpr int kThreads = 256;
-constexpr int kMaxActive = 512;
+constexpr int kMaxActive = 4096;
 
 struct KArgs {
     // output nodes
@@ -64,155 +58,249 @@ __device__ __forceinline__ double op_norm(const KArgs &a, int m, int d, int c, i
     return a.op_norms[a.op_noff[m * a.ndepth + d] + (long long)c * (2 * L + 1) + (l + L)];
 }
 
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_apply_fp64(KArgs a) {
-    constexpr int N2 = N * N, N3 = N2 * N, NQ = (N3 + kThreads - 1) / kThreads;
-    extern __shared__ double smem[];
-    double *fblk = smem;              // N3
-    double *X = fblk + N3;            // 2 * N3
-    double *Y = X + 2 * N3;           // 4 * N3
-    double *Ob = Y + 4 * N3;          // [dim][gbit][N2]
-    __shared__ unsigned int mask_s[2 * kMaxActive];
+// One workgroup per job (output node g, output component gt); its NW waves
+// share the job's 16-wide tiles (TW tiles each).  The active terms of the
+// job's depth are taken in chunks of 64; for the chunk, the 1D norms of every
+// term, component and translation in the band are staged in LDS, so that the
+// screening of a band node f reads six LDS values per lane: the bound
+// ||O_x|| ||O_y|| ||O_z|| ||f_ft|| for every input component ft, one ballot per
+// ft.  Every wave repeats the screening and holds the same ballot.  Each
+// surviving (ft, term) is applied as three chains of v_mfma_f64_16x16x4:
+//   x: X[(jz,jy)][ix]  = F[(jz,jy)][jx] . Ox^T[jx][ix]   F tiles in registers
+//   y: Y[iy][(jz,ix)]  = Oy[iy][jy] . X'[jy][(jz,ix)]
+//   z: acc[iz][(iy,ix)] += Oz[iz][jz] . Y''[jz][(iy,ix)]  acc tiles persist
+// X and Y change layout through one padded LDS buffer whose every lane
+// address is a base plus a compile-time offset (padding entries stay zero or
+// finite and only meet zero operands).  The three operator blocks of the next
+// term are loaded into a double-buffered LDS area while the current term
+// computes.  MFMA lane layout (checked on gfx950): A[i][k] in lane i + 16k,
+// B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i mod 4), register i / 4.
+typedef double mwg_double4 __attribute__((ext_vector_type(4)));
 
-    const int g = blockIdx.x, tid = threadIdx.x;
+template <int N>
+struct JobShape {
+    static constexpr int N2 = N * N, N3 = N2 * N;
+    static constexpr int NT = (N2 + 15) / 16;  // 16-wide tiles over N^2
+    static constexpr int KS = (N + 3) / 4;     // 4-deep k steps over N
+    static constexpr int TW = 4;               // tiles per wave
+    static constexpr int NW = (NT + TW - 1) / TW;
+    static constexpr int T = 64 * NW;          // threads per job
+    static constexpr int NTP = NW * TW;        // tiles including padding tiles
+    // Xp[row][16], row = (jz,jy) flattened; Yp[16][CP], [iy][(jz,ix)]
+    static constexpr int XROWS_W = 16 * NTP;
+    static constexpr int XROWS_R = ((16 * NTP - 1) / N) * N + 4 * KS;
+    static constexpr int XROWS = XROWS_W > XROWS_R ? XROWS_W : XROWS_R;
+    static constexpr int CP0 = 16 * NTP > 4 * KS * N ? 16 * NTP : 4 * KS * N;
+    static constexpr int CP = CP0 + ((16 * NTP - 1) / N >= 16 ? 0 : 0);
+    static constexpr int XSZ = XROWS * 16, YSZ = 16 * CP;
+    static constexpr int BUF = XSZ > YSZ ? XSZ : YSZ;
+    static constexpr int OB = 3 * N2;          // operator doubles per stage
+    static constexpr int OPL = (OB + T - 1) / T;
+};
+
+constexpr int kMaxBand = 9;  // band half-width limit of the LDS norm stage
+
+template <int N>
+__global__ __launch_bounds__(JobShape<N>::T) void k_apply_fp64(KArgs a) {
+    using S = JobShape<N>;
+    constexpr int N2 = S::N2, N3 = S::N3, KS = S::KS, TW = S::TW, T = S::T;
+    constexpr int CP = S::CP, BUF = S::BUF, OB = S::OB, OPL = S::OPL;
+    __shared__ double buf[BUF];
+    __shared__ double ops[2][OB];
+    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
+    __shared__ int ltab[64];
+    __shared__ long long btab[64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lc = lane & 15, lq = lane >> 4;
+    for (int e = tid; e < BUF; e += T) buf[e] = 0.0;
+
+    const int job = blockIdx.x;
+    const int g = job >> 3, gt = job & 7;
     const int d = a.out_depth[g];
     const double thr = a.out_thr[g];
     const int w = a.band_w[g], side = 2 * w + 1;
     const long long boff = a.band_off[g];
     const int a0 = a.act_off[d], nact = a.act_off[d + 1] - a0;
-    const int *act = a.act_terms + a0;
+    const int ndepth = a.ndepth;
+    const int gbx = gt & 1, gby = (gt >> 1) & 1, gbz = (gt >> 2) & 1;
 
-    double acc[8][NQ];
+    // this wave's tiles: t = wv*TW + tt
+    int ybase[TW], zbase[TW];
 #pragma unroll
-    for (int t = 0; t < 8; t++)
+    for (int tt = 0; tt < TW; tt++) {
+        const int col = 16 * (wv * TW + tt) + lc;
+        const int q = col / N, r = col % N;
+        ybase[tt] = (q * N + lq) * 16 + r;  // Xp[(jz*N + 4k + lq)][ix]
+        zbase[tt] = q * CP + r + lq * N;    // Yp[iy][(4k + lq)*N + ix]
+    }
+    mwg_double4 acc[TW];
 #pragma unroll
-        for (int q = 0; q < NQ; q++) acc[t][q] = 0.0;
+    for (int tt = 0; tt < TW; tt++) acc[tt] = mwg_double4{0.0, 0.0, 0.0, 0.0};
+    double fa[TW][KS];  // x-pass A operand: F[16t + lc][4k + lq]
     unsigned long long napps = 0;
-
     const int nband = side * side * side;
-    for (int p = 0; p < nband; p++) {
-        const int s = a.band[boff + p];
-        if (s < 0) continue;
-        const int lx = p % side - w, ly = (p / side) % side - w, lz = p / (side * side) - w;
-        double fn[8];
-#pragma unroll
-        for (int t = 0; t < 8; t++) fn[t] = a.slot_norm[(long long)s * 8 + t];
-        for (int i = tid; i < 2 * nact; i += kThreads) mask_s[i] = 0u;
+    const int nl = 2 * w + 1;
+
+    for (int base = 0; base < nact; base += 64) {
+        const int cnt = min(64, nact - base);
         __syncthreads();
-        for (int idx = tid; idx < nact * 64; idx += kThreads) {
-            const int ai = idx >> 6, ft = (idx >> 3) & 7, gt = idx & 7;
-            if (!(fn[ft] > 0.0)) continue;
-            if (d > 0 && ft == 0 && gt == 0) continue;
-            const int m = act[ai];
-            const int cx = ((gt & 1) << 1) | (ft & 1);
-            const int cy = (((gt >> 1) & 1) << 1) | ((ft >> 1) & 1);
-            const int cz = (((gt >> 2) & 1) << 1) | ((ft >> 2) & 1);
-            const int *bwm = a.bw + ((long long)d * a.M + m) * 4;
-            if (abs(lx) > bwm[cx] || abs(ly) > bwm[cy] || abs(lz) > bwm[cz]) continue;
-            const double nx = op_norm(a, m, d, cx, lx);
-            const double ny = op_norm(a, m, d, cy, ly);
-            const double nz = op_norm(a, m, d, cz, lz);
-            const double bound = ((nx * ny) * nz) * fn[ft];
-            if (bound > thr) {
-                atomicOr(&mask_s[ai * 2 + (ft >> 2)], 1u << ((ft & 3) * 8 + gt));
-                napps++;
+        // stage the chunk's table metadata and band norms
+        for (int e = tid; e < 64; e += T) {
+            const int m = (e < cnt) ? a.act_terms[a0 + base + e] : 0;
+            ltab[e] = (e < cnt) ? a.opL[m * ndepth + d] : -1;
+            btab[e] = (e < cnt) ? a.op_boff[m * ndepth + d] : 0;
+        }
+        for (int e = tid; e < 64 * 4 * nl; e += T) {
+            const int ti = e / (4 * nl), c = (e / nl) % 4, li = e % nl, l = li - w;
+            double v = 0.0;
+            if (ti < cnt) {
+                const int m = a.act_terms[a0 + base + ti];
+                const int L = a.opL[m * ndepth + d];
+                const int bwc = a.bw[((long long)d * a.M + m) * 4 + c];
+                if (abs(l) <= bwc && abs(l) <= L)
+                    v = a.op_norms[a.op_noff[m * ndepth + d] + (long long)c * (2 * L + 1) + (l + L)];
             }
+            nrm_s[e] = v;
         }
         __syncthreads();
-        const double *fsrc = a.pool + a.slot_off[s];
-        for (int ft = 0; ft < 8; ft++) {
-            if (!(fn[ft] > 0.0)) continue;
-            bool loaded = false;
-            for (int ai = 0; ai < nact; ai++) {
-                const unsigned int msk = (mask_s[ai * 2 + (ft >> 2)] >> ((ft & 3) * 8)) & 0xffu;
-                if (msk == 0u) continue;
-                const int m = act[ai];
-                if (!loaded) {
-                    for (int e = tid; e < N3; e += kThreads) fblk[e] = fsrc[ft * N3 + e];
-                    loaded = true;
-                }
-                // which g-bits are needed per direction
-                unsigned int needx = 0, needy = 0, needz = 0;
+        const bool have = lane < cnt;
+        for (int p = 0; p < nband; p++) {
+            const int s = a.band[boff + p];
+            if (s < 0) continue;
+            const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+            double nrm[2][3];
+            const int gb[3] = {gbx, gby, gbz};
 #pragma unroll
-                for (int gt = 0; gt < 8; gt++)
-                    if ((msk >> gt) & 1u) {
-                        needx |= 1u << (gt & 1);
-                        needy |= 1u << ((gt >> 1) & 1);
-                        needz |= 1u << ((gt >> 2) & 1);
-                    }
-                const int L = a.opL[m * a.ndepth + d];
-                const double *ob = a.op_blocks + a.op_boff[m * a.ndepth + d];
-                const int ll[3] = {lx, ly, lz};
-                const unsigned int need[3] = {needx, needy, needz};
-                for (int e = tid; e < 6 * N2; e += kThreads) {
-                    const int dim = e / (2 * N2), gb = (e / N2) & 1, r = e % N2;
-                    if (!((need[dim] >> gb) & 1u)) continue;
-                    const int fb = (ft >> dim) & 1;
-                    const int c = (gb << 1) | fb;
-                    Ob[e] = ob[((long long)c * (2 * L + 1) + (ll[dim] + L)) * N2 + r];
-                }
-                __syncthreads();
-                // x pass: X[gx][jz][jy][ix] = sum_jx Ox[gx][ix][jx] f[jz][jy][jx]
-                for (int e = tid; e < 2 * N3; e += kThreads) {
-                    const int gx = e / N3, r = e % N3;
-                    if (!((needx >> gx) & 1u)) continue;
-                    const int ix = r % N, row = r / N;
-                    const double *o = Ob + (0 * 2 + gx) * N2 + ix * N;
-                    const double *f = fblk + row * N;
-                    double v = 0.0;
+            for (int dim = 0; dim < 3; dim++)
 #pragma unroll
-                    for (int j = 0; j < N; j++) v = fma(o[j], f[j], v);
-                    X[e] = v;
-                }
-                __syncthreads();
-                // y pass: Y[gx + 2 gy][jz][iy][ix] = sum_jy Oy[gy][iy][jy] X[gx][jz][jy][ix]
-                for (int e = tid; e < 4 * N3; e += kThreads) {
-                    const int v4 = e / N3, r = e % N3;
-                    const int gx = v4 & 1, gy = v4 >> 1;
-                    if (!((needx >> gx) & 1u) || !((needy >> gy) & 1u)) continue;
-                    const int ix = r % N, iy = (r / N) % N, jz = r / N2;
-                    const double *o = Ob + (1 * 2 + gy) * N2 + iy * N;
-                    const double *x = X + gx * N3 + jz * N2 + ix;
-                    double v = 0.0;
+                for (int fb = 0; fb < 2; fb++)
+                    nrm[fb][dim] = nrm_s[(lane * 4 + (2 * gb[dim] + fb)) * nl + (ll[dim] + w)];
+            const double *fnp = a.slot_norm + (long long)s * 8;
+            for (int ft = 0; ft < 8; ft++) {
+                if (d > 0 && ft == 0 && gt == 0) continue;
+                const double fn = fnp[ft];
+                if (!(fn > 0.0)) continue;
+                const int fx = ft & 1, fy = (ft >> 1) & 1, fz = (ft >> 2) & 1;
+                const double bound = ((nrm[fx][0] * nrm[fy][1]) * nrm[fz][2]) * fn;
+                unsigned long long mask = __ballot(have && bound > thr);
+                if (!mask) continue;
+                {
+                    const double *fsrc = a.pool + a.slot_off[s] + (long long)ft * N3;
 #pragma unroll
-                    for (int j = 0; j < N; j++) v = fma(o[j], x[j * N], v);
-                    Y[e] = v;
-                }
-                __syncthreads();
-                // z pass into the register accumulators
+                    for (int tt = 0; tt < TW; tt++) {
+                        const int row = 16 * (wv * TW + tt) + lc;
 #pragma unroll
-                for (int gt = 0; gt < 8; gt++) {
-                    if (!((msk >> gt) & 1u)) continue;
-                    const double *o = Ob + (2 * 2 + ((gt >> 2) & 1)) * N2;
-                    const double *y = Y + ((gt & 1) + 2 * ((gt >> 1) & 1)) * N3;
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) {
-                        const int e = tid + q * kThreads;
-                        if (e < N3) {
-                            const int iz = e / N2, r = e % N2;
-                            double v = 0.0;
-#pragma unroll
-                            for (int j = 0; j < N; j++) v = fma(o[iz * N + j], y[j * N2 + r], v);
-                            acc[gt][q] += v;
+                        for (int k = 0; k < KS; k++) {
+                            const int jx = 4 * k + lq;
+                            fa[tt][k] = (row < N2 && jx < N) ? fsrc[row * N + jx] : 0.0;
                         }
                     }
                 }
+                const int cxyz[3] = {(gbx << 1) | fx, (gby << 1) | fy, (gbz << 1) | fz};
+                auto fetch = [&](int b, double *regs) {
+                    const int Lb = ltab[b];
+                    const long long bb = btab[b];
+#pragma unroll
+                    for (int u = 0; u < OPL; u++) {
+                        const int e = tid + T * u;
+                        regs[u] = 0.0;
+                        if (e < OB) {
+                            const int dim = e / N2, r = e % N2;
+                            regs[u] = a.op_blocks[bb + ((long long)cxyz[dim] * (2 * Lb + 1) + (ll[dim] + Lb)) * N2 + r];
+                        }
+                    }
+                };
+                double pre[OPL];
+                int b = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                fetch(b, pre);
+                int stage = 0;
+                while (true) {
+#pragma unroll
+                    for (int u = 0; u < OPL; u++) {
+                        const int e = tid + T * u;
+                        if (e < OB) ops[stage][e] = pre[u];
+                    }
+                    const bool more = mask != 0;
+                    if (more) {
+                        b = __ffsll((long long)mask) - 1;
+                        mask &= mask - 1;
+                        fetch(b, pre);  // next term's blocks in flight during this one
+                    }
+                    __syncthreads();
+                    napps++;
+                    // MFMA operands of the operators: Ox[lc][4k+lq] (B of x,
+                    // as Ox^T), Oy[lc][4k+lq], Oz[lc][4k+lq] (A of y, z)
+                    double ox[KS], oy[KS], oz[KS];
+                    const double *O = ops[stage];
+#pragma unroll
+                    for (int k = 0; k < KS; k++) {
+                        const int j = 4 * k + lq;
+                        const bool v = lc < N && j < N;
+                        ox[k] = v ? O[lc * N + j] : 0.0;
+                        oy[k] = v ? O[N2 + lc * N + j] : 0.0;
+                        oz[k] = v ? O[2 * N2 + lc * N + j] : 0.0;
+                    }
+                    // x pass -> Xp[row][ix], row = 16t + lq + 4r, ix = lc
+                    {
+                        mwg_double4 dx[TW];
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++) dx[tt] = mwg_double4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int k = 0; k < KS; k++)
+#pragma unroll
+                            for (int tt = 0; tt < TW; tt++)
+                                dx[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[tt][k], ox[k], dx[tt], 0, 0, 0);
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                buf[(lq * 16 + lc) + (16 * (wv * TW + tt) + 4 * r) * 16] = dx[tt][r];
+                    }
+                    __syncthreads();
+                    // y pass: B[jy][col] = X[jz*N + jy][ix] -> Yp[iy][col]
+                    {
+                        mwg_double4 dy[TW];
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++) dy[tt] = mwg_double4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                        for (int k = 0; k < KS; k++)
+#pragma unroll
+                            for (int tt = 0; tt < TW; tt++)
+                                dy[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(oy[k], buf[ybase[tt] + 64 * k], dy[tt], 0, 0, 0);
+                        __syncthreads();
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                buf[(lq * CP + lc) + (4 * r * CP + 16 * (wv * TW + tt))] = dy[tt][r];
+                    }
+                    __syncthreads();
+                    // z pass: B[jz][col'] = Yp[iy][jz*N + ix], col' = iy*N + ix
+#pragma unroll
+                    for (int k = 0; k < KS; k++)
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++)
+                            acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(oz[k], buf[zbase[tt] + 4 * k * N], acc[tt], 0, 0, 0);
+                    if (!more) break;
+                    stage ^= 1;
+                }
                 __syncthreads();
             }
-            if (loaded) __syncthreads();
         }
     }
-    // one atomic per wavefront for the work counter
-    for (int off = 32; off > 0; off >>= 1) napps += __shfl_down(napps, off);
-    if ((tid & 63) == 0 && napps) atomicAdd(&a.counters[0], napps);
-    double *dst = a.out_ns + (long long)g * 8 * N3;
+    if (tid == 0 && napps) atomicAdd(&a.counters[0], napps);
+    // acc[tt][r] = out[iz = lq + 4r][col = 16t + lc]
+    double *dst = a.out_ns + (long long)g * 8 * N3 + (long long)gt * N3;
 #pragma unroll
-    for (int t = 0; t < 8; t++)
+    for (int tt = 0; tt < TW; tt++) {
+        const int col = 16 * (wv * TW + tt) + lc;
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const int e = tid + q * kThreads;
-            if (e < N3) dst[t * N3 + e] = acc[t][q];
+        for (int r = 0; r < 4; r++) {
+            const int iz = lq + 4 * r;
+            if (col < N2 && iz < N) dst[iz * N2 + col] = acc[tt][r];
         }
+    }
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -236,7 +324,7 @@ struct DeviceCtx {
 
 struct DevicePlan {
     DeviceCtx *ctx = nullptr;
-    int n_out = 0, kp1 = 0;
+    int n_out = 0, kp1 = 0, wmax = 0;
     KArgs args{};
     std::vector<void *> bufs;
     size_t out_bytes = 0;
@@ -343,6 +431,11 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
             err = "more than " + std::to_string(kMaxActive) + " active terms at one depth";
             return -1;
         }
+    for (int wq : plan.band_w)
+        if (wq > kMaxBand) {
+            err = "band half-width " + std::to_string(wq) + " exceeds the kernel limit " + std::to_string(kMaxBand);
+            return -1;
+        }
     if (plan.kp1 < 2 || plan.kp1 > 12) {
         err = "unsupported basis size k+1 = " + std::to_string(plan.kp1);
         return -1;
@@ -354,6 +447,7 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
     p->ctx = c;
     p->n_out = (int)plan.out_node.size();
     p->kp1 = plan.kp1;
+    for (int wq : plan.band_w) p->wmax = std::max(p->wmax, wq);
     KArgs &a = p->args;
     rc = plan_copy(p, plan.out_depth, &a.out_depth, err);
     if (!rc) rc = plan_copy(p, plan.out_thr, &a.out_thr, err);
@@ -409,11 +503,9 @@ void dev_plan_destroy(DevicePlan *p) {
 }
 
 template <int N>
-static int launch_n(const KArgs &a, int n_out, hipStream_t s, std::string &err) {
-    constexpr int N2 = N * N, N3 = N2 * N;
-    size_t shmem = sizeof(double) * (7 * N3 + 6 * N2);
-    HIP_TRY(hipFuncSetAttribute((const void *)k_apply_fp64<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    hipLaunchKernelGGL(k_apply_fp64<N>, dim3(n_out), dim3(kThreads), shmem, s, a);
+static int launch_n(const KArgs &a, int n_out, int wmax, hipStream_t s, std::string &err) {
+    const size_t shmem = sizeof(double) * 64 * 4 * (2 * wmax + 1);
+    hipLaunchKernelGGL(k_apply_fp64<N>, dim3(8 * n_out), dim3(JobShape<N>::T), shmem, s, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -433,17 +525,17 @@ int dev_plan_run(DevicePlan *p, double *kernel_ms, unsigned long long *term_apps
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = 0;
     switch (p->kp1) {
-        case 2: rc = launch_n<2>(p->args, p->n_out, c->stream, err); break;
-        case 3: rc = launch_n<3>(p->args, p->n_out, c->stream, err); break;
-        case 4: rc = launch_n<4>(p->args, p->n_out, c->stream, err); break;
-        case 5: rc = launch_n<5>(p->args, p->n_out, c->stream, err); break;
-        case 6: rc = launch_n<6>(p->args, p->n_out, c->stream, err); break;
-        case 7: rc = launch_n<7>(p->args, p->n_out, c->stream, err); break;
-        case 8: rc = launch_n<8>(p->args, p->n_out, c->stream, err); break;
-        case 9: rc = launch_n<9>(p->args, p->n_out, c->stream, err); break;
-        case 10: rc = launch_n<10>(p->args, p->n_out, c->stream, err); break;
-        case 11: rc = launch_n<11>(p->args, p->n_out, c->stream, err); break;
-        case 12: rc = launch_n<12>(p->args, p->n_out, c->stream, err); break;
+        case 2: rc = launch_n<2>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 3: rc = launch_n<3>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 4: rc = launch_n<4>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 5: rc = launch_n<5>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 6: rc = launch_n<6>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 7: rc = launch_n<7>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 8: rc = launch_n<8>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 9: rc = launch_n<9>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 10: rc = launch_n<10>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 11: rc = launch_n<11>(p->args, p->n_out, p->wmax, c->stream, err); break;
+        case 12: rc = launch_n<12>(p->args, p->n_out, p->wmax, c->stream, err); break;
         default: err = "unsupported basis size"; return -1;
     }
     if (rc) return rc;
