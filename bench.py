@@ -84,61 +84,95 @@ def main():
     log("adaptive apply: %d output nodes in %.2fs (kernel %.1f ms, plan %.1f ms, %d launches)" %
         (V.size(), t_adapt, ast["kernel_ms"], ast["plan_ms"], ast["launches"]))
     thr = V.thresholds()
+    od, ol, _ = V.nodes()
+    # two plans on the same frozen grid and thresholds: plain FP64 and mixed
+    V64 = V
+    Vmx = mw.FunctionTree(mra).set_nodes(od, ol)
     t0 = time.time()
-    plan = mw.ApplyPlan(V, op, rho, thr, ctx=ctx)
+    plan64 = mw.ApplyPlan(V64, op, rho, thr, mode="fp64", ctx=ctx)
+    planmx = mw.ApplyPlan(Vmx, op, rho, thr, mode="mixed", ctx=ctx)
     t_plan = time.time() - t0
-    pinfo = plan.info()
-    log("frozen plan: %d out nodes, %d input slots (%d generated), plan %.2fs" %
+    pinfo = plan64.info()
+    log("frozen plans: %d out nodes, %d input slots (%d generated), %.2fs" %
         (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], t_plan))
-    for _ in range(args.warmup):
-        plan.execute()
-    kms = []
-    apps = None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ms, apps = plan.execute()
-        kms.append(ms)
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    kmean = float(np.mean(kms))
+
+    def timed(plan):
+        for _ in range(args.warmup):
+            plan.execute()
+        runs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            runs.append(plan.execute())
+        wall = time.perf_counter() - t0
+        return runs, 1e3 * wall / args.steps
+
+    runs64, ms64 = timed(plan64)
+    runsmx, msmx = timed(planmx)
     n_out = pinfo["out_nodes"]
-    flop = apps * 6.0 * (k + 1) ** 4
-    achieved = flop / (kmean * 1e-3) / 1e12
-    log("steps: wall %.3f ms/step, kernel mean %.3f ms, term-apps %d, %.2f TFLOP/s" %
-        (ms_per_step, kmean, apps, achieved))
+    apps = runs64[-1]["term_apps"]
+    assert runsmx[-1]["term_apps"] == apps, "mixed and FP64 paths applied different term sets"
+    lp_apps = runsmx[-1]["lp_term_apps"]
+    fl = 6.0 * (k + 1) ** 4
+    k64 = float(np.mean([r["ms"] for r in runs64]))
+    kmx = float(np.mean([r["ms"] for r in runsmx]))
+    kmx_lp = float(np.mean([r["ms_lp"] for r in runsmx]))
+    kmx_64 = float(np.mean([r["ms_fp64"] for r in runsmx]))
+    ach64 = apps * fl / (k64 * 1e-3) / 1e12
+    ach_lp = lp_apps * fl / (kmx_lp * 1e-3) / 1e12
+    log("fp64 : wall %.3f ms/step, kernel %.3f ms, %d term-apps, %.2f TFLOP/s" % (ms64, k64, apps, ach64))
+    log("mixed: wall %.3f ms/step, kernel %.3f ms (fp64 part %.3f ms, fp32 part %.3f ms), %d of %d term-apps "
+        "on FP32 (%.2f TFLOP/s)" % (msmx, kmx, kmx_64, kmx_lp, lp_apps, apps, ach_lp))
+
+    # error of the mixed path against the FP64 path on the identical grid
+    ns64 = plan64.download_ns()
+    nsmx = planmx.download_ns()
+    coef_err = float(np.max(np.abs(nsmx - ns64)) / np.max(np.abs(ns64)))
+    plan64.finish()
+    planmx.finish()
+    c64 = V64.nodes()[2]
+    cmx = Vmx.nodes()[2]
+    tree_err = float(np.linalg.norm(cmx - c64) / np.linalg.norm(c64))
+    log("mixed vs fp64: max coefficient error %.3e (relative to max), tree-norm error %.3e" % (coef_err, tree_err))
 
     # result check: energy against the closed form
     from oracle import closed_forms as cf
-    plan.finish()
     c, a, R = cfg["densities"][0]
-    E = rho.dot(V)
+    E = rho.dot(V64)
+    Emx = rho.dot(Vmx)
     mu = cfg["mu"][0] if cfg["kind"] == "helmholtz" else 0.0
     E_ex = cf.pair_energy(c, a, R, mu=mu)
-    check = {"energy": E, "energy_closed_form": E_ex, "abs_err": abs(E - E_ex)}
+    check = {"energy_fp64": E, "energy_mixed": Emx, "energy_closed_form": E_ex, "abs_err_fp64": abs(E - E_ex),
+             "abs_err_mixed": abs(Emx - E_ex), "mixed_vs_fp64_max_coef_rel": coef_err,
+             "mixed_vs_fp64_tree_rel": tree_err}
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mra, op, rho, V, thr, plan, args.cpu_seconds, check)
+        cpu = cpu_baseline(mra, op, rho, V64, thr, ns64, args.cpu_seconds, check)
 
     line = {
         "metric": METRIC,
-        "value": n_out / (ms_per_step * 1e-3),
+        "value": n_out / (msmx * 1e-3),
         "unit": "nodes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": msmx,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64 (mixed: FP64 for bound > %g thr)" % ctx_tau(ctx),
         "data": "synthetic (SplitMix64-seeded Gaussian sums)",
         "config": {"workload": cfg["name"], "order": k, "prec": cfg["prec"], "out_nodes": n_out,
-                   "in_slots": pinfo["in_slots"], "terms": op.size(), "path": "plain FP64"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None},
-        "kernel_ms_mean": kmean,
+                   "in_slots": pinfo["in_slots"], "terms": op.size(), "path": "mixed FP32/FP64 MFMA"},
+        "roofline": {"bound": "mfma", "achieved": ach_lp, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                     "frac": ach_lp / PEAK_TFLOPS["f32"], "traffic": None,
+                     "kernel": "k_apply<%d, float>" % (k + 1)},
+        "fp64_path": {"value": n_out / (ms64 * 1e-3), "ms_per_step": ms64, "kernel_ms": k64,
+                      "roofline": {"achieved": ach64, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                                   "frac": ach64 / PEAK_TFLOPS["f64"]}},
+        "speedup_vs_fp64": ms64 / msmx,
         "term_apps": apps,
+        "lp_term_apps": lp_apps,
         "check": check,
         "setup": dict(setup, adaptive_apply_s=t_adapt, plan_s=t_plan),
     }
@@ -147,7 +181,11 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(mra, op, rho, V, thr, plan, target_s, check):
+def ctx_tau(ctx):
+    return 65536.0
+
+
+def cpu_baseline(mra, op, rho, V, thr, ns_gpu, target_s, check):
     """FP64 OpenMP oracle on a bounded sample of the frozen output grid."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
@@ -156,7 +194,6 @@ def cpu_baseline(mra, op, rho, V, thr, plan, target_s, check):
     n = len(od)
     tables = op.tables()
     inp = rho.nodes()
-    ns_gpu = plan.download_ns()
     rng = np.random.default_rng(12345)
     order = rng.permutation(n)
     m = min(n, 32)

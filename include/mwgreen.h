@@ -37,8 +37,12 @@ extern "C" {
 #define MWG_BASIS_INTERPOLATING 0
 #define MWG_BASIS_LEGENDRE 1
 
-/* Arithmetic of the per-node contractions. */
+/* Arithmetic of the per-node contractions.  MIXED applies every
+ * term-application whose screening bound exceeds tau * threshold in FP64 and
+ * the others on FP32 MFMA, adding their sum to the FP64 output; the set of
+ * terms applied is the same in both modes. */
 #define MWG_MODE_FP64 0
+#define MWG_MODE_MIXED 1
 
 typedef struct mwg_context_s *mwg_context;
 typedef struct mwg_mra_s *mwg_mra;
@@ -52,6 +56,9 @@ const char *mwg_last_error(void);
  * Replaces the OpenMP/MPI environment mrcpp::apply runs in (src/mrenv.cpp:129-135). */
 int mwg_context_create(int device, mwg_context *ctx);
 int mwg_context_destroy(mwg_context ctx);
+/* Arithmetic used by mwg_apply / mwg_apply_prec_trees on this context
+ * (default FP64; tau default 65536).  No counterpart in MRChem's calls. */
+int mwg_context_set_precision(mwg_context ctx, int mode, double tau);
 
 /* MultiResolutionAnalysis<3>: BoundingBox(min_scale, corner, boxes) with a
  * scaling factor, basis of the given order and type, and max depth.
@@ -123,15 +130,19 @@ int mwg_last_apply_stats(mwg_context ctx, double *stats, int n);
  * MRChem's calls; it serves repeated applies on one grid and measurement. */
 int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode,
                     mwg_plan *plan);
-/* One execution of the per-node calculator on resident data; kernel_ms is the
- * device time, term_apps the term-applications that passed screening. */
-int mwg_plan_execute(mwg_plan plan, double *kernel_ms, double *term_apps);
+/* One execution of the per-node calculator on resident data, with the plan's
+ * `mode` (tau from the context).  stats (n <= 5 values): [0] device ms,
+ * [1] term-applications that passed screening, [2] of them on FP32 MFMA,
+ * [3] device ms of the FP64 launch, [4] of the FP32 launch. */
+int mwg_plan_execute(mwg_plan plan, double *stats, int n);
 /* Copy the result into `out` and run the TopDown/BottomUp transforms. */
 int mwg_plan_finish(mwg_plan plan);
 /* Raw non-standard blocks of the last execution (n_out x 8(k+1)^3, tree
  * order), before the TopDown/BottomUp transforms. */
 int mwg_plan_download_ns(mwg_plan plan, double *ns);
-/* [0] output nodes, [1] input slots, [2] generated slots, [3] host plan ms. */
+/* [0] output nodes, [1] input slots, [2] generated slots, [3] host plan ms,
+ * [4] term-applications counted on the host with the screening rule,
+ * [5] work segments (workgroups per launch). */
 int mwg_plan_info(mwg_plan plan, double *info, int n);
 int mwg_plan_destroy(mwg_plan plan);
 

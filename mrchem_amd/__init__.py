@@ -76,13 +76,14 @@ _sig("mwg_apply_prec_trees", _p, _d, _p, _p, _p, ctypes.POINTER(_p), _i, _i, _i)
 _sig("mwg_apply_fixed", _p, _p, _p, _p, _dp, _i)
 _sig("mwg_last_apply_stats", _p, _dp, _i)
 _sig("mwg_plan_create", _p, _p, _p, _p, _dp, _i, ctypes.POINTER(_p))
-_sig("mwg_plan_execute", _p, _dp, _dp)
+_sig("mwg_plan_execute", _p, _dp, _i)
+_sig("mwg_context_set_precision", _p, _i, _d)
 _sig("mwg_plan_finish", _p)
 _sig("mwg_plan_info", _p, _dp, _i)
 _sig("mwg_plan_download_ns", _p, _dp)
 _sig("mwg_plan_destroy", _p)
 
-MODES = {"fp64": 0}
+MODES = {"fp64": 0, "mixed": 1}
 
 
 class MWGError(RuntimeError):
@@ -113,6 +114,11 @@ class Context:
         if cls._default is None:
             cls._default = Context(0)
         return cls._default
+
+    def set_precision(self, mode="fp64", tau=65536.0):
+        """Arithmetic of mw.apply on this context: 'fp64' or 'mixed' (FP64 for
+        terms whose bound exceeds tau x threshold, FP32 MFMA for the rest)."""
+        _check(lib.mwg_context_set_precision(self._h, MODES[mode], tau))
 
     def last_stats(self):
         s = np.zeros(8)
@@ -350,9 +356,11 @@ class ApplyPlan:
         self._keep = (out, oper, inp)
 
     def execute(self):
-        ms, apps = _d(), _d()
-        _check(lib.mwg_plan_execute(self._h, ctypes.byref(ms), ctypes.byref(apps)))
-        return ms.value, int(apps.value)
+        """One execution; returns a dict: ms (device), term_apps, lp_term_apps
+        (on FP32 MFMA), ms_fp64 and ms_lp (device time of the two launches)."""
+        v = np.zeros(5)
+        _check(lib.mwg_plan_execute(self._h, _ptr(v, _d), 5))
+        return {"ms": v[0], "term_apps": int(v[1]), "lp_term_apps": int(v[2]), "ms_fp64": v[3], "ms_lp": v[4]}
 
     def finish(self):
         _check(lib.mwg_plan_finish(self._h))
@@ -365,9 +373,10 @@ class ApplyPlan:
         return ns
 
     def info(self):
-        v = np.zeros(4)
-        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 4))
-        return {"out_nodes": int(v[0]), "in_slots": int(v[1]), "generated": int(v[2]), "plan_ms": v[3]}
+        v = np.zeros(6)
+        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 6))
+        return {"out_nodes": int(v[0]), "in_slots": int(v[1]), "generated": int(v[2]), "plan_ms": v[3],
+                "host_term_apps": int(v[4]), "segments": int(v[5])}
 
     def __del__(self):
         h = getattr(self, "_h", None)

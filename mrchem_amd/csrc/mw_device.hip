@@ -50,6 +50,9 @@ struct KArgs {
     const int *act_off, *act_terms;
     // [0] term-applications passing screening (algorithmic work)
     unsigned long long *counters;
+    // work segments: job = 8*out + gt, band positions [p0, p1)
+    const int *seg_job, *seg_p0, *seg_p1;
+    double thr_floor;  // positive floor of the class threshold
 };
 
 __device__ __forceinline__ double op_norm(const KArgs &a, int m, int d, int c, int l) {
@@ -59,23 +62,52 @@ __device__ __forceinline__ double op_norm(const KArgs &a, int m, int d, int c, i
 }
 
 // One workgroup per job (output node g, output component gt); its NW waves
-// share the job's 16-wide tiles (TW tiles each).  The active terms of the
-// job's depth are taken in chunks of 64; for the chunk, the 1D norms of every
-// term, component and translation in the band are staged in LDS, so that the
-// screening of a band node f reads six LDS values per lane: the bound
-// ||O_x|| ||O_y|| ||O_z|| ||f_ft|| for every input component ft, one ballot per
-// ft.  Every wave repeats the screening and holds the same ballot.  Each
-// surviving (ft, term) is applied as three chains of v_mfma_f64_16x16x4:
+// share the job's 16-wide tiles (TW tiles each).
+//
+// Screening: the active terms of the job's depth are taken in chunks of 64
+// (one term per lane).  For the chunk, the 1D norms of every term, component
+// and band translation are staged in LDS; for each band node f the bound
+// ||O_x|| ||O_y|| ||O_z|| ||f_ft|| is formed for every input component ft and
+// a ballot gives the surviving terms, which are appended to an LDS worklist
+// of (band node, ft, term) items.  A term is taken by this launch when it
+// passes the reference rule (bound > thr) and falls in the launch's bound
+// class (bound > lo*thr and not bound > hi*thr), so launches with disjoint
+// classes together apply exactly the oracle's term set.
+//
+// Application: each item is three chains of 16x16x4 MFMAs in the launch's
+// arithmetic T (FP64 or FP32):
 //   x: X[(jz,jy)][ix]  = F[(jz,jy)][jx] . Ox^T[jx][ix]   F tiles in registers
 //   y: Y[iy][(jz,ix)]  = Oy[iy][jy] . X'[jy][(jz,ix)]
 //   z: acc[iz][(iy,ix)] += Oz[iz][jz] . Y''[jz][(iy,ix)]  acc tiles persist
-// X and Y change layout through one padded LDS buffer whose every lane
-// address is a base plus a compile-time offset (padding entries stay zero or
-// finite and only meet zero operands).  The three operator blocks of the next
-// term are loaded into a double-buffered LDS area while the current term
-// computes.  MFMA lane layout (checked on gfx950): A[i][k] in lane i + 16k,
-// B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i mod 4), register i / 4.
+// X and Y change layout through one padded LDS buffer whose lane addresses
+// are a base plus a compile-time offset (padding entries stay zero or finite
+// and only meet zero operands).  While an item computes, the operator blocks
+// of the next item are loaded (registers, then a double-buffered LDS stage)
+// and, when the next item starts a new (f, ft) group, its F tiles too.
+// MFMA lane layouts (checked on gfx950): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k; FP64 D[i][j] in lane j + 16 (i mod 4), register i / 4; FP32
+// D[i][j] in lane j + 16 (i / 4), register i mod 4.
 typedef double mwg_double4 __attribute__((ext_vector_type(4)));
+typedef float mwg_float4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef mwg_double4 acc_t;
+    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lq, int r) { return lq + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    typedef mwg_float4 acc_t;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lq, int r) { return 4 * lq + r; }
+};
 
 template <int N>
 struct JobShape {
@@ -90,39 +122,48 @@ struct JobShape {
     static constexpr int XROWS_W = 16 * NTP;
     static constexpr int XROWS_R = ((16 * NTP - 1) / N) * N + 4 * KS;
     static constexpr int XROWS = XROWS_W > XROWS_R ? XROWS_W : XROWS_R;
-    static constexpr int CP0 = 16 * NTP > 4 * KS * N ? 16 * NTP : 4 * KS * N;
-    static constexpr int CP = CP0 + ((16 * NTP - 1) / N >= 16 ? 0 : 0);
+    static constexpr int CP = 16 * NTP > 4 * KS * N ? 16 * NTP : 4 * KS * N;
     static constexpr int XSZ = XROWS * 16, YSZ = 16 * CP;
     static constexpr int BUF = XSZ > YSZ ? XSZ : YSZ;
-    static constexpr int OB = 3 * N2;          // operator doubles per stage
+    static constexpr int OB = 3 * N2;          // operator values per stage
     static constexpr int OPL = (OB + T - 1) / T;
 };
 
-constexpr int kMaxBand = 9;  // band half-width limit of the LDS norm stage
+constexpr int kMaxBand = 9;     // band half-width limit
+constexpr int kWorkCap = 1024;  // worklist items per batch (>= 8 * 64 + 512)
 
-template <int N>
-__global__ __launch_bounds__(JobShape<N>::T) void k_apply_fp64(KArgs a) {
+template <int N, typename T>
+__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const T *opblk, double lo_mult,
+                                                          double hi_mult, int cls_counter) {
     using S = JobShape<N>;
-    constexpr int N2 = S::N2, N3 = S::N3, KS = S::KS, TW = S::TW, T = S::T;
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc_t;
+    constexpr int N2 = S::N2, N3 = S::N3, KS = S::KS, TW = S::TW, NTH = S::T;
     constexpr int CP = S::CP, BUF = S::BUF, OB = S::OB, OPL = S::OPL;
-    __shared__ double buf[BUF];
-    __shared__ double ops[2][OB];
-    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
+    __shared__ T buf[BUF];
+    __shared__ T ops[2][OB];
+    __shared__ int wl[kWorkCap];
     __shared__ int ltab[64];
     __shared__ long long btab[64];
+    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
-    for (int e = tid; e < BUF; e += T) buf[e] = 0.0;
+    for (int e = tid; e < BUF; e += NTH) buf[e] = T(0);
 
-    const int job = blockIdx.x;
+    const int seg = blockIdx.x;
+    const int job = a.seg_job[seg], p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
     const int g = job >> 3, gt = job & 7;
     const int d = a.out_depth[g];
     const double thr = a.out_thr[g];
-    const int w = a.band_w[g], side = 2 * w + 1;
+    // class cuts relative to the threshold (floored, so that nodes computed
+    // without screening still split by magnitude)
+    const double thr_cls = thr > a.thr_floor ? thr : a.thr_floor;
+    const double thr_lo = thr_cls * lo_mult, thr_hi = thr_cls * hi_mult;
+    const int w = a.band_w[g], side = 2 * w + 1, nl = 2 * w + 1;
     const long long boff = a.band_off[g];
     const int a0 = a.act_off[d], nact = a.act_off[d + 1] - a0;
     const int ndepth = a.ndepth;
-    const int gbx = gt & 1, gby = (gt >> 1) & 1, gbz = (gt >> 2) & 1;
+    const int gb[3] = {gt & 1, (gt >> 1) & 1, (gt >> 2) & 1};
 
     // this wave's tiles: t = wv*TW + tt
     int ybase[TW], zbase[TW];
@@ -133,24 +174,51 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply_fp64(KArgs a) {
         ybase[tt] = (q * N + lq) * 16 + r;  // Xp[(jz*N + 4k + lq)][ix]
         zbase[tt] = q * CP + r + lq * N;    // Yp[iy][(4k + lq)*N + ix]
     }
-    mwg_double4 acc[TW];
+    acc_t acc[TW];
 #pragma unroll
-    for (int tt = 0; tt < TW; tt++) acc[tt] = mwg_double4{0.0, 0.0, 0.0, 0.0};
-    double fa[TW][KS];  // x-pass A operand: F[16t + lc][4k + lq]
+    for (int tt = 0; tt < TW; tt++) acc[tt] = acc_t{T(0), T(0), T(0), T(0)};
     unsigned long long napps = 0;
-    const int nband = side * side * side;
-    const int nl = 2 * w + 1;
+    // F tiles of band node p, component ft: x-pass A operand F[16t + lc][4k + lq]
+    auto load_f = [&](int p, int ft, T (*dst)[KS]) {
+        const int s = a.band[boff + p];
+        const T *fsrc = pool + a.slot_off[s] + (long long)ft * N3;
+#pragma unroll
+        for (int tt = 0; tt < TW; tt++) {
+            const int row = 16 * (wv * TW + tt) + lc;
+#pragma unroll
+            for (int k = 0; k < KS; k++) {
+                const int jx = 4 * k + lq;
+                dst[tt][k] = (row < N2 && jx < N) ? fsrc[row * N + jx] : T(0);
+            }
+        }
+    };
+    // operator blocks Ox, Oy, Oz of worklist item `it` into this thread's share
+    auto fetch = [&](int it, T *regs) {
+        const int p = it >> 9, ft = (it >> 6) & 7, b = it & 63;
+        const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+        const int Lb = ltab[b];
+        const long long bb = btab[b];
+#pragma unroll
+        for (int u = 0; u < OPL; u++) {
+            const int e = tid + NTH * u;
+            regs[u] = T(0);
+            if (e < OB) {
+                const int dim = e / N2, r = e % N2;
+                const int c = (gb[dim] << 1) | ((ft >> dim) & 1);
+                regs[u] = opblk[bb + ((long long)c * (2 * Lb + 1) + (ll[dim] + Lb)) * N2 + r];
+            }
+        }
+    };
 
     for (int base = 0; base < nact; base += 64) {
         const int cnt = min(64, nact - base);
         __syncthreads();
-        // stage the chunk's table metadata and band norms
-        for (int e = tid; e < 64; e += T) {
+        for (int e = tid; e < 64; e += NTH) {
             const int m = (e < cnt) ? a.act_terms[a0 + base + e] : 0;
             ltab[e] = (e < cnt) ? a.opL[m * ndepth + d] : -1;
             btab[e] = (e < cnt) ? a.op_boff[m * ndepth + d] : 0;
         }
-        for (int e = tid; e < 64 * 4 * nl; e += T) {
+        for (int e = tid; e < 64 * 4 * nl; e += NTH) {
             const int ti = e / (4 * nl), c = (e / nl) % 4, li = e % nl, l = li - w;
             double v = 0.0;
             if (ti < cnt) {
@@ -164,143 +232,146 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply_fp64(KArgs a) {
         }
         __syncthreads();
         const bool have = lane < cnt;
-        for (int p = 0; p < nband; p++) {
-            const int s = a.band[boff + p];
-            if (s < 0) continue;
-            const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
-            double nrm[2][3];
-            const int gb[3] = {gbx, gby, gbz};
+        int p = p_begin;
+        while (p < p_end) {
+            // screening into the worklist (every wave computes the same
+            // ballots; wave 0 writes the items)
+            int n = 0;
+            for (; p < p_end && n + 8 * 64 <= kWorkCap; p++) {
+                const int s = a.band[boff + p];
+                if (s < 0) continue;
+                const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+                double nrm[2][3];
 #pragma unroll
-            for (int dim = 0; dim < 3; dim++)
+                for (int dim = 0; dim < 3; dim++)
 #pragma unroll
-                for (int fb = 0; fb < 2; fb++)
-                    nrm[fb][dim] = nrm_s[(lane * 4 + (2 * gb[dim] + fb)) * nl + (ll[dim] + w)];
-            const double *fnp = a.slot_norm + (long long)s * 8;
-            for (int ft = 0; ft < 8; ft++) {
-                if (d > 0 && ft == 0 && gt == 0) continue;
-                const double fn = fnp[ft];
-                if (!(fn > 0.0)) continue;
-                const int fx = ft & 1, fy = (ft >> 1) & 1, fz = (ft >> 2) & 1;
-                const double bound = ((nrm[fx][0] * nrm[fy][1]) * nrm[fz][2]) * fn;
-                unsigned long long mask = __ballot(have && bound > thr);
-                if (!mask) continue;
-                {
-                    const double *fsrc = a.pool + a.slot_off[s] + (long long)ft * N3;
-#pragma unroll
-                    for (int tt = 0; tt < TW; tt++) {
-                        const int row = 16 * (wv * TW + tt) + lc;
-#pragma unroll
-                        for (int k = 0; k < KS; k++) {
-                            const int jx = 4 * k + lq;
-                            fa[tt][k] = (row < N2 && jx < N) ? fsrc[row * N + jx] : 0.0;
-                        }
+                    for (int fb = 0; fb < 2; fb++)
+                        nrm[fb][dim] = nrm_s[(lane * 4 + (2 * gb[dim] + fb)) * nl + (ll[dim] + w)];
+                const double *fnp = a.slot_norm + (long long)s * 8;
+                for (int ft = 0; ft < 8; ft++) {
+                    if (d > 0 && ft == 0 && gt == 0) continue;
+                    const double fn = fnp[ft];
+                    if (!(fn > 0.0)) continue;
+                    const double bound = ((nrm[ft & 1][0] * nrm[(ft >> 1) & 1][1]) * nrm[(ft >> 2) & 1][2]) * fn;
+                    const bool take = have && bound > thr && bound > thr_lo && !(bound > thr_hi);
+                    const unsigned long long mask = __ballot(take);
+                    if (take && wv == 0) {
+                        const int pos = n + __popcll(mask & ((1ull << lane) - 1ull));
+                        wl[pos] = (p << 9) | (ft << 6) | lane;
                     }
+                    n += __popcll(mask);
                 }
-                const int cxyz[3] = {(gbx << 1) | fx, (gby << 1) | fy, (gbz << 1) | fz};
-                auto fetch = [&](int b, double *regs) {
-                    const int Lb = ltab[b];
-                    const long long bb = btab[b];
+            }
+            __syncthreads();
+            if (n == 0) continue;
+            napps += n;
+            // pipelined application of the worklist
+            T fa[TW][KS], fnext[TW][KS];
+            T pre[OPL];
+            int it = wl[0];
+            load_f(it >> 9, (it >> 6) & 7, fa);
+            fetch(it, pre);
+            int stage = 0;
+            for (int i = 0; i < n; i++) {
 #pragma unroll
-                    for (int u = 0; u < OPL; u++) {
-                        const int e = tid + T * u;
-                        regs[u] = 0.0;
-                        if (e < OB) {
-                            const int dim = e / N2, r = e % N2;
-                            regs[u] = a.op_blocks[bb + ((long long)cxyz[dim] * (2 * Lb + 1) + (ll[dim] + Lb)) * N2 + r];
-                        }
-                    }
-                };
-                double pre[OPL];
-                int b = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                fetch(b, pre);
-                int stage = 0;
-                while (true) {
+                for (int u = 0; u < OPL; u++) {
+                    const int e = tid + NTH * u;
+                    if (e < OB) ops[stage][e] = pre[u];
+                }
+                const bool more = i + 1 < n;
+                const int nx = more ? wl[i + 1] : it;
+                const bool newgroup = more && ((nx >> 6) != (it >> 6));
+                if (more) fetch(nx, pre);
+                if (newgroup) load_f(nx >> 9, (nx >> 6) & 7, fnext);
+                __syncthreads();
+                // operator operands: Ox[lc][4k+lq] (B of x as Ox^T), Oy, Oz (A of y, z)
+                T ox[KS], oy[KS], oz[KS];
+                const T *O = ops[stage];
 #pragma unroll
-                    for (int u = 0; u < OPL; u++) {
-                        const int e = tid + T * u;
-                        if (e < OB) ops[stage][e] = pre[u];
-                    }
-                    const bool more = mask != 0;
-                    if (more) {
-                        b = __ffsll((long long)mask) - 1;
-                        mask &= mask - 1;
-                        fetch(b, pre);  // next term's blocks in flight during this one
-                    }
-                    __syncthreads();
-                    napps++;
-                    // MFMA operands of the operators: Ox[lc][4k+lq] (B of x,
-                    // as Ox^T), Oy[lc][4k+lq], Oz[lc][4k+lq] (A of y, z)
-                    double ox[KS], oy[KS], oz[KS];
-                    const double *O = ops[stage];
+                for (int k = 0; k < KS; k++) {
+                    const int j = 4 * k + lq;
+                    const bool v = lc < N && j < N;
+                    ox[k] = v ? O[lc * N + j] : T(0);
+                    oy[k] = v ? O[N2 + lc * N + j] : T(0);
+                    oz[k] = v ? O[2 * N2 + lc * N + j] : T(0);
+                }
+                // x pass -> Xp[row][ix]
+                {
+                    acc_t dx[TW];
 #pragma unroll
-                    for (int k = 0; k < KS; k++) {
-                        const int j = 4 * k + lq;
-                        const bool v = lc < N && j < N;
-                        ox[k] = v ? O[lc * N + j] : 0.0;
-                        oy[k] = v ? O[N2 + lc * N + j] : 0.0;
-                        oz[k] = v ? O[2 * N2 + lc * N + j] : 0.0;
-                    }
-                    // x pass -> Xp[row][ix], row = 16t + lq + 4r, ix = lc
-                    {
-                        mwg_double4 dx[TW];
-#pragma unroll
-                        for (int tt = 0; tt < TW; tt++) dx[tt] = mwg_double4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                        for (int k = 0; k < KS; k++)
-#pragma unroll
-                            for (int tt = 0; tt < TW; tt++)
-                                dx[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[tt][k], ox[k], dx[tt], 0, 0, 0);
-#pragma unroll
-                        for (int tt = 0; tt < TW; tt++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                buf[(lq * 16 + lc) + (16 * (wv * TW + tt) + 4 * r) * 16] = dx[tt][r];
-                    }
-                    __syncthreads();
-                    // y pass: B[jy][col] = X[jz*N + jy][ix] -> Yp[iy][col]
-                    {
-                        mwg_double4 dy[TW];
-#pragma unroll
-                        for (int tt = 0; tt < TW; tt++) dy[tt] = mwg_double4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                        for (int k = 0; k < KS; k++)
-#pragma unroll
-                            for (int tt = 0; tt < TW; tt++)
-                                dy[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(oy[k], buf[ybase[tt] + 64 * k], dy[tt], 0, 0, 0);
-                        __syncthreads();
-#pragma unroll
-                        for (int tt = 0; tt < TW; tt++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                buf[(lq * CP + lc) + (4 * r * CP + 16 * (wv * TW + tt))] = dy[tt][r];
-                    }
-                    __syncthreads();
-                    // z pass: B[jz][col'] = Yp[iy][jz*N + ix], col' = iy*N + ix
+                    for (int tt = 0; tt < TW; tt++) dx[tt] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
                     for (int k = 0; k < KS; k++)
 #pragma unroll
-                        for (int tt = 0; tt < TW; tt++)
-                            acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(oz[k], buf[zbase[tt] + 4 * k * N], acc[tt], 0, 0, 0);
-                    if (!more) break;
-                    stage ^= 1;
+                        for (int tt = 0; tt < TW; tt++) dx[tt] = MF::run(fa[tt][k], ox[k], dx[tt]);
+#pragma unroll
+                    for (int tt = 0; tt < TW; tt++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            buf[(16 * (wv * TW + tt) + MF::drow(lq, r)) * 16 + lc] = dx[tt][r];
                 }
                 __syncthreads();
+                // y pass: B[jy][col] = X[jz*N + jy][ix] -> Yp[iy][col]
+                {
+                    acc_t dy[TW];
+#pragma unroll
+                    for (int tt = 0; tt < TW; tt++) dy[tt] = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                    for (int k = 0; k < KS; k++)
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++) dy[tt] = MF::run(oy[k], buf[ybase[tt] + 64 * k], dy[tt]);
+                    __syncthreads();
+#pragma unroll
+                    for (int tt = 0; tt < TW; tt++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) buf[MF::drow(lq, r) * CP + 16 * (wv * TW + tt) + lc] = dy[tt][r];
+                }
+                __syncthreads();
+                // z pass: B[jz][col'] = Yp[iy][jz*N + ix], col' = iy*N + ix
+#pragma unroll
+                for (int k = 0; k < KS; k++)
+#pragma unroll
+                    for (int tt = 0; tt < TW; tt++) acc[tt] = MF::run(oz[k], buf[zbase[tt] + 4 * k * N], acc[tt]);
+                if (newgroup) {
+#pragma unroll
+                    for (int tt = 0; tt < TW; tt++)
+#pragma unroll
+                        for (int k = 0; k < KS; k++) fa[tt][k] = fnext[tt][k];
+                }
+                it = nx;
+                stage ^= 1;
             }
+            __syncthreads();
         }
     }
-    if (tid == 0 && napps) atomicAdd(&a.counters[0], napps);
-    // acc[tt][r] = out[iz = lq + 4r][col = 16t + lc]
+    if (tid == 0 && napps) {
+        atomicAdd(&a.counters[0], napps);
+        if (cls_counter > 0) atomicAdd(&a.counters[cls_counter], napps);
+    }
+    // acc[tt][r] = out[iz = drow(lq, r)][col = 16t + lc], added in FP64 to the
+    // (zeroed) output: several segments and launches may share a job
+    if (napps == 0) return;
     double *dst = a.out_ns + (long long)g * 8 * N3 + (long long)gt * N3;
 #pragma unroll
     for (int tt = 0; tt < TW; tt++) {
         const int col = 16 * (wv * TW + tt) + lc;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int iz = lq + 4 * r;
-            if (col < N2 && iz < N) dst[iz * N2 + col] = acc[tt][r];
+            const int iz = MF::drow(lq, r);
+            if (col < N2 && iz < N) atomicAdd(&dst[iz * N2 + col], (double)acc[tt][r]);
         }
     }
+}
+
+__global__ void k_zero_f64(double *p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// FP64 -> FP32 copy (input pool and operator tables of the FP32 launches)
+__global__ void k_to_f32(const double *src, float *dst, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -313,24 +384,30 @@ struct DevOper {
     int *L = nullptr;
     long long *boff = nullptr, *noff = nullptr;
     double *blocks = nullptr, *norms = nullptr;
+    float *blocks32 = nullptr;  // FP32 copy, made on first mixed-precision use
+    long long nblk = 0;
 };
 
 struct DeviceCtx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     std::map<const void *, DevOper> opers;
 };
 
 struct DevicePlan {
     DeviceCtx *ctx = nullptr;
-    int n_out = 0, kp1 = 0, wmax = 0;
+    const void *op_key = nullptr;
+    int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0;
+    long long npool = 0;
+    float *pool32 = nullptr;
     KArgs args{};
     std::vector<void *> bufs;
     size_t out_bytes = 0;
     ~DevicePlan() {
         for (void *p : bufs)
             if (p) (void)hipFree(p);
+        if (pool32) (void)hipFree(pool32);
     }
 };
 
@@ -347,6 +424,7 @@ int dev_create(int device, DeviceCtx **out, std::string &err) {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&c->evm);
     if (e != hipSuccess) {
         err = std::string("stream/event creation: ") + hipGetErrorString(e);
         delete c;
@@ -362,6 +440,7 @@ static void free_oper(DevOper &o) {
     if (o.noff) (void)hipFree(o.noff);
     if (o.blocks) (void)hipFree(o.blocks);
     if (o.norms) (void)hipFree(o.norms);
+    if (o.blocks32) (void)hipFree(o.blocks32);
     o = DevOper();
 }
 
@@ -372,6 +451,7 @@ void dev_destroy(DeviceCtx *c) {
     for (auto &kv : c->opers) free_oper(kv.second);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evm) (void)hipEventDestroy(c->evm);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -392,6 +472,16 @@ static int alloc_copy(const std::vector<T> &v, T **out, hipStream_t s, std::stri
     return 0;
 }
 
+static int to_f32(const double *src, float **dst, long long n, hipStream_t s, std::string &err) {
+    HIP_TRY(hipMalloc((void **)dst, sizeof(float) * std::max<long long>(n, 1)));
+    if (n > 0) {
+        const int bs = 256;
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, src, *dst, n);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
 static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **out, std::string &err) {
     auto it = c->opers.find(key);
     if (it != c->opers.end()) {
@@ -404,6 +494,7 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
     if (!rc) rc = alloc_copy(f.noff, &o.noff, c->stream, err);
     if (!rc) rc = alloc_copy(f.blocks, &o.blocks, c->stream, err);
     if (!rc) rc = alloc_copy(f.norms, &o.norms, c->stream, err);
+    o.nblk = (long long)f.blocks.size();
     if (rc) {
         free_oper(o);
         return rc;
@@ -436,6 +527,10 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
             err = "band half-width " + std::to_string(wq) + " exceeds the kernel limit " + std::to_string(kMaxBand);
             return -1;
         }
+    if ((long long)(2 * kMaxBand + 1) * (2 * kMaxBand + 1) * (2 * kMaxBand + 1) >= (1 << 22)) {
+        err = "band index does not fit the worklist encoding";
+        return -1;
+    }
     if (plan.kp1 < 2 || plan.kp1 > 12) {
         err = "unsupported basis size k+1 = " + std::to_string(plan.kp1);
         return -1;
@@ -445,8 +540,10 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
     if (rc) return rc;
     auto *p = new DevicePlan;
     p->ctx = c;
+    p->op_key = op_key;
     p->n_out = (int)plan.out_node.size();
     p->kp1 = plan.kp1;
+    p->npool = (long long)plan.in_coefs.size();
     for (int wq : plan.band_w) p->wmax = std::max(p->wmax, wq);
     KArgs &a = p->args;
     rc = plan_copy(p, plan.out_depth, &a.out_depth, err);
@@ -460,10 +557,15 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
     if (!rc) rc = plan_copy(p, plan.bw, &a.bw, err);
     if (!rc) rc = plan_copy(p, plan.active_off, &a.act_off, err);
     if (!rc) rc = plan_copy(p, plan.active_terms, &a.act_terms, err);
+    if (!rc) rc = plan_copy(p, plan.seg_job, &a.seg_job, err);
+    if (!rc) rc = plan_copy(p, plan.seg_p0, &a.seg_p0, err);
+    if (!rc) rc = plan_copy(p, plan.seg_p1, &a.seg_p1, err);
     if (rc) {
         delete p;
         return rc;
     }
+    a.thr_floor = plan.thr_floor;
+    p->n_seg = (int)plan.seg_job.size();
     const int N = plan.kp1;
     p->out_bytes = sizeof(double) * (size_t)std::max(p->n_out, 1) * 8 * N * N * N;
     void *d_out = nullptr, *d_cnt = nullptr;
@@ -502,41 +604,85 @@ void dev_plan_destroy(DevicePlan *p) {
     delete p;
 }
 
-template <int N>
-static int launch_n(const KArgs &a, int n_out, int wmax, hipStream_t s, std::string &err) {
+template <int N, typename T>
+static int launch_nt(const KArgs &a, const T *pool, const T *ob, int n_seg, int wmax, double lo, double hi, int cls,
+                     hipStream_t s, std::string &err) {
     const size_t shmem = sizeof(double) * 64 * 4 * (2 * wmax + 1);
-    hipLaunchKernelGGL(k_apply_fp64<N>, dim3(8 * n_out), dim3(JobShape<N>::T), shmem, s, a);
+    hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, ob, lo, hi, cls);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int dev_plan_run(DevicePlan *p, double *kernel_ms, unsigned long long *term_apps, std::string &err) {
+template <typename T>
+static int launch_t(const DevicePlan *p, const T *pool, const T *ob, double lo, double hi, int cls, std::string &err) {
+    const KArgs &a = p->args;
+    hipStream_t s = p->ctx->stream;
+    switch (p->kp1) {
+        case 2: return launch_nt<2, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 3: return launch_nt<3, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 4: return launch_nt<4, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 5: return launch_nt<5, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 6: return launch_nt<6, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 7: return launch_nt<7, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 8: return launch_nt<8, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 9: return launch_nt<9, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 10: return launch_nt<10, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 11: return launch_nt<11, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 12: return launch_nt<12, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        default: err = "unsupported basis size"; return -1;
+    }
+}
+
+int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
+                 unsigned long long *lp_apps, std::string &err, double *split_ms) {
     DeviceCtx *c = p->ctx;
     HIP_TRY(hipSetDevice(c->device));
-    if (p->n_out == 0) {
-        if (kernel_ms) *kernel_ms = 0.0;
-        if (term_apps) *term_apps = 0;
-        return 0;
+    if (split_ms) split_ms[0] = split_ms[1] = 0.0;
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (term_apps) *term_apps = 0;
+    if (lp_apps) *lp_apps = 0;
+    if (p->n_out == 0) return 0;
+    if (mode != 0 && mode != 1) {
+        err = "unknown arithmetic mode";
+        return -1;
+    }
+    if (mode == 1 && !(tau >= 1.0)) {
+        err = "mixed-precision ratio must be >= 1";
+        return -1;
+    }
+    if (mode == 1) {
+        auto it = c->opers.find(p->op_key);
+        if (it == c->opers.end()) {
+            err = "operator tables not resident";
+            return -1;
+        }
+        DevOper &o = it->second;
+        int rc = 0;
+        if (!o.blocks32) rc = to_f32(o.blocks, &o.blocks32, o.nblk, c->stream, err);
+        if (!rc && !p->pool32) rc = to_f32(p->args.pool, &p->pool32, p->npool, c->stream, err);
+        if (rc) return rc;
     }
     // counters cleared by a kernel (not a memset node, so the launch sequence
     // can be captured into a graph)
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, p->args.counters, 8);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    {
+        const long long nz = (long long)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1;
+        const int bs = 256;
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((nz + bs - 1) / bs)), dim3(bs), 0, c->stream, p->args.out_ns, nz);
+        HIP_TRY(hipGetLastError());
+    }
     int rc = 0;
-    switch (p->kp1) {
-        case 2: rc = launch_n<2>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 3: rc = launch_n<3>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 4: rc = launch_n<4>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 5: rc = launch_n<5>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 6: rc = launch_n<6>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 7: rc = launch_n<7>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 8: rc = launch_n<8>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 9: rc = launch_n<9>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 10: rc = launch_n<10>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 11: rc = launch_n<11>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        case 12: rc = launch_n<12>(p->args, p->n_out, p->wmax, c->stream, err); break;
-        default: err = "unsupported basis size"; return -1;
+    const double inf = HUGE_VAL;
+    if (mode == 0) {
+        // lo = 0: every term that passes screening
+        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, 0.0, inf, 0, err);
+    } else {
+        const float *ob32 = c->opers[p->op_key].blocks32;
+        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, tau, inf, 0, err);
+        if (!rc) HIP_TRY(hipEventRecord(c->evm, c->stream));
+        if (!rc) rc = launch_t<float>(p, p->pool32, ob32, 0.0, tau, 1, err);
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -546,7 +692,19 @@ int dev_plan_run(DevicePlan *p, double *kernel_ms, unsigned long long *term_apps
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (kernel_ms) *kernel_ms = ms;
+    if (split_ms) {
+        if (mode == 0) {
+            split_ms[0] = ms;
+        } else {
+            float m1 = 0.f, m2 = 0.f;
+            HIP_TRY(hipEventElapsedTime(&m1, c->ev0, c->evm));
+            HIP_TRY(hipEventElapsedTime(&m2, c->evm, c->ev1));
+            split_ms[0] = m1;
+            split_ms[1] = m2;
+        }
+    }
     if (term_apps) *term_apps = cnt[0];
+    if (lp_apps) *lp_apps = cnt[1];
     return 0;
 }
 
@@ -560,12 +718,12 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
     return 0;
 }
 
-int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, double *out_ns,
-                   double *kernel_ms, unsigned long long *term_apps, std::string &err) {
+int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, int mode, double tau,
+                   double *out_ns, double *kernel_ms, unsigned long long *term_apps, std::string &err) {
     DevicePlan *p = nullptr;
     int rc = dev_plan_create(c, op_key, f, plan, &p, err);
     if (rc) return rc;
-    rc = dev_plan_run(p, kernel_ms, term_apps, err);
+    rc = dev_plan_run(p, mode, tau, kernel_ms, term_apps, nullptr, err, nullptr);
     if (!rc) rc = dev_plan_download(p, out_ns, err);
     dev_plan_destroy(p);
     return rc;

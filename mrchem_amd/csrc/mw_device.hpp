@@ -21,12 +21,17 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
 // Run the per-node calculator over the resident plan.  kernel_ms receives the
 // device time of the launch, term_apps the number of term-applications that
 // passed screening.
-int dev_plan_run(DevicePlan *p, double *kernel_ms, unsigned long long *term_apps, std::string &err);
+// mode 0: every term in FP64.  mode 1: terms with bound > tau * thr in FP64,
+// the rest on FP32 MFMA, their sum added to the FP64 output.  lp_apps
+// receives the number of FP32 term-applications.
+// split_ms (optional, 2 values): device time of the FP64 and FP32 launches.
+int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
+                 unsigned long long *lp_apps, std::string &err, double *split_ms = nullptr);
 // Copy the raw non-standard blocks (n_out x 8(k+1)^3) to the host.
 int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err);
 void dev_plan_destroy(DevicePlan *p);
 // create + run + download + destroy
-int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, double *out_ns,
-                   double *kernel_ms, unsigned long long *term_apps, std::string &err);
+int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, int mode, double tau,
+                   double *out_ns, double *kernel_ms, unsigned long long *term_apps, std::string &err);
 
 }  // namespace mwg

@@ -194,6 +194,13 @@ struct ApplyPlan {
     std::vector<int> bw;
     std::vector<int> active_off, active_terms;  // per depth, terms with any w >= 0
     long long n_generated = 0;
+    // Work segments: job = 8*out + gt; a segment covers band positions
+    // [seg_p0, seg_p1) of one job.  Jobs are cut so that no segment holds
+    // many more than `seg_target` term-applications (counted exactly here with
+    // the kernel's screening rule).
+    std::vector<int> seg_job, seg_p0, seg_p1;
+    long long term_apps = 0;   // exact count of term-applications in the plan
+    double thr_floor = 0.0;    // smallest positive threshold of the plan
 };
 
 struct ApplyStats {
@@ -208,7 +215,7 @@ struct ApplyStats {
 // thr_param (thr_g = thr_param * prec_fac[g]).
 void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
                 const std::vector<int> &nodes, const std::vector<double> &thr,
-                int max_band_w);
+                int max_band_w, int seg_target = 256);
 double oper_norm(const Operator &op, int m, int d, int c, long long l);
 
 // Operator tables flattened for the device: per (m, d) the half-width L, the

@@ -48,7 +48,7 @@ double oper_norm(const Operator &op, int m, int d, int c, long long l) {
 }
 
 void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
-                const std::vector<int> &nodes, const std::vector<double> &thr, int max_band_w) {
+                const std::vector<int> &nodes, const std::vector<double> &thr, int max_band_w, int seg_target) {
     const MRA &mra = *out.mra;
     const int n3 = mra.kp1_3(), blk = mra.block();
     const int M = op.size(), ndepth = op.max_depth + 1;
@@ -225,6 +225,82 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         for (int e = 0; e < n3; e++) q += dst[e] * dst[e];
         plan.in_norms[(size_t)s * 8] = std::sqrt(q);
         plan.n_generated++;
+    }
+
+    // Exact term-application counts per (job, band position) with the
+    // kernel's rule: a candidate (term m, input component ft) of band node f
+    // for output component gt passes when |l_d| <= bw(m, d, c_d) in every
+    // direction and ((N_x N_y) N_z) ||f_ft|| > thr.  Jobs are cut into
+    // segments of about seg_target passing terms.
+    plan.thr_floor = 0.0;
+    for (double t : thr)
+        if (t > 0.0 && (plan.thr_floor == 0.0 || t < plan.thr_floor)) plan.thr_floor = t;
+    const int nq = (int)nodes.size();
+    std::vector<std::vector<int>> counts((size_t)nq * 8);
+    long long napps = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : napps)
+    for (int q = 0; q < nq; q++) {
+        const int d = plan.out_depth[q], w = plan.band_w[q], side = 2 * w + 1;
+        const int nb = side * side * side;
+        const double th = plan.out_thr[q];
+        const int a0 = plan.active_off[d], a1 = plan.active_off[d + 1];
+        // per term, component and translation: the norm the kernel would use
+        const int nl = 2 * w + 1;
+        std::vector<double> tn((size_t)(a1 - a0) * 4 * nl, 0.0);
+        for (int ai = a0; ai < a1; ai++) {
+            const int m = plan.active_terms[ai];
+            const OperDepth &od = op.terms[m].depth[d];
+            for (int c = 0; c < 4; c++) {
+                const int bwc = plan.bw[((size_t)d * M + m) * 4 + c];
+                for (int l = -w; l <= w; l++)
+                    if (std::abs(l) <= bwc && std::abs(l) <= od.L)
+                        tn[((size_t)(ai - a0) * 4 + c) * nl + (l + w)] = oper_norm(op, m, d, c, l);
+            }
+        }
+        for (int gt = 0; gt < 8; gt++) {
+            std::vector<int> &cp = counts[(size_t)q * 8 + gt];
+            cp.assign(nb, 0);
+            for (int p = 0; p < nb; p++) {
+                const int sl = plan.band[plan.band_off[q] + p];
+                if (sl < 0) continue;
+                const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+                for (int ft = 0; ft < 8; ft++) {
+                    if (d > 0 && ft == 0 && gt == 0) continue;
+                    const double fn = plan.in_norms[(size_t)sl * 8 + ft];
+                    if (!(fn > 0.0)) continue;
+                    int cxyz[3];
+                    for (int dim = 0; dim < 3; dim++) cxyz[dim] = (((gt >> dim) & 1) << 1) | ((ft >> dim) & 1);
+                    for (int ai = 0; ai < a1 - a0; ai++) {
+                        const double nx = tn[((size_t)ai * 4 + cxyz[0]) * nl + (ll[0] + w)];
+                        const double ny = tn[((size_t)ai * 4 + cxyz[1]) * nl + (ll[1] + w)];
+                        const double nz = tn[((size_t)ai * 4 + cxyz[2]) * nl + (ll[2] + w)];
+                        if (((nx * ny) * nz) * fn > th) cp[p]++;
+                    }
+                }
+                napps += cp[p];
+            }
+        }
+    }
+    plan.term_apps = napps;
+    for (int q = 0; q < nq; q++) {
+        const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
+        for (int gt = 0; gt < 8; gt++) {
+            const std::vector<int> &cp = counts[(size_t)q * 8 + gt];
+            int p0 = 0, acc = 0;
+            for (int p = 0; p < nb; p++) {
+                acc += cp[p];
+                if (acc >= seg_target && p + 1 < nb) {
+                    plan.seg_job.push_back(q * 8 + gt);
+                    plan.seg_p0.push_back(p0);
+                    plan.seg_p1.push_back(p + 1);
+                    p0 = p + 1;
+                    acc = 0;
+                }
+            }
+            plan.seg_job.push_back(q * 8 + gt);
+            plan.seg_p0.push_back(p0);
+            plan.seg_p1.push_back(nb);
+        }
     }
 }
 

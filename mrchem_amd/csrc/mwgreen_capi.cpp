@@ -28,6 +28,8 @@ struct mwg_context_s {
     DeviceCtx *dev = nullptr;
     std::vector<double> last_stats = std::vector<double>(8, 0.0);
     double term_apps = 0.0;  // accumulated by run_plan within one apply
+    int mode = MWG_MODE_FP64;
+    double tau = 65536.0;    // mixed mode: FP64 for terms with bound > tau * thr
 };
 struct mwg_plan_s {
     mwg_context ctx = nullptr;
@@ -35,7 +37,8 @@ struct mwg_plan_s {
     mwg_tree out = nullptr;
     std::vector<double> thr;
     int n_out = 0;
-    double plan_ms = 0.0, slots = 0.0, gen = 0.0;
+    int mode = MWG_MODE_FP64;
+    double plan_ms = 0.0, slots = 0.0, gen = 0.0, host_apps = 0.0, segs = 0.0;
 };
 struct mwg_mra_s {
     MRA mra;
@@ -122,7 +125,7 @@ int run_plan(mwg_context ctx, mwg_operator_s *op, const Tree &inp, Tree &out, co
     std::string err;
     double ms = 0.0;
     unsigned long long apps = 0;
-    int rc = dev_apply_plan(ctx->dev, op, op->flat, plan, ns.data(), &ms, &apps, err);
+    int rc = dev_apply_plan(ctx->dev, op, op->flat, plan, ctx->mode, ctx->tau, ns.data(), &ms, &apps, err);
     if (rc) return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
     kms += ms;
     ctx->term_apps += (double)apps;
@@ -534,7 +537,7 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
     return guarded([&] {
         if (!ctx || !out || !op || !inp || !thr) return fail(MWG_ERR_ARG, "null argument");
         if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
-        if (mode != MWG_MODE_FP64) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+        if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
         Tree &o = out->tree;
         check_compatible(o, op->op, inp->tree);
         const int n = o.size(), blk = o.mra->block();
@@ -547,7 +550,10 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
         double kms = 0, pms = 0, slots = 0, gen = 0;
         std::vector<double> ns;
         ctx->term_apps = 0.0;
+        const int saved = ctx->mode;
+        ctx->mode = mode;
         int rc = run_plan(ctx, op, inp->tree, o, nodes, th, ns, kms, pms, &slots, &gen);
+        ctx->mode = saved;
         if (rc) return rc;
         std::memcpy(o.coefs.data(), ns.data(), sizeof(double) * (size_t)n * blk);
         finish_output(o);
@@ -570,7 +576,7 @@ int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
     return guarded([&] {
         if (!ctx || !ctx->dev || !out || !op || !inp || !thr || !plan) return fail(MWG_ERR_ARG, "null argument");
         if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
-        if (mode != MWG_MODE_FP64) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+        if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
         Tree &o = out->tree;
         check_compatible(o, op->op, inp->tree);
         const int n = o.size();
@@ -587,6 +593,8 @@ int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
         p->plan_ms = now_ms() - t0;
         p->slots = (double)ap.slot_off.size();
         p->gen = (double)ap.n_generated;
+        p->host_apps = (double)ap.term_apps;
+        p->segs = (double)ap.seg_job.size();
         std::string err;
         int rc = dev_plan_create(ctx->dev, op, op->flat, ap, &p->dev, err);
         if (rc) {
@@ -597,21 +605,22 @@ int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
         p->out = out;
         p->thr = th;
         p->n_out = n;
+        p->mode = mode;
         *plan = p;
         return MWG_OK;
     });
 }
 
-int mwg_plan_execute(mwg_plan plan, double *kernel_ms, double *term_apps) {
+int mwg_plan_execute(mwg_plan plan, double *stats, int n) {
     return guarded([&] {
         if (!plan) return fail(MWG_ERR_ARG, "null plan");
         std::string err;
-        double ms = 0.0;
-        unsigned long long apps = 0;
-        int rc = dev_plan_run(plan->dev, &ms, &apps, err);
-        if (rc) return fail(MWG_ERR_DEVICE, err);
-        if (kernel_ms) *kernel_ms = ms;
-        if (term_apps) *term_apps = (double)apps;
+        double ms = 0.0, split[2] = {0.0, 0.0};
+        unsigned long long apps = 0, lp = 0;
+        int rc = dev_plan_run(plan->dev, plan->mode, plan->ctx->tau, &ms, &apps, &lp, err, split);
+        if (rc) return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
+        const double v[5] = {ms, (double)apps, (double)lp, split[0], split[1]};
+        for (int i = 0; stats && i < n && i < 5; i++) stats[i] = v[i];
         return MWG_OK;
     });
 }
@@ -642,8 +651,8 @@ int mwg_plan_download_ns(mwg_plan plan, double *ns) {
 
 int mwg_plan_info(mwg_plan plan, double *info, int n) {
     if (!plan || !info) return fail(MWG_ERR_ARG, "null pointer argument");
-    double v[4] = {(double)plan->n_out, plan->slots, plan->gen, plan->plan_ms};
-    for (int i = 0; i < n && i < 4; i++) info[i] = v[i];
+    double v[6] = {(double)plan->n_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs};
+    for (int i = 0; i < n && i < 6; i++) info[i] = v[i];
     return MWG_OK;
 }
 
@@ -651,6 +660,15 @@ int mwg_plan_destroy(mwg_plan plan) {
     if (!plan) return MWG_OK;
     dev_plan_destroy(plan->dev);
     delete plan;
+    return MWG_OK;
+}
+
+int mwg_context_set_precision(mwg_context ctx, int mode, double tau) {
+    if (!ctx) return fail(MWG_ERR_ARG, "null context");
+    if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+    if (!(tau >= 1.0) || !std::isfinite(tau)) return fail(MWG_ERR_ARG, "mixed-precision ratio must be finite and >= 1");
+    ctx->mode = mode;
+    ctx->tau = tau;
     return MWG_OK;
 }
 

@@ -67,3 +67,45 @@ def test_helmholtz_gaussian_sum(ctx):
     E = rho.dot(V)
     E_ex = cf.pair_energy(coef, exps, cen, mu=mu)
     assert abs(E - E_ex) < 30 * prec * max(1.0, abs(E_ex))
+
+
+@pytest.mark.parametrize("order", [7, 9])
+def test_mixed_precision_term_set_and_error(ctx, order):
+    """Mixed mode applies exactly the FP64 term set (screening identical, the
+    FP32 class is the complement of the FP64 class) and stays within a small
+    fraction of prec of the FP64 result on the same frozen grid."""
+    mra = _mra(order)
+    prec = 1e-6
+    P = mw.PoissonOperator(mra, prec)
+    rng = np.random.default_rng(11)
+    n = 4
+    coef = rng.normal(size=n)
+    exps = np.exp(rng.uniform(np.log(1.0), np.log(60.0), size=n))
+    cen = rng.uniform(-1.0, 1.0, size=(n, 3))
+    rho = mw.FunctionTree(mra).project_gaussians(prec, coef, exps, cen)
+    V = mw.FunctionTree(mra)
+    mw.apply(prec / rho.norm(), V, P, rho, ctx=ctx)
+    thr = V.thresholds()
+    od, ol, _ = V.nodes()
+    V2 = mw.FunctionTree(mra).set_nodes(od, ol)
+    p64 = mw.ApplyPlan(V, P, rho, thr, mode="fp64", ctx=ctx)
+    pmx = mw.ApplyPlan(V2, P, rho, thr, mode="mixed", ctx=ctx)
+    r64 = p64.execute()
+    rmx = pmx.execute()
+    assert r64["term_apps"] == rmx["term_apps"]
+    assert r64["term_apps"] == p64.info()["host_term_apps"]
+    assert 0 < rmx["lp_term_apps"] < rmx["term_apps"]
+    ns64 = p64.download_ns()
+    nsmx = pmx.download_ns()
+    err = np.max(np.abs(nsmx - ns64)) / np.max(np.abs(ns64))
+    print("order %d: %d of %d term-apps on FP32, max coef err vs FP64 %.2e" %
+          (order, rmx["lp_term_apps"], rmx["term_apps"], err))
+    # requirement: within prec of the FP64 result, relative to the largest
+    # coefficient (observed ~1e-7 at tau = 65536)
+    assert err < prec
+    # and the FP64 plan reproduces the oracle on this grid
+    ref, st = oracle.apply_fixed(mra, P.tables(), rho.nodes(), od, ol, thr, want_ns=False)
+    p64.finish()
+    c64 = V.nodes()[2]
+    assert np.max(np.abs(ref - c64)) / np.max(np.abs(ref)) < TOL_FP64
+    assert st["term_apps"] == r64["term_apps"]
