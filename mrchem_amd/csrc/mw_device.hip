@@ -133,8 +133,8 @@ constexpr int kMaxBand = 9;     // band half-width limit
 constexpr int kWorkCap = 1024;  // worklist items per batch (>= 8 * 64 + 512)
 
 template <int N, typename T>
-__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const T *opblk, double lo_mult,
-                                                          double hi_mult, int cls_counter) {
+__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const T *opblk, const int *seg_list,
+                                                          double lo_mult, double hi_mult, int cls_counter) {
     using S = JobShape<N>;
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
@@ -150,7 +150,8 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
     const int lc = lane & 15, lq = lane >> 4;
     for (int e = tid; e < BUF; e += NTH) buf[e] = T(0);
 
-    const int seg = blockIdx.x;
+    // the launch's segments: all of them, or the listed ones
+    const int seg = seg_list ? seg_list[blockIdx.x] : (int)blockIdx.x;
     const int job = a.seg_job[seg], p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
     const int g = job >> 3, gt = job & 7;
     const int d = a.out_depth[g];
@@ -404,10 +405,18 @@ struct DevicePlan {
     KArgs args{};
     std::vector<void *> bufs;
     size_t out_bytes = 0;
+    // per segment: class cut base thr_cls and the range of passing bounds
+    std::vector<double> seg_thr, seg_bmin, seg_bmax;
+    // segment lists of the two classes of the mixed mode, for list_tau
+    double list_tau = -1.0;
+    int *list_hi = nullptr, *list_lo = nullptr;
+    int n_hi = 0, n_lo = 0;
     ~DevicePlan() {
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (pool32) (void)hipFree(pool32);
+        if (list_hi) (void)hipFree(list_hi);
+        if (list_lo) (void)hipFree(list_lo);
     }
 };
 
@@ -566,6 +575,13 @@ int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const A
     }
     a.thr_floor = plan.thr_floor;
     p->n_seg = (int)plan.seg_job.size();
+    p->seg_bmin = plan.seg_bmin;
+    p->seg_bmax = plan.seg_bmax;
+    p->seg_thr.resize(p->n_seg);
+    for (int i = 0; i < p->n_seg; i++) {
+        const double t = plan.out_thr[plan.seg_job[i] >> 3];
+        p->seg_thr[i] = t > plan.thr_floor ? t : plan.thr_floor;  // as the kernel forms thr_cls
+    }
     const int N = plan.kp1;
     p->out_bytes = sizeof(double) * (size_t)std::max(p->n_out, 1) * 8 * N * N * N;
     void *d_out = nullptr, *d_cnt = nullptr;
@@ -605,32 +621,62 @@ void dev_plan_destroy(DevicePlan *p) {
 }
 
 template <int N, typename T>
-static int launch_nt(const KArgs &a, const T *pool, const T *ob, int n_seg, int wmax, double lo, double hi, int cls,
-                     hipStream_t s, std::string &err) {
+static int launch_nt(const KArgs &a, const T *pool, const T *ob, const int *list, int n_seg, int wmax, double lo,
+                     double hi, int cls, hipStream_t s, std::string &err) {
+    if (n_seg == 0) return 0;
     const size_t shmem = sizeof(double) * 64 * 4 * (2 * wmax + 1);
-    hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, ob, lo, hi, cls);
+    hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, ob, list, lo, hi, cls);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 template <typename T>
-static int launch_t(const DevicePlan *p, const T *pool, const T *ob, double lo, double hi, int cls, std::string &err) {
+static int launch_t(const DevicePlan *p, const T *pool, const T *ob, const int *list, int n_seg, double lo, double hi,
+                    int cls, std::string &err) {
     const KArgs &a = p->args;
     hipStream_t s = p->ctx->stream;
+#define MWG_LAUNCH(NN) launch_nt<NN, T>(a, pool, ob, list, n_seg, p->wmax, lo, hi, cls, s, err)
     switch (p->kp1) {
-        case 2: return launch_nt<2, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 3: return launch_nt<3, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 4: return launch_nt<4, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 5: return launch_nt<5, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 6: return launch_nt<6, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 7: return launch_nt<7, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 8: return launch_nt<8, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 9: return launch_nt<9, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 10: return launch_nt<10, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 11: return launch_nt<11, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
-        case 12: return launch_nt<12, T>(a, pool, ob, p->n_seg, p->wmax, lo, hi, cls, s, err);
+        case 2: return MWG_LAUNCH(2);
+        case 3: return MWG_LAUNCH(3);
+        case 4: return MWG_LAUNCH(4);
+        case 5: return MWG_LAUNCH(5);
+        case 6: return MWG_LAUNCH(6);
+        case 7: return MWG_LAUNCH(7);
+        case 8: return MWG_LAUNCH(8);
+        case 9: return MWG_LAUNCH(9);
+        case 10: return MWG_LAUNCH(10);
+        case 11: return MWG_LAUNCH(11);
+        case 12: return MWG_LAUNCH(12);
         default: err = "unsupported basis size"; return -1;
     }
+#undef MWG_LAUNCH
+}
+
+// Segment lists of the mixed mode: segments holding a term with bound above
+// tau * thr_cls (the FP64 class) and segments holding one at or below it.  The
+// lists are widened by a relative 1e-9 so that rounding can only add segments;
+// the kernel's own class test decides each term.
+static int make_lists(DevicePlan *p, double tau, std::string &err) {
+    if (p->list_tau == tau) return 0;
+    std::vector<int> hi, lo;
+    for (int i = 0; i < p->n_seg; i++) {
+        const double cut = tau * p->seg_thr[i];
+        if (p->seg_bmax[i] > cut * (1.0 - 1e-9)) hi.push_back(i);
+        if (p->seg_bmin[i] <= cut * (1.0 + 1e-9)) lo.push_back(i);
+    }
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    if (p->list_hi) HIP_TRY(hipFree(p->list_hi));
+    if (p->list_lo) HIP_TRY(hipFree(p->list_lo));
+    p->list_hi = p->list_lo = nullptr;
+    p->list_tau = -1.0;
+    int rc = alloc_copy(hi, &p->list_hi, p->ctx->stream, err);
+    if (!rc) rc = alloc_copy(lo, &p->list_lo, p->ctx->stream, err);
+    if (rc) return rc;
+    p->n_hi = (int)hi.size();
+    p->n_lo = (int)lo.size();
+    p->list_tau = tau;
+    return 0;
 }
 
 int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
@@ -660,6 +706,7 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
         int rc = 0;
         if (!o.blocks32) rc = to_f32(o.blocks, &o.blocks32, o.nblk, c->stream, err);
         if (!rc && !p->pool32) rc = to_f32(p->args.pool, &p->pool32, p->npool, c->stream, err);
+        if (!rc) rc = make_lists(p, tau, err);
         if (rc) return rc;
     }
     // counters cleared by a kernel (not a memset node, so the launch sequence
@@ -677,12 +724,12 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     const double inf = HUGE_VAL;
     if (mode == 0) {
         // lo = 0: every term that passes screening
-        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, 0.0, inf, 0, err);
+        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, nullptr, p->n_seg, 0.0, inf, 0, err);
     } else {
         const float *ob32 = c->opers[p->op_key].blocks32;
-        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, tau, inf, 0, err);
+        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, p->list_hi, p->n_hi, tau, inf, 0, err);
         if (!rc) HIP_TRY(hipEventRecord(c->evm, c->stream));
-        if (!rc) rc = launch_t<float>(p, p->pool32, ob32, 0.0, tau, 1, err);
+        if (!rc) rc = launch_t<float>(p, p->pool32, ob32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));

@@ -146,6 +146,9 @@ struct Tree {
 //           out = [s, d1..d7] (component bit d set = wavelet along d).
 void mw_compress(const Basis &b, const double *in, double *out);
 void mw_reconstruct(const Basis &b, const double *in, double *out);
+// Scaling block of child t alone (t = bx + 2 by + 4 bz); `in` is a full
+// compressed block, or only its scaling part when s_only.
+void mw_child_scaling(const Basis &b, const double *in, bool s_only, int t, double *out);
 void component_norms(const MRA &mra, const double *blk, double *norms8);
 double tree_square_norm(const Tree &t);
 double tree_dot(const Tree &a, const Tree &b);
@@ -197,8 +200,12 @@ struct ApplyPlan {
     // Work segments: job = 8*out + gt; a segment covers band positions
     // [seg_p0, seg_p1) of one job.  Jobs are cut so that no segment holds
     // many more than `seg_target` term-applications (counted exactly here with
-    // the kernel's screening rule).
+    // the kernel's screening rule); jobs without any are left out.
     std::vector<int> seg_job, seg_p0, seg_p1;
+    // per segment: passing terms and the smallest / largest screening bound
+    // among them (for choosing the segments of each precision class)
+    std::vector<int> seg_apps;
+    std::vector<double> seg_bmin, seg_bmax;
     long long term_apps = 0;   // exact count of term-applications in the plan
     double thr_floor = 0.0;    // smallest positive threshold of the plan
 };

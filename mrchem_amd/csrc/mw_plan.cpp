@@ -163,7 +163,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     }
 
     // Coefficient pool: real slots keep the full block, generated slots the
-    // scaling part only.  Generated blocks are memoised along their paths.
+    // scaling part only.
     const int ns = (int)slot_keys.size();
     plan.slot_off.resize(ns);
     long long pool = 0;
@@ -174,58 +174,89 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     plan.in_coefs.assign(pool, 0.0);
     plan.in_norms.assign((size_t)ns * 8, 0.0);
     plan.in_depth.resize(ns);
-    std::unordered_map<GenKey, std::vector<double>, GenKeyH> gen;  // s-part of generated nodes
-    std::vector<double> tmp(blk), rec(blk);
-    for (int s = 0; s < ns; s++) {
-        const GenKey k = slot_keys[s];
-        plan.in_depth[s] = k.d;
-        double *dst = plan.in_coefs.data() + plan.slot_off[s];
-        if (slot_real[s] >= 0) {
-            std::memcpy(dst, inp.node(slot_real[s]), sizeof(double) * blk);
-            component_norms(mra, dst, &plan.in_norms[(size_t)s * 8]);
-            continue;
-        }
-        // walk up to the nearest memoised or real ancestor
+    // Generated nodes (and the generated ancestors they descend from) are
+    // collected first, then filled depth by depth in parallel: each is one
+    // child of its parent, which is either a real node (full block) or a
+    // generated node of the previous depth (scaling part).
+    std::unordered_map<GenKey, int, GenKeyH> gen_index;
+    std::vector<GenKey> gen_keys;
+    std::vector<int> gen_parent;  // generated parent index, or -(real node + 1)
+    auto need = [&](const GenKey &k0) -> int {
+        // index of generated key k0, registering it and its generated ancestors
         std::vector<GenKey> path;
-        GenKey a = k;
-        const double *src_full = nullptr;
-        const std::vector<double> *src_gen = nullptr;
+        GenKey a = k0;
+        int parent = 0;
+        bool found = false;
         while (true) {
-            auto gi = gen.find(a);
-            if (gi != gen.end()) {
-                src_gen = &gi->second;
+            auto gi = gen_index.find(a);
+            if (gi != gen_index.end() && !path.empty()) {
+                parent = gi->second;
+                found = true;
                 break;
             }
+            if (gi != gen_index.end()) return gi->second;
             long long l[3] = {a.x, a.y, a.z};
             int r = inp.find(a.d, l);
-            if (r >= 0) {
-                src_full = inp.node(r);
+            if (r >= 0 && !path.empty()) {
+                parent = -(r + 1);
+                found = true;
                 break;
             }
             if (a.d == 0) break;
             path.push_back(a);
             a = GenKey{a.d - 1, a.x >> 1, a.y >> 1, a.z >> 1};
         }
-        if (!src_full && !src_gen) throw std::runtime_error("apply: input tree has no root covering a band node");
-        if (src_full) std::memcpy(tmp.data(), src_full, sizeof(double) * blk);
-        else {
-            std::fill(tmp.begin(), tmp.end(), 0.0);
-            std::memcpy(tmp.data(), src_gen->data(), sizeof(double) * n3);
-        }
+        if (!found) throw std::runtime_error("apply: input tree has no root covering a band node");
         for (int p = (int)path.size() - 1; p >= 0; p--) {
-            const GenKey &ch = path[p];
-            int t = (int)((ch.x & 1) | ((ch.y & 1) << 1) | ((ch.z & 1) << 2));
-            mw_reconstruct(mra.basis, tmp.data(), rec.data());
-            std::fill(tmp.begin(), tmp.end(), 0.0);
-            std::memcpy(tmp.data(), rec.data() + (size_t)t * n3, sizeof(double) * n3);
-            gen.emplace(ch, std::vector<double>(tmp.begin(), tmp.begin() + n3));
+            const int idx = (int)gen_keys.size();
+            gen_index.emplace(path[p], idx);
+            gen_keys.push_back(path[p]);
+            gen_parent.push_back(parent);
+            parent = idx;
         }
-        std::memcpy(dst, tmp.data(), sizeof(double) * n3);
+        return parent;
+    };
+    std::vector<int> slot_gen(ns, -1);
+    for (int s = 0; s < ns; s++)
+        if (slot_real[s] < 0) slot_gen[s] = need(slot_keys[s]);
+    const int ng = (int)gen_keys.size();
+    std::vector<double> gen_s((size_t)ng * n3);
+    {
+        std::vector<int> order(ng);
+        for (int i = 0; i < ng; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gen_keys[x].d < gen_keys[y].d; });
+        size_t i0 = 0;
+        while (i0 < order.size()) {
+            size_t i1 = i0;
+            while (i1 < order.size() && gen_keys[order[i1]].d == gen_keys[order[i0]].d) i1++;
+#pragma omp parallel for schedule(dynamic, 8)
+            for (long long ii = (long long)i0; ii < (long long)i1; ii++) {
+                const int gi = order[ii];
+                const GenKey &ch = gen_keys[gi];
+                const int t = (int)((ch.x & 1) | ((ch.y & 1) << 1) | ((ch.z & 1) << 2));
+                const int par = gen_parent[gi];
+                double *o = gen_s.data() + (size_t)gi * n3;
+                if (par < 0) mw_child_scaling(mra.basis, inp.node(-par - 1), false, t, o);
+                else mw_child_scaling(mra.basis, gen_s.data() + (size_t)par * n3, true, t, o);
+            }
+            i0 = i1;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int s = 0; s < ns; s++) {
+        plan.in_depth[s] = slot_keys[s].d;
+        double *dst = plan.in_coefs.data() + plan.slot_off[s];
+        if (slot_real[s] >= 0) {
+            std::memcpy(dst, inp.node(slot_real[s]), sizeof(double) * blk);
+            component_norms(mra, dst, &plan.in_norms[(size_t)s * 8]);
+            continue;
+        }
+        std::memcpy(dst, gen_s.data() + (size_t)slot_gen[s] * n3, sizeof(double) * n3);
         double q = 0.0;
         for (int e = 0; e < n3; e++) q += dst[e] * dst[e];
         plan.in_norms[(size_t)s * 8] = std::sqrt(q);
-        plan.n_generated++;
     }
+    for (int s = 0; s < ns; s++) plan.n_generated += (slot_real[s] < 0);
 
     // Exact term-application counts per (job, band position) with the
     // kernel's rule: a candidate (term m, input component ft) of band node f
@@ -237,6 +268,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         if (t > 0.0 && (plan.thr_floor == 0.0 || t < plan.thr_floor)) plan.thr_floor = t;
     const int nq = (int)nodes.size();
     std::vector<std::vector<int>> counts((size_t)nq * 8);
+    std::vector<std::vector<double>> bmin((size_t)nq * 8), bmax((size_t)nq * 8);
     long long napps = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : napps)
     for (int q = 0; q < nq; q++) {
@@ -259,7 +291,10 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         }
         for (int gt = 0; gt < 8; gt++) {
             std::vector<int> &cp = counts[(size_t)q * 8 + gt];
+            std::vector<double> &bl = bmin[(size_t)q * 8 + gt], &bh = bmax[(size_t)q * 8 + gt];
             cp.assign(nb, 0);
+            bl.assign(nb, HUGE_VAL);
+            bh.assign(nb, 0.0);
             for (int p = 0; p < nb; p++) {
                 const int sl = plan.band[plan.band_off[q] + p];
                 if (sl < 0) continue;
@@ -274,7 +309,12 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
                         const double nx = tn[((size_t)ai * 4 + cxyz[0]) * nl + (ll[0] + w)];
                         const double ny = tn[((size_t)ai * 4 + cxyz[1]) * nl + (ll[1] + w)];
                         const double nz = tn[((size_t)ai * 4 + cxyz[2]) * nl + (ll[2] + w)];
-                        if (((nx * ny) * nz) * fn > th) cp[p]++;
+                        const double b = ((nx * ny) * nz) * fn;
+                        if (b > th) {
+                            cp[p]++;
+                            bl[p] = std::min(bl[p], b);
+                            bh[p] = std::max(bh[p], b);
+                        }
                     }
                 }
                 napps += cp[p];
@@ -286,20 +326,30 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
         for (int gt = 0; gt < 8; gt++) {
             const std::vector<int> &cp = counts[(size_t)q * 8 + gt];
+            const std::vector<double> &bl = bmin[(size_t)q * 8 + gt], &bh = bmax[(size_t)q * 8 + gt];
             int p0 = 0, acc = 0;
+            double lo = HUGE_VAL, hi = 0.0;
             for (int p = 0; p < nb; p++) {
                 acc += cp[p];
-                if (acc >= seg_target && p + 1 < nb) {
-                    plan.seg_job.push_back(q * 8 + gt);
-                    plan.seg_p0.push_back(p0);
-                    plan.seg_p1.push_back(p + 1);
+                lo = std::min(lo, bl[p]);
+                hi = std::max(hi, bh[p]);
+                if ((acc >= seg_target && p + 1 < nb) || p + 1 == nb) {
+                    // segments without passing terms are dropped: the output
+                    // is zeroed before the launches
+                    if (acc > 0) {
+                        plan.seg_job.push_back(q * 8 + gt);
+                        plan.seg_p0.push_back(p0);
+                        plan.seg_p1.push_back(p + 1);
+                        plan.seg_apps.push_back(acc);
+                        plan.seg_bmin.push_back(lo);
+                        plan.seg_bmax.push_back(hi);
+                    }
                     p0 = p + 1;
                     acc = 0;
+                    lo = HUGE_VAL;
+                    hi = 0.0;
                 }
             }
-            plan.seg_job.push_back(q * 8 + gt);
-            plan.seg_p0.push_back(p0);
-            plan.seg_p1.push_back(nb);
         }
     }
 }

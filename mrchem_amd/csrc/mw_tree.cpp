@@ -145,6 +145,54 @@ void mw_reconstruct(const Basis &b, const double *in, double *out) {
             if (!((t >> dim) & 1)) pair_transform(b, out + t * n3, out + (t | (1 << dim)) * n3, dim, true);
 }
 
+namespace {
+// One direction of the inverse two-scale step towards child half `bit`:
+// out[i] = sum_j H_bit[j][i] lo[j] + G_bit[j][i] hi[j] along dim, where
+// H_bit, G_bit are the (bit) column halves of the filter rows [H0 H1], [G0 G1].
+// hi == nullptr drops the wavelet half.
+void child_step(const Basis &b, const double *lo, const double *hi, int dim, int bit, double *out) {
+    const int n = b.kp1, n2 = 2 * n;
+    const double *F = b.F.data();
+    const int stride = (dim == 0) ? 1 : (dim == 1 ? n : n * n);
+    double v[32], u[32];
+    for (int a = 0; a < n; a++) {
+        for (int c = 0; c < n; c++) {
+            int base;
+            if (dim == 0) base = a * n + c * n * n;
+            else if (dim == 1) base = a + c * n * n;
+            else base = a + c * n;
+            for (int i = 0; i < n; i++) {
+                v[i] = lo[base + i * stride];
+                u[i] = hi ? hi[base + i * stride] : 0.0;
+            }
+            for (int i = 0; i < n; i++) {
+                double s = 0.0;
+                for (int j = 0; j < n; j++) s += F[j * n2 + bit * n + i] * v[j];
+                if (hi)
+                    for (int j = 0; j < n; j++) s += F[(n + j) * n2 + bit * n + i] * u[j];
+                out[base + i * stride] = s;
+            }
+        }
+    }
+}
+}  // namespace
+
+void mw_child_scaling(const Basis &b, const double *in, bool s_only, int t, double *out) {
+    const int n3 = b.kp1 * b.kp1 * b.kp1;
+    if (s_only) {
+        std::vector<double> t1(n3), t2(n3);
+        child_step(b, in, nullptr, 2, (t >> 2) & 1, t1.data());
+        child_step(b, t1.data(), nullptr, 1, (t >> 1) & 1, t2.data());
+        child_step(b, t2.data(), nullptr, 0, t & 1, out);
+        return;
+    }
+    // z halves: components c and c|4 -> 4 blocks keyed by the (x, y) bits of c
+    std::vector<double> z(4 * (size_t)n3), y(2 * (size_t)n3);
+    for (int c = 0; c < 4; c++) child_step(b, in + c * n3, in + (c | 4) * n3, 2, (t >> 2) & 1, z.data() + c * n3);
+    for (int c = 0; c < 2; c++) child_step(b, z.data() + c * n3, z.data() + (c | 2) * n3, 1, (t >> 1) & 1, y.data() + c * n3);
+    child_step(b, y.data(), y.data() + n3, 0, t & 1, out);
+}
+
 void component_norms(const MRA &mra, const double *blk, double *norms8) {
     const int n3 = mra.kp1_3();
     for (int t = 0; t < 8; t++) {
