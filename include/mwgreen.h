@@ -114,6 +114,28 @@ int mwg_apply(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_t
 /* mrcpp::apply(prec, out, oper, inp, precTrees, max_iter, abs_prec). */
 int mwg_apply_prec_trees(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp,
                          const mwg_tree *prec_trees, int n_prec, int max_iter, int abs_prec);
+/* One apply of a batch: mrcpp::apply(prec, out, op, inp[, prec_trees],
+ * max_iter, abs_prec). */
+typedef struct {
+    mwg_tree out;
+    mwg_operator op;
+    mwg_tree inp;
+    double prec;
+    int max_iter;
+    int abs_prec;
+    const mwg_tree *prec_trees; /* may be NULL when n_prec == 0 */
+    int n_prec;
+} mwg_apply_desc;
+/* Many applies at once, with the result of calling mwg_apply_prec_trees on
+ * each entry: every refinement level of every apply still refining is
+ * computed in the same launches.  This is the batch MRChem runs as sequential
+ * loops: all pair densities of ExchangePotentialD1::setupInternal
+ * (src/qmoperators/two_electron/ExchangePotentialD1.cpp:158-236) and all
+ * orbitals of HelmholtzVector::apply (src/scf_solver/HelmholtzVector.cpp:141-159).
+ * Output trees must be distinct and no output may be an input; operators and
+ * inputs may be shared.  All trees and operators must share one MRA. */
+int mwg_apply_batch(mwg_context ctx, int n, const mwg_apply_desc *desc);
+
 /* Frozen-grid apply: out's current grid is kept, thr[g] (tree order) is the
  * screening threshold of node g; out receives the compressed result. */
 int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode);
@@ -130,19 +152,25 @@ int mwg_last_apply_stats(mwg_context ctx, double *stats, int n);
  * MRChem's calls; it serves repeated applies on one grid and measurement. */
 int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode,
                     mwg_plan *plan);
+/* Batched frozen-grid plan: n applies (out[i], op[i], inp[i], thresholds
+ * thr[i]) resident together and run in the same launches. */
+int mwg_plan_create_batch(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *op, const mwg_tree *inp,
+                          const double *const *thr, int mode, mwg_plan *plan);
 /* One execution of the per-node calculator on resident data, with the plan's
  * `mode` (tau from the context).  stats (n <= 5 values): [0] device ms,
  * [1] term-applications that passed screening, [2] of them on FP32 MFMA,
  * [3] device ms of the FP64 launch, [4] of the FP32 launch. */
 int mwg_plan_execute(mwg_plan plan, double *stats, int n);
-/* Copy the result into `out` and run the TopDown/BottomUp transforms. */
+/* Copy the result into the output tree(s) and run the TopDown/BottomUp
+ * transforms. */
 int mwg_plan_finish(mwg_plan plan);
 /* Raw non-standard blocks of the last execution (n_out x 8(k+1)^3, tree
- * order), before the TopDown/BottomUp transforms. */
+ * order, applies of a batch one after the other), before the TopDown/BottomUp
+ * transforms. */
 int mwg_plan_download_ns(mwg_plan plan, double *ns);
 /* [0] output nodes, [1] input slots, [2] generated slots, [3] host plan ms,
  * [4] term-applications counted on the host with the screening rule,
- * [5] work segments (workgroups per launch). */
+ * [5] work segments (workgroups of an FP64 launch), [6] applies. */
 int mwg_plan_info(mwg_plan plan, double *info, int n);
 int mwg_plan_destroy(mwg_plan plan);
 

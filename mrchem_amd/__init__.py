@@ -76,6 +76,17 @@ _sig("mwg_apply_prec_trees", _p, _d, _p, _p, _p, ctypes.POINTER(_p), _i, _i, _i)
 _sig("mwg_apply_fixed", _p, _p, _p, _p, _dp, _i)
 _sig("mwg_last_apply_stats", _p, _dp, _i)
 _sig("mwg_plan_create", _p, _p, _p, _p, _dp, _i, ctypes.POINTER(_p))
+_sig("mwg_plan_create_batch", _p, _i, ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
+     ctypes.POINTER(_dp), _i, ctypes.POINTER(_p))
+
+
+class _ApplyDesc(ctypes.Structure):
+    """mwg_apply_desc of include/mwgreen.h."""
+    _fields_ = [("out", _p), ("op", _p), ("inp", _p), ("prec", _d), ("max_iter", _i), ("abs_prec", _i),
+                ("prec_trees", ctypes.POINTER(_p)), ("n_prec", _i)]
+
+
+_sig("mwg_apply_batch", _p, _i, ctypes.POINTER(_ApplyDesc))
 _sig("mwg_plan_execute", _p, _dp, _i)
 _sig("mwg_context_set_precision", _p, _i, _d)
 _sig("mwg_plan_finish", _p)
@@ -342,18 +353,55 @@ def apply_fixed(out, oper, inp, thr, mode="fp64", ctx=None):
     return out
 
 
+def apply_batch(entries, ctx=None):
+    """Many applies in shared launches (mwg_apply_batch).  Each entry is a dict
+    with keys prec, out, oper, inp and optionally max_iter, abs_prec,
+    prec_trees, with the meaning of apply()."""
+    ctx = ctx or Context.default()
+    n = len(entries)
+    descs = (_ApplyDesc * max(n, 1))()
+    keep = []
+    for i, e in enumerate(entries):
+        pt = e.get("prec_trees") or []
+        arr = (_p * max(len(pt), 1))(*[t._h for t in pt])
+        keep.append(arr)
+        descs[i] = _ApplyDesc(e["out"]._h, e["oper"]._h, e["inp"]._h, float(e["prec"]), int(e.get("max_iter", -1)),
+                              int(bool(e.get("abs_prec", False))), ctypes.cast(arr, ctypes.POINTER(_p)), len(pt))
+    _check(lib.mwg_apply_batch(ctx._h, n, descs))
+    return [e["out"] for e in entries]
+
+
 class ApplyPlan:
-    """Frozen-grid apply resident on the device; execute() reruns the kernel."""
+    """Frozen-grid apply resident on the device; execute() reruns the kernel.
+
+    out, oper, inp and thr may be lists (one entry per apply) for a batched
+    plan whose applies run in the same launches."""
 
     def __init__(self, out, oper, inp, thr, mode="fp64", ctx=None):
         self.ctx = ctx or Context.default()
-        t = np.ascontiguousarray(thr, dtype=np.float64)
-        if t.size != out.size():
-            raise ValueError("one threshold per output node is required")
+        batched = isinstance(out, (list, tuple))
+        outs = list(out) if batched else [out]
+        n = len(outs)
+        opers = list(oper) if isinstance(oper, (list, tuple)) else [oper] * n
+        inps = list(inp) if isinstance(inp, (list, tuple)) else [inp] * n
+        thrs = list(thr) if batched else [thr]
+        if not (len(opers) == len(inps) == len(thrs) == n) or n == 0:
+            raise ValueError("one operator, input tree and threshold array per output tree")
+        ts = []
+        for o, t in zip(outs, thrs):
+            a = np.ascontiguousarray(t, dtype=np.float64)
+            if a.size != o.size():
+                raise ValueError("one threshold per output node is required")
+            ts.append(a)
         h = _p()
-        _check(lib.mwg_plan_create(self.ctx._h, out._h, oper._h, inp._h, _ptr(t, _d), MODES[mode], ctypes.byref(h)))
+        arr_o = (_p * n)(*[o._h for o in outs])
+        arr_p = (_p * n)(*[o._h for o in opers])
+        arr_i = (_p * n)(*[o._h for o in inps])
+        arr_t = (_dp * n)(*[_ptr(a, _d) for a in ts])
+        _check(lib.mwg_plan_create_batch(self.ctx._h, n, arr_o, arr_p, arr_i, arr_t, MODES[mode], ctypes.byref(h)))
         self._h = h
-        self._keep = (out, oper, inp)
+        self._outs = outs
+        self._keep = (outs, opers, inps, ts)
 
     def execute(self):
         """One execution; returns a dict: ms (device), term_apps, lp_term_apps
@@ -363,20 +411,22 @@ class ApplyPlan:
         return {"ms": v[0], "term_apps": int(v[1]), "lp_term_apps": int(v[2]), "ms_fp64": v[3], "ms_lp": v[4]}
 
     def finish(self):
+        """Result into the output tree(s); returns the tree (or the list)."""
         _check(lib.mwg_plan_finish(self._h))
-        return self._keep[0]
+        return self._outs[0] if len(self._outs) == 1 else self._outs
 
     def download_ns(self):
-        n = self._keep[0].size()
-        ns = np.zeros((n, self._keep[0].mra.block))
+        """Raw non-standard blocks, all applies one after the other."""
+        n = sum(o.size() for o in self._outs)
+        ns = np.zeros((n, self._outs[0].mra.block))
         _check(lib.mwg_plan_download_ns(self._h, _ptr(ns, _d)))
         return ns
 
     def info(self):
-        v = np.zeros(6)
-        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 6))
+        v = np.zeros(7)
+        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 7))
         return {"out_nodes": int(v[0]), "in_slots": int(v[1]), "generated": int(v[2]), "plan_ms": v[3],
-                "host_term_apps": int(v[4]), "segments": int(v[5])}
+                "host_term_apps": int(v[4]), "segments": int(v[5]), "applies": int(v[6])}
 
     def __del__(self):
         h = getattr(self, "_h", None)

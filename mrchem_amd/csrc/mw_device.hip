@@ -28,37 +28,52 @@ namespace mwg {
 
 constexpr int kMaxActive = 4096;
 
+// Per apply of a batched plan: its operator tables (device copies resident in
+// the context) and where its screening tables start.
+struct AppDesc {
+    const int *opL;                   // [m * ndepth + d]
+    const long long *op_boff, *op_noff;
+    const double *op_norms;
+    const double *blk64;              // FP64 blocks
+    const float *blk32;               // FP32 copy (mixed mode), or null
+    int M, ndepth;
+    long long bw_off;                 // start of this apply in bw
+    int act_base;                     // start of this apply in act_off
+    int pad_;
+    double thr_floor;                 // positive floor of the class threshold
+};
+
 struct KArgs {
-    // output nodes
+    // output nodes (all applies, concatenated)
     const int *out_depth;
+    const int *out_app;               // apply of each output node
     const double *out_thr;
     const long long *band_off;
     const int *band_w;
     const int *band;
     double *out_ns;
-    // input slots
+    // input slots (all applies, concatenated)
     const long long *slot_off;
-    const double *pool;
     const double *slot_norm;
-    // operator
-    const int *opL;
-    const long long *op_boff, *op_noff;
-    const double *op_blocks, *op_norms;
-    int M, ndepth;
     // screening tables
     const int *bw;
     const int *act_off, *act_terms;
+    const AppDesc *apps;
     // [0] term-applications passing screening (algorithmic work)
     unsigned long long *counters;
     // work segments: job = 8*out + gt, band positions [p0, p1)
     const int *seg_job, *seg_p0, *seg_p1;
-    double thr_floor;  // positive floor of the class threshold
 };
 
-__device__ __forceinline__ double op_norm(const KArgs &a, int m, int d, int c, int l) {
-    int L = a.opL[m * a.ndepth + d];
-    if (l < -L || l > L) return 0.0;
-    return a.op_norms[a.op_noff[m * a.ndepth + d] + (long long)c * (2 * L + 1) + (l + L)];
+template <typename T>
+__device__ __forceinline__ const T *op_blocks(const AppDesc &ad);
+template <>
+__device__ __forceinline__ const double *op_blocks<double>(const AppDesc &ad) {
+    return ad.blk64;
+}
+template <>
+__device__ __forceinline__ const float *op_blocks<float>(const AppDesc &ad) {
+    return ad.blk32;
 }
 
 // One workgroup per job (output node g, output component gt); its NW waves
@@ -133,8 +148,8 @@ constexpr int kMaxBand = 9;     // band half-width limit
 constexpr int kWorkCap = 1024;  // worklist items per batch (>= 8 * 64 + 512)
 
 template <int N, typename T>
-__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const T *opblk, const int *seg_list,
-                                                          double lo_mult, double hi_mult, int cls_counter) {
+__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const int *seg_list, double lo_mult,
+                                                          double hi_mult, int cls_counter) {
     using S = JobShape<N>;
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
@@ -156,14 +171,16 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
     const int g = job >> 3, gt = job & 7;
     const int d = a.out_depth[g];
     const double thr = a.out_thr[g];
+    const AppDesc ad = a.apps[a.out_app[g]];
+    const T *opblk = op_blocks<T>(ad);
     // class cuts relative to the threshold (floored, so that nodes computed
     // without screening still split by magnitude)
-    const double thr_cls = thr > a.thr_floor ? thr : a.thr_floor;
+    const double thr_cls = thr > ad.thr_floor ? thr : ad.thr_floor;
     const double thr_lo = thr_cls * lo_mult, thr_hi = thr_cls * hi_mult;
     const int w = a.band_w[g], side = 2 * w + 1, nl = 2 * w + 1;
     const long long boff = a.band_off[g];
-    const int a0 = a.act_off[d], nact = a.act_off[d + 1] - a0;
-    const int ndepth = a.ndepth;
+    const int a0 = a.act_off[ad.act_base + d], nact = a.act_off[ad.act_base + d + 1] - a0;
+    const int ndepth = ad.ndepth;
     const int gb[3] = {gt & 1, (gt >> 1) & 1, (gt >> 2) & 1};
 
     // this wave's tiles: t = wv*TW + tt
@@ -216,18 +233,18 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
         __syncthreads();
         for (int e = tid; e < 64; e += NTH) {
             const int m = (e < cnt) ? a.act_terms[a0 + base + e] : 0;
-            ltab[e] = (e < cnt) ? a.opL[m * ndepth + d] : -1;
-            btab[e] = (e < cnt) ? a.op_boff[m * ndepth + d] : 0;
+            ltab[e] = (e < cnt) ? ad.opL[m * ndepth + d] : -1;
+            btab[e] = (e < cnt) ? ad.op_boff[m * ndepth + d] : 0;
         }
         for (int e = tid; e < 64 * 4 * nl; e += NTH) {
             const int ti = e / (4 * nl), c = (e / nl) % 4, li = e % nl, l = li - w;
             double v = 0.0;
             if (ti < cnt) {
                 const int m = a.act_terms[a0 + base + ti];
-                const int L = a.opL[m * ndepth + d];
-                const int bwc = a.bw[((long long)d * a.M + m) * 4 + c];
+                const int L = ad.opL[m * ndepth + d];
+                const int bwc = a.bw[ad.bw_off + ((long long)d * ad.M + m) * 4 + c];
                 if (abs(l) <= bwc && abs(l) <= L)
-                    v = a.op_norms[a.op_noff[m * ndepth + d] + (long long)c * (2 * L + 1) + (l + L)];
+                    v = ad.op_norms[ad.op_noff[m * ndepth + d] + (long long)c * (2 * L + 1) + (l + L)];
             }
             nrm_s[e] = v;
         }
@@ -387,6 +404,7 @@ struct DevOper {
     double *blocks = nullptr, *norms = nullptr;
     float *blocks32 = nullptr;  // FP32 copy, made on first mixed-precision use
     long long nblk = 0;
+    int M = 0, ndepth = 0;
 };
 
 struct DeviceCtx {
@@ -398,11 +416,14 @@ struct DeviceCtx {
 
 struct DevicePlan {
     DeviceCtx *ctx = nullptr;
-    const void *op_key = nullptr;
+    std::vector<const void *> op_keys;  // per part
     int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0;
     long long npool = 0;
+    double *pool = nullptr;
     float *pool32 = nullptr;
     KArgs args{};
+    std::vector<AppDesc> apps;          // host copy of args.apps
+    AppDesc *d_apps = nullptr;
     std::vector<void *> bufs;
     size_t out_bytes = 0;
     // per segment: class cut base thr_cls and the range of passing bounds
@@ -504,6 +525,8 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
     if (!rc) rc = alloc_copy(f.blocks, &o.blocks, c->stream, err);
     if (!rc) rc = alloc_copy(f.norms, &o.norms, c->stream, err);
     o.nblk = (long long)f.blocks.size();
+    o.M = f.M;
+    o.ndepth = f.ndepth;
     if (rc) {
         free_oper(o);
         return rc;
@@ -513,102 +536,206 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
     return 0;
 }
 
+// Device buffer of n elements owned by the plan.
 template <typename T>
-static int plan_copy(DevicePlan *p, const std::vector<T> &v, const T **out, std::string &err) {
-    T *d = nullptr;
-    int rc = alloc_copy(v, &d, p->ctx->stream, err);
+static int plan_alloc(DevicePlan *p, size_t n, T **out, std::string &err) {
+    void *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(T) * std::max<size_t>(n, 1)));
     p->bufs.push_back(d);
-    if (rc) return rc;
-    *out = d;
+    *out = static_cast<T *>(d);
     return 0;
 }
 
-int dev_plan_create(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, DevicePlan **out,
-                    std::string &err) {
+template <typename T>
+static int upload_at(T *dst, const std::vector<T> &v, hipStream_t s, std::string &err) {
+    if (!v.empty()) HIP_TRY(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+    return 0;
+}
+
+int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan **out, std::string &err) {
     HIP_TRY(hipSetDevice(c->device));
-    for (size_t d = 0; d + 1 < plan.active_off.size(); d++)
-        if (plan.active_off[d + 1] - plan.active_off[d] > kMaxActive) {
-            err = "more than " + std::to_string(kMaxActive) + " active terms at one depth";
-            return -1;
-        }
-    for (int wq : plan.band_w)
-        if (wq > kMaxBand) {
-            err = "band half-width " + std::to_string(wq) + " exceeds the kernel limit " + std::to_string(kMaxBand);
-            return -1;
-        }
+    if (parts.empty()) {
+        err = "empty plan";
+        return -1;
+    }
+    const int N = parts[0].plan->kp1;
+    if (N < 2 || N > 12) {
+        err = "unsupported basis size k+1 = " + std::to_string(N);
+        return -1;
+    }
     if ((long long)(2 * kMaxBand + 1) * (2 * kMaxBand + 1) * (2 * kMaxBand + 1) >= (1 << 22)) {
         err = "band index does not fit the worklist encoding";
         return -1;
     }
-    if (plan.kp1 < 2 || plan.kp1 > 12) {
-        err = "unsupported basis size k+1 = " + std::to_string(plan.kp1);
+    // sizes and validation
+    long long n_out = 0, n_band = 0, n_slot = 0, n_pool = 0, n_bw = 0, n_actoff = 0, n_act = 0, n_seg = 0;
+    int wmax = 0;
+    for (const PlanPart &pp : parts) {
+        const ApplyPlan &pl = *pp.plan;
+        if (pl.kp1 != N) {
+            err = "all applies of a batch must share one basis size";
+            return -1;
+        }
+        for (size_t d = 0; d + 1 < pl.active_off.size(); d++)
+            if (pl.active_off[d + 1] - pl.active_off[d] > kMaxActive) {
+                err = "more than " + std::to_string(kMaxActive) + " active terms at one depth";
+                return -1;
+            }
+        for (int wq : pl.band_w) {
+            if (wq > kMaxBand) {
+                err = "band half-width " + std::to_string(wq) + " exceeds the kernel limit " + std::to_string(kMaxBand);
+                return -1;
+            }
+            wmax = std::max(wmax, wq);
+        }
+        n_out += (long long)pl.out_node.size();
+        n_band += (long long)pl.band.size();
+        n_slot += (long long)pl.slot_off.size();
+        n_pool += (long long)pl.in_coefs.size();
+        n_bw += (long long)pl.bw.size();
+        n_actoff += (long long)pl.active_off.size();
+        n_act += (long long)pl.active_terms.size();
+        n_seg += (long long)pl.seg_job.size();
+    }
+    if (n_out >= (1LL << 27) || n_seg >= (1LL << 31) || n_slot >= (1LL << 31) || n_band >= (1LL << 31)) {
+        err = "batch too large for 32-bit node, segment or slot indices";
         return -1;
     }
-    DevOper *o = nullptr;
-    int rc = get_oper(c, op_key, f, &o, err);
-    if (rc) return rc;
     auto *p = new DevicePlan;
     p->ctx = c;
-    p->op_key = op_key;
-    p->n_out = (int)plan.out_node.size();
-    p->kp1 = plan.kp1;
-    p->npool = (long long)plan.in_coefs.size();
-    for (int wq : plan.band_w) p->wmax = std::max(p->wmax, wq);
+    p->n_out = (int)n_out;
+    p->kp1 = N;
+    p->wmax = wmax;
+    p->npool = n_pool;
+    p->n_seg = (int)n_seg;
     KArgs &a = p->args;
-    rc = plan_copy(p, plan.out_depth, &a.out_depth, err);
-    if (!rc) rc = plan_copy(p, plan.out_thr, &a.out_thr, err);
-    if (!rc) rc = plan_copy(p, plan.band_off, &a.band_off, err);
-    if (!rc) rc = plan_copy(p, plan.band_w, &a.band_w, err);
-    if (!rc) rc = plan_copy(p, plan.band, &a.band, err);
-    if (!rc) rc = plan_copy(p, plan.slot_off, &a.slot_off, err);
-    if (!rc) rc = plan_copy(p, plan.in_coefs, &a.pool, err);
-    if (!rc) rc = plan_copy(p, plan.in_norms, &a.slot_norm, err);
-    if (!rc) rc = plan_copy(p, plan.bw, &a.bw, err);
-    if (!rc) rc = plan_copy(p, plan.active_off, &a.act_off, err);
-    if (!rc) rc = plan_copy(p, plan.active_terms, &a.act_terms, err);
-    if (!rc) rc = plan_copy(p, plan.seg_job, &a.seg_job, err);
-    if (!rc) rc = plan_copy(p, plan.seg_p0, &a.seg_p0, err);
-    if (!rc) rc = plan_copy(p, plan.seg_p1, &a.seg_p1, err);
+    hipStream_t s = c->stream;
+    int rc = 0;
+    int *d_out_depth, *d_out_app, *d_band_w, *d_band, *d_bw, *d_act_off, *d_act, *d_sj, *d_s0, *d_s1;
+    double *d_thr, *d_norm;
+    long long *d_band_off, *d_slot_off;
+    rc = plan_alloc(p, n_out, &d_out_depth, err);
+    if (!rc) rc = plan_alloc(p, n_out, &d_out_app, err);
+    if (!rc) rc = plan_alloc(p, n_out, &d_thr, err);
+    if (!rc) rc = plan_alloc(p, n_out, &d_band_off, err);
+    if (!rc) rc = plan_alloc(p, n_out, &d_band_w, err);
+    if (!rc) rc = plan_alloc(p, n_band, &d_band, err);
+    if (!rc) rc = plan_alloc(p, n_slot, &d_slot_off, err);
+    if (!rc) rc = plan_alloc(p, n_pool, &p->pool, err);
+    if (!rc) rc = plan_alloc(p, n_slot * 8, &d_norm, err);
+    if (!rc) rc = plan_alloc(p, n_bw, &d_bw, err);
+    if (!rc) rc = plan_alloc(p, n_actoff, &d_act_off, err);
+    if (!rc) rc = plan_alloc(p, n_act, &d_act, err);
+    if (!rc) rc = plan_alloc(p, n_seg, &d_sj, err);
+    if (!rc) rc = plan_alloc(p, n_seg, &d_s0, err);
+    if (!rc) rc = plan_alloc(p, n_seg, &d_s1, err);
+    if (!rc) rc = plan_alloc(p, parts.size(), &p->d_apps, err);
+    p->out_bytes = sizeof(double) * (size_t)std::max<long long>(n_out, 1) * 8 * N * N * N;
+    if (!rc) rc = plan_alloc(p, p->out_bytes / sizeof(double), &a.out_ns, err);
+    if (!rc) rc = plan_alloc(p, 8, &a.counters, err);
+    // host staging of the arrays that need index fix-ups (kept alive until
+    // the stream has consumed them)
+    std::vector<std::vector<int>> keep_i;
+    std::vector<std::vector<long long>> keep_l;
+    long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0, o_seg = 0;
+    p->seg_thr.reserve(n_seg);
+    p->seg_bmin.reserve(n_seg);
+    p->seg_bmax.reserve(n_seg);
+    for (size_t ai = 0; ai < parts.size() && !rc; ai++) {
+        const PlanPart &pp = parts[ai];
+        const ApplyPlan &pl = *pp.plan;
+        DevOper *o = nullptr;
+        rc = get_oper(c, pp.op_key, *pp.flat, &o, err);
+        if (rc) break;
+        p->op_keys.push_back(pp.op_key);
+        AppDesc ad{};
+        ad.opL = o->L;
+        ad.op_boff = o->boff;
+        ad.op_noff = o->noff;
+        ad.op_norms = o->norms;
+        ad.blk64 = o->blocks;
+        ad.blk32 = o->blocks32;
+        ad.M = pp.flat->M;
+        ad.ndepth = pp.flat->ndepth;
+        ad.bw_off = o_bw;
+        ad.act_base = (int)o_actoff;
+        ad.thr_floor = pl.thr_floor;
+        p->apps.push_back(ad);
+        const size_t no = pl.out_node.size();
+        // outputs
+        rc = upload_at(d_out_depth + o_out, pl.out_depth, s, err);
+        keep_i.emplace_back(no, (int)ai);
+        if (!rc) rc = upload_at(d_out_app + o_out, keep_i.back(), s, err);
+        if (!rc) rc = upload_at(d_thr + o_out, pl.out_thr, s, err);
+        keep_l.emplace_back(pl.band_off);
+        for (auto &v : keep_l.back()) v += o_band;
+        if (!rc) rc = upload_at(d_band_off + o_out, keep_l.back(), s, err);
+        if (!rc) rc = upload_at(d_band_w + o_out, pl.band_w, s, err);
+        keep_i.emplace_back(pl.band);
+        for (auto &v : keep_i.back())
+            if (v >= 0) v += (int)o_slot;
+        if (!rc) rc = upload_at(d_band + o_band, keep_i.back(), s, err);
+        // inputs
+        keep_l.emplace_back(pl.slot_off);
+        for (auto &v : keep_l.back()) v += o_pool;
+        if (!rc) rc = upload_at(d_slot_off + o_slot, keep_l.back(), s, err);
+        if (!rc) rc = upload_at(p->pool + o_pool, pl.in_coefs, s, err);
+        if (!rc) rc = upload_at(d_norm + o_slot * 8, pl.in_norms, s, err);
+        // screening tables
+        if (!rc) rc = upload_at(d_bw + o_bw, pl.bw, s, err);
+        keep_i.emplace_back(pl.active_off);
+        for (auto &v : keep_i.back()) v += (int)o_act;
+        if (!rc) rc = upload_at(d_act_off + o_actoff, keep_i.back(), s, err);
+        if (!rc) rc = upload_at(d_act + o_act, pl.active_terms, s, err);
+        // segments
+        keep_i.emplace_back(pl.seg_job);
+        for (auto &v : keep_i.back()) v += (int)(8 * o_out);
+        if (!rc) rc = upload_at(d_sj + o_seg, keep_i.back(), s, err);
+        if (!rc) rc = upload_at(d_s0 + o_seg, pl.seg_p0, s, err);
+        if (!rc) rc = upload_at(d_s1 + o_seg, pl.seg_p1, s, err);
+        for (size_t i = 0; i < pl.seg_job.size(); i++) {
+            const double t = pl.out_thr[pl.seg_job[i] >> 3];
+            p->seg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);  // as the kernel forms thr_cls
+            p->seg_bmin.push_back(pl.seg_bmin[i]);
+            p->seg_bmax.push_back(pl.seg_bmax[i]);
+        }
+        o_out += (long long)no;
+        o_band += (long long)pl.band.size();
+        o_slot += (long long)pl.slot_off.size();
+        o_pool += (long long)pl.in_coefs.size();
+        o_bw += (long long)pl.bw.size();
+        o_actoff += (long long)pl.active_off.size();
+        o_act += (long long)pl.active_terms.size();
+        o_seg += (long long)pl.seg_job.size();
+    }
+    if (!rc) rc = upload_at(p->d_apps, p->apps, s, err);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            err = std::string("upload: ") + hipGetErrorString(e);
+            rc = -2;
+        }
+    }
     if (rc) {
+        (void)hipStreamSynchronize(s);
         delete p;
         return rc;
     }
-    a.thr_floor = plan.thr_floor;
-    p->n_seg = (int)plan.seg_job.size();
-    p->seg_bmin = plan.seg_bmin;
-    p->seg_bmax = plan.seg_bmax;
-    p->seg_thr.resize(p->n_seg);
-    for (int i = 0; i < p->n_seg; i++) {
-        const double t = plan.out_thr[plan.seg_job[i] >> 3];
-        p->seg_thr[i] = t > plan.thr_floor ? t : plan.thr_floor;  // as the kernel forms thr_cls
-    }
-    const int N = plan.kp1;
-    p->out_bytes = sizeof(double) * (size_t)std::max(p->n_out, 1) * 8 * N * N * N;
-    void *d_out = nullptr, *d_cnt = nullptr;
-    hipError_t e = hipMalloc(&d_out, p->out_bytes);
-    if (e == hipSuccess) p->bufs.push_back(d_out);
-    if (e == hipSuccess) e = hipMalloc(&d_cnt, 64);
-    if (e == hipSuccess) p->bufs.push_back(d_cnt);
-    if (e != hipSuccess) {
-        err = std::string("hipMalloc: ") + hipGetErrorString(e);
-        delete p;
-        return -2;
-    }
-    a.out_ns = static_cast<double *>(d_out);
-    a.counters = static_cast<unsigned long long *>(d_cnt);
-    a.opL = o->L;
-    a.op_boff = o->boff;
-    a.op_noff = o->noff;
-    a.op_blocks = o->blocks;
-    a.op_norms = o->norms;
-    a.M = f.M;
-    a.ndepth = f.ndepth;
-    e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        err = std::string("upload: ") + hipGetErrorString(e);
-        delete p;
-        return -2;
-    }
+    a.out_depth = d_out_depth;
+    a.out_app = d_out_app;
+    a.out_thr = d_thr;
+    a.band_off = d_band_off;
+    a.band_w = d_band_w;
+    a.band = d_band;
+    a.slot_off = d_slot_off;
+    a.slot_norm = d_norm;
+    a.bw = d_bw;
+    a.act_off = d_act_off;
+    a.act_terms = d_act;
+    a.apps = p->d_apps;
+    a.seg_job = d_sj;
+    a.seg_p0 = d_s0;
+    a.seg_p1 = d_s1;
     *out = p;
     return 0;
 }
@@ -621,21 +748,21 @@ void dev_plan_destroy(DevicePlan *p) {
 }
 
 template <int N, typename T>
-static int launch_nt(const KArgs &a, const T *pool, const T *ob, const int *list, int n_seg, int wmax, double lo,
-                     double hi, int cls, hipStream_t s, std::string &err) {
+static int launch_nt(const KArgs &a, const T *pool, const int *list, int n_seg, int wmax, double lo, double hi, int cls,
+                     hipStream_t s, std::string &err) {
     if (n_seg == 0) return 0;
     const size_t shmem = sizeof(double) * 64 * 4 * (2 * wmax + 1);
-    hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, ob, list, lo, hi, cls);
+    hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, list, lo, hi, cls);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 template <typename T>
-static int launch_t(const DevicePlan *p, const T *pool, const T *ob, const int *list, int n_seg, double lo, double hi,
-                    int cls, std::string &err) {
+static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n_seg, double lo, double hi, int cls,
+                    std::string &err) {
     const KArgs &a = p->args;
     hipStream_t s = p->ctx->stream;
-#define MWG_LAUNCH(NN) launch_nt<NN, T>(a, pool, ob, list, n_seg, p->wmax, lo, hi, cls, s, err)
+#define MWG_LAUNCH(NN) launch_nt<NN, T>(a, pool, list, n_seg, p->wmax, lo, hi, cls, s, err)
     switch (p->kp1) {
         case 2: return MWG_LAUNCH(2);
         case 3: return MWG_LAUNCH(3);
@@ -679,6 +806,34 @@ static int make_lists(DevicePlan *p, double tau, std::string &err) {
     return 0;
 }
 
+// FP32 copies of the pool and of every part's operator blocks.
+static int ensure_f32(DevicePlan *p, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    bool changed = false;
+    for (size_t i = 0; i < p->op_keys.size(); i++) {
+        auto it = c->opers.find(p->op_keys[i]);
+        if (it == c->opers.end()) {
+            err = "operator tables not resident (operator destroyed while a plan uses it)";
+            return -1;
+        }
+        DevOper &o = it->second;
+        if (!o.blocks32) {
+            int rc = to_f32(o.blocks, &o.blocks32, o.nblk, c->stream, err);
+            if (rc) return rc;
+        }
+        if (p->apps[i].blk32 != o.blocks32) {
+            p->apps[i].blk32 = o.blocks32;
+            changed = true;
+        }
+    }
+    if (changed) {
+        int rc = upload_at(p->d_apps, p->apps, c->stream, err);
+        if (rc) return rc;
+    }
+    if (!p->pool32) return to_f32(p->pool, &p->pool32, p->npool, c->stream, err);
+    return 0;
+}
+
 int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
                  unsigned long long *lp_apps, std::string &err, double *split_ms) {
     DeviceCtx *c = p->ctx;
@@ -697,15 +852,7 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
         return -1;
     }
     if (mode == 1) {
-        auto it = c->opers.find(p->op_key);
-        if (it == c->opers.end()) {
-            err = "operator tables not resident";
-            return -1;
-        }
-        DevOper &o = it->second;
-        int rc = 0;
-        if (!o.blocks32) rc = to_f32(o.blocks, &o.blocks32, o.nblk, c->stream, err);
-        if (!rc && !p->pool32) rc = to_f32(p->args.pool, &p->pool32, p->npool, c->stream, err);
+        int rc = ensure_f32(p, err);
         if (!rc) rc = make_lists(p, tau, err);
         if (rc) return rc;
     }
@@ -724,12 +871,11 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     const double inf = HUGE_VAL;
     if (mode == 0) {
         // lo = 0: every term that passes screening
-        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, nullptr, p->n_seg, 0.0, inf, 0, err);
+        rc = launch_t<double>(p, p->pool, nullptr, p->n_seg, 0.0, inf, 0, err);
     } else {
-        const float *ob32 = c->opers[p->op_key].blocks32;
-        rc = launch_t<double>(p, p->args.pool, p->args.op_blocks, p->list_hi, p->n_hi, tau, inf, 0, err);
+        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, err);
         if (!rc) HIP_TRY(hipEventRecord(c->evm, c->stream));
-        if (!rc) rc = launch_t<float>(p, p->pool32, ob32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
+        if (!rc) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -763,17 +909,6 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
                            hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
-}
-
-int dev_apply_plan(DeviceCtx *c, const void *op_key, const OperFlat &f, const ApplyPlan &plan, int mode, double tau,
-                   double *out_ns, double *kernel_ms, unsigned long long *term_apps, std::string &err) {
-    DevicePlan *p = nullptr;
-    int rc = dev_plan_create(c, op_key, f, plan, &p, err);
-    if (rc) return rc;
-    rc = dev_plan_run(p, mode, tau, kernel_ms, term_apps, nullptr, err, nullptr);
-    if (!rc) rc = dev_plan_download(p, out_ns, err);
-    dev_plan_destroy(p);
-    return rc;
 }
 
 }  // namespace mwg

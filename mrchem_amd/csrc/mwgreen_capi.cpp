@@ -27,16 +27,16 @@ using namespace mwg;
 struct mwg_context_s {
     DeviceCtx *dev = nullptr;
     std::vector<double> last_stats = std::vector<double>(8, 0.0);
-    double term_apps = 0.0;  // accumulated by run_plan within one apply
     int mode = MWG_MODE_FP64;
     double tau = 65536.0;    // mixed mode: FP64 for terms with bound > tau * thr
 };
 struct mwg_plan_s {
     mwg_context ctx = nullptr;
     DevicePlan *dev = nullptr;
-    mwg_tree out = nullptr;
-    std::vector<double> thr;
-    int n_out = 0;
+    std::vector<mwg_tree> outs;           // per apply of the batch
+    std::vector<std::vector<double>> thr;
+    std::vector<int> n_out;
+    int total_out = 0;
     int mode = MWG_MODE_FP64;
     double plan_ms = 0.0, slots = 0.0, gen = 0.0, host_apps = 0.0, segs = 0.0;
 };
@@ -112,25 +112,7 @@ void check_compatible(const Tree &out, const Operator &op, const Tree &inp) {
         if (!std::isfinite(v)) throw std::invalid_argument("apply: input tree holds NaN or Inf");
 }
 
-int run_plan(mwg_context ctx, mwg_operator_s *op, const Tree &inp, Tree &out, const std::vector<int> &nodes,
-             const std::vector<double> &thr, std::vector<double> &ns, double &kms, double &plan_ms, double *slots,
-             double *gen) {
-    double t0 = now_ms();
-    ApplyPlan plan;
-    build_plan(plan, op->op, inp, out, nodes, thr, 64);
-    plan_ms += now_ms() - t0;
-    *slots += (double)plan.slot_off.size();
-    *gen += (double)plan.n_generated;
-    ns.assign(nodes.size() * (size_t)out.mra->block(), 0.0);
-    std::string err;
-    double ms = 0.0;
-    unsigned long long apps = 0;
-    int rc = dev_apply_plan(ctx->dev, op, op->flat, plan, ctx->mode, ctx->tau, ns.data(), &ms, &apps, err);
-    if (rc) return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
-    kms += ms;
-    ctx->term_apps += (double)apps;
-    return MWG_OK;
-}
+int dev_fail(int rc, const std::string &err) { return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err); }
 
 void finish_output(Tree &out) {
     tree_topdown_add(out);
@@ -138,69 +120,162 @@ void finish_output(Tree &out) {
     out.sqnorm = tree_square_norm(out);
 }
 
-int apply_adaptive(mwg_context ctx, double prec, mwg_tree_s *out, mwg_operator_s *op, mwg_tree_s *inp,
-                   const std::vector<const Tree *> &prec_trees, int max_iter, bool abs_prec) {
-    if (!ctx || !ctx->dev) return fail(MWG_ERR_ARG, "apply: null context");
-    if (!(prec >= 0.0) || !std::isfinite(prec)) return fail(MWG_ERR_ARG, "apply: precision must be finite and >= 0");
-    Tree &o = out->tree;
-    const Tree &in = inp->tree;
-    check_compatible(o, op->op, in);
-    const MRA &mra = *o.mra;
-    const int n3 = mra.kp1_3(), blk = mra.block();
-    const int M = op->op.size();
-    const int max_depth = std::min(mra.max_depth, op->op.max_depth);
-    o.clear();
-    o.make_roots();
-    ctx->term_apps = 0.0;
-    std::vector<int> work;
-    for (int i = 0; i < o.size(); i++) work.push_back(i);
-    std::vector<double> thr_all(o.size(), 0.0);
-    double s_norm = 0.0, w_norm = 0.0, kms = 0.0, pms = 0.0, slots = 0.0, gen = 0.0;
-    int iter = 0, launches = 0;
-    double computed = 0.0;
-    std::vector<double> ns;
-    while (!work.empty()) {
-        const double Nsq = s_norm + w_norm;
-        std::vector<double> thr(work.size());
-        for (size_t q = 0; q < work.size(); q++)
-            thr[q] = prec * prec_factor(prec_trees, o.keys[work[q]]) * std::sqrt(Nsq / M);
-        int rc = run_plan(ctx, op, in, o, work, thr, ns, kms, pms, &slots, &gen);
-        if (rc) return rc;
-        launches++;
-        computed += (double)work.size();
-        thr_all.resize(o.size(), 0.0);
-        std::vector<double> wn(work.size());
-        for (size_t q = 0; q < work.size(); q++) {
-            const int i = work[q];
-            std::memcpy(o.node(i), ns.data() + q * blk, sizeof(double) * blk);
-            thr_all[i] = thr[q];
-            double w = 0.0, s = 0.0;
-            const double *b = ns.data() + q * blk;
-            for (int e = 0; e < n3; e++) s += b[e] * b[e];
-            for (int e = n3; e < blk; e++) w += b[e] * b[e];
-            wn[q] = std::sqrt(w);
-            w_norm += w;
-            if (o.keys[i].depth == 0) s_norm += s;
-        }
-        const double tnorm = abs_prec ? 1.0 : std::sqrt(s_norm + w_norm);
-        std::vector<int> next;
-        if (max_iter < 0 || iter < max_iter) {
-            for (size_t q = 0; q < work.size(); q++) {
-                const int i = work[q];
-                const NodeKey k = o.keys[i];
-                if (k.depth >= max_depth) continue;
-                double pf = prec_factor(prec_trees, k);
-                if (!split_check(mra, k.depth, wn[q], prec * pf, tnorm)) continue;
-                int c0 = o.split(i);
-                for (int c = 0; c < 8; c++) next.push_back(c0 + c);
-            }
-        }
-        work.swap(next);
-        iter++;
+// One apply of a batch and its refinement state.
+struct Job {
+    mwg_tree_s *out = nullptr;
+    mwg_operator_s *op = nullptr;
+    const Tree *inp = nullptr;
+    std::vector<const Tree *> prec_trees;
+    double prec = 0.0;
+    int max_iter = -1;
+    bool abs_prec = false;
+    std::vector<int> work;        // output nodes of the current level
+    std::vector<double> thr_all;  // threshold per output node
+    double s_norm = 0.0, w_norm = 0.0;
+    int iter = 0;
+    ApplyPlan plan;
+    std::string err;
+};
+
+void validate_jobs(const std::vector<Job> &jobs) {
+    std::set<const void *> outs, ins;
+    for (const Job &j : jobs) ins.insert(j.inp);
+    for (const Job &j : jobs) {
+        if (!(j.prec >= 0.0) || !std::isfinite(j.prec))
+            throw std::invalid_argument("apply: precision must be finite and >= 0");
+        check_compatible(j.out->tree, j.op->op, *j.inp);
+        if (j.out->tree.mra != jobs[0].out->tree.mra)
+            throw std::invalid_argument("apply: all applies of a batch must share one MRA");
+        if (!outs.insert(&j.out->tree).second) throw std::invalid_argument("apply: an output tree appears twice");
+        if (ins.count(&j.out->tree)) throw std::invalid_argument("apply: a tree is both output and input");
+        for (const Tree *t : j.prec_trees)
+            if (t->mra != j.op->op.mra) throw std::invalid_argument("apply: prec tree on another MRA");
     }
-    thr_all.resize(o.size(), 0.0);
-    finish_output(o);
-    out->apply_thr = thr_all;
+}
+
+// Build the host plans of `active`, in parallel across applies when there are
+// several (build_plan parallelises internally otherwise).
+void build_plans(std::vector<Job *> &active, const std::vector<std::vector<int>> &nodes,
+                 const std::vector<std::vector<double>> &thr) {
+    const int n = (int)active.size();
+#pragma omp parallel for schedule(dynamic, 1) if (n > 1)
+    for (int i = 0; i < n; i++) {
+        Job &j = *active[i];
+        try {
+            build_plan(j.plan, j.op->op, *j.inp, j.out->tree, nodes[i], thr[i], 64);
+        } catch (const std::exception &e) {
+            j.err = e.what();
+        }
+    }
+    for (Job *j : active)
+        if (!j->err.empty()) throw std::runtime_error(j->err);
+}
+
+// Adaptive applies with mrcpp::apply semantics per job; every refinement
+// level of every job still refining is computed in one shared device plan.
+int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
+    if (!ctx || !ctx->dev) return fail(MWG_ERR_ARG, "apply: null context");
+    if (jobs.empty()) return MWG_OK;
+    validate_jobs(jobs);
+    const MRA &mra = *jobs[0].out->tree.mra;
+    const int n3 = mra.kp1_3(), blk = mra.block();
+    for (Job &j : jobs) {
+        Tree &o = j.out->tree;
+        o.clear();
+        o.make_roots();
+        j.work.clear();
+        for (int i = 0; i < o.size(); i++) j.work.push_back(i);
+        j.thr_all.assign(o.size(), 0.0);
+    }
+    double kms = 0.0, pms = 0.0, slots = 0.0, gen = 0.0, computed = 0.0, apps_total = 0.0;
+    int launches = 0;
+    std::vector<double> ns;
+    while (true) {
+        std::vector<Job *> active;
+        for (Job &j : jobs)
+            if (!j.work.empty()) active.push_back(&j);
+        if (active.empty()) break;
+        std::vector<std::vector<int>> nodes(active.size());
+        std::vector<std::vector<double>> thr(active.size());
+        for (size_t a = 0; a < active.size(); a++) {
+            Job &j = *active[a];
+            const int M = j.op->op.size();
+            const double Nsq = j.s_norm + j.w_norm;
+            nodes[a] = j.work;
+            thr[a].resize(j.work.size());
+            for (size_t q = 0; q < j.work.size(); q++)
+                thr[a][q] = j.prec * prec_factor(j.prec_trees, j.out->tree.keys[j.work[q]]) * std::sqrt(Nsq / M);
+        }
+        const double t0 = now_ms();
+        build_plans(active, nodes, thr);
+        pms += now_ms() - t0;
+        std::vector<PlanPart> parts;
+        size_t total = 0;
+        for (Job *j : active) {
+            parts.push_back(PlanPart{j->op, &j->op->flat, &j->plan});
+            slots += (double)j->plan.slot_off.size();
+            gen += (double)j->plan.n_generated;
+            total += j->work.size();
+        }
+        std::string err;
+        DevicePlan *dp = nullptr;
+        int rc = dev_plan_create(ctx->dev, parts, &dp, err);
+        if (rc) return dev_fail(rc, err);
+        double ms = 0.0;
+        unsigned long long apps = 0;
+        ns.resize(total * (size_t)blk);
+        rc = dev_plan_run(dp, ctx->mode, ctx->tau, &ms, &apps, nullptr, err, nullptr);
+        if (!rc) rc = dev_plan_download(dp, ns.data(), err);
+        dev_plan_destroy(dp);
+        if (rc) return dev_fail(rc, err);
+        kms += ms;
+        apps_total += (double)apps;
+        launches++;
+        computed += (double)total;
+        size_t off = 0;
+        for (size_t a = 0; a < active.size(); a++) {
+            Job &j = *active[a];
+            j.plan = ApplyPlan();
+            Tree &o = j.out->tree;
+            const int max_depth = std::min(mra.max_depth, j.op->op.max_depth);
+            std::vector<double> wn(j.work.size());
+            for (size_t q = 0; q < j.work.size(); q++) {
+                const int i = j.work[q];
+                const double *b = ns.data() + (off + q) * blk;
+                std::memcpy(o.node(i), b, sizeof(double) * blk);
+                j.thr_all[i] = thr[a][q];
+                double w = 0.0, sq = 0.0;
+                for (int e = 0; e < n3; e++) sq += b[e] * b[e];
+                for (int e = n3; e < blk; e++) w += b[e] * b[e];
+                wn[q] = std::sqrt(w);
+                j.w_norm += w;
+                if (o.keys[i].depth == 0) j.s_norm += sq;
+            }
+            off += j.work.size();
+            const double tnorm = j.abs_prec ? 1.0 : std::sqrt(j.s_norm + j.w_norm);
+            std::vector<int> next;
+            if (j.max_iter < 0 || j.iter < j.max_iter) {
+                for (size_t q = 0; q < j.work.size(); q++) {
+                    const int i = j.work[q];
+                    const NodeKey k = o.keys[i];
+                    if (k.depth >= max_depth) continue;
+                    const double pf = prec_factor(j.prec_trees, k);
+                    if (!split_check(mra, k.depth, wn[q], j.prec * pf, tnorm)) continue;
+                    const int c0 = o.split(i);
+                    for (int c = 0; c < 8; c++) next.push_back(c0 + c);
+                }
+            }
+            j.work.swap(next);
+            j.thr_all.resize(o.size(), 0.0);
+            j.iter++;
+        }
+    }
+    const int nj = (int)jobs.size();
+#pragma omp parallel for schedule(dynamic, 1) if (nj > 1)
+    for (int i = 0; i < nj; i++) {
+        finish_output(jobs[i].out->tree);
+        jobs[i].out->apply_thr = jobs[i].thr_all;
+    }
     auto &st = ctx->last_stats;
     st.assign(8, 0.0);
     st[0] = kms;
@@ -209,7 +284,86 @@ int apply_adaptive(mwg_context ctx, double prec, mwg_tree_s *out, mwg_operator_s
     st[3] = gen;
     st[4] = pms;
     st[5] = launches;
-    st[6] = ctx->term_apps;
+    st[6] = apps_total;
+    return MWG_OK;
+}
+
+// Frozen-grid device plan over the given applies (thr[i] per output node of
+// out[i], tree order).
+int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *op, const mwg_tree *inp,
+              const double *const *thr, int mode, mwg_plan *plan) {
+    if (!ctx || !ctx->dev || n < 1 || !out || !op || !inp || !thr || !plan) return fail(MWG_ERR_ARG, "null argument");
+    if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+    std::vector<Job> jobs(n);
+    for (int i = 0; i < n; i++) {
+        if (!out[i] || !op[i] || !inp[i] || !thr[i]) return fail(MWG_ERR_ARG, "null handle in batch");
+        jobs[i].out = out[i];
+        jobs[i].op = op[i];
+        jobs[i].inp = &inp[i]->tree;
+    }
+    validate_jobs(jobs);
+    auto p = std::make_unique<mwg_plan_s>();
+    std::vector<Job *> active;
+    std::vector<std::vector<int>> nodes(n);
+    std::vector<std::vector<double>> th(n);
+    for (int i = 0; i < n; i++) {
+        const int no = out[i]->tree.size();
+        nodes[i].resize(no);
+        th[i].assign(thr[i], thr[i] + no);
+        for (int g = 0; g < no; g++) {
+            nodes[i][g] = g;
+            if (!(th[i][g] >= 0.0) || !std::isfinite(th[i][g]))
+                return fail(MWG_ERR_ARG, "thresholds must be finite and >= 0");
+        }
+        active.push_back(&jobs[i]);
+    }
+    const double t0 = now_ms();
+    build_plans(active, nodes, th);
+    p->plan_ms = now_ms() - t0;
+    std::vector<PlanPart> parts;
+    for (int i = 0; i < n; i++) {
+        const ApplyPlan &ap = jobs[i].plan;
+        parts.push_back(PlanPart{op[i], &op[i]->flat, &ap});
+        p->slots += (double)ap.slot_off.size();
+        p->gen += (double)ap.n_generated;
+        p->host_apps += (double)ap.term_apps;
+        p->segs += (double)ap.seg_job.size();
+        p->outs.push_back(out[i]);
+        p->n_out.push_back((int)nodes[i].size());
+        p->total_out += (int)nodes[i].size();
+    }
+    p->thr = std::move(th);
+    std::string err;
+    int rc = dev_plan_create(ctx->dev, parts, &p->dev, err);
+    if (rc) return dev_fail(rc, err);
+    p->ctx = ctx;
+    p->mode = mode;
+    *plan = p.release();
+    return MWG_OK;
+}
+
+// Download a plan's result into its output trees and finish them.
+int plan_finish(mwg_plan plan) {
+    for (size_t i = 0; i < plan->outs.size(); i++)
+        if (plan->outs[i]->tree.size() != plan->n_out[i])
+            return fail(MWG_ERR_ARG, "output tree changed since the plan was made");
+    const int blk = plan->outs[0]->tree.mra->block();
+    std::vector<double> ns((size_t)plan->total_out * blk);
+    std::string err;
+    int rc = dev_plan_download(plan->dev, ns.data(), err);
+    if (rc) return dev_fail(rc, err);
+    size_t off = 0;
+    for (size_t i = 0; i < plan->outs.size(); i++) {
+        Tree &o = plan->outs[i]->tree;
+        std::memcpy(o.coefs.data(), ns.data() + off, sizeof(double) * (size_t)plan->n_out[i] * blk);
+        off += (size_t)plan->n_out[i] * blk;
+    }
+    const int nj = (int)plan->outs.size();
+#pragma omp parallel for schedule(dynamic, 1) if (nj > 1)
+    for (int i = 0; i < nj; i++) {
+        finish_output(plan->outs[i]->tree);
+        plan->outs[i]->apply_thr = plan->thr[i];
+    }
     return MWG_OK;
 }
 
@@ -510,63 +664,70 @@ int mwg_tree_thresholds(mwg_tree t, double *thr) {
 }
 
 int mwg_apply(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp, int max_iter, int abs_prec) {
-    return guarded([&] {
-        if (!out || !op || !inp) return fail(MWG_ERR_ARG, "null handle");
-        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
-        return apply_adaptive(ctx, prec, out, op, inp, {}, max_iter, abs_prec != 0);
-    });
+    return mwg_apply_prec_trees(ctx, prec, out, op, inp, nullptr, 0, max_iter, abs_prec);
 }
 
 int mwg_apply_prec_trees(mwg_context ctx, double prec, mwg_tree out, mwg_operator op, mwg_tree inp,
                          const mwg_tree *prec_trees, int n_prec, int max_iter, int abs_prec) {
+    mwg_apply_desc d;
+    d.out = out;
+    d.op = op;
+    d.inp = inp;
+    d.prec = prec;
+    d.max_iter = max_iter;
+    d.abs_prec = abs_prec;
+    d.prec_trees = prec_trees;
+    d.n_prec = n_prec;
+    return mwg_apply_batch(ctx, 1, &d);
+}
+
+int mwg_apply_batch(mwg_context ctx, int n, const mwg_apply_desc *desc) {
     return guarded([&] {
-        if (!out || !op || !inp) return fail(MWG_ERR_ARG, "null handle");
-        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
-        if (n_prec < 0 || (n_prec > 0 && !prec_trees)) return fail(MWG_ERR_ARG, "bad prec tree list");
-        std::vector<const Tree *> pt;
-        for (int i = 0; i < n_prec; i++) {
-            if (!prec_trees[i]) return fail(MWG_ERR_ARG, "null prec tree");
-            if (prec_trees[i]->tree.mra != op->op.mra) return fail(MWG_ERR_ARG, "prec tree on another MRA");
-            pt.push_back(&prec_trees[i]->tree);
+        if (!ctx || n < 0 || (n > 0 && !desc)) return fail(MWG_ERR_ARG, "bad arguments");
+        std::vector<Job> jobs(n);
+        for (int i = 0; i < n; i++) {
+            const mwg_apply_desc &d = desc[i];
+            if (!d.out || !d.op || !d.inp) return fail(MWG_ERR_ARG, "null handle");
+            if (d.out == d.inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
+            if (d.n_prec < 0 || (d.n_prec > 0 && !d.prec_trees)) return fail(MWG_ERR_ARG, "bad prec tree list");
+            Job &j = jobs[i];
+            j.out = d.out;
+            j.op = d.op;
+            j.inp = &d.inp->tree;
+            j.prec = d.prec;
+            j.max_iter = d.max_iter;
+            j.abs_prec = d.abs_prec != 0;
+            for (int t = 0; t < d.n_prec; t++) {
+                if (!d.prec_trees[t]) return fail(MWG_ERR_ARG, "null prec tree");
+                j.prec_trees.push_back(&d.prec_trees[t]->tree);
+            }
         }
-        return apply_adaptive(ctx, prec, out, op, inp, pt, max_iter, abs_prec != 0);
+        return apply_batch(ctx, jobs);
     });
 }
 
 int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode) {
     return guarded([&] {
-        if (!ctx || !out || !op || !inp || !thr) return fail(MWG_ERR_ARG, "null argument");
-        if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
-        if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
-        Tree &o = out->tree;
-        check_compatible(o, op->op, inp->tree);
-        const int n = o.size(), blk = o.mra->block();
-        std::vector<int> nodes(n);
-        std::vector<double> th(thr, thr + n);
-        for (int i = 0; i < n; i++) {
-            nodes[i] = i;
-            if (!(th[i] >= 0.0) || !std::isfinite(th[i])) return fail(MWG_ERR_ARG, "thresholds must be finite and >= 0");
-        }
-        double kms = 0, pms = 0, slots = 0, gen = 0;
-        std::vector<double> ns;
-        ctx->term_apps = 0.0;
-        const int saved = ctx->mode;
-        ctx->mode = mode;
-        int rc = run_plan(ctx, op, inp->tree, o, nodes, th, ns, kms, pms, &slots, &gen);
-        ctx->mode = saved;
+        mwg_plan plan = nullptr;
+        int rc = make_plan(ctx, 1, &out, &op, &inp, &thr, mode, &plan);
         if (rc) return rc;
-        std::memcpy(o.coefs.data(), ns.data(), sizeof(double) * (size_t)n * blk);
-        finish_output(o);
-        out->apply_thr = th;
+        std::unique_ptr<mwg_plan_s, int (*)(mwg_plan)> hold(plan, mwg_plan_destroy);
+        std::string err;
+        double ms = 0.0;
+        unsigned long long apps = 0;
+        rc = dev_plan_run(plan->dev, mode, ctx->tau, &ms, &apps, nullptr, err, nullptr);
+        if (rc) return dev_fail(rc, err);
+        rc = plan_finish(plan);
+        if (rc) return rc;
         auto &st = ctx->last_stats;
         st.assign(8, 0.0);
-        st[0] = kms;
-        st[1] = n;
-        st[2] = slots;
-        st[3] = gen;
-        st[4] = pms;
+        st[0] = ms;
+        st[1] = plan->total_out;
+        st[2] = plan->slots;
+        st[3] = plan->gen;
+        st[4] = plan->plan_ms;
         st[5] = 1;
-        st[6] = ctx->term_apps;
+        st[6] = (double)apps;
         return MWG_OK;
     });
 }
@@ -574,41 +735,14 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
 int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode,
                     mwg_plan *plan) {
     return guarded([&] {
-        if (!ctx || !ctx->dev || !out || !op || !inp || !thr || !plan) return fail(MWG_ERR_ARG, "null argument");
         if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
-        if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
-        Tree &o = out->tree;
-        check_compatible(o, op->op, inp->tree);
-        const int n = o.size();
-        std::vector<int> nodes(n);
-        std::vector<double> th(thr, thr + n);
-        for (int i = 0; i < n; i++) {
-            nodes[i] = i;
-            if (!(th[i] >= 0.0) || !std::isfinite(th[i])) return fail(MWG_ERR_ARG, "thresholds must be finite and >= 0");
-        }
-        double t0 = now_ms();
-        ApplyPlan ap;
-        build_plan(ap, op->op, inp->tree, o, nodes, th, 64);
-        auto *p = new mwg_plan_s;
-        p->plan_ms = now_ms() - t0;
-        p->slots = (double)ap.slot_off.size();
-        p->gen = (double)ap.n_generated;
-        p->host_apps = (double)ap.term_apps;
-        p->segs = (double)ap.seg_job.size();
-        std::string err;
-        int rc = dev_plan_create(ctx->dev, op, op->flat, ap, &p->dev, err);
-        if (rc) {
-            delete p;
-            return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
-        }
-        p->ctx = ctx;
-        p->out = out;
-        p->thr = th;
-        p->n_out = n;
-        p->mode = mode;
-        *plan = p;
-        return MWG_OK;
+        return make_plan(ctx, 1, &out, &op, &inp, &thr, mode, plan);
     });
+}
+
+int mwg_plan_create_batch(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *op, const mwg_tree *inp,
+                          const double *const *thr, int mode, mwg_plan *plan) {
+    return guarded([&] { return make_plan(ctx, n, out, op, inp, thr, mode, plan); });
 }
 
 int mwg_plan_execute(mwg_plan plan, double *stats, int n) {
@@ -618,7 +752,7 @@ int mwg_plan_execute(mwg_plan plan, double *stats, int n) {
         double ms = 0.0, split[2] = {0.0, 0.0};
         unsigned long long apps = 0, lp = 0;
         int rc = dev_plan_run(plan->dev, plan->mode, plan->ctx->tau, &ms, &apps, &lp, err, split);
-        if (rc) return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err);
+        if (rc) return dev_fail(rc, err);
         const double v[5] = {ms, (double)apps, (double)lp, split[0], split[1]};
         for (int i = 0; stats && i < n && i < 5; i++) stats[i] = v[i];
         return MWG_OK;
@@ -628,14 +762,7 @@ int mwg_plan_execute(mwg_plan plan, double *stats, int n) {
 int mwg_plan_finish(mwg_plan plan) {
     return guarded([&] {
         if (!plan) return fail(MWG_ERR_ARG, "null plan");
-        Tree &o = plan->out->tree;
-        if (o.size() != plan->n_out) return fail(MWG_ERR_ARG, "output tree changed since the plan was made");
-        std::string err;
-        int rc = dev_plan_download(plan->dev, o.coefs.data(), err);
-        if (rc) return fail(MWG_ERR_DEVICE, err);
-        finish_output(o);
-        plan->out->apply_thr = plan->thr;
-        return MWG_OK;
+        return plan_finish(plan);
     });
 }
 
@@ -644,15 +771,16 @@ int mwg_plan_download_ns(mwg_plan plan, double *ns) {
         if (!plan || !ns) return fail(MWG_ERR_ARG, "null pointer argument");
         std::string err;
         int rc = dev_plan_download(plan->dev, ns, err);
-        if (rc) return fail(MWG_ERR_DEVICE, err);
+        if (rc) return dev_fail(rc, err);
         return MWG_OK;
     });
 }
 
 int mwg_plan_info(mwg_plan plan, double *info, int n) {
     if (!plan || !info) return fail(MWG_ERR_ARG, "null pointer argument");
-    double v[6] = {(double)plan->n_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs};
-    for (int i = 0; i < n && i < 6; i++) info[i] = v[i];
+    double v[7] = {(double)plan->total_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs,
+                   (double)plan->outs.size()};
+    for (int i = 0; i < n && i < 7; i++) info[i] = v[i];
     return MWG_OK;
 }
 
