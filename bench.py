@@ -1,24 +1,26 @@
 """Benchmark: output nodes/s of the multiwavelet Helmholtz/Poisson apply.
 
-Workload (default): the bound-state Helmholtz apply, mu = 1.2, on a 10-centre
-Gaussian sum (exponents log-uniform in [0.5, 50], coefficients N(0,1), centres
-uniform in [-3,3]^3), basis order k = 9, prec 1e-6, absolute precision as in
-HelmholtzVector (src/scf_solver/HelmholtzVector.cpp:149).  The output grid
-and per-node screening thresholds come from one adaptive apply and are then
-frozen; a step is one apply over that frozen grid with the input tree, band
-tables and operator tables already resident in HBM.
+Default workload (c2): the bound-state Helmholtz apply, mu = 1.2, on a
+10-centre Gaussian sum (exponents log-uniform in [0.5, 50], coefficients
+N(0,1), centres uniform in [-3,3]^3), basis order k = 9, prec 1e-6, absolute
+precision as in HelmholtzVector (src/scf_solver/HelmholtzVector.cpp:149).
+--config c3 / c5 run the batched workloads: 136 exchange pair densities
+(Poisson, k = 7) and 64 orbitals each with its own Helmholtz mu_i (k = 11).
+
+The output grids and per-node screening thresholds come from one adaptive
+(batched) apply and are then frozen; a step is one apply of every entry over
+those grids with the input pools, band tables and operator tables already
+resident in HBM.
 
 Prints one JSON line.  Roofline: algorithmic flops = term-applications that
 passed screening x 6(k+1)^4 (counted by the kernel itself, equal to the
-oracle's count), divided by the kernel's mean device time (HIP events on the
-launch stream), against the MI355X FP64 peak.  cpu_baseline: the FP64 OpenMP
-oracle (a restatement of the reference algorithm; MRCPP itself is not
-available) timed on a bounded sample of the same output nodes on the host's
-cores, in the same process.
+oracle's count), divided by the device time of the launch (HIP events on the
+launch stream).  cpu_baseline: the FP64 OpenMP oracle (a restatement of the
+reference algorithm; MRCPP itself is not available) timed on a bounded sample
+of the same output nodes on the host's cores, in the same process.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -37,23 +39,33 @@ def log(*a):
 
 
 def build_workload(cfg_name, max_depth=16):
+    """Operators and projected input trees of one configuration (one entry
+    per apply)."""
     import mrchem_amd as mw
     from mrchem_amd import workloads
     cfg = workloads.config(cfg_name)
     mra = mw.MultiResolutionAnalysis(order=cfg["order"], basis="interpolating", max_depth=max_depth,
                                      **workloads.WORLD)
     prec = cfg["prec"]
+    dens = cfg["densities"]
     t0 = time.time()
     if cfg["kind"] == "helmholtz":
-        op = mw.HelmholtzOperator(mra, cfg["mu"][0], prec)
+        mus = cfg["mu"] if len(cfg["mu"]) == len(dens) else [cfg["mu"][0]] * len(dens)
+        cache = {}
+        ops = []
+        for m in mus:
+            if m not in cache:
+                cache[m] = mw.HelmholtzOperator(mra, m, prec)
+            ops.append(cache[m])
     else:
-        op = mw.PoissonOperator(mra, prec)
+        mus = [0.0] * len(dens)
+        P = mw.PoissonOperator(mra, prec)
+        ops = [P] * len(dens)
     t_op = time.time() - t0
-    c, a, R = cfg["densities"][0]
     t0 = time.time()
-    rho = mw.FunctionTree(mra).project_gaussians(prec, c, a, R)
+    inps = [mw.FunctionTree(mra).project_gaussians(prec, c, a, R) for (c, a, R) in dens]
     t_proj = time.time() - t0
-    return cfg, mra, op, rho, {"operator_build_s": t_op, "projection_s": t_proj}
+    return cfg, mra, ops, inps, mus, {"operator_build_s": t_op, "projection_s": t_proj}
 
 
 def main():
@@ -71,30 +83,35 @@ def main():
         sys.exit(2)
 
     import mrchem_amd as mw
-    cfg, mra, op, rho, setup = build_workload(args.config)
+    cfg, mra, ops, inps, mus, setup = build_workload(args.config)
     k = cfg["order"]
-    log("workload %s: operator %d terms (%.2fs), input %d nodes (%.2fs)" %
-        (cfg["name"], op.size(), setup["operator_build_s"], rho.size(), setup["projection_s"]))
+    n_app = len(inps)
+    log("workload %s: %d applies, operator(s) %d terms (%.2fs), inputs %d nodes (%.2fs)" %
+        (cfg["name"], n_app, ops[0].size(), setup["operator_build_s"], sum(f.size() for f in inps),
+         setup["projection_s"]))
     ctx = mw.Context.default()
-    V = mw.FunctionTree(mra)
+    outs = [mw.FunctionTree(mra) for _ in inps]
+    prec = cfg["prec"]
+    entries = [dict(prec=prec if cfg["abs_prec"] else prec / f.norm(), out=o, oper=op, inp=f,
+                    abs_prec=cfg["abs_prec"]) for o, op, f in zip(outs, ops, inps)]
     t0 = time.time()
-    mw.apply(cfg["prec"], V, op, rho, abs_prec=cfg["abs_prec"], ctx=ctx)
+    mw.apply_batch(entries, ctx=ctx)
     t_adapt = time.time() - t0
     ast = ctx.last_stats()
+    n_out = sum(o.size() for o in outs)
     log("adaptive apply: %d output nodes in %.2fs (kernel %.1f ms, plan %.1f ms, %d launches)" %
-        (V.size(), t_adapt, ast["kernel_ms"], ast["plan_ms"], ast["launches"]))
-    thr = V.thresholds()
-    od, ol, _ = V.nodes()
-    # two plans on the same frozen grid and thresholds: plain FP64 and mixed
-    V64 = V
-    Vmx = mw.FunctionTree(mra).set_nodes(od, ol)
+        (n_out, t_adapt, ast["kernel_ms"], ast["plan_ms"], ast["launches"]))
+    thr = [o.thresholds() for o in outs]
+    grids = [o.nodes()[:2] for o in outs]
+    # two plans on the same frozen grids and thresholds: plain FP64 and mixed
+    outs_mx = [mw.FunctionTree(mra).set_nodes(*g) for g in grids]
     t0 = time.time()
-    plan64 = mw.ApplyPlan(V64, op, rho, thr, mode="fp64", ctx=ctx)
-    planmx = mw.ApplyPlan(Vmx, op, rho, thr, mode="mixed", ctx=ctx)
+    plan64 = mw.ApplyPlan(outs, ops, inps, thr, mode="fp64", ctx=ctx)
+    planmx = mw.ApplyPlan(outs_mx, ops, inps, thr, mode="mixed", ctx=ctx)
     t_plan = time.time() - t0
     pinfo = plan64.info()
-    log("frozen plans: %d out nodes, %d input slots (%d generated), %.2fs" %
-        (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], t_plan))
+    log("frozen plans: %d out nodes, %d input slots (%d generated), %d segments, %.2fs" %
+        (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], pinfo["segments"], t_plan))
 
     def timed(plan):
         for _ in range(args.warmup):
@@ -108,9 +125,9 @@ def main():
 
     runs64, ms64 = timed(plan64)
     runsmx, msmx = timed(planmx)
-    n_out = pinfo["out_nodes"]
     apps = runs64[-1]["term_apps"]
     assert runsmx[-1]["term_apps"] == apps, "mixed and FP64 paths applied different term sets"
+    assert apps == pinfo["host_term_apps"], "kernel and host screening counts differ"
     lp_apps = runsmx[-1]["lp_term_apps"]
     fl = 6.0 * (k + 1) ** 4
     k64 = float(np.mean([r["ms"] for r in runs64]))
@@ -123,31 +140,34 @@ def main():
     log("mixed: wall %.3f ms/step, kernel %.3f ms (fp64 part %.3f ms, fp32 part %.3f ms), %d of %d term-apps "
         "on FP32 (%.2f TFLOP/s)" % (msmx, kmx, kmx_64, kmx_lp, lp_apps, apps, ach_lp))
 
-    # error of the mixed path against the FP64 path on the identical grid
+    # error of the mixed path against the FP64 path on the identical grids
     ns64 = plan64.download_ns()
     nsmx = planmx.download_ns()
     coef_err = float(np.max(np.abs(nsmx - ns64)) / np.max(np.abs(ns64)))
     plan64.finish()
     planmx.finish()
-    c64 = V64.nodes()[2]
-    cmx = Vmx.nodes()[2]
+    c64 = np.concatenate([o.nodes()[2] for o in outs])
+    cmx = np.concatenate([o.nodes()[2] for o in outs_mx])
     tree_err = float(np.linalg.norm(cmx - c64) / np.linalg.norm(c64))
     log("mixed vs fp64: max coefficient error %.3e (relative to max), tree-norm error %.3e" % (coef_err, tree_err))
 
-    # result check: energy against the closed form
+    # result check: energies against the closed forms
     from oracle import closed_forms as cf
-    c, a, R = cfg["densities"][0]
-    E = rho.dot(V64)
-    Emx = rho.dot(Vmx)
-    mu = cfg["mu"][0] if cfg["kind"] == "helmholtz" else 0.0
-    E_ex = cf.pair_energy(c, a, R, mu=mu)
-    check = {"energy_fp64": E, "energy_mixed": Emx, "energy_closed_form": E_ex, "abs_err_fp64": abs(E - E_ex),
-             "abs_err_mixed": abs(Emx - E_ex), "mixed_vs_fp64_max_coef_rel": coef_err,
-             "mixed_vs_fp64_tree_rel": tree_err}
+    e_err64, e_errmx, e_max = 0.0, 0.0, 0.0
+    for f, o64, omx, d, mu in zip(inps, outs, outs_mx, cfg["densities"], mus):
+        E_ex = cf.pair_energy(*d, mu=mu)
+        e_err64 = max(e_err64, abs(f.dot(o64) - E_ex))
+        e_errmx = max(e_errmx, abs(f.dot(omx) - E_ex))
+        e_max = max(e_max, abs(E_ex))
+    check = {"energy_max_abs_err_fp64": e_err64, "energy_max_abs_err_mixed": e_errmx, "energy_max_abs": e_max,
+             "mixed_vs_fp64_max_coef_rel": coef_err, "mixed_vs_fp64_tree_rel": tree_err}
+    if n_app == 1:
+        check["energy_closed_form"] = cf.pair_energy(*cfg["densities"][0], mu=mus[0])
+    log("energies vs closed form: max abs err fp64 %.3e, mixed %.3e (max |E| %.3e)" % (e_err64, e_errmx, e_max))
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mra, op, rho, V64, thr, ns64, args.cpu_seconds, check)
+        cpu = cpu_baseline(mra, ops, inps, outs, thr, ns64, args.cpu_seconds, check)
 
     line = {
         "metric": METRIC,
@@ -160,10 +180,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64 (mixed: FP64 for bound > %g thr)" % ctx_tau(ctx),
+        "dtype": "f32+f64 (mixed: FP64 for bound > %g thr)" % ctx.tau,
         "data": "synthetic (SplitMix64-seeded Gaussian sums)",
-        "config": {"workload": cfg["name"], "order": k, "prec": cfg["prec"], "out_nodes": n_out,
-                   "in_slots": pinfo["in_slots"], "terms": op.size(), "path": "mixed FP32/FP64 MFMA"},
+        "config": {"workload": cfg["name"], "order": k, "prec": prec, "applies": n_app, "out_nodes": n_out,
+                   "in_slots": pinfo["in_slots"], "terms": ops[0].size(), "path": "mixed FP32/FP64 MFMA"},
         "roofline": {"bound": "mfma", "achieved": ach_lp, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
                      "frac": ach_lp / PEAK_TFLOPS["f32"], "traffic": None,
                      "kernel": "k_apply<%d, float>" % (k + 1)},
@@ -181,37 +201,59 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def ctx_tau(ctx):
-    return 65536.0
-
-
-def cpu_baseline(mra, op, rho, V, thr, ns_gpu, target_s, check):
-    """FP64 OpenMP oracle on a bounded sample of the frozen output grid."""
+def cpu_baseline(mra, ops, inps, outs, thr, ns_gpu, target_s, check):
+    """FP64 OpenMP oracle on a bounded sample of the frozen output grids:
+    whole applies for a batch (in a seeded random order), random nodes of the
+    one grid for a single apply."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
-    od, ol, _ = V.nodes()
-    n = len(od)
-    tables = op.tables()
-    inp = rho.nodes()
     rng = np.random.default_rng(12345)
-    order = rng.permutation(n)
-    m = min(n, 32)
-    while True:
-        idx = np.sort(order[:m])
-        t0 = time.perf_counter()
-        ref, st = oracle.apply_fixed(mra, tables, inp, od[idx], ol[idx], thr[idx], post=False)
-        dt = time.perf_counter() - t0
-        if m >= n or dt >= 0.5 * target_s:
-            break
-        m = min(n, max(m * 2, int(m * target_s / max(dt, 1e-3))))
-    err = float(np.max(np.abs(ref - ns_gpu[idx])) / max(np.max(np.abs(ref)), 1e-300))
+    offs = np.cumsum([0] + [o.size() for o in outs])
+    n_tot = int(offs[-1])
+    if len(outs) == 1:
+        od, ol, _ = outs[0].nodes()
+        n = len(od)
+        order = rng.permutation(n)
+        m = min(n, 32)
+        tables = ops[0].tables()
+        inp = inps[0].nodes()
+        while True:
+            idx = np.sort(order[:m])
+            t0 = time.perf_counter()
+            ref, _ = oracle.apply_fixed(mra, tables, inp, od[idx], ol[idx], thr[0][idx], post=False)
+            dt = time.perf_counter() - t0
+            if m >= n or dt >= 0.5 * target_s:
+                break
+            m = min(n, max(m * 2, int(m * target_s / max(dt, 1e-3))))
+        err = float(np.max(np.abs(ref - ns_gpu[idx])) / max(np.max(np.abs(ref)), 1e-300))
+        sample = "%d of %d frozen-grid output nodes (random, seed 12345)" % (m, n)
+        done = m
+    else:
+        order = rng.permutation(len(outs))
+        done, dt, err, used = 0, 0.0, 0.0, 0
+        tabs = {}
+        for a in order:
+            od, ol, _ = outs[a].nodes()
+            key = id(ops[a])
+            if key not in tabs:
+                tabs[key] = ops[a].tables()
+            t0 = time.perf_counter()
+            ref, _ = oracle.apply_fixed(mra, tabs[key], inps[a].nodes(), od, ol, thr[a], post=False)
+            dt += time.perf_counter() - t0
+            g = ns_gpu[offs[a]:offs[a + 1]]
+            err = max(err, float(np.max(np.abs(ref - g)) / max(np.max(np.abs(ref)), 1e-300)))
+            done += len(od)
+            used += 1
+            if dt >= target_s:
+                break
+        m = done
+        sample = "%d of %d applies (%d of %d output nodes, random order, seed 12345)" % (used, len(outs), done, n_tot)
     check["sample_max_rel_err_vs_oracle"] = err
     log("cpu baseline: %d of %d nodes in %.2fs on %d threads; sample |gpu-oracle|/max = %.2e" %
-        (m, n, dt, threads, err))
+        (m, n_tot, dt, threads, err))
     return {"value": m / dt, "unit": "nodes/s", "cores": threads, "kind": "port",
-            "sample": "%d of %d frozen-grid output nodes (random, seed 12345), FP64 OpenMP oracle, "
-                      "no post transforms" % (m, n)}
+            "sample": sample + ", FP64 OpenMP oracle, no post transforms"}
 
 
 if __name__ == "__main__":

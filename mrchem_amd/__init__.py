@@ -119,6 +119,7 @@ class Context:
         h = _p()
         _check(lib.mwg_context_create(device, ctypes.byref(h)))
         self._h = h
+        self.mode, self.tau = "fp64", 65536.0
 
     @classmethod
     def default(cls):
@@ -130,6 +131,7 @@ class Context:
         """Arithmetic of mw.apply on this context: 'fp64' or 'mixed' (FP64 for
         terms whose bound exceeds tau x threshold, FP32 MFMA for the rest)."""
         _check(lib.mwg_context_set_precision(self._h, MODES[mode], tau))
+        self.mode, self.tau = mode, float(tau)
 
     def last_stats(self):
         s = np.zeros(8)

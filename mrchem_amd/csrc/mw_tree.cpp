@@ -299,36 +299,56 @@ double GaussSum::square_norm() const {
     return s;
 }
 
-void project_scaling(const MRA &mra, const GaussSum &f, int depth, const long long *l, double *out) {
+namespace {
+// 1D integrals I[i] = sqrt(h) sum_p L2B[i][p] int_0^1 P~_p(v) exp(-A (v - c)^2) dv
+// of Gaussian g along direction d on translation l at depth; false when all
+// vanish.
+bool project_1d(const MRA &mra, const GaussSum &f, int depth, int d, long long l, int g, double *I) {
+    const int n = mra.kp1();
+    const double h = mra.box_size(depth), sh = std::sqrt(h);
+    const double a = f.exp[g], A = a * h * h;
+    const double c = f.pos[3 * g + d] / h - (double)l;
+    double mom[32];
+    gauss_moments01(n, A, c, mom);
+    bool any = false;
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int p = 0; p < n; p++) s += mra.basis.L2B[i * n + p] * mom[p];
+        I[i] = sh * s;
+        any |= (s != 0.0);
+    }
+    return any;
+}
+
+// out[(z n + y) n + x] = sum_g c_g (a_g/pi)^{3/2} Iz[z] Iy[y] Ix[x]; Ig[d] points
+// at the n integrals of Gaussian g along d, or is null when they all vanish.
+void combine_1d(const MRA &mra, const GaussSum &f, const double *const *Ig, double *out) {
     const int n = mra.kp1(), n3 = n * n * n;
     std::fill(out, out + n3, 0.0);
-    double h = mra.box_size(depth);
-    double sh = std::sqrt(h);
-    std::vector<double> mom(n), I(3 * n);
     for (int g = 0; g < f.size(); g++) {
-        double a = f.exp[g];
-        double A = a * h * h;
-        bool zero = false;
-        for (int d = 0; d < 3 && !zero; d++) {
-            double c = f.pos[3 * g + d] / h - (double)l[d];
-            gauss_moments01(n, A, c, mom.data());
-            bool any = false;
-            for (int i = 0; i < n; i++) {
-                double s = 0.0;
-                for (int p = 0; p < n; p++) s += mra.basis.L2B[i * n + p] * mom[p];
-                I[d * n + i] = sh * s;
-                any |= (s != 0.0);
-            }
-            if (!any) zero = true;
-        }
-        if (zero) continue;
-        double pre = f.coef[g] * std::pow(a / M_PI, 1.5);
+        const double *Ix = Ig[3 * g], *Iy = Ig[3 * g + 1], *Iz = Ig[3 * g + 2];
+        if (!Ix || !Iy || !Iz) continue;
+        const double pre = f.coef[g] * std::pow(f.exp[g] / M_PI, 1.5);
         for (int z = 0; z < n; z++)
             for (int y = 0; y < n; y++) {
-                double yz = pre * I[2 * n + z] * I[n + y];
-                for (int x = 0; x < n; x++) out[(z * n + y) * n + x] += yz * I[x];
+                const double yz = pre * Iz[z] * Iy[y];
+                for (int x = 0; x < n; x++) out[(z * n + y) * n + x] += yz * Ix[x];
             }
     }
+}
+}  // namespace
+
+void project_scaling(const MRA &mra, const GaussSum &f, int depth, const long long *l, double *out) {
+    const int n = mra.kp1();
+    std::vector<double> I((size_t)f.size() * 3 * n);
+    std::vector<const double *> Ig((size_t)f.size() * 3, nullptr);
+    for (int g = 0; g < f.size(); g++)
+        for (int d = 0; d < 3; d++) {
+            double *dst = I.data() + ((size_t)g * 3 + d) * n;
+            if (!project_1d(mra, f, depth, d, l[d], g, dst)) break;
+            Ig[3 * g + d] = dst;
+        }
+    combine_1d(mra, f, Ig.data(), out);
 }
 
 bool split_check(const MRA &mra, int depth, double wnorm, double prec, double tnorm) {
@@ -340,7 +360,7 @@ bool split_check(const MRA &mra, int depth, double wnorm, double prec, double tn
 
 void project_adaptive(Tree &t, const GaussSum &f, double prec, bool abs_prec, int max_depth) {
     const MRA &mra = *t.mra;
-    const int n3 = mra.kp1_3(), blk = mra.block();
+    const int n = mra.kp1(), n3 = mra.kp1_3(), blk = mra.block(), ng = f.size();
     if (max_depth < 0 || max_depth > mra.max_depth) max_depth = mra.max_depth;
     t.clear();
     t.make_roots();
@@ -348,15 +368,53 @@ void project_adaptive(Tree &t, const GaussSum &f, double prec, bool abs_prec, in
     std::vector<int> work;
     for (int i = 0; i < t.size(); i++) work.push_back(i);
     while (!work.empty()) {
+        // the nodes of one level share their depth; their children's 1D
+        // integrals are tabulated once per (direction, translation, Gaussian)
+        const int depth = t.keys[work[0]].depth, cd = depth + 1;
+        std::vector<long long> lo(3), span(3);
+        for (int d = 0; d < 3; d++) {
+            long long a = 2 * t.keys[work[0]].l[d], b = a + 1;
+            for (int i : work) {
+                a = std::min(a, 2 * t.keys[i].l[d]);
+                b = std::max(b, 2 * t.keys[i].l[d] + 1);
+            }
+            lo[d] = a;
+            span[d] = b - a + 1;
+        }
+        const long long ntab = (span[0] + span[1] + span[2]) * ng;
+        std::vector<double> tab((size_t)ntab * n);
+        std::vector<char> nz((size_t)ntab, 0);
+        auto tab_index = [&](int d, long long l, int g) -> long long {
+            long long base = 0;
+            for (int e = 0; e < d; e++) base += span[e];
+            return (base + (l - lo[d])) * ng + g;
+        };
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long long e = 0; e < ntab; e++) {
+            const int g = (int)(e % ng);
+            long long r = e / ng;
+            int d = 0;
+            while (r >= span[d]) r -= span[d++];
+            nz[e] = project_1d(mra, f, cd, d, lo[d] + r, g, tab.data() + e * n) ? 1 : 0;
+        }
         std::vector<char> do_split(work.size(), 0);
 #pragma omp parallel for schedule(dynamic, 4)
         for (int q = 0; q < (int)work.size(); q++) {
             int i = work[q];
             const NodeKey k = t.keys[i];
             std::vector<double> ch(blk);
+            std::vector<const double *> Ig((size_t)ng * 3);
             for (int c = 0; c < 8; c++) {
                 long long cl[3] = {2 * k.l[0] + (c & 1), 2 * k.l[1] + ((c >> 1) & 1), 2 * k.l[2] + ((c >> 2) & 1)};
-                project_scaling(mra, f, k.depth + 1, cl, ch.data() + c * n3);
+                for (int g = 0; g < ng; g++) {
+                    bool dead = false;
+                    for (int d = 0; d < 3; d++) {
+                        const long long e = tab_index(d, cl[d], g);
+                        Ig[3 * g + d] = (!dead && nz[e]) ? tab.data() + e * n : nullptr;
+                        dead = dead || !nz[e];
+                    }
+                }
+                combine_1d(mra, f, Ig.data(), ch.data() + c * n3);
             }
             double *dst = t.coefs.data() + (size_t)i * blk;
             mw_compress(mra.basis, ch.data(), dst);
