@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="mixed,split16", help="low-precision modes timed beside plain FP64")
+    ap.add_argument("--mode", default="split16", help="the mode the headline value is quoted for")
+    ap.add_argument("--tau", type=float, default=None, help="FP64 class cut (bound > tau x threshold)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -90,6 +93,8 @@ def main():
         (cfg["name"], n_app, ops[0].size(), setup["operator_build_s"], sum(f.size() for f in inps),
          setup["projection_s"]))
     ctx = mw.Context.default()
+    if args.tau is not None:
+        ctx.set_precision("fp64", args.tau)
     outs = [mw.FunctionTree(mra) for _ in inps]
     prec = cfg["prec"]
     entries = [dict(prec=prec if cfg["abs_prec"] else prec / f.norm(), out=o, oper=op, inp=f,
@@ -103,13 +108,18 @@ def main():
         (n_out, t_adapt, ast["kernel_ms"], ast["plan_ms"], ast["launches"]))
     thr = [o.thresholds() for o in outs]
     grids = [o.nodes()[:2] for o in outs]
-    # two plans on the same frozen grids and thresholds: plain FP64 and mixed
-    outs_mx = [mw.FunctionTree(mra).set_nodes(*g) for g in grids]
+    # plans on the same frozen grids and thresholds: plain FP64 and the
+    # low-precision modes (FP64 for bound > tau thr, the rest in FP32 or as
+    # FP16 split products)
+    modes = ["fp64"] + [m for m in args.modes.split(",") if m != "fp64"]
     t0 = time.time()
-    plan64 = mw.ApplyPlan(outs, ops, inps, thr, mode="fp64", ctx=ctx)
-    planmx = mw.ApplyPlan(outs_mx, ops, inps, thr, mode="mixed", ctx=ctx)
+    plans = {}
+    trees = {}
+    for m in modes:
+        trees[m] = outs if m == "fp64" else [mw.FunctionTree(mra).set_nodes(*g) for g in grids]
+        plans[m] = mw.ApplyPlan(trees[m], ops, inps, thr, mode=m, ctx=ctx)
     t_plan = time.time() - t0
-    pinfo = plan64.info()
+    pinfo = plans["fp64"].info()
     log("frozen plans: %d out nodes, %d input slots (%d generated), %d segments, %.2fs" %
         (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], pinfo["segments"], t_plan))
 
@@ -123,52 +133,62 @@ def main():
         wall = time.perf_counter() - t0
         return runs, 1e3 * wall / args.steps
 
-    runs64, ms64 = timed(plan64)
-    runsmx, msmx = timed(planmx)
-    apps = runs64[-1]["term_apps"]
-    assert runsmx[-1]["term_apps"] == apps, "mixed and FP64 paths applied different term sets"
-    assert apps == pinfo["host_term_apps"], "kernel and host screening counts differ"
-    lp_apps = runsmx[-1]["lp_term_apps"]
     fl = 6.0 * (k + 1) ** 4
-    k64 = float(np.mean([r["ms"] for r in runs64]))
-    kmx = float(np.mean([r["ms"] for r in runsmx]))
-    kmx_lp = float(np.mean([r["ms_lp"] for r in runsmx]))
-    kmx_64 = float(np.mean([r["ms_fp64"] for r in runsmx]))
-    ach64 = apps * fl / (k64 * 1e-3) / 1e12
-    ach_lp = lp_apps * fl / (kmx_lp * 1e-3) / 1e12
-    log("fp64 : wall %.3f ms/step, kernel %.3f ms, %d term-apps, %.2f TFLOP/s" % (ms64, k64, apps, ach64))
-    log("mixed: wall %.3f ms/step, kernel %.3f ms (fp64 part %.3f ms, fp32 part %.3f ms), %d of %d term-apps "
-        "on FP32 (%.2f TFLOP/s)" % (msmx, kmx, kmx_64, kmx_lp, lp_apps, apps, ach_lp))
-
-    # error of the mixed path against the FP64 path on the identical grids
-    ns64 = plan64.download_ns()
-    nsmx = planmx.download_ns()
-    coef_err = float(np.max(np.abs(nsmx - ns64)) / np.max(np.abs(ns64)))
-    plan64.finish()
-    planmx.finish()
+    res = {}
+    for m in modes:
+        runs, wall = timed(plans[m])
+        r = {"wall_ms": wall, "kernel_ms": float(np.mean([x["ms"] for x in runs])), "term_apps": runs[-1]["term_apps"],
+             "lp_term_apps": runs[-1]["lp_term_apps"], "ms_fp64": float(np.mean([x["ms_fp64"] for x in runs])),
+             "ms_lp": float(np.mean([x["ms_lp"] for x in runs]))}
+        res[m] = r
+    apps = res["fp64"]["term_apps"]
+    assert apps == pinfo["host_term_apps"], "kernel and host screening counts differ"
+    for m in modes:
+        assert res[m]["term_apps"] == apps, "%s applied a different term set" % m
+    ach64 = apps * fl / (res["fp64"]["kernel_ms"] * 1e-3) / 1e12
+    log("fp64   : wall %.3f ms/step, kernel %.3f ms, %d term-apps, %.2f TFLOP/s" %
+        (res["fp64"]["wall_ms"], res["fp64"]["kernel_ms"], apps, ach64))
+    ns64 = plans["fp64"].download_ns()
+    for m in modes[1:]:
+        r = res[m]
+        r["lp_tflops"] = r["lp_term_apps"] * fl / (r["ms_lp"] * 1e-3) / 1e12 if r["ms_lp"] > 0 else 0.0
+        ns = plans[m].download_ns()
+        r["coef_err"] = float(np.max(np.abs(ns - ns64)) / np.max(np.abs(ns64)))
+        log("%-7s: wall %.3f ms/step, kernel %.3f ms (fp64 part %.3f ms, low-precision part %.3f ms), %d of %d "
+            "term-apps low-precision (%.2f TFLOP/s algorithmic), max coef err vs fp64 %.3e" %
+            (m, r["wall_ms"], r["kernel_ms"], r["ms_fp64"], r["ms_lp"], r["lp_term_apps"], apps, r["lp_tflops"],
+             r["coef_err"]))
+    for m in modes:
+        plans[m].finish()
     c64 = np.concatenate([o.nodes()[2] for o in outs])
-    cmx = np.concatenate([o.nodes()[2] for o in outs_mx])
-    tree_err = float(np.linalg.norm(cmx - c64) / np.linalg.norm(c64))
-    log("mixed vs fp64: max coefficient error %.3e (relative to max), tree-norm error %.3e" % (coef_err, tree_err))
+    for m in modes[1:]:
+        cm = np.concatenate([o.nodes()[2] for o in trees[m]])
+        res[m]["tree_err"] = float(np.linalg.norm(cm - c64) / np.linalg.norm(c64))
+    head = args.mode if args.mode in res else modes[-1]
+    hr = res[head]
 
     # result check: energies against the closed forms
     from oracle import closed_forms as cf
-    e_err64, e_errmx, e_max = 0.0, 0.0, 0.0
-    for f, o64, omx, d, mu in zip(inps, outs, outs_mx, cfg["densities"], mus):
+    e_err = {m: 0.0 for m in modes}
+    e_max = 0.0
+    for a_i, (f, d, mu) in enumerate(zip(inps, cfg["densities"], mus)):
         E_ex = cf.pair_energy(*d, mu=mu)
-        e_err64 = max(e_err64, abs(f.dot(o64) - E_ex))
-        e_errmx = max(e_errmx, abs(f.dot(omx) - E_ex))
         e_max = max(e_max, abs(E_ex))
-    check = {"energy_max_abs_err_fp64": e_err64, "energy_max_abs_err_mixed": e_errmx, "energy_max_abs": e_max,
-             "mixed_vs_fp64_max_coef_rel": coef_err, "mixed_vs_fp64_tree_rel": tree_err}
+        for m in modes:
+            e_err[m] = max(e_err[m], abs(f.dot(trees[m][a_i]) - E_ex))
+    check = {"energy_max_abs_err": e_err, "energy_max_abs": e_max,
+             "vs_fp64_max_coef_rel": {m: res[m]["coef_err"] for m in modes[1:]},
+             "vs_fp64_tree_rel": {m: res[m]["tree_err"] for m in modes[1:]}}
     if n_app == 1:
         check["energy_closed_form"] = cf.pair_energy(*cfg["densities"][0], mu=mus[0])
-    log("energies vs closed form: max abs err fp64 %.3e, mixed %.3e (max |E| %.3e)" % (e_err64, e_errmx, e_max))
-
+    log("energies vs closed form: max abs err %s (max |E| %.3e)" %
+        (", ".join("%s %.3e" % (m, e_err[m]) for m in modes), e_max))
+    msmx, ms64 = hr["wall_ms"], res["fp64"]["wall_ms"]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mra, ops, inps, outs, thr, ns64, args.cpu_seconds, check)
 
+    peak_key = "f16" if head == "split16" else "f32"
     line = {
         "metric": METRIC,
         "value": n_out / (msmx * 1e-3),
@@ -180,19 +200,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64 (mixed: FP64 for bound > %g thr)" % ctx.tau,
+        "dtype": ("f16x3+f64 (FP16 hi/lo split products, FP32 accumulate; FP64 for bound > %g thr)" if head == "split16"
+                  else "f32+f64 (mixed: FP64 for bound > %g thr)") % ctx.tau,
         "data": "synthetic (SplitMix64-seeded Gaussian sums)",
         "config": {"workload": cfg["name"], "order": k, "prec": prec, "applies": n_app, "out_nodes": n_out,
-                   "in_slots": pinfo["in_slots"], "terms": ops[0].size(), "path": "mixed FP32/FP64 MFMA"},
-        "roofline": {"bound": "mfma", "achieved": ach_lp, "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
-                     "frac": ach_lp / PEAK_TFLOPS["f32"], "traffic": None,
-                     "kernel": "k_apply<%d, float>" % (k + 1)},
-        "fp64_path": {"value": n_out / (ms64 * 1e-3), "ms_per_step": ms64, "kernel_ms": k64,
+                   "in_slots": pinfo["in_slots"], "terms": ops[0].size(), "path": head},
+        "roofline": {"bound": "mfma", "achieved": hr["lp_tflops"], "peak": PEAK_TFLOPS[peak_key], "unit": "TFLOP/s",
+                     "frac": hr["lp_tflops"] / PEAK_TFLOPS[peak_key], "traffic": None,
+                     "kernel": ("k_apply_h<%d>" if head == "split16" else "k_apply<%d, float>") % (k + 1)},
+        "fp64_path": {"value": n_out / (ms64 * 1e-3), "ms_per_step": ms64, "kernel_ms": res["fp64"]["kernel_ms"],
                       "roofline": {"achieved": ach64, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                                    "frac": ach64 / PEAK_TFLOPS["f64"]}},
+        "modes": {m: dict(res[m], value=n_out / (res[m]["wall_ms"] * 1e-3)) for m in modes},
         "speedup_vs_fp64": ms64 / msmx,
         "term_apps": apps,
-        "lp_term_apps": lp_apps,
+        "lp_term_apps": hr["lp_term_apps"],
         "check": check,
         "setup": dict(setup, adaptive_apply_s=t_adapt, plan_s=t_plan),
     }

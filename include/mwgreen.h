@@ -39,10 +39,13 @@ extern "C" {
 
 /* Arithmetic of the per-node contractions.  MIXED applies every
  * term-application whose screening bound exceeds tau * threshold in FP64 and
- * the others on FP32 MFMA, adding their sum to the FP64 output; the set of
- * terms applied is the same in both modes. */
+ * the others on FP32 MFMA; SPLIT16 applies the others with FP16 hi/lo split
+ * products on FP16 MFMA (hi*hi + lo*hi + hi*lo, FP32 accumulation).  Both add
+ * their sums to the FP64 output; the set of terms applied is the same in all
+ * modes. */
 #define MWG_MODE_FP64 0
 #define MWG_MODE_MIXED 1
+#define MWG_MODE_SPLIT16 2
 
 typedef struct mwg_context_s *mwg_context;
 typedef struct mwg_mra_s *mwg_mra;
@@ -158,8 +161,9 @@ int mwg_plan_create_batch(mwg_context ctx, int n, const mwg_tree *out, const mwg
                           const double *const *thr, int mode, mwg_plan *plan);
 /* One execution of the per-node calculator on resident data, with the plan's
  * `mode` (tau from the context).  stats (n <= 5 values): [0] device ms,
- * [1] term-applications that passed screening, [2] of them on FP32 MFMA,
- * [3] device ms of the FP64 launch, [4] of the FP32 launch. */
+ * [1] term-applications that passed screening, [2] of them on the
+ * low-precision path (FP32 or FP16 split),
+ * [3] device ms of the FP64 launch, [4] of the low-precision launch. */
 int mwg_plan_execute(mwg_plan plan, double *stats, int n);
 /* Copy the result into the output tree(s) and run the TopDown/BottomUp
  * transforms. */

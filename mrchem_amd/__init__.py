@@ -94,7 +94,7 @@ _sig("mwg_plan_info", _p, _dp, _i)
 _sig("mwg_plan_download_ns", _p, _dp)
 _sig("mwg_plan_destroy", _p)
 
-MODES = {"fp64": 0, "mixed": 1}
+MODES = {"fp64": 0, "mixed": 1, "split16": 2}
 
 
 class MWGError(RuntimeError):
@@ -128,8 +128,9 @@ class Context:
         return cls._default
 
     def set_precision(self, mode="fp64", tau=65536.0):
-        """Arithmetic of mw.apply on this context: 'fp64' or 'mixed' (FP64 for
-        terms whose bound exceeds tau x threshold, FP32 MFMA for the rest)."""
+        """Arithmetic of mw.apply on this context: 'fp64', 'mixed' (FP64 for
+        terms whose bound exceeds tau x threshold, FP32 MFMA for the rest) or
+        'split16' (the rest as FP16 hi/lo split products on FP16 MFMA)."""
         _check(lib.mwg_context_set_precision(self._h, MODES[mode], tau))
         self.mode, self.tau = mode, float(tau)
 
@@ -407,7 +408,8 @@ class ApplyPlan:
 
     def execute(self):
         """One execution; returns a dict: ms (device), term_apps, lp_term_apps
-        (on FP32 MFMA), ms_fp64 and ms_lp (device time of the two launches)."""
+        (on the low-precision path: FP32 MFMA or FP16 split products), ms_fp64
+        and ms_lp (device time of the two launches)."""
         v = np.zeros(5)
         _check(lib.mwg_plan_execute(self._h, _ptr(v, _d), 5))
         return {"ms": v[0], "term_apps": int(v[1]), "lp_term_apps": int(v[2]), "ms_fp64": v[3], "ms_lp": v[4]}

@@ -36,6 +36,8 @@ struct AppDesc {
     const double *op_norms;
     const double *blk64;              // FP64 blocks
     const float *blk32;               // FP32 copy (mixed mode), or null
+    const _Float16 *blk_h;            // FP16 split rows (split mode), or null
+    const double *blk_hsc;            // their per-block scales
     int M, ndepth;
     long long bw_off;                 // start of this apply in bw
     int act_base;                     // start of this apply in act_off
@@ -147,6 +149,102 @@ struct JobShape {
 constexpr int kMaxBand = 9;     // band half-width limit
 constexpr int kWorkCap = 1024;  // worklist items per batch (>= 8 * 64 + 512)
 
+// Output job of a work segment and its screening context.
+struct JobCtx {
+    int g, gt, d, w, side, nl, a0, nact, ndepth;
+    long long boff;
+    double thr, thr_lo, thr_hi;
+    int gb[3];
+};
+
+__device__ __forceinline__ JobCtx job_ctx(const KArgs &a, const AppDesc &ad, int job, double lo_mult,
+                                          double hi_mult) {
+    JobCtx j;
+    j.g = job >> 3;
+    j.gt = job & 7;
+    j.d = a.out_depth[j.g];
+    j.thr = a.out_thr[j.g];
+    // class cuts relative to the threshold (floored, so that nodes computed
+    // without screening still split by magnitude)
+    const double thr_cls = j.thr > ad.thr_floor ? j.thr : ad.thr_floor;
+    j.thr_lo = thr_cls * lo_mult;
+    j.thr_hi = thr_cls * hi_mult;
+    j.w = a.band_w[j.g];
+    j.side = 2 * j.w + 1;
+    j.nl = 2 * j.w + 1;
+    j.boff = a.band_off[j.g];
+    j.a0 = a.act_off[ad.act_base + j.d];
+    j.nact = a.act_off[ad.act_base + j.d + 1] - j.a0;
+    j.ndepth = ad.ndepth;
+    j.gb[0] = j.gt & 1;
+    j.gb[1] = (j.gt >> 1) & 1;
+    j.gb[2] = (j.gt >> 2) & 1;
+    return j;
+}
+
+// Stage the chunk of active terms [base, base + cnt): per lane-term its block
+// half-width and block offset, and its 1D norms [term][c][l + W] for every
+// component and band translation (zero outside the term's bandwidth).
+__device__ __forceinline__ void stage_terms(const KArgs &a, const AppDesc &ad, const JobCtx &j, int base, int cnt,
+                                            int *ltab, long long *btab, double *nrm_s, int tid, int nth) {
+    for (int e = tid; e < 64; e += nth) {
+        const int m = (e < cnt) ? a.act_terms[j.a0 + base + e] : 0;
+        ltab[e] = (e < cnt) ? ad.opL[m * j.ndepth + j.d] : -1;
+        btab[e] = (e < cnt) ? ad.op_boff[m * j.ndepth + j.d] : 0;
+    }
+    for (int e = tid; e < 64 * 4 * j.nl; e += nth) {
+        const int ti = e / (4 * j.nl), c = (e / j.nl) % 4, li = e % j.nl, l = li - j.w;
+        double v = 0.0;
+        if (ti < cnt) {
+            const int m = a.act_terms[j.a0 + base + ti];
+            const int L = ad.opL[m * j.ndepth + j.d];
+            const int bwc = a.bw[ad.bw_off + ((long long)j.d * ad.M + m) * 4 + c];
+            if (abs(l) <= bwc && abs(l) <= L)
+                v = ad.op_norms[ad.op_noff[m * j.ndepth + j.d] + (long long)c * (2 * L + 1) + (l + L)];
+        }
+        nrm_s[e] = v;
+    }
+}
+
+// Screening of band positions from p on into the worklist `wl` (capacity
+// cap) until it could overflow: for every input component ft the lane-term's
+// bound ((N_x N_y) N_z) ||f_ft|| is formed in FP64 in the oracle's operation
+// order and a ballot appends the terms that pass the reference rule
+// (bound > thr) and fall in the launch's class (bound > thr_lo and not
+// bound > thr_hi) as items (p << 9 | ft << 6 | lane).  Every wave computes the
+// same ballots; the writer wave stores the items.  Returns the item count.
+__device__ __forceinline__ int screen_items(const KArgs &a, const JobCtx &j, int &p, int p_end, int cnt,
+                                            const double *nrm_s, int *wl, int cap, int lane, bool writer) {
+    const bool have = lane < cnt;
+    int n = 0;
+    for (; p < p_end && n + 8 * 64 <= cap; p++) {
+        const int s = a.band[j.boff + p];
+        if (s < 0) continue;
+        const int ll[3] = {p % j.side - j.w, (p / j.side) % j.side - j.w, p / (j.side * j.side) - j.w};
+        double nrm[2][3];
+#pragma unroll
+        for (int dim = 0; dim < 3; dim++)
+#pragma unroll
+            for (int fb = 0; fb < 2; fb++)
+                nrm[fb][dim] = nrm_s[(lane * 4 + (2 * j.gb[dim] + fb)) * j.nl + (ll[dim] + j.w)];
+        const double *fnp = a.slot_norm + (long long)s * 8;
+        for (int ft = 0; ft < 8; ft++) {
+            if (j.d > 0 && ft == 0 && j.gt == 0) continue;
+            const double fn = fnp[ft];
+            if (!(fn > 0.0)) continue;
+            const double bound = ((nrm[ft & 1][0] * nrm[(ft >> 1) & 1][1]) * nrm[(ft >> 2) & 1][2]) * fn;
+            const bool take = have && bound > j.thr && bound > j.thr_lo && !(bound > j.thr_hi);
+            const unsigned long long mask = __ballot(take);
+            if (take && writer) {
+                const int pos = n + __popcll(mask & ((1ull << lane) - 1ull));
+                wl[pos] = (p << 9) | (ft << 6) | lane;
+            }
+            n += __popcll(mask);
+        }
+    }
+    return n;
+}
+
 template <int N, typename T>
 __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const int *seg_list, double lo_mult,
                                                           double hi_mult, int cls_counter) {
@@ -167,21 +265,13 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
 
     // the launch's segments: all of them, or the listed ones
     const int seg = seg_list ? seg_list[blockIdx.x] : (int)blockIdx.x;
-    const int job = a.seg_job[seg], p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
-    const int g = job >> 3, gt = job & 7;
-    const int d = a.out_depth[g];
-    const double thr = a.out_thr[g];
-    const AppDesc ad = a.apps[a.out_app[g]];
+    const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
+    const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
+    const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
+    const int g = j.g, gt = j.gt, w = j.w, side = j.side;
+    const long long boff = j.boff;
+    const int *gb = j.gb;
     const T *opblk = op_blocks<T>(ad);
-    // class cuts relative to the threshold (floored, so that nodes computed
-    // without screening still split by magnitude)
-    const double thr_cls = thr > ad.thr_floor ? thr : ad.thr_floor;
-    const double thr_lo = thr_cls * lo_mult, thr_hi = thr_cls * hi_mult;
-    const int w = a.band_w[g], side = 2 * w + 1, nl = 2 * w + 1;
-    const long long boff = a.band_off[g];
-    const int a0 = a.act_off[ad.act_base + d], nact = a.act_off[ad.act_base + d + 1] - a0;
-    const int ndepth = ad.ndepth;
-    const int gb[3] = {gt & 1, (gt >> 1) & 1, (gt >> 2) & 1};
 
     // this wave's tiles: t = wv*TW + tt
     int ybase[TW], zbase[TW];
@@ -228,58 +318,15 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
         }
     };
 
-    for (int base = 0; base < nact; base += 64) {
-        const int cnt = min(64, nact - base);
+    for (int base = 0; base < j.nact; base += 64) {
+        const int cnt = min(64, j.nact - base);
         __syncthreads();
-        for (int e = tid; e < 64; e += NTH) {
-            const int m = (e < cnt) ? a.act_terms[a0 + base + e] : 0;
-            ltab[e] = (e < cnt) ? ad.opL[m * ndepth + d] : -1;
-            btab[e] = (e < cnt) ? ad.op_boff[m * ndepth + d] : 0;
-        }
-        for (int e = tid; e < 64 * 4 * nl; e += NTH) {
-            const int ti = e / (4 * nl), c = (e / nl) % 4, li = e % nl, l = li - w;
-            double v = 0.0;
-            if (ti < cnt) {
-                const int m = a.act_terms[a0 + base + ti];
-                const int L = ad.opL[m * ndepth + d];
-                const int bwc = a.bw[ad.bw_off + ((long long)d * ad.M + m) * 4 + c];
-                if (abs(l) <= bwc && abs(l) <= L)
-                    v = ad.op_norms[ad.op_noff[m * ndepth + d] + (long long)c * (2 * L + 1) + (l + L)];
-            }
-            nrm_s[e] = v;
-        }
+        stage_terms(a, ad, j, base, cnt, ltab, btab, nrm_s, tid, NTH);
         __syncthreads();
-        const bool have = lane < cnt;
         int p = p_begin;
         while (p < p_end) {
-            // screening into the worklist (every wave computes the same
-            // ballots; wave 0 writes the items)
-            int n = 0;
-            for (; p < p_end && n + 8 * 64 <= kWorkCap; p++) {
-                const int s = a.band[boff + p];
-                if (s < 0) continue;
-                const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
-                double nrm[2][3];
-#pragma unroll
-                for (int dim = 0; dim < 3; dim++)
-#pragma unroll
-                    for (int fb = 0; fb < 2; fb++)
-                        nrm[fb][dim] = nrm_s[(lane * 4 + (2 * gb[dim] + fb)) * nl + (ll[dim] + w)];
-                const double *fnp = a.slot_norm + (long long)s * 8;
-                for (int ft = 0; ft < 8; ft++) {
-                    if (d > 0 && ft == 0 && gt == 0) continue;
-                    const double fn = fnp[ft];
-                    if (!(fn > 0.0)) continue;
-                    const double bound = ((nrm[ft & 1][0] * nrm[(ft >> 1) & 1][1]) * nrm[(ft >> 2) & 1][2]) * fn;
-                    const bool take = have && bound > thr && bound > thr_lo && !(bound > thr_hi);
-                    const unsigned long long mask = __ballot(take);
-                    if (take && wv == 0) {
-                        const int pos = n + __popcll(mask & ((1ull << lane) - 1ull));
-                        wl[pos] = (p << 9) | (ft << 6) | lane;
-                    }
-                    n += __popcll(mask);
-                }
-            }
+            // screening into the worklist (wave 0 writes the items)
+            const int n = screen_items(a, j, p, p_end, cnt, nrm_s, wl, kWorkCap, lane, wv == 0);
             __syncthreads();
             if (n == 0) continue;
             napps += n;
@@ -381,6 +428,290 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
     }
 }
 
+// ------------------------------------------------------ FP16 split products
+//
+// Each operand value v of a block, scaled into (-1, 1) by the block's
+// power-of-two scale, is split as v = hi + lo (both FP16, lo = v - hi rounded).
+// A row of up to 12 values is stored as 24 halfs, 48 bytes:
+//   hi[0..7] | lo[0..7] | hi[8..11] lo[8..11]
+// so that a lane reads any 8-value chunk with one 16-byte load.  A product is
+// hi*hi + lo*hi + hi*lo (the lo*lo term is below the representation error):
+// the data side supplies K chunks [hi | lo | hi | tail] and the operator side
+// [hi | hi | lo | tail], one v_mfma_f32_16x16x32_f16 per 16x16 tile for
+// j < 8 plus the tail j = 8, 9 (N <= 10); for N = 11, 12 the tail carries
+// hi*hi + lo*hi of j = 8..11 and a v_mfma_f32_16x16x16f16 adds hi*lo of them.
+typedef _Float16 mwg_h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 mwg_h4 __attribute__((ext_vector_type(4)));
+constexpr int kRowH = 24;
+
+template <int N>
+struct SplitShape {
+    static constexpr int N2 = N * N, N3 = N2 * N;
+    static constexpr int G = (N + 3) / 4;       // 4-line groups per line of N
+    static constexpr int NG = N * G;            // groups of an (outer, inner) plane
+    static constexpr int NTG = (NG + 3) / 4;    // 16-row tiles of groups (x and y passes)
+    static constexpr int NTZ = (N2 + 15) / 16;  // 16-column tiles (z pass)
+    static constexpr int NW = 2;                // waves per job
+    static constexpr int T = 64 * NW;
+    static constexpr int TWG = (NTG + NW - 1) / NW;
+    static constexpr int TWZ = (NTZ + NW - 1) / NW;
+    static constexpr bool TWO = N > 10;
+};
+
+// Lane chunk c (= lane / 16) of a split row: data side (DATA) or operator
+// side; ext is the operand of the second MFMA (N > 10), in chunk 0 only.
+template <int N, bool DATA>
+__device__ __forceinline__ void split_chunk(const _Float16 *row, int c, mwg_h8 &v, mwg_h4 &ext) {
+    const mwg_h8 *r8 = reinterpret_cast<const mwg_h8 *>(row);
+    const _Float16 z = (_Float16)0.0f;
+    ext = mwg_h4{z, z, z, z};
+    if (c < 3) {
+        v = r8[(DATA ? (c == 1) : (c == 2)) ? 1 : 0];
+        if (SplitShape<N>::TWO && c == 0) {
+            const mwg_h8 t = r8[2];
+            ext = DATA ? mwg_h4{t[0], t[1], t[2], t[3]} : mwg_h4{t[4], t[5], t[6], t[7]};
+        }
+        return;
+    }
+    if (N <= 8) {
+        v = mwg_h8{z, z, z, z, z, z, z, z};
+        return;
+    }
+    const mwg_h8 t = r8[2];  // hi8..11, lo8..11
+    if (N <= 10)
+        v = DATA ? mwg_h8{t[0], t[1], t[4], t[5], t[0], t[1], z, z} : mwg_h8{t[0], t[1], t[0], t[1], t[4], t[5], z, z};
+    else
+        v = DATA ? t : mwg_h8{t[0], t[1], t[2], t[3], t[0], t[1], t[2], t[3]};
+}
+
+// The second MFMA gets its own zero accumulator: issued directly on the
+// accumulator just written by the 16x16x32 MFMA (no wait states between
+// them in the generated code) it gave wrong sums on gfx950, while separate
+// accumulators added afterwards are exact.
+template <int N>
+__device__ __forceinline__ mwg_float4 split_mfma(const mwg_h8 &a, const mwg_h4 &ae, const mwg_h8 &b, const mwg_h4 &be,
+                                                 mwg_float4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    if (SplitShape<N>::TWO) {
+        const mwg_float4 e = __builtin_amdgcn_mfma_f32_16x16x16f16(ae, be, mwg_float4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; r++) c[r] += e[r];
+    }
+    return c;
+}
+
+// Store four consecutive values (positions j0..j0+3 of a row) as hi and lo.
+__device__ __forceinline__ void split_store4(_Float16 *row, int j0, const mwg_float4 &v) {
+    mwg_h4 hi, lo;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        hi[r] = (_Float16)v[r];
+        lo[r] = (_Float16)(v[r] - (float)hi[r]);
+    }
+    *reinterpret_cast<mwg_h4 *>(row + (j0 < 8 ? j0 : 16)) = hi;
+    *reinterpret_cast<mwg_h4 *>(row + (j0 < 8 ? 8 + j0 : 20)) = lo;
+}
+
+// Per-item passes with FP16 split operands (the job, the screening and the
+// worklist are those of k_apply).  With unit-scaled operands F_u, O_u:
+//   x: X[(jz,jy)][ix] = F_u . Ox_u^T, stored transposed as Xt[(ix,jz)][jy]
+//   y: Y[(ix,jz)][iy] = Xt . Oy_u^T,  stored as Yt[(iy,ix)][jz]
+//   z: Z[iz][(iy,ix)] = Oz_u . Yt^T,  acc += sF sOx sOy sOz Z  (FP32)
+// The rows of the x and y passes are taken in groups of four consecutive
+// inner indices of one outer line, so every lane stores its four results
+// with two 8-byte writes.  Intermediates are split again before the next
+// pass; their magnitudes stay below N^2, well inside the FP16 range.
+template <int N>
+__global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
+                                                             const int *seg_list, double lo_mult, double hi_mult,
+                                                             int cls_counter) {
+    using S = SplitShape<N>;
+    constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTG = S::NTG, NTZ = S::NTZ;
+    constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T;
+    __shared__ __align__(16) _Float16 xt[N2 * kRowH];
+    __shared__ __align__(16) _Float16 yt[N2 * kRowH];
+    __shared__ int wl[kWorkCap];
+    __shared__ int ltab[64];
+    __shared__ long long btab[64];
+    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lc = lane & 15, lq = lane >> 4;
+
+    const int seg = seg_list ? seg_list[blockIdx.x] : (int)blockIdx.x;
+    const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
+    const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
+    const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
+    const _Float16 z16 = (_Float16)0.0f;
+    const mwg_h8 zero8 = {z16, z16, z16, z16, z16, z16, z16, z16};
+    const mwg_h4 zero4 = {z16, z16, z16, z16};
+
+    mwg_float4 acc[TWZ];
+#pragma unroll
+    for (int tt = 0; tt < TWZ; tt++) acc[tt] = mwg_float4{0.f, 0.f, 0.f, 0.f};
+    unsigned long long napps = 0;
+
+    // x-pass A operand (data side): rows (jz, jy) of the F block, in groups
+    mwg_h8 fa[TWG];
+    mwg_h4 fe[TWG];
+    double sF = 0.0;
+    auto load_f = [&](int p, int ft) {
+        const int s = a.band[j.boff + p];
+        const long long cb = a.slot_off[s] / N3 + ft;  // component block of the pool
+        sF = pool_sc[cb];
+        const _Float16 *base = pool_h + cb * (long long)(N2 * kRowH);
+#pragma unroll
+        for (int tt = 0; tt < TWG; tt++) {
+            const int t = wv + S::NW * tt;
+            const int grp = 4 * t + (lc >> 2), jy = 4 * (grp % G) + (lc & 3);
+            if (t < NTG && grp < NG && jy < N) {
+                split_chunk<N, true>(base + ((grp / G) * N + jy) * kRowH, lq, fa[tt], fe[tt]);
+            } else {
+                fa[tt] = zero8;
+                fe[tt] = zero4;
+            }
+        }
+    };
+    // operator rows of item `it` along dim: row lc of the block, chunk lq
+    auto load_op = [&](int it, int dim, mwg_h8 &v, mwg_h4 &e, double &sc) {
+        const int p = it >> 9, ft = (it >> 6) & 7, b = it & 63;
+        const int l = (dim == 0 ? p % j.side : (dim == 1 ? (p / j.side) % j.side : p / (j.side * j.side))) - j.w;
+        const int Lb = ltab[b];
+        const int c = (j.gb[dim] << 1) | ((ft >> dim) & 1);
+        const long long blk = btab[b] / N2 + (long long)c * (2 * Lb + 1) + (l + Lb);
+        sc = ad.blk_hsc[blk];
+        if (lc < N) {
+            split_chunk<N, false>(ad.blk_h + blk * (long long)(N * kRowH) + lc * kRowH, lq, v, e);
+        } else {
+            v = zero8;
+            e = zero4;
+        }
+    };
+
+    for (int base = 0; base < j.nact; base += 64) {
+        const int cnt = min(64, j.nact - base);
+        __syncthreads();
+        stage_terms(a, ad, j, base, cnt, ltab, btab, nrm_s, tid, NTH);
+        __syncthreads();
+        int p = p_begin;
+        while (p < p_end) {
+            const int n = screen_items(a, j, p, p_end, cnt, nrm_s, wl, kWorkCap, lane, wv == 0);
+            __syncthreads();
+            if (n == 0) continue;
+            napps += n;
+            mwg_h8 ov[3], ovn[3];
+            mwg_h4 oe[3], oen[3];
+            double so[3], son[3];
+            int it = wl[0];
+            load_f(it >> 9, (it >> 6) & 7);
+#pragma unroll
+            for (int dim = 0; dim < 3; dim++) load_op(it, dim, ov[dim], oe[dim], so[dim]);
+            for (int i = 0; i < n; i++) {
+                const bool more = i + 1 < n;
+                const int nx = more ? wl[i + 1] : it;
+                if (more) {
+#pragma unroll
+                    for (int dim = 0; dim < 3; dim++) load_op(nx, dim, ovn[dim], oen[dim], son[dim]);
+                }
+                // x pass -> Xt[(ix, jz)][jy]
+#pragma unroll
+                for (int tt = 0; tt < TWG; tt++) {
+                    const int t = wv + S::NW * tt;
+                    if (t >= NTG) continue;
+                    const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], ov[0], oe[0], mwg_float4{0.f, 0.f, 0.f, 0.f});
+                    const int grp = 4 * t + lq;
+                    if (grp < NG && lc < N) split_store4(xt + (lc * N + grp / G) * kRowH, 4 * (grp % G), dx);
+                }
+                __syncthreads();
+                // y pass: rows (ix, jz) of Xt in groups -> Yt[(iy, ix)][jz]
+#pragma unroll
+                for (int tt = 0; tt < TWG; tt++) {
+                    const int t = wv + S::NW * tt;
+                    if (t >= NTG) continue;
+                    mwg_h8 va = zero8;
+                    mwg_h4 ve = zero4;
+                    const int grp = 4 * t + (lc >> 2), jz = 4 * (grp % G) + (lc & 3);
+                    if (grp < NG && jz < N) split_chunk<N, true>(xt + ((grp / G) * N + jz) * kRowH, lq, va, ve);
+                    const mwg_float4 dy = split_mfma<N>(va, ve, ov[1], oe[1], mwg_float4{0.f, 0.f, 0.f, 0.f});
+                    const int grp2 = 4 * t + lq;
+                    if (grp2 < NG && lc < N) split_store4(yt + (lc * N + grp2 / G) * kRowH, 4 * (grp2 % G), dy);
+                }
+                __syncthreads();
+                // z pass: acc[iz][(iy, ix)] += sigma Oz_u . Yt^T
+                const float sigma = (float)(sF * so[0] * so[1] * so[2]);
+#pragma unroll
+                for (int tt = 0; tt < TWZ; tt++) {
+                    const int t = wv + S::NW * tt;
+                    if (t >= NTZ) continue;
+                    mwg_h8 vb = zero8;
+                    mwg_h4 vbe = zero4;
+                    const int col = 16 * t + lc;
+                    if (col < N2) split_chunk<N, true>(yt + col * kRowH, lq, vb, vbe);
+                    const mwg_float4 dz = split_mfma<N>(ov[2], oe[2], vb, vbe, mwg_float4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[tt][r] = fmaf(sigma, dz[r], acc[tt][r]);
+                }
+                if (more) {
+                    if ((nx >> 6) != (it >> 6)) load_f(nx >> 9, (nx >> 6) & 7);
+#pragma unroll
+                    for (int dim = 0; dim < 3; dim++) {
+                        ov[dim] = ovn[dim];
+                        oe[dim] = oen[dim];
+                        so[dim] = son[dim];
+                    }
+                }
+                it = nx;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0 && napps) {
+        atomicAdd(&a.counters[0], napps);
+        if (cls_counter > 0) atomicAdd(&a.counters[cls_counter], napps);
+    }
+    if (napps == 0) return;
+    // acc[tt][r] = Z[iz = 4 lq + r][col = 16 t + lc], added in FP64 to the output
+    double *dst = a.out_ns + (long long)j.g * 8 * N3 + (long long)j.gt * N3;
+#pragma unroll
+    for (int tt = 0; tt < TWZ; tt++) {
+        const int t = wv + S::NW * tt;
+        const int col = 16 * t + lc;
+        if (t >= NTZ || col >= N2) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int iz = 4 * lq + r;
+            if (iz < N) atomicAdd(&dst[iz * N2 + col], (double)acc[tt][r]);
+        }
+    }
+}
+
+// FP16 split rows of FP64 blocks (rows x N values each, row-major): one wave
+// per block; scale[b] = 2^e with max |v| < 2^e (0 for a zero block).
+template <int N>
+__global__ void k_split_rows(const double *src, long long nblocks, int rows, _Float16 *dst, double *scale) {
+    const long long b = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (b >= nblocks) return;
+    const double *s = src + b * (long long)rows * N;
+    double m = 0.0;
+    for (int e = lane; e < rows * N; e += 64) m = fmax(m, fabs(s[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    int ex = 0;
+    if (m > 0.0) (void)frexp(m, &ex);
+    const double inv = m > 0.0 ? ldexp(1.0, -ex) : 0.0;
+    if (lane == 0) scale[b] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
+    for (int r = lane; r < rows; r += 64) {
+        _Float16 *o = dst + (b * rows + r) * kRowH;
+        for (int jj = 0; jj < 12; jj++) {
+            const double v = jj < N ? s[r * N + jj] * inv : 0.0;
+            const _Float16 hi = (_Float16)v;
+            const _Float16 lo = (_Float16)(v - (double)hi);
+            o[jj < 8 ? jj : 8 + jj] = hi;       // hi[0..7] at 0..7, hi[8..11] at 16..19
+            o[jj < 8 ? 8 + jj : 12 + jj] = lo;  // lo[0..7] at 8..15, lo[8..11] at 20..23
+        }
+    }
+}
+
 __global__ void k_zero_f64(double *p, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -403,6 +734,8 @@ struct DevOper {
     long long *boff = nullptr, *noff = nullptr;
     double *blocks = nullptr, *norms = nullptr;
     float *blocks32 = nullptr;  // FP32 copy, made on first mixed-precision use
+    _Float16 *blocks_h = nullptr;  // FP16 split rows, made on first split use
+    double *blocks_hsc = nullptr;
     long long nblk = 0;
     int M = 0, ndepth = 0;
 };
@@ -421,6 +754,8 @@ struct DevicePlan {
     long long npool = 0;
     double *pool = nullptr;
     float *pool32 = nullptr;
+    _Float16 *pool_h = nullptr;  // FP16 split rows of the pool (split mode)
+    double *pool_hsc = nullptr;
     KArgs args{};
     std::vector<AppDesc> apps;          // host copy of args.apps
     AppDesc *d_apps = nullptr;
@@ -436,6 +771,8 @@ struct DevicePlan {
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (pool32) (void)hipFree(pool32);
+        if (pool_h) (void)hipFree(pool_h);
+        if (pool_hsc) (void)hipFree(pool_hsc);
         if (list_hi) (void)hipFree(list_hi);
         if (list_lo) (void)hipFree(list_lo);
     }
@@ -471,6 +808,8 @@ static void free_oper(DevOper &o) {
     if (o.blocks) (void)hipFree(o.blocks);
     if (o.norms) (void)hipFree(o.norms);
     if (o.blocks32) (void)hipFree(o.blocks32);
+    if (o.blocks_h) (void)hipFree(o.blocks_h);
+    if (o.blocks_hsc) (void)hipFree(o.blocks_hsc);
     o = DevOper();
 }
 
@@ -655,6 +994,8 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         ad.op_norms = o->norms;
         ad.blk64 = o->blocks;
         ad.blk32 = o->blocks32;
+        ad.blk_h = o->blocks_h;
+        ad.blk_hsc = o->blocks_hsc;
         ad.M = pp.flat->M;
         ad.ndepth = pp.flat->ndepth;
         ad.bw_off = o_bw;
@@ -834,6 +1175,101 @@ static int ensure_f32(DevicePlan *p, std::string &err) {
     return 0;
 }
 
+template <int N>
+static int split_rows_n(const double *src, long long nblocks, int rows, _Float16 **dst, double **scale,
+                        hipStream_t s, std::string &err) {
+    HIP_TRY(hipMalloc((void **)dst, sizeof(_Float16) * (size_t)std::max<long long>(nblocks * rows, 1) * kRowH));
+    HIP_TRY(hipMalloc((void **)scale, sizeof(double) * (size_t)std::max<long long>(nblocks, 1)));
+    if (nblocks > 0) {
+        hipLaunchKernelGGL(k_split_rows<N>, dim3((unsigned)nblocks), dim3(64), 0, s, src, nblocks, rows, *dst, *scale);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// FP16 split rows of FP64 blocks of `rows` x N values.
+static int split_rows(int N, const double *src, long long nblocks, int rows, _Float16 **dst, double **scale,
+                      hipStream_t s, std::string &err) {
+    if (nblocks >= (1LL << 31)) {
+        err = "too many blocks for the FP16 split conversion";
+        return -1;
+    }
+    switch (N) {
+        case 2: return split_rows_n<2>(src, nblocks, rows, dst, scale, s, err);
+        case 3: return split_rows_n<3>(src, nblocks, rows, dst, scale, s, err);
+        case 4: return split_rows_n<4>(src, nblocks, rows, dst, scale, s, err);
+        case 5: return split_rows_n<5>(src, nblocks, rows, dst, scale, s, err);
+        case 6: return split_rows_n<6>(src, nblocks, rows, dst, scale, s, err);
+        case 7: return split_rows_n<7>(src, nblocks, rows, dst, scale, s, err);
+        case 8: return split_rows_n<8>(src, nblocks, rows, dst, scale, s, err);
+        case 9: return split_rows_n<9>(src, nblocks, rows, dst, scale, s, err);
+        case 10: return split_rows_n<10>(src, nblocks, rows, dst, scale, s, err);
+        case 11: return split_rows_n<11>(src, nblocks, rows, dst, scale, s, err);
+        case 12: return split_rows_n<12>(src, nblocks, rows, dst, scale, s, err);
+        default: err = "unsupported basis size"; return -1;
+    }
+}
+
+// FP16 split copies of the pool and of every part's operator blocks.
+static int ensure_h16(DevicePlan *p, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    const int N = p->kp1;
+    bool changed = false;
+    for (size_t i = 0; i < p->op_keys.size(); i++) {
+        auto it = c->opers.find(p->op_keys[i]);
+        if (it == c->opers.end()) {
+            err = "operator tables not resident (operator destroyed while a plan uses it)";
+            return -1;
+        }
+        DevOper &o = it->second;
+        if (!o.blocks_h) {
+            int rc = split_rows(N, o.blocks, o.nblk / ((long long)N * N), N, &o.blocks_h, &o.blocks_hsc, c->stream, err);
+            if (rc) return rc;
+        }
+        if (p->apps[i].blk_h != o.blocks_h) {
+            p->apps[i].blk_h = o.blocks_h;
+            p->apps[i].blk_hsc = o.blocks_hsc;
+            changed = true;
+        }
+    }
+    if (changed) {
+        int rc = upload_at(p->d_apps, p->apps, c->stream, err);
+        if (rc) return rc;
+    }
+    if (!p->pool_h)
+        return split_rows(N, p->pool, p->npool / ((long long)N * N * N), N * N, &p->pool_h, &p->pool_hsc, c->stream,
+                          err);
+    return 0;
+}
+
+template <int N>
+static int launch_h_n(const DevicePlan *p, const int *list, int n_seg, double lo, double hi, int cls,
+                      std::string &err) {
+    if (n_seg == 0) return 0;
+    const size_t shmem = sizeof(double) * 64 * 4 * (2 * p->wmax + 1);
+    hipLaunchKernelGGL((k_apply_h<N>), dim3(n_seg), dim3(SplitShape<N>::T), shmem, p->ctx->stream, p->args,
+                       p->pool_h, p->pool_hsc, list, lo, hi, cls);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int launch_h(const DevicePlan *p, const int *list, int n_seg, double lo, double hi, int cls, std::string &err) {
+    switch (p->kp1) {
+        case 2: return launch_h_n<2>(p, list, n_seg, lo, hi, cls, err);
+        case 3: return launch_h_n<3>(p, list, n_seg, lo, hi, cls, err);
+        case 4: return launch_h_n<4>(p, list, n_seg, lo, hi, cls, err);
+        case 5: return launch_h_n<5>(p, list, n_seg, lo, hi, cls, err);
+        case 6: return launch_h_n<6>(p, list, n_seg, lo, hi, cls, err);
+        case 7: return launch_h_n<7>(p, list, n_seg, lo, hi, cls, err);
+        case 8: return launch_h_n<8>(p, list, n_seg, lo, hi, cls, err);
+        case 9: return launch_h_n<9>(p, list, n_seg, lo, hi, cls, err);
+        case 10: return launch_h_n<10>(p, list, n_seg, lo, hi, cls, err);
+        case 11: return launch_h_n<11>(p, list, n_seg, lo, hi, cls, err);
+        case 12: return launch_h_n<12>(p, list, n_seg, lo, hi, cls, err);
+        default: err = "unsupported basis size"; return -1;
+    }
+}
+
 int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
                  unsigned long long *lp_apps, std::string &err, double *split_ms) {
     DeviceCtx *c = p->ctx;
@@ -843,16 +1279,16 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     if (term_apps) *term_apps = 0;
     if (lp_apps) *lp_apps = 0;
     if (p->n_out == 0) return 0;
-    if (mode != 0 && mode != 1) {
+    if (mode < 0 || mode > 2) {
         err = "unknown arithmetic mode";
         return -1;
     }
-    if (mode == 1 && !(tau >= 1.0)) {
+    if (mode != 0 && !(tau >= 1.0)) {
         err = "mixed-precision ratio must be >= 1";
         return -1;
     }
-    if (mode == 1) {
-        int rc = ensure_f32(p, err);
+    if (mode != 0) {
+        int rc = mode == 1 ? ensure_f32(p, err) : ensure_h16(p, err);
         if (!rc) rc = make_lists(p, tau, err);
         if (rc) return rc;
     }
@@ -875,7 +1311,10 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     } else {
         rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, err);
         if (!rc) HIP_TRY(hipEventRecord(c->evm, c->stream));
-        if (!rc) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
+        if (!rc) {
+            if (mode == 1) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
+            else rc = launch_h(p, p->list_lo, p->n_lo, 0.0, tau, 1, err);
+        }
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));

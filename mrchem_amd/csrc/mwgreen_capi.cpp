@@ -293,7 +293,8 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
 int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *op, const mwg_tree *inp,
               const double *const *thr, int mode, mwg_plan *plan) {
     if (!ctx || !ctx->dev || n < 1 || !out || !op || !inp || !thr || !plan) return fail(MWG_ERR_ARG, "null argument");
-    if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+    if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED && mode != MWG_MODE_SPLIT16)
+        return fail(MWG_ERR_ARG, "unknown arithmetic mode");
     std::vector<Job> jobs(n);
     for (int i = 0; i < n; i++) {
         if (!out[i] || !op[i] || !inp[i] || !thr[i]) return fail(MWG_ERR_ARG, "null handle in batch");
@@ -793,7 +794,8 @@ int mwg_plan_destroy(mwg_plan plan) {
 
 int mwg_context_set_precision(mwg_context ctx, int mode, double tau) {
     if (!ctx) return fail(MWG_ERR_ARG, "null context");
-    if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED) return fail(MWG_ERR_ARG, "unknown arithmetic mode");
+    if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED && mode != MWG_MODE_SPLIT16)
+        return fail(MWG_ERR_ARG, "unknown arithmetic mode");
     if (!(tau >= 1.0) || !std::isfinite(tau)) return fail(MWG_ERR_ARG, "mixed-precision ratio must be finite and >= 1");
     ctx->mode = mode;
     ctx->tau = tau;

@@ -109,3 +109,39 @@ def test_mixed_precision_term_set_and_error(ctx, order):
     c64 = V.nodes()[2]
     assert np.max(np.abs(ref - c64)) / np.max(np.abs(ref)) < TOL_FP64
     assert st["term_apps"] == r64["term_apps"]
+
+
+@pytest.mark.parametrize("order,prec", [(4, 1e-5), (7, 1e-6), (9, 1e-6), (11, 1e-5)])
+def test_split16_term_set_and_error(ctx, order, prec):
+    """FP16 hi/lo split products: same term set as FP64 and within prec of
+    the FP64 result on the identical grid (N <= 8, N = 9..10 and N = 11..12
+    take different operand layouts)."""
+    mra = _mra(order)
+    P = mw.PoissonOperator(mra, prec)
+    rng = np.random.default_rng(23)
+    n = 4
+    coef = rng.normal(size=n)
+    exps = np.exp(rng.uniform(np.log(1.0), np.log(60.0), size=n))
+    cen = rng.uniform(-1.0, 1.0, size=(n, 3))
+    rho = mw.FunctionTree(mra).project_gaussians(prec, coef, exps, cen)
+    V = mw.FunctionTree(mra)
+    mw.apply(prec / rho.norm(), V, P, rho, ctx=ctx)
+    thr = V.thresholds()
+    od, ol, _ = V.nodes()
+    V2 = mw.FunctionTree(mra).set_nodes(od, ol)
+    p64 = mw.ApplyPlan(V, P, rho, thr, mode="fp64", ctx=ctx)
+    psp = mw.ApplyPlan(V2, P, rho, thr, mode="split16", ctx=ctx)
+    r64 = p64.execute()
+    rsp = psp.execute()
+    assert r64["term_apps"] == rsp["term_apps"]
+    assert 0 < rsp["lp_term_apps"] < rsp["term_apps"]
+    ns64 = p64.download_ns()
+    nssp = psp.download_ns()
+    err = np.max(np.abs(nssp - ns64)) / np.max(np.abs(ns64))
+    print("order %d: %d of %d term-apps as FP16 split products, max coef err vs FP64 %.2e" %
+          (order, rsp["lp_term_apps"], rsp["term_apps"], err))
+    assert err < prec
+    p64.finish()
+    psp.finish()
+    E64, Esp = rho.dot(V), rho.dot(V2)
+    assert abs(Esp - E64) < prec * max(1.0, abs(E64))
