@@ -38,6 +38,7 @@ struct AppDesc {
     const float *blk32;               // FP32 copy (mixed mode), or null
     const _Float16 *blk_h;            // FP16 split rows (split mode), or null
     const double *blk_hsc;            // their per-block scales
+    long long blk_hzero;              // offset (halfs) of an all-zero row in blk_h
     int M, ndepth;
     long long bw_off;                 // start of this apply in bw
     int act_base;                     // start of this apply in act_off
@@ -431,18 +432,22 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
 // ------------------------------------------------------ FP16 split products
 //
 // Each operand value v of a block, scaled into (-1, 1) by the block's
-// power-of-two scale, is split as v = hi + lo (both FP16, lo = v - hi rounded).
-// A row of up to 12 values is stored as 24 halfs, 48 bytes:
-//   hi[0..7] | lo[0..7] | hi[8..11] lo[8..11]
-// so that a lane reads any 8-value chunk with one 16-byte load.  A product is
-// hi*hi + lo*hi + hi*lo (the lo*lo term is below the representation error):
-// the data side supplies K chunks [hi | lo | hi | tail] and the operator side
-// [hi | hi | lo | tail], one v_mfma_f32_16x16x32_f16 per 16x16 tile for
-// j < 8 plus the tail j = 8, 9 (N <= 10); for N = 11, 12 the tail carries
-// hi*hi + lo*hi of j = 8..11 and a v_mfma_f32_16x16x16f16 adds hi*lo of them.
+// power-of-two scale, is split as v = hi + lo (both FP16, lo = v - hi
+// rounded).  A product is formed as hi*hi + lo*hi + hi*lo (lo*lo is below the
+// representation error) by one v_mfma_f32_16x16x32_f16 per 16x16 tile: lane
+// chunk c = lane / 16 holds K slots 8c..8c+7, the data side supplies
+//   [hi 0..7 | lo 0..7 | hi 0..7 | tail]
+// and the operator side
+//   [hi 0..7 | hi 0..7 | lo 0..7 | tail]
+// where the tails carry the same three products for j = 8, 9 (N <= 10:
+// data {hi8 hi9 lo8 lo9 hi8 hi9 0 0}, operator {hi8 hi9 hi8 hi9 lo8 lo9 0 0})
+// or hi*hi + lo*hi for j = 8..11 (N = 11, 12: data {hi8..11 lo8..11},
+// operator {hi8..11 hi8..11}); for N = 11, 12 a fifth chunk holds hi8..11
+// (data) / lo8..11 (operator) for a v_mfma_f32_16x16x16f16 in lane chunk 0.
+// Rows are stored in exactly this chunk order (16 bytes per chunk), so every
+// lane loads its operand with one 16-byte read and no selects.
 typedef _Float16 mwg_h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 mwg_h4 __attribute__((ext_vector_type(4)));
-constexpr int kRowH = 24;
 
 template <int N>
 struct SplitShape {
@@ -456,32 +461,61 @@ struct SplitShape {
     static constexpr int TWG = (NTG + NW - 1) / NW;
     static constexpr int TWZ = (NTZ + NW - 1) / NW;
     static constexpr bool TWO = N > 10;
+    static constexpr int NC = TWO ? 5 : 4;      // 16-byte chunks per row
+    static constexpr int RH = 8 * NC;           // halfs per row
 };
 
-// Lane chunk c (= lane / 16) of a split row: data side (DATA) or operator
-// side; ext is the operand of the second MFMA (N > 10), in chunk 0 only.
+// Split row of n <= 12 values u (|u| < 1): data side (DATA) or operator side.
 template <int N, bool DATA>
-__device__ __forceinline__ void split_chunk(const _Float16 *row, int c, mwg_h8 &v, mwg_h4 &ext) {
-    const mwg_h8 *r8 = reinterpret_cast<const mwg_h8 *>(row);
+__device__ __forceinline__ void split_row(const double *u, _Float16 *row) {
+    _Float16 hi[12], lo[12];
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+        const double v = j < N ? u[j] : 0.0;
+        hi[j] = (_Float16)v;
+        lo[j] = (_Float16)(v - (double)hi[j]);
+    }
     const _Float16 z = (_Float16)0.0f;
-    ext = mwg_h4{z, z, z, z};
-    if (c < 3) {
-        v = r8[(DATA ? (c == 1) : (c == 2)) ? 1 : 0];
-        if (SplitShape<N>::TWO && c == 0) {
-            const mwg_h8 t = r8[2];
-            ext = DATA ? mwg_h4{t[0], t[1], t[2], t[3]} : mwg_h4{t[4], t[5], t[6], t[7]};
-        }
-        return;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        row[j] = hi[j];
+        row[8 + j] = DATA ? lo[j] : hi[j];
+        row[16 + j] = DATA ? hi[j] : lo[j];
     }
+    _Float16 t[8];
     if (N <= 8) {
-        v = mwg_h8{z, z, z, z, z, z, z, z};
-        return;
+        for (int e = 0; e < 8; e++) t[e] = z;
+    } else if (N <= 10) {
+        const _Float16 td[8] = {hi[8], hi[9], lo[8], lo[9], hi[8], hi[9], z, z};
+        const _Float16 to[8] = {hi[8], hi[9], hi[8], hi[9], lo[8], lo[9], z, z};
+        for (int e = 0; e < 8; e++) t[e] = DATA ? td[e] : to[e];
+    } else {
+        for (int e = 0; e < 4; e++) {
+            t[e] = hi[8 + e];
+            t[4 + e] = DATA ? lo[8 + e] : hi[8 + e];
+        }
     }
-    const mwg_h8 t = r8[2];  // hi8..11, lo8..11
-    if (N <= 10)
-        v = DATA ? mwg_h8{t[0], t[1], t[4], t[5], t[0], t[1], z, z} : mwg_h8{t[0], t[1], t[0], t[1], t[4], t[5], z, z};
-    else
-        v = DATA ? t : mwg_h8{t[0], t[1], t[2], t[3], t[0], t[1], t[2], t[3]};
+#pragma unroll
+    for (int e = 0; e < 8; e++) row[24 + e] = t[e];
+    if (SplitShape<N>::TWO) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            row[32 + e] = DATA ? hi[8 + e] : lo[8 + e];
+            row[36 + e] = z;
+        }
+    }
+}
+
+// Operand of lane chunk c from a split row (ext: fifth chunk, in chunk 0).
+template <int N>
+__device__ __forceinline__ void split_operand(const _Float16 *row, int c, mwg_h8 &v, mwg_h4 &ext) {
+    const mwg_h8 *r8 = reinterpret_cast<const mwg_h8 *>(row);
+    v = r8[c];
+    if (SplitShape<N>::TWO) {
+        const _Float16 z = (_Float16)0.0f;
+        const mwg_h8 e = r8[4];
+        ext = c == 0 ? mwg_h4{e[0], e[1], e[2], e[3]} : mwg_h4{z, z, z, z};
+    }
 }
 
 // The second MFMA gets its own zero accumulator: issued directly on the
@@ -489,9 +523,8 @@ __device__ __forceinline__ void split_chunk(const _Float16 *row, int c, mwg_h8 &
 // them in the generated code) it gave wrong sums on gfx950, while separate
 // accumulators added afterwards are exact.
 template <int N>
-__device__ __forceinline__ mwg_float4 split_mfma(const mwg_h8 &a, const mwg_h4 &ae, const mwg_h8 &b, const mwg_h4 &be,
-                                                 mwg_float4 c) {
-    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+__device__ __forceinline__ mwg_float4 split_mfma(const mwg_h8 &a, const mwg_h4 &ae, const mwg_h8 &b, const mwg_h4 &be) {
+    mwg_float4 c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, mwg_float4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     if (SplitShape<N>::TWO) {
         const mwg_float4 e = __builtin_amdgcn_mfma_f32_16x16x16f16(ae, be, mwg_float4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 #pragma unroll
@@ -500,37 +533,51 @@ __device__ __forceinline__ mwg_float4 split_mfma(const mwg_h8 &a, const mwg_h4 &
     return c;
 }
 
-// Store four consecutive values (positions j0..j0+3 of a row) as hi and lo.
-__device__ __forceinline__ void split_store4(_Float16 *row, int j0, const mwg_float4 &v) {
-    mwg_h4 hi, lo;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        hi[r] = (_Float16)v[r];
-        lo[r] = (_Float16)(v[r] - (float)hi[r]);
-    }
-    *reinterpret_cast<mwg_h4 *>(row + (j0 < 8 ? j0 : 16)) = hi;
-    *reinterpret_cast<mwg_h4 *>(row + (j0 < 8 ? 8 + j0 : 20)) = lo;
+// Global-memory view of a pointer, so that loads through it are global, not
+// flat, loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
 }
 
-// Per-item passes with FP16 split operands (the job, the screening and the
-// worklist are those of k_apply).  With unit-scaled operands F_u, O_u:
+// Worklist item of the split kernel: the three operator block ids (block =
+// N x N values of one term, depth, component and translation) and the input
+// group (band position << 3 | component), bit kSingleBit set when the item's
+// intermediates are rounded once (no lo part).
+struct SplitItem {
+    int bx, by, bz, grp;
+};
+constexpr int kSingleBit = 30;
+
+// Per-item passes with FP16 split operands.  The job, the work segment and
+// the screening rule are those of k_apply; the screening lane (one term)
+// also resolves the item's three operator block ids.  With unit-scaled
+// operands F_u, O_u:
 //   x: X[(jz,jy)][ix] = F_u . Ox_u^T, stored transposed as Xt[(ix,jz)][jy]
 //   y: Y[(ix,jz)][iy] = Xt . Oy_u^T,  stored as Yt[(iy,ix)][jz]
 //   z: Z[iz][(iy,ix)] = Oz_u . Yt^T,  acc += sF sOx sOy sOz Z  (FP32)
 // The rows of the x and y passes are taken in groups of four consecutive
 // inner indices of one outer line, so every lane stores its four results
-// with two 8-byte writes.  Intermediates are split again before the next
-// pass; their magnitudes stay below N^2, well inside the FP16 range.
+// with aligned 8-byte writes.  Intermediates are split again (hi: the value
+// cut to an 11-bit mantissa, lo: the exact FP32 remainder, both packed to
+// FP16) before the next pass; their magnitudes stay below N^2.  Items whose
+// bound is at most single_mult * thr_cls keep the intermediates rounded
+// once (lo = 0): their arithmetic error stays below ~1e-3 of the bound.
+// Lanes without a row to read take an all-zero row, lanes without a row to
+// write write a scratch row, so no pass branches on the lane.
+constexpr int kSplitCap = 640;  // worklist items per batch (>= 8 * 64 + 128)
+
 template <int N>
 __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
-                                                             const int *seg_list, double lo_mult, double hi_mult,
-                                                             int cls_counter) {
+                                                             long long pool_zero, const int *seg_list, double lo_mult,
+                                                             double hi_mult, double single_mult, int cls_counter) {
     using S = SplitShape<N>;
-    constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTG = S::NTG, NTZ = S::NTZ;
-    constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T;
-    __shared__ __align__(16) _Float16 xt[N2 * kRowH];
-    __shared__ __align__(16) _Float16 yt[N2 * kRowH];
-    __shared__ int wl[kWorkCap];
+    constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTZ = S::NTZ;
+    constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, NW = S::NW;
+    constexpr int ZROW = N2, DROW = N2 + 1;  // all-zero row, scratch row
+    __shared__ __align__(16) _Float16 xt[(N2 + 2) * RH];
+    __shared__ __align__(16) _Float16 yt[(N2 + 2) * RH];
+    __shared__ __align__(16) SplitItem wl[kSplitCap];
     __shared__ int ltab[64];
     __shared__ long long btab[64];
     extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
@@ -541,49 +588,146 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
     const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
     const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
+    const double thr_single = (j.thr > ad.thr_floor ? j.thr : ad.thr_floor) * single_mult;
     const _Float16 z16 = (_Float16)0.0f;
     const mwg_h8 zero8 = {z16, z16, z16, z16, z16, z16, z16, z16};
     const mwg_h4 zero4 = {z16, z16, z16, z16};
+    // rows and chunks the passes never write are read as operands: zeros
+    for (int e = tid; e < (N2 + 2) * RH / 8; e += NTH) {
+        reinterpret_cast<mwg_h8 *>(xt)[e] = zero8;
+        reinterpret_cast<mwg_h8 *>(yt)[e] = zero8;
+    }
+    const __attribute__((address_space(1))) _Float16 *opg = gmem(ad.blk_h);
+    const __attribute__((address_space(1))) double *opsc = gmem(ad.blk_hsc);
+    const __attribute__((address_space(1))) _Float16 *poolg = gmem(pool_h);
+
+    // Per-lane LDS offsets (halfs) of each tile: x-pass store row and the
+    // position of its four values, y-pass load, y-pass store, z-pass load.
+    int xs_row[TWG], xs_j0[TWG], yl_off[TWG], ys_row[TWG], ys_j0[TWG], zl_off[TWZ];
+#pragma unroll
+    for (int tt = 0; tt < TWG; tt++) {
+        const int t = wv + NW * tt;
+        const int g4 = 4 * t + lq;  // group of this lane's four results
+        const bool okx = g4 < NG && lc < N;
+        xs_row[tt] = (okx ? lc * N + g4 / G : DROW) * RH;
+        xs_j0[tt] = 4 * (g4 % G);
+        ys_row[tt] = xs_row[tt];  // same (lane, group) layout: (iy, ix) rows of Yt
+        ys_j0[tt] = xs_j0[tt];
+        const int g5 = 4 * t + (lc >> 2), jz = 4 * (g5 % G) + (lc & 3);  // A row of the y pass
+        yl_off[tt] = ((g5 < NG && jz < N) ? (g5 / G) * N + jz : ZROW) * RH;
+    }
+#pragma unroll
+    for (int tt = 0; tt < TWZ; tt++) {
+        const int col = 16 * (wv + NW * tt) + lc;
+        zl_off[tt] = (col < N2 ? col : ZROW) * RH;
+    }
 
     mwg_float4 acc[TWZ];
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) acc[tt] = mwg_float4{0.f, 0.f, 0.f, 0.f};
     unsigned long long napps = 0;
 
+    // operand of lane chunk lq from a split row at `row` (halfs); ext from
+    // the fifth chunk for lanes of chunk 0 (N > 10), else from `zrow`
+    auto lds_operand = [&](const _Float16 *base, int row, mwg_h8 &v, mwg_h4 &e) {
+        v = *reinterpret_cast<const mwg_h8 *>(base + row + 8 * lq);
+        if (S::TWO) e = *reinterpret_cast<const mwg_h4 *>(base + (lq == 0 ? row + 32 : ZROW * RH));
+    };
+    auto g_operand = [&](const __attribute__((address_space(1))) _Float16 *row, bool ok, mwg_h8 &v, mwg_h4 &e) {
+        v = zero8;
+        e = zero4;
+        if (ok) {
+            v = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(row + 8 * lq);
+            if (S::TWO && lq == 0) e = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h4 *>(row + 32);
+        }
+    };
+    // four results of one tile into a data-side split row
+    auto store4 = [&](_Float16 *base, int row, int j0, const mwg_float4 &v, bool single) {
+        mwg_h4 hi, lo;
+        if (single) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) hi[r] = (_Float16)v[r];
+            lo = zero4;
+        } else {
+            float h[4], l[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                h[r] = __uint_as_float(__float_as_uint(v[r]) & 0xffffe000u);
+                l[r] = v[r] - h[r];
+            }
+            hi = mwg_h4{(_Float16)h[0], (_Float16)h[1], (_Float16)h[2], (_Float16)h[3]};
+            lo = mwg_h4{(_Float16)l[0], (_Float16)l[1], (_Float16)l[2], (_Float16)l[3]};
+        }
+        // j0 < 8: hi -> chunk 0, lo -> chunk 1, hi -> chunk 2
+        // j0 = 8, N <= 10: chunk 3 = {hi0 hi1 lo0 lo1 | hi0 hi1 0 0}
+        // j0 = 8, N > 10: chunk 3 = {hi0..3 | lo0..3}, chunk 4 = {hi0..3 | -}
+        const bool tail = j0 >= 8;
+        mwg_h4 d1 = hi, d2 = lo;
+        int o1 = row + j0, o2 = row + 8 + j0, o3 = row + 16 + j0;
+        if (N > 8 && N <= 10) {
+            d1 = tail ? mwg_h4{hi[0], hi[1], lo[0], lo[1]} : hi;
+            d2 = tail ? mwg_h4{hi[0], hi[1], z16, z16} : lo;
+            o1 = tail ? row + 24 : o1;
+            o2 = tail ? row + 28 : o2;
+            o3 = tail ? DROW * RH : o3;
+        } else if (N > 10) {
+            o1 = tail ? row + 24 : o1;
+            o2 = tail ? row + 28 : o2;
+            o3 = tail ? row + 32 : o3;
+        }
+        *reinterpret_cast<mwg_h4 *>(base + o1) = d1;
+        *reinterpret_cast<mwg_h4 *>(base + o2) = d2;
+        *reinterpret_cast<mwg_h4 *>(base + o3) = hi;
+    };
+
     // x-pass A operand (data side): rows (jz, jy) of the F block, in groups
     mwg_h8 fa[TWG];
     mwg_h4 fe[TWG];
     double sF = 0.0;
-    auto load_f = [&](int p, int ft) {
-        const int s = a.band[j.boff + p];
-        const long long cb = a.slot_off[s] / N3 + ft;  // component block of the pool
+    auto load_f = [&](int grp) {
+        const int s = a.band[j.boff + (grp >> 3)];
+        const long long cb = a.slot_off[s] / N3 + (grp & 7);  // component block of the pool
         sF = pool_sc[cb];
-        const _Float16 *base = pool_h + cb * (long long)(N2 * kRowH);
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
-            const int t = wv + S::NW * tt;
-            const int grp = 4 * t + (lc >> 2), jy = 4 * (grp % G) + (lc & 3);
-            if (t < NTG && grp < NG && jy < N) {
-                split_chunk<N, true>(base + ((grp / G) * N + jy) * kRowH, lq, fa[tt], fe[tt]);
-            } else {
-                fa[tt] = zero8;
-                fe[tt] = zero4;
-            }
+            const int t = wv + NW * tt;
+            const int g4 = 4 * t + (lc >> 2), jy = 4 * (g4 % G) + (lc & 3);
+            const bool ok = g4 < NG && jy < N;
+            g_operand(poolg + (ok ? cb * (long long)(N2 * RH) + ((g4 / G) * N + jy) * RH : pool_zero), true, fa[tt],
+                      fe[tt]);
         }
     };
-    // operator rows of item `it` along dim: row lc of the block, chunk lq
-    auto load_op = [&](int it, int dim, mwg_h8 &v, mwg_h4 &e, double &sc) {
-        const int p = it >> 9, ft = (it >> 6) & 7, b = it & 63;
-        const int l = (dim == 0 ? p % j.side : (dim == 1 ? (p / j.side) % j.side : p / (j.side * j.side))) - j.w;
-        const int Lb = ltab[b];
-        const int c = (j.gb[dim] << 1) | ((ft >> dim) & 1);
-        const long long blk = btab[b] / N2 + (long long)c * (2 * Lb + 1) + (l + Lb);
-        sc = ad.blk_hsc[blk];
-        if (lc < N) {
-            split_chunk<N, false>(ad.blk_h + blk * (long long)(N * kRowH) + lc * kRowH, lq, v, e);
-        } else {
-            v = zero8;
-            e = zero4;
+    // operator row lc of block blk (uniform), chunk lq
+    auto load_op = [&](int blk, mwg_h8 &v, mwg_h4 &e, double &sc) {
+        sc = opsc[blk];
+        g_operand(opg + (lc < N ? blk * (long long)(N * RH) + lc * RH : ad.blk_hzero), true, v, e);
+    };
+    // one item with operators (o, oe, so); the F tiles are current
+    auto apply_item = [&](const mwg_h8 (&o)[3], const mwg_h4 (&oe)[3], const double (&so)[3], bool single) {
+#pragma unroll
+        for (int tt = 0; tt < TWG; tt++) {
+            const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], o[0], oe[0]);
+            store4(xt, xs_row[tt], xs_j0[tt], dx, single);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < TWG; tt++) {
+            mwg_h8 va;
+            mwg_h4 ve = zero4;
+            lds_operand(xt, yl_off[tt], va, ve);
+            const mwg_float4 dy = split_mfma<N>(va, ve, o[1], oe[1]);
+            store4(yt, ys_row[tt], ys_j0[tt], dy, single);
+        }
+        __syncthreads();
+        const float sigma = (float)(sF * so[0] * so[1] * so[2]);
+#pragma unroll
+        for (int tt = 0; tt < TWZ; tt++) {
+            mwg_h8 vb;
+            mwg_h4 vbe = zero4;
+            lds_operand(yt, zl_off[tt], vb, vbe);
+            const mwg_float4 dz = split_mfma<N>(o[2], oe[2], vb, vbe);
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[tt][r] = fmaf(sigma, dz[r], acc[tt][r]);
         }
     };
 
@@ -592,74 +736,87 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         __syncthreads();
         stage_terms(a, ad, j, base, cnt, ltab, btab, nrm_s, tid, NTH);
         __syncthreads();
+        // this lane's term: operator block base and half-width
+        const int my_L = ltab[lane];
+        const int my_b0 = (int)(btab[lane] / N2) + my_L;
         int p = p_begin;
         while (p < p_end) {
-            const int n = screen_items(a, j, p, p_end, cnt, nrm_s, wl, kWorkCap, lane, wv == 0);
+            // screening (the rule of screen_items) into SplitItems
+            const bool have = lane < cnt;
+            int n = 0;
+            for (; p < p_end && n + 8 * 64 <= kSplitCap; p++) {
+                const int s = a.band[j.boff + p];
+                if (s < 0) continue;
+                const int ll[3] = {p % j.side - j.w, (p / j.side) % j.side - j.w, p / (j.side * j.side) - j.w};
+                double nrm[2][3];
+#pragma unroll
+                for (int dim = 0; dim < 3; dim++)
+#pragma unroll
+                    for (int fb = 0; fb < 2; fb++)
+                        nrm[fb][dim] = nrm_s[(lane * 4 + (2 * j.gb[dim] + fb)) * j.nl + (ll[dim] + j.w)];
+                const double *fnp = a.slot_norm + (long long)s * 8;
+                for (int ft = 0; ft < 8; ft++) {
+                    if (j.d > 0 && ft == 0 && j.gt == 0) continue;
+                    const double fn = fnp[ft];
+                    if (!(fn > 0.0)) continue;
+                    const double bound = ((nrm[ft & 1][0] * nrm[(ft >> 1) & 1][1]) * nrm[(ft >> 2) & 1][2]) * fn;
+                    const bool take = have && bound > j.thr && bound > j.thr_lo && !(bound > j.thr_hi);
+                    const unsigned long long mask = __ballot(take);
+                    if (take && wv == 0) {
+                        const int pos = n + __popcll(mask & ((1ull << lane) - 1ull));
+                        SplitItem itm;
+                        itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * (2 * my_L + 1)) + ll[0];
+                        itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * (2 * my_L + 1)) + ll[1];
+                        itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + ll[2];
+                        itm.grp = (p << 3) | ft | ((bound <= thr_single ? 1 : 0) << kSingleBit);
+                        wl[pos] = itm;
+                    }
+                    n += __popcll(mask);
+                }
+            }
             __syncthreads();
             if (n == 0) continue;
             napps += n;
-            mwg_h8 ov[3], ovn[3];
-            mwg_h4 oe[3], oen[3];
-            double so[3], son[3];
-            int it = wl[0];
-            load_f(it >> 9, (it >> 6) & 7);
-#pragma unroll
-            for (int dim = 0; dim < 3; dim++) load_op(it, dim, ov[dim], oe[dim], so[dim]);
-            for (int i = 0; i < n; i++) {
-                const bool more = i + 1 < n;
-                const int nx = more ? wl[i + 1] : it;
-                if (more) {
-#pragma unroll
-                    for (int dim = 0; dim < 3; dim++) load_op(nx, dim, ovn[dim], oen[dim], son[dim]);
+            // two operator register sets, used in turn, so that the next
+            // item's loads are in flight while this one computes
+            mwg_h8 oA[3], oB[3];
+            mwg_h4 eA[3], eB[3];
+            double sA[3], sB[3];
+            const int mask_grp = (1 << kSingleBit) - 1;
+            int cur_grp = __builtin_amdgcn_readfirstlane(wl[0].grp);
+            load_f(cur_grp & mask_grp);
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), oA[0], eA[0], sA[0]);
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].by), oA[1], eA[1], sA[1]);
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].bz), oA[2], eA[2], sA[2]);
+            for (int i = 0; i < n; i += 2) {
+                // item i on set A, prefetch i + 1 into set B
+                const bool has1 = i + 1 < n;
+                int grp1 = cur_grp;
+                if (has1) {
+                    const SplitItem nx = wl[i + 1];
+                    grp1 = __builtin_amdgcn_readfirstlane(nx.grp);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oB[0], eB[0], sB[0]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oB[1], eB[1], sB[1]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oB[2], eB[2], sB[2]);
                 }
-                // x pass -> Xt[(ix, jz)][jy]
-#pragma unroll
-                for (int tt = 0; tt < TWG; tt++) {
-                    const int t = wv + S::NW * tt;
-                    if (t >= NTG) continue;
-                    const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], ov[0], oe[0], mwg_float4{0.f, 0.f, 0.f, 0.f});
-                    const int grp = 4 * t + lq;
-                    if (grp < NG && lc < N) split_store4(xt + (lc * N + grp / G) * kRowH, 4 * (grp % G), dx);
+                apply_item(oA, eA, sA, (cur_grp >> kSingleBit) & 1);
+                if (!has1) break;
+                if ((grp1 & mask_grp) != (cur_grp & mask_grp)) load_f(grp1 & mask_grp);
+                cur_grp = grp1;
+                // item i + 1 on set B, prefetch i + 2 into set A
+                const bool has2 = i + 2 < n;
+                int grp2 = cur_grp;
+                if (has2) {
+                    const SplitItem nx = wl[i + 2];
+                    grp2 = __builtin_amdgcn_readfirstlane(nx.grp);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oA[0], eA[0], sA[0]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oA[1], eA[1], sA[1]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oA[2], eA[2], sA[2]);
                 }
-                __syncthreads();
-                // y pass: rows (ix, jz) of Xt in groups -> Yt[(iy, ix)][jz]
-#pragma unroll
-                for (int tt = 0; tt < TWG; tt++) {
-                    const int t = wv + S::NW * tt;
-                    if (t >= NTG) continue;
-                    mwg_h8 va = zero8;
-                    mwg_h4 ve = zero4;
-                    const int grp = 4 * t + (lc >> 2), jz = 4 * (grp % G) + (lc & 3);
-                    if (grp < NG && jz < N) split_chunk<N, true>(xt + ((grp / G) * N + jz) * kRowH, lq, va, ve);
-                    const mwg_float4 dy = split_mfma<N>(va, ve, ov[1], oe[1], mwg_float4{0.f, 0.f, 0.f, 0.f});
-                    const int grp2 = 4 * t + lq;
-                    if (grp2 < NG && lc < N) split_store4(yt + (lc * N + grp2 / G) * kRowH, 4 * (grp2 % G), dy);
-                }
-                __syncthreads();
-                // z pass: acc[iz][(iy, ix)] += sigma Oz_u . Yt^T
-                const float sigma = (float)(sF * so[0] * so[1] * so[2]);
-#pragma unroll
-                for (int tt = 0; tt < TWZ; tt++) {
-                    const int t = wv + S::NW * tt;
-                    if (t >= NTZ) continue;
-                    mwg_h8 vb = zero8;
-                    mwg_h4 vbe = zero4;
-                    const int col = 16 * t + lc;
-                    if (col < N2) split_chunk<N, true>(yt + col * kRowH, lq, vb, vbe);
-                    const mwg_float4 dz = split_mfma<N>(ov[2], oe[2], vb, vbe, mwg_float4{0.f, 0.f, 0.f, 0.f});
-#pragma unroll
-                    for (int r = 0; r < 4; r++) acc[tt][r] = fmaf(sigma, dz[r], acc[tt][r]);
-                }
-                if (more) {
-                    if ((nx >> 6) != (it >> 6)) load_f(nx >> 9, (nx >> 6) & 7);
-#pragma unroll
-                    for (int dim = 0; dim < 3; dim++) {
-                        ov[dim] = ovn[dim];
-                        oe[dim] = oen[dim];
-                        so[dim] = son[dim];
-                    }
-                }
-                it = nx;
+                apply_item(oB, eB, sB, (cur_grp >> kSingleBit) & 1);
+                if (!has2) break;
+                if ((grp2 & mask_grp) != (cur_grp & mask_grp)) load_f(grp2 & mask_grp);
+                cur_grp = grp2;
             }
             __syncthreads();
         }
@@ -673,7 +830,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     double *dst = a.out_ns + (long long)j.g * 8 * N3 + (long long)j.gt * N3;
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) {
-        const int t = wv + S::NW * tt;
+        const int t = wv + NW * tt;
         const int col = 16 * t + lc;
         if (t >= NTZ || col >= N2) continue;
 #pragma unroll
@@ -684,13 +841,20 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     }
 }
 
-// FP16 split rows of FP64 blocks (rows x N values each, row-major): one wave
-// per block; scale[b] = 2^e with max |v| < 2^e (0 for a zero block).
-template <int N>
+// FP16 split rows of FP64 blocks (rows x N values each, row-major), data or
+// operator side: one wave per block; scale[b] = 2^e with max |v| < 2^e
+// (0 for a zero block).  Block nblocks (one past the last) is written as
+// zeros: lanes without an operand row read it.
+template <int N, bool DATA>
 __global__ void k_split_rows(const double *src, long long nblocks, int rows, _Float16 *dst, double *scale) {
     const long long b = blockIdx.x;
     const int lane = threadIdx.x;
-    if (b >= nblocks) return;
+    if (b >= nblocks) {
+        // the extra block after the last one: all zeros
+        _Float16 *o = dst + nblocks * rows * SplitShape<N>::RH;
+        for (int e = lane; e < rows * SplitShape<N>::RH; e += 64) o[e] = (_Float16)0.0f;
+        return;
+    }
     const double *s = src + b * (long long)rows * N;
     double m = 0.0;
     for (int e = lane; e < rows * N; e += 64) m = fmax(m, fabs(s[e]));
@@ -701,14 +865,10 @@ __global__ void k_split_rows(const double *src, long long nblocks, int rows, _Fl
     const double inv = m > 0.0 ? ldexp(1.0, -ex) : 0.0;
     if (lane == 0) scale[b] = m > 0.0 ? ldexp(1.0, ex) : 0.0;
     for (int r = lane; r < rows; r += 64) {
-        _Float16 *o = dst + (b * rows + r) * kRowH;
-        for (int jj = 0; jj < 12; jj++) {
-            const double v = jj < N ? s[r * N + jj] * inv : 0.0;
-            const _Float16 hi = (_Float16)v;
-            const _Float16 lo = (_Float16)(v - (double)hi);
-            o[jj < 8 ? jj : 8 + jj] = hi;       // hi[0..7] at 0..7, hi[8..11] at 16..19
-            o[jj < 8 ? 8 + jj : 12 + jj] = lo;  // lo[0..7] at 8..15, lo[8..11] at 20..23
-        }
+        double u[12];
+#pragma unroll
+        for (int jj = 0; jj < 12; jj++) u[jj] = jj < N ? s[r * N + jj] * inv : 0.0;
+        split_row<N, DATA>(u, dst + (b * rows + r) * SplitShape<N>::RH);
     }
 }
 
@@ -736,6 +896,7 @@ struct DevOper {
     float *blocks32 = nullptr;  // FP32 copy, made on first mixed-precision use
     _Float16 *blocks_h = nullptr;  // FP16 split rows, made on first split use
     double *blocks_hsc = nullptr;
+    long long blocks_hzero = 0;    // offset (halfs) of their all-zero block
     long long nblk = 0;
     int M = 0, ndepth = 0;
 };
@@ -756,6 +917,7 @@ struct DevicePlan {
     float *pool32 = nullptr;
     _Float16 *pool_h = nullptr;  // FP16 split rows of the pool (split mode)
     double *pool_hsc = nullptr;
+    long long pool_hzero = 0;    // offset (halfs) of its all-zero block
     KArgs args{};
     std::vector<AppDesc> apps;          // host copy of args.apps
     AppDesc *d_apps = nullptr;
@@ -1175,40 +1337,51 @@ static int ensure_f32(DevicePlan *p, std::string &err) {
     return 0;
 }
 
+// halfs per split row for basis size N
+static int split_rh(int N) { return N > 10 ? 40 : 32; }
+
 template <int N>
-static int split_rows_n(const double *src, long long nblocks, int rows, _Float16 **dst, double **scale,
+static int split_rows_n(const double *src, long long nblocks, int rows, bool data, _Float16 **dst, double **scale,
                         hipStream_t s, std::string &err) {
-    HIP_TRY(hipMalloc((void **)dst, sizeof(_Float16) * (size_t)std::max<long long>(nblocks * rows, 1) * kRowH));
+    HIP_TRY(hipMalloc((void **)dst, sizeof(_Float16) * (size_t)(nblocks + 1) * rows * SplitShape<N>::RH));
     HIP_TRY(hipMalloc((void **)scale, sizeof(double) * (size_t)std::max<long long>(nblocks, 1)));
-    if (nblocks > 0) {
-        hipLaunchKernelGGL(k_split_rows<N>, dim3((unsigned)nblocks), dim3(64), 0, s, src, nblocks, rows, *dst, *scale);
-        HIP_TRY(hipGetLastError());
-    }
+    if (data)
+        hipLaunchKernelGGL((k_split_rows<N, true>), dim3((unsigned)(nblocks + 1)), dim3(64), 0, s, src, nblocks, rows,
+                           *dst, *scale);
+    else
+        hipLaunchKernelGGL((k_split_rows<N, false>), dim3((unsigned)(nblocks + 1)), dim3(64), 0, s, src, nblocks, rows,
+                           *dst, *scale);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
-// FP16 split rows of FP64 blocks of `rows` x N values.
-static int split_rows(int N, const double *src, long long nblocks, int rows, _Float16 **dst, double **scale,
-                      hipStream_t s, std::string &err) {
-    if (nblocks >= (1LL << 31)) {
+// FP16 split rows of FP64 blocks of `rows` x N values (data or operator side).
+static int split_rows(int N, const double *src, long long nblocks, int rows, bool data, _Float16 **dst,
+                      double **scale, hipStream_t s, std::string &err) {
+    if (nblocks + 1 >= (1LL << 31)) {
         err = "too many blocks for the FP16 split conversion";
         return -1;
     }
     switch (N) {
-        case 2: return split_rows_n<2>(src, nblocks, rows, dst, scale, s, err);
-        case 3: return split_rows_n<3>(src, nblocks, rows, dst, scale, s, err);
-        case 4: return split_rows_n<4>(src, nblocks, rows, dst, scale, s, err);
-        case 5: return split_rows_n<5>(src, nblocks, rows, dst, scale, s, err);
-        case 6: return split_rows_n<6>(src, nblocks, rows, dst, scale, s, err);
-        case 7: return split_rows_n<7>(src, nblocks, rows, dst, scale, s, err);
-        case 8: return split_rows_n<8>(src, nblocks, rows, dst, scale, s, err);
-        case 9: return split_rows_n<9>(src, nblocks, rows, dst, scale, s, err);
-        case 10: return split_rows_n<10>(src, nblocks, rows, dst, scale, s, err);
-        case 11: return split_rows_n<11>(src, nblocks, rows, dst, scale, s, err);
-        case 12: return split_rows_n<12>(src, nblocks, rows, dst, scale, s, err);
+        case 2: return split_rows_n<2>(src, nblocks, rows, data, dst, scale, s, err);
+        case 3: return split_rows_n<3>(src, nblocks, rows, data, dst, scale, s, err);
+        case 4: return split_rows_n<4>(src, nblocks, rows, data, dst, scale, s, err);
+        case 5: return split_rows_n<5>(src, nblocks, rows, data, dst, scale, s, err);
+        case 6: return split_rows_n<6>(src, nblocks, rows, data, dst, scale, s, err);
+        case 7: return split_rows_n<7>(src, nblocks, rows, data, dst, scale, s, err);
+        case 8: return split_rows_n<8>(src, nblocks, rows, data, dst, scale, s, err);
+        case 9: return split_rows_n<9>(src, nblocks, rows, data, dst, scale, s, err);
+        case 10: return split_rows_n<10>(src, nblocks, rows, data, dst, scale, s, err);
+        case 11: return split_rows_n<11>(src, nblocks, rows, data, dst, scale, s, err);
+        case 12: return split_rows_n<12>(src, nblocks, rows, data, dst, scale, s, err);
         default: err = "unsupported basis size"; return -1;
     }
 }
+
+// Split-mode items whose bound is at most kSingleRatio x thr_cls keep their
+// intermediates rounded once to FP16 (relative error ~1e-3 of a term bounded
+// by 16 thresholds); larger ones use hi/lo intermediates.
+constexpr double kSingleRatio = 16.0;
 
 // FP16 split copies of the pool and of every part's operator blocks.
 static int ensure_h16(DevicePlan *p, std::string &err) {
@@ -1223,12 +1396,15 @@ static int ensure_h16(DevicePlan *p, std::string &err) {
         }
         DevOper &o = it->second;
         if (!o.blocks_h) {
-            int rc = split_rows(N, o.blocks, o.nblk / ((long long)N * N), N, &o.blocks_h, &o.blocks_hsc, c->stream, err);
+            const long long nb = o.nblk / ((long long)N * N);
+            int rc = split_rows(N, o.blocks, nb, N, false, &o.blocks_h, &o.blocks_hsc, c->stream, err);
             if (rc) return rc;
+            o.blocks_hzero = nb * N * split_rh(N);
         }
         if (p->apps[i].blk_h != o.blocks_h) {
             p->apps[i].blk_h = o.blocks_h;
             p->apps[i].blk_hsc = o.blocks_hsc;
+            p->apps[i].blk_hzero = o.blocks_hzero;
             changed = true;
         }
     }
@@ -1236,9 +1412,11 @@ static int ensure_h16(DevicePlan *p, std::string &err) {
         int rc = upload_at(p->d_apps, p->apps, c->stream, err);
         if (rc) return rc;
     }
-    if (!p->pool_h)
-        return split_rows(N, p->pool, p->npool / ((long long)N * N * N), N * N, &p->pool_h, &p->pool_hsc, c->stream,
-                          err);
+    if (!p->pool_h) {
+        const long long nb = p->npool / ((long long)N * N * N);
+        p->pool_hzero = nb * N * N * split_rh(N);
+        return split_rows(N, p->pool, nb, N * N, true, &p->pool_h, &p->pool_hsc, c->stream, err);
+    }
     return 0;
 }
 
@@ -1248,7 +1426,7 @@ static int launch_h_n(const DevicePlan *p, const int *list, int n_seg, double lo
     if (n_seg == 0) return 0;
     const size_t shmem = sizeof(double) * 64 * 4 * (2 * p->wmax + 1);
     hipLaunchKernelGGL((k_apply_h<N>), dim3(n_seg), dim3(SplitShape<N>::T), shmem, p->ctx->stream, p->args,
-                       p->pool_h, p->pool_hsc, list, lo, hi, cls);
+                       p->pool_h, p->pool_hsc, p->pool_hzero, list, lo, hi, kSingleRatio, cls);
     HIP_TRY(hipGetLastError());
     return 0;
 }
