@@ -456,12 +456,15 @@ struct SplitShape {
     static constexpr int NG = N * G;            // groups of an (outer, inner) plane
     static constexpr int NTG = (NG + 3) / 4;    // 16-row tiles of groups (x and y passes)
     static constexpr int NTZ = (N2 + 15) / 16;  // 16-column tiles (z pass)
-    static constexpr int NW = 2;                // waves per job
+    static constexpr int NW = 4;                // waves per job
     static constexpr int T = 64 * NW;
     static constexpr int TWG = (NTG + NW - 1) / NW;
     static constexpr int TWZ = (NTZ + NW - 1) / NW;
     static constexpr bool TWO = N > 10;
-    static constexpr int NC = TWO ? 5 : 4;      // 16-byte chunks per row
+    // 16-byte chunks per row: 4 operand chunks and the fifth chunk (N > 10;
+    // zeros otherwise).  The odd number of chunks makes the 80-byte row pitch
+    // spread 16 consecutive rows over distinct LDS banks.
+    static constexpr int NC = 5;
     static constexpr int RH = 8 * NC;           // halfs per row
 };
 
@@ -497,12 +500,10 @@ __device__ __forceinline__ void split_row(const double *u, _Float16 *row) {
     }
 #pragma unroll
     for (int e = 0; e < 8; e++) row[24 + e] = t[e];
-    if (SplitShape<N>::TWO) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            row[32 + e] = DATA ? hi[8 + e] : lo[8 + e];
-            row[36 + e] = z;
-        }
+    for (int e = 0; e < 4; e++) {
+        row[32 + e] = SplitShape<N>::TWO ? (DATA ? hi[8 + e] : lo[8 + e]) : z;
+        row[36 + e] = z;
     }
 }
 
@@ -578,6 +579,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     __shared__ __align__(16) _Float16 xt[(N2 + 2) * RH];
     __shared__ __align__(16) _Float16 yt[(N2 + 2) * RH];
     __shared__ __align__(16) SplitItem wl[kSplitCap];
+    __shared__ float wsig[kSplitCap];  // product of the item's three operator scales
     __shared__ int ltab[64];
     __shared__ long long btab[64];
     extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
@@ -683,11 +685,11 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     // x-pass A operand (data side): rows (jz, jy) of the F block, in groups
     mwg_h8 fa[TWG];
     mwg_h4 fe[TWG];
-    double sF = 0.0;
+    float sF = 0.f;
     auto load_f = [&](int grp) {
         const int s = a.band[j.boff + (grp >> 3)];
         const long long cb = a.slot_off[s] / N3 + (grp & 7);  // component block of the pool
-        sF = pool_sc[cb];
+        sF = (float)pool_sc[cb];
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
             const int t = wv + NW * tt;
@@ -698,12 +700,11 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         }
     };
     // operator row lc of block blk (uniform), chunk lq
-    auto load_op = [&](int blk, mwg_h8 &v, mwg_h4 &e, double &sc) {
-        sc = opsc[blk];
+    auto load_op = [&](int blk, mwg_h8 &v, mwg_h4 &e) {
         g_operand(opg + (lc < N ? blk * (long long)(N * RH) + lc * RH : ad.blk_hzero), true, v, e);
     };
-    // one item with operators (o, oe, so); the F tiles are current
-    auto apply_item = [&](const mwg_h8 (&o)[3], const mwg_h4 (&oe)[3], const double (&so)[3], bool single) {
+    // one item with operators (o, oe) and operator scale so; F tiles current
+    auto apply_item = [&](const mwg_h8 (&o)[3], const mwg_h4 (&oe)[3], float so, bool single) {
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
             const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], o[0], oe[0]);
@@ -719,7 +720,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             store4(yt, ys_row[tt], ys_j0[tt], dy, single);
         }
         __syncthreads();
-        const float sigma = (float)(sF * so[0] * so[1] * so[2]);
+        const float sigma = sF * so;
 #pragma unroll
         for (int tt = 0; tt < TWZ; tt++) {
             mwg_h8 vb;
@@ -770,6 +771,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                         itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + ll[2];
                         itm.grp = (p << 3) | ft | ((bound <= thr_single ? 1 : 0) << kSingleBit);
                         wl[pos] = itm;
+                        wsig[pos] = (float)(opsc[itm.bx] * opsc[itm.by] * opsc[itm.bz]);
                     }
                     n += __popcll(mask);
                 }
@@ -781,13 +783,13 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             // item's loads are in flight while this one computes
             mwg_h8 oA[3], oB[3];
             mwg_h4 eA[3], eB[3];
-            double sA[3], sB[3];
             const int mask_grp = (1 << kSingleBit) - 1;
             int cur_grp = __builtin_amdgcn_readfirstlane(wl[0].grp);
+            float sA = wsig[0], sB = 0.f;
             load_f(cur_grp & mask_grp);
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), oA[0], eA[0], sA[0]);
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].by), oA[1], eA[1], sA[1]);
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].bz), oA[2], eA[2], sA[2]);
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), oA[0], eA[0]);
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].by), oA[1], eA[1]);
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].bz), oA[2], eA[2]);
             for (int i = 0; i < n; i += 2) {
                 // item i on set A, prefetch i + 1 into set B
                 const bool has1 = i + 1 < n;
@@ -795,9 +797,10 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                 if (has1) {
                     const SplitItem nx = wl[i + 1];
                     grp1 = __builtin_amdgcn_readfirstlane(nx.grp);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oB[0], eB[0], sB[0]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oB[1], eB[1], sB[1]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oB[2], eB[2], sB[2]);
+                    sB = wsig[i + 1];
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oB[0], eB[0]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oB[1], eB[1]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oB[2], eB[2]);
                 }
                 apply_item(oA, eA, sA, (cur_grp >> kSingleBit) & 1);
                 if (!has1) break;
@@ -809,9 +812,10 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                 if (has2) {
                     const SplitItem nx = wl[i + 2];
                     grp2 = __builtin_amdgcn_readfirstlane(nx.grp);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oA[0], eA[0], sA[0]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oA[1], eA[1], sA[1]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oA[2], eA[2], sA[2]);
+                    sA = wsig[i + 2];
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oA[0], eA[0]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oA[1], eA[1]);
+                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oA[2], eA[2]);
                 }
                 apply_item(oB, eB, sB, (cur_grp >> kSingleBit) & 1);
                 if (!has2) break;
@@ -1338,7 +1342,7 @@ static int ensure_f32(DevicePlan *p, std::string &err) {
 }
 
 // halfs per split row for basis size N
-static int split_rh(int N) { return N > 10 ? 40 : 32; }
+static int split_rh(int) { return 40; }
 
 template <int N>
 static int split_rows_n(const double *src, long long nblocks, int rows, bool data, _Float16 **dst, double **scale,
