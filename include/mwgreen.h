@@ -163,7 +163,8 @@ int mwg_plan_create_batch(mwg_context ctx, int n, const mwg_tree *out, const mwg
  * `mode` (tau from the context).  stats (n <= 5 values): [0] device ms,
  * [1] term-applications that passed screening, [2] of them on the
  * low-precision path (FP32 or FP16 split),
- * [3] device ms of the FP64 launch, [4] of the low-precision launch. */
+ * [3] device ms of the FP64-class launch, [4] of the low-precision launch
+ * (the two run concurrently; [0] covers both). */
 int mwg_plan_execute(mwg_plan plan, double *stats, int n);
 /* Copy the result into the output tree(s) and run the TopDown/BottomUp
  * transforms. */

@@ -908,7 +908,10 @@ struct DevOper {
 struct DeviceCtx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
+    // second stream (default priority): the FP64-class launch of the mixed
+    // modes runs on it beside the low-precision launch
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evz = nullptr, ev2s = nullptr, ev2e = nullptr;
     std::map<const void *, DevOper> opers;
 };
 
@@ -958,6 +961,10 @@ int dev_create(int device, DeviceCtx **out, std::string &err) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipEventCreate(&c->evm);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->evz);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev2s);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev2e);
     if (e != hipSuccess) {
         err = std::string("stream/event creation: ") + hipGetErrorString(e);
         delete c;
@@ -987,6 +994,13 @@ void dev_destroy(DeviceCtx *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->evm) (void)hipEventDestroy(c->evm);
+    if (c->evz) (void)hipEventDestroy(c->evz);
+    if (c->ev2s) (void)hipEventDestroy(c->ev2s);
+    if (c->ev2e) (void)hipEventDestroy(c->ev2e);
+    if (c->stream2) {
+        (void)hipStreamSynchronize(c->stream2);
+        (void)hipStreamDestroy(c->stream2);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1266,9 +1280,9 @@ static int launch_nt(const KArgs &a, const T *pool, const int *list, int n_seg, 
 
 template <typename T>
 static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n_seg, double lo, double hi, int cls,
-                    std::string &err) {
+                    std::string &err, hipStream_t s = nullptr) {
     const KArgs &a = p->args;
-    hipStream_t s = p->ctx->stream;
+    if (!s) s = p->ctx->stream;
 #define MWG_LAUNCH(NN) launch_nt<NN, T>(a, pool, list, n_seg, p->wmax, lo, hi, cls, s, err)
     switch (p->kp1) {
         case 2: return MWG_LAUNCH(2);
@@ -1490,15 +1504,22 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     if (mode == 0) {
         // lo = 0: every term that passes screening
         rc = launch_t<double>(p, p->pool, nullptr, p->n_seg, 0.0, inf, 0, err);
+        if (rc) return rc;
     } else {
-        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, err);
-        if (!rc) HIP_TRY(hipEventRecord(c->evm, c->stream));
-        if (!rc) {
-            if (mode == 1) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
-            else rc = launch_h(p, p->list_lo, p->n_lo, 0.0, tau, 1, err);
-        }
+        // The FP64 class runs on the second stream while the low-precision
+        // class runs on the first; both only add into the zeroed output.
+        HIP_TRY(hipEventRecord(c->evz, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream2, c->evz, 0));
+        HIP_TRY(hipEventRecord(c->ev2s, c->stream2));
+        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, err, c->stream2);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
+        if (mode == 1) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
+        else rc = launch_h(p, p->list_lo, p->n_lo, 0.0, tau, 1, err);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(c->evm, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev2e, 0));
     }
-    if (rc) return rc;
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[8];
     HIP_TRY(hipMemcpyAsync(cnt, p->args.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
@@ -1510,9 +1531,10 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
         if (mode == 0) {
             split_ms[0] = ms;
         } else {
+            // device time of each class launch (they overlap)
             float m1 = 0.f, m2 = 0.f;
-            HIP_TRY(hipEventElapsedTime(&m1, c->ev0, c->evm));
-            HIP_TRY(hipEventElapsedTime(&m2, c->evm, c->ev1));
+            HIP_TRY(hipEventElapsedTime(&m1, c->ev2s, c->ev2e));
+            HIP_TRY(hipEventElapsedTime(&m2, c->evz, c->evm));
             split_ms[0] = m1;
             split_ms[1] = m2;
         }

@@ -35,7 +35,8 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
 // mode 0: every term in FP64.  mode 1: terms with bound > tau * thr in FP64,
 // the rest on FP32 MFMA, their sum added to the FP64 output.  lp_apps
 // receives the number of FP32 term-applications.
-// split_ms (optional, 2 values): device time of the FP64 and FP32 launches.
+// split_ms (optional, 2 values): device time of the FP64-class launch and of
+// the low-precision launch (they run concurrently, on two streams).
 int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
                  unsigned long long *lp_apps, std::string &err, double *split_ms = nullptr);
 // Copy the raw non-standard blocks of every part, in part order
