@@ -442,10 +442,12 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
 // where the tails carry the same three products for j = 8, 9 (N <= 10:
 // data {hi8 hi9 lo8 lo9 hi8 hi9 0 0}, operator {hi8 hi9 hi8 hi9 lo8 lo9 0 0})
 // or hi*hi + lo*hi for j = 8..11 (N = 11, 12: data {hi8..11 lo8..11},
-// operator {hi8..11 hi8..11}); for N = 11, 12 a fifth chunk holds hi8..11
+// operator {hi8..11 hi8..11}); for N = 11, 12 an extra chunk holds hi8..11
 // (data) / lo8..11 (operator) for a v_mfma_f32_16x16x16f16 in lane chunk 0.
-// Rows are stored in exactly this chunk order (16 bytes per chunk), so every
-// lane loads its operand with one 16-byte read and no selects.
+// A row stores the distinct chunks once, [hi 0..7 | lo 0..7 | tail] (+ the
+// extra chunk and a pad chunk for N > 10), and each lane reads the chunk its
+// K slots need with one 16-byte load: data lanes chunks (0, 1, 0, 2),
+// operator lanes chunks (0, 0, 1, 2).
 typedef _Float16 mwg_h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 mwg_h4 __attribute__((ext_vector_type(4)));
 
@@ -461,10 +463,11 @@ struct SplitShape {
     static constexpr int TWG = (NTG + NW - 1) / NW;
     static constexpr int TWZ = (NTZ + NW - 1) / NW;
     static constexpr bool TWO = N > 10;
-    // 16-byte chunks per row: 4 operand chunks and the fifth chunk (N > 10;
-    // zeros otherwise).  The odd number of chunks makes the 80-byte row pitch
-    // spread 16 consecutive rows over distinct LDS banks.
-    static constexpr int NC = 5;
+    // 16-byte chunks per row: hi, lo, tail (+ the extra chunk for N > 10).
+    // For N <= 10 the odd number of chunks makes the row pitch spread 16
+    // consecutive rows over distinct LDS banks; for N > 10 the four chunks
+    // keep the double-buffered intermediates within 64 KB of LDS.
+    static constexpr int NC = TWO ? 4 : 3;
     static constexpr int RH = 8 * NC;           // halfs per row
 };
 
@@ -482,8 +485,7 @@ __device__ __forceinline__ void split_row(const double *u, _Float16 *row) {
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         row[j] = hi[j];
-        row[8 + j] = DATA ? lo[j] : hi[j];
-        row[16 + j] = DATA ? hi[j] : lo[j];
+        row[8 + j] = lo[j];
     }
     _Float16 t[8];
     if (N <= 8) {
@@ -499,23 +501,13 @@ __device__ __forceinline__ void split_row(const double *u, _Float16 *row) {
         }
     }
 #pragma unroll
-    for (int e = 0; e < 8; e++) row[24 + e] = t[e];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        row[32 + e] = SplitShape<N>::TWO ? (DATA ? hi[8 + e] : lo[8 + e]) : z;
-        row[36 + e] = z;
-    }
-}
-
-// Operand of lane chunk c from a split row (ext: fifth chunk, in chunk 0).
-template <int N>
-__device__ __forceinline__ void split_operand(const _Float16 *row, int c, mwg_h8 &v, mwg_h4 &ext) {
-    const mwg_h8 *r8 = reinterpret_cast<const mwg_h8 *>(row);
-    v = r8[c];
+    for (int e = 0; e < 8; e++) row[16 + e] = t[e];
     if (SplitShape<N>::TWO) {
-        const _Float16 z = (_Float16)0.0f;
-        const mwg_h8 e = r8[4];
-        ext = c == 0 ? mwg_h4{e[0], e[1], e[2], e[3]} : mwg_h4{z, z, z, z};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            row[24 + e] = DATA ? hi[8 + e] : lo[8 + e];
+            row[28 + e] = z;
+        }
     }
 }
 
@@ -576,8 +568,9 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTZ = S::NTZ;
     constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, NW = S::NW;
     constexpr int ZROW = N2, DROW = N2 + 1;  // all-zero row, scratch row
-    __shared__ __align__(16) _Float16 xt[(N2 + 2) * RH];
-    __shared__ __align__(16) _Float16 yt[(N2 + 2) * RH];
+    constexpr int BUFH = (N2 + 2) * RH;  // halfs per intermediate buffer
+    __shared__ __align__(16) _Float16 xt[2 * BUFH];  // X of items q, q - 1 (by parity)
+    __shared__ __align__(16) _Float16 yt[2 * BUFH];  // Y of items q - 1, q - 2
     __shared__ __align__(16) SplitItem wl[kSplitCap];
     __shared__ float wsig[kSplitCap];  // product of the item's three operator scales
     __shared__ int ltab[64];
@@ -595,7 +588,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     const mwg_h8 zero8 = {z16, z16, z16, z16, z16, z16, z16, z16};
     const mwg_h4 zero4 = {z16, z16, z16, z16};
     // rows and chunks the passes never write are read as operands: zeros
-    for (int e = tid; e < (N2 + 2) * RH / 8; e += NTH) {
+    for (int e = tid; e < 2 * BUFH / 8; e += NTH) {
         reinterpret_cast<mwg_h8 *>(xt)[e] = zero8;
         reinterpret_cast<mwg_h8 *>(yt)[e] = zero8;
     }
@@ -629,19 +622,20 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     for (int tt = 0; tt < TWZ; tt++) acc[tt] = mwg_float4{0.f, 0.f, 0.f, 0.f};
     unsigned long long napps = 0;
 
-    // operand of lane chunk lq from a split row at `row` (halfs); ext from
-    // the fifth chunk for lanes of chunk 0 (N > 10), else from `zrow`
+    // chunk offsets (halfs) this lane reads: data side (0, 1, 0, 2), operator
+    // side (0, 0, 1, 2); the extra chunk (N > 10) only in lane chunk 0
+    const int dch = 8 * (lq == 1 ? 1 : (lq == 3 ? 2 : 0));
+    const int och = 8 * (lq == 2 ? 1 : (lq == 3 ? 2 : 0));
+    // data-side operand from the LDS row at `row` (halfs)
     auto lds_operand = [&](const _Float16 *base, int row, mwg_h8 &v, mwg_h4 &e) {
-        v = *reinterpret_cast<const mwg_h8 *>(base + row + 8 * lq);
-        if (S::TWO) e = *reinterpret_cast<const mwg_h4 *>(base + (lq == 0 ? row + 32 : ZROW * RH));
+        v = *reinterpret_cast<const mwg_h8 *>(base + row + dch);
+        if (S::TWO) e = *reinterpret_cast<const mwg_h4 *>(base + (lq == 0 ? row + 24 : ZROW * RH));
     };
-    auto g_operand = [&](const __attribute__((address_space(1))) _Float16 *row, bool ok, mwg_h8 &v, mwg_h4 &e) {
-        v = zero8;
+    // operand from a global split row (ch: the lane's chunk offset)
+    auto g_operand = [&](const __attribute__((address_space(1))) _Float16 *row, int ch, mwg_h8 &v, mwg_h4 &e) {
+        v = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(row + ch);
         e = zero4;
-        if (ok) {
-            v = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(row + 8 * lq);
-            if (S::TWO && lq == 0) e = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h4 *>(row + 32);
-        }
+        if (S::TWO && lq == 0) e = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h4 *>(row + 24);
     };
     // four results of one tile into a data-side split row
     auto store4 = [&](_Float16 *base, int row, int j0, const mwg_float4 &v, bool single) {
@@ -660,26 +654,24 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             hi = mwg_h4{(_Float16)h[0], (_Float16)h[1], (_Float16)h[2], (_Float16)h[3]};
             lo = mwg_h4{(_Float16)l[0], (_Float16)l[1], (_Float16)l[2], (_Float16)l[3]};
         }
-        // j0 < 8: hi -> chunk 0, lo -> chunk 1, hi -> chunk 2
-        // j0 = 8, N <= 10: chunk 3 = {hi0 hi1 lo0 lo1 | hi0 hi1 0 0}
-        // j0 = 8, N > 10: chunk 3 = {hi0..3 | lo0..3}, chunk 4 = {hi0..3 | -}
+        // j0 < 8: hi -> chunk 0, lo -> chunk 1
+        // j0 = 8, N <= 10: tail chunk = {hi0 hi1 lo0 lo1 | hi0 hi1 0 0}
+        // j0 = 8, N > 10: tail chunk = {hi0..3 | lo0..3}, extra chunk = {hi0..3 | -}
         const bool tail = j0 >= 8;
         mwg_h4 d1 = hi, d2 = lo;
-        int o1 = row + j0, o2 = row + 8 + j0, o3 = row + 16 + j0;
+        int o1 = row + j0, o2 = row + 8 + j0;
         if (N > 8 && N <= 10) {
             d1 = tail ? mwg_h4{hi[0], hi[1], lo[0], lo[1]} : hi;
             d2 = tail ? mwg_h4{hi[0], hi[1], z16, z16} : lo;
-            o1 = tail ? row + 24 : o1;
-            o2 = tail ? row + 28 : o2;
-            o3 = tail ? DROW * RH : o3;
+            o1 = tail ? row + 16 : o1;
+            o2 = tail ? row + 20 : o2;
         } else if (N > 10) {
-            o1 = tail ? row + 24 : o1;
-            o2 = tail ? row + 28 : o2;
-            o3 = tail ? row + 32 : o3;
+            o1 = tail ? row + 16 : o1;
+            o2 = tail ? row + 20 : o2;
+            *reinterpret_cast<mwg_h4 *>(base + (tail ? row + 24 : DROW * RH)) = hi;
         }
         *reinterpret_cast<mwg_h4 *>(base + o1) = d1;
         *reinterpret_cast<mwg_h4 *>(base + o2) = d2;
-        *reinterpret_cast<mwg_h4 *>(base + o3) = hi;
     };
 
     // x-pass A operand (data side): rows (jz, jy) of the F block, in groups
@@ -695,38 +687,43 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             const int t = wv + NW * tt;
             const int g4 = 4 * t + (lc >> 2), jy = 4 * (g4 % G) + (lc & 3);
             const bool ok = g4 < NG && jy < N;
-            g_operand(poolg + (ok ? cb * (long long)(N2 * RH) + ((g4 / G) * N + jy) * RH : pool_zero), true, fa[tt],
+            g_operand(poolg + (ok ? cb * (long long)(N2 * RH) + ((g4 / G) * N + jy) * RH : pool_zero), dch, fa[tt],
                       fe[tt]);
         }
     };
     // operator row lc of block blk (uniform), chunk lq
     auto load_op = [&](int blk, mwg_h8 &v, mwg_h4 &e) {
-        g_operand(opg + (lc < N ? blk * (long long)(N * RH) + lc * RH : ad.blk_hzero), true, v, e);
+        g_operand(opg + (lc < N ? blk * (long long)(N * RH) + lc * RH : ad.blk_hzero), och, v, e);
     };
-    // one item with operators (o, oe) and operator scale so; F tiles current
-    auto apply_item = [&](const mwg_h8 (&o)[3], const mwg_h4 (&oe)[3], float so, bool single) {
+    // The passes of one phase: x of item q into X[q & 1], y of item q - 1
+    // from X[(q - 1) & 1] into Y[(q - 1) & 1], z of item q - 2 from
+    // Y[q & 1] into acc.  Each buffer a pass reads was written in the
+    // previous phase and each buffer it writes was last read in the previous
+    // phase, so one barrier per phase orders them.
+    auto x_pass = [&](_Float16 *X, const mwg_h8 &o, const mwg_h4 &oe, bool single) {
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
-            const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], o[0], oe[0]);
-            store4(xt, xs_row[tt], xs_j0[tt], dx, single);
+            const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], o, oe);
+            store4(X, xs_row[tt], xs_j0[tt], dx, single);
         }
-        __syncthreads();
+    };
+    auto y_pass = [&](const _Float16 *X, _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, bool single) {
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
             mwg_h8 va;
             mwg_h4 ve = zero4;
-            lds_operand(xt, yl_off[tt], va, ve);
-            const mwg_float4 dy = split_mfma<N>(va, ve, o[1], oe[1]);
-            store4(yt, ys_row[tt], ys_j0[tt], dy, single);
+            lds_operand(X, yl_off[tt], va, ve);
+            const mwg_float4 dy = split_mfma<N>(va, ve, o, oe);
+            store4(Y, ys_row[tt], ys_j0[tt], dy, single);
         }
-        __syncthreads();
-        const float sigma = sF * so;
+    };
+    auto z_pass = [&](const _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, float sigma) {
 #pragma unroll
         for (int tt = 0; tt < TWZ; tt++) {
             mwg_h8 vb;
             mwg_h4 vbe = zero4;
-            lds_operand(yt, zl_off[tt], vb, vbe);
-            const mwg_float4 dz = split_mfma<N>(o[2], oe[2], vb, vbe);
+            lds_operand(Y, zl_off[tt], vb, vbe);
+            const mwg_float4 dz = split_mfma<N>(o, oe, vb, vbe);
 #pragma unroll
             for (int r = 0; r < 4; r++) acc[tt][r] = fmaf(sigma, dz[r], acc[tt][r]);
         }
@@ -779,48 +776,45 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             __syncthreads();
             if (n == 0) continue;
             napps += n;
-            // two operator register sets, used in turn, so that the next
-            // item's loads are in flight while this one computes
-            mwg_h8 oA[3], oB[3];
-            mwg_h4 eA[3], eB[3];
+            // software pipeline over the batch: phase q runs x(q), y(q - 1),
+            // z(q - 2); the operators of the next phase load meanwhile
             const int mask_grp = (1 << kSingleBit) - 1;
-            int cur_grp = __builtin_amdgcn_readfirstlane(wl[0].grp);
-            float sA = wsig[0], sB = 0.f;
-            load_f(cur_grp & mask_grp);
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), oA[0], eA[0]);
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].by), oA[1], eA[1]);
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].bz), oA[2], eA[2]);
-            for (int i = 0; i < n; i += 2) {
-                // item i on set A, prefetch i + 1 into set B
-                const bool has1 = i + 1 < n;
-                int grp1 = cur_grp;
-                if (has1) {
-                    const SplitItem nx = wl[i + 1];
-                    grp1 = __builtin_amdgcn_readfirstlane(nx.grp);
-                    sB = wsig[i + 1];
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oB[0], eB[0]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oB[1], eB[1]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oB[2], eB[2]);
+            mwg_h8 ox, oy, oz, oxn, oyn, ozn;
+            mwg_h4 ex = zero4, ey = zero4, ez = zero4, exn = zero4, eyn = zero4, ezn = zero4;
+            int cur_grp = -1;
+            float sig1 = 0.f, sig2 = 0.f;  // scales of items q - 1, q - 2
+            bool sng1 = false;             // single rounding of item q - 1
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), ox, ex);
+            for (int q = 0; q < n + 2; q++) {
+                const bool hx = q < n, hy = q >= 1 && q <= n, hz = q >= 2;
+                // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1)
+                if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), oxn, exn);
+                if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), oyn, eyn);
+                if (hy) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz), ozn, ezn);
+                float sig0 = 0.f;
+                bool sng0 = false;
+                if (hx) {
+                    const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
+                    if ((g & mask_grp) != cur_grp) {
+                        cur_grp = g & mask_grp;
+                        load_f(cur_grp);
+                    }
+                    sng0 = (g >> kSingleBit) & 1;
+                    sig0 = sF * wsig[q];
+                    x_pass(xt + (q & 1) * BUFH, ox, ex, sng0);
                 }
-                apply_item(oA, eA, sA, (cur_grp >> kSingleBit) & 1);
-                if (!has1) break;
-                if ((grp1 & mask_grp) != (cur_grp & mask_grp)) load_f(grp1 & mask_grp);
-                cur_grp = grp1;
-                // item i + 1 on set B, prefetch i + 2 into set A
-                const bool has2 = i + 2 < n;
-                int grp2 = cur_grp;
-                if (has2) {
-                    const SplitItem nx = wl[i + 2];
-                    grp2 = __builtin_amdgcn_readfirstlane(nx.grp);
-                    sA = wsig[i + 2];
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bx), oA[0], eA[0]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.by), oA[1], eA[1]);
-                    load_op(__builtin_amdgcn_readfirstlane(nx.bz), oA[2], eA[2]);
-                }
-                apply_item(oB, eB, sB, (cur_grp >> kSingleBit) & 1);
-                if (!has2) break;
-                if ((grp2 & mask_grp) != (cur_grp & mask_grp)) load_f(grp2 & mask_grp);
-                cur_grp = grp2;
+                if (hy) y_pass(xt + ((q - 1) & 1) * BUFH, yt + ((q - 1) & 1) * BUFH, oy, ey, sng1);
+                if (hz) z_pass(yt + (q & 1) * BUFH, oz, ez, sig2);
+                __syncthreads();
+                ox = oxn;
+                ex = exn;
+                oy = oyn;
+                ey = eyn;
+                oz = ozn;
+                ez = ezn;
+                sig2 = sig1;
+                sig1 = sig0;
+                sng1 = sng0;
             }
             __syncthreads();
         }
@@ -1356,7 +1350,7 @@ static int ensure_f32(DevicePlan *p, std::string &err) {
 }
 
 // halfs per split row for basis size N
-static int split_rh(int) { return 40; }
+static int split_rh(int N) { return N > 10 ? 32 : 24; }
 
 template <int N>
 static int split_rows_n(const double *src, long long nblocks, int rows, bool data, _Float16 **dst, double **scale,
@@ -1443,6 +1437,15 @@ static int launch_h_n(const DevicePlan *p, const int *list, int n_seg, double lo
                       std::string &err) {
     if (n_seg == 0) return 0;
     const size_t shmem = sizeof(double) * 64 * 4 * (2 * p->wmax + 1);
+    hipFuncAttributes fa;
+    HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_apply_h<N>)));
+    int lds_max = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->ctx->device));
+    if (fa.sharedSizeBytes + shmem > (size_t)lds_max) {
+        err = "band half-width " + std::to_string(p->wmax) + " needs more LDS than the split kernel has at k+1 = " +
+              std::to_string(N) + "; use the FP64 or mixed mode";
+        return -1;
+    }
     hipLaunchKernelGGL((k_apply_h<N>), dim3(n_seg), dim3(SplitShape<N>::T), shmem, p->ctx->stream, p->args,
                        p->pool_h, p->pool_hsc, p->pool_hzero, list, lo, hi, kSingleRatio, cls);
     HIP_TRY(hipGetLastError());
