@@ -183,6 +183,11 @@ __device__ __forceinline__ JobCtx job_ctx(const KArgs &a, const AppDesc &ad, int
     return j;
 }
 
+// Norm table row stride (doubles) of one term: 4 components x (2W + 1)
+// translations plus one pad, so that the 64 lane-terms fall on spread LDS
+// banks.
+__host__ __device__ __forceinline__ int nrm_stride(int nl) { return 4 * nl + 1; }
+
 // Stage the chunk of active terms [base, base + cnt): per lane-term its block
 // half-width and block offset, and its 1D norms [term][c][l + W] for every
 // component and band translation (zero outside the term's bandwidth).
@@ -203,7 +208,7 @@ __device__ __forceinline__ void stage_terms(const KArgs &a, const AppDesc &ad, c
             if (abs(l) <= bwc && abs(l) <= L)
                 v = ad.op_norms[ad.op_noff[m * j.ndepth + j.d] + (long long)c * (2 * L + 1) + (l + L)];
         }
-        nrm_s[e] = v;
+        nrm_s[ti * nrm_stride(j.nl) + c * j.nl + li] = v;
     }
 }
 
@@ -227,7 +232,7 @@ __device__ __forceinline__ int screen_items(const KArgs &a, const JobCtx &j, int
         for (int dim = 0; dim < 3; dim++)
 #pragma unroll
             for (int fb = 0; fb < 2; fb++)
-                nrm[fb][dim] = nrm_s[(lane * 4 + (2 * j.gb[dim] + fb)) * j.nl + (ll[dim] + j.w)];
+                nrm[fb][dim] = nrm_s[lane * nrm_stride(j.nl) + (2 * j.gb[dim] + fb) * j.nl + (ll[dim] + j.w)];
         const double *fnp = a.slot_norm + (long long)s * 8;
         for (int ft = 0; ft < 8; ft++) {
             if (j.d > 0 && ft == 0 && j.gt == 0) continue;
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
     __shared__ int wl[kWorkCap];
     __shared__ int ltab[64];
     __shared__ long long btab[64];
-    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
+    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x nrm_stride(2 Wmax + 1)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
     for (int e = tid; e < BUF; e += NTH) buf[e] = T(0);
@@ -575,7 +580,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     __shared__ float wsig[kSplitCap];  // product of the item's three operator scales
     __shared__ int ltab[64];
     __shared__ long long btab[64];
-    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x 4 x (2 Wmax + 1)
+    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x nrm_stride(2 Wmax + 1)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
 
@@ -739,7 +744,8 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         const int my_b0 = (int)(btab[lane] / N2) + my_L;
         int p = p_begin;
         while (p < p_end) {
-            // screening (the rule of screen_items) into SplitItems
+            // screening (the rule of screen_items) into SplitItems: every
+            // wave forms the same ballots, wave 0 writes the items
             const bool have = lane < cnt;
             int n = 0;
             for (; p < p_end && n + 8 * 64 <= kSplitCap; p++) {
@@ -751,7 +757,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                 for (int dim = 0; dim < 3; dim++)
 #pragma unroll
                     for (int fb = 0; fb < 2; fb++)
-                        nrm[fb][dim] = nrm_s[(lane * 4 + (2 * j.gb[dim] + fb)) * j.nl + (ll[dim] + j.w)];
+                        nrm[fb][dim] = nrm_s[lane * nrm_stride(j.nl) + (2 * j.gb[dim] + fb) * j.nl + (ll[dim] + j.w)];
                 const double *fnp = a.slot_norm + (long long)s * 8;
                 for (int ft = 0; ft < 8; ft++) {
                     if (j.d > 0 && ft == 0 && j.gt == 0) continue;
@@ -1266,7 +1272,7 @@ template <int N, typename T>
 static int launch_nt(const KArgs &a, const T *pool, const int *list, int n_seg, int wmax, double lo, double hi, int cls,
                      hipStream_t s, std::string &err) {
     if (n_seg == 0) return 0;
-    const size_t shmem = sizeof(double) * 64 * 4 * (2 * wmax + 1);
+    const size_t shmem = sizeof(double) * 64 * nrm_stride(2 * wmax + 1);
     hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, list, lo, hi, cls);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1436,7 +1442,7 @@ template <int N>
 static int launch_h_n(const DevicePlan *p, const int *list, int n_seg, double lo, double hi, int cls,
                       std::string &err) {
     if (n_seg == 0) return 0;
-    const size_t shmem = sizeof(double) * 64 * 4 * (2 * p->wmax + 1);
+    const size_t shmem = sizeof(double) * 64 * nrm_stride(2 * p->wmax + 1);
     hipFuncAttributes fa;
     HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_apply_h<N>)));
     int lds_max = 0;
