@@ -38,6 +38,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summaries (profiles/*traffic*.md, lines
+    'traffic_bytes <kernel> <bytes>'), or None when not measured."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        for line in open(path):
+            if line.startswith("traffic_bytes ") and line[len("traffic_bytes "):].rsplit(" ", 1)[0] == kernel:
+                return float(line.rsplit(" ", 1)[1])
+    return None
+
+
 def build_workload(cfg_name, max_depth=16):
     """Operators and projected input trees of one configuration (one entry
     per apply)."""
@@ -189,6 +201,7 @@ def main():
         cpu = cpu_baseline(mra, ops, inps, outs, thr, ns64, args.cpu_seconds, check)
 
     peak_key = "f16" if head == "split16" else "f32"
+    kname = ("k_apply_h<%d>" if head == "split16" else "k_apply<%d, float>") % (k + 1)
     line = {
         "metric": METRIC,
         "value": n_out / (msmx * 1e-3),
@@ -206,8 +219,8 @@ def main():
         "config": {"workload": cfg["name"], "order": k, "prec": prec, "applies": n_app, "out_nodes": n_out,
                    "in_slots": pinfo["in_slots"], "terms": ops[0].size(), "path": head},
         "roofline": {"bound": "mfma", "achieved": hr["lp_tflops"], "peak": PEAK_TFLOPS[peak_key], "unit": "TFLOP/s",
-                     "frac": hr["lp_tflops"] / PEAK_TFLOPS[peak_key], "traffic": None,
-                     "kernel": ("k_apply_h<%d>" if head == "split16" else "k_apply<%d, float>") % (k + 1)},
+                     "frac": hr["lp_tflops"] / PEAK_TFLOPS[peak_key], "traffic": measured_traffic(kname),
+                     "kernel": kname},
         "fp64_path": {"value": n_out / (ms64 * 1e-3), "ms_per_step": ms64, "kernel_ms": res["fp64"]["kernel_ms"],
                       "roofline": {"achieved": ach64, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                                    "frac": ach64 / PEAK_TFLOPS["f64"]}},
