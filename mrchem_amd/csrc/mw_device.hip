@@ -42,8 +42,13 @@ struct AppDesc {
     int M, ndepth;
     long long bw_off;                 // start of this apply in bw
     int act_base;                     // start of this apply in act_off
-    int pad_;
+    int mt_d0;                        // first output depth of mtab
     double thr_floor;                 // positive floor of the class threshold
+    // pre-screen: per output depth d >= mt_d0, component combination
+    // cmb = cx | cy << 2 | cz << 4 and translation |l| <= mt_w, the largest
+    // ((N_x N_y) N_z) over the depth's active terms, [d - mt_d0][cmb][lz][ly][lx]
+    const double *mtab;
+    int mt_w, pad_;
 };
 
 struct KArgs {
@@ -84,10 +89,12 @@ __device__ __forceinline__ const float *op_blocks<float>(const AppDesc &ad) {
 //
 // Screening: the active terms of the job's depth are taken in chunks of 64
 // (one term per lane).  For the chunk, the 1D norms of every term, component
-// and band translation are staged in LDS; for each band node f the bound
-// ||O_x|| ||O_y|| ||O_z|| ||f_ft|| is formed for every input component ft and
-// a ballot gives the surviving terms, which are appended to an LDS worklist
-// of (band node, ft, term) items.  A term is taken by this launch when it
+// and band translation are staged in LDS.  The (band node f, input component
+// ft) pairs are pre-screened 64 at a time against the bound of the depth's
+// largest term; for each candidate pair the bound
+// ||O_x|| ||O_y|| ||O_z|| ||f_ft|| of every lane-term is formed and a ballot
+// gives the surviving terms, which are appended to an LDS worklist of
+// (band node, ft, term) items.  A term is taken by this launch when it
 // passes the reference rule (bound > thr) and falls in the launch's bound
 // class (bound > lo*thr and not bound > hi*thr), so launches with disjoint
 // classes together apply exactly the oracle's term set.
@@ -148,7 +155,7 @@ struct JobShape {
 };
 
 constexpr int kMaxBand = 9;     // band half-width limit
-constexpr int kWorkCap = 1024;  // worklist items per batch (>= 8 * 64 + 512)
+constexpr int kWorkCap = 1024;  // worklist items per batch (>= 64 + 512)
 
 // Output job of a work segment and its screening context.
 struct JobCtx {
@@ -212,41 +219,64 @@ __device__ __forceinline__ void stage_terms(const KArgs &a, const AppDesc &ad, c
     }
 }
 
-// Screening of band positions from p on into the worklist `wl` (capacity
-// cap) until it could overflow: for every input component ft the lane-term's
+// Screening of the (band position p, input component ft) pairs, pair index
+// 8 p + ft, from pi on.  The wave first tests 64 pairs at once against the
+// bound of the depth's largest term: with M the largest ((N_x N_y) N_z) of
+// the active terms, fl(M ||f_ft||) is the largest of the terms' bounds
+// (rounding is monotone), so a pair without a candidate flag holds no
+// passing term.  For each candidate pair, in pair order, the lane-term's
 // bound ((N_x N_y) N_z) ||f_ft|| is formed in FP64 in the oracle's operation
 // order and a ballot appends the terms that pass the reference rule
 // (bound > thr) and fall in the launch's class (bound > thr_lo and not
-// bound > thr_hi) as items (p << 9 | ft << 6 | lane).  Every wave computes the
-// same ballots; the writer wave stores the items.  Returns the item count.
-__device__ __forceinline__ int screen_items(const KArgs &a, const JobCtx &j, int &p, int p_end, int cnt,
-                                            const double *nrm_s, int *wl, int cap, int lane, bool writer) {
+// bound > thr_hi): emit(pos, p, ft, ll, bound) is called by each taking lane
+// with its worklist position.  Stops before a pair that could overflow the
+// worklist (cap items) and leaves pi there.  Returns the item count.
+template <typename Emit>
+__device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, const JobCtx &j, int &pi, int pi_end,
+                                           int cnt, const double *nrm_s, int cap, int lane, Emit &&emit) {
     const bool have = lane < cnt;
+    const int S = 2 * ad.mt_w + 1;
+    const double *mt = ad.mtab + (long long)(j.d - ad.mt_d0) * 64 * S * S * S;
+    const double *nr = nrm_s + lane * nrm_stride(j.nl);
     int n = 0;
-    for (; p < p_end && n + 8 * 64 <= cap; p++) {
-        const int s = a.band[j.boff + p];
-        if (s < 0) continue;
-        const int ll[3] = {p % j.side - j.w, (p / j.side) % j.side - j.w, p / (j.side * j.side) - j.w};
-        double nrm[2][3];
-#pragma unroll
-        for (int dim = 0; dim < 3; dim++)
-#pragma unroll
-            for (int fb = 0; fb < 2; fb++)
-                nrm[fb][dim] = nrm_s[lane * nrm_stride(j.nl) + (2 * j.gb[dim] + fb) * j.nl + (ll[dim] + j.w)];
-        const double *fnp = a.slot_norm + (long long)s * 8;
-        for (int ft = 0; ft < 8; ft++) {
-            if (j.d > 0 && ft == 0 && j.gt == 0) continue;
-            const double fn = fnp[ft];
-            if (!(fn > 0.0)) continue;
-            const double bound = ((nrm[ft & 1][0] * nrm[(ft >> 1) & 1][1]) * nrm[(ft >> 2) & 1][2]) * fn;
+    while (pi < pi_end) {
+        const int pr = pi + lane;
+        bool cand = false;
+        if (pr < pi_end) {
+            const int p = pr >> 3, ft = pr & 7;
+            const int s = a.band[j.boff + p];
+            if (s >= 0 && !(j.d > 0 && ft == 0 && j.gt == 0)) {
+                const double fn = a.slot_norm[(long long)s * 8 + ft];
+                const int lx = p % j.side - j.w, ly = (p / j.side) % j.side - j.w, lz = p / (j.side * j.side) - j.w;
+                const int cmb = (2 * j.gb[0] + (ft & 1)) | ((2 * j.gb[1] + ((ft >> 1) & 1)) << 2) |
+                                ((2 * j.gb[2] + ((ft >> 2) & 1)) << 4);
+                const double mb =
+                    mt[(((long long)cmb * S + (lz + ad.mt_w)) * S + (ly + ad.mt_w)) * S + (lx + ad.mt_w)] * fn;
+                cand = fn > 0.0 && mb > j.thr && mb > j.thr_lo;
+            }
+        }
+        unsigned long long cm = __ballot(cand);
+        while (cm) {
+            const int k = __builtin_ctzll(cm);
+            if (n + 64 > cap) {
+                pi += k;
+                return n;
+            }
+            cm &= cm - 1;
+            const int p = (pi + k) >> 3, ft = (pi + k) & 7;
+            const int s = a.band[j.boff + p];
+            const double fn = a.slot_norm[(long long)s * 8 + ft];
+            const int ll[3] = {p % j.side - j.w, (p / j.side) % j.side - j.w, p / (j.side * j.side) - j.w};
+            const double nx = nr[(2 * j.gb[0] + (ft & 1)) * j.nl + (ll[0] + j.w)];
+            const double ny = nr[(2 * j.gb[1] + ((ft >> 1) & 1)) * j.nl + (ll[1] + j.w)];
+            const double nz = nr[(2 * j.gb[2] + ((ft >> 2) & 1)) * j.nl + (ll[2] + j.w)];
+            const double bound = ((nx * ny) * nz) * fn;
             const bool take = have && bound > j.thr && bound > j.thr_lo && !(bound > j.thr_hi);
             const unsigned long long mask = __ballot(take);
-            if (take && writer) {
-                const int pos = n + __popcll(mask & ((1ull << lane) - 1ull));
-                wl[pos] = (p << 9) | (ft << 6) | lane;
-            }
+            if (take) emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll, bound);
             n += __popcll(mask);
         }
+        pi += 64;
     }
     return n;
 }
@@ -329,10 +359,13 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
         __syncthreads();
         stage_terms(a, ad, j, base, cnt, ltab, btab, nrm_s, tid, NTH);
         __syncthreads();
-        int p = p_begin;
-        while (p < p_end) {
+        int pi = 8 * p_begin;
+        while (pi < 8 * p_end) {
             // screening into the worklist (wave 0 writes the items)
-            const int n = screen_items(a, j, p, p_end, cnt, nrm_s, wl, kWorkCap, lane, wv == 0);
+            const int n = screen_scan(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane,
+                                      [&](int pos, int p, int ft, const int *, double) {
+                                          if (wv == 0) wl[pos] = (p << 9) | (ft << 6) | lane;
+                                      });
             __syncthreads();
             if (n == 0) continue;
             napps += n;
@@ -563,7 +596,7 @@ constexpr int kSingleBit = 30;
 // once (lo = 0): their arithmetic error stays below ~1e-3 of the bound.
 // Lanes without a row to read take an all-zero row, lanes without a row to
 // write write a scratch row, so no pass branches on the lane.
-constexpr int kSplitCap = 640;  // worklist items per batch (>= 8 * 64 + 128)
+constexpr int kSplitCap = 640;  // worklist items per batch (>= 64 + 128)
 
 template <int N>
 __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
@@ -742,43 +775,22 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         // this lane's term: operator block base and half-width
         const int my_L = ltab[lane];
         const int my_b0 = (int)(btab[lane] / N2) + my_L;
-        int p = p_begin;
-        while (p < p_end) {
-            // screening (the rule of screen_items) into SplitItems: every
-            // wave forms the same ballots, wave 0 writes the items
-            const bool have = lane < cnt;
-            int n = 0;
-            for (; p < p_end && n + 8 * 64 <= kSplitCap; p++) {
-                const int s = a.band[j.boff + p];
-                if (s < 0) continue;
-                const int ll[3] = {p % j.side - j.w, (p / j.side) % j.side - j.w, p / (j.side * j.side) - j.w};
-                double nrm[2][3];
-#pragma unroll
-                for (int dim = 0; dim < 3; dim++)
-#pragma unroll
-                    for (int fb = 0; fb < 2; fb++)
-                        nrm[fb][dim] = nrm_s[lane * nrm_stride(j.nl) + (2 * j.gb[dim] + fb) * j.nl + (ll[dim] + j.w)];
-                const double *fnp = a.slot_norm + (long long)s * 8;
-                for (int ft = 0; ft < 8; ft++) {
-                    if (j.d > 0 && ft == 0 && j.gt == 0) continue;
-                    const double fn = fnp[ft];
-                    if (!(fn > 0.0)) continue;
-                    const double bound = ((nrm[ft & 1][0] * nrm[(ft >> 1) & 1][1]) * nrm[(ft >> 2) & 1][2]) * fn;
-                    const bool take = have && bound > j.thr && bound > j.thr_lo && !(bound > j.thr_hi);
-                    const unsigned long long mask = __ballot(take);
-                    if (take && wv == 0) {
-                        const int pos = n + __popcll(mask & ((1ull << lane) - 1ull));
-                        SplitItem itm;
-                        itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * (2 * my_L + 1)) + ll[0];
-                        itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * (2 * my_L + 1)) + ll[1];
-                        itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + ll[2];
-                        itm.grp = (p << 3) | ft | ((bound <= thr_single ? 1 : 0) << kSingleBit);
-                        wl[pos] = itm;
-                        wsig[pos] = (float)(opsc[itm.bx] * opsc[itm.by] * opsc[itm.bz]);
-                    }
-                    n += __popcll(mask);
-                }
-            }
+        int pi = 8 * p_begin;
+        while (pi < 8 * p_end) {
+            // screening into SplitItems: the lane-term also resolves the
+            // item's three operator block ids; wave 0 writes the items
+            const int n = screen_scan(
+                a, ad, j, pi, 8 * p_end, cnt, nrm_s, kSplitCap, lane,
+                [&](int pos, int p, int ft, const int *ll, double bound) {
+                    if (wv != 0) return;
+                    SplitItem itm;
+                    itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * (2 * my_L + 1)) + ll[0];
+                    itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * (2 * my_L + 1)) + ll[1];
+                    itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + ll[2];
+                    itm.grp = (p << 3) | ft | ((bound <= thr_single ? 1 : 0) << kSingleBit);
+                    wl[pos] = itm;
+                    wsig[pos] = (float)(opsc[itm.bx] * opsc[itm.by] * opsc[itm.bz]);
+                });
             __syncthreads();
             if (n == 0) continue;
             napps += n;
@@ -1156,6 +1168,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     // the stream has consumed them)
     std::vector<std::vector<int>> keep_i;
     std::vector<std::vector<long long>> keep_l;
+    std::vector<std::vector<double>> keep_d;
     long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0, o_seg = 0;
     p->seg_thr.reserve(n_seg);
     p->seg_bmin.reserve(n_seg);
@@ -1181,6 +1194,13 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         ad.bw_off = o_bw;
         ad.act_base = (int)o_actoff;
         ad.thr_floor = pl.thr_floor;
+        keep_d.emplace_back();
+        build_prescreen(pl, *pp.flat, keep_d.back(), ad.mt_d0, ad.mt_w);
+        double *d_mt = nullptr;
+        rc = plan_alloc(p, keep_d.back().size(), &d_mt, err);
+        if (!rc) rc = upload_at(d_mt, keep_d.back(), s, err);
+        if (rc) break;
+        ad.mtab = d_mt;
         p->apps.push_back(ad);
         const size_t no = pl.out_node.size();
         // outputs

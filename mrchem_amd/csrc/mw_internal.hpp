@@ -234,5 +234,10 @@ struct OperFlat {
     std::vector<double> blocks, norms;
 };
 void flatten_operator(const Operator &op, OperFlat &f);
+// Pre-screen table of an apply plan for the device screening: for each
+// output depth d in [d0, d0 + nd), component combination cmb = cx | cy << 2 |
+// cz << 4 and translation |l| <= w (the plan's widest band), the largest
+// ((N_x N_y) N_z) over the depth's active terms.
+void build_prescreen(const ApplyPlan &pl, const OperFlat &f, std::vector<double> &mt, int &d0, int &w);
 
 }  // namespace mwg
