@@ -584,12 +584,14 @@ constexpr int kSingleBit = 30;
 // the screening rule are those of k_apply; the screening lane (one term)
 // also resolves the item's three operator block ids.  With unit-scaled
 // operands F_u, O_u:
-//   x: X[(jz,jy)][ix] = F_u . Ox_u^T, stored transposed as Xt[(ix,jz)][jy]
-//   y: Y[(ix,jz)][iy] = Xt . Oy_u^T,  stored as Yt[(iy,ix)][jz]
-//   z: Z[iz][(iy,ix)] = Oz_u . Yt^T,  acc += sF sOx sOy sOz Z  (FP32)
+//   x: X[(jz,jy)][ix] = F_u . Ox_u^T, stored transposed as Xt[(jz,ix)][jy]
+//   y: Y[(ix,jz)][iy] = Xt . Oy_u^T,  stored as Yt[(ix,iy)][jz]
+//   z: Z[iz][(ix,iy)] = Oz_u . Yt^T,  acc += sF sOx sOy sOz Z  (FP32)
 // The rows of the x and y passes are taken in groups of four consecutive
-// inner indices of one outer line, so every lane stores its four results
-// with aligned 8-byte writes.  Intermediates are split again (hi: the value
+// inner indices (x: (jz, 4 jy), y: (ix, 4 jz), the y groups ordered with ix
+// fastest), so every lane stores its four results with aligned 8-byte
+// writes, the 16 lanes of a store write 16 consecutive rows and the 8 lanes
+// of a 16-byte read hit distinct LDS banks.  Intermediates are split again (hi: the value
 // cut to an 11-bit mantissa, lo: the exact FP32 remainder, both packed to
 // FP16) before the next pass; their magnitudes stay below N^2.  Items whose
 // bound is at most single_mult * thr_cls keep the intermediates rounded
@@ -642,12 +644,15 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         const int t = wv + NW * tt;
         const int g4 = 4 * t + lq;  // group of this lane's four results
         const bool okx = g4 < NG && lc < N;
-        xs_row[tt] = (okx ? lc * N + g4 / G : DROW) * RH;
+        // x: group g4 = (jz, jy / 4) = (g4 / G, g4 % G), lane ix = lc -> Xt row jz N + ix
+        xs_row[tt] = (okx ? (g4 / G) * N + lc : DROW) * RH;
         xs_j0[tt] = 4 * (g4 % G);
-        ys_row[tt] = xs_row[tt];  // same (lane, group) layout: (iy, ix) rows of Yt
-        ys_j0[tt] = xs_j0[tt];
-        const int g5 = 4 * t + (lc >> 2), jz = 4 * (g5 % G) + (lc & 3);  // A row of the y pass
-        yl_off[tt] = ((g5 < NG && jz < N) ? (g5 / G) * N + jz : ZROW) * RH;
+        // y: group g4 = (ix, jz / 4) = (g4 % N, g4 / N), lane iy = lc -> Yt row ix N + iy
+        ys_row[tt] = (okx ? (g4 % N) * N + lc : DROW) * RH;
+        ys_j0[tt] = 4 * (g4 / N);
+        // y-pass A row of lane lc: group g5, jz = 4 (g5 / N) + (lc & 3), ix = g5 % N
+        const int g5 = 4 * t + (lc >> 2), jz = 4 * (g5 / N) + (lc & 3);
+        yl_off[tt] = ((g5 < NG && jz < N) ? jz * N + g5 % N : ZROW) * RH;
     }
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) {
@@ -852,7 +857,8 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int iz = 4 * lq + r;
-            if (iz < N) atomicAdd(&dst[iz * N2 + col], (double)acc[tt][r]);
+            // col = ix N + iy
+            if (iz < N) atomicAdd(&dst[iz * N2 + (col % N) * N + col / N], (double)acc[tt][r]);
         }
     }
 }
