@@ -132,8 +132,8 @@ def main():
         plans[m] = mw.ApplyPlan(trees[m], ops, inps, thr, mode=m, ctx=ctx)
     t_plan = time.time() - t0
     pinfo = plans["fp64"].info()
-    log("frozen plans: %d out nodes, %d input slots (%d generated), %d segments, %.2fs" %
-        (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], pinfo["segments"], t_plan))
+    log("frozen plans: %d out nodes, %d input slots (%d generated), %d segments, band half-width <= %d, %.2fs" %
+        (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], pinfo["segments"], pinfo["band_w"], t_plan))
 
     def timed(plan):
         for _ in range(args.warmup):

@@ -175,7 +175,8 @@ int mwg_plan_finish(mwg_plan plan);
 int mwg_plan_download_ns(mwg_plan plan, double *ns);
 /* [0] output nodes, [1] input slots, [2] generated slots, [3] host plan ms,
  * [4] term-applications counted on the host with the screening rule,
- * [5] work segments (workgroups of an FP64 launch), [6] applies. */
+ * [5] work segments (workgroups of an FP64 launch), [6] applies,
+ * [7] widest band half-width (translations |l| <= W per direction). */
 int mwg_plan_info(mwg_plan plan, double *info, int n);
 int mwg_plan_destroy(mwg_plan plan);
 

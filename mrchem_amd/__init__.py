@@ -427,10 +427,10 @@ class ApplyPlan:
         return ns
 
     def info(self):
-        v = np.zeros(7)
-        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 7))
+        v = np.zeros(8)
+        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 8))
         return {"out_nodes": int(v[0]), "in_slots": int(v[1]), "generated": int(v[2]), "plan_ms": v[3],
-                "host_term_apps": int(v[4]), "segments": int(v[5]), "applies": int(v[6])}
+                "host_term_apps": int(v[4]), "segments": int(v[5]), "applies": int(v[6]), "band_w": int(v[7])}
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -598,7 +598,7 @@ constexpr int kSingleBit = 30;
 // once (lo = 0): their arithmetic error stays below ~1e-3 of the bound.
 // Lanes without a row to read take an all-zero row, lanes without a row to
 // write write a scratch row, so no pass branches on the lane.
-constexpr int kSplitCap = 640;  // worklist items per batch (>= 64 + 128)
+constexpr int kSplitCap = 384;  // worklist items per batch (>= 64 + 128)
 
 template <int N>
 __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,

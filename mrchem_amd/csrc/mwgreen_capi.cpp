@@ -38,7 +38,7 @@ struct mwg_plan_s {
     std::vector<int> n_out;
     int total_out = 0;
     int mode = MWG_MODE_FP64;
-    double plan_ms = 0.0, slots = 0.0, gen = 0.0, host_apps = 0.0, segs = 0.0;
+    double plan_ms = 0.0, slots = 0.0, gen = 0.0, host_apps = 0.0, segs = 0.0, band_w = 0.0;
 };
 struct mwg_mra_s {
     MRA mra;
@@ -329,6 +329,7 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
         p->gen += (double)ap.n_generated;
         p->host_apps += (double)ap.term_apps;
         p->segs += (double)ap.seg_job.size();
+        for (int w : ap.band_w) p->band_w = std::max(p->band_w, (double)w);
         p->outs.push_back(out[i]);
         p->n_out.push_back((int)nodes[i].size());
         p->total_out += (int)nodes[i].size();
@@ -779,9 +780,9 @@ int mwg_plan_download_ns(mwg_plan plan, double *ns) {
 
 int mwg_plan_info(mwg_plan plan, double *info, int n) {
     if (!plan || !info) return fail(MWG_ERR_ARG, "null pointer argument");
-    double v[7] = {(double)plan->total_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs,
-                   (double)plan->outs.size()};
-    for (int i = 0; i < n && i < 7; i++) info[i] = v[i];
+    double v[8] = {(double)plan->total_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs,
+                   (double)plan->outs.size(), plan->band_w};
+    for (int i = 0; i < n && i < 8; i++) info[i] = v[i];
     return MWG_OK;
 }
 
