@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mw_device.hpp"
@@ -38,7 +39,6 @@ struct AppDesc {
     const float *blk32;               // FP32 copy (mixed mode), or null
     const _Float16 *blk_h;            // FP16 split rows (split mode), or null
     const double *blk_hsc;            // their per-block scales
-    long long blk_hzero;              // offset (halfs) of an all-zero row in blk_h
     int M, ndepth;
     long long bw_off;                 // start of this apply in bw
     int act_base;                     // start of this apply in act_off
@@ -228,8 +228,8 @@ __device__ __forceinline__ void stage_terms(const KArgs &a, const AppDesc &ad, c
 // bound ((N_x N_y) N_z) ||f_ft|| is formed in FP64 in the oracle's operation
 // order and a ballot appends the terms that pass the reference rule
 // (bound > thr) and fall in the launch's class (bound > thr_lo and not
-// bound > thr_hi): emit(pos, p, ft, ll, bound) is called by each taking lane
-// with its worklist position.  Stops before a pair that could overflow the
+// bound > thr_hi): emit(pos, p, ft, lx, ly, lz, bound) is called by each
+// taking lane with its worklist position.  Stops before a pair that could overflow the
 // worklist (cap items) and leaves pi there.  Returns the item count.
 template <typename Emit>
 __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, const JobCtx &j, int &pi, int pi_end,
@@ -273,7 +273,7 @@ __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, co
             const double bound = ((nx * ny) * nz) * fn;
             const bool take = have && bound > j.thr && bound > j.thr_lo && !(bound > j.thr_hi);
             const unsigned long long mask = __ballot(take);
-            if (take) emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll, bound);
+            if (take) emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll[0], ll[1], ll[2], bound);
             n += __popcll(mask);
         }
         pi += 64;
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
         while (pi < 8 * p_end) {
             // screening into the worklist (wave 0 writes the items)
             const int n = screen_scan(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane,
-                                      [&](int pos, int p, int ft, const int *, double) {
+                                      [&](int pos, int p, int ft, int, int, int, double) {
                                           if (wv == 0) wl[pos] = (p << 9) | (ft << 6) | lane;
                                       });
             __syncthreads();
@@ -477,8 +477,10 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
 //   [hi 0..7 | lo 0..7 | hi 0..7 | tail]
 // and the operator side
 //   [hi 0..7 | hi 0..7 | lo 0..7 | tail]
-// where the tails carry the same three products for j = 8, 9 (N <= 10:
-// data {hi8 hi9 lo8 lo9 hi8 hi9 0 0}, operator {hi8 hi9 hi8 hi9 lo8 lo9 0 0})
+// where the tails carry the same three products for j = 8, 9 (N = 9, 10:
+// data {hi8 hi9 hi8 hi9 lo8 lo9 lo8 lo9}, operator {hi8 hi9 lo8 lo9 hi8 hi9 0 0};
+// the data tail is the plain {hi | lo} of lines 8, 9, 8, 9, so a pass that
+// computes lines 10, 11 as copies of lines 8, 9 stores it like any chunk)
 // or hi*hi + lo*hi for j = 8..11 (N = 11, 12: data {hi8..11 lo8..11},
 // operator {hi8..11 hi8..11}); for N = 11, 12 an extra chunk holds hi8..11
 // (data) / lo8..11 (operator) for a v_mfma_f32_16x16x16f16 in lane chunk 0.
@@ -529,8 +531,8 @@ __device__ __forceinline__ void split_row(const double *u, _Float16 *row) {
     if (N <= 8) {
         for (int e = 0; e < 8; e++) t[e] = z;
     } else if (N <= 10) {
-        const _Float16 td[8] = {hi[8], hi[9], lo[8], lo[9], hi[8], hi[9], z, z};
-        const _Float16 to[8] = {hi[8], hi[9], hi[8], hi[9], lo[8], lo[9], z, z};
+        const _Float16 td[8] = {hi[8], hi[9], hi[8], hi[9], lo[8], lo[9], lo[8], lo[9]};
+        const _Float16 to[8] = {hi[8], hi[9], lo[8], lo[9], hi[8], hi[9], z, z};
         for (int e = 0; e < 8; e++) t[e] = DATA ? td[e] : to[e];
     } else {
         for (int e = 0; e < 4; e++) {
@@ -562,6 +564,15 @@ __device__ __forceinline__ mwg_float4 split_mfma(const mwg_h8 &a, const mwg_h4 &
         for (int r = 0; r < 4; r++) c[r] += e[r];
     }
     return c;
+}
+
+// a for P = 0, b for P = 1, chosen at compile time
+template <int P, typename T>
+__device__ __forceinline__ T &pick(T &a, T &b) {
+    if constexpr (P == 0)
+        return a;
+    else
+        return b;
 }
 
 // Global-memory view of a pointer, so that loads through it are global, not
@@ -596,14 +607,17 @@ constexpr int kSingleBit = 30;
 // FP16) before the next pass; their magnitudes stay below N^2.  Items whose
 // bound is at most single_mult * thr_cls keep the intermediates rounded
 // once (lo = 0): their arithmetic error stays below ~1e-3 of the bound.
-// Lanes without a row to read take an all-zero row, lanes without a row to
-// write write a scratch row, so no pass branches on the lane.
+// Lanes whose operand row lies past the last line read a line that exists
+// (line j - 2 or N - 1, or row 0 of the block): their products meet zero
+// operand slots or land in output rows and columns that are not stored.
+// Lanes without a row to write write a scratch row, and the single-rounding
+// choice is a bit mask, so no pass branches on the lane or the item.
 constexpr int kSplitCap = 384;  // worklist items per batch (>= 64 + 128)
 
 template <int N>
 __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
-                                                             long long pool_zero, const int *seg_list, double lo_mult,
-                                                             double hi_mult, double single_mult, int cls_counter) {
+                                                             const int *seg_list, double lo_mult, double hi_mult,
+                                                             double single_mult, int cls_counter) {
     using S = SplitShape<N>;
     constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTZ = S::NTZ;
     constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, NW = S::NW;
@@ -636,137 +650,156 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     const __attribute__((address_space(1))) double *opsc = gmem(ad.blk_hsc);
     const __attribute__((address_space(1))) _Float16 *poolg = gmem(pool_h);
 
-    // Per-lane LDS offsets (halfs) of each tile: x-pass store row and the
-    // position of its four values, y-pass load, y-pass store, z-pass load.
-    int xs_row[TWG], xs_j0[TWG], yl_off[TWG], ys_row[TWG], ys_j0[TWG], zl_off[TWZ];
+    // chunk offsets (halfs) this lane reads: data side (0, 1, 0, 2), operator
+    // side (0, 0, 1, 2); the extra chunk (N > 10) only in lane chunk 0
+    const int dch = 8 * (lq == 1 ? 1 : (lq == 3 ? 2 : 0));
+    const int och = 8 * (lq == 2 ? 1 : (lq == 3 ? 2 : 0));
+    // line read for a row index past the last line
+    auto line = [](int jj) { return jj < N ? jj : (jj - 2 < N ? jj - 2 : N - 1); };
+
+    // Per-lane offsets (halfs), fixed for the kernel: LDS stores of the x and
+    // y passes (hi part, lo part, extra chunk), LDS operand reads of the y and
+    // z passes (chunk, extra chunk), F-block reads of the x pass (chunk,
+    // extra chunk) and operator-block reads (row min(lc, N - 1)).
+    int xs1[TWG], xs2[TWG], xse[TWG], ys1[TWG], ys2[TWG], yse[TWG];
+    int yl[TWG], yle[TWG], fl[TWG], fle[TWG], zl[TWZ], zle[TWZ];
 #pragma unroll
     for (int tt = 0; tt < TWG; tt++) {
         const int t = wv + NW * tt;
         const int g4 = 4 * t + lq;  // group of this lane's four results
         const bool okx = g4 < NG && lc < N;
         // x: group g4 = (jz, jy / 4) = (g4 / G, g4 % G), lane ix = lc -> Xt row jz N + ix
-        xs_row[tt] = (okx ? (g4 / G) * N + lc : DROW) * RH;
-        xs_j0[tt] = 4 * (g4 % G);
+        {
+            const int row = (okx ? (g4 / G) * N + lc : DROW) * RH, j0 = 4 * (g4 % G);
+            const bool tail = N > 8 && j0 >= 8;
+            xs1[tt] = row + (tail ? 16 : j0);
+            xs2[tt] = row + (tail ? 20 : 8 + j0);
+            xse[tt] = (S::TWO && tail) ? row + 24 : DROW * RH;
+        }
         // y: group g4 = (ix, jz / 4) = (g4 % N, g4 / N), lane iy = lc -> Yt row ix N + iy
-        ys_row[tt] = (okx ? (g4 % N) * N + lc : DROW) * RH;
-        ys_j0[tt] = 4 * (g4 / N);
-        // y-pass A row of lane lc: group g5, jz = 4 (g5 / N) + (lc & 3), ix = g5 % N
-        const int g5 = 4 * t + (lc >> 2), jz = 4 * (g5 / N) + (lc & 3);
-        yl_off[tt] = ((g5 < NG && jz < N) ? jz * N + g5 % N : ZROW) * RH;
+        {
+            const int row = (okx ? (g4 % N) * N + lc : DROW) * RH, j0 = 4 * (g4 / N);
+            const bool tail = N > 8 && j0 >= 8;
+            ys1[tt] = row + (tail ? 16 : j0);
+            ys2[tt] = row + (tail ? 20 : 8 + j0);
+            yse[tt] = (S::TWO && tail) ? row + 24 : DROW * RH;
+        }
+        // operand rows of lane lc: group g5 = 4 t + lc / 4, line 4 (g5 % G or
+        // g5 / N) + lc % 4 of the x pass (F: jz = g5 / G) and the y pass
+        // (Xt: ix = g5 % N)
+        const int g5 = 4 * t + (lc >> 2);
+        {
+            const int jz = 4 * (g5 / N) + (lc & 3);
+            const int r5 = (g5 < NG ? line(jz) * N + g5 % N : ZROW) * RH;
+            yl[tt] = r5 + dch;
+            yle[tt] = lq == 0 ? r5 + 24 : ZROW * RH;
+        }
+        {
+            const int jy = 4 * (g5 % G) + (lc & 3);
+            const int r5 = (g5 < NG ? (g5 / G) * N + line(jy) : 0) * RH;
+            fl[tt] = r5 + dch;
+            fle[tt] = r5 + 24;
+        }
     }
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) {
         const int col = 16 * (wv + NW * tt) + lc;
-        zl_off[tt] = (col < N2 ? col : ZROW) * RH;
+        const int r = (col < N2 ? col : ZROW) * RH;
+        zl[tt] = r + dch;
+        zle[tt] = lq == 0 ? r + 24 : ZROW * RH;
     }
+    const int ol = min(lc, N - 1) * RH + och, ole = min(lc, N - 1) * RH + 24;
 
     mwg_float4 acc[TWZ];
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) acc[tt] = mwg_float4{0.f, 0.f, 0.f, 0.f};
     unsigned long long napps = 0;
 
-    // chunk offsets (halfs) this lane reads: data side (0, 1, 0, 2), operator
-    // side (0, 0, 1, 2); the extra chunk (N > 10) only in lane chunk 0
-    const int dch = 8 * (lq == 1 ? 1 : (lq == 3 ? 2 : 0));
-    const int och = 8 * (lq == 2 ? 1 : (lq == 3 ? 2 : 0));
-    // data-side operand from the LDS row at `row` (halfs)
-    auto lds_operand = [&](const _Float16 *base, int row, mwg_h8 &v, mwg_h4 &e) {
-        v = *reinterpret_cast<const mwg_h8 *>(base + row + dch);
-        if (S::TWO) e = *reinterpret_cast<const mwg_h4 *>(base + (lq == 0 ? row + 24 : ZROW * RH));
-    };
-    // operand from a global split row (ch: the lane's chunk offset)
-    auto g_operand = [&](const __attribute__((address_space(1))) _Float16 *row, int ch, mwg_h8 &v, mwg_h4 &e) {
-        v = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(row + ch);
-        e = zero4;
-        if (S::TWO && lq == 0) e = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h4 *>(row + 24);
-    };
-    // four results of one tile into a data-side split row
-    auto store4 = [&](_Float16 *base, int row, int j0, const mwg_float4 &v, bool single) {
-        mwg_h4 hi, lo;
-        if (single) {
+    // four results of one tile into a data-side split row: hi keeps the bits
+    // of msk (0xffffe000: an 11-bit mantissa, so hi converts exactly and
+    // lo = v - hi is the exact remainder; all ones: single rounding, lo = 0)
+    auto store4 = [&](_Float16 *base, int o1, int o2, int oe, const mwg_float4 &v, unsigned msk) {
+        float h[4], l[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) hi[r] = (_Float16)v[r];
-            lo = zero4;
-        } else {
-            float h[4], l[4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                h[r] = __uint_as_float(__float_as_uint(v[r]) & 0xffffe000u);
-                l[r] = v[r] - h[r];
-            }
-            hi = mwg_h4{(_Float16)h[0], (_Float16)h[1], (_Float16)h[2], (_Float16)h[3]};
-            lo = mwg_h4{(_Float16)l[0], (_Float16)l[1], (_Float16)l[2], (_Float16)l[3]};
+        for (int r = 0; r < 4; r++) {
+            h[r] = __uint_as_float(__float_as_uint(v[r]) & msk);
+            l[r] = v[r] - h[r];
         }
-        // j0 < 8: hi -> chunk 0, lo -> chunk 1
-        // j0 = 8, N <= 10: tail chunk = {hi0 hi1 lo0 lo1 | hi0 hi1 0 0}
-        // j0 = 8, N > 10: tail chunk = {hi0..3 | lo0..3}, extra chunk = {hi0..3 | -}
-        const bool tail = j0 >= 8;
-        mwg_h4 d1 = hi, d2 = lo;
-        int o1 = row + j0, o2 = row + 8 + j0;
-        if (N > 8 && N <= 10) {
-            d1 = tail ? mwg_h4{hi[0], hi[1], lo[0], lo[1]} : hi;
-            d2 = tail ? mwg_h4{hi[0], hi[1], z16, z16} : lo;
-            o1 = tail ? row + 16 : o1;
-            o2 = tail ? row + 20 : o2;
-        } else if (N > 10) {
-            o1 = tail ? row + 16 : o1;
-            o2 = tail ? row + 20 : o2;
-            *reinterpret_cast<mwg_h4 *>(base + (tail ? row + 24 : DROW * RH)) = hi;
-        }
-        *reinterpret_cast<mwg_h4 *>(base + o1) = d1;
-        *reinterpret_cast<mwg_h4 *>(base + o2) = d2;
+        const mwg_h4 hi = mwg_h4{(_Float16)h[0], (_Float16)h[1], (_Float16)h[2], (_Float16)h[3]};
+        const mwg_h4 lo = mwg_h4{(_Float16)l[0], (_Float16)l[1], (_Float16)l[2], (_Float16)l[3]};
+        *reinterpret_cast<mwg_h4 *>(base + o1) = hi;
+        *reinterpret_cast<mwg_h4 *>(base + o2) = lo;
+        if (S::TWO) *reinterpret_cast<mwg_h4 *>(base + oe) = hi;
+    };
+    auto lds_operand = [&](const _Float16 *base, int off, int offe, mwg_h8 &v, mwg_h4 &e) {
+        v = *reinterpret_cast<const mwg_h8 *>(base + off);
+        if (S::TWO) e = *reinterpret_cast<const mwg_h4 *>(base + offe);
     };
 
     // x-pass A operand (data side): rows (jz, jy) of the F block, in groups
     mwg_h8 fa[TWG];
     mwg_h4 fe[TWG];
+#pragma unroll
+    for (int tt = 0; tt < TWG; tt++) fe[tt] = zero4;
     float sF = 0.f;
     auto load_f = [&](int grp) {
         const int s = a.band[j.boff + (grp >> 3)];
         const long long cb = a.slot_off[s] / N3 + (grp & 7);  // component block of the pool
         sF = (float)pool_sc[cb];
+        const __attribute__((address_space(1))) _Float16 *fb = poolg + cb * (long long)(N2 * RH);
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
-            const int t = wv + NW * tt;
-            const int g4 = 4 * t + (lc >> 2), jy = 4 * (g4 % G) + (lc & 3);
-            const bool ok = g4 < NG && jy < N;
-            g_operand(poolg + (ok ? cb * (long long)(N2 * RH) + ((g4 / G) * N + jy) * RH : pool_zero), dch, fa[tt],
-                      fe[tt]);
+            fa[tt] = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(fb + fl[tt]);
+            if (S::TWO) {
+                const mwg_h4 e = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h4 *>(fb + fle[tt]);
+                fe[tt] = lq == 0 ? e : zero4;
+            }
         }
     };
-    // operator row lc of block blk (uniform), chunk lq
+    // operator block blk (uniform): row min(lc, N - 1), the lane's chunk; the
+    // extra chunk of lanes lq > 0 meets zero data slots
     auto load_op = [&](int blk, mwg_h8 &v, mwg_h4 &e) {
-        g_operand(opg + (lc < N ? blk * (long long)(N * RH) + lc * RH : ad.blk_hzero), och, v, e);
+        const __attribute__((address_space(1))) _Float16 *b = opg + (long long)blk * (N * RH);
+        v = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(b + ol);
+        if (S::TWO) e = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h4 *>(b + ole);
     };
     // The passes of one phase: x of item q into X[q & 1], y of item q - 1
     // from X[(q - 1) & 1] into Y[(q - 1) & 1], z of item q - 2 from
     // Y[q & 1] into acc.  Each buffer a pass reads was written in the
     // previous phase and each buffer it writes was last read in the previous
     // phase, so one barrier per phase orders them.
-    auto x_pass = [&](_Float16 *X, const mwg_h8 &o, const mwg_h4 &oe, bool single) {
+    auto x_pass = [&](_Float16 *X, const mwg_h8 &o, const mwg_h4 &oe, unsigned msk) {
+        mwg_float4 d[TWG];
 #pragma unroll
-        for (int tt = 0; tt < TWG; tt++) {
-            const mwg_float4 dx = split_mfma<N>(fa[tt], fe[tt], o, oe);
-            store4(X, xs_row[tt], xs_j0[tt], dx, single);
-        }
+        for (int tt = 0; tt < TWG; tt++) d[tt] = split_mfma<N>(fa[tt], fe[tt], o, oe);
+#pragma unroll
+        for (int tt = 0; tt < TWG; tt++) store4(X, xs1[tt], xs2[tt], xse[tt], d[tt], msk);
     };
-    auto y_pass = [&](const _Float16 *X, _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, bool single) {
+    auto y_pass = [&](const _Float16 *X, _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, unsigned msk) {
+        mwg_h8 va[TWG];
+        mwg_h4 ve[TWG];
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
-            mwg_h8 va;
-            mwg_h4 ve = zero4;
-            lds_operand(X, yl_off[tt], va, ve);
-            const mwg_float4 dy = split_mfma<N>(va, ve, o, oe);
-            store4(Y, ys_row[tt], ys_j0[tt], dy, single);
+            ve[tt] = zero4;
+            lds_operand(X, yl[tt], yle[tt], va[tt], ve[tt]);
         }
+        mwg_float4 d[TWG];
+#pragma unroll
+        for (int tt = 0; tt < TWG; tt++) d[tt] = split_mfma<N>(va[tt], ve[tt], o, oe);
+#pragma unroll
+        for (int tt = 0; tt < TWG; tt++) store4(Y, ys1[tt], ys2[tt], yse[tt], d[tt], msk);
     };
     auto z_pass = [&](const _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, float sigma) {
+        mwg_h8 vb[TWZ];
+        mwg_h4 vbe[TWZ];
 #pragma unroll
         for (int tt = 0; tt < TWZ; tt++) {
-            mwg_h8 vb;
-            mwg_h4 vbe = zero4;
-            lds_operand(Y, zl_off[tt], vb, vbe);
-            const mwg_float4 dz = split_mfma<N>(o, oe, vb, vbe);
+            vbe[tt] = zero4;
+            lds_operand(Y, zl[tt], zle[tt], vb[tt], vbe[tt]);
+        }
+#pragma unroll
+        for (int tt = 0; tt < TWZ; tt++) {
+            const mwg_float4 dz = split_mfma<N>(o, oe, vb[tt], vbe[tt]);
 #pragma unroll
             for (int r = 0; r < 4; r++) acc[tt][r] = fmaf(sigma, dz[r], acc[tt][r]);
         }
@@ -786,12 +819,12 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             // item's three operator block ids; wave 0 writes the items
             const int n = screen_scan(
                 a, ad, j, pi, 8 * p_end, cnt, nrm_s, kSplitCap, lane,
-                [&](int pos, int p, int ft, const int *ll, double bound) {
+                [&](int pos, int p, int ft, int lx, int ly, int lz, double bound) {
                     if (wv != 0) return;
                     SplitItem itm;
-                    itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * (2 * my_L + 1)) + ll[0];
-                    itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * (2 * my_L + 1)) + ll[1];
-                    itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + ll[2];
+                    itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * (2 * my_L + 1)) + lx;
+                    itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * (2 * my_L + 1)) + ly;
+                    itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + lz;
                     itm.grp = (p << 3) | ft | ((bound <= thr_single ? 1 : 0) << kSingleBit);
                     wl[pos] = itm;
                     wsig[pos] = (float)(opsc[itm.bx] * opsc[itm.by] * opsc[itm.bz]);
@@ -800,44 +833,51 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             if (n == 0) continue;
             napps += n;
             // software pipeline over the batch: phase q runs x(q), y(q - 1),
-            // z(q - 2); the operators of the next phase load meanwhile
+            // z(q - 2) with operator set q & 1 while the next phase's
+            // operators load into the other set; phases are unrolled in pairs
+            // so that the set and the LDS buffers are fixed in each copy
             const int mask_grp = (1 << kSingleBit) - 1;
-            mwg_h8 ox, oy, oz, oxn, oyn, ozn;
-            mwg_h4 ex = zero4, ey = zero4, ez = zero4, exn = zero4, eyn = zero4, ezn = zero4;
+            mwg_h8 o0[3], o1[3];  // operator sets: Ox, Oy, Oz
+            mwg_h4 e0[3], e1[3];
+#pragma unroll
+            for (int u = 0; u < 3; u++) e0[u] = e1[u] = zero4;
             int cur_grp = -1;
             float sig1 = 0.f, sig2 = 0.f;  // scales of items q - 1, q - 2
-            bool sng1 = false;             // single rounding of item q - 1
-            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), ox, ex);
-            for (int q = 0; q < n + 2; q++) {
+            unsigned msk1 = 0u;            // hi mask of item q - 1
+            load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), o0[0], e0[0]);
+            auto phase = [&](int q, auto par) {
+                constexpr int P = decltype(par)::value;
+                auto &oc = pick<P>(o0, o1);
+                auto &ec = pick<P>(e0, e1);
+                auto &on = pick<1 - P>(o0, o1);
+                auto &en = pick<1 - P>(e0, e1);
                 const bool hx = q < n, hy = q >= 1 && q <= n, hz = q >= 2;
                 // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1)
-                if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), oxn, exn);
-                if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), oyn, eyn);
-                if (hy) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz), ozn, ezn);
+                if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), on[0], en[0]);
+                if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), on[1], en[1]);
+                if (hy) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz), on[2], en[2]);
                 float sig0 = 0.f;
-                bool sng0 = false;
+                unsigned msk0 = 0u;
                 if (hx) {
                     const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
                     if ((g & mask_grp) != cur_grp) {
                         cur_grp = g & mask_grp;
                         load_f(cur_grp);
                     }
-                    sng0 = (g >> kSingleBit) & 1;
+                    msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
                     sig0 = sF * wsig[q];
-                    x_pass(xt + (q & 1) * BUFH, ox, ex, sng0);
+                    x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
                 }
-                if (hy) y_pass(xt + ((q - 1) & 1) * BUFH, yt + ((q - 1) & 1) * BUFH, oy, ey, sng1);
-                if (hz) z_pass(yt + (q & 1) * BUFH, oz, ez, sig2);
+                if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
+                if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], sig2);
                 __syncthreads();
-                ox = oxn;
-                ex = exn;
-                oy = oyn;
-                ey = eyn;
-                oz = ozn;
-                ez = ezn;
                 sig2 = sig1;
                 sig1 = sig0;
-                sng1 = sng0;
+                msk1 = msk0;
+            };
+            for (int q = 0; q < n + 2; q += 2) {
+                phase(q, std::integral_constant<int, 0>{});
+                if (q + 1 < n + 2) phase(q + 1, std::integral_constant<int, 1>{});
             }
             __syncthreads();
         }
@@ -865,18 +905,12 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
 
 // FP16 split rows of FP64 blocks (rows x N values each, row-major), data or
 // operator side: one wave per block; scale[b] = 2^e with max |v| < 2^e
-// (0 for a zero block).  Block nblocks (one past the last) is written as
-// zeros: lanes without an operand row read it.
+// (0 for a zero block).
 template <int N, bool DATA>
 __global__ void k_split_rows(const double *src, long long nblocks, int rows, _Float16 *dst, double *scale) {
     const long long b = blockIdx.x;
     const int lane = threadIdx.x;
-    if (b >= nblocks) {
-        // the extra block after the last one: all zeros
-        _Float16 *o = dst + nblocks * rows * SplitShape<N>::RH;
-        for (int e = lane; e < rows * SplitShape<N>::RH; e += 64) o[e] = (_Float16)0.0f;
-        return;
-    }
+    if (b >= nblocks) return;
     const double *s = src + b * (long long)rows * N;
     double m = 0.0;
     for (int e = lane; e < rows * N; e += 64) m = fmax(m, fabs(s[e]));
@@ -918,7 +952,6 @@ struct DevOper {
     float *blocks32 = nullptr;  // FP32 copy, made on first mixed-precision use
     _Float16 *blocks_h = nullptr;  // FP16 split rows, made on first split use
     double *blocks_hsc = nullptr;
-    long long blocks_hzero = 0;    // offset (halfs) of their all-zero block
     long long nblk = 0;
     int M = 0, ndepth = 0;
 };
@@ -942,7 +975,6 @@ struct DevicePlan {
     float *pool32 = nullptr;
     _Float16 *pool_h = nullptr;  // FP16 split rows of the pool (split mode)
     double *pool_hsc = nullptr;
-    long long pool_hzero = 0;    // offset (halfs) of its all-zero block
     KArgs args{};
     std::vector<AppDesc> apps;          // host copy of args.apps
     AppDesc *d_apps = nullptr;
@@ -1387,14 +1419,15 @@ static int split_rh(int N) { return N > 10 ? 32 : 24; }
 template <int N>
 static int split_rows_n(const double *src, long long nblocks, int rows, bool data, _Float16 **dst, double **scale,
                         hipStream_t s, std::string &err) {
-    HIP_TRY(hipMalloc((void **)dst, sizeof(_Float16) * (size_t)(nblocks + 1) * rows * SplitShape<N>::RH));
+    HIP_TRY(hipMalloc((void **)dst, sizeof(_Float16) * (size_t)std::max<long long>(nblocks, 1) * rows * SplitShape<N>::RH));
     HIP_TRY(hipMalloc((void **)scale, sizeof(double) * (size_t)std::max<long long>(nblocks, 1)));
+    if (nblocks == 0) return 0;
     if (data)
-        hipLaunchKernelGGL((k_split_rows<N, true>), dim3((unsigned)(nblocks + 1)), dim3(64), 0, s, src, nblocks, rows,
-                           *dst, *scale);
+        hipLaunchKernelGGL((k_split_rows<N, true>), dim3((unsigned)nblocks), dim3(64), 0, s, src, nblocks, rows, *dst,
+                           *scale);
     else
-        hipLaunchKernelGGL((k_split_rows<N, false>), dim3((unsigned)(nblocks + 1)), dim3(64), 0, s, src, nblocks, rows,
-                           *dst, *scale);
+        hipLaunchKernelGGL((k_split_rows<N, false>), dim3((unsigned)nblocks), dim3(64), 0, s, src, nblocks, rows, *dst,
+                           *scale);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1443,12 +1476,10 @@ static int ensure_h16(DevicePlan *p, std::string &err) {
             const long long nb = o.nblk / ((long long)N * N);
             int rc = split_rows(N, o.blocks, nb, N, false, &o.blocks_h, &o.blocks_hsc, c->stream, err);
             if (rc) return rc;
-            o.blocks_hzero = nb * N * split_rh(N);
         }
         if (p->apps[i].blk_h != o.blocks_h) {
             p->apps[i].blk_h = o.blocks_h;
             p->apps[i].blk_hsc = o.blocks_hsc;
-            p->apps[i].blk_hzero = o.blocks_hzero;
             changed = true;
         }
     }
@@ -1458,7 +1489,6 @@ static int ensure_h16(DevicePlan *p, std::string &err) {
     }
     if (!p->pool_h) {
         const long long nb = p->npool / ((long long)N * N * N);
-        p->pool_hzero = nb * N * N * split_rh(N);
         return split_rows(N, p->pool, nb, N * N, true, &p->pool_h, &p->pool_hsc, c->stream, err);
     }
     return 0;
@@ -1479,7 +1509,7 @@ static int launch_h_n(const DevicePlan *p, const int *list, int n_seg, double lo
         return -1;
     }
     hipLaunchKernelGGL((k_apply_h<N>), dim3(n_seg), dim3(SplitShape<N>::T), shmem, p->ctx->stream, p->args,
-                       p->pool_h, p->pool_hsc, p->pool_hzero, list, lo, hi, kSingleRatio, cls);
+                       p->pool_h, p->pool_hsc, list, lo, hi, kSingleRatio, cls);
     HIP_TRY(hipGetLastError());
     return 0;
 }

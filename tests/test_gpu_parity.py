@@ -111,11 +111,12 @@ def test_mixed_precision_term_set_and_error(ctx, order):
     assert st["term_apps"] == r64["term_apps"]
 
 
-@pytest.mark.parametrize("order,prec", [(4, 1e-5), (7, 1e-6), (9, 1e-6), (11, 1e-5)])
+@pytest.mark.parametrize("order,prec", [(4, 1e-5), (5, 1e-5), (7, 1e-6), (8, 1e-6), (9, 1e-6), (10, 1e-5),
+                                        (11, 1e-5)])
 def test_split16_term_set_and_error(ctx, order, prec):
     """FP16 hi/lo split products: same term set as FP64 and within prec of
     the FP64 result on the identical grid (N <= 8, N = 9..10 and N = 11..12
-    take different operand layouts)."""
+    take different operand layouts; N = 9 and 11 have partly empty tails)."""
     mra = _mra(order)
     P = mw.PoissonOperator(mra, prec)
     rng = np.random.default_rng(23)
