@@ -508,7 +508,13 @@ struct SplitShape {
     // consecutive rows over distinct LDS banks; for N > 10 the four chunks
     // keep the double-buffered intermediates within 64 KB of LDS.
     static constexpr int NC = TWO ? 4 : 3;
-    static constexpr int RH = 8 * NC;           // halfs per row
+    static constexpr int RH = 8 * NC;           // halfs per global row
+    // LDS rows of the intermediates: hi, lo, tail in 48 bytes (an odd number
+    // of 16-byte chunks, so 16-byte reads of 8 consecutive rows hit distinct
+    // banks); the extra chunk of N > 10 sits in a region of its own, 8 bytes
+    // per row, after the rows.
+    static constexpr int RL = 24;               // halfs per LDS row
+    static constexpr int RE = TWO ? 4 : 0;      // halfs per LDS extra chunk
 };
 
 // Split row of n <= 12 values u (|u| < 1): data side (DATA) or operator side.
@@ -620,9 +626,10 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                                                              double single_mult, int cls_counter) {
     using S = SplitShape<N>;
     constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTZ = S::NTZ;
-    constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, NW = S::NW;
+    constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, RL = S::RL, RE = S::RE, NW = S::NW;
     constexpr int ZROW = N2, DROW = N2 + 1;  // all-zero row, scratch row
-    constexpr int BUFH = (N2 + 2) * RH;  // halfs per intermediate buffer
+    constexpr int EXT0 = (N2 + 2) * RL;        // start of the extra chunks
+    constexpr int BUFH = (N2 + 2) * (RL + RE);  // halfs per intermediate buffer
     __shared__ __align__(16) _Float16 xt[2 * BUFH];  // X of items q, q - 1 (by parity)
     __shared__ __align__(16) _Float16 yt[2 * BUFH];  // Y of items q - 1, q - 2
     __shared__ __align__(16) SplitItem wl[kSplitCap];
@@ -670,19 +677,19 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         const bool okx = g4 < NG && lc < N;
         // x: group g4 = (jz, jy / 4) = (g4 / G, g4 % G), lane ix = lc -> Xt row jz N + ix
         {
-            const int row = (okx ? (g4 / G) * N + lc : DROW) * RH, j0 = 4 * (g4 % G);
+            const int ri = okx ? (g4 / G) * N + lc : DROW, row = ri * RL, j0 = 4 * (g4 % G);
             const bool tail = N > 8 && j0 >= 8;
             xs1[tt] = row + (tail ? 16 : j0);
             xs2[tt] = row + (tail ? 20 : 8 + j0);
-            xse[tt] = (S::TWO && tail) ? row + 24 : DROW * RH;
+            xse[tt] = EXT0 + (tail ? ri : DROW) * RE;
         }
         // y: group g4 = (ix, jz / 4) = (g4 % N, g4 / N), lane iy = lc -> Yt row ix N + iy
         {
-            const int row = (okx ? (g4 % N) * N + lc : DROW) * RH, j0 = 4 * (g4 / N);
+            const int ri = okx ? (g4 % N) * N + lc : DROW, row = ri * RL, j0 = 4 * (g4 / N);
             const bool tail = N > 8 && j0 >= 8;
             ys1[tt] = row + (tail ? 16 : j0);
             ys2[tt] = row + (tail ? 20 : 8 + j0);
-            yse[tt] = (S::TWO && tail) ? row + 24 : DROW * RH;
+            yse[tt] = EXT0 + (tail ? ri : DROW) * RE;
         }
         // operand rows of lane lc: group g5 = 4 t + lc / 4, line 4 (g5 % G or
         // g5 / N) + lc % 4 of the x pass (F: jz = g5 / G) and the y pass
@@ -690,9 +697,9 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         const int g5 = 4 * t + (lc >> 2);
         {
             const int jz = 4 * (g5 / N) + (lc & 3);
-            const int r5 = (g5 < NG ? line(jz) * N + g5 % N : ZROW) * RH;
-            yl[tt] = r5 + dch;
-            yle[tt] = lq == 0 ? r5 + 24 : ZROW * RH;
+            const int ri = g5 < NG ? line(jz) * N + g5 % N : ZROW;
+            yl[tt] = ri * RL + dch;
+            yle[tt] = EXT0 + (lq == 0 ? ri : ZROW) * RE;
         }
         {
             const int jy = 4 * (g5 % G) + (lc & 3);
@@ -704,9 +711,9 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) {
         const int col = 16 * (wv + NW * tt) + lc;
-        const int r = (col < N2 ? col : ZROW) * RH;
-        zl[tt] = r + dch;
-        zle[tt] = lq == 0 ? r + 24 : ZROW * RH;
+        const int ri = col < N2 ? col : ZROW;
+        zl[tt] = ri * RL + dch;
+        zle[tt] = EXT0 + (lq == 0 ? ri : ZROW) * RE;
     }
     const int ol = min(lc, N - 1) * RH + och, ole = min(lc, N - 1) * RH + 24;
 
