@@ -894,8 +894,12 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         if (cls_counter > 0) atomicAdd(&a.counters[cls_counter], napps);
     }
     if (napps == 0) return;
-    // acc[tt][r] = Z[iz = 4 lq + r][col = 16 t + lc], added in FP64 to the output
-    double *dst = a.out_ns + (long long)j.g * 8 * N3 + (long long)j.gt * N3;
+    // acc[tt][r] = Z[iz = 4 lq + r][col = 16 t + lc] with col = ix N + iy:
+    // put in (iz, iy, ix) order in LDS, then added in FP64 to the output
+    // with consecutive lanes on consecutive elements
+    static_assert(N3 * sizeof(float) <= sizeof(xt), "output staging does not fit");
+    float *zs = reinterpret_cast<float *>(xt);
+    __syncthreads();
 #pragma unroll
     for (int tt = 0; tt < TWZ; tt++) {
         const int t = wv + NW * tt;
@@ -904,10 +908,12 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int iz = 4 * lq + r;
-            // col = ix N + iy
-            if (iz < N) atomicAdd(&dst[iz * N2 + (col % N) * N + col / N], (double)acc[tt][r]);
+            if (iz < N) zs[iz * N2 + (col % N) * N + col / N] = acc[tt][r];
         }
     }
+    __syncthreads();
+    double *dst = a.out_ns + (long long)j.g * 8 * N3 + (long long)j.gt * N3;
+    for (int e = tid; e < N3; e += NTH) atomicAdd(&dst[e], (double)zs[e]);
 }
 
 // FP16 split rows of FP64 blocks (rows x N values each, row-major), data or
