@@ -81,6 +81,16 @@ int mwg_helmholtz_create(mwg_mra mra, double mu, double build_prec, mwg_operator
  * counterpart in MRChem's calls; used for verification). */
 int mwg_gaussian_operator_create(mwg_mra mra, int n_terms, const double *alpha, const double *beta,
                                  int max_depth, mwg_operator *op);
+/* Build n operators at once on the device of ctx: mu[i] = 0 gives the
+ * PoissonOperator(mra, build_prec) of src/driver.cpp:1131,1283, mu[i] > 0 the
+ * HelmholtzOperator(mra, mu[i], build_prec) that
+ * src/scf_solver/HelmholtzVector.cpp:144 builds for every orbital in every SCF
+ * iteration.  Same kernel fit and tables as mwg_poisson_create /
+ * mwg_helmholtz_create (values to a few ulps); the tables stay on ctx's
+ * device (mwg_operator_tables copies them out) and the operators can be used
+ * only with ctx. */
+int mwg_operators_create_device(mwg_context ctx, mwg_mra mra, int n, const double *mu, double build_prec,
+                                mwg_operator *ops);
 int mwg_operator_destroy(mwg_operator op);
 int mwg_operator_info(mwg_operator op, int *n_terms, int *n_depth, int *kp1, double *r_min, double *r_max,
                       double *fit_err);

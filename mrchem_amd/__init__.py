@@ -56,6 +56,7 @@ _sig("mwg_poisson_create", _p, _d, ctypes.POINTER(_p))
 _sig("mwg_helmholtz_create", _p, _d, _d, ctypes.POINTER(_p))
 _sig("mwg_gaussian_operator_create", _p, _i, _dp, _dp, _i, ctypes.POINTER(_p))
 _sig("mwg_operator_destroy", _p)
+_sig("mwg_operators_create_device", _p, _p, _i, _dp, _d, ctypes.POINTER(_p))
 _sig("mwg_operator_info", _p, _ip, _ip, _ip, _dp, _dp, _dp)
 _sig("mwg_operator_terms", _p, _dp, _dp)
 _sig("mwg_operator_table_sizes", _p, _llp, _llp)
@@ -245,6 +246,24 @@ class HelmholtzOperator(_Operator):
         _check(lib.mwg_helmholtz_create(mra._h, mu, prec, ctypes.byref(h)))
         super().__init__(mra, h)
         self.mu, self.prec = mu, prec
+
+
+def build_operators(mra, mus, prec, ctx=None):
+    """PoissonOperator (mu = 0) or HelmholtzOperator (mu > 0) for every mu,
+    built together on the device of ctx (HelmholtzVector.cpp:144 builds one
+    per orbital in every SCF iteration).  Same kernel fit and tables as the
+    host constructors; the tables stay on the device and the operators work
+    only with ctx."""
+    ctx = ctx or Context.default()
+    mus = np.ascontiguousarray(np.atleast_1d(mus), dtype=np.float64)
+    hs = (_p * len(mus))()
+    _check(lib.mwg_operators_create_device(ctx._h, mra._h, len(mus), _ptr(mus, _d), float(prec), hs))
+    ops = []
+    for i, mu in enumerate(mus):
+        op = _Operator(mra, _p(hs[i]))
+        op.mu, op.prec, op.ctx = float(mu), float(prec), ctx
+        ops.append(op)
+    return ops
 
 
 class GaussianOperator(_Operator):

@@ -1100,6 +1100,10 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
         *out = &it->second;
         return 0;
     }
+    if (f.device_blocks > 0) {
+        err = "operator tables were built on another context's device";
+        return -1;
+    }
     DevOper o;
     int rc = alloc_copy(f.L, &o.L, c->stream, err);
     if (!rc) rc = alloc_copy(f.boff, &o.boff, c->stream, err);
@@ -1115,6 +1119,60 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
     }
     c->opers[key] = o;
     *out = &c->opers[key];
+    return 0;
+}
+
+int dev_device(DeviceCtx *c) { return c->device; }
+void *dev_stream(DeviceCtx *c) { return c->stream; }
+
+int dev_alloc_oper_tables(DeviceCtx *c, const OperFlat &f, DevOperTables &t, std::string &err) {
+    t = DevOperTables();
+    HIP_TRY(hipMalloc((void **)&t.blocks, sizeof(double) * std::max<long long>(f.device_blocks, 1)));
+    HIP_TRY(hipMalloc((void **)&t.norms, sizeof(double) * std::max<size_t>(f.norms.size(), 1)));
+    HIP_TRY(hipMemsetAsync(t.blocks, 0, sizeof(double) * std::max<long long>(f.device_blocks, 1), c->stream));
+    HIP_TRY(hipMemsetAsync(t.norms, 0, sizeof(double) * std::max<size_t>(f.norms.size(), 1), c->stream));
+    int rc = alloc_copy(f.L, &t.L, c->stream, err);
+    if (!rc) rc = alloc_copy(f.boff, &t.boff, c->stream, err);
+    if (!rc) rc = alloc_copy(f.noff, &t.noff, c->stream, err);
+    return rc;
+}
+
+int dev_register_operator(DeviceCtx *c, const void *key, const OperFlat &f, const DevOperTables &t, std::string &err) {
+    (void)err;
+    auto it = c->opers.find(key);
+    if (it != c->opers.end()) {
+        (void)hipStreamSynchronize(c->stream);
+        free_oper(it->second);
+        c->opers.erase(it);
+    }
+    DevOper o;
+    o.L = t.L;
+    o.boff = t.boff;
+    o.noff = t.noff;
+    o.blocks = t.blocks;
+    o.norms = t.norms;
+    o.nblk = f.device_blocks;
+    o.M = f.M;
+    o.ndepth = f.ndepth;
+    c->opers[key] = o;
+    return 0;
+}
+
+int dev_download_operator_blocks(DeviceCtx *c, const void *key, double *out, long long n, std::string &err) {
+    HIP_TRY(hipSetDevice(c->device));
+    auto it = c->opers.find(key);
+    if (it == c->opers.end()) {
+        err = "operator tables are not resident on this context";
+        return -1;
+    }
+    if (n != it->second.nblk) {
+        err = "operator table size mismatch";
+        return -1;
+    }
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(out, it->second.blocks, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     return 0;
 }
 

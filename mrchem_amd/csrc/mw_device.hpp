@@ -44,4 +44,32 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
 int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err);
 void dev_plan_destroy(DevicePlan *p);
 
+// Device construction of operator tables (mw_opbuild.hip).  Each job holds an
+// operator prepared by prepare_operator (terms, no tables); the build fills
+// its L and norms on the host, its flat table header (f.blocks stays empty,
+// f.device_blocks counts the device values) and registers the device tables
+// in the context under `key`.
+struct OpBuildJob {
+    Operator *op;
+    OperFlat *flat;
+    const void *key;
+};
+int dev_build_operators(DeviceCtx *c, const MRA &mra, const std::vector<OpBuildJob> &jobs, std::string &err);
+// Copy the device table of a device-built operator (f.device_blocks values).
+int dev_download_operator_blocks(DeviceCtx *c, const void *key, double *out, long long n, std::string &err);
+
+// Helpers shared by the HIP sources.
+struct DevOperTables {
+    int *L = nullptr;
+    long long *boff = nullptr, *noff = nullptr;
+    double *blocks = nullptr, *norms = nullptr;
+};
+int dev_device(DeviceCtx *c);
+void *dev_stream(DeviceCtx *c);
+// Allocate the device tables of f (blocks and norms cleared, L/boff/noff uploaded).
+int dev_alloc_oper_tables(DeviceCtx *c, const OperFlat &f, DevOperTables &t, std::string &err);
+// Make t the context's resident copy of operator `key` (ownership passes to
+// the context).
+int dev_register_operator(DeviceCtx *c, const void *key, const OperFlat &f, const DevOperTables &t, std::string &err);
+
 }  // namespace mwg

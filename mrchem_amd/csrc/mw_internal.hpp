@@ -232,8 +232,22 @@ struct OperFlat {
     std::vector<int> L;             // [m*ndepth + d]
     std::vector<long long> boff, noff;
     std::vector<double> blocks, norms;
+    // > 0: the blocks were built on a device and live only there
+    // (this many values); blocks is then empty
+    long long device_blocks = 0;
 };
 void flatten_operator(const Operator &op, OperFlat &f);
+// Storage floor parameter of an operator built to build_prec.
+double operator_tau(double build_prec);
+// Operator header without tables: world extent, kernel fit and the terms
+// (alpha, beta, coef1d), each with max_depth + 1 empty depths.
+void prepare_operator(Operator &op, const MRA &mra, double mu, double build_prec, int max_depth);
+// Translation scan of one (term, depth): false when the depth stores nothing;
+// otherwise the scan bound Lscan and the position `peak` after which three
+// quiet translations end the scan.
+bool term_depth_scan(const MRA &mra, double alpha, int d, double tau, long long &Lscan, double &peak);
+// Gauss-Legendre rule of n points on [0, 1] (cached).
+void quad_rule01(int n, const double **x, const double **w);
 // Pre-screen table of an apply plan for the device screening: for each
 // output depth d in [d0, d0 + nd), component combination cmb = cx | cy << 2 |
 // cz << 4 and translation |l| <= w (the plan's widest band), the largest

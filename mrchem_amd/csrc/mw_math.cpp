@@ -87,6 +87,12 @@ const Rule &cached_rule(int n) {
 }
 }  // namespace
 
+void quad_rule01(int n, const double **x, const double **w) {
+    const Rule &r = cached_rule(n);
+    *x = r.x.data();
+    *w = r.w.data();
+}
+
 void gauss_moments01(int n, double A, double c, double *out) {
     for (int p = 0; p < n; p++) out[p] = 0.0;
     if (A < 0) throw std::invalid_argument("gauss_moments01: A < 0");

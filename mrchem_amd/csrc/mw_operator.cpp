@@ -121,19 +121,9 @@ void build_term_depth(const MRA &mra, OperTerm &term, int d, double tau) {
     od.L = -1;
     od.blocks.clear();
     od.norms.clear();
-    double hd = mra.box_size(d), hd1 = mra.box_size(d + 1);
-    double Ad = term.alpha * hd * hd, A1 = term.alpha * hd1 * hd1;
-    if (d > 0) {
-        // Wavelet components of a wide Gaussian are bounded by
-        // A^{(k+1)/2} sqrt(2^{k+1}/(k+1)!) relative to its s<-s block.
-        double lg = 0.5 * (n)*std::log(std::max(Ad, 1e-300)) + 0.5 * (n * std::log(2.0) - std::lgamma(n + 1.0));
-        if (lg < std::log(0.1 * tau)) return;
-    }
-    int bmax = std::max(mra.boxes[0], std::max(mra.boxes[1], mra.boxes[2]));
-    long long Lw = ((long long)bmax << d) - 1;
-    long long Ltail = 1 + (long long)std::ceil(std::sqrt(46.0 / std::max(A1, 1e-300)) / 2.0);
-    long long Lscan = std::min(Lw, std::min(Ltail, (long long)1 << 22));
-    double peak = std::sqrt(2.0 * (2 * n) / std::max(A1, 1e-300)) / 2.0 + 2.0;
+    long long Lscan = 0;
+    double peak = 0.0;
+    if (!term_depth_scan(mra, term.alpha, d, tau, Lscan, peak)) return;
     TCache tc{&mra, term.alpha, term.coef1d, d + 1, {}};
     std::vector<std::vector<double>> pos, neg;  // [l] -> 4*nn
     std::vector<double> pn, nnrm;               // norms [l][c]
@@ -185,6 +175,52 @@ void build_term_depth(const MRA &mra, OperTerm &term, int d, double tau) {
 }
 
 }  // namespace
+
+bool term_depth_scan(const MRA &mra, double alpha, int d, double tau, long long &Lscan, double &peak) {
+    const int n = mra.kp1();
+    double hd = mra.box_size(d), hd1 = mra.box_size(d + 1);
+    double Ad = alpha * hd * hd, A1 = alpha * hd1 * hd1;
+    if (d > 0) {
+        // Wavelet components of a wide Gaussian are bounded by
+        // A^{(k+1)/2} sqrt(2^{k+1}/(k+1)!) relative to its s<-s block.
+        double lg = 0.5 * (n)*std::log(std::max(Ad, 1e-300)) + 0.5 * (n * std::log(2.0) - std::lgamma(n + 1.0));
+        if (lg < std::log(0.1 * tau)) return false;
+    }
+    int bmax = std::max(mra.boxes[0], std::max(mra.boxes[1], mra.boxes[2]));
+    long long Lw = ((long long)bmax << d) - 1;
+    long long Ltail = 1 + (long long)std::ceil(std::sqrt(46.0 / std::max(A1, 1e-300)) / 2.0);
+    Lscan = std::min(Lw, std::min(Ltail, (long long)1 << 22));
+    peak = std::sqrt(2.0 * (2 * n) / std::max(A1, 1e-300)) / 2.0 + 2.0;
+    return true;
+}
+
+double operator_tau(double build_prec) { return std::min(1e-10, 1e-4 * build_prec); }
+
+void prepare_operator(Operator &op, const MRA &mra, double mu, double build_prec, int max_depth) {
+    if (!(build_prec > 0)) throw std::invalid_argument("operator build precision must be > 0");
+    if (max_depth < 0 || max_depth > mra.max_depth) max_depth = mra.max_depth;
+    op.mra = &mra;
+    op.mu = mu;
+    op.build_prec = build_prec;
+    op.max_depth = max_depth;
+    double diag = 0.0;
+    for (int d = 0; d < 3; d++) {
+        double w = mra.world_hi(d) - mra.world_lo(d);
+        diag += w * w;
+    }
+    op.r_max = std::sqrt(diag);
+    double k_prec = build_prec / 10.0;
+    double h_min = mra.box_size(mra.max_depth);
+    op.r_min = std::sqrt(k_prec * h_min);
+    std::vector<GaussTerm> g = fit_kernel(mu, k_prec, op.r_min, op.r_max, &op.fit_err);
+    op.terms.assign(g.size(), OperTerm());
+    for (size_t m = 0; m < g.size(); m++) {
+        op.terms[m].alpha = g[m].alpha;
+        op.terms[m].beta = g[m].beta;
+        op.terms[m].coef1d = std::cbrt(g[m].beta);
+        op.terms[m].depth.assign(max_depth + 1, OperDepth());
+    }
+}
 
 void flatten_operator(const Operator &op, OperFlat &f) {
     const int n = op.mra->kp1(), nn = n * n;
@@ -238,23 +274,10 @@ void build_operator_terms(Operator &op, const MRA &mra, const std::vector<GaussT
 }
 
 void build_operator(Operator &op, const MRA &mra, double mu, double build_prec, int max_depth) {
-    if (!(build_prec > 0)) throw std::invalid_argument("operator build precision must be > 0");
-    if (max_depth < 0 || max_depth > mra.max_depth) max_depth = mra.max_depth;
-    op.mra = &mra;
-    op.mu = mu;
-    op.build_prec = build_prec;
-    op.max_depth = max_depth;
-    double diag = 0.0;
-    for (int d = 0; d < 3; d++) {
-        double w = mra.world_hi(d) - mra.world_lo(d);
-        diag += w * w;
-    }
-    op.r_max = std::sqrt(diag);
-    double k_prec = build_prec / 10.0;
-    double h_min = mra.box_size(mra.max_depth);
-    op.r_min = std::sqrt(k_prec * h_min);
-    std::vector<GaussTerm> g = fit_kernel(mu, k_prec, op.r_min, op.r_max, &op.fit_err);
-    build_operator_terms(op, mra, g, max_depth, std::min(1e-10, 1e-4 * build_prec));
+    prepare_operator(op, mra, mu, build_prec, max_depth);
+    std::vector<GaussTerm> g(op.terms.size());
+    for (size_t m = 0; m < g.size(); m++) g[m] = {op.terms[m].alpha, op.terms[m].beta};
+    build_operator_terms(op, mra, g, op.max_depth, operator_tau(build_prec));
 }
 
 }  // namespace mwg

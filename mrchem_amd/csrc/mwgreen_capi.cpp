@@ -47,6 +47,7 @@ struct mwg_operator_s {
     Operator op;
     OperFlat flat;
     std::shared_ptr<MRA> mra_keepalive;
+    mwg_context dev_ctx = nullptr;  // device-built: the context holding the tables
 };
 struct mwg_tree_s {
     Tree tree;
@@ -490,6 +491,44 @@ int mwg_gaussian_operator_create(mwg_mra mra, int n_terms, const double *alpha, 
     });
 }
 
+int mwg_operators_create_device(mwg_context ctx, mwg_mra mra, int n, const double *mu, double build_prec,
+                                mwg_operator *ops) {
+    return guarded([&] {
+        if (!ctx || !mra || !mu || !ops) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (n < 1) return fail(MWG_ERR_ARG, "need at least one operator");
+        if (!(build_prec > 0.0 && build_prec < 1.0)) return fail(MWG_ERR_ARG, "build precision must be in (0, 1)");
+        for (int i = 0; i < n; i++)
+            if (!(mu[i] >= 0.0) || !std::isfinite(mu[i])) return fail(MWG_ERR_ARG, "mu must be finite and >= 0");
+        std::vector<std::unique_ptr<mwg_operator_s>> made(n);
+        for (int i = 0; i < n; i++) made[i].reset(new mwg_operator_s);
+        // kernel fits, one per operator, in parallel
+        std::vector<std::string> fit_err(n);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < n; i++) {
+            try {
+                prepare_operator(made[i]->op, mra->mra, mu[i], build_prec, mra->mra.max_depth);
+            } catch (const std::exception &e) {
+                fit_err[i] = e.what();
+            }
+        }
+        for (int i = 0; i < n; i++)
+            if (!fit_err[i].empty()) return fail(MWG_ERR_ARG, fit_err[i]);
+        std::vector<OpBuildJob> jobs;
+        for (int i = 0; i < n; i++) {
+            made[i]->dev_ctx = ctx;
+            jobs.push_back(OpBuildJob{&made[i]->op, &made[i]->flat, made[i].get()});
+        }
+        std::string err;
+        int rc = dev_build_operators(ctx->dev, mra->mra, jobs, err);
+        if (rc) {
+            for (auto &o : made) dev_forget_operator(ctx->dev, o.get());
+            return dev_fail(rc, err);
+        }
+        for (int i = 0; i < n; i++) ops[i] = made[i].release();
+        return MWG_OK;
+    });
+}
+
 int mwg_operator_destroy(mwg_operator op) {
     if (!op) return MWG_OK;
     {
@@ -523,7 +562,7 @@ int mwg_operator_terms(mwg_operator op, double *alpha, double *beta) {
 
 int mwg_operator_table_sizes(mwg_operator op, long long *nb, long long *nn) {
     if (!op) return fail(MWG_ERR_ARG, "null operator");
-    if (nb) *nb = (long long)op->flat.blocks.size();
+    if (nb) *nb = op->flat.device_blocks > 0 ? op->flat.device_blocks : (long long)op->flat.blocks.size();
     if (nn) *nn = (long long)op->flat.norms.size();
     return MWG_OK;
 }
@@ -534,7 +573,18 @@ int mwg_operator_tables(mwg_operator op, int *L, long long *boff, long long *nof
     if (L) std::memcpy(L, f.L.data(), sizeof(int) * f.L.size());
     if (boff) std::memcpy(boff, f.boff.data(), sizeof(long long) * f.boff.size());
     if (noff) std::memcpy(noff, f.noff.data(), sizeof(long long) * f.noff.size());
-    if (blocks) std::memcpy(blocks, f.blocks.data(), sizeof(double) * f.blocks.size());
+    if (blocks) {
+        if (f.device_blocks > 0) {
+            std::lock_guard<std::mutex> g(g_ctx_mtx);
+            if (!g_contexts.count(op->dev_ctx))
+                return fail(MWG_ERR_ARG, "the context holding this operator's tables was destroyed");
+            std::string err;
+            const int rc = dev_download_operator_blocks(op->dev_ctx->dev, op, blocks, f.device_blocks, err);
+            if (rc) return dev_fail(rc, err);
+        } else {
+            std::memcpy(blocks, f.blocks.data(), sizeof(double) * f.blocks.size());
+        }
+    }
     if (norms) std::memcpy(norms, f.norms.data(), sizeof(double) * f.norms.size());
     return MWG_OK;
 }
