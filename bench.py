@@ -107,6 +107,16 @@ def main():
     ctx = mw.Context.default()
     if args.tau is not None:
         ctx.set_precision("fp64", args.tau)
+    # the same operators built on the device (one build for all distinct mu);
+    # the applies below use the host-built ones
+    umus = sorted(set(mus))
+    mw.build_operators(mra, umus[:1], cfg["prec"], ctx=ctx)
+    t0 = time.time()
+    dev_ops = mw.build_operators(mra, umus, cfg["prec"], ctx=ctx)
+    setup["operator_build_device_s"] = time.time() - t0
+    del dev_ops
+    log("operators built on the device: %d in %.3fs (host: %.2fs)" % (len(umus), setup["operator_build_device_s"],
+                                                                      setup["operator_build_s"]))
     outs = [mw.FunctionTree(mra) for _ in inps]
     prec = cfg["prec"]
     entries = [dict(prec=prec if cfg["abs_prec"] else prec / f.norm(), out=o, oper=op, inp=f,
