@@ -274,6 +274,14 @@ def cpu_baseline(mra, ops, inps, outs, thr, ns_gpu, target_s, check):
         err = float(np.max(np.abs(ref - ns_gpu[idx])) / max(np.max(np.abs(ref)), 1e-300))
         sample = "%d of %d frozen-grid output nodes (random, seed 12345)" % (m, n)
         done = m
+        # one thread on a smaller sample of the same nodes
+        m1 = max(8, m // 32)
+        idx1 = np.sort(order[:m1])
+        oracle.set_threads(1)
+        t0 = time.perf_counter()
+        oracle.apply_fixed(mra, tables, inp, od[idx1], ol[idx1], thr[0][idx1], post=False)
+        one_thread = m1 / (time.perf_counter() - t0)
+        oracle.set_threads(threads)
     else:
         order = rng.permutation(len(outs))
         done, dt, err, used = 0, 0.0, 0.0, 0
@@ -297,8 +305,23 @@ def cpu_baseline(mra, ops, inps, outs, thr, ns_gpu, target_s, check):
     check["sample_max_rel_err_vs_oracle"] = err
     log("cpu baseline: %d of %d nodes in %.2fs on %d threads; sample |gpu-oracle|/max = %.2e" %
         (m, n_tot, dt, threads, err))
-    return {"value": m / dt, "unit": "nodes/s", "cores": threads, "kind": "port",
-            "sample": sample + ", FP64 OpenMP oracle, no post transforms"}
+    res = {"value": m / dt, "unit": "nodes/s", "cores": threads, "kind": "port",
+           "sample": sample + ", FP64 OpenMP oracle, no post transforms", "cpu_model": cpu_model()}
+    if len(outs) == 1:
+        res["one_thread_value"] = one_thread
+        log("cpu baseline, 1 thread: %.1f nodes/s" % one_thread)
+    return res
+
+
+def cpu_model():
+    """Model name of the host CPU (from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

@@ -852,13 +852,15 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             float sig1 = 0.f, sig2 = 0.f;  // scales of items q - 1, q - 2
             unsigned msk1 = 0u;            // hi mask of item q - 1
             load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), o0[0], e0[0]);
-            auto phase = [&](int q, auto par) {
+            // full: a steady phase (all three passes), known at compile time
+            auto phase = [&](int q, auto par, auto full) {
                 constexpr int P = decltype(par)::value;
+                constexpr bool FULL = decltype(full)::value;
                 auto &oc = pick<P>(o0, o1);
                 auto &ec = pick<P>(e0, e1);
                 auto &on = pick<1 - P>(o0, o1);
                 auto &en = pick<1 - P>(e0, e1);
-                const bool hx = q < n, hy = q >= 1 && q <= n, hz = q >= 2;
+                const bool hx = FULL || q < n, hy = FULL || (q >= 1 && q <= n), hz = FULL || q >= 2;
                 // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1)
                 if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), on[0], en[0]);
                 if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), on[1], en[1]);
@@ -882,9 +884,17 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                 sig1 = sig0;
                 msk1 = msk0;
             };
+            // N <= 8 (registers to spare): steady phase pairs get a copy of
+            // their own without the per-pass conditions (c3, k = 7: -13 %;
+            // at N >= 9 the copy costs a wave per SIMD and is slower)
             for (int q = 0; q < n + 2; q += 2) {
-                phase(q, std::integral_constant<int, 0>{});
-                if (q + 1 < n + 2) phase(q + 1, std::integral_constant<int, 1>{});
+                if (N <= 8 && q >= 2 && q + 1 < n) {
+                    phase(q, std::integral_constant<int, 0>{}, std::integral_constant<bool, N <= 8>{});
+                    phase(q + 1, std::integral_constant<int, 1>{}, std::integral_constant<bool, N <= 8>{});
+                } else {
+                    phase(q, std::integral_constant<int, 0>{}, std::false_type{});
+                    if (q + 1 < n + 2) phase(q + 1, std::integral_constant<int, 1>{}, std::false_type{});
+                }
             }
             __syncthreads();
         }
