@@ -498,7 +498,9 @@ struct SplitShape {
     static constexpr int NG = N * G;            // groups of an (outer, inner) plane
     static constexpr int NTG = (NG + 3) / 4;    // 16-row tiles of groups (x and y passes)
     static constexpr int NTZ = (N2 + 15) / 16;  // 16-column tiles (z pass)
-    static constexpr int NW = 4;                // waves per job
+    // waves per job: as many as keep every wave's tiles busy (N <= 4: one
+    // tile per pass; N = 5, 6 and 11, 12: tile counts of 3 and 9)
+    static constexpr int NW = N <= 4 ? 1 : (N <= 6 || N > 10 ? 3 : 4);
     static constexpr int T = 64 * NW;
     static constexpr int TWG = (NTG + NW - 1) / NW;
     static constexpr int TWZ = (NTZ + NW - 1) / NW;
