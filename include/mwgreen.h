@@ -117,6 +117,18 @@ int mwg_tree_square_norm(mwg_tree tree, double *sq_norm);
 int mwg_tree_dot(mwg_tree a, mwg_tree b, double *dot);
 int mwg_tree_eval(mwg_tree tree, int n_points, const double *r, double *values);
 int mwg_tree_rescale(mwg_tree tree, double factor);
+/* out = coef * a * b, built adaptively to prec (absolute with abs_prec,
+ * relative to the product's norm otherwise): mrcpp::multiply(prec, out, coef,
+ * a, b, max_iter, abs_prec), which MRChem calls through
+ * mrcpp::cplxfunc::multiply for the pair densities rho_ij = phi_i phi_j
+ * (src/qmoperators/two_electron/ExchangePotential.cpp:175) and the products
+ * phi_k V_ij (ExchangePotential.cpp:213,223).  The output grid starts as the
+ * union of the inputs' grids; each leaf's product is formed on the device
+ * from the inputs' values at its children's points, and leaves whose wavelet
+ * norm exceeds the split threshold are refined (max_iter < 0: no limit).
+ * Interpolating basis only; out must differ from a and b. */
+int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
+                 int abs_prec);
 /* Screening threshold of every node used by the last apply into this tree. */
 int mwg_tree_thresholds(mwg_tree tree, double *thr);
 

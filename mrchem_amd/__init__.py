@@ -57,6 +57,7 @@ _sig("mwg_helmholtz_create", _p, _d, _d, ctypes.POINTER(_p))
 _sig("mwg_gaussian_operator_create", _p, _i, _dp, _dp, _i, ctypes.POINTER(_p))
 _sig("mwg_operator_destroy", _p)
 _sig("mwg_operators_create_device", _p, _p, _i, _dp, _d, ctypes.POINTER(_p))
+_sig("mwg_multiply", _p, _d, _p, _d, _p, _p, _i, _i)
 _sig("mwg_operator_info", _p, _ip, _ip, _ip, _dp, _dp, _dp)
 _sig("mwg_operator_terms", _p, _dp, _dp)
 _sig("mwg_operator_table_sizes", _p, _llp, _llp)
@@ -362,6 +363,14 @@ def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False, prec_trees=None, ct
                                         int(abs_prec)))
     else:
         _check(lib.mwg_apply(ctx._h, prec, out._h, oper._h, inp._h, max_iter, int(abs_prec)))
+    return out
+
+
+def multiply(prec, out, a, b, coef=1.0, max_iter=-1, abs_prec=False, ctx=None):
+    """out = coef * a * b to prec on the device (mrcpp::multiply; interpolating
+    basis)."""
+    ctx = ctx or Context.default()
+    _check(lib.mwg_multiply(ctx._h, prec, out._h, coef, a._h, b._h, max_iter, 1 if abs_prec else 0))
     return out
 
 

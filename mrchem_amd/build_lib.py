@@ -16,7 +16,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HOST_SOURCES = ["mw_math.cpp", "mw_kernel.cpp", "mw_operator.cpp", "mw_tree.cpp", "mw_plan.cpp",
                 "mwgreen_capi.cpp"]
-HIP_SOURCES = ["mw_device.hip", "mw_opbuild.hip"]
+HIP_SOURCES = ["mw_device.hip", "mw_opbuild.hip", "mw_multiply.hip"]
 # gfx950 only; no packed FP32 instructions.  MFMA results go to VGPRs directly
 # (the AGPR form costs four v_accvgpr_read per tile: -9 % on the FP64 apply).
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -58,6 +58,26 @@ int dev_build_operators(DeviceCtx *c, const MRA &mra, const std::vector<OpBuildJ
 // Copy the device table of a device-built operator (f.device_blocks values).
 int dev_download_operator_blocks(DeviceCtx *c, const void *key, double *out, long long n, std::string &err);
 
+// Node-wise products for the multiplication (mw_multiply.hip).  For one
+// output node: each input's source node (its deepest node on the path, block
+// offset in the input's coefficient pool), the levels from it down to the
+// output node and the child index of each level (3 bits per level, the first
+// level below the source in the lowest bits), and h^{-3/2} of the output
+// node's children.
+struct MulNode {
+    long long src_a, src_b;
+    int dep_a, dep_b;
+    unsigned path_a, path_b;
+    double hfac;
+};
+struct DevMulSession;
+int dev_mul_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool_a, const std::vector<double> &pool_b,
+                  DevMulSession **out, std::string &err);
+// out_blocks: nodes.size() x 8(k+1)^3 on the host.
+int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,
+                  std::string &err);
+void dev_mul_end(DevMulSession *m);
+
 // Helpers shared by the HIP sources.
 struct DevOperTables {
     int *L = nullptr;

@@ -1,0 +1,291 @@
+// Pointwise product of two multiwavelet functions, node by node (gfx950).
+//
+// mrcpp::multiply(prec, out, c, a, b, maxIter, absPrec), reached from MRChem
+// through mrcpp::cplxfunc::multiply for the pair densities rho_ij = phi_i phi_j
+// (src/qmoperators/two_electron/ExchangePotential.cpp:175) and the exchange
+// products phi_k V_ij (ExchangePotential.cpp:213,223).  In the interpolating
+// basis the scaling coefficients of a node are the function's values at the
+// node's Gauss-Legendre points, scaled: f(x_ijk) = s_ijk h^{-3/2} /
+// sqrt(w_i w_j w_k) for a box of side h.  For one output node at depth n the
+// device forms both inputs' scaling blocks at its 8 children (reconstructing
+// from each input's deepest node on the path, wavelet parts zero below it),
+// multiplies them point by point, c s^a s^b h_{n+1}^{-3/2} / sqrt(w_i w_j w_k),
+// and compresses the 8 children into the node's [s, d1..d7] block.  The host
+// keeps the grid and the refinement loop (WaveletAdaptor's split rule).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "mw_device.hpp"
+
+namespace mwg {
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                       \
+            return -2;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+namespace {
+
+constexpr int kMulThreads = 256;
+constexpr int kMulBlocks = 1024;  // resident workgroups, each with its own scratch
+
+// Index of the element at position i along dim of the line (a, c) in an
+// n^3 block (x fastest): the two other indices are a (the lower direction)
+// and c (the higher one), as in the host transforms (mw_tree.cpp).
+__device__ __forceinline__ int line_index(int n, int dim, int a, int c, int i) {
+    return dim == 0 ? a * n + c * n * n + i : (dim == 1 ? a + c * n * n + i * n : a + c * n + i * n * n);
+}
+
+// One direction of the two-scale transform on all 4 component pairs
+// (t, t | bit) of an 8-component block, src -> dst: compress (M = F) or
+// reconstruct (M = F^T): [u_lo ; u_hi] = M [v_lo ; v_hi] along dim.
+__device__ void pair_pass(int n, const double *F, const double *src, double *dst, int dim, bool transpose, int tid) {
+    const int n2 = 2 * n, n3 = n * n * n, bit = 1 << dim;
+    for (int e = tid; e < 8 * n3; e += kMulThreads) {
+        const int t = e / n3, rem = e % n3;
+        // rem = line_index(dim, a, c, i): recover (a, c, i)
+        int i, a, c;
+        if (dim == 0) {
+            i = rem % n;
+            a = (rem / n) % n;
+            c = rem / (n * n);
+        } else if (dim == 1) {
+            a = rem % n;
+            i = (rem / n) % n;
+            c = rem / (n * n);
+        } else {
+            a = rem % n;
+            c = (rem / n) % n;
+            i = rem / (n * n);
+        }
+        const int t0 = t & ~bit, t1 = t0 | bit;
+        const int r = (t & bit) ? n + i : i;
+        const double *lo = src + (size_t)t0 * n3, *hi = src + (size_t)t1 * n3;
+        double s = 0.0;
+        for (int j = 0; j < n; j++) {
+            const int q = line_index(n, dim, a, c, j);
+            const double mlo = transpose ? F[j * n2 + r] : F[r * n2 + j];
+            const double mhi = transpose ? F[(n + j) * n2 + r] : F[r * n2 + n + j];
+            s += mlo * lo[q] + mhi * hi[q];
+        }
+        dst[e] = s;
+    }
+}
+
+// One direction of the inverse step towards child half `bit` for nb block
+// pairs: dst[k] = sum_j H_bit[j][i] lo_k[j] + G_bit[j][i] hi_k[j] along dim,
+// with lo_k = src + lo_off[k] blocks, hi_k likewise (hi_off < 0: none).
+__device__ void child_pass(int n, const double *F, const double *src, const int *lo_blk, const int *hi_blk, int nb,
+                           double *dst, int dim, int bit, int tid) {
+    const int n2 = 2 * n, n3 = n * n * n;
+    for (int e = tid; e < nb * n3; e += kMulThreads) {
+        const int k = e / n3, rem = e % n3;
+        int i, a, c;
+        if (dim == 0) {
+            i = rem % n;
+            a = (rem / n) % n;
+            c = rem / (n * n);
+        } else if (dim == 1) {
+            a = rem % n;
+            i = (rem / n) % n;
+            c = rem / (n * n);
+        } else {
+            a = rem % n;
+            c = (rem / n) % n;
+            i = rem / (n * n);
+        }
+        const double *lo = src + (size_t)lo_blk[k] * n3;
+        const double *hi = hi_blk[k] >= 0 ? src + (size_t)hi_blk[k] * n3 : nullptr;
+        double s = 0.0;
+        for (int j = 0; j < n; j++) {
+            const int q = line_index(n, dim, a, c, j);
+            s += F[j * n2 + bit * n + i] * lo[q];
+            if (hi) s += F[(n + j) * n2 + bit * n + i] * hi[q];
+        }
+        dst[(size_t)k * n3 + rem] = s;
+    }
+}
+
+// Scaling blocks of the 8 children of the work node, from the source node
+// block `src` (8 components) `dep` levels above it along `path`, into X
+// (Y: scratch of 8 blocks).
+__device__ void children_of(int n, const double *F, const double *src, int dep, unsigned path, double *X, double *Y,
+                            int tid) {
+    const int n3 = n * n * n;
+    if (dep == 0) {
+        for (int e = tid; e < 8 * n3; e += kMulThreads) Y[e] = src[e];
+        __syncthreads();
+    } else {
+        // first level: the full block (scaling and wavelet parts) towards child t
+        const int t = path & 7;
+        {
+            const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
+            child_pass(n, F, src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
+        }
+        __syncthreads();
+        {
+            const int lo[2] = {0, 1}, hi[2] = {2, 3};
+            child_pass(n, F, X, lo, hi, 2, Y, 1, (t >> 1) & 1, tid);
+        }
+        __syncthreads();
+        {
+            const int lo[1] = {0}, hi[1] = {1};
+            child_pass(n, F, Y, lo, hi, 1, X, 0, t & 1, tid);
+        }
+        __syncthreads();
+        // further levels: scaling part only
+        for (int lv = 1; lv < dep; lv++) {
+            const int tl = (path >> (3 * lv)) & 7;
+            const int lo[1] = {0}, none[1] = {-1};
+            child_pass(n, F, X, lo, none, 1, Y, 2, (tl >> 2) & 1, tid);
+            __syncthreads();
+            child_pass(n, F, Y, lo, none, 1, Y + n3, 1, (tl >> 1) & 1, tid);
+            __syncthreads();
+            child_pass(n, F, Y + n3, lo, none, 1, X, 0, tl & 1, tid);
+            __syncthreads();
+        }
+        // the work node's block: scaling part X[0], wavelet parts zero
+        for (int e = tid; e < 8 * n3; e += kMulThreads) Y[e] = e < n3 ? X[e] : 0.0;
+        __syncthreads();
+    }
+    // reconstruct Y -> 8 children (directions z, y, x), ending in X
+    pair_pass(n, F, Y, X, 2, true, tid);
+    __syncthreads();
+    pair_pass(n, F, X, Y, 1, true, tid);
+    __syncthreads();
+    pair_pass(n, F, Y, X, 0, true, tid);
+    __syncthreads();
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kMulThreads) void k_mul_nodes(int n, const MulNode *nodes, int n_nodes,
+                                                           const double *pool_a, const double *pool_b,
+                                                           const double *F, const double *wsc, double coef,
+                                                           double *scratch, double *out) {
+    const int tid = threadIdx.x;
+    const int n3 = n * n * n;
+    double *X = scratch + (size_t)blockIdx.x * 24 * n3, *Y = X + 8 * n3, *P = Y + 8 * n3;
+    for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
+        const MulNode nd = nodes[w];
+        children_of(n, F, pool_a + nd.src_a, nd.dep_a, nd.path_a, X, Y, tid);
+        for (int e = tid; e < 8 * n3; e += kMulThreads) P[e] = X[e];
+        __syncthreads();
+        children_of(n, F, pool_b + nd.src_b, nd.dep_b, nd.path_b, X, Y, tid);
+        // values product at the children's points, back to coefficients
+        for (int e = tid; e < 8 * n3; e += kMulThreads) P[e] = coef * P[e] * X[e] * nd.hfac * wsc[e % n3];
+        __syncthreads();
+        // compress the 8 children (directions x, y, z) into [s, d1..d7]
+        pair_pass(n, F, P, X, 0, false, tid);
+        __syncthreads();
+        pair_pass(n, F, X, Y, 1, false, tid);
+        __syncthreads();
+        double *dst = out + (size_t)w * 8 * n3;
+        pair_pass(n, F, Y, dst, 2, false, tid);
+        __syncthreads();
+    }
+}
+
+struct DevMulSession {
+    DeviceCtx *ctx = nullptr;
+    int n = 0;
+    double *pool_a = nullptr, *pool_b = nullptr, *F = nullptr, *wsc = nullptr, *scratch = nullptr;
+    ~DevMulSession() {
+        for (double *p : {pool_a, pool_b, F, wsc, scratch})
+            if (p) (void)hipFree(p);
+    }
+};
+
+int dev_mul_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool_a, const std::vector<double> &pool_b,
+                  DevMulSession **out, std::string &err) {
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    if (mra.basis.type != BASIS_INTERPOLATING) {
+        err = "multiplication needs the interpolating basis";
+        return -1;
+    }
+    auto *m = new DevMulSession;
+    m->ctx = c;
+    const int n = mra.kp1(), n3 = n * n * n;
+    m->n = n;
+    std::vector<double> x(n), w(n), wsc(n3);
+    const double *qx = nullptr, *qw = nullptr;
+    quad_rule01(n, &qx, &qw);
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) wsc[(k * n + j) * n + i] = 1.0 / std::sqrt(qw[i] * qw[j] * qw[k]);
+    auto up = [&](const std::vector<double> &v, double **d) -> int {
+        HIP_TRY(hipMalloc((void **)d, sizeof(double) * std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) HIP_TRY(hipMemcpyAsync(*d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
+        return 0;
+    };
+    int rc = up(pool_a, &m->pool_a);
+    if (!rc) rc = up(pool_b, &m->pool_b);
+    if (!rc) rc = up(mra.basis.F, &m->F);
+    if (!rc) rc = up(wsc, &m->wsc);
+    if (!rc) {
+        hipError_t e = hipMalloc((void **)&m->scratch, sizeof(double) * (size_t)kMulBlocks * 24 * n3);
+        if (e != hipSuccess) {
+            err = std::string("hipMalloc(scratch): ") + hipGetErrorString(e);
+            rc = -2;
+        }
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,
+                  std::string &err) {
+    if (nodes.empty()) return 0;
+    HIP_TRY(hipSetDevice(dev_device(m->ctx)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(m->ctx));
+    const int n3 = m->n * m->n * m->n;
+    MulNode *dn = nullptr;
+    double *dout = nullptr;
+    HIP_TRY(hipMalloc((void **)&dn, sizeof(MulNode) * nodes.size()));
+    hipError_t e = hipMalloc((void **)&dout, sizeof(double) * nodes.size() * 8 * n3);
+    if (e != hipSuccess) {
+        (void)hipFree(dn);
+        err = std::string("hipMalloc(out): ") + hipGetErrorString(e);
+        return -2;
+    }
+    e = hipMemcpyAsync(dn, nodes.data(), sizeof(MulNode) * nodes.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        const int grid = (int)std::min<size_t>(nodes.size(), kMulBlocks);
+        hipLaunchKernelGGL(k_mul_nodes, dim3(grid), dim3(kMulThreads), 0, s, m->n, dn, (int)nodes.size(), m->pool_a,
+                           m->pool_b, m->F, m->wsc, coef, m->scratch, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out_blocks, dout, sizeof(double) * nodes.size() * 8 * n3, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dn);
+    (void)hipFree(dout);
+    if (e != hipSuccess) {
+        err = std::string("multiply launch: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+void dev_mul_end(DevMulSession *m) {
+    if (!m) return;
+    (void)hipSetDevice(dev_device(m->ctx));
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(dev_stream(m->ctx)));
+    delete m;
+}
+
+}  // namespace mwg

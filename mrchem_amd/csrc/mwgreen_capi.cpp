@@ -8,6 +8,7 @@
 // level on the host (TreeBuilder + WaveletAdaptor), each level's nodes are
 // computed by one HIP launch, and the result is finished by the TopDown(add)
 // and BottomUp transforms.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -525,6 +526,115 @@ int mwg_operators_create_device(mwg_context ctx, mwg_mra mra, int n, const doubl
             return dev_fail(rc, err);
         }
         for (int i = 0; i < n; i++) ops[i] = made[i].release();
+        return MWG_OK;
+    });
+}
+
+namespace {
+// Source of output node `k` in input tree f: its deepest node on the path
+// from the roots, the levels below it and the child index of each level.
+bool mul_source(const Tree &f, const NodeKey &k, long long &off, int &dep, unsigned &path) {
+    for (int d = k.depth; d >= 0; d--) {
+        long long l[3];
+        for (int q = 0; q < 3; q++) l[q] = k.l[q] >> (k.depth - d);
+        const int idx = f.find(d, l);
+        if (idx < 0) continue;
+        dep = k.depth - d;
+        if (dep > 10) return false;
+        path = 0;
+        for (int lv = 1; lv <= dep; lv++) {
+            unsigned t = 0;
+            for (int q = 0; q < 3; q++) t |= (unsigned)((k.l[q] >> (k.depth - d - lv)) & 1) << q;
+            path |= t << (3 * (lv - 1));
+        }
+        off = (long long)idx * f.mra->block();
+        return true;
+    }
+    return false;
+}
+}  // namespace
+
+int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
+                 int abs_prec) {
+    return guarded([&] {
+        if (!ctx || !out || !a || !b) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (out == a || out == b) return fail(MWG_ERR_ARG, "the output tree must differ from the inputs");
+        const MRA &mra = *out->tree.mra;
+        if (a->tree.mra != &mra || b->tree.mra != &mra) return fail(MWG_ERR_ARG, "trees of different MRAs");
+        if (mra.basis.type != BASIS_INTERPOLATING) return fail(MWG_ERR_ARG, "multiplication needs the interpolating basis");
+        if (!(prec >= 0.0) || !std::isfinite(coef)) return fail(MWG_ERR_ARG, "bad precision or coefficient");
+        Tree &T = out->tree;
+        T.clear();
+        T.make_roots();
+        // the union of the input grids
+        for (const Tree *f : {&a->tree, &b->tree}) {
+            std::vector<int> order(f->size());
+            for (int i = 0; i < f->size(); i++) order[i] = i;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return f->keys[x].depth < f->keys[y].depth; });
+            for (int i : order) {
+                const NodeKey &k = f->keys[i];
+                if (k.depth == 0 || T.find(k.depth, k.l) >= 0) continue;
+                long long lp[3] = {k.l[0] >> 1, k.l[1] >> 1, k.l[2] >> 1};
+                const int par = T.find(k.depth - 1, lp);
+                if (par < 0) return fail(MWG_ERR_ARG, "input tree is not complete from its roots");
+                T.split(par);
+            }
+        }
+        std::vector<int> work;
+        for (int i = 0; i < T.size(); i++)
+            if (T.child0[i] < 0) work.push_back(i);
+        std::string err;
+        DevMulSession *m = nullptr;
+        int rc = dev_mul_begin(ctx->dev, mra, a->tree.coefs, b->tree.coefs, &m, err);
+        if (rc) return dev_fail(rc, err);
+        const int blk = mra.block(), n3 = mra.kp1_3();
+        for (int iter = 0;; iter++) {
+            std::vector<MulNode> nodes(work.size());
+            for (size_t w = 0; w < work.size(); w++) {
+                const NodeKey &k = T.keys[work[w]];
+                MulNode &nd = nodes[w];
+                if (!mul_source(a->tree, k, nd.src_a, nd.dep_a, nd.path_a) ||
+                    !mul_source(b->tree, k, nd.src_b, nd.dep_b, nd.path_b)) {
+                    dev_mul_end(m);
+                    return fail(MWG_ERR_ARG, "output node more than 10 levels below an input's finest node");
+                }
+                nd.hfac = std::pow(mra.box_size(k.depth + 1), -1.5);
+            }
+            std::vector<double> blocks(work.size() * (size_t)blk);
+            rc = dev_mul_nodes(m, nodes, coef, blocks.data(), err);
+            if (rc) {
+                dev_mul_end(m);
+                return dev_fail(rc, err);
+            }
+            for (size_t w = 0; w < work.size(); w++)
+                std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
+            if (max_iter >= 0 && iter >= max_iter) break;
+            // norm estimate of the product from the leaves' scaling parts
+            double tnorm = 1.0;
+            if (!abs_prec) {
+                double sq = 0.0;
+                for (int i = 0; i < T.size(); i++)
+                    if (T.child0[i] < 0)
+                        for (int e = 0; e < n3; e++) sq += T.node(i)[e] * T.node(i)[e];
+                tnorm = std::sqrt(sq);
+            }
+            std::vector<int> next;
+            for (int wi : work) {
+                const int d = T.keys[wi].depth;
+                double wsq = 0.0;
+                for (int e = n3; e < blk; e++) wsq += T.node(wi)[e] * T.node(wi)[e];
+                if (d < mra.max_depth && split_check(mra, d, std::sqrt(wsq), prec, tnorm)) {
+                    const int c0 = T.split(wi);
+                    for (int t = 0; t < 8; t++) next.push_back(c0 + t);
+                }
+            }
+            if (next.empty()) break;
+            work.swap(next);
+        }
+        dev_mul_end(m);
+        tree_compress(T);
+        T.sqnorm = tree_square_norm(T);
         return MWG_OK;
     });
 }
