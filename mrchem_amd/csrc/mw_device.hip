@@ -228,15 +228,22 @@ __device__ __forceinline__ void stage_terms(const KArgs &a, const AppDesc &ad, c
 // bound ((N_x N_y) N_z) ||f_ft|| is formed in FP64 in the oracle's operation
 // order and a ballot appends the terms that pass the reference rule
 // (bound > thr) and fall in the launch's class (bound > thr_lo and not
-// bound > thr_hi): emit(pos, p, ft, lx, ly, lz, bound) is called by each
-// taking lane with its worklist position.  Stops before a pair that could overflow the
-// worklist (cap items) and leaves pi there.  Returns the item count.
-template <typename Emit>
+// bound > thr_hi).  With PAIR the job stands for the output components gt
+// and gt | 4 (gt < 4), which share N_x and N_y: both bounds are formed and a
+// term is appended when either passes.  emit(pos, p, ft, lx, ly, lz, b0, b1,
+// t0, t1) is called by each appending lane with its worklist position, the
+// bounds for gt and gt | 4 and which of them pass (b1, t1 unused without
+// PAIR).  Stops before a pair that could overflow the worklist (cap items)
+// and leaves pi there.  Returns the item count; apps0, apps1 count the
+// passing term-applications of gt and gt | 4.
+template <bool PAIR, typename Emit>
 __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, const JobCtx &j, int &pi, int pi_end,
-                                           int cnt, const double *nrm_s, int cap, int lane, Emit &&emit) {
+                                           int cnt, const double *nrm_s, int cap, int lane, unsigned long long &apps0,
+                                           unsigned long long &apps1, Emit &&emit) {
     const bool have = lane < cnt;
     const int S = 2 * ad.mt_w + 1;
-    const double *mt = ad.mtab + (long long)(j.d - ad.mt_d0) * 64 * S * S * S;
+    const long long S3 = (long long)S * S * S;
+    const double *mt = ad.mtab + (long long)(j.d - ad.mt_d0) * 64 * S3;
     const double *nr = nrm_s + lane * nrm_stride(j.nl);
     int n = 0;
     while (pi < pi_end) {
@@ -245,13 +252,16 @@ __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, co
         if (pr < pi_end) {
             const int p = pr >> 3, ft = pr & 7;
             const int s = a.band[j.boff + p];
-            if (s >= 0 && !(j.d > 0 && ft == 0 && j.gt == 0)) {
+            // the NS form has no scaling-to-scaling part below the roots
+            const bool ex0 = j.d > 0 && ft == 0 && j.gt == 0;
+            if (s >= 0 && (PAIR || !ex0)) {
                 const double fn = a.slot_norm[(long long)s * 8 + ft];
                 const int lx = p % j.side - j.w, ly = (p / j.side) % j.side - j.w, lz = p / (j.side * j.side) - j.w;
-                const int cmb = (2 * j.gb[0] + (ft & 1)) | ((2 * j.gb[1] + ((ft >> 1) & 1)) << 2) |
-                                ((2 * j.gb[2] + ((ft >> 2) & 1)) << 4);
-                const double mb =
-                    mt[(((long long)cmb * S + (lz + ad.mt_w)) * S + (ly + ad.mt_w)) * S + (lx + ad.mt_w)] * fn;
+                const long long li = ((long long)(lz + ad.mt_w) * S + (ly + ad.mt_w)) * S + (lx + ad.mt_w);
+                const int cxy = (2 * j.gb[0] + (ft & 1)) | ((2 * j.gb[1] + ((ft >> 1) & 1)) << 2);
+                const int cz = 2 * j.gb[2] + ((ft >> 2) & 1);
+                double mb = ex0 ? 0.0 : mt[(cxy | (cz << 4)) * S3 + li] * fn;
+                if (PAIR) mb = fmax(mb, mt[(cxy | ((cz + 2) << 4)) * S3 + li] * fn);
                 cand = fn > 0.0 && mb > j.thr && mb > j.thr_lo;
             }
         }
@@ -269,11 +279,24 @@ __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, co
             const int ll[3] = {p % j.side - j.w, (p / j.side) % j.side - j.w, p / (j.side * j.side) - j.w};
             const double nx = nr[(2 * j.gb[0] + (ft & 1)) * j.nl + (ll[0] + j.w)];
             const double ny = nr[(2 * j.gb[1] + ((ft >> 1) & 1)) * j.nl + (ll[1] + j.w)];
-            const double nz = nr[(2 * j.gb[2] + ((ft >> 2) & 1)) * j.nl + (ll[2] + j.w)];
-            const double bound = ((nx * ny) * nz) * fn;
-            const bool take = have && bound > j.thr && bound > j.thr_lo && !(bound > j.thr_hi);
-            const unsigned long long mask = __ballot(take);
-            if (take) emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll[0], ll[1], ll[2], bound);
+            const int cz = 2 * j.gb[2] + ((ft >> 2) & 1);
+            const double b0 = ((nx * ny) * nr[cz * j.nl + (ll[2] + j.w)]) * fn;
+            const bool ex0 = PAIR && j.d > 0 && ft == 0 && j.gt == 0;
+            const bool t0 = have && !ex0 && b0 > j.thr && b0 > j.thr_lo && !(b0 > j.thr_hi);
+            double b1 = 0.0;
+            bool t1 = false;
+            if (PAIR) {
+                b1 = ((nx * ny) * nr[(cz + 2) * j.nl + (ll[2] + j.w)]) * fn;
+                t1 = have && b1 > j.thr && b1 > j.thr_lo && !(b1 > j.thr_hi);
+            }
+            const unsigned long long mask = __ballot(t0 || t1);
+            if (PAIR) {
+                apps0 += __popcll(__ballot(t0));
+                apps1 += __popcll(__ballot(t1));
+            } else {
+                apps0 += __popcll(mask);
+            }
+            if (t0 || t1) emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll[0], ll[1], ll[2], b0, b1, t0, t1);
             n += __popcll(mask);
         }
         pi += 64;
@@ -362,13 +385,13 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
         int pi = 8 * p_begin;
         while (pi < 8 * p_end) {
             // screening into the worklist (wave 0 writes the items)
-            const int n = screen_scan(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane,
-                                      [&](int pos, int p, int ft, int, int, int, double) {
-                                          if (wv == 0) wl[pos] = (p << 9) | (ft << 6) | lane;
-                                      });
+            unsigned long long n1 = 0;
+            const int n = screen_scan<false>(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane, napps, n1,
+                                             [&](int pos, int p, int ft, int, int, int, double, double, bool, bool) {
+                                                 if (wv == 0) wl[pos] = (p << 9) | (ft << 6) | lane;
+                                             });
             __syncthreads();
             if (n == 0) continue;
-            napps += n;
             // pipelined application of the worklist
             T fa[TW][KS], fnext[TW][KS];
             T pre[OPL];
@@ -590,22 +613,28 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const
     return (const __attribute__((address_space(1))) T *)p;
 }
 
-// Worklist item of the split kernel: the three operator block ids (block =
-// N x N values of one term, depth, component and translation) and the input
-// group (band position << 3 | component), bit kSingleBit set when the item's
-// intermediates are rounded once (no lo part).
+// Worklist item of the split kernel: the operator block ids (block = N x N
+// values of one term, depth, component and translation) of x, y and of z
+// for the output components gt and gt | 4, and the input group (band
+// position << 3 | component) with flag bits: kZ0Bit, kZ1Bit when the term
+// passes for gt, gt | 4; kSingleBit when the item's intermediates are
+// rounded once (no lo part).
 struct SplitItem {
-    int bx, by, bz, grp;
+    int bx, by, bz0, bz1, grp;
 };
-constexpr int kSingleBit = 30;
+constexpr int kZ0Bit = 28, kZ1Bit = 29, kSingleBit = 30;
 
-// Per-item passes with FP16 split operands.  The job, the work segment and
-// the screening rule are those of k_apply; the screening lane (one term)
-// also resolves the item's three operator block ids.  With unit-scaled
+// Per-item passes with FP16 split operands.  The screening rule is that of
+// k_apply; a job is an output node g and a pair of output components
+// (gt, gt | 4), gt < 4, whose operators differ only along z, so an item
+// (band node, ft, term) runs the x and y passes once and the z pass for
+// each component of the pair its term passes for.  The screening lane (one
+// term) also resolves the item's operator block ids.  With unit-scaled
 // operands F_u, O_u:
 //   x: X[(jz,jy)][ix] = F_u . Ox_u^T, stored transposed as Xt[(jz,ix)][jy]
 //   y: Y[(ix,jz)][iy] = Xt . Oy_u^T,  stored as Yt[(ix,iy)][jz]
-//   z: Z[iz][(ix,iy)] = Oz_u . Yt^T,  acc += sF sOx sOy sOz Z  (FP32)
+//   z: Z[iz][(ix,iy)] = Oz_u . Yt^T,  acc_h += sF sOx sOy sOz_h Z  (FP32)
+//      for h = 0, 1 (gt, gt | 4), one read of Yt for both
 // The rows of the x and y passes are taken in groups of four consecutive
 // inner indices (x: (jz, 4 jy), y: (ix, 4 jz), the y groups ordered with ix
 // fastest), so every lane stores its four results with aligned 8-byte
@@ -634,8 +663,8 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     constexpr int BUFH = (N2 + 2) * (RL + RE);  // halfs per intermediate buffer
     __shared__ __align__(16) _Float16 xt[2 * BUFH];  // X of items q, q - 1 (by parity)
     __shared__ __align__(16) _Float16 yt[2 * BUFH];  // Y of items q - 1, q - 2
-    __shared__ __align__(16) SplitItem wl[kSplitCap];
-    __shared__ float wsig[kSplitCap];  // product of the item's three operator scales
+    __shared__ SplitItem wl[kSplitCap];
+    __shared__ float2 wsig[kSplitCap];  // products of the item's operator scales, z of gt and of gt | 4
     __shared__ int ltab[64];
     __shared__ long long btab[64];
     extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x nrm_stride(2 Wmax + 1)
@@ -719,10 +748,10 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
     }
     const int ol = min(lc, N - 1) * RH + och, ole = min(lc, N - 1) * RH + 24;
 
-    mwg_float4 acc[TWZ];
+    mwg_float4 acc0[TWZ], acc1[TWZ];  // Z sums of gt and gt | 4
 #pragma unroll
-    for (int tt = 0; tt < TWZ; tt++) acc[tt] = mwg_float4{0.f, 0.f, 0.f, 0.f};
-    unsigned long long napps = 0;
+    for (int tt = 0; tt < TWZ; tt++) acc0[tt] = acc1[tt] = mwg_float4{0.f, 0.f, 0.f, 0.f};
+    unsigned long long napps0 = 0, napps1 = 0;
 
     // four results of one tile into a data-side split row: hi keeps the bits
     // of msk (0xffffe000: an 11-bit mantissa, so hi converts exactly and
@@ -798,7 +827,9 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) store4(Y, ys1[tt], ys2[tt], yse[tt], d[tt], msk);
     };
-    auto z_pass = [&](const _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, float sigma) {
+    // z of gt (zm bit 0, operator o0) and of gt | 4 (bit 1, operator o1)
+    auto z_pass = [&](const _Float16 *Y, const mwg_h8 &o0, const mwg_h4 &oe0, const mwg_h8 &o1, const mwg_h4 &oe1,
+                      float2 sigma, int zm) {
         mwg_h8 vb[TWZ];
         mwg_h4 vbe[TWZ];
 #pragma unroll
@@ -806,11 +837,21 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             vbe[tt] = zero4;
             lds_operand(Y, zl[tt], zle[tt], vb[tt], vbe[tt]);
         }
+        if (zm & 1) {
 #pragma unroll
-        for (int tt = 0; tt < TWZ; tt++) {
-            const mwg_float4 dz = split_mfma<N>(o, oe, vb[tt], vbe[tt]);
+            for (int tt = 0; tt < TWZ; tt++) {
+                const mwg_float4 dz = split_mfma<N>(o0, oe0, vb[tt], vbe[tt]);
 #pragma unroll
-            for (int r = 0; r < 4; r++) acc[tt][r] = fmaf(sigma, dz[r], acc[tt][r]);
+                for (int r = 0; r < 4; r++) acc0[tt][r] = fmaf(sigma.x, dz[r], acc0[tt][r]);
+            }
+        }
+        if (zm & 2) {
+#pragma unroll
+            for (int tt = 0; tt < TWZ; tt++) {
+                const mwg_float4 dz = split_mfma<N>(o1, oe1, vb[tt], vbe[tt]);
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc1[tt][r] = fmaf(sigma.y, dz[r], acc1[tt][r]);
+            }
         }
     };
 
@@ -826,33 +867,40 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
         while (pi < 8 * p_end) {
             // screening into SplitItems: the lane-term also resolves the
             // item's three operator block ids; wave 0 writes the items
-            const int n = screen_scan(
-                a, ad, j, pi, 8 * p_end, cnt, nrm_s, kSplitCap, lane,
-                [&](int pos, int p, int ft, int lx, int ly, int lz, double bound) {
+            const int n = screen_scan<true>(
+                a, ad, j, pi, 8 * p_end, cnt, nrm_s, kSplitCap, lane, napps0, napps1,
+                [&](int pos, int p, int ft, int lx, int ly, int lz, double b0, double b1, bool t0, bool t1) {
                     if (wv != 0) return;
+                    // a passing term has |l| <= L in every direction, so all
+                    // four block ids exist
+                    const int span = 2 * my_L + 1;
                     SplitItem itm;
-                    itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * (2 * my_L + 1)) + lx;
-                    itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * (2 * my_L + 1)) + ly;
-                    itm.bz = my_b0 + (((j.gb[2] << 1) | ((ft >> 2) & 1)) * (2 * my_L + 1)) + lz;
-                    itm.grp = (p << 3) | ft | ((bound <= thr_single ? 1 : 0) << kSingleBit);
+                    itm.bx = my_b0 + (((j.gb[0] << 1) | (ft & 1)) * span) + lx;
+                    itm.by = my_b0 + (((j.gb[1] << 1) | ((ft >> 1) & 1)) * span) + ly;
+                    itm.bz0 = my_b0 + (((ft >> 2) & 1) * span) + lz;
+                    itm.bz1 = itm.bz0 + 2 * span;
+                    const double bmax = fmax(t0 ? b0 : 0.0, t1 ? b1 : 0.0);
+                    itm.grp = (p << 3) | ft | ((bmax <= thr_single ? 1 : 0) << kSingleBit) | ((t0 ? 1 : 0) << kZ0Bit) |
+                              ((t1 ? 1 : 0) << kZ1Bit);
                     wl[pos] = itm;
-                    wsig[pos] = (float)(opsc[itm.bx] * opsc[itm.by] * opsc[itm.bz]);
+                    const double sxy = opsc[itm.bx] * opsc[itm.by];
+                    wsig[pos] = make_float2(t0 ? (float)(sxy * opsc[itm.bz0]) : 0.f, t1 ? (float)(sxy * opsc[itm.bz1]) : 0.f);
                 });
             __syncthreads();
             if (n == 0) continue;
-            napps += n;
             // software pipeline over the batch: phase q runs x(q), y(q - 1),
             // z(q - 2) with operator set q & 1 while the next phase's
             // operators load into the other set; phases are unrolled in pairs
             // so that the set and the LDS buffers are fixed in each copy
-            const int mask_grp = (1 << kSingleBit) - 1;
-            mwg_h8 o0[3], o1[3];  // operator sets: Ox, Oy, Oz
-            mwg_h4 e0[3], e1[3];
+            const int mask_grp = (1 << kZ0Bit) - 1;
+            mwg_h8 o0[4], o1[4];  // operator sets: Ox, Oy, Oz of gt, Oz of gt | 4
+            mwg_h4 e0[4], e1[4];
 #pragma unroll
-            for (int u = 0; u < 3; u++) e0[u] = e1[u] = zero4;
+            for (int u = 0; u < 4; u++) e0[u] = e1[u] = zero4;
             int cur_grp = -1;
-            float sig1 = 0.f, sig2 = 0.f;  // scales of items q - 1, q - 2
-            unsigned msk1 = 0u;            // hi mask of item q - 1
+            float2 sig1 = make_float2(0.f, 0.f), sig2 = make_float2(0.f, 0.f);  // scales of items q - 1, q - 2
+            int zm1 = 0, zm2 = 0;  // z masks of items q - 1, q - 2
+            unsigned msk1 = 0u;    // hi mask of item q - 1
             load_op(__builtin_amdgcn_readfirstlane(wl[0].bx), o0[0], e0[0]);
             // full: a steady phase (all three passes), known at compile time
             auto phase = [&](int q, auto par, auto full) {
@@ -863,12 +911,15 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                 auto &on = pick<1 - P>(o0, o1);
                 auto &en = pick<1 - P>(e0, e1);
                 const bool hx = FULL || q < n, hy = FULL || (q >= 1 && q <= n), hz = FULL || q >= 2;
-                // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1)
+                // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1) of the
+                // components item q - 1 passes for
                 if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), on[0], en[0]);
                 if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), on[1], en[1]);
-                if (hy) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz), on[2], en[2]);
-                float sig0 = 0.f;
+                if (hy && (zm1 & 1)) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz0), on[2], en[2]);
+                if (hy && (zm1 & 2)) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz1), on[3], en[3]);
+                float2 sig0 = make_float2(0.f, 0.f);
                 unsigned msk0 = 0u;
+                int zm0 = 0;
                 if (hx) {
                     const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
                     if ((g & mask_grp) != cur_grp) {
@@ -876,14 +927,18 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
                         load_f(cur_grp);
                     }
                     msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
-                    sig0 = sF * wsig[q];
+                    zm0 = (g >> kZ0Bit) & 3;
+                    const float2 w2 = wsig[q];
+                    sig0 = make_float2(sF * w2.x, sF * w2.y);
                     x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
                 }
                 if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
-                if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], sig2);
+                if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], oc[3], ec[3], sig2, zm2);
                 __syncthreads();
                 sig2 = sig1;
                 sig1 = sig0;
+                zm2 = zm1;
+                zm1 = zm0;
                 msk1 = msk0;
             };
             // N <= 8 (registers to spare): steady phase pairs get a copy of
@@ -901,31 +956,35 @@ __global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Fl
             __syncthreads();
         }
     }
+    const unsigned long long napps = napps0 + napps1;
     if (tid == 0 && napps) {
         atomicAdd(&a.counters[0], napps);
         if (cls_counter > 0) atomicAdd(&a.counters[cls_counter], napps);
     }
-    if (napps == 0) return;
     // acc[tt][r] = Z[iz = 4 lq + r][col = 16 t + lc] with col = ix N + iy:
     // put in (iz, iy, ix) order in LDS, then added in FP64 to the output
     // with consecutive lanes on consecutive elements
     static_assert(N3 * sizeof(float) <= sizeof(xt), "output staging does not fit");
     float *zs = reinterpret_cast<float *>(xt);
-    __syncthreads();
+    auto write_out = [&](const mwg_float4 (&acc)[TWZ], int gt) {
+        __syncthreads();
 #pragma unroll
-    for (int tt = 0; tt < TWZ; tt++) {
-        const int t = wv + NW * tt;
-        const int col = 16 * t + lc;
-        if (t >= NTZ || col >= N2) continue;
+        for (int tt = 0; tt < TWZ; tt++) {
+            const int t = wv + NW * tt;
+            const int col = 16 * t + lc;
+            if (t >= NTZ || col >= N2) continue;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int iz = 4 * lq + r;
-            if (iz < N) zs[iz * N2 + (col % N) * N + col / N] = acc[tt][r];
+            for (int r = 0; r < 4; r++) {
+                const int iz = 4 * lq + r;
+                if (iz < N) zs[iz * N2 + (col % N) * N + col / N] = acc[tt][r];
+            }
         }
-    }
-    __syncthreads();
-    double *dst = a.out_ns + (long long)j.g * 8 * N3 + (long long)j.gt * N3;
-    for (int e = tid; e < N3; e += NTH) atomicAdd(&dst[e], (double)zs[e]);
+        __syncthreads();
+        double *dst = a.out_ns + (long long)j.g * 8 * N3 + (long long)gt * N3;
+        for (int e = tid; e < N3; e += NTH) atomicAdd(&dst[e], (double)zs[e]);
+    };
+    if (napps0) write_out(acc0, j.gt);
+    if (napps1) write_out(acc1, j.gt | 4);
 }
 
 // FP16 split rows of FP64 blocks (rows x N values each, row-major), data or
@@ -994,7 +1053,8 @@ struct DeviceCtx {
 struct DevicePlan {
     DeviceCtx *ctx = nullptr;
     std::vector<const void *> op_keys;  // per part
-    int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0;
+    // segments: n_seg single-component ones, then n_pseg pair ones
+    int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0, n_pseg = 0;
     long long npool = 0;
     double *pool = nullptr;
     float *pool32 = nullptr;
@@ -1007,8 +1067,11 @@ struct DevicePlan {
     size_t out_bytes = 0;
     // per segment: class cut base thr_cls and the range of passing bounds
     std::vector<double> seg_thr, seg_bmin, seg_bmax;
-    // segment lists of the two classes of the mixed mode, for list_tau
+    std::vector<double> pseg_thr, pseg_bmin, pseg_bmax;  // the same for pair segments
+    // segment lists of the two classes, for list_tau and list_mode (the low
+    // class of the split mode lists pair segments)
     double list_tau = -1.0;
+    int list_mode = -1;
     int *list_hi = nullptr, *list_lo = nullptr;
     int n_hi = 0, n_lo = 0;
     ~DevicePlan() {
@@ -1220,7 +1283,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         return -1;
     }
     // sizes and validation
-    long long n_out = 0, n_band = 0, n_slot = 0, n_pool = 0, n_bw = 0, n_actoff = 0, n_act = 0, n_seg = 0;
+    long long n_out = 0, n_band = 0, n_slot = 0, n_pool = 0, n_bw = 0, n_actoff = 0, n_act = 0, n_seg = 0, n_pseg = 0;
     int wmax = 0;
     for (const PlanPart &pp : parts) {
         const ApplyPlan &pl = *pp.plan;
@@ -1248,8 +1311,9 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         n_actoff += (long long)pl.active_off.size();
         n_act += (long long)pl.active_terms.size();
         n_seg += (long long)pl.seg_job.size();
+        n_pseg += (long long)pl.pseg_job.size();
     }
-    if (n_out >= (1LL << 27) || n_seg >= (1LL << 31) || n_slot >= (1LL << 31) || n_band >= (1LL << 31)) {
+    if (n_out >= (1LL << 27) || n_seg + n_pseg >= (1LL << 31) || n_slot >= (1LL << 31) || n_band >= (1LL << 31)) {
         err = "batch too large for 32-bit node, segment or slot indices";
         return -1;
     }
@@ -1260,6 +1324,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     p->wmax = wmax;
     p->npool = n_pool;
     p->n_seg = (int)n_seg;
+    p->n_pseg = (int)n_pseg;
     KArgs &a = p->args;
     hipStream_t s = c->stream;
     int rc = 0;
@@ -1278,9 +1343,9 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     if (!rc) rc = plan_alloc(p, n_bw, &d_bw, err);
     if (!rc) rc = plan_alloc(p, n_actoff, &d_act_off, err);
     if (!rc) rc = plan_alloc(p, n_act, &d_act, err);
-    if (!rc) rc = plan_alloc(p, n_seg, &d_sj, err);
-    if (!rc) rc = plan_alloc(p, n_seg, &d_s0, err);
-    if (!rc) rc = plan_alloc(p, n_seg, &d_s1, err);
+    if (!rc) rc = plan_alloc(p, n_seg + n_pseg, &d_sj, err);
+    if (!rc) rc = plan_alloc(p, n_seg + n_pseg, &d_s0, err);
+    if (!rc) rc = plan_alloc(p, n_seg + n_pseg, &d_s1, err);
     if (!rc) rc = plan_alloc(p, parts.size(), &p->d_apps, err);
     p->out_bytes = sizeof(double) * (size_t)std::max<long long>(n_out, 1) * 8 * N * N * N;
     if (!rc) rc = plan_alloc(p, p->out_bytes / sizeof(double), &a.out_ns, err);
@@ -1291,6 +1356,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     std::vector<std::vector<long long>> keep_l;
     std::vector<std::vector<double>> keep_d;
     long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0, o_seg = 0;
+    long long o_pseg = n_seg;
     p->seg_thr.reserve(n_seg);
     p->seg_bmin.reserve(n_seg);
     p->seg_bmax.reserve(n_seg);
@@ -1361,6 +1427,18 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
             p->seg_bmin.push_back(pl.seg_bmin[i]);
             p->seg_bmax.push_back(pl.seg_bmax[i]);
         }
+        keep_i.emplace_back(pl.pseg_job);
+        for (auto &v : keep_i.back()) v += (int)(8 * o_out);
+        if (!rc) rc = upload_at(d_sj + o_pseg, keep_i.back(), s, err);
+        if (!rc) rc = upload_at(d_s0 + o_pseg, pl.pseg_p0, s, err);
+        if (!rc) rc = upload_at(d_s1 + o_pseg, pl.pseg_p1, s, err);
+        for (size_t i = 0; i < pl.pseg_job.size(); i++) {
+            const double t = pl.out_thr[pl.pseg_job[i] >> 3];
+            p->pseg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);
+            p->pseg_bmin.push_back(pl.pseg_bmin[i]);
+            p->pseg_bmax.push_back(pl.pseg_bmax[i]);
+        }
+        o_pseg += (long long)pl.pseg_job.size();
         o_out += (long long)no;
         o_band += (long long)pl.band.size();
         o_slot += (long long)pl.slot_off.size();
@@ -1446,14 +1524,17 @@ static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n_s
 // tau * thr_cls (the FP64 class) and segments holding one at or below it.  The
 // lists are widened by a relative 1e-9 so that rounding can only add segments;
 // the kernel's own class test decides each term.
-static int make_lists(DevicePlan *p, double tau, std::string &err) {
-    if (p->list_tau == tau) return 0;
+static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
+    if (p->list_tau == tau && p->list_mode == mode) return 0;
     std::vector<int> hi, lo;
     for (int i = 0; i < p->n_seg; i++) {
         const double cut = tau * p->seg_thr[i];
         if (p->seg_bmax[i] > cut * (1.0 - 1e-9)) hi.push_back(i);
-        if (p->seg_bmin[i] <= cut * (1.0 + 1e-9)) lo.push_back(i);
+        if (mode == 1 && p->seg_bmin[i] <= cut * (1.0 + 1e-9)) lo.push_back(i);
     }
+    if (mode == 2)
+        for (int i = 0; i < p->n_pseg; i++)
+            if (p->pseg_bmin[i] <= tau * p->pseg_thr[i] * (1.0 + 1e-9)) lo.push_back(p->n_seg + i);
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
     if (p->list_hi) HIP_TRY(hipFree(p->list_hi));
     if (p->list_lo) HIP_TRY(hipFree(p->list_lo));
@@ -1465,6 +1546,7 @@ static int make_lists(DevicePlan *p, double tau, std::string &err) {
     p->n_hi = (int)hi.size();
     p->n_lo = (int)lo.size();
     p->list_tau = tau;
+    p->list_mode = mode;
     return 0;
 }
 
@@ -1633,7 +1715,7 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     }
     if (mode != 0) {
         int rc = mode == 1 ? ensure_f32(p, err) : ensure_h16(p, err);
-        if (!rc) rc = make_lists(p, tau, err);
+        if (!rc) rc = make_lists(p, mode, tau, err);
         if (rc) return rc;
     }
     // counters cleared by a kernel (not a memset node, so the launch sequence

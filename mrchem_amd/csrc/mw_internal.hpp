@@ -206,6 +206,11 @@ struct ApplyPlan {
     // among them (for choosing the segments of each precision class)
     std::vector<int> seg_apps;
     std::vector<double> seg_bmin, seg_bmax;
+    // Pair segments: the same for the output component pairs (gt, gt | 4),
+    // gt < 4 (job = 8*out + gt), cut by the passing terms of both; the FP16
+    // split kernel shares the x and y passes of a pair.
+    std::vector<int> pseg_job, pseg_p0, pseg_p1, pseg_apps;
+    std::vector<double> pseg_bmin, pseg_bmax;
     long long term_apps = 0;   // exact count of term-applications in the plan
     double thr_floor = 0.0;    // smallest positive threshold of the plan
 };

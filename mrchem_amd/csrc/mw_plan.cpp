@@ -322,34 +322,45 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         }
     }
     plan.term_apps = napps;
-    for (int q = 0; q < nq; q++) {
+    // Cut the band positions of one job (components gts[0..ng)) into
+    // segments of about seg_target passing terms.
+    auto cut = [&](int q, const int *gts, int ng, std::vector<int> &sj, std::vector<int> &s0, std::vector<int> &s1,
+                   std::vector<int> &sa, std::vector<double> &sl, std::vector<double> &sh) {
         const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
-        for (int gt = 0; gt < 8; gt++) {
-            const std::vector<int> &cp = counts[(size_t)q * 8 + gt];
-            const std::vector<double> &bl = bmin[(size_t)q * 8 + gt], &bh = bmax[(size_t)q * 8 + gt];
-            int p0 = 0, acc = 0;
-            double lo = HUGE_VAL, hi = 0.0;
-            for (int p = 0; p < nb; p++) {
-                acc += cp[p];
-                lo = std::min(lo, bl[p]);
-                hi = std::max(hi, bh[p]);
-                if ((acc >= seg_target && p + 1 < nb) || p + 1 == nb) {
-                    // segments without passing terms are dropped: the output
-                    // is zeroed before the launches
-                    if (acc > 0) {
-                        plan.seg_job.push_back(q * 8 + gt);
-                        plan.seg_p0.push_back(p0);
-                        plan.seg_p1.push_back(p + 1);
-                        plan.seg_apps.push_back(acc);
-                        plan.seg_bmin.push_back(lo);
-                        plan.seg_bmax.push_back(hi);
-                    }
-                    p0 = p + 1;
-                    acc = 0;
-                    lo = HUGE_VAL;
-                    hi = 0.0;
-                }
+        int p0 = 0, acc = 0;
+        double lo = HUGE_VAL, hi = 0.0;
+        for (int p = 0; p < nb; p++) {
+            for (int e = 0; e < ng; e++) {
+                const size_t jq = (size_t)q * 8 + gts[e];
+                acc += counts[jq][p];
+                lo = std::min(lo, bmin[jq][p]);
+                hi = std::max(hi, bmax[jq][p]);
             }
+            if ((acc >= seg_target && p + 1 < nb) || p + 1 == nb) {
+                // segments without passing terms are dropped: the output
+                // is zeroed before the launches
+                if (acc > 0) {
+                    sj.push_back(q * 8 + gts[0]);
+                    s0.push_back(p0);
+                    s1.push_back(p + 1);
+                    sa.push_back(acc);
+                    sl.push_back(lo);
+                    sh.push_back(hi);
+                }
+                p0 = p + 1;
+                acc = 0;
+                lo = HUGE_VAL;
+                hi = 0.0;
+            }
+        }
+    };
+    for (int q = 0; q < nq; q++) {
+        for (int gt = 0; gt < 8; gt++)
+            cut(q, &gt, 1, plan.seg_job, plan.seg_p0, plan.seg_p1, plan.seg_apps, plan.seg_bmin, plan.seg_bmax);
+        for (int gt = 0; gt < 4; gt++) {
+            const int pair[2] = {gt, gt | 4};
+            cut(q, pair, 2, plan.pseg_job, plan.pseg_p0, plan.pseg_p1, plan.pseg_apps, plan.pseg_bmin,
+                plan.pseg_bmax);
         }
     }
 }
