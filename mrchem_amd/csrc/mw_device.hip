@@ -528,6 +528,10 @@ struct SplitShape {
     static constexpr int TWG = (NTG + NW - 1) / NW;
     static constexpr int TWZ = (NTZ + NW - 1) / NW;
     static constexpr bool TWO = N > 10;
+    // waves per SIMD the register allocation is held to (VGPRs <= 512 / WPE):
+    // N > 10 at 3 (C5, k = 11: -6 % although 140 bytes per lane spill); no
+    // limit below, where a cap of 4 spills at N = 5, 7, 8 or schedules worse
+    static constexpr int WPE = N > 10 ? 3 : 1;
     // 16-byte chunks per row: hi, lo, tail (+ the extra chunk for N > 10).
     // For N <= 10 the odd number of chunks makes the row pitch spread 16
     // consecutive rows over distinct LDS banks; for N > 10 the four chunks
@@ -652,7 +656,7 @@ constexpr int kZ0Bit = 28, kZ1Bit = 29, kSingleBit = 30;
 constexpr int kSplitCap = 384;  // worklist items per batch (>= 64 + 128)
 
 template <int N>
-__global__ __launch_bounds__(SplitShape<N>::T) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
+__global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_eu(SplitShape<N>::WPE))) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
                                                              const int *seg_list, double lo_mult, double hi_mult,
                                                              double single_mult, int cls_counter) {
     using S = SplitShape<N>;
@@ -1053,8 +1057,9 @@ struct DeviceCtx {
 struct DevicePlan {
     DeviceCtx *ctx = nullptr;
     std::vector<const void *> op_keys;  // per part
-    // segments: n_seg single-component ones, then n_pseg pair ones
-    int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0, n_pseg = 0;
+    // segments: n_seg single-component ones, n_pseg pair ones, n_fseg fine
+    // single-component ones
+    int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0, n_pseg = 0, n_fseg = 0;
     long long npool = 0;
     double *pool = nullptr;
     float *pool32 = nullptr;
@@ -1068,8 +1073,9 @@ struct DevicePlan {
     // per segment: class cut base thr_cls and the range of passing bounds
     std::vector<double> seg_thr, seg_bmin, seg_bmax;
     std::vector<double> pseg_thr, pseg_bmin, pseg_bmax;  // the same for pair segments
-    // segment lists of the two classes, for list_tau and list_mode (the low
-    // class of the split mode lists pair segments)
+    std::vector<double> fseg_thr, fseg_bmin, fseg_bmax;  // and for fine segments
+    // segment lists of the two classes, for list_tau and list_mode (the FP64
+    // class lists fine segments, the low class of the split mode pair ones)
     double list_tau = -1.0;
     int list_mode = -1;
     int *list_hi = nullptr, *list_lo = nullptr;
@@ -1283,7 +1289,8 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         return -1;
     }
     // sizes and validation
-    long long n_out = 0, n_band = 0, n_slot = 0, n_pool = 0, n_bw = 0, n_actoff = 0, n_act = 0, n_seg = 0, n_pseg = 0;
+    long long n_out = 0, n_band = 0, n_slot = 0, n_pool = 0, n_bw = 0, n_actoff = 0, n_act = 0;
+    long long n_seg = 0, n_pseg = 0, n_fseg = 0;
     int wmax = 0;
     for (const PlanPart &pp : parts) {
         const ApplyPlan &pl = *pp.plan;
@@ -1312,8 +1319,10 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         n_act += (long long)pl.active_terms.size();
         n_seg += (long long)pl.seg_job.size();
         n_pseg += (long long)pl.pseg_job.size();
+        n_fseg += (long long)pl.fseg_job.size();
     }
-    if (n_out >= (1LL << 27) || n_seg + n_pseg >= (1LL << 31) || n_slot >= (1LL << 31) || n_band >= (1LL << 31)) {
+    const long long n_all = n_seg + n_pseg + n_fseg;
+    if (n_out >= (1LL << 27) || n_all >= (1LL << 31) || n_slot >= (1LL << 31) || n_band >= (1LL << 31)) {
         err = "batch too large for 32-bit node, segment or slot indices";
         return -1;
     }
@@ -1325,6 +1334,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     p->npool = n_pool;
     p->n_seg = (int)n_seg;
     p->n_pseg = (int)n_pseg;
+    p->n_fseg = (int)n_fseg;
     KArgs &a = p->args;
     hipStream_t s = c->stream;
     int rc = 0;
@@ -1343,9 +1353,9 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     if (!rc) rc = plan_alloc(p, n_bw, &d_bw, err);
     if (!rc) rc = plan_alloc(p, n_actoff, &d_act_off, err);
     if (!rc) rc = plan_alloc(p, n_act, &d_act, err);
-    if (!rc) rc = plan_alloc(p, n_seg + n_pseg, &d_sj, err);
-    if (!rc) rc = plan_alloc(p, n_seg + n_pseg, &d_s0, err);
-    if (!rc) rc = plan_alloc(p, n_seg + n_pseg, &d_s1, err);
+    if (!rc) rc = plan_alloc(p, n_all, &d_sj, err);
+    if (!rc) rc = plan_alloc(p, n_all, &d_s0, err);
+    if (!rc) rc = plan_alloc(p, n_all, &d_s1, err);
     if (!rc) rc = plan_alloc(p, parts.size(), &p->d_apps, err);
     p->out_bytes = sizeof(double) * (size_t)std::max<long long>(n_out, 1) * 8 * N * N * N;
     if (!rc) rc = plan_alloc(p, p->out_bytes / sizeof(double), &a.out_ns, err);
@@ -1356,7 +1366,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     std::vector<std::vector<long long>> keep_l;
     std::vector<std::vector<double>> keep_d;
     long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0, o_seg = 0;
-    long long o_pseg = n_seg;
+    long long o_pseg = n_seg, o_fseg = n_seg + n_pseg;
     p->seg_thr.reserve(n_seg);
     p->seg_bmin.reserve(n_seg);
     p->seg_bmax.reserve(n_seg);
@@ -1439,6 +1449,18 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
             p->pseg_bmax.push_back(pl.pseg_bmax[i]);
         }
         o_pseg += (long long)pl.pseg_job.size();
+        keep_i.emplace_back(pl.fseg_job);
+        for (auto &v : keep_i.back()) v += (int)(8 * o_out);
+        if (!rc) rc = upload_at(d_sj + o_fseg, keep_i.back(), s, err);
+        if (!rc) rc = upload_at(d_s0 + o_fseg, pl.fseg_p0, s, err);
+        if (!rc) rc = upload_at(d_s1 + o_fseg, pl.fseg_p1, s, err);
+        for (size_t i = 0; i < pl.fseg_job.size(); i++) {
+            const double t = pl.out_thr[pl.fseg_job[i] >> 3];
+            p->fseg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);
+            p->fseg_bmin.push_back(pl.fseg_bmin[i]);
+            p->fseg_bmax.push_back(pl.fseg_bmax[i]);
+        }
+        o_fseg += (long long)pl.fseg_job.size();
         o_out += (long long)no;
         o_band += (long long)pl.band.size();
         o_slot += (long long)pl.slot_off.size();
@@ -1527,11 +1549,11 @@ static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n_s
 static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
     if (p->list_tau == tau && p->list_mode == mode) return 0;
     std::vector<int> hi, lo;
-    for (int i = 0; i < p->n_seg; i++) {
-        const double cut = tau * p->seg_thr[i];
-        if (p->seg_bmax[i] > cut * (1.0 - 1e-9)) hi.push_back(i);
-        if (mode == 1 && p->seg_bmin[i] <= cut * (1.0 + 1e-9)) lo.push_back(i);
-    }
+    for (int i = 0; i < p->n_fseg; i++)
+        if (p->fseg_bmax[i] > tau * p->fseg_thr[i] * (1.0 - 1e-9)) hi.push_back(p->n_seg + p->n_pseg + i);
+    if (mode == 1)
+        for (int i = 0; i < p->n_seg; i++)
+            if (p->seg_bmin[i] <= tau * p->seg_thr[i] * (1.0 + 1e-9)) lo.push_back(i);
     if (mode == 2)
         for (int i = 0; i < p->n_pseg; i++)
             if (p->pseg_bmin[i] <= tau * p->pseg_thr[i] * (1.0 + 1e-9)) lo.push_back(p->n_seg + i);

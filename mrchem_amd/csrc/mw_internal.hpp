@@ -211,6 +211,11 @@ struct ApplyPlan {
     // split kernel shares the x and y passes of a pair.
     std::vector<int> pseg_job, pseg_p0, pseg_p1, pseg_apps;
     std::vector<double> pseg_bmin, pseg_bmax;
+    // Fine segments: single-component jobs cut at about seg_target / 8
+    // passing terms, for the launch of the FP64 class of the mixed and split
+    // modes, whose terms crowd into few segments (coarse output nodes).
+    std::vector<int> fseg_job, fseg_p0, fseg_p1, fseg_apps;
+    std::vector<double> fseg_bmin, fseg_bmax;
     long long term_apps = 0;   // exact count of term-applications in the plan
     double thr_floor = 0.0;    // smallest positive threshold of the plan
 };

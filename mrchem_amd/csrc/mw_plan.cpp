@@ -323,9 +323,9 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     }
     plan.term_apps = napps;
     // Cut the band positions of one job (components gts[0..ng)) into
-    // segments of about seg_target passing terms.
-    auto cut = [&](int q, const int *gts, int ng, std::vector<int> &sj, std::vector<int> &s0, std::vector<int> &s1,
-                   std::vector<int> &sa, std::vector<double> &sl, std::vector<double> &sh) {
+    // segments of about `target` passing terms.
+    auto cut = [&](int q, const int *gts, int ng, int target, std::vector<int> &sj, std::vector<int> &s0,
+                   std::vector<int> &s1, std::vector<int> &sa, std::vector<double> &sl, std::vector<double> &sh) {
         const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
         int p0 = 0, acc = 0;
         double lo = HUGE_VAL, hi = 0.0;
@@ -336,7 +336,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
                 lo = std::min(lo, bmin[jq][p]);
                 hi = std::max(hi, bmax[jq][p]);
             }
-            if ((acc >= seg_target && p + 1 < nb) || p + 1 == nb) {
+            if ((acc >= target && p + 1 < nb) || p + 1 == nb) {
                 // segments without passing terms are dropped: the output
                 // is zeroed before the launches
                 if (acc > 0) {
@@ -354,12 +354,17 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
             }
         }
     };
+    const int fine_target = std::max(1, seg_target / 8);
     for (int q = 0; q < nq; q++) {
-        for (int gt = 0; gt < 8; gt++)
-            cut(q, &gt, 1, plan.seg_job, plan.seg_p0, plan.seg_p1, plan.seg_apps, plan.seg_bmin, plan.seg_bmax);
+        for (int gt = 0; gt < 8; gt++) {
+            cut(q, &gt, 1, seg_target, plan.seg_job, plan.seg_p0, plan.seg_p1, plan.seg_apps, plan.seg_bmin,
+                plan.seg_bmax);
+            cut(q, &gt, 1, fine_target, plan.fseg_job, plan.fseg_p0, plan.fseg_p1, plan.fseg_apps, plan.fseg_bmin,
+                plan.fseg_bmax);
+        }
         for (int gt = 0; gt < 4; gt++) {
             const int pair[2] = {gt, gt | 4};
-            cut(q, pair, 2, plan.pseg_job, plan.pseg_p0, plan.pseg_p1, plan.pseg_apps, plan.pseg_bmin,
+            cut(q, pair, 2, seg_target, plan.pseg_job, plan.pseg_p0, plan.pseg_p1, plan.pseg_apps, plan.pseg_bmin,
                 plan.pseg_bmax);
         }
     }
