@@ -1050,7 +1050,8 @@ struct DeviceCtx {
     // second stream (default priority): the FP64-class launch of the mixed
     // modes runs on it beside the low-precision launch
     hipStream_t stream2 = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evz = nullptr, ev2s = nullptr, ev2e = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evz = nullptr, evl = nullptr, ev2s = nullptr,
+               ev2e = nullptr;
     std::map<const void *, DevOper> opers;
 };
 
@@ -1107,6 +1108,7 @@ int dev_create(int device, DeviceCtx **out, std::string &err) {
     if (e == hipSuccess) e = hipEventCreate(&c->evm);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->evz);
+    if (e == hipSuccess) e = hipEventCreate(&c->evl);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2s);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2e);
     if (e != hipSuccess) {
@@ -1139,6 +1141,7 @@ void dev_destroy(DeviceCtx *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->evm) (void)hipEventDestroy(c->evm);
     if (c->evz) (void)hipEventDestroy(c->evz);
+    if (c->evl) (void)hipEventDestroy(c->evl);
     if (c->ev2s) (void)hipEventDestroy(c->ev2s);
     if (c->ev2e) (void)hipEventDestroy(c->ev2e);
     if (c->stream2) {
@@ -1766,6 +1769,7 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
         rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, err, c->stream2);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
+        HIP_TRY(hipEventRecord(c->evl, c->stream));
         if (mode == 1) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
         else rc = launch_h(p, p->list_lo, p->n_lo, 0.0, tau, 1, err);
         if (rc) return rc;
@@ -1786,7 +1790,7 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
             // device time of each class launch (they overlap)
             float m1 = 0.f, m2 = 0.f;
             HIP_TRY(hipEventElapsedTime(&m1, c->ev2s, c->ev2e));
-            HIP_TRY(hipEventElapsedTime(&m2, c->evz, c->evm));
+            HIP_TRY(hipEventElapsedTime(&m2, c->evl, c->evm));
             split_ms[0] = m1;
             split_ms[1] = m2;
         }
