@@ -117,6 +117,22 @@ def main():
     del dev_ops
     log("operators built on the device: %d in %.3fs (host: %.2fs)" % (len(umus), setup["operator_build_device_s"],
                                                                       setup["operator_build_s"]))
+    if cfg.get("orbitals"):
+        # pair density rho_01 = phi_0 phi_1 formed on the device (mwg_multiply)
+        # against the projection of the exact product (the apply's input 1)
+        phi = [mw.FunctionTree(mra).project_gaussians(cfg["prec"], *o) for o in cfg["orbitals"][:2]]
+        reps = []
+        for _ in range(3):
+            t0 = time.time()
+            rho = mw.multiply(cfg["prec"], mw.FunctionTree(mra), phi[0], phi[1], abs_prec=True, ctx=ctx)
+            reps.append(time.time() - t0)
+        ref = inps[1]
+        diff = float(np.sqrt(max(rho.norm() ** 2 + ref.norm() ** 2 - 2.0 * rho.dot(ref), 0.0)))
+        setup["pair_density_device"] = {"s": min(reps), "out_nodes": rho.size(),
+                                        "in_nodes": [phi[0].size(), phi[1].size()], "diff_norm": diff,
+                                        "norm": ref.norm()}
+        log("pair density phi_0 phi_1 on the device: %d nodes (inputs %d, %d) in %.3fs, |rho - exact| %.2e "
+            "(|exact| %.3e)" % (rho.size(), phi[0].size(), phi[1].size(), min(reps), diff, ref.norm()))
     outs = [mw.FunctionTree(mra) for _ in inps]
     prec = cfg["prec"]
     entries = [dict(prec=prec if cfg["abs_prec"] else prec / f.norm(), out=o, oper=op, inp=f,

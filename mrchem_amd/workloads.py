@@ -66,7 +66,7 @@ def config(name):
         orbs = [gaussian_sum(rng, 6, 0.3, 30.0, 4.0) for _ in range(16)]
         dens = [pair_density(orbs[i], orbs[j]) for i in range(16) for j in range(i, 16)]
         return dict(name="c3_exchange_136_pairs_k7", kind="poisson", order=7, prec=1e-5, abs_prec=True,
-                    densities=dens)
+                    densities=dens, orbitals=orbs)
     if name == "c4":
         c, a, R = gaussian_sum(rng, 8, 1.0, 100.0, 3.0)
         return dict(name="c4_poisson_8gauss_k9", kind="poisson", order=9, prec=1e-6, abs_prec=False,
