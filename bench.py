@@ -133,6 +133,24 @@ def main():
                                         "norm": ref.norm()}
         log("pair density phi_0 phi_1 on the device: %d nodes (inputs %d, %d) in %.3fs, |rho - exact| %.2e "
             "(|exact| %.3e)" % (rho.size(), phi[0].size(), phi[1].size(), min(reps), diff, ref.norm()))
+        # linear combination of 16 pair densities on their union grid (the
+        # exchange sums, mwg_add); checked by the norm of the sum against the
+        # pairwise inner products
+        m = 16
+        cs = np.linspace(1.0, -1.0, m)
+        reps = []
+        for _ in range(3):
+            t0 = time.time()
+            lc = mw.add(-1.0, mw.FunctionTree(mra), cs, inps[:m], ctx=ctx)
+            reps.append(time.time() - t0)
+        gram = np.array([[inps[i].dot(inps[j]) for j in range(m)] for i in range(m)])
+        sq = float(cs @ gram @ cs)
+        setup["linear_combination_device"] = {"s": min(reps), "inputs": m, "out_nodes": lc.size(),
+                                              "in_nodes": int(sum(f.size() for f in inps[:m])),
+                                              "norm_rel_err": abs(lc.norm() ** 2 - sq) / sq}
+        log("linear combination of %d pair densities on the device: %d nodes (inputs %d nodes) in %.3fs, "
+            "norm identity %.1e" % (m, lc.size(), setup["linear_combination_device"]["in_nodes"], min(reps),
+                                     setup["linear_combination_device"]["norm_rel_err"]))
     outs = [mw.FunctionTree(mra) for _ in inps]
     prec = cfg["prec"]
     entries = [dict(prec=prec if cfg["abs_prec"] else prec / f.norm(), out=o, oper=op, inp=f,

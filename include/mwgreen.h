@@ -129,6 +129,27 @@ int mwg_tree_rescale(mwg_tree tree, double factor);
  * Interpolating basis only; out must differ from a and b. */
 int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
                  int abs_prec);
+/* out = sum_i coefs[i] inputs[i]: mrcpp::add(prec, out, FunctionTreeVector,
+ * max_iter, abs_prec), which MRChem calls through
+ * mrcpp::cplxfunc::linear_combination for the exchange sums
+ * (src/qmoperators/two_electron/ExchangePotentialD1.cpp:317,366).  Each
+ * output node's block is summed on the device from the inputs' blocks at the
+ * node (below an input's finest node: its scaling part reconstructed from
+ * there).  prec < 0: on the union of the input grids; prec >= 0: built from
+ * the roots, a node split while the sum's wavelet norm fails the split check
+ * (absolute with abs_prec, else relative to the sum's norm), for at most
+ * max_iter levels (< 0: no limit).  out must differ from every input. */
+int mwg_add(mwg_context ctx, double prec, mwg_tree out, int n, const double *coefs, const mwg_tree *inputs,
+            int max_iter, int abs_prec);
+/* tree = tree + coef * inp on the union of the two grids, in place:
+ * FunctionTree::add(c, inp) (ExchangePotentialD1.cpp:367). */
+int mwg_tree_add(mwg_context ctx, mwg_tree tree, double coef, mwg_tree inp);
+/* Remove the children of every branch node whose children are all leaves
+ * and whose wavelet norm is at most max(2 eps, prec * t * 2^{-split_fac
+ * (scale + 1) / 2}) (t = 1 with abs_prec, else the tree's norm), bottom-up:
+ * FunctionTree::crop(prec, splitFac, absPrec) (ExchangePotentialD1.cpp:323,371).
+ * Host operation on the tree structure. */
+int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec);
 /* Screening threshold of every node used by the last apply into this tree. */
 int mwg_tree_thresholds(mwg_tree tree, double *thr);
 

@@ -58,6 +58,9 @@ _sig("mwg_gaussian_operator_create", _p, _i, _dp, _dp, _i, ctypes.POINTER(_p))
 _sig("mwg_operator_destroy", _p)
 _sig("mwg_operators_create_device", _p, _p, _i, _dp, _d, ctypes.POINTER(_p))
 _sig("mwg_multiply", _p, _d, _p, _d, _p, _p, _i, _i)
+_sig("mwg_add", _p, _d, _p, _i, _dp, ctypes.POINTER(_p), _i, _i)
+_sig("mwg_tree_add", _p, _p, _d, _p)
+_sig("mwg_tree_crop", _p, _d, _d, _i)
 _sig("mwg_operator_info", _p, _ip, _ip, _ip, _dp, _dp, _dp)
 _sig("mwg_operator_terms", _p, _dp, _dp)
 _sig("mwg_operator_table_sizes", _p, _llp, _llp)
@@ -342,6 +345,19 @@ class FunctionTree:
         _check(lib.mwg_tree_rescale(self._h, f))
         return self
 
+    def add(self, c, inp, ctx=None):
+        """self += c * inp on the union of the grids (FunctionTree::add; the
+        node sums run on the device)."""
+        ctx = ctx or Context.default()
+        _check(lib.mwg_tree_add(ctx._h, self._h, c, inp._h))
+        return self
+
+    def crop(self, prec, split_fac=1.0, abs_prec=True):
+        """FunctionTree::crop(prec, splitFac, absPrec): drop the children of
+        branch nodes with small wavelet norms, bottom-up."""
+        _check(lib.mwg_tree_crop(self._h, prec, split_fac, 1 if abs_prec else 0))
+        return self
+
     def thresholds(self):
         thr = np.zeros(self.size())
         _check(lib.mwg_tree_thresholds(self._h, _ptr(thr, _d)))
@@ -371,6 +387,18 @@ def multiply(prec, out, a, b, coef=1.0, max_iter=-1, abs_prec=False, ctx=None):
     basis)."""
     ctx = ctx or Context.default()
     _check(lib.mwg_multiply(ctx._h, prec, out._h, coef, a._h, b._h, max_iter, 1 if abs_prec else 0))
+    return out
+
+
+def add(prec, out, coefs, trees, max_iter=-1, abs_prec=False, ctx=None):
+    """out = sum_i coefs[i] trees[i] (mrcpp::add / cplxfunc::linear_combination);
+    prec < 0: on the union grid, else built adaptively to prec."""
+    ctx = ctx or Context.default()
+    c = np.ascontiguousarray(coefs, dtype=np.float64)
+    if len(c) != len(trees):
+        raise ValueError("one coefficient per tree")
+    arr = (_p * len(trees))(*[t._h for t in trees])
+    _check(lib.mwg_add(ctx._h, prec, out._h, len(trees), _ptr(c, _d), arr, max_iter, 1 if abs_prec else 0))
     return out
 
 

@@ -78,6 +78,25 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
                   std::string &err);
 void dev_mul_end(DevMulSession *m);
 
+// Node-wise sums (mw_multiply.hip): for one output node and one input, the
+// input's source node (its deepest node on the path: block offset in the
+// concatenated pool of all inputs, or -1 when the input has no node on the
+// path), the levels from it down to the output node and the child index of
+// each level (3 bits per level, the first level below the source lowest).
+struct AddSrc {
+    long long off;
+    int dep, pad_;
+    unsigned long long path;
+};
+constexpr int kAddMaxLevels = 21;  // levels a 64-bit path holds
+struct DevAddSession;
+int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool, const std::vector<double> &coefs,
+                  DevAddSession **out, std::string &err);
+// src: n_nodes x n_inputs entries (node-major); out_blocks: n_nodes x
+// 8(k+1)^3 on the host.
+int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err);
+void dev_add_end(DevAddSession *m);
+
 // Helpers shared by the HIP sources.
 struct DevOperTables {
     int *L = nullptr;

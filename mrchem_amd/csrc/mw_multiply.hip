@@ -114,6 +114,42 @@ __device__ void child_pass(int n, const double *F, const double *src, const int 
     }
 }
 
+// Scaling block of the node `dep` >= 1 levels below the source node block
+// `src` (8 components) along `path` (3 bits per level, first level lowest)
+// into X[0, n^3); X holds 4 and Y 2 blocks of scratch.
+__device__ void descend(int n, const double *F, const double *src, int dep, unsigned long long path, double *X,
+                        double *Y, int tid) {
+    const int n3 = n * n * n;
+    // first level: the full block (scaling and wavelet parts) towards child t
+    const int t = path & 7;
+    {
+        const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
+        child_pass(n, F, src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
+    }
+    __syncthreads();
+    {
+        const int lo[2] = {0, 1}, hi[2] = {2, 3};
+        child_pass(n, F, X, lo, hi, 2, Y, 1, (t >> 1) & 1, tid);
+    }
+    __syncthreads();
+    {
+        const int lo[1] = {0}, hi[1] = {1};
+        child_pass(n, F, Y, lo, hi, 1, X, 0, t & 1, tid);
+    }
+    __syncthreads();
+    // further levels: scaling part only
+    for (int lv = 1; lv < dep; lv++) {
+        const int tl = (path >> (3 * lv)) & 7;
+        const int lo[1] = {0}, none[1] = {-1};
+        child_pass(n, F, X, lo, none, 1, Y, 2, (tl >> 2) & 1, tid);
+        __syncthreads();
+        child_pass(n, F, Y, lo, none, 1, Y + n3, 1, (tl >> 1) & 1, tid);
+        __syncthreads();
+        child_pass(n, F, Y + n3, lo, none, 1, X, 0, tl & 1, tid);
+        __syncthreads();
+    }
+}
+
 // Scaling blocks of the 8 children of the work node, from the source node
 // block `src` (8 components) `dep` levels above it along `path`, into X
 // (Y: scratch of 8 blocks).
@@ -124,34 +160,7 @@ __device__ void children_of(int n, const double *F, const double *src, int dep, 
         for (int e = tid; e < 8 * n3; e += kMulThreads) Y[e] = src[e];
         __syncthreads();
     } else {
-        // first level: the full block (scaling and wavelet parts) towards child t
-        const int t = path & 7;
-        {
-            const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
-            child_pass(n, F, src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
-        }
-        __syncthreads();
-        {
-            const int lo[2] = {0, 1}, hi[2] = {2, 3};
-            child_pass(n, F, X, lo, hi, 2, Y, 1, (t >> 1) & 1, tid);
-        }
-        __syncthreads();
-        {
-            const int lo[1] = {0}, hi[1] = {1};
-            child_pass(n, F, Y, lo, hi, 1, X, 0, t & 1, tid);
-        }
-        __syncthreads();
-        // further levels: scaling part only
-        for (int lv = 1; lv < dep; lv++) {
-            const int tl = (path >> (3 * lv)) & 7;
-            const int lo[1] = {0}, none[1] = {-1};
-            child_pass(n, F, X, lo, none, 1, Y, 2, (tl >> 2) & 1, tid);
-            __syncthreads();
-            child_pass(n, F, Y, lo, none, 1, Y + n3, 1, (tl >> 1) & 1, tid);
-            __syncthreads();
-            child_pass(n, F, Y + n3, lo, none, 1, X, 0, tl & 1, tid);
-            __syncthreads();
-        }
+        descend(n, F, src, dep, path, X, Y, tid);
         // the work node's block: scaling part X[0], wavelet parts zero
         for (int e = tid; e < 8 * n3; e += kMulThreads) Y[e] = e < n3 ? X[e] : 0.0;
         __syncthreads();
@@ -191,6 +200,38 @@ __global__ __launch_bounds__(kMulThreads) void k_mul_nodes(int n, const MulNode 
         double *dst = out + (size_t)w * 8 * n3;
         pair_pass(n, F, Y, dst, 2, false, tid);
         __syncthreads();
+    }
+}
+
+// Sums: out block of node w = sum_i coefs[i] x (input i's block at the node,
+// or, below the input's finest node on the path, the scaling part
+// reconstructed from that node with zero wavelet parts).  src[w * n_in + i]
+// locates input i's source node (off < 0: the input has no node on the path
+// and contributes nothing).  Each workgroup owns its output blocks, so the
+// sums need no atomics; every thread adds into the same elements for every
+// input.
+__global__ __launch_bounds__(kMulThreads) void k_add_nodes(int n, const AddSrc *src, int n_in, int n_nodes,
+                                                           const double *pool, const double *coefs, const double *F,
+                                                           double *scratch, double *out) {
+    const int tid = threadIdx.x;
+    const int n3 = n * n * n;
+    double *X = scratch + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
+    for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
+        double *dst = out + (size_t)w * 8 * n3;
+        for (int e = tid; e < 8 * n3; e += kMulThreads) dst[e] = 0.0;
+        for (int i = 0; i < n_in; i++) {
+            const AddSrc sr = src[(size_t)w * n_in + i];
+            if (sr.off < 0) continue;
+            const double c = coefs[i];
+            if (sr.dep == 0) {
+                const double *b = pool + sr.off;
+                for (int e = tid; e < 8 * n3; e += kMulThreads) dst[e] += c * b[e];
+            } else {
+                descend(n, F, pool + sr.off, sr.dep, sr.path, X, Y, tid);
+                for (int e = tid; e < n3; e += kMulThreads) dst[e] += c * X[e];
+                __syncthreads();  // X is rewritten for the next input
+            }
+        }
     }
 }
 
@@ -282,6 +323,93 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
 }
 
 void dev_mul_end(DevMulSession *m) {
+    if (!m) return;
+    (void)hipSetDevice(dev_device(m->ctx));
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(dev_stream(m->ctx)));
+    delete m;
+}
+
+struct DevAddSession {
+    DeviceCtx *ctx = nullptr;
+    int n = 0, n_in = 0;
+    double *pool = nullptr, *coefs = nullptr, *F = nullptr, *scratch = nullptr;
+    ~DevAddSession() {
+        for (double *p : {pool, coefs, F, scratch})
+            if (p) (void)hipFree(p);
+    }
+};
+
+int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool, const std::vector<double> &coefs,
+                  DevAddSession **out, std::string &err) {
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    auto *m = new DevAddSession;
+    m->ctx = c;
+    m->n = mra.kp1();
+    m->n_in = (int)coefs.size();
+    const int n3 = m->n * m->n * m->n;
+    auto up = [&](const std::vector<double> &v, double **d) -> int {
+        HIP_TRY(hipMalloc((void **)d, sizeof(double) * std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) HIP_TRY(hipMemcpyAsync(*d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
+        return 0;
+    };
+    int rc = up(pool, &m->pool);
+    if (!rc) rc = up(coefs, &m->coefs);
+    if (!rc) rc = up(mra.basis.F, &m->F);
+    if (!rc) {
+        hipError_t e = hipMalloc((void **)&m->scratch, sizeof(double) * (size_t)kMulBlocks * 16 * n3);
+        if (e != hipSuccess) {
+            err = std::string("hipMalloc(scratch): ") + hipGetErrorString(e);
+            rc = -2;
+        }
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err) {
+    const size_t n_nodes = m->n_in > 0 ? src.size() / (size_t)m->n_in : 0;
+    if (n_nodes == 0) return 0;
+    if (src.size() != n_nodes * (size_t)m->n_in) {
+        err = "source table size is not a multiple of the input count";
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(dev_device(m->ctx)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(m->ctx));
+    const int n3 = m->n * m->n * m->n;
+    AddSrc *dsrc = nullptr;
+    double *dout = nullptr;
+    HIP_TRY(hipMalloc((void **)&dsrc, sizeof(AddSrc) * src.size()));
+    hipError_t e = hipMalloc((void **)&dout, sizeof(double) * n_nodes * 8 * n3);
+    if (e != hipSuccess) {
+        (void)hipFree(dsrc);
+        err = std::string("hipMalloc(out): ") + hipGetErrorString(e);
+        return -2;
+    }
+    e = hipMemcpyAsync(dsrc, src.data(), sizeof(AddSrc) * src.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        const int grid = (int)std::min<size_t>(n_nodes, kMulBlocks);
+        hipLaunchKernelGGL(k_add_nodes, dim3(grid), dim3(kMulThreads), 0, s, m->n, dsrc, m->n_in, (int)n_nodes,
+                           m->pool, m->coefs, m->F, m->scratch, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dsrc);
+    (void)hipFree(dout);
+    if (e != hipSuccess) {
+        err = std::string("add launch: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+void dev_add_end(DevAddSession *m) {
     if (!m) return;
     (void)hipSetDevice(dev_device(m->ctx));
     (void)hipStreamSynchronize(static_cast<hipStream_t>(dev_stream(m->ctx)));

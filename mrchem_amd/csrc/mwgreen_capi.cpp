@@ -552,6 +552,129 @@ bool mul_source(const Tree &f, const NodeKey &k, long long &off, int &dep, unsig
     }
     return false;
 }
+
+// Source node of input f for the sum at output node k: f's deepest node on
+// the path, its block offset (base + index x block), the levels below it and
+// the child index of each level.  An input without any node on the path
+// (an empty tree) gets off = -1.  False when the node lies more than
+// kAddMaxLevels levels below f's finest node on the path.
+bool add_source(const Tree &f, const NodeKey &k, long long base, AddSrc &sr) {
+    sr.off = -1;
+    sr.dep = 0;
+    sr.pad_ = 0;
+    sr.path = 0;
+    for (int d = k.depth; d >= 0; d--) {
+        long long l[3];
+        for (int q = 0; q < 3; q++) l[q] = k.l[q] >> (k.depth - d);
+        const int idx = f.find(d, l);
+        if (idx < 0) continue;
+        sr.dep = k.depth - d;
+        if (sr.dep > kAddMaxLevels) return false;
+        for (int lv = 1; lv <= sr.dep; lv++) {
+            unsigned long long t = 0;
+            for (int q = 0; q < 3; q++) t |= (unsigned long long)((k.l[q] >> (k.depth - d - lv)) & 1) << q;
+            sr.path |= t << (3 * (lv - 1));
+        }
+        sr.off = base + (long long)idx * f.mra->block();
+        return true;
+    }
+    return true;
+}
+
+// Nodes of f in order of depth, each after its parent.
+std::vector<int> by_depth(const Tree &f) {
+    std::vector<int> order(f.size());
+    for (int i = 0; i < f.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return f.keys[x].depth < f.keys[y].depth; });
+    return order;
+}
+
+// T = sum_i coefs[i] ins[i] (mrcpp::add).  prec < 0: on the union of the
+// input grids, every node computed in one launch.  prec >= 0: built from the
+// roots, level by level, splitting a node when the sum's wavelet norm fails
+// the split check (absolute, or relative to the sum's norm estimated from the
+// leaves), for at most max_iter levels (< 0: no limit) and the MRA's depth.
+int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tree *> &ins,
+              const std::vector<double> &coefs, int max_iter, bool abs_prec) {
+    const MRA &mra = *T.mra;
+    const int blk = mra.block(), n3 = mra.kp1_3();
+    std::vector<double> pool;
+    std::vector<long long> base(ins.size());
+    size_t total = 0;
+    for (const Tree *f : ins) total += f->coefs.size();
+    pool.reserve(total);
+    for (size_t i = 0; i < ins.size(); i++) {
+        base[i] = (long long)pool.size();
+        pool.insert(pool.end(), ins[i]->coefs.begin(), ins[i]->coefs.end());
+    }
+    T.clear();
+    T.make_roots();
+    if (prec < 0.0) {
+        for (const Tree *f : ins)
+            for (int i : by_depth(*f)) {
+                const NodeKey &k = f->keys[i];
+                if (k.depth == 0 || T.find(k.depth, k.l) >= 0) continue;
+                long long lp[3] = {k.l[0] >> 1, k.l[1] >> 1, k.l[2] >> 1};
+                const int par = T.find(k.depth - 1, lp);
+                if (par < 0) return fail(MWG_ERR_ARG, "input tree is not complete from its roots");
+                T.split(par);
+            }
+    }
+    std::vector<int> work;
+    if (prec < 0.0) {
+        for (int i = 0; i < T.size(); i++) work.push_back(i);
+    } else {
+        for (int i = 0; i < T.size(); i++)
+            if (T.keys[i].depth == 0) work.push_back(i);
+    }
+    std::string err;
+    DevAddSession *m = nullptr;
+    int rc = dev_add_begin(ctx->dev, mra, pool, coefs, &m, err);
+    if (rc) return dev_fail(rc, err);
+    const size_t n_in = ins.size();
+    for (int iter = 0;; iter++) {
+        std::vector<AddSrc> src(work.size() * n_in);
+        for (size_t w = 0; w < work.size(); w++)
+            for (size_t i = 0; i < n_in; i++)
+                if (!add_source(*ins[i], T.keys[work[w]], base[i], src[w * n_in + i])) {
+                    dev_add_end(m);
+                    return fail(MWG_ERR_ARG, "output node more than " + std::to_string(kAddMaxLevels) +
+                                                 " levels below an input's finest node");
+                }
+        std::vector<double> blocks(work.size() * (size_t)blk);
+        rc = dev_add_nodes(m, src, blocks.data(), err);
+        if (rc) {
+            dev_add_end(m);
+            return dev_fail(rc, err);
+        }
+        for (size_t w = 0; w < work.size(); w++)
+            std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
+        if (prec < 0.0 || (max_iter >= 0 && iter >= max_iter)) break;
+        double tnorm = 1.0;
+        if (!abs_prec) {
+            double sq = 0.0;
+            for (int i = 0; i < T.size(); i++)
+                if (T.child0[i] < 0)
+                    for (int e = 0; e < n3; e++) sq += T.node(i)[e] * T.node(i)[e];
+            tnorm = std::sqrt(sq);
+        }
+        std::vector<int> next;
+        for (int wi : work) {
+            const int d = T.keys[wi].depth;
+            double wsq = 0.0;
+            for (int e = n3; e < blk; e++) wsq += T.node(wi)[e] * T.node(wi)[e];
+            if (d < mra.max_depth && split_check(mra, d, std::sqrt(wsq), prec, tnorm)) {
+                const int c0 = T.split(wi);
+                for (int t = 0; t < 8; t++) next.push_back(c0 + t);
+            }
+        }
+        if (next.empty()) break;
+        work.swap(next);
+    }
+    dev_add_end(m);
+    T.sqnorm = tree_square_norm(T);
+    return MWG_OK;
+}
 }  // namespace
 
 int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
@@ -636,6 +759,107 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
         tree_compress(T);
         T.sqnorm = tree_square_norm(T);
         return MWG_OK;
+    });
+}
+
+int mwg_add(mwg_context ctx, double prec, mwg_tree out, int n, const double *coefs, const mwg_tree *inputs,
+            int max_iter, int abs_prec) {
+    return guarded([&] {
+        if (!ctx || !out || !coefs || !inputs) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (n < 1) return fail(MWG_ERR_ARG, "at least one input tree is required");
+        if (std::isnan(prec)) return fail(MWG_ERR_ARG, "bad precision");
+        std::vector<const Tree *> ins(n);
+        std::vector<double> cs(coefs, coefs + n);
+        for (int i = 0; i < n; i++) {
+            if (!inputs[i]) return fail(MWG_ERR_ARG, "null input tree");
+            if (inputs[i] == out) return fail(MWG_ERR_ARG, "the output tree must differ from the inputs");
+            if (inputs[i]->tree.mra != out->tree.mra) return fail(MWG_ERR_ARG, "trees of different MRAs");
+            if (!std::isfinite(cs[i])) return fail(MWG_ERR_ARG, "non-finite coefficient");
+            ins[i] = &inputs[i]->tree;
+        }
+        return add_trees(ctx, prec, out->tree, ins, cs, max_iter, abs_prec != 0);
+    });
+}
+
+int mwg_tree_add(mwg_context ctx, mwg_tree tree, double coef, mwg_tree inp) {
+    return guarded([&] {
+        if (!ctx || !tree || !inp) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (tree == inp) {
+            // f + c f on f's own grid
+            if (!std::isfinite(coef)) return fail(MWG_ERR_ARG, "non-finite coefficient");
+            for (double &v : tree->tree.coefs) v *= 1.0 + coef;
+            tree->tree.sqnorm = tree_square_norm(tree->tree);
+            return (int)MWG_OK;
+        }
+        if (inp->tree.mra != tree->tree.mra) return fail(MWG_ERR_ARG, "trees of different MRAs");
+        if (!std::isfinite(coef)) return fail(MWG_ERR_ARG, "non-finite coefficient");
+        Tree sum;
+        sum.mra = tree->tree.mra;
+        const std::vector<const Tree *> ins = {&tree->tree, &inp->tree};
+        const int rc = add_trees(ctx, -1.0, sum, ins, {1.0, coef}, -1, true);
+        if (rc != MWG_OK) return rc;
+        tree->tree = std::move(sum);
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec) {
+    return guarded([&] {
+        if (!tree) return fail(MWG_ERR_ARG, "null tree");
+        if (std::isnan(prec) || !(split_fac >= 0.0)) return fail(MWG_ERR_ARG, "bad precision or split factor");
+        Tree &T = tree->tree;
+        if (T.size() == 0 || !(prec > 0.0)) return (int)MWG_OK;
+        const MRA &mra = *T.mra;
+        const int blk = mra.block(), n3 = mra.kp1_3();
+        const double tnorm = abs_prec ? 1.0 : std::sqrt(std::max(tree_square_norm(T), 0.0));
+        // post-order: a branch whose children are all leaves (after their own
+        // crop) and whose wavelet norm fails the split check loses them
+        std::vector<char> keep(T.size(), 1), leaf(T.size(), 0);
+        std::vector<int> order = by_depth(T);
+        for (auto it = order.rbegin(); it != order.rend(); ++it) {
+            const int i = *it;
+            if (T.child0[i] < 0) {
+                leaf[i] = 1;
+                continue;
+            }
+            bool all_leaves = true;
+            for (int t = 0; t < 8; t++) all_leaves = all_leaves && leaf[T.child0[i] + t];
+            if (!all_leaves) continue;
+            double wsq = 0.0;
+            for (int e = n3; e < blk; e++) wsq += T.node(i)[e] * T.node(i)[e];
+            const int d = T.keys[i].depth;
+            // the split check with the wavelet threshold's scale exponent
+            // multiplied by split_fac (1: the WaveletAdaptor rule)
+            const double scale = mra.n0 + d;
+            const double thr = std::max(2.0 * 2.220446049250313e-16,
+                                        prec * tnorm * std::pow(2.0, -0.5 * split_fac * (scale + 1)));
+            if (!(std::sqrt(wsq) > thr)) {
+                for (int t = 0; t < 8; t++) keep[T.child0[i] + t] = 0;
+                leaf[i] = 1;
+            }
+        }
+        // a removed node's descendants go with it
+        for (int i : order)
+            if (T.parent[i] >= 0 && !keep[T.parent[i]]) keep[i] = 0;
+        Tree C;
+        C.mra = T.mra;
+        C.make_roots();
+        for (int i : order) {
+            if (!keep[i]) continue;
+            const NodeKey &k = T.keys[i];
+            int j = C.find(k.depth, k.l);
+            if (j < 0) {
+                long long lp[3] = {k.l[0] >> 1, k.l[1] >> 1, k.l[2] >> 1};
+                const int par = C.find(k.depth - 1, lp);
+                if (par < 0) return fail(MWG_ERR_RUNTIME, "crop: parent of a kept node missing");
+                C.split(par);
+                j = C.find(k.depth, k.l);
+            }
+            std::copy(T.node(i), T.node(i) + blk, C.node(j));
+        }
+        C.sqnorm = tree_square_norm(C);
+        T = std::move(C);
+        return (int)MWG_OK;
     });
 }
 
