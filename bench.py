@@ -160,8 +160,10 @@ def main():
     t_adapt = time.time() - t0
     ast = ctx.last_stats()
     n_out = sum(o.size() for o in outs)
-    log("adaptive apply: %d output nodes in %.2fs (kernel %.1f ms, plan %.1f ms, %d launches)" %
-        (n_out, t_adapt, ast["kernel_ms"], ast["plan_ms"], ast["launches"]))
+    log("adaptive apply: %d output nodes in %.2fs (kernel %.1f ms, plan %.1f ms, %d launches; device plan "
+        "create/run/download %.1f ms, split checks %.1f ms, post transforms %.1f ms)" %
+        (n_out, t_adapt, ast["kernel_ms"], ast["plan_ms"], ast["launches"], ast["device_ms"], ast["split_ms"],
+         ast["post_ms"]))
     thr = [o.thresholds() for o in outs]
     grids = [o.nodes()[:2] for o in outs]
     # plans on the same frozen grids and thresholds: plain FP64 and the
@@ -273,7 +275,9 @@ def main():
         "term_apps": apps,
         "lp_term_apps": hr["lp_term_apps"],
         "check": check,
-        "setup": dict(setup, adaptive_apply_s=t_adapt, plan_s=t_plan),
+        "setup": dict(setup, adaptive_apply_s=t_adapt, plan_s=t_plan,
+                      adaptive_apply_ms={k: ast[k] for k in ("kernel_ms", "plan_ms", "device_ms", "split_ms",
+                                                             "post_ms", "wall_ms")}),
     }
     if cpu is not None:
         line["cpu_baseline"] = cpu

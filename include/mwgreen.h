@@ -189,7 +189,10 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
 /* Statistics of the last apply on this context: [0] kernel milliseconds,
  * [1] output nodes computed, [2] input slots, [3] generated slots,
  * [4] host plan milliseconds, [5] kernel launches, [6] term-applications
- * that passed screening (each 6(k+1)^4 flops of algorithmic work). */
+ * that passed screening (each 6(k+1)^4 flops of algorithmic work),
+ * wall milliseconds of [7] device plan creation, runs and downloads,
+ * [8] the host split checks, [9] the post transforms (TopDown, BottomUp,
+ * norms), [10] the whole apply. */
 int mwg_last_apply_stats(mwg_context ctx, double *stats, int n);
 
 /* Frozen-grid apply split into a resident device plan (bands, input slots,

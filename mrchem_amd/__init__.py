@@ -140,10 +140,11 @@ class Context:
         self.mode, self.tau = mode, float(tau)
 
     def last_stats(self):
-        s = np.zeros(8)
-        _check(lib.mwg_last_apply_stats(self._h, _ptr(s, _d), 8))
+        s = np.zeros(11)
+        _check(lib.mwg_last_apply_stats(self._h, _ptr(s, _d), 11))
         return {"kernel_ms": s[0], "out_nodes": int(s[1]), "in_slots": int(s[2]), "generated": int(s[3]),
-                "plan_ms": s[4], "launches": int(s[5]), "term_apps": int(s[6])}
+                "plan_ms": s[4], "launches": int(s[5]), "term_apps": int(s[6]), "device_ms": s[7],
+                "split_ms": s[8], "post_ms": s[9], "wall_ms": s[10]}
 
     def __del__(self):
         h = getattr(self, "_h", None)

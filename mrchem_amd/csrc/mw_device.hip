@@ -1810,4 +1810,19 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
     return 0;
 }
 
+int dev_plan_download_range(DevicePlan *p, long long first, long long count, double *dst, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    if (first < 0 || count < 0 || first + count > p->n_out) {
+        err = "output node range outside the plan";
+        return -1;
+    }
+    if (count == 0) return 0;
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t blk = (size_t)8 * p->kp1 * p->kp1 * p->kp1;
+    HIP_TRY(hipMemcpyAsync(dst, p->args.out_ns + (size_t)first * blk, sizeof(double) * (size_t)count * blk,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 }  // namespace mwg

@@ -42,6 +42,8 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
 // Copy the raw non-standard blocks of every part, in part order
 // (sum of n_out x 8(k+1)^3), to the host.
 int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err);
+// Output blocks [first, first + count) of the plan's concatenated output nodes.
+int dev_plan_download_range(DevicePlan *p, long long first, long long count, double *dst, std::string &err);
 void dev_plan_destroy(DevicePlan *p);
 
 // Device construction of operator tables (mw_opbuild.hip).  Each job holds an

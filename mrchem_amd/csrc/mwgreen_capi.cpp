@@ -190,6 +190,8 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         j.thr_all.assign(o.size(), 0.0);
     }
     double kms = 0.0, pms = 0.0, slots = 0.0, gen = 0.0, computed = 0.0, apps_total = 0.0;
+    double t_dev = 0.0, t_split = 0.0;  // device plan create + run + download; host split checks
+    const double t_all = now_ms();
     int launches = 0;
     std::vector<double> ns;
     while (true) {
@@ -221,57 +223,116 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         }
         std::string err;
         DevicePlan *dp = nullptr;
+        const double t1 = now_ms();
         int rc = dev_plan_create(ctx->dev, parts, &dp, err);
         if (rc) return dev_fail(rc, err);
         double ms = 0.0;
         unsigned long long apps = 0;
-        ns.resize(total * (size_t)blk);
         rc = dev_plan_run(dp, ctx->mode, ctx->tau, &ms, &apps, nullptr, err, nullptr);
-        if (!rc) rc = dev_plan_download(dp, ns.data(), err);
+        // each apply's blocks straight into its tree: a level's nodes are
+        // consecutive in the tree (the roots, or the children appended by
+        // the previous level's splits), else through a staging buffer
+        std::vector<char> direct(active.size(), 0);
+        {
+            long long first = 0;
+            for (size_t a = 0; a < active.size() && !rc; a++) {
+                Job &j = *active[a];
+                const long long nw = (long long)j.work.size();
+                bool contig = true;
+                for (long long q = 0; q < nw && contig; q++) contig = j.work[q] == j.work[0] + q;
+                if (contig) {
+                    rc = dev_plan_download_range(dp, first, nw, j.out->tree.node(j.work[0]), err);
+                    direct[a] = 1;
+                }
+                first += nw;
+            }
+            bool staged = false;
+            for (char d : direct) staged = staged || !d;
+            if (!rc && staged) {
+                ns.resize(total * (size_t)blk);
+                rc = dev_plan_download(dp, ns.data(), err);
+            }
+        }
         dev_plan_destroy(dp);
         if (rc) return dev_fail(rc, err);
+        t_dev += now_ms() - t1;
+        const double t2 = now_ms();
         kms += ms;
         apps_total += (double)apps;
         launches++;
         computed += (double)total;
-        size_t off = 0;
-        for (size_t a = 0; a < active.size(); a++) {
-            Job &j = *active[a];
+        // Blocks into the trees, norms and split decisions: in parallel over
+        // the applies, or over the nodes of a single apply.  The per-node
+        // squared norms are summed serially in node order, so thresholds and
+        // decisions do not depend on the thread count.
+        std::vector<size_t> offs(active.size());
+        for (size_t a = 0, off = 0; a < active.size(); a++) {
+            offs[a] = off;
+            off += active[a]->work.size();
+        }
+        const int na = (int)active.size();
+#pragma omp parallel for schedule(dynamic, 1) if (na > 1)
+        for (int a = 0; a < na; a++) {
+          Job &j = *active[a];
+          try {
             j.plan = ApplyPlan();
             Tree &o = j.out->tree;
             const int max_depth = std::min(mra.max_depth, j.op->op.max_depth);
-            std::vector<double> wn(j.work.size());
-            for (size_t q = 0; q < j.work.size(); q++) {
+            const int nw = (int)j.work.size();
+            std::vector<double> wsq(nw), ssq(nw);
+#pragma omp parallel for schedule(static) if (na == 1)
+            for (int q = 0; q < nw; q++) {
                 const int i = j.work[q];
-                const double *b = ns.data() + (off + q) * blk;
-                std::memcpy(o.node(i), b, sizeof(double) * blk);
+                if (!direct[a]) std::memcpy(o.node(i), ns.data() + (offs[a] + q) * blk, sizeof(double) * blk);
+                const double *b = o.node(i);
                 j.thr_all[i] = thr[a][q];
                 double w = 0.0, sq = 0.0;
                 for (int e = 0; e < n3; e++) sq += b[e] * b[e];
                 for (int e = n3; e < blk; e++) w += b[e] * b[e];
-                wn[q] = std::sqrt(w);
-                j.w_norm += w;
-                if (o.keys[i].depth == 0) j.s_norm += sq;
+                wsq[q] = w;
+                ssq[q] = sq;
             }
-            off += j.work.size();
+            for (int q = 0; q < nw; q++) {
+                j.w_norm += wsq[q];
+                if (o.keys[j.work[q]].depth == 0) j.s_norm += ssq[q];
+            }
             const double tnorm = j.abs_prec ? 1.0 : std::sqrt(j.s_norm + j.w_norm);
-            std::vector<int> next;
+            std::vector<int> splits;
             if (j.max_iter < 0 || j.iter < j.max_iter) {
-                for (size_t q = 0; q < j.work.size(); q++) {
+                for (int q = 0; q < nw; q++) {
                     const int i = j.work[q];
-                    const NodeKey k = o.keys[i];
+                    const NodeKey &k = o.keys[i];
                     if (k.depth >= max_depth) continue;
                     const double pf = prec_factor(j.prec_trees, k);
-                    if (!split_check(mra, k.depth, wn[q], j.prec * pf, tnorm)) continue;
-                    const int c0 = o.split(i);
-                    for (int c = 0; c < 8; c++) next.push_back(c0 + c);
+                    if (split_check(mra, k.depth, std::sqrt(wsq[q]), j.prec * pf, tnorm)) splits.push_back(i);
                 }
+            }
+            // at most one reallocation of the tree per level, growing
+            // geometrically (an exact reserve would copy the tree every level)
+            const size_t grown = (size_t)o.size() + 8 * splits.size();
+            if (grown * (size_t)blk > o.coefs.capacity())
+                o.coefs.reserve(std::max(grown * (size_t)blk, 2 * o.coefs.capacity()));
+            o.keys.reserve(grown);
+            o.parent.reserve(grown);
+            o.child0.reserve(grown);
+            std::vector<int> next;
+            next.reserve(8 * splits.size());
+            for (int i : splits) {
+                const int c0 = o.split(i);
+                for (int c = 0; c < 8; c++) next.push_back(c0 + c);
             }
             j.work.swap(next);
             j.thr_all.resize(o.size(), 0.0);
             j.iter++;
+          } catch (const std::exception &e) {
+            j.err = e.what();
+          }
         }
+        for (Job *j : active)
+            if (!j->err.empty()) throw std::runtime_error(j->err);
+        t_split += now_ms() - t2;
     }
+    const double t3 = now_ms();
     const int nj = (int)jobs.size();
 #pragma omp parallel for schedule(dynamic, 1) if (nj > 1)
     for (int i = 0; i < nj; i++) {
@@ -279,7 +340,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         jobs[i].out->apply_thr = jobs[i].thr_all;
     }
     auto &st = ctx->last_stats;
-    st.assign(8, 0.0);
+    st.assign(12, 0.0);
     st[0] = kms;
     st[1] = computed;
     st[2] = slots;
@@ -287,6 +348,10 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
     st[4] = pms;
     st[5] = launches;
     st[6] = apps_total;
+    st[7] = t_dev;
+    st[8] = t_split;
+    st[9] = now_ms() - t3;
+    st[10] = now_ms() - t_all;
     return MWG_OK;
 }
 
