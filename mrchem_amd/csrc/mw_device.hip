@@ -1270,8 +1270,8 @@ static int plan_alloc(DevicePlan *p, size_t n, T **out, std::string &err) {
     return 0;
 }
 
-template <typename T>
-static int upload_at(T *dst, const std::vector<T> &v, hipStream_t s, std::string &err) {
+template <typename T, typename A>
+static int upload_at(T *dst, const std::vector<T, A> &v, hipStream_t s, std::string &err) {
     if (!v.empty()) HIP_TRY(hipMemcpyAsync(dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
     return 0;
 }

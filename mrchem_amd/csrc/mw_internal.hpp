@@ -11,6 +11,9 @@
 #include <cstdint>
 #include <string>
 #include <unordered_map>
+#include <memory>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace mwg {
@@ -179,6 +182,28 @@ bool split_check(const MRA &mra, int depth, double wnorm, double prec, double tn
 // (one refinement level or a whole fixed grid), the input nodes of their
 // bands (real or generated by reconstruction), the dense band table, and the
 // screening thresholds.  Built on the host; consumed by the HIP kernels.
+// Allocator whose new elements are left uninitialised (default-init), for
+// large buffers written in full right after allocation: the pages are then
+// first touched by the (parallel) writer instead of a serial zero fill.
+template <typename T>
+struct UninitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() noexcept = default;
+    template <typename U>
+    UninitAlloc(const UninitAlloc<U> &) noexcept {}
+    template <typename U>
+    void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new ((void *)p) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U *p, Args &&...args) {
+        ::new ((void *)p) U(std::forward<Args>(args)...);
+    }
+};
+
 struct ApplyPlan {
     int kp1 = 0, ndepth = 0, M = 0;
     // output nodes (in the order of out_idx)
@@ -190,7 +215,7 @@ struct ApplyPlan {
     std::vector<int> band;          // input slot or -1, order dz,dy,dx with dx fastest
     // input slots (real or generated nodes)
     std::vector<long long> slot_off;  // offset of the slot's block in in_coefs
-    std::vector<double> in_coefs;   // pool: full blocks (real) or scaling part (generated)
+    std::vector<double, UninitAlloc<double>> in_coefs;  // pool: full blocks (real) or scaling part (generated)
     std::vector<double> in_norms;   // slots x 8 (wavelet norms 0 for generated slots)
     std::vector<int> in_depth;
     // screening-derived bandwidths: w[(d*M + m)*4 + c]

@@ -171,7 +171,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         plan.slot_off[s] = pool;
         pool += (slot_real[s] >= 0) ? blk : n3;
     }
-    plan.in_coefs.assign(pool, 0.0);
+    plan.in_coefs.resize(pool);  // every element is written below
     plan.in_norms.assign((size_t)ns * 8, 0.0);
     plan.in_depth.resize(ns);
     // Generated nodes (and the generated ancestors they descend from) are
@@ -220,7 +220,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     for (int s = 0; s < ns; s++)
         if (slot_real[s] < 0) slot_gen[s] = need(slot_keys[s]);
     const int ng = (int)gen_keys.size();
-    std::vector<double> gen_s((size_t)ng * n3);
+    std::vector<double, UninitAlloc<double>> gen_s((size_t)ng * n3);  // written depth by depth below
     {
         std::vector<int> order(ng);
         for (int i = 0; i < ng; i++) order[i] = i;

@@ -151,27 +151,56 @@ namespace {
 // H_bit, G_bit are the (bit) column halves of the filter rows [H0 H1], [G0 G1].
 // hi == nullptr drops the wavelet half.
 void child_step(const Basis &b, const double *lo, const double *hi, int dim, int bit, double *out) {
-    const int n = b.kp1, n2 = 2 * n;
+    const int n = b.kp1, n2 = 2 * n, nn = n * n;
     const double *F = b.F.data();
-    const int stride = (dim == 0) ? 1 : (dim == 1 ? n : n * n);
-    double v[32], u[32];
-    for (int a = 0; a < n; a++) {
+    // Each output element sums j = 0..n-1 over lo, then over hi, in this
+    // order in every branch; the z and y branches run it for many elements
+    // at once along contiguous memory (out must not alias lo or hi).
+    if (dim == 2) {
+        for (int i = 0; i < n; i++) {
+            double *o = out + (size_t)i * nn;
+            for (int e = 0; e < nn; e++) o[e] = 0.0;
+            for (int j = 0; j < n; j++) {
+                const double f = F[j * n2 + bit * n + i], *l = lo + (size_t)j * nn;
+                for (int e = 0; e < nn; e++) o[e] += f * l[e];
+            }
+            if (hi)
+                for (int j = 0; j < n; j++) {
+                    const double f = F[(n + j) * n2 + bit * n + i], *h = hi + (size_t)j * nn;
+                    for (int e = 0; e < nn; e++) o[e] += f * h[e];
+                }
+        }
+        return;
+    }
+    if (dim == 1) {
         for (int c = 0; c < n; c++) {
-            int base;
-            if (dim == 0) base = a * n + c * n * n;
-            else if (dim == 1) base = a + c * n * n;
-            else base = a + c * n;
+            const double *L = lo + (size_t)c * nn, *H = hi ? hi + (size_t)c * nn : nullptr;
             for (int i = 0; i < n; i++) {
-                v[i] = lo[base + i * stride];
-                u[i] = hi ? hi[base + i * stride] : 0.0;
+                double *o = out + (size_t)c * nn + i * n;
+                for (int x = 0; x < n; x++) o[x] = 0.0;
+                for (int j = 0; j < n; j++) {
+                    const double f = F[j * n2 + bit * n + i];
+                    for (int x = 0; x < n; x++) o[x] += f * L[j * n + x];
+                }
+                if (H)
+                    for (int j = 0; j < n; j++) {
+                        const double f = F[(n + j) * n2 + bit * n + i];
+                        for (int x = 0; x < n; x++) o[x] += f * H[j * n + x];
+                    }
             }
-            for (int i = 0; i < n; i++) {
-                double s = 0.0;
-                for (int j = 0; j < n; j++) s += F[j * n2 + bit * n + i] * v[j];
-                if (hi)
-                    for (int j = 0; j < n; j++) s += F[(n + j) * n2 + bit * n + i] * u[j];
-                out[base + i * stride] = s;
-            }
+        }
+        return;
+    }
+    // x: lines of n contiguous values
+    for (int r = 0; r < nn; r++) {
+        const double *v = lo + (size_t)r * n, *u = hi ? hi + (size_t)r * n : nullptr;
+        double *o = out + (size_t)r * n;
+        for (int i = 0; i < n; i++) {
+            double s = 0.0;
+            for (int j = 0; j < n; j++) s += F[j * n2 + bit * n + i] * v[j];
+            if (u)
+                for (int j = 0; j < n; j++) s += F[(n + j) * n2 + bit * n + i] * u[j];
+            o[i] = s;
         }
     }
 }
