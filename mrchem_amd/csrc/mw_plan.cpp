@@ -54,7 +54,12 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     const int M = op.size(), ndepth = op.max_depth + 1;
     if (inp.mra != out.mra || op.mra != out.mra) throw std::invalid_argument("apply: trees and operator must share one MRA");
     if (thr.size() != nodes.size()) throw std::invalid_argument("apply: one threshold per output node");
+    // a fresh plan that keeps the previous pool's storage (refinement levels
+    // reuse it instead of faulting in new pages)
+    auto pool_storage = std::move(plan.in_coefs);
     plan = ApplyPlan();
+    pool_storage.clear();
+    plan.in_coefs = std::move(pool_storage);
     plan.kp1 = mra.kp1();
     plan.ndepth = ndepth;
     plan.M = M;

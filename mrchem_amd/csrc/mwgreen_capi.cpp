@@ -275,7 +275,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         for (int a = 0; a < na; a++) {
           Job &j = *active[a];
           try {
-            j.plan = ApplyPlan();
+            // j.plan stays: the next level's build reuses its pool storage
             Tree &o = j.out->tree;
             const int max_depth = std::min(mra.max_depth, j.op->op.max_depth);
             const int nw = (int)j.work.size();
