@@ -150,6 +150,22 @@ int mwg_tree_add(mwg_context ctx, mwg_tree tree, double coef, mwg_tree inp);
  * FunctionTree::crop(prec, splitFac, absPrec) (ExchangePotentialD1.cpp:323,371).
  * Host operation on the tree structure. */
 int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec);
+/* Packed form of a tree: one contiguous little-endian buffer, the wire format
+ * for sending a tree in one message (MRChem sends trees with
+ * mrcpp::send_tree in chunks, src/parallel.cpp:325-336) and the file format
+ * of mwg_tree_save (MRChem writes orbitals with saveOrbital,
+ * src/qmfunctions/orbital_utils.cpp:397-401).  Layout: a 72-byte header
+ * (magic "MWGTREE1", int32 version 1, kp1, basis type, root scale,
+ * corner[3], boxes[3], double scaling factor, int64 node count n, double
+ * squared norm), int32 depth[n] zero-padded to a multiple of 8 bytes, int64
+ * l[n][3], double blocks[n][8 (k+1)^3], nodes parents first.  Unpacking
+ * requires the same MRA (order, basis, root scale, corner, boxes, scaling
+ * factor) and a complete tree. */
+int mwg_tree_pack_size(mwg_tree tree, long long *bytes);
+int mwg_tree_pack(mwg_tree tree, void *buffer, long long bytes);
+int mwg_tree_unpack(mwg_tree tree, const void *buffer, long long bytes);
+int mwg_tree_save(mwg_tree tree, const char *path);
+int mwg_tree_load(mwg_tree tree, const char *path);
 /* Screening threshold of every node used by the last apply into this tree. */
 int mwg_tree_thresholds(mwg_tree tree, double *thr);
 

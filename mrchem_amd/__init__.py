@@ -61,6 +61,11 @@ _sig("mwg_multiply", _p, _d, _p, _d, _p, _p, _i, _i)
 _sig("mwg_add", _p, _d, _p, _i, _dp, ctypes.POINTER(_p), _i, _i)
 _sig("mwg_tree_add", _p, _p, _d, _p)
 _sig("mwg_tree_crop", _p, _d, _d, _i)
+_sig("mwg_tree_pack_size", _p, _llp)
+_sig("mwg_tree_pack", _p, ctypes.c_void_p, ctypes.c_longlong)
+_sig("mwg_tree_unpack", _p, ctypes.c_void_p, ctypes.c_longlong)
+_sig("mwg_tree_save", _p, ctypes.c_char_p)
+_sig("mwg_tree_load", _p, ctypes.c_char_p)
 _sig("mwg_operator_info", _p, _ip, _ip, _ip, _dp, _dp, _dp)
 _sig("mwg_operator_terms", _p, _dp, _dp)
 _sig("mwg_operator_table_sizes", _p, _llp, _llp)
@@ -351,6 +356,28 @@ class FunctionTree:
         node sums run on the device)."""
         ctx = ctx or Context.default()
         _check(lib.mwg_tree_add(ctx._h, self._h, c, inp._h))
+        return self
+
+    def pack(self):
+        """The tree's packed form (bytes): the wire and file format."""
+        n = ctypes.c_longlong(0)
+        _check(lib.mwg_tree_pack_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib.mwg_tree_pack(self._h, buf, n.value))
+        return buf.raw
+
+    def unpack(self, data):
+        """Replace the tree by a packed one written on the same MRA."""
+        b = bytes(data)
+        _check(lib.mwg_tree_unpack(self._h, b, len(b)))
+        return self
+
+    def save(self, path):
+        _check(lib.mwg_tree_save(self._h, os.fsencode(path)))
+        return self
+
+    def load(self, path):
+        _check(lib.mwg_tree_load(self._h, os.fsencode(path)))
         return self
 
     def crop(self, prec, split_fac=1.0, abs_prec=True):

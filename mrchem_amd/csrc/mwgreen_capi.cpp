@@ -9,6 +9,7 @@
 // computed by one HIP launch, and the result is finished by the TopDown(add)
 // and BottomUp transforms.
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1078,6 +1079,146 @@ int mwg_tree_set(mwg_tree t, int n, const int *depth, const long long *l, const 
         t->tree.mra = &mra;
         t->apply_thr.clear();
         return MWG_OK;
+    });
+}
+
+// Packed tree (wire and file format), little-endian:
+//   header (72 bytes): magic "MWGTREE1", int32 version (1), kp1, basis type,
+//     root scale, corner[3], boxes[3], double scaling factor, int64 node
+//     count n, double squared norm;
+//   int32 depth[n], zero padding to a multiple of 8 bytes;
+//   int64 l[n][3];
+//   double coefs[n][8 (k+1)^3] (compressed blocks);
+// nodes in tree order (every parent before its children).
+namespace {
+struct PackedHeader {
+    char magic[8];
+    int32_t version, kp1, basis, n0;
+    int32_t corner[3], boxes[3];
+    double sfac;
+    int64_t n_nodes;
+    double sqnorm;
+};
+static_assert(sizeof(PackedHeader) == 72, "packed header layout");
+constexpr char kPackMagic[8] = {'M', 'W', 'G', 'T', 'R', 'E', 'E', '1'};
+
+size_t pad8(size_t b) { return (b + 7) / 8 * 8; }
+size_t packed_bytes(long long n, int blk) {
+    return sizeof(PackedHeader) + pad8(sizeof(int32_t) * (size_t)n) + sizeof(int64_t) * 3 * (size_t)n +
+           sizeof(double) * (size_t)n * blk;
+}
+}  // namespace
+
+int mwg_tree_pack_size(mwg_tree t, long long *bytes) {
+    if (!t || !bytes) return fail(MWG_ERR_ARG, "null pointer argument");
+    *bytes = (long long)packed_bytes(t->tree.size(), t->tree.mra->block());
+    return MWG_OK;
+}
+
+int mwg_tree_pack(mwg_tree t, void *buffer, long long bytes) {
+    return guarded([&] {
+        if (!t || !buffer) return fail(MWG_ERR_ARG, "null pointer argument");
+        const Tree &T = t->tree;
+        const MRA &mra = *T.mra;
+        const long long n = T.size();
+        if (bytes < 0 || (size_t)bytes < packed_bytes(n, mra.block())) return fail(MWG_ERR_ARG, "buffer too small");
+        PackedHeader h{};
+        std::memcpy(h.magic, kPackMagic, 8);
+        h.version = 1;
+        h.kp1 = mra.kp1();
+        h.basis = mra.basis.type;
+        h.n0 = mra.n0;
+        for (int d = 0; d < 3; d++) {
+            h.corner[d] = mra.corner[d];
+            h.boxes[d] = mra.boxes[d];
+        }
+        h.sfac = mra.sfac;
+        h.n_nodes = n;
+        h.sqnorm = T.sqnorm;
+        char *p = static_cast<char *>(buffer);
+        std::memcpy(p, &h, sizeof(h));
+        p += sizeof(h);
+        std::vector<int32_t> dep(n);
+        std::vector<int64_t> l(3 * n);
+        for (long long i = 0; i < n; i++) {
+            dep[i] = T.keys[i].depth;
+            for (int d = 0; d < 3; d++) l[3 * i + d] = T.keys[i].l[d];
+        }
+        const size_t db = sizeof(int32_t) * (size_t)n;
+        std::memcpy(p, dep.data(), db);
+        std::memset(p + db, 0, pad8(db) - db);
+        p += pad8(db);
+        std::memcpy(p, l.data(), sizeof(int64_t) * 3 * (size_t)n);
+        p += sizeof(int64_t) * 3 * (size_t)n;
+        std::memcpy(p, T.coefs.data(), sizeof(double) * T.coefs.size());
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_tree_unpack(mwg_tree t, const void *buffer, long long bytes) {
+    return guarded([&] {
+        if (!t || !buffer) return fail(MWG_ERR_ARG, "null pointer argument");
+        const MRA &mra = *t->tree.mra;
+        if (bytes < (long long)sizeof(PackedHeader)) return fail(MWG_ERR_ARG, "packed tree: truncated header");
+        PackedHeader h;
+        std::memcpy(&h, buffer, sizeof(h));
+        if (std::memcmp(h.magic, kPackMagic, 8) != 0) return fail(MWG_ERR_ARG, "packed tree: bad magic");
+        if (h.version != 1) return fail(MWG_ERR_ARG, "packed tree: unknown version");
+        bool same = h.kp1 == mra.kp1() && h.basis == mra.basis.type && h.n0 == mra.n0 &&
+                    std::memcmp(&h.sfac, &mra.sfac, sizeof(double)) == 0;
+        for (int d = 0; d < 3; d++) same = same && h.corner[d] == mra.corner[d] && h.boxes[d] == mra.boxes[d];
+        if (!same) return fail(MWG_ERR_ARG, "packed tree: written on a different MRA");
+        if (h.n_nodes < 1 || h.n_nodes >= (1LL << 31)) return fail(MWG_ERR_ARG, "packed tree: bad node count");
+        const long long n = h.n_nodes;
+        if ((size_t)bytes != packed_bytes(n, mra.block())) return fail(MWG_ERR_ARG, "packed tree: size mismatch");
+        const char *p = static_cast<const char *>(buffer) + sizeof(h);
+        std::vector<int> dep(n);
+        std::vector<int32_t> d32(n);
+        std::memcpy(d32.data(), p, sizeof(int32_t) * (size_t)n);
+        for (long long i = 0; i < n; i++) dep[i] = d32[i];
+        p += pad8(sizeof(int32_t) * (size_t)n);
+        std::vector<long long> l(3 * n);
+        std::vector<int64_t> l64(3 * n);
+        std::memcpy(l64.data(), p, sizeof(int64_t) * 3 * (size_t)n);
+        for (long long i = 0; i < 3 * n; i++) l[i] = l64[i];
+        p += sizeof(int64_t) * 3 * (size_t)n;
+        std::vector<double> c((size_t)n * mra.block());
+        std::memcpy(c.data(), p, sizeof(double) * c.size());
+        return mwg_tree_set(t, (int)n, dep.data(), l.data(), c.data());
+    });
+}
+
+int mwg_tree_save(mwg_tree t, const char *path) {
+    return guarded([&] {
+        if (!t || !path) return fail(MWG_ERR_ARG, "null pointer argument");
+        long long nb = 0;
+        int rc = mwg_tree_pack_size(t, &nb);
+        if (rc) return rc;
+        std::vector<char> buf((size_t)nb);
+        rc = mwg_tree_pack(t, buf.data(), nb);
+        if (rc) return rc;
+        FILE *f = std::fopen(path, "wb");
+        if (!f) return fail(MWG_ERR_RUNTIME, std::string("cannot open ") + path + " for writing");
+        const size_t w = std::fwrite(buf.data(), 1, buf.size(), f);
+        const int cl = std::fclose(f);
+        if (w != buf.size() || cl != 0) return fail(MWG_ERR_RUNTIME, std::string("write error on ") + path);
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_tree_load(mwg_tree t, const char *path) {
+    return guarded([&] {
+        if (!t || !path) return fail(MWG_ERR_ARG, "null pointer argument");
+        FILE *f = std::fopen(path, "rb");
+        if (!f) return fail(MWG_ERR_RUNTIME, std::string("cannot open ") + path);
+        std::vector<char> buf;
+        char tmp[1 << 16];
+        size_t r;
+        while ((r = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + r);
+        const bool err = std::ferror(f) != 0;
+        std::fclose(f);
+        if (err) return fail(MWG_ERR_RUNTIME, std::string("read error on ") + path);
+        return mwg_tree_unpack(t, buf.data(), (long long)buf.size());
     });
 }
 
