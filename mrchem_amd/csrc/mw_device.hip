@@ -1810,6 +1810,14 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
     return 0;
 }
 
+double *dev_plan_take_output(DevicePlan *p) {
+    double *o = p->args.out_ns;
+    for (void *&b : p->bufs)
+        if (b == o) b = nullptr;
+    p->args.out_ns = nullptr;
+    return o;
+}
+
 int dev_plan_download_range(DevicePlan *p, long long first, long long count, double *dst, std::string &err) {
     DeviceCtx *c = p->ctx;
     if (first < 0 || count < 0 || first + count > p->n_out) {

@@ -42,6 +42,9 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
 // Copy the raw non-standard blocks of every part, in part order
 // (sum of n_out x 8(k+1)^3), to the host.
 int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err);
+// The plan's output blocks (all applies' output nodes, 8(k+1)^3 each),
+// detached from the plan: the caller frees them with dev_free.
+double *dev_plan_take_output(DevicePlan *p);
 // Output blocks [first, first + count) of the plan's concatenated output nodes.
 int dev_plan_download_range(DevicePlan *p, long long first, long long count, double *dst, std::string &err);
 void dev_plan_destroy(DevicePlan *p);
@@ -79,6 +82,24 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool_
 int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,
                   std::string &err);
 void dev_mul_end(DevMulSession *m);
+
+// Node blocks resident on the device (mw_multiply.hip).  Per node of
+// blocks[0, n): its scaling part's and wavelet parts' squared norms.
+int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host_s2, double *host_w2,
+                   std::string &err);
+// A branch node's block and its 8 children's blocks (consecutive).
+struct PostPair {
+    double *node, *child0;
+};
+// Post transforms of an NS output tree whose blocks are on the device, the
+// branch nodes grouped by depth (levels[d]): TopDown (each branch's
+// reconstruction added into its children's scaling parts, shallow to deep),
+// then BottomUp (each branch's block recomputed from its children's scaling
+// parts, deep to shallow).
+int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
+                       std::string &err);
+int dev_download(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err);
+void dev_free(DeviceCtx *c, void *p);
 
 // Node-wise sums (mw_multiply.hip): for one output node and one input, the
 // input's source node (its deepest node on the path: block offset in the

@@ -235,6 +235,144 @@ __global__ __launch_bounds__(kMulThreads) void k_add_nodes(int n, const AddSrc *
     }
 }
 
+// Squared norms of each node's scaling part and wavelet parts: one wave
+// per node.
+__global__ __launch_bounds__(64) void k_node_norms(const double *blocks, long long n, int n3, double *s2, double *w2) {
+    const long long b = blockIdx.x;
+    if (b >= n) return;
+    const double *x = blocks + b * 8 * n3;
+    double s = 0.0, w = 0.0;
+    for (int e = threadIdx.x; e < n3; e += 64) s += x[e] * x[e];
+    for (int e = n3 + threadIdx.x; e < 8 * n3; e += 64) w += x[e] * x[e];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        w += __shfl_xor(w, o);
+    }
+    if (threadIdx.x == 0) {
+        s2[b] = s;
+        w2[b] = w;
+    }
+}
+
+// TopDown step for one level: each branch's block reconstructed into its 8
+// children's scaling parts, added to theirs.  One workgroup per branch.
+__global__ __launch_bounds__(kMulThreads) void k_topdown(int n, const PostPair *pairs, int n_pairs, const double *F,
+                                                         double *scratch) {
+    const int tid = threadIdx.x, n3 = n * n * n;
+    double *X = scratch + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
+    for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
+        const PostPair pr = pairs[w];
+        pair_pass(n, F, pr.node, Y, 2, true, tid);
+        __syncthreads();
+        pair_pass(n, F, Y, X, 1, true, tid);
+        __syncthreads();
+        pair_pass(n, F, X, Y, 0, true, tid);
+        __syncthreads();
+        for (int e = tid; e < 8 * n3; e += kMulThreads) pr.child0[(size_t)(e / n3) * 8 * n3 + e % n3] += Y[e];
+        __syncthreads();
+    }
+}
+
+// BottomUp step for one level: each branch's [s, d1..d7] from its children's
+// scaling parts.
+__global__ __launch_bounds__(kMulThreads) void k_compress(int n, const PostPair *pairs, int n_pairs, const double *F,
+                                                          double *scratch) {
+    const int tid = threadIdx.x, n3 = n * n * n;
+    double *X = scratch + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
+    for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
+        const PostPair pr = pairs[w];
+        for (int e = tid; e < 8 * n3; e += kMulThreads) X[e] = pr.child0[(size_t)(e / n3) * 8 * n3 + e % n3];
+        __syncthreads();
+        pair_pass(n, F, X, Y, 0, false, tid);
+        __syncthreads();
+        pair_pass(n, F, Y, X, 1, false, tid);
+        __syncthreads();
+        pair_pass(n, F, X, pr.node, 2, false, tid);
+        __syncthreads();
+    }
+}
+
+int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host_s2, double *host_w2,
+                   std::string &err) {
+    if (n <= 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * 2 * (size_t)n));
+    hipLaunchKernelGGL(k_node_norms, dim3((unsigned)n), dim3(64), 0, s, blocks, n, kp1 * kp1 * kp1, d, d + n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host_s2, d, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_w2, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        err = std::string("node norms: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
+                       std::string &err) {
+    size_t most = 0;
+    for (const auto &l : levels) most = std::max(most, l.size());
+    if (most == 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    const int n = mra.kp1(), n3 = n * n * n;
+    const int grid = (int)std::min<size_t>(most, kMulBlocks);
+    double *F = nullptr, *scratch = nullptr;
+    PostPair *pairs = nullptr;
+    hipError_t e = hipMalloc((void **)&F, sizeof(double) * mra.basis.F.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&scratch, sizeof(double) * (size_t)grid * 16 * n3);
+    if (e == hipSuccess) e = hipMalloc((void **)&pairs, sizeof(PostPair) * most);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size(), hipMemcpyHostToDevice, s);
+    // TopDown, shallow to deep, then BottomUp, deep to shallow; each level's
+    // pairs are uploaded before its launch, in stream order, so the buffer
+    // is rewritten only after the previous launch has read it
+    const int nl = (int)levels.size();
+    for (int step = 0; step < 2 * nl && e == hipSuccess; step++) {
+        const bool down = step < nl;
+        const int d = down ? step : 2 * nl - 1 - step;
+        const std::vector<PostPair> &l = levels[d];
+        if (l.empty()) continue;
+        e = hipMemcpyAsync(pairs, l.data(), sizeof(PostPair) * l.size(), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) break;
+        const int g = (int)std::min<size_t>(l.size(), kMulBlocks);
+        if (down)
+            hipLaunchKernelGGL(k_topdown, dim3(g), dim3(kMulThreads), 0, s, n, pairs, (int)l.size(), F, scratch);
+        else
+            hipLaunchKernelGGL(k_compress, dim3(g), dim3(kMulThreads), 0, s, n, pairs, (int)l.size(), F, scratch);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *p : {(void *)F, (void *)scratch, (void *)pairs})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) {
+        err = std::string("post transforms: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+int dev_download(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err) {
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+void dev_free(DeviceCtx *c, void *p) {
+    if (!p) return;
+    (void)hipSetDevice(dev_device(c));
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(dev_stream(c)));
+    (void)hipFree(p);
+}
+
 struct DevMulSession {
     DeviceCtx *ctx = nullptr;
     int n = 0;

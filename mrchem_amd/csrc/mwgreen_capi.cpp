@@ -123,6 +123,13 @@ void finish_output(Tree &out) {
     out.sqnorm = tree_square_norm(out);
 }
 
+// Where a node's block is on the device: a level's output buffer and the
+// position in it.
+struct NodeLoc {
+    int level = -1;
+    long long pos = 0;
+};
+
 // One apply of a batch and its refinement state.
 struct Job {
     mwg_tree_s *out = nullptr;
@@ -181,7 +188,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
     if (jobs.empty()) return MWG_OK;
     validate_jobs(jobs);
     const MRA &mra = *jobs[0].out->tree.mra;
-    const int n3 = mra.kp1_3(), blk = mra.block();
+    const int blk = mra.block();
     for (Job &j : jobs) {
         Tree &o = j.out->tree;
         o.clear();
@@ -194,7 +201,17 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
     double t_dev = 0.0, t_split = 0.0;  // device plan create + run + download; host split checks
     const double t_all = now_ms();
     int launches = 0;
-    std::vector<double> ns;
+    // Every level's output blocks stay on the device until the post
+    // transforms; each node records where its block is.
+    struct LevelBufs {
+        DeviceCtx *c;
+        std::vector<double *> bufs;
+        std::vector<long long> count;
+        ~LevelBufs() {
+            for (double *p : bufs) dev_free(c, p);
+        }
+    } keep{ctx->dev, {}, {}};
+    std::vector<std::vector<NodeLoc>> locs(jobs.size());
     while (true) {
         std::vector<Job *> active;
         for (Job &j : jobs)
@@ -230,31 +247,15 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         double ms = 0.0;
         unsigned long long apps = 0;
         rc = dev_plan_run(dp, ctx->mode, ctx->tau, &ms, &apps, nullptr, err, nullptr);
-        // each apply's blocks straight into its tree: a level's nodes are
-        // consecutive in the tree (the roots, or the children appended by
-        // the previous level's splits), else through a staging buffer
-        std::vector<char> direct(active.size(), 0);
-        {
-            long long first = 0;
-            for (size_t a = 0; a < active.size() && !rc; a++) {
-                Job &j = *active[a];
-                const long long nw = (long long)j.work.size();
-                bool contig = true;
-                for (long long q = 0; q < nw && contig; q++) contig = j.work[q] == j.work[0] + q;
-                if (contig) {
-                    rc = dev_plan_download_range(dp, first, nw, j.out->tree.node(j.work[0]), err);
-                    direct[a] = 1;
-                }
-                first += nw;
-            }
-            bool staged = false;
-            for (char d : direct) staged = staged || !d;
-            if (!rc && staged) {
-                ns.resize(total * (size_t)blk);
-                rc = dev_plan_download(dp, ns.data(), err);
-            }
-        }
+        // the level's blocks stay on the device; the host takes their norms
+        double *lvl = rc ? nullptr : dev_plan_take_output(dp);
         dev_plan_destroy(dp);
+        if (rc) return dev_fail(rc, err);
+        const int level = (int)keep.bufs.size();
+        keep.bufs.push_back(lvl);
+        keep.count.push_back((long long)total);
+        std::vector<double> s2(total), w2(total);
+        rc = dev_node_norms(ctx->dev, mra.kp1(), lvl, (long long)total, s2.data(), w2.data(), err);
         if (rc) return dev_fail(rc, err);
         t_dev += now_ms() - t1;
         const double t2 = now_ms();
@@ -262,10 +263,9 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         apps_total += (double)apps;
         launches++;
         computed += (double)total;
-        // Blocks into the trees, norms and split decisions: in parallel over
-        // the applies, or over the nodes of a single apply.  The per-node
-        // squared norms are summed serially in node order, so thresholds and
-        // decisions do not depend on the thread count.
+        // Norms and split decisions: in parallel over the applies.  The
+        // per-node squared norms are summed serially in node order, so the
+        // thresholds and decisions do not depend on the thread count.
         std::vector<size_t> offs(active.size());
         for (size_t a = 0, off = 0; a < active.size(); a++) {
             offs[a] = off;
@@ -278,24 +278,16 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
           try {
             // j.plan stays: the next level's build reuses its pool storage
             Tree &o = j.out->tree;
+            std::vector<NodeLoc> &loc = locs[&j - jobs.data()];
             const int max_depth = std::min(mra.max_depth, j.op->op.max_depth);
             const int nw = (int)j.work.size();
-            std::vector<double> wsq(nw), ssq(nw);
-#pragma omp parallel for schedule(static) if (na == 1)
+            loc.resize(o.size());
             for (int q = 0; q < nw; q++) {
                 const int i = j.work[q];
-                if (!direct[a]) std::memcpy(o.node(i), ns.data() + (offs[a] + q) * blk, sizeof(double) * blk);
-                const double *b = o.node(i);
+                loc[i] = NodeLoc{level, (long long)(offs[a] + q)};
                 j.thr_all[i] = thr[a][q];
-                double w = 0.0, sq = 0.0;
-                for (int e = 0; e < n3; e++) sq += b[e] * b[e];
-                for (int e = n3; e < blk; e++) w += b[e] * b[e];
-                wsq[q] = w;
-                ssq[q] = sq;
-            }
-            for (int q = 0; q < nw; q++) {
-                j.w_norm += wsq[q];
-                if (o.keys[j.work[q]].depth == 0) j.s_norm += ssq[q];
+                j.w_norm += w2[offs[a] + q];
+                if (o.keys[i].depth == 0) j.s_norm += s2[offs[a] + q];
             }
             const double tnorm = j.abs_prec ? 1.0 : std::sqrt(j.s_norm + j.w_norm);
             std::vector<int> splits;
@@ -305,7 +297,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
                     const NodeKey &k = o.keys[i];
                     if (k.depth >= max_depth) continue;
                     const double pf = prec_factor(j.prec_trees, k);
-                    if (split_check(mra, k.depth, std::sqrt(wsq[q]), j.prec * pf, tnorm)) splits.push_back(i);
+                    if (split_check(mra, k.depth, std::sqrt(w2[offs[a] + q]), j.prec * pf, tnorm)) splits.push_back(i);
                 }
             }
             // at most one reallocation of the tree per level, growing
@@ -334,11 +326,56 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         t_split += now_ms() - t2;
     }
     const double t3 = now_ms();
-    const int nj = (int)jobs.size();
-#pragma omp parallel for schedule(dynamic, 1) if (nj > 1)
-    for (int i = 0; i < nj; i++) {
-        finish_output(jobs[i].out->tree);
-        jobs[i].out->apply_thr = jobs[i].thr_all;
+    {
+        // Post transforms on the device over all levels' blocks: the 8
+        // children of a branch are consecutive in the tree and were computed
+        // consecutively in one level, so their blocks are adjacent.
+        std::string err;
+        int maxd = 0;
+        for (Job &j : jobs) maxd = std::max(maxd, j.out->tree.max_depth());
+        std::vector<std::vector<PostPair>> levels(maxd + 1);
+        auto addr = [&](const NodeLoc &l) { return keep.bufs[l.level] + (size_t)l.pos * blk; };
+        for (size_t ji = 0; ji < jobs.size(); ji++) {
+            const Tree &o = jobs[ji].out->tree;
+            const std::vector<NodeLoc> &loc = locs[ji];
+            for (int i = 0; i < o.size(); i++) {
+                const int c0 = o.child0[i];
+                if (c0 < 0) continue;
+                for (int c = 1; c < 8; c++)
+                    if (loc[c0 + c].level != loc[c0].level || loc[c0 + c].pos != loc[c0].pos + c)
+                        throw std::runtime_error("apply: a node's children were not computed together");
+                levels[o.keys[i].depth].push_back(PostPair{addr(loc[i]), addr(loc[c0])});
+            }
+        }
+        int rc = dev_post_transform(ctx->dev, mra, levels, err);
+        // norms of the transformed blocks, level by level
+        std::vector<std::vector<double>> s2(keep.bufs.size()), w2(keep.bufs.size());
+        for (size_t l = 0; l < keep.bufs.size() && !rc; l++) {
+            s2[l].resize(keep.count[l]);
+            w2[l].resize(keep.count[l]);
+            rc = dev_node_norms(ctx->dev, mra.kp1(), keep.bufs[l], keep.count[l], s2[l].data(), w2[l].data(), err);
+        }
+        // the blocks into the trees, in runs of nodes adjacent in both
+        for (size_t ji = 0; ji < jobs.size() && !rc; ji++) {
+            Tree &o = jobs[ji].out->tree;
+            const std::vector<NodeLoc> &loc = locs[ji];
+            int i = 0;
+            while (i < o.size() && !rc) {
+                int e = i + 1;
+                while (e < o.size() && loc[e].level == loc[i].level && loc[e].pos == loc[i].pos + (e - i)) e++;
+                rc = dev_download(ctx->dev, addr(loc[i]), (size_t)(e - i) * blk, o.node(i), err);
+                i = e;
+            }
+            double sq = 0.0;
+            for (int n = 0; n < o.size(); n++) {
+                const NodeLoc &l = loc[n];
+                sq += w2[l.level][l.pos];
+                if (o.parent[n] < 0) sq += s2[l.level][l.pos];
+            }
+            o.sqnorm = sq;
+            jobs[ji].out->apply_thr = jobs[ji].thr_all;
+        }
+        if (rc) return dev_fail(rc, err);
     }
     auto &st = ctx->last_stats;
     st.assign(12, 0.0);
