@@ -1810,6 +1810,23 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
     return 0;
 }
 
+int dev_plan_clone_output(DevicePlan *p, double **out, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t n = (size_t)std::max(p->n_out, 1) * 8 * p->kp1 * p->kp1 * p->kp1;
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * n));
+    hipError_t e = hipMemcpyAsync(d, p->args.out_ns, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        err = std::string("output copy: ") + hipGetErrorString(e);
+        return -2;
+    }
+    *out = d;
+    return 0;
+}
+
 double *dev_plan_take_output(DevicePlan *p) {
     double *o = p->args.out_ns;
     for (void *&b : p->bufs)

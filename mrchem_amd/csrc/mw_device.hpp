@@ -45,6 +45,8 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err);
 // The plan's output blocks (all applies' output nodes, 8(k+1)^3 each),
 // detached from the plan: the caller frees them with dev_free.
 double *dev_plan_take_output(DevicePlan *p);
+// A device copy of the plan's output blocks (freed with dev_free).
+int dev_plan_clone_output(DevicePlan *p, double **out, std::string &err);
 // Output blocks [first, first + count) of the plan's concatenated output nodes.
 int dev_plan_download_range(DevicePlan *p, long long first, long long count, double *dst, std::string &err);
 void dev_plan_destroy(DevicePlan *p);

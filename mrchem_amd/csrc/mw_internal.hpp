@@ -157,8 +157,6 @@ double tree_square_norm(const Tree &t);
 double tree_dot(const Tree &a, const Tree &b);
 // bottom-up: recompute every branch node's [s,d] from its children's s parts
 void tree_compress(Tree &t);
-// top-down: add each branch node's reconstruction into its children's s part
-void tree_topdown_add(Tree &t);
 
 // Gaussian sums: f(r) = sum_i c_i (a_i/pi)^{3/2} exp(-a_i |r - R_i|^2)
 struct GaussSum {

@@ -276,27 +276,6 @@ void tree_compress(Tree &t) {
     }
 }
 
-void tree_topdown_add(Tree &t) {
-    const int n3 = t.mra->kp1_3(), blk = t.mra->block();
-    int maxd = t.max_depth();
-    std::vector<std::vector<int>> by_depth(maxd + 1);
-    for (int i = 0; i < t.size(); i++)
-        if (t.child0[i] >= 0) by_depth[t.keys[i].depth].push_back(i);
-    for (int d = 0; d <= maxd; d++) {
-        auto &v = by_depth[d];
-#pragma omp parallel for schedule(static)
-        for (int q = 0; q < (int)v.size(); q++) {
-            int i = v[q];
-            std::vector<double> out(blk);
-            mw_reconstruct(t.mra->basis, t.node(i), out.data());
-            for (int c = 0; c < 8; c++) {
-                double *dst = t.node(t.child0[i] + c);
-                for (int e = 0; e < n3; e++) dst[e] += out[c * n3 + e];
-            }
-        }
-    }
-}
-
 // ----------------------------------------------------------- Gaussians
 double GaussSum::eval(const double *r) const {
     double s = 0.0;

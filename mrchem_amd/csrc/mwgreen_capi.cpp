@@ -117,12 +117,6 @@ void check_compatible(const Tree &out, const Operator &op, const Tree &inp) {
 
 int dev_fail(int rc, const std::string &err) { return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err); }
 
-void finish_output(Tree &out) {
-    tree_topdown_add(out);
-    tree_compress(out);
-    out.sqnorm = tree_square_norm(out);
-}
-
 // Where a node's block is on the device: a level's output buffer and the
 // position in it.
 struct NodeLoc {
@@ -449,28 +443,54 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
     return MWG_OK;
 }
 
-// Download a plan's result into its output trees and finish them.
+// A plan's result into its output trees, finished: the post transforms run
+// on the device on a copy of the output blocks (the plan keeps its NS blocks
+// for further executions and downloads).
 int plan_finish(mwg_plan plan) {
     for (size_t i = 0; i < plan->outs.size(); i++)
         if (plan->outs[i]->tree.size() != plan->n_out[i])
             return fail(MWG_ERR_ARG, "output tree changed since the plan was made");
-    const int blk = plan->outs[0]->tree.mra->block();
-    std::vector<double> ns((size_t)plan->total_out * blk);
+    const MRA &mra = *plan->outs[0]->tree.mra;
+    const int blk = mra.block();
     std::string err;
-    int rc = dev_plan_download(plan->dev, ns.data(), err);
+    DeviceCtx *dc = plan->ctx->dev;
+    double *tmp = nullptr;
+    int rc = dev_plan_clone_output(plan->dev, &tmp, err);
     if (rc) return dev_fail(rc, err);
-    size_t off = 0;
-    for (size_t i = 0; i < plan->outs.size(); i++) {
-        Tree &o = plan->outs[i]->tree;
-        std::memcpy(o.coefs.data(), ns.data() + off, sizeof(double) * (size_t)plan->n_out[i] * blk);
-        off += (size_t)plan->n_out[i] * blk;
+    struct Free {
+        DeviceCtx *c;
+        double *p;
+        ~Free() { dev_free(c, p); }
+    } hold{dc, tmp};
+    // output node g of apply i sits at (base_i + g) in the plan's order
+    int maxd = 0;
+    for (auto *o : plan->outs) maxd = std::max(maxd, o->tree.max_depth());
+    std::vector<std::vector<PostPair>> levels(maxd + 1);
+    std::vector<size_t> base(plan->outs.size());
+    for (size_t i = 0, off = 0; i < plan->outs.size(); i++) {
+        base[i] = off;
+        const Tree &o = plan->outs[i]->tree;
+        for (int g = 0; g < o.size(); g++)
+            if (o.child0[g] >= 0)
+                levels[o.keys[g].depth].push_back(
+                    PostPair{tmp + (off + g) * (size_t)blk, tmp + (off + o.child0[g]) * (size_t)blk});
+        off += (size_t)plan->n_out[i];
     }
-    const int nj = (int)plan->outs.size();
-#pragma omp parallel for schedule(dynamic, 1) if (nj > 1)
-    for (int i = 0; i < nj; i++) {
-        finish_output(plan->outs[i]->tree);
+    rc = dev_post_transform(dc, mra, levels, err);
+    std::vector<double> s2(plan->total_out), w2(plan->total_out);
+    if (!rc) rc = dev_node_norms(dc, mra.kp1(), tmp, plan->total_out, s2.data(), w2.data(), err);
+    for (size_t i = 0; i < plan->outs.size() && !rc; i++) {
+        Tree &o = plan->outs[i]->tree;
+        rc = dev_download(dc, tmp + base[i] * blk, (size_t)plan->n_out[i] * blk, o.coefs.data(), err);
+        double sq = 0.0;
+        for (int g = 0; g < o.size(); g++) {
+            sq += w2[base[i] + g];
+            if (o.parent[g] < 0) sq += s2[base[i] + g];
+        }
+        o.sqnorm = sq;
         plan->outs[i]->apply_thr = plan->thr[i];
     }
+    if (rc) return dev_fail(rc, err);
     return MWG_OK;
 }
 
