@@ -296,7 +296,8 @@ __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, co
             } else {
                 apps0 += __popcll(mask);
             }
-            if (t0 || t1) emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll[0], ll[1], ll[2], b0, b1, t0, t1);
+            if (t0 || t1)
+                emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll[0], ll[1], ll[2], b0, b1, t0, t1);
             n += __popcll(mask);
         }
         pi += 64;
@@ -656,9 +657,9 @@ constexpr int kZ0Bit = 28, kZ1Bit = 29, kSingleBit = 30;
 constexpr int kSplitCap = 384;  // worklist items per batch (>= 64 + 128)
 
 template <int N>
-__global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_eu(SplitShape<N>::WPE))) void k_apply_h(KArgs a, const _Float16 *pool_h, const double *pool_sc,
-                                                             const int *seg_list, double lo_mult, double hi_mult,
-                                                             double single_mult, int cls_counter) {
+__global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_eu(SplitShape<N>::WPE))) void k_apply_h(
+    KArgs a, const _Float16 *pool_h, const double *pool_sc, const int *seg_list, double lo_mult, double hi_mult,
+    double single_mult, int cls_counter) {
     using S = SplitShape<N>;
     constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTZ = S::NTZ;
     constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, RL = S::RL, RE = S::RE, NW = S::NW;
@@ -888,7 +889,8 @@ __global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_e
                               ((t1 ? 1 : 0) << kZ1Bit);
                     wl[pos] = itm;
                     const double sxy = opsc[itm.bx] * opsc[itm.by];
-                    wsig[pos] = make_float2(t0 ? (float)(sxy * opsc[itm.bz0]) : 0.f, t1 ? (float)(sxy * opsc[itm.bz1]) : 0.f);
+                    wsig[pos] = make_float2(t0 ? (float)(sxy * opsc[itm.bz0]) : 0.f,
+                                            t1 ? (float)(sxy * opsc[itm.bz1]) : 0.f);
                 });
             __syncthreads();
             if (n == 0) continue;

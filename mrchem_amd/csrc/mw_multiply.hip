@@ -536,7 +536,8 @@ int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_
                            m->pool, m->coefs, m->F, m->scratch, dout);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(dsrc);
     (void)hipFree(dout);

@@ -268,52 +268,53 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         const int na = (int)active.size();
 #pragma omp parallel for schedule(dynamic, 1) if (na > 1)
         for (int a = 0; a < na; a++) {
-          Job &j = *active[a];
-          try {
-            // j.plan stays: the next level's build reuses its pool storage
-            Tree &o = j.out->tree;
-            std::vector<NodeLoc> &loc = locs[&j - jobs.data()];
-            const int max_depth = std::min(mra.max_depth, j.op->op.max_depth);
-            const int nw = (int)j.work.size();
-            loc.resize(o.size());
-            for (int q = 0; q < nw; q++) {
-                const int i = j.work[q];
-                loc[i] = NodeLoc{level, (long long)(offs[a] + q)};
-                j.thr_all[i] = thr[a][q];
-                j.w_norm += w2[offs[a] + q];
-                if (o.keys[i].depth == 0) j.s_norm += s2[offs[a] + q];
-            }
-            const double tnorm = j.abs_prec ? 1.0 : std::sqrt(j.s_norm + j.w_norm);
-            std::vector<int> splits;
-            if (j.max_iter < 0 || j.iter < j.max_iter) {
+            Job &j = *active[a];
+            try {
+                // j.plan stays: the next level's build reuses its pool storage
+                Tree &o = j.out->tree;
+                std::vector<NodeLoc> &loc = locs[&j - jobs.data()];
+                const int max_depth = std::min(mra.max_depth, j.op->op.max_depth);
+                const int nw = (int)j.work.size();
+                loc.resize(o.size());
                 for (int q = 0; q < nw; q++) {
                     const int i = j.work[q];
-                    const NodeKey &k = o.keys[i];
-                    if (k.depth >= max_depth) continue;
-                    const double pf = prec_factor(j.prec_trees, k);
-                    if (split_check(mra, k.depth, std::sqrt(w2[offs[a] + q]), j.prec * pf, tnorm)) splits.push_back(i);
+                    loc[i] = NodeLoc{level, (long long)(offs[a] + q)};
+                    j.thr_all[i] = thr[a][q];
+                    j.w_norm += w2[offs[a] + q];
+                    if (o.keys[i].depth == 0) j.s_norm += s2[offs[a] + q];
                 }
+                const double tnorm = j.abs_prec ? 1.0 : std::sqrt(j.s_norm + j.w_norm);
+                std::vector<int> splits;
+                if (j.max_iter < 0 || j.iter < j.max_iter) {
+                    for (int q = 0; q < nw; q++) {
+                        const int i = j.work[q];
+                        const NodeKey &k = o.keys[i];
+                        if (k.depth >= max_depth) continue;
+                        const double pf = prec_factor(j.prec_trees, k);
+                        if (split_check(mra, k.depth, std::sqrt(w2[offs[a] + q]), j.prec * pf, tnorm))
+                            splits.push_back(i);
+                    }
+                }
+                // at most one reallocation of the tree per level, growing
+                // geometrically (an exact reserve would copy the tree every level)
+                const size_t grown = (size_t)o.size() + 8 * splits.size();
+                if (grown * (size_t)blk > o.coefs.capacity())
+                    o.coefs.reserve(std::max(grown * (size_t)blk, 2 * o.coefs.capacity()));
+                o.keys.reserve(grown);
+                o.parent.reserve(grown);
+                o.child0.reserve(grown);
+                std::vector<int> next;
+                next.reserve(8 * splits.size());
+                for (int i : splits) {
+                    const int c0 = o.split(i);
+                    for (int c = 0; c < 8; c++) next.push_back(c0 + c);
+                }
+                j.work.swap(next);
+                j.thr_all.resize(o.size(), 0.0);
+                j.iter++;
+            } catch (const std::exception &e) {
+                j.err = e.what();
             }
-            // at most one reallocation of the tree per level, growing
-            // geometrically (an exact reserve would copy the tree every level)
-            const size_t grown = (size_t)o.size() + 8 * splits.size();
-            if (grown * (size_t)blk > o.coefs.capacity())
-                o.coefs.reserve(std::max(grown * (size_t)blk, 2 * o.coefs.capacity()));
-            o.keys.reserve(grown);
-            o.parent.reserve(grown);
-            o.child0.reserve(grown);
-            std::vector<int> next;
-            next.reserve(8 * splits.size());
-            for (int i : splits) {
-                const int c0 = o.split(i);
-                for (int c = 0; c < 8; c++) next.push_back(c0 + c);
-            }
-            j.work.swap(next);
-            j.thr_all.resize(o.size(), 0.0);
-            j.iter++;
-          } catch (const std::exception &e) {
-            j.err = e.what();
-          }
         }
         for (Job *j : active)
             if (!j->err.empty()) throw std::runtime_error(j->err);
