@@ -131,6 +131,17 @@ def main():
         setup["pair_density_device"] = {"s": min(reps), "out_nodes": rho.size(),
                                         "in_nodes": [phi[0].size(), phi[1].size()], "diff_norm": diff,
                                         "norm": ref.norm()}
+        if not args.no_cpu:
+            # the same node products by the oracle's restatement on the host
+            # cores, at every node of the device's output grid
+            import oracle
+            od, ol, _ = rho.nodes()
+            t0 = time.time()
+            oracle.multiply(mra, phi[0].nodes(), phi[1].nodes(), od, ol)
+            t_cpu = time.time() - t0
+            setup["pair_density_device"]["cpu_oracle_s"] = t_cpu
+            setup["pair_density_device"]["cpu_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            log("pair density on the host (oracle, all %d nodes): %.3fs" % (len(od), t_cpu))
         log("pair density phi_0 phi_1 on the device: %d nodes (inputs %d, %d) in %.3fs, |rho - exact| %.2e "
             "(|exact| %.3e)" % (rho.size(), phi[0].size(), phi[1].size(), min(reps), diff, ref.norm()))
         # linear combination of 16 pair densities on their union grid (the

@@ -705,6 +705,15 @@ bool add_source(const Tree &f, const NodeKey &k, long long base, AddSrc &sr) {
     return true;
 }
 
+// Room for n nodes in t (one allocation instead of repeated growth).
+void reserve_nodes(Tree &t, size_t n) {
+    t.coefs.reserve(n * (size_t)t.mra->block());
+    t.keys.reserve(n);
+    t.parent.reserve(n);
+    t.child0.reserve(n);
+    t.index.reserve(n);
+}
+
 // Nodes of f in order of depth, each after its parent.
 std::vector<int> by_depth(const Tree &f) {
     std::vector<int> order(f.size());
@@ -732,6 +741,8 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tre
         pool.insert(pool.end(), ins[i]->coefs.begin(), ins[i]->coefs.end());
     }
     T.clear();
+    // the union grid holds at most all input nodes
+    if (prec < 0.0) reserve_nodes(T, total / (size_t)blk);
     T.make_roots();
     if (prec < 0.0) {
         for (const Tree *f : ins)
@@ -771,7 +782,8 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tre
             dev_add_end(m);
             return dev_fail(rc, err);
         }
-        for (size_t w = 0; w < work.size(); w++)
+#pragma omp parallel for schedule(static)
+        for (long long w = 0; w < (long long)work.size(); w++)
             std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
         if (prec < 0.0 || (max_iter >= 0 && iter >= max_iter)) break;
         double tnorm = 1.0;
@@ -812,6 +824,8 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
         if (!(prec >= 0.0) || !std::isfinite(coef)) return fail(MWG_ERR_ARG, "bad precision or coefficient");
         Tree &T = out->tree;
         T.clear();
+        // the union grid holds at most all input nodes
+        reserve_nodes(T, (size_t)a->tree.size() + (size_t)b->tree.size());
         T.make_roots();
         // the union of the input grids
         for (const Tree *f : {&a->tree, &b->tree}) {
@@ -854,7 +868,8 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
                 dev_mul_end(m);
                 return dev_fail(rc, err);
             }
-            for (size_t w = 0; w < work.size(); w++)
+#pragma omp parallel for schedule(static)
+            for (long long w = 0; w < (long long)work.size(); w++)
                 std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
             if (max_iter >= 0 && iter >= max_iter) break;
             // norm estimate of the product from the leaves' scaling parts

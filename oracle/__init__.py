@@ -57,6 +57,9 @@ def _lib():
 
 
 def _declare(lib):
+    lib.oracle_multiply.argtypes = [_i, _i, _ip, _ip, _dp, _dp, _i, _ip, _llp, _dp, _i, _ip, _llp, _dp, _i, _ip, _llp,
+                                    _dp, _d, _dp]
+    lib.oracle_multiply.restype = _i
     lib.oracle_set_threads.argtypes = [_i]
     lib.oracle_set_threads.restype = None
     lib.oracle_apply.argtypes = [_i, _i, _ip, _ip, _dp, _i, _i, _ip, _llp, _llp, _dp, _dp, _i, _ip, _llp, _dp, _i, _ip,
@@ -120,6 +123,30 @@ def apply_fixed(mra, tables, in_tree, out_depth, out_l, thr, band_w=-1, want_ns=
     st = {"term_apps": stats[0], "candidates": stats[1], "slots": stats[2], "generated": stats[3],
           "band_w": int(stats[4])}
     return (out, st, ns) if want_ns else (out, st)
+
+
+def multiply(mra, a_tree, b_tree, out_depth, out_l, coef=1.0):
+    """coef * a * b at the given output nodes (interpolating basis): each
+    node's compressed block from the product at its 8 children's points.
+    a_tree, b_tree = (depth, l, coefs) of complete compressed trees."""
+    corner = _c(mra.corner, np.int32)
+    boxes = _c(mra.boxes, np.int32)
+    F = _c(mra.filter(), np.float64)
+    _, w = np.polynomial.legendre.leggauss(mra.kp1)
+    wq = _c(w / 2.0, np.float64)
+    ad, al, ac = (_c(a_tree[0], np.int32), _c(a_tree[1], np.int64), _c(a_tree[2], np.float64))
+    bd, bl, bc = (_c(b_tree[0], np.int32), _c(b_tree[1], np.int64), _c(b_tree[2], np.float64))
+    od = _c(out_depth, np.int32)
+    ol = _c(out_l, np.int64)
+    hfac = _c([mra.box_size(int(d) + 1) ** -1.5 for d in od], np.float64)
+    out = np.zeros((len(od), mra.block))
+    bad = _lib().oracle_multiply(mra.kp1, mra.min_scale, _ptr(corner, _i), _ptr(boxes, _i), _ptr(F, _d), _ptr(wq, _d),
+                                 len(ad), _ptr(ad, _i), _ptr(al, ctypes.c_longlong), _ptr(ac, _d), len(bd),
+                                 _ptr(bd, _i), _ptr(bl, ctypes.c_longlong), _ptr(bc, _d), len(od), _ptr(od, _i),
+                                 _ptr(ol, ctypes.c_longlong), _ptr(hfac, _d), float(coef), _ptr(out, _d))
+    if bad:
+        raise RuntimeError("oracle_multiply: %d output nodes outside the inputs' trees" % bad)
+    return out
 
 
 def apply_adaptive(mra, tables, in_tree, prec, abs_prec=False, max_iter=-1, max_depth=-1):

@@ -578,6 +578,78 @@ void *oracle_apply_adaptive(int kp1, int n0, const int *corner, const int *boxes
     return res;
 }
 
+// Pointwise products in the interpolating basis (mrcpp::multiply, reached
+// from src/qmoperators/two_electron/ExchangePotential.cpp:175 through
+// mrcpp::cplxfunc::multiply) at given output nodes: for each node, both
+// inputs' scaling blocks at its 8 children - from each input's deepest node
+// on the path, reconstruct8 level by level keeping the child's scaling part
+// (wavelet parts zero below that node), then reconstruct8 of the node's own
+// block - the product coef s^a s^b hfac[q] / sqrt(w_i w_j w_k) at the
+// children's points, and compress8 into the node's [s, d1..d7] block.
+// hfac[q] = h^{-3/2} of the node's children (h their side); wq: the kp1
+// Gauss-Legendre weights on [0, 1].
+int oracle_multiply(int kp1, int n0, const int *corner, const int *boxes, const double *F, const double *wq, int na,
+                    const int *a_dep, const long long *a_l, const double *a_c, int nb, const int *b_dep,
+                    const long long *b_l, const double *b_c, int n_out, const int *o_dep, const long long *o_l,
+                    const double *hfac, double coef, double *out) {
+    Geo geo;
+    geo.kp1 = kp1;
+    geo.n3 = kp1 * kp1 * kp1;
+    geo.blk = 8 * geo.n3;
+    geo.n0 = n0;
+    for (int d = 0; d < 3; d++) {
+        geo.corner[d] = corner[d];
+        geo.boxes[d] = boxes[d];
+    }
+    geo.F = F;
+    const int n3 = geo.n3, blk = geo.blk;
+    KeyMap ma, mb;
+    for (int i = 0; i < na; i++) ma[Key{a_dep[i], a_l[3 * i], a_l[3 * i + 1], a_l[3 * i + 2]}] = i;
+    for (int i = 0; i < nb; i++) mb[Key{b_dep[i], b_l[3 * i], b_l[3 * i + 1], b_l[3 * i + 2]}] = i;
+    std::vector<double> wsc(n3);
+    for (int k = 0; k < kp1; k++)
+        for (int j = 0; j < kp1; j++)
+            for (int i = 0; i < kp1; i++) wsc[(k * kp1 + j) * kp1 + i] = 1.0 / std::sqrt(wq[i] * wq[j] * wq[k]);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : bad)
+    for (int q = 0; q < n_out; q++) {
+        const Key k{o_dep[q], o_l[3 * q], o_l[3 * q + 1], o_l[3 * q + 2]};
+        std::vector<double> X[2], B(blk);
+        const KeyMap *maps[2] = {&ma, &mb};
+        const double *cs[2] = {a_c, b_c};
+        for (int f = 0; f < 2; f++) {
+            int d0 = k.d, idx = -1;
+            for (; d0 >= 0; d0--) {
+                const int up = k.d - d0;
+                auto it = maps[f]->find(Key{d0, k.x >> up, k.y >> up, k.z >> up});
+                if (it != maps[f]->end()) {
+                    idx = it->second;
+                    break;
+                }
+            }
+            if (idx < 0) {
+                bad++;
+                X[f].assign(blk, 0.0);
+                continue;
+            }
+            std::copy(cs[f] + (size_t)idx * blk, cs[f] + (size_t)(idx + 1) * blk, B.begin());
+            for (int lv = d0; lv < k.d; lv++) {
+                reconstruct8(geo, B.data());
+                const int up = k.d - lv - 1;
+                const int t = (int)(((k.x >> up) & 1) | (((k.y >> up) & 1) << 1) | (((k.z >> up) & 1) << 2));
+                std::copy(B.begin() + (size_t)t * n3, B.begin() + (size_t)(t + 1) * n3, B.begin());
+                std::fill(B.begin() + n3, B.end(), 0.0);
+            }
+            reconstruct8(geo, B.data());
+            X[f] = B;
+        }
+        double *o = out + (size_t)q * blk;
+        for (int e = 0; e < blk; e++) o[e] = coef * X[0][e] * X[1][e] * hfac[q] * wsc[e % n3];
+        compress8(geo, o);
+    }
+    return bad;
+}
+
 int oracle_result_size(void *h) { return (int)static_cast<AdaptiveResult *>(h)->depth.size(); }
 void oracle_result_get(void *h, int *depth, long long *l, double *thr, double *coefs) {
     auto *r = static_cast<AdaptiveResult *>(h);

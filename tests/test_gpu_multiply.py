@@ -15,6 +15,7 @@ import pytest
 
 import mrchem_amd as mw
 from mrchem_amd import workloads
+from mw_numpy_ref import compress, reconstruct, scaling_at
 
 pytestmark = pytest.mark.gpu
 
@@ -50,63 +51,6 @@ def test_product_of_gaussian_sums(ctx, order, prec):
     assert pv < 100 * prec * max(1.0, scale)
 
 
-# numpy restatement of the node transforms (component t: bit d set = wavelet
-# along d; blocks indexed [z][y][x])
-def _pair_apply(blk, F, n, dim, transpose):
-    out = blk.copy()
-    M = F.T if transpose else F
-    axis = 3 - dim  # block axes: (t, z, y, x)
-    for t in range(8):
-        if (t >> dim) & 1:
-            continue
-        lo, hi = blk[t], blk[t | (1 << dim)]
-        v = np.concatenate([lo, hi], axis=axis - 1)
-        u = np.tensordot(M, np.moveaxis(v, axis - 1, 0), axes=(1, 0))
-        u = np.moveaxis(u, 0, axis - 1)
-        sl = [slice(None)] * 3
-        sl[axis - 1] = slice(0, n)
-        out[t] = u[tuple(sl)]
-        sl[axis - 1] = slice(n, 2 * n)
-        out[t | (1 << dim)] = u[tuple(sl)]
-    return out
-
-
-def _reconstruct(blk, F, n):
-    for dim in (2, 1, 0):
-        blk = _pair_apply(blk, F, n, dim, True)
-    return blk
-
-
-def _compress(blk, F, n):
-    for dim in (0, 1, 2):
-        blk = _pair_apply(blk, F, n, dim, False)
-    return blk
-
-
-def _child_scaling(blk, F, n, t):
-    """Scaling block of child t of a full node block (reconstruct, pick)."""
-    return _reconstruct(blk, F, n)[t]
-
-
-def _scaling_at(tree_nodes, depth, l, F, n):
-    """Scaling block of (depth, l) from the tree's deepest node on its path."""
-    d_arr, l_arr, c_arr = tree_nodes
-    for d in range(depth, -1, -1):
-        lp = np.array(l) >> (depth - d)
-        hit = np.where((d_arr == d) & np.all(l_arr == lp, axis=1))[0]
-        if len(hit) == 0:
-            continue
-        blk = c_arr[hit[0]].reshape(8, n, n, n)
-        for lv in range(d + 1, depth + 1):
-            bits = (np.array(l) >> (depth - lv)) & 1
-            t = int(bits[0] + 2 * bits[1] + 4 * bits[2])
-            s = _child_scaling(blk, F, n, t)
-            blk = np.zeros_like(blk)
-            blk[0] = s
-        return blk
-    raise AssertionError("no source node")
-
-
 def test_node_math_against_numpy(ctx):
     order, prec = 3, 1e-4
     n = order + 1
@@ -131,11 +75,11 @@ def test_node_math_against_numpy(ctx):
     errs, scale = [], 0.0
     for i in leaves[:: max(1, len(leaves) // 64)]:
         depth, l = int(od[i]), ol[i]
-        sa, sb = _scaling_at(an, depth, l, F, n), _scaling_at(bn, depth, l, F, n)
-        ca, cb = _reconstruct(sa, F, n), _reconstruct(sb, F, n)
+        sa, sb = scaling_at(an, depth, l, F, n), scaling_at(bn, depth, l, F, n)
+        ca, cb = reconstruct(sa, F, n), reconstruct(sb, F, n)
         h = mra.box_size(depth + 1)
         p = coef * ca * cb * h ** -1.5 * wsc[None]
-        ref = _compress(p, F, n).ravel()
+        ref = compress(p, F, n).ravel()
         errs.append(np.max(np.abs(oc[i] - ref)))
         scale = max(scale, np.max(np.abs(ref)))
         checked += 1
@@ -145,6 +89,24 @@ def test_node_math_against_numpy(ctx):
           (checked, deps, err))
     assert checked >= 16 and deps >= 1
     assert err < 1e-13
+
+
+def test_device_against_oracle(ctx):
+    """Every leaf of an adaptive device product against the oracle's product
+    restatement at the same nodes (oracle.multiply, pinned on the CPU by the
+    identity f x 1 = f and the numpy restatement)."""
+    import oracle
+    mra = _mra(5)
+    fi, fj = _orbitals(4711, 2, 3, 0.5, 20.0, 2.0)
+    a = mw.FunctionTree(mra).project_gaussians(1e-6, *fi)
+    b = mw.FunctionTree(mra).project_gaussians(1e-6, *fj)
+    out = mw.multiply(1e-5, mw.FunctionTree(mra), a, b, coef=0.7, abs_prec=True, ctx=ctx)
+    od, ol, oc = out.nodes()
+    leaves = np.array([not np.any((od == od[i] + 1) & np.all((ol >> 1) == ol[i], axis=1)) for i in range(len(od))])
+    ref = oracle.multiply(mra, a.nodes(), b.nodes(), od[leaves], ol[leaves], coef=0.7)
+    err = np.max(np.abs(oc[leaves] - ref)) / np.max(np.abs(ref))
+    print("%d leaves of %d nodes: device vs oracle %.2e" % (leaves.sum(), len(od), err))
+    assert leaves.sum() > 100 and err < 1e-13
 
 
 def test_multiply_argument_errors(ctx):
