@@ -1,6 +1,6 @@
 // ORACLE - test infrastructure only.  Nothing in the product links this file;
-// only tests/, the smoke check and bench.py's cpu_baseline leg load it, as the
-// checker of the HIP apply.
+// only tests/, the smoke check and bench.py's CPU baseline legs load it, as
+// the checker of the HIP apply and tree product.
 //
 // A plain FP64 OpenMP restatement of the multiwavelet Green's-function apply,
 // mrcpp::apply(prec, out, ConvolutionOperator&, inp[, precTrees], maxIter,
@@ -27,6 +27,9 @@
 //     by point values of the potential, and by linearity/symmetry invariants
 //     (tests/test_oracle.py);
 //   * node generation: pinned by reconstruction round-trips;
+//   * oracle_multiply (tree product): pinned by the identity f x 1 = f over
+//     many levels, constant scaling and symmetry, and a numpy restatement of
+//     the transforms (tests/test_oracle_multiply.py);
 //   * bit-level parity with upstream MRCPP: parity unpinned (MRCPP absent).
 #include <algorithm>
 #include <cmath>
