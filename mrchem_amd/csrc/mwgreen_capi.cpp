@@ -117,6 +117,15 @@ void check_compatible(const Tree &out, const Operator &op, const Tree &inp) {
 
 int dev_fail(int rc, const std::string &err) { return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err); }
 
+// Room for n nodes in t (one allocation instead of repeated growth).
+void reserve_nodes(Tree &t, size_t n) {
+    t.coefs.reserve(n * (size_t)t.mra->block());
+    t.keys.reserve(n);
+    t.parent.reserve(n);
+    t.child0.reserve(n);
+    t.index.reserve(n);
+}
+
 // Where a node's block is on the device: a level's output buffer and the
 // position in it.
 struct NodeLoc {
@@ -186,6 +195,9 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
     for (Job &j : jobs) {
         Tree &o = j.out->tree;
         o.clear();
+        // room for an output grid up to twice the input's size without
+        // regrowing (untouched pages cost nothing; larger grids still grow)
+        reserve_nodes(o, std::max<size_t>(64, 2 * (size_t)j.inp->size()));
         o.make_roots();
         j.work.clear();
         for (int i = 0; i < o.size(); i++) j.work.push_back(i);
@@ -703,15 +715,6 @@ bool add_source(const Tree &f, const NodeKey &k, long long base, AddSrc &sr) {
         return true;
     }
     return true;
-}
-
-// Room for n nodes in t (one allocation instead of repeated growth).
-void reserve_nodes(Tree &t, size_t n) {
-    t.coefs.reserve(n * (size_t)t.mra->block());
-    t.keys.reserve(n);
-    t.parent.reserve(n);
-    t.child0.reserve(n);
-    t.index.reserve(n);
 }
 
 // Nodes of f in order of depth, each after its parent.
