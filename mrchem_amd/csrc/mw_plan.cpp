@@ -274,16 +274,22 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     const int nq = (int)nodes.size();
     std::vector<std::vector<int>> counts((size_t)nq * 8);
     std::vector<std::vector<double>> bmin((size_t)nq * 8), bmax((size_t)nq * 8);
-    long long napps = 0;
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : napps)
-    for (int q = 0; q < nq; q++) {
-        const int d = plan.out_depth[q], w = plan.band_w[q], side = 2 * w + 1;
-        const int nb = side * side * side;
-        const double th = plan.out_thr[q];
-        const int a0 = plan.active_off[d], a1 = plan.active_off[d + 1];
-        // per term, component and translation: the norm the kernel would use
-        const int nl = 2 * w + 1;
-        std::vector<double> tn((size_t)(a1 - a0) * 4 * nl, 0.0);
+    // Per output depth (band half-width w): the norm the kernel would use per
+    // active term, component and translation (tn), and the largest
+    // ((N_x N_y) N_z) over the terms per component combination cmb = cx |
+    // cy << 2 | cz << 4 and translation (pre); a (band position, ft) pair
+    // whose fl(pre ||f_ft||) does not pass holds no passing term (rounding is
+    // monotone) and is skipped, so the counts are those of the full scan.
+    std::vector<int> wd(plan.active_off.size(), -1);
+    for (int q = 0; q < nq; q++) wd[plan.out_depth[q]] = plan.band_w[q];
+    std::vector<std::vector<double>> tn_d(wd.size()), pre_d(wd.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int d = 0; d < (int)wd.size(); d++) {
+        const int w = wd[d];
+        if (w < 0 || d + 1 >= (int)plan.active_off.size()) continue;
+        const int nl = 2 * w + 1, a0 = plan.active_off[d], a1 = plan.active_off[d + 1];
+        std::vector<double> &tn = tn_d[d];
+        tn.assign((size_t)(a1 - a0) * 4 * nl, 0.0);
         for (int ai = a0; ai < a1; ai++) {
             const int m = plan.active_terms[ai];
             const OperDepth &od = op.terms[m].depth[d];
@@ -294,6 +300,36 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
                         tn[((size_t)(ai - a0) * 4 + c) * nl + (l + w)] = oper_norm(op, m, d, c, l);
             }
         }
+        const size_t nl3 = (size_t)nl * nl * nl;
+        std::vector<double> &pre = pre_d[d];
+        pre.assign(64 * nl3, 0.0);
+        for (int cmb = 0; cmb < 64; cmb++) {
+            const int cx = cmb & 3, cy = (cmb >> 2) & 3, cz = (cmb >> 4) & 3;
+            double *dst = pre.data() + cmb * nl3;
+            for (int ai = 0; ai < a1 - a0; ai++) {
+                const double *tx = &tn[((size_t)ai * 4 + cx) * nl], *ty = &tn[((size_t)ai * 4 + cy) * nl];
+                const double *tz = &tn[((size_t)ai * 4 + cz) * nl];
+                for (int lz = 0; lz < nl; lz++)
+                    for (int ly = 0; ly < nl; ly++)
+                        for (int lx = 0; lx < nl; lx++) {
+                            const double b = (tx[lx] * ty[ly]) * tz[lz];
+                            double &o = dst[((size_t)lz * nl + ly) * nl + lx];
+                            if (b > o) o = b;
+                        }
+            }
+        }
+    }
+    long long napps = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : napps)
+    for (int q = 0; q < nq; q++) {
+        const int d = plan.out_depth[q], w = plan.band_w[q], side = 2 * w + 1;
+        const int nb = side * side * side;
+        const double th = plan.out_thr[q];
+        const int a0 = plan.active_off[d], a1 = plan.active_off[d + 1];
+        const int nl = 2 * w + 1;
+        const std::vector<double> &tn = tn_d[d];
+        const double *pre = pre_d[d].data();
+        const size_t nl3 = (size_t)nl * nl * nl;
         for (int gt = 0; gt < 8; gt++) {
             std::vector<int> &cp = counts[(size_t)q * 8 + gt];
             std::vector<double> &bl = bmin[(size_t)q * 8 + gt], &bh = bmax[(size_t)q * 8 + gt];
@@ -310,6 +346,9 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
                     if (!(fn > 0.0)) continue;
                     int cxyz[3];
                     for (int dim = 0; dim < 3; dim++) cxyz[dim] = (((gt >> dim) & 1) << 1) | ((ft >> dim) & 1);
+                    const int cmb = cxyz[0] | (cxyz[1] << 2) | (cxyz[2] << 4);
+                    if (!(pre[cmb * nl3 + ((size_t)(ll[2] + w) * nl + (ll[1] + w)) * nl + (ll[0] + w)] * fn > th))
+                        continue;
                     for (int ai = 0; ai < a1 - a0; ai++) {
                         const double nx = tn[((size_t)ai * 4 + cxyz[0]) * nl + (ll[0] + w)];
                         const double ny = tn[((size_t)ai * 4 + cxyz[1]) * nl + (ll[1] + w)];

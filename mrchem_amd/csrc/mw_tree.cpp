@@ -191,17 +191,22 @@ void child_step(const Basis &b, const double *lo, const double *hi, int dim, int
         }
         return;
     }
-    // x: lines of n contiguous values
+    // x: lines of n contiguous values; the i loop runs over a row of F
     for (int r = 0; r < nn; r++) {
         const double *v = lo + (size_t)r * n, *u = hi ? hi + (size_t)r * n : nullptr;
         double *o = out + (size_t)r * n;
-        for (int i = 0; i < n; i++) {
-            double s = 0.0;
-            for (int j = 0; j < n; j++) s += F[j * n2 + bit * n + i] * v[j];
-            if (u)
-                for (int j = 0; j < n; j++) s += F[(n + j) * n2 + bit * n + i] * u[j];
-            o[i] = s;
+        for (int i = 0; i < n; i++) o[i] = 0.0;
+        for (int j = 0; j < n; j++) {
+            const double *f = F + j * n2 + bit * n;
+            const double vj = v[j];
+            for (int i = 0; i < n; i++) o[i] += f[i] * vj;
         }
+        if (u)
+            for (int j = 0; j < n; j++) {
+                const double *f = F + (n + j) * n2 + bit * n;
+                const double uj = u[j];
+                for (int i = 0; i < n; i++) o[i] += f[i] * uj;
+            }
     }
 }
 }  // namespace
