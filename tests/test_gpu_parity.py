@@ -111,13 +111,18 @@ def test_mixed_precision_term_set_and_error(ctx, order):
     assert st["term_apps"] == r64["term_apps"]
 
 
-@pytest.mark.parametrize("order,prec", [(2, 1e-4), (3, 1e-4), (4, 1e-5), (5, 1e-5), (7, 1e-6), (8, 1e-6),
-                                        (9, 1e-6), (10, 1e-5), (11, 1e-5)])
-def test_split16_term_set_and_error(ctx, order, prec):
+@pytest.mark.parametrize("order,prec,basis", [(2, 1e-4, "interpolating"), (3, 1e-4, "interpolating"),
+                                              (4, 1e-5, "interpolating"), (5, 1e-5, "interpolating"),
+                                              (7, 1e-6, "interpolating"), (8, 1e-6, "interpolating"),
+                                              (9, 1e-6, "interpolating"), (10, 1e-5, "interpolating"),
+                                              (11, 1e-5, "interpolating"), (5, 1e-5, "legendre"),
+                                              (9, 1e-6, "legendre"), (11, 1e-5, "legendre")])
+def test_split16_term_set_and_error(ctx, order, prec, basis):
     """FP16 hi/lo split products: same term set as FP64 and within prec of
     the FP64 result on the identical grid (N <= 8, N = 9..10 and N = 11..12
-    take different operand layouts; N = 9 and 11 have partly empty tails)."""
-    mra = _mra(order)
+    take different operand layouts; N = 9 and 11 have partly empty tails), in
+    both bases."""
+    mra = _mra(order, basis)
     P = mw.PoissonOperator(mra, prec)
     rng = np.random.default_rng(23)
     n = 4
@@ -139,8 +144,8 @@ def test_split16_term_set_and_error(ctx, order, prec):
     ns64 = p64.download_ns()
     nssp = psp.download_ns()
     err = np.max(np.abs(nssp - ns64)) / np.max(np.abs(ns64))
-    print("order %d: %d of %d term-apps as FP16 split products, max coef err vs FP64 %.2e" %
-          (order, rsp["lp_term_apps"], rsp["term_apps"], err))
+    print("order %d %s: %d of %d term-apps as FP16 split products, max coef err vs FP64 %.2e" %
+          (order, basis, rsp["lp_term_apps"], rsp["term_apps"], err))
     assert err < prec
     p64.finish()
     psp.finish()
