@@ -110,6 +110,10 @@ double prec_factor(const std::vector<const Tree *> &pt, const NodeKey &k) {
 void check_compatible(const Tree &out, const Operator &op, const Tree &inp) {
     if (out.mra != op.mra || inp.mra != op.mra)
         throw std::invalid_argument("apply: trees and operator were built on different MRAs");
+    // the device kernels are instantiated for k + 1 = 2..12
+    if (op.mra->kp1() > 12)
+        throw std::invalid_argument("apply: the device kernels take k + 1 <= 12 (order <= 11), this MRA has k + 1 = " +
+                                    std::to_string(op.mra->kp1()));
     if (inp.size() == 0) throw std::invalid_argument("apply: input tree is empty");
     for (double v : inp.coefs)
         if (!std::isfinite(v)) throw std::invalid_argument("apply: input tree holds NaN or Inf");
