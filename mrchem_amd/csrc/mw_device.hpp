@@ -78,7 +78,7 @@ struct MulNode {
     double hfac;
 };
 struct DevMulSession;
-int dev_mul_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool_a, const std::vector<double> &pool_b,
+int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a, const double *pool_b, size_t n_b,
                   DevMulSession **out, std::string &err);
 // out_blocks: nodes.size() x 8(k+1)^3 on the host.
 int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,

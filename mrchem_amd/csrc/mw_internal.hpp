@@ -122,12 +122,57 @@ struct KeyHash {
     }
 };
 
+// Allocator for large host arrays (tree coefficients, input pools).
+// Requests of at least 4 MiB are 2 MiB aligned and marked for transparent
+// huge pages (MADV_HUGEPAGE), so a multi-GB tree is first touched in 2 MiB
+// pages instead of 4 KiB ones (a 13 GB tree: 0.43 s instead of 1.17 s of
+// page faults on 8 threads); smaller requests come from operator new.
+// INIT = false leaves new elements uninitialised (default-init), for
+// buffers written in full right after allocation: their pages are then
+// first touched by the (parallel) writer instead of a serial zero fill.
+constexpr size_t kBigAllocMin = (size_t)4 << 20;
+constexpr size_t kHugePage = (size_t)2 << 20;
+void *big_alloc(size_t bytes);
+void big_free(void *p, size_t bytes);
+
+template <typename T, bool INIT>
+struct BigAlloc {
+    using value_type = T;
+    template <typename U>
+    struct rebind {
+        using other = BigAlloc<U, INIT>;
+    };
+    BigAlloc() noexcept = default;
+    template <typename U>
+    BigAlloc(const BigAlloc<U, INIT> &) noexcept {}
+    T *allocate(size_t n) { return static_cast<T *>(big_alloc(n * sizeof(T))); }
+    void deallocate(T *p, size_t n) noexcept { big_free(p, n * sizeof(T)); }
+    template <typename U, typename... Args>
+    void construct(U *p, Args &&...args) {
+        if constexpr (!INIT && sizeof...(Args) == 0)
+            ::new ((void *)p) U;
+        else
+            ::new ((void *)p) U(std::forward<Args>(args)...);
+    }
+    template <typename U>
+    bool operator==(const BigAlloc<U, INIT> &) const noexcept {
+        return true;
+    }
+    template <typename U>
+    bool operator!=(const BigAlloc<U, INIT> &) const noexcept {
+        return false;
+    }
+};
+template <typename T>
+using UninitAlloc = BigAlloc<T, false>;
+using CoefVec = std::vector<double, BigAlloc<double, true>>;
+
 struct Tree {
     const MRA *mra = nullptr;
     std::vector<NodeKey> keys;
     std::vector<int> parent;   // -1 for roots
     std::vector<int> child0;   // first of 8 contiguous children, -1 for leaves
-    std::vector<double> coefs; // nodes x block; compressed [s, d1..d7] at every node
+    CoefVec coefs;             // nodes x block; compressed [s, d1..d7] at every node
     std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> index;
     double sqnorm = 0.0;
 
@@ -180,28 +225,6 @@ bool split_check(const MRA &mra, int depth, double wnorm, double prec, double tn
 // (one refinement level or a whole fixed grid), the input nodes of their
 // bands (real or generated by reconstruction), the dense band table, and the
 // screening thresholds.  Built on the host; consumed by the HIP kernels.
-// Allocator whose new elements are left uninitialised (default-init), for
-// large buffers written in full right after allocation: the pages are then
-// first touched by the (parallel) writer instead of a serial zero fill.
-template <typename T>
-struct UninitAlloc : std::allocator<T> {
-    template <typename U>
-    struct rebind {
-        using other = UninitAlloc<U>;
-    };
-    UninitAlloc() noexcept = default;
-    template <typename U>
-    UninitAlloc(const UninitAlloc<U> &) noexcept {}
-    template <typename U>
-    void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
-        ::new ((void *)p) U;
-    }
-    template <typename U, typename... Args>
-    void construct(U *p, Args &&...args) {
-        ::new ((void *)p) U(std::forward<Args>(args)...);
-    }
-};
-
 struct ApplyPlan {
     int kp1 = 0, ndepth = 0, M = 0;
     // output nodes (in the order of out_idx)

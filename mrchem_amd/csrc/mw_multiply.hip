@@ -383,7 +383,7 @@ struct DevMulSession {
     }
 };
 
-int dev_mul_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool_a, const std::vector<double> &pool_b,
+int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a, const double *pool_b, size_t n_b,
                   DevMulSession **out, std::string &err) {
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
@@ -401,15 +401,15 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool_
     for (int k = 0; k < n; k++)
         for (int j = 0; j < n; j++)
             for (int i = 0; i < n; i++) wsc[(k * n + j) * n + i] = 1.0 / std::sqrt(qw[i] * qw[j] * qw[k]);
-    auto up = [&](const std::vector<double> &v, double **d) -> int {
-        HIP_TRY(hipMalloc((void **)d, sizeof(double) * std::max<size_t>(v.size(), 1)));
-        if (!v.empty()) HIP_TRY(hipMemcpyAsync(*d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
+    auto up = [&](const double *v, size_t nv, double **d) -> int {
+        HIP_TRY(hipMalloc((void **)d, sizeof(double) * std::max<size_t>(nv, 1)));
+        if (nv) HIP_TRY(hipMemcpyAsync(*d, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
         return 0;
     };
-    int rc = up(pool_a, &m->pool_a);
-    if (!rc) rc = up(pool_b, &m->pool_b);
-    if (!rc) rc = up(mra.basis.F, &m->F);
-    if (!rc) rc = up(wsc, &m->wsc);
+    int rc = up(pool_a, n_a, &m->pool_a);
+    if (!rc) rc = up(pool_b, n_b, &m->pool_b);
+    if (!rc) rc = up(mra.basis.F.data(), mra.basis.F.size(), &m->F);
+    if (!rc) rc = up(wsc.data(), wsc.size(), &m->wsc);
     if (!rc) {
         hipError_t e = hipMalloc((void **)&m->scratch, sizeof(double) * (size_t)kMulBlocks * 24 * n3);
         if (e != hipSuccess) {

@@ -8,11 +8,32 @@
 // full compressed [s, d] block, as upstream MRCPP nodes do.
 #include "mw_internal.hpp"
 
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <new>
 #include <stdexcept>
 
 namespace mwg {
+
+void *big_alloc(size_t bytes) {
+    if (bytes < kBigAllocMin) return ::operator new(bytes);
+    const size_t sz = (bytes + kHugePage - 1) / kHugePage * kHugePage;
+    void *p = std::aligned_alloc(kHugePage, sz);
+    if (!p) throw std::bad_alloc();
+    (void)madvise(p, sz, MADV_HUGEPAGE);  // a hint: without THP the pages stay 4 KiB
+    return p;
+}
+
+void big_free(void *p, size_t bytes) {
+    if (!p) return;
+    if (bytes < kBigAllocMin)
+        ::operator delete(p);
+    else
+        std::free(p);
+}
 
 uint64_t morton3(uint64_t x, uint64_t y, uint64_t z) {
     auto spread = [](uint64_t v) {

@@ -854,7 +854,8 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
             if (T.child0[i] < 0) work.push_back(i);
         std::string err;
         DevMulSession *m = nullptr;
-        int rc = dev_mul_begin(ctx->dev, mra, a->tree.coefs, b->tree.coefs, &m, err);
+        int rc = dev_mul_begin(ctx->dev, mra, a->tree.coefs.data(), a->tree.coefs.size(), b->tree.coefs.data(),
+                               b->tree.coefs.size(), &m, err);
         if (rc) return dev_fail(rc, err);
         const int blk = mra.block(), n3 = mra.kp1_3();
         for (int iter = 0;; iter++) {
