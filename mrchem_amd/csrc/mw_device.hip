@@ -9,8 +9,10 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -1055,6 +1057,10 @@ struct DeviceCtx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evz = nullptr, evl = nullptr, ev2s = nullptr,
                ev2e = nullptr;
     std::map<const void *, DevOper> opers;
+    // pinned staging buffers of the large device-to-host copies (allocated
+    // on first use) and the events of their DMAs
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct DevicePlan {
@@ -1146,12 +1152,67 @@ void dev_destroy(DeviceCtx *c) {
     if (c->evl) (void)hipEventDestroy(c->evl);
     if (c->ev2s) (void)hipEventDestroy(c->ev2s);
     if (c->ev2e) (void)hipEventDestroy(c->ev2e);
+    for (int b = 0; b < 2; b++) {
+        if (c->stage[b]) (void)hipHostFree(c->stage[b]);
+        if (c->stage_ev[b]) (void)hipEventDestroy(c->stage_ev[b]);
+    }
     if (c->stream2) {
         (void)hipStreamSynchronize(c->stream2);
         (void)hipStreamDestroy(c->stream2);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+// Device-to-host copies into pageable memory run at about 17 GB/s on
+// MI355X hosts and DMA into pinned memory at about 57 GB/s, so copies of
+// kStageMin bytes or more go through the context's two pinned staging
+// buffers: the DMA of chunk i overlaps the host copy of chunk i - 1, which
+// kStageThreads threads share.
+constexpr size_t kStageChunk = (size_t)64 << 20, kStageMin = (size_t)32 << 20;
+constexpr int kStageThreads = 8;
+
+static void par_memcpy(void *dst, const void *src, size_t n) {
+    auto part = [&](int t) {
+        const size_t a = n * t / kStageThreads, b = n * (t + 1) / kStageThreads;
+        std::memcpy(static_cast<char *>(dst) + a, static_cast<const char *>(src) + a, b - a);
+    };
+    std::thread th[kStageThreads - 1];
+    for (int t = 1; t < kStageThreads; t++) th[t - 1] = std::thread(part, t);
+    part(0);
+    for (auto &x : th) x.join();
+}
+
+int dev_copy_d2h(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err) {
+    if (bytes == 0) return 0;
+    HIP_TRY(hipSetDevice(c->device));
+    if (bytes < kStageMin) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    for (int b = 0; b < 2; b++) {
+        if (!c->stage[b]) HIP_TRY(hipHostMalloc(&c->stage[b], kStageChunk, hipHostMallocDefault));
+        if (!c->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
+    }
+    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
+    auto len = [&](size_t i) { return std::min(kStageChunk, bytes - i * kStageChunk); };
+    // chunk i goes through buffer i & 1; its DMA is queued only after the
+    // host has copied chunk i - 2 out of that buffer (the previous iteration)
+    for (size_t i = 0; i <= nch; i++) {
+        if (i < nch) {
+            const int b = (int)(i & 1);
+            HIP_TRY(hipMemcpyAsync(c->stage[b], static_cast<const char *>(src) + i * kStageChunk, len(i),
+                                   hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipEventRecord(c->stage_ev[b], c->stream));
+        }
+        if (i > 0) {
+            const int b = (int)((i - 1) & 1);
+            HIP_TRY(hipEventSynchronize(c->stage_ev[b]));
+            par_memcpy(static_cast<char *>(dst) + (i - 1) * kStageChunk, c->stage[b], len(i - 1));
+        }
+    }
+    return 0;
 }
 
 void dev_forget_operator(DeviceCtx *c, const void *key) {
@@ -1806,10 +1867,8 @@ int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err) {
     DeviceCtx *c = p->ctx;
     HIP_TRY(hipSetDevice(c->device));
     if (p->n_out == 0) return 0;
-    HIP_TRY(hipMemcpyAsync(out_ns, p->args.out_ns, sizeof(double) * (size_t)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1,
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return dev_copy_d2h(c, out_ns, p->args.out_ns, sizeof(double) * (size_t)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1,
+                        err);
 }
 
 int dev_plan_clone_output(DevicePlan *p, double **out, std::string &err) {
@@ -1846,10 +1905,7 @@ int dev_plan_download_range(DevicePlan *p, long long first, long long count, dou
     if (count == 0) return 0;
     HIP_TRY(hipSetDevice(c->device));
     const size_t blk = (size_t)8 * p->kp1 * p->kp1 * p->kp1;
-    HIP_TRY(hipMemcpyAsync(dst, p->args.out_ns + (size_t)first * blk, sizeof(double) * (size_t)count * blk,
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return 0;
+    return dev_copy_d2h(c, dst, p->args.out_ns + (size_t)first * blk, sizeof(double) * (size_t)count * blk, err);
 }
 
 }  // namespace mwg

@@ -101,6 +101,9 @@ struct PostPair {
 int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
                        std::string &err);
 int dev_download(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err);
+// Device-to-host copy on the context's stream, finished on return; large
+// copies go through pinned staging buffers (mw_device.hip).
+int dev_copy_d2h(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err);
 void dev_free(DeviceCtx *c, void *p);
 
 // Node-wise sums (mw_multiply.hip): for one output node and one input, the

@@ -358,12 +358,7 @@ int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vect
 }
 
 int dev_download(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err) {
-    if (n == 0) return 0;
-    HIP_TRY(hipSetDevice(dev_device(c)));
-    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
-    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    return 0;
+    return dev_copy_d2h(c, dst, src, sizeof(double) * n, err);
 }
 
 void dev_free(DeviceCtx *c, void *p) {
@@ -448,11 +443,11 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
                            m->pool_b, m->F, m->wsc, coef, m->scratch, dout);
         e = hipGetLastError();
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(out_blocks, dout, sizeof(double) * nodes.size() * 8 * n3, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    int crc = 0;
+    if (e == hipSuccess) crc = dev_copy_d2h(m->ctx, out_blocks, dout, sizeof(double) * nodes.size() * 8 * n3, err);
     (void)hipFree(dn);
     (void)hipFree(dout);
+    if (crc) return crc;
     if (e != hipSuccess) {
         err = std::string("multiply launch: ") + hipGetErrorString(e);
         return -2;
@@ -536,11 +531,11 @@ int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_
                            m->pool, m->coefs, m->F, m->scratch, dout);
         e = hipGetLastError();
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    int crc = 0;
+    if (e == hipSuccess) crc = dev_copy_d2h(m->ctx, out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, err);
     (void)hipFree(dsrc);
     (void)hipFree(dout);
+    if (crc) return crc;
     if (e != hipSuccess) {
         err = std::string("add launch: ") + hipGetErrorString(e);
         return -2;

@@ -87,3 +87,36 @@ def test_unsupported_order_is_an_argument_error(ctx):
     V5 = mw.FunctionTree(mra5)
     mw.apply(1e-4, V5, P5, rho5, abs_prec=True, ctx=ctx)
     assert V5.size() > 0 and V5.norm() > 0.0
+
+
+def test_large_download_through_staging_matches_oracle(ctx):
+    """Outputs of 32 MiB and more reach the host through two pinned staging
+    buffers in 64 MiB chunks.  The C2 Helmholtz workload (k = 9, 2,688 output
+    nodes, 172 MB of non-standard blocks: two full chunks and a partial one)
+    downloaded that way equals the oracle's raw non-standard blocks on a
+    sample of nodes taken from every chunk, including both sides of each
+    chunk boundary and the last node."""
+    from mrchem_amd import workloads
+    cfg = workloads.config("c2")
+    mra = mw.MultiResolutionAnalysis(order=cfg["order"], basis="interpolating", max_depth=16, **workloads.WORLD)
+    H = mw.HelmholtzOperator(mra, cfg["mu"][0], cfg["prec"])
+    c, a, R = cfg["densities"][0]
+    f = mw.FunctionTree(mra).project_gaussians(cfg["prec"], c, a, R)
+    V = mw.FunctionTree(mra)
+    mw.apply(cfg["prec"], V, H, f, abs_prec=True, ctx=ctx)
+    od, ol, _ = V.nodes()
+    thr = V.thresholds()
+    plan = mw.ApplyPlan(mw.FunctionTree(mra).set_nodes(od, ol), H, f, thr, mode="fp64", ctx=ctx)
+    plan.execute()
+    ns = plan.download_ns()
+    node_bytes = 8 * mra.block
+    assert len(od) * node_bytes > 2 * (64 << 20)
+    pick = set(range(0, len(od), 97)) | {len(od) - 1}
+    for b in range(1, len(od) * node_bytes // (64 << 20) + 1):
+        q = b * (64 << 20) // node_bytes  # the node that straddles chunk boundary b
+        pick |= {q - 1, q, min(q + 1, len(od) - 1)}
+    pick = np.array(sorted(pick))
+    _, _, ref = oracle.apply_fixed(mra, H.tables(), f.nodes(), od[pick], ol[pick], thr[pick], want_ns=True, post=False)
+    err = np.max(np.abs(ns[pick] - ref)) / np.max(np.abs(ref))
+    print("%d nodes (%.0f MB), %d sampled: max rel err %.2e" % (len(od), len(od) * node_bytes / 1e6, len(pick), err))
+    assert err < 1e-12
