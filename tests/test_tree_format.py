@@ -5,7 +5,8 @@ Pins: the documented layout (header fields at their offsets, the padded
 depth array, translations, blocks in tree order) read back with struct and
 numpy independently of the library; bit-identical round trips through memory
 and through a file; refusal of another MRA, a truncated or resized buffer and
-a bad magic."""
+a bad magic; zero-initialised nodes and round trips of an 80 MB tree
+(the huge-page allocation path)."""
 import struct
 
 import numpy as np
@@ -82,3 +83,23 @@ def test_refuses_bad_input():
         mw.FunctionTree(mra).unpack(b"X" + b[1:])
     with pytest.raises(mw.MWGError):
         mw.FunctionTree(mra).load("/nonexistent/dir/tree.bin")
+
+
+def test_large_tree_zero_init_and_round_trip():
+    """Trees whose coefficients take 4 MiB or more are allocated in 2 MiB
+    aligned, huge-page-advised blocks: nodes created without coefficients
+    still read as zeros there, and an 80 MB tree packs, unpacks and sets
+    back bit-identically."""
+    cfg = workloads.config("c2")
+    mra = mw.MultiResolutionAnalysis(order=cfg["order"], basis="interpolating", max_depth=16, **workloads.WORLD)
+    c, a, R = cfg["densities"][0]
+    f = mw.FunctionTree(mra).project_gaussians(cfg["prec"], c, a, R)
+    d, l, blocks = f.nodes()
+    assert blocks.nbytes > 64 << 20
+    empty = mw.FunctionTree(mra).set_nodes(d, l)
+    assert not np.any(empty.nodes()[2])
+    g = mw.FunctionTree(mra).unpack(f.pack())
+    for x, y in zip(f.nodes(), g.nodes()):
+        assert np.array_equal(x, y)
+    h = mw.FunctionTree(mra).set_nodes(d, l, blocks)
+    assert np.array_equal(h.nodes()[2], blocks) and h.square_norm() == f.square_norm()
