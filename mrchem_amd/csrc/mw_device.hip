@@ -1186,15 +1186,23 @@ static void par_memcpy(void *dst, const void *src, size_t n) {
 int dev_copy_d2h(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err) {
     if (bytes == 0) return 0;
     HIP_TRY(hipSetDevice(c->device));
-    if (bytes < kStageMin) {
+    // without pinned staging memory (allocation refused) the copy goes
+    // straight from the device into the pageable destination
+    bool staged = bytes >= kStageMin;
+    for (int b = 0; b < 2 && staged; b++) {
+        if (!c->stage[b] && hipHostMalloc(&c->stage[b], kStageChunk, hipHostMallocDefault) != hipSuccess) {
+            c->stage[b] = nullptr;
+            (void)hipGetLastError();
+            staged = false;
+        }
+    }
+    if (!staged) {
         HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         return 0;
     }
-    for (int b = 0; b < 2; b++) {
-        if (!c->stage[b]) HIP_TRY(hipHostMalloc(&c->stage[b], kStageChunk, hipHostMallocDefault));
+    for (int b = 0; b < 2; b++)
         if (!c->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
-    }
     const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
     auto len = [&](size_t i) { return std::min(kStageChunk, bytes - i * kStageChunk); };
     // chunk i goes through buffer i & 1; its DMA is queued only after the
