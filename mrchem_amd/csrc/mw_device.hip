@@ -532,8 +532,9 @@ struct SplitShape {
     static constexpr int TWZ = (NTZ + NW - 1) / NW;
     static constexpr bool TWO = N > 10;
     // waves per SIMD the register allocation is held to (VGPRs <= 512 / WPE):
-    // N > 10 at 3 (C5, k = 11: -6 % although 140 bytes per lane spill); no
-    // limit below, where a cap of 4 spills at N = 5, 7, 8 or schedules worse
+    // N > 10 at 3 (64 Helmholtz applies at k = 11: -6 % although 140 bytes
+    // per lane spill); no limit below, where a cap of 4 spills at N = 5, 7, 8
+    // or schedules worse
     static constexpr int WPE = N > 10 ? 3 : 1;
     // 16-byte chunks per row: hi, lo, tail (+ the extra chunk for N > 10).
     // For N <= 10 the odd number of chunks makes the row pitch spread 16
@@ -950,8 +951,9 @@ __global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_e
                 msk1 = msk0;
             };
             // N <= 8 (registers to spare): steady phase pairs get a copy of
-            // their own without the per-pass conditions (c3, k = 7: -13 %;
-            // at N >= 9 the copy costs a wave per SIMD and is slower)
+            // their own without the per-pass conditions (136 exchange pair
+            // applies at k = 7: -13 %; at N >= 9 the copy costs a wave per
+            // SIMD and is slower)
             for (int q = 0; q < n + 2; q += 2) {
                 if (N <= 8 && q >= 2 && q + 1 < n) {
                     phase(q, std::integral_constant<int, 0>{}, std::integral_constant<bool, N <= 8>{});
