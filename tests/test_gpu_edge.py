@@ -91,11 +91,12 @@ def test_unsupported_order_is_an_argument_error(ctx):
 
 def test_large_download_through_staging_matches_oracle(ctx):
     """Outputs of 32 MiB and more reach the host through two pinned staging
-    buffers in 64 MiB chunks.  The C2 Helmholtz workload (k = 9, 2,688 output
-    nodes, 172 MB of non-standard blocks: two full chunks and a partial one)
-    downloaded that way equals the oracle's raw non-standard blocks on a
-    sample of nodes taken from every chunk, including both sides of each
-    chunk boundary and the last node."""
+    buffers in 64 MiB chunks.  The Helmholtz workload "c2" (mu 1.2, a sum of
+    10 Gaussians, k = 9, prec 1e-6; 2,688 output nodes, 172 MB of
+    non-standard blocks: two full chunks and a partial one) downloaded that
+    way equals the oracle's raw non-standard blocks on a sample of nodes
+    taken from every chunk, including both sides of each chunk boundary and
+    the last node."""
     from mrchem_amd import workloads
     cfg = workloads.config("c2")
     mra = mw.MultiResolutionAnalysis(order=cfg["order"], basis="interpolating", max_depth=16, **workloads.WORLD)
