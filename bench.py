@@ -8,9 +8,12 @@ precision as in HelmholtzVector (src/scf_solver/HelmholtzVector.cpp:149).
 (Poisson, k = 7) and 64 orbitals each with its own Helmholtz mu_i (k = 11).
 
 The output grids and per-node screening thresholds come from one adaptive
-(batched) apply and are then frozen; a step is one apply of every entry over
-those grids with the input pools, band tables and operator tables already
-resident in HBM.
+(batched) apply and are then frozen; a step is one whole apply of every
+entry over those grids with the input pools, band tables and operator tables
+already resident in HBM: the per-node calculator, then the TopDown(add) and
+BottomUp transforms and the node norms of the output, all on the device.
+The adaptive apply itself (host refinement loop, the call MRChem makes) is
+reported as its own rate, "adaptive_apply".
 
 Prints one JSON line.  Roofline: algorithmic flops = term-applications that
 passed screening x 6(k+1)^4 (counted by the kernel itself, equal to the
@@ -91,6 +94,7 @@ def main():
     ap.add_argument("--modes", default="mixed,split16", help="low-precision modes timed beside plain FP64")
     ap.add_argument("--mode", default="split16", help="the mode the headline value is quoted for")
     ap.add_argument("--tau", type=float, default=None, help="FP64 class cut (bound > tau x threshold)")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -193,40 +197,56 @@ def main():
         (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], pinfo["segments"], pinfo["band_w"], t_plan))
 
     def timed(plan):
+        """Warmup steps, eager steps (HIP events around each launch: the
+        per-launch device times), then the timed steps, each one replay of
+        the captured graph of the whole apply (or eager with --no-graph)."""
         for _ in range(args.warmup):
-            plan.execute()
+            plan.apply()
+        eager = [plan.apply() for _ in range(max(3, args.steps // 4))]
+        if not args.no_graph:
+            plan.use_graph(True)
+            plan.apply()  # capture and first replay
         runs = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            runs.append(plan.execute())
+            runs.append(plan.apply())
         wall = time.perf_counter() - t0
-        return runs, 1e3 * wall / args.steps
+        plan.use_graph(False)
+        return runs, eager, 1e3 * wall / args.steps
 
     fl = 6.0 * (k + 1) ** 4
     res = {}
     for m in modes:
-        runs, wall = timed(plans[m])
+        runs, eager, wall = timed(plans[m])
         r = {"wall_ms": wall, "kernel_ms": float(np.mean([x["ms"] for x in runs])), "term_apps": runs[-1]["term_apps"],
-             "lp_term_apps": runs[-1]["lp_term_apps"], "ms_fp64": float(np.mean([x["ms_fp64"] for x in runs])),
-             "ms_lp": float(np.mean([x["ms_lp"] for x in runs]))}
+             "lp_term_apps": runs[-1]["lp_term_apps"], "ms_fp64": float(np.mean([x["ms_fp64"] for x in eager])),
+             "ms_lp": float(np.mean([x["ms_lp"] for x in eager])),
+             "post_ms": float(np.mean([x["post_ms"] for x in eager])),
+             "eager_ms": float(np.mean([x["ms"] for x in eager])), "graph": not args.no_graph}
         res[m] = r
     apps = res["fp64"]["term_apps"]
     assert apps == pinfo["host_term_apps"], "kernel and host screening counts differ"
     for m in modes:
         assert res[m]["term_apps"] == apps, "%s applied a different term set" % m
-    ach64 = apps * fl / (res["fp64"]["kernel_ms"] * 1e-3) / 1e12
-    log("fp64   : wall %.3f ms/step, kernel %.3f ms, %d term-apps, %.2f TFLOP/s" %
-        (res["fp64"]["wall_ms"], res["fp64"]["kernel_ms"], apps, ach64))
+    # the FP64 calculator alone (its one launch), for its roofline
+    calc64 = res["fp64"]["ms_fp64"]
+    ach64 = apps * fl / (calc64 * 1e-3) / 1e12
+    log("fp64   : wall %.3f ms/step, device %.3f ms (calculator %.3f ms, post transforms + norms %.3f ms), %d "
+        "term-apps, %.2f TFLOP/s" % (res["fp64"]["wall_ms"], res["fp64"]["kernel_ms"], calc64,
+                                      res["fp64"]["post_ms"], apps, ach64))
+    # raw non-standard blocks for the checks: one calculator run per mode
+    for m in modes:
+        plans[m].execute()
     ns64 = plans["fp64"].download_ns()
     for m in modes[1:]:
         r = res[m]
         r["lp_tflops"] = r["lp_term_apps"] * fl / (r["ms_lp"] * 1e-3) / 1e12 if r["ms_lp"] > 0 else 0.0
         ns = plans[m].download_ns()
         r["coef_err"] = float(np.max(np.abs(ns - ns64)) / np.max(np.abs(ns64)))
-        log("%-7s: wall %.3f ms/step, kernel %.3f ms (fp64 part %.3f ms, low-precision part %.3f ms), %d of %d "
-            "term-apps low-precision (%.2f TFLOP/s algorithmic), max coef err vs fp64 %.3e" %
-            (m, r["wall_ms"], r["kernel_ms"], r["ms_fp64"], r["ms_lp"], r["lp_term_apps"], apps, r["lp_tflops"],
-             r["coef_err"]))
+        log("%-7s: wall %.3f ms/step, device %.3f ms (fp64 part %.3f ms, low-precision part %.3f ms, post "
+            "transforms + norms %.3f ms), %d of %d term-apps low-precision (%.2f TFLOP/s algorithmic), max coef err "
+            "vs fp64 %.3e" % (m, r["wall_ms"], r["kernel_ms"], r["ms_fp64"], r["ms_lp"], r["post_ms"],
+                              r["lp_term_apps"], apps, r["lp_tflops"], r["coef_err"]))
     for m in modes:
         plans[m].finish()
     c64 = np.concatenate([o.nodes()[2] for o in outs])
@@ -280,7 +300,12 @@ def main():
                      "kernel": kname},
         "fp64_path": {"value": n_out / (ms64 * 1e-3), "ms_per_step": ms64, "kernel_ms": res["fp64"]["kernel_ms"],
                       "roofline": {"achieved": ach64, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                                   "frac": ach64 / PEAK_TFLOPS["f64"]}},
+                                   "frac": ach64 / PEAK_TFLOPS["f64"], "kernel": "k_apply<%d, double>" % (k + 1)}},
+        "adaptive_apply": {"value": n_out / t_adapt, "unit": "nodes/s", "wall_ms": 1e3 * t_adapt,
+                           "note": "the adaptive drop-in call (mwg_apply_batch, FP64) that fixed the grid: host "
+                                   "refinement loop, per-level plans and launches, post transforms, download",
+                           "stages_ms": {k: ast[k] for k in ("kernel_ms", "plan_ms", "device_ms", "split_ms",
+                                                             "post_ms", "wall_ms")}},
         "modes": {m: dict(res[m], value=n_out / (res[m]["wall_ms"] * 1e-3)) for m in modes},
         "speedup_vs_fp64": ms64 / msmx,
         "term_apps": apps,
@@ -295,12 +320,38 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def host_cpu():
+    """(model name, physical cores per socket, sockets) from lscpu, and the
+    CPUs this process may run on."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    model = info.get("Model name") or cpu_model()
+    try:
+        per_socket = int(info.get("Core(s) per socket", "0"))
+        sockets = int(info.get("Socket(s)", "1"))
+    except ValueError:
+        per_socket, sockets = 0, 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, per_socket, sockets, usable
+
+
 def cpu_baseline(mra, ops, inps, outs, thr, ns_gpu, target_s, check):
     """FP64 OpenMP oracle on a bounded sample of the frozen output grids:
     whole applies for a batch (in a seeded random order), random nodes of the
-    one grid for a single apply."""
+    one grid for a single apply.  Threads: the physical cores of one socket,
+    or fewer when this process may run on fewer CPUs (the GPU box gives each
+    GPU a share of its host CPUs)."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    model, per_socket, sockets, usable = host_cpu()
+    threads = min(per_socket, usable) if per_socket > 0 else usable
     oracle.set_threads(threads)
     rng = np.random.default_rng(12345)
     offs = np.cumsum([0] + [o.size() for o in outs])
@@ -355,7 +406,8 @@ def cpu_baseline(mra, ops, inps, outs, thr, ns_gpu, target_s, check):
     log("cpu baseline: %d of %d nodes in %.2fs on %d threads; sample |gpu-oracle|/max = %.2e" %
         (m, n_tot, dt, threads, err))
     res = {"value": m / dt, "unit": "nodes/s", "cores": threads, "kind": "port",
-           "sample": sample + ", FP64 OpenMP oracle, no post transforms", "cpu_model": cpu_model()}
+           "sample": sample + ", FP64 OpenMP oracle, no post transforms", "cpu_model": model,
+           "cores_per_socket": per_socket, "sockets": sockets, "cpus_usable": usable}
     if len(outs) == 1:
         res["one_thread_value"] = one_thread
         log("cpu baseline, 1 thread: %.1f nodes/s" % one_thread)

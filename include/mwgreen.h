@@ -21,6 +21,11 @@
  *     (bit d set = wavelet along direction d), then z, y, x with x fastest.
  *   - Handles are opaque pointers; none may be shared between threads while a
  *     call on it runs.  An apply takes `inp` and the operator read-only.
+ *   - Lifetimes: a plan (mwg_plan_create*) holds device pointers into its
+ *     operators' resident tables and writes into its output trees, so its
+ *     context, operators and output trees must outlive it.  Destroying any of
+ *     them while a plan that uses it is alive returns MWG_ERR_ARG and leaves
+ *     the handle intact.  Input trees are copied when the plan is made.
  */
 #ifndef MWGREEN_H
 #define MWGREEN_H
@@ -54,6 +59,9 @@ typedef struct mwg_tree_s *mwg_tree;
 typedef struct mwg_plan_s *mwg_plan;
 
 const char *mwg_last_error(void);
+/* Message of the last failed call on ctx or on a plan of ctx (any thread);
+ * "" before the first failure.  Valid until the next failure on ctx. */
+const char *mwg_context_last_error(mwg_context ctx);
 
 /* Device context: one HIP device, one stream, cached operator tables.
  * Replaces the OpenMP/MPI environment mrcpp::apply runs in (src/mrenv.cpp:129-135). */
@@ -226,19 +234,55 @@ int mwg_plan_create_batch(mwg_context ctx, int n, const mwg_tree *out, const mwg
  * [1] term-applications that passed screening, [2] of them on the
  * low-precision path (FP32 or FP16 split),
  * [3] device ms of the FP64-class launch, [4] of the low-precision launch
- * (the two run concurrently; [0] covers both). */
+ * (MIXED, SPLIT16: the two run concurrently on two streams, [0] covers
+ * both; FP64: [3] is the calculator launch). */
 int mwg_plan_execute(mwg_plan plan, double *stats, int n);
+/* One whole frozen-grid apply on resident data: the per-node calculator as
+ * in mwg_plan_execute, then the TopDown(add) and BottomUp transforms and the
+ * node norms of the output, all on the device and in place on the plan's
+ * output blocks (what mrcpp::apply does after its calculator).  stats (n <= 6
+ * values): [0]-[4] as mwg_plan_execute, [0] now covering the whole apply,
+ * [5] device ms of the post transforms and norms.  No counterpart in
+ * MRChem's calls; mwg_plan_finish then only downloads. */
+int mwg_plan_apply(mwg_plan plan, double *stats, int n);
+/* Autotuner of a frozen-grid plan.  Runs the plan once in FP64 as the
+ * reference, then searches, one knob at a time from the current setting:
+ * the FP64-class cut tau (MIXED, SPLIT16: terms with bound > tau x threshold
+ * stay FP64), the single-rounding cut of the FP16 split class (SPLIT16), and
+ * the order of the work queue (segment order, or the largest passing-term
+ * count first).  Each configuration is timed over `reps` executions (device
+ * time) and counts only if its largest output difference to the reference is
+ * at most max_rel_err of the reference's largest value.  The fastest is kept
+ * in the plan and cached in the context under (k + 1, operator terms, widest
+ * band, mode): later plans with that key start from it.  report (n <= 8):
+ * [0] tau, [1] single-rounding ratio, [2] order, [3] its device ms,
+ * [4] device ms of the starting setting, [5] its relative error,
+ * [6] configurations timed, [7] speedup [4]/[3].  No counterpart in MRChem's
+ * calls. */
+int mwg_plan_tune(mwg_plan plan, double max_rel_err, int reps, double *report, int n);
+/* With on != 0, later mwg_plan_execute / mwg_plan_apply calls replay one
+ * hipGraph of the whole step (output and counters cleared by kernels, the
+ * calculator launch or launches, and for mwg_plan_apply the post transforms
+ * and norms), captured on the plan after the first eager run of that step
+ * and kept until the plan is destroyed; a replay reports the whole device
+ * time only (stats [3]-[5] are 0).  Default off.  No counterpart in MRChem's
+ * calls; it serves repeated applies on one grid. */
+int mwg_plan_use_graph(mwg_plan plan, int on);
 /* Copy the result into the output tree(s) and run the TopDown/BottomUp
- * transforms. */
+ * transforms (after mwg_plan_apply: download the transformed blocks).  MWG_ERR_ARG when an output tree's grid is no longer the one
+ * the plan was made on. */
 int mwg_plan_finish(mwg_plan plan);
 /* Raw non-standard blocks of the last execution (n_out x 8(k+1)^3, tree
  * order, applies of a batch one after the other), before the TopDown/BottomUp
- * transforms. */
-int mwg_plan_download_ns(mwg_plan plan, double *ns);
+ * transforms.  capacity: doubles ns holds; MWG_ERR_ARG when it is less than
+ * n_out x 8(k+1)^3 (mwg_plan_info [0] gives n_out), or when the last
+ * execution was mwg_plan_apply (its blocks are transformed in place). */
+int mwg_plan_download_ns(mwg_plan plan, double *ns, long long capacity);
 /* [0] output nodes, [1] input slots, [2] generated slots, [3] host plan ms,
  * [4] term-applications counted on the host with the screening rule,
- * [5] work segments (workgroups of an FP64 launch), [6] applies,
- * [7] widest band half-width (translations |l| <= W per direction). */
+ * [5] work segments (entries of the FP64 work queue), [6] applies,
+ * [7] widest band half-width (translations |l| <= W per direction),
+ * [8] FP64-class cut tau, [9] single-rounding ratio, [10] queue order. */
 int mwg_plan_info(mwg_plan plan, double *info, int n);
 int mwg_plan_destroy(mwg_plan plan);
 

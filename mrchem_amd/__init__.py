@@ -98,10 +98,15 @@ class _ApplyDesc(ctypes.Structure):
 
 _sig("mwg_apply_batch", _p, _i, ctypes.POINTER(_ApplyDesc))
 _sig("mwg_plan_execute", _p, _dp, _i)
+_sig("mwg_plan_apply", _p, _dp, _i)
+_sig("mwg_plan_use_graph", _p, _i)
+_sig("mwg_plan_tune", _p, _d, _i, _dp, _i)
 _sig("mwg_context_set_precision", _p, _i, _d)
 _sig("mwg_plan_finish", _p)
 _sig("mwg_plan_info", _p, _dp, _i)
-_sig("mwg_plan_download_ns", _p, _dp)
+_sig("mwg_plan_download_ns", _p, _dp, _ll)
+lib.mwg_context_last_error.restype = ctypes.c_char_p
+lib.mwg_context_last_error.argtypes = [_p]
 _sig("mwg_plan_destroy", _p)
 
 MODES = {"fp64": 0, "mixed": 1, "split16": 2}
@@ -143,6 +148,10 @@ class Context:
         'split16' (the rest as FP16 hi/lo split products on FP16 MFMA)."""
         _check(lib.mwg_context_set_precision(self._h, MODES[mode], tau))
         self.mode, self.tau = mode, float(tau)
+
+    def last_error(self):
+        """Message of the last failed call on this context ("" if none)."""
+        return lib.mwg_context_last_error(self._h).decode()
 
     def last_stats(self):
         s = np.zeros(11)
@@ -498,6 +507,34 @@ class ApplyPlan:
         _check(lib.mwg_plan_execute(self._h, _ptr(v, _d), 5))
         return {"ms": v[0], "term_apps": int(v[1]), "lp_term_apps": int(v[2]), "ms_fp64": v[3], "ms_lp": v[4]}
 
+    def tune(self, max_rel_err, reps=3):
+        """Autotune the plan's knobs (FP64-class cut tau, single-rounding cut,
+        queue order) against an FP64 run of the same plan; the choice stays
+        in the plan and is cached in the context for plans of the same
+        (k + 1, terms, band width, mode).  Returns the picks and times."""
+        v = np.zeros(8)
+        _check(lib.mwg_plan_tune(self._h, float(max_rel_err), int(reps), _ptr(v, _d), 8))
+        return {"tau": v[0], "single_ratio": v[1], "order": int(v[2]), "ms": v[3], "start_ms": v[4],
+                "rel_err": v[5], "tried": int(v[6]), "speedup": v[7]}
+
+    def use_graph(self, on=True):
+        """Later execute()/apply() calls replay one captured hipGraph of the
+        whole step (after one eager run); a replay reports the whole device
+        time only."""
+        _check(lib.mwg_plan_use_graph(self._h, 1 if on else 0))
+        return self
+
+    def apply(self):
+        """One whole apply on resident data: the calculator, then the TopDown
+        and BottomUp transforms and the norms on the device, in place (what
+        mrcpp::apply does after its calculator).  Returns the execute() dict
+        with ms covering all of it and post_ms the post transforms' share;
+        finish() then only downloads."""
+        v = np.zeros(6)
+        _check(lib.mwg_plan_apply(self._h, _ptr(v, _d), 6))
+        return {"ms": v[0], "term_apps": int(v[1]), "lp_term_apps": int(v[2]), "ms_fp64": v[3], "ms_lp": v[4],
+                "post_ms": v[5]}
+
     def finish(self):
         """Result into the output tree(s); returns the tree (or the list)."""
         _check(lib.mwg_plan_finish(self._h))
@@ -505,16 +542,17 @@ class ApplyPlan:
 
     def download_ns(self):
         """Raw non-standard blocks, all applies one after the other."""
-        n = sum(o.size() for o in self._outs)
-        ns = np.zeros((n, self._outs[0].mra.block))
-        _check(lib.mwg_plan_download_ns(self._h, _ptr(ns, _d)))
+        n = self.info()["out_nodes"]
+        ns = np.empty((n, self._outs[0].mra.block))
+        _check(lib.mwg_plan_download_ns(self._h, _ptr(ns, _d), ns.size))
         return ns
 
     def info(self):
-        v = np.zeros(8)
-        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 8))
+        v = np.zeros(11)
+        _check(lib.mwg_plan_info(self._h, _ptr(v, _d), 11))
         return {"out_nodes": int(v[0]), "in_slots": int(v[1]), "generated": int(v[2]), "plan_ms": v[3],
-                "host_term_apps": int(v[4]), "segments": int(v[5]), "applies": int(v[6]), "band_w": int(v[7])}
+                "host_term_apps": int(v[4]), "segments": int(v[5]), "applies": int(v[6]), "band_w": int(v[7]),
+                "tau": v[8], "single_ratio": v[9], "order": int(v[10])}
 
     def __del__(self):
         h = getattr(self, "_h", None)

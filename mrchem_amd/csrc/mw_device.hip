@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -307,26 +308,37 @@ __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, co
     return n;
 }
 
+// LDS of one job of apply_segment: the pass buffer, two operator stages,
+// the worklist and the staged term table.
 template <int N, typename T>
-__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const int *seg_list, double lo_mult,
-                                                          double hi_mult, int cls_counter) {
+struct ApplySmem {
+    T buf[JobShape<N>::BUF];
+    T ops[2][JobShape<N>::OB];
+    int wl[kWorkCap];
+    int ltab[64];
+    long long btab[64];
+};
+
+// One work segment: the terms of output job seg_job[seg] over band positions
+// [seg_p0, seg_p1) in the launch's class, added into the FP64 output.  nrm_s
+// is the norm table [term][c][l+W], 64 x nrm_stride(2 Wmax + 1) doubles.
+template <int N, typename T>
+__device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int seg, double lo_mult, double hi_mult,
+                                              int cls_counter, ApplySmem<N, T> &sm, double *nrm_s) {
     using S = JobShape<N>;
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
     constexpr int N2 = S::N2, N3 = S::N3, KS = S::KS, TW = S::TW, NTH = S::T;
     constexpr int CP = S::CP, BUF = S::BUF, OB = S::OB, OPL = S::OPL;
-    __shared__ T buf[BUF];
-    __shared__ T ops[2][OB];
-    __shared__ int wl[kWorkCap];
-    __shared__ int ltab[64];
-    __shared__ long long btab[64];
-    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x nrm_stride(2 Wmax + 1)
+    T *buf = sm.buf;
+    T(*ops)[OB] = sm.ops;
+    int *wl = sm.wl;
+    int *ltab = sm.ltab;
+    long long *btab = sm.btab;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
     for (int e = tid; e < BUF; e += NTH) buf[e] = T(0);
 
-    // the launch's segments: all of them, or the listed ones
-    const int seg = seg_list ? seg_list[blockIdx.x] : (int)blockIdx.x;
     const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
     const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
     const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
@@ -365,7 +377,7 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
     // operator blocks Ox, Oy, Oz of worklist item `it` into this thread's share
     auto fetch = [&](int it, T *regs) {
         const int p = it >> 9, ft = (it >> 6) & 7, b = it & 63;
-        const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+        const int lx = p % side - w, ly = (p / side) % side - w, lz = p / (side * side) - w;
         const int Lb = ltab[b];
         const long long bb = btab[b];
 #pragma unroll
@@ -374,8 +386,12 @@ __global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool
             regs[u] = T(0);
             if (e < OB) {
                 const int dim = e / N2, r = e % N2;
-                const int c = (gb[dim] << 1) | ((ft >> dim) & 1);
-                regs[u] = opblk[bb + ((long long)c * (2 * Lb + 1) + (ll[dim] + Lb)) * N2 + r];
+                // selects, not a runtime index into a small array (which would
+                // live in scratch)
+                const int gbd = dim == 0 ? gb[0] : (dim == 1 ? gb[1] : gb[2]);
+                const int ld = dim == 0 ? lx : (dim == 1 ? ly : lz);
+                const int c = (gbd << 1) | ((ft >> dim) & 1);
+                regs[u] = opblk[bb + ((long long)c * (2 * Lb + 1) + (ld + Lb)) * N2 + r];
             }
         }
     };
@@ -548,6 +564,8 @@ struct SplitShape {
     // per row, after the rows.
     static constexpr int RL = 24;               // halfs per LDS row
     static constexpr int RE = TWO ? 4 : 0;      // halfs per LDS extra chunk
+    static constexpr int EXT0 = (N2 + 2) * RL;          // start of the extra chunks
+    static constexpr int BUFH = (N2 + 2) * (RL + RE);   // halfs per intermediate buffer
 };
 
 // Split row of n <= 12 values u (|u| < 1): data side (DATA) or operator side.
@@ -659,27 +677,38 @@ constexpr int kZ0Bit = 28, kZ1Bit = 29, kSingleBit = 30;
 // choice is a bit mask, so no pass branches on the lane or the item.
 constexpr int kSplitCap = 384;  // worklist items per batch (>= 64 + 128)
 
+// LDS of one job of split_segment: the double-buffered intermediates, the
+// worklist with its scale products and the staged term table.
 template <int N>
-__global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_eu(SplitShape<N>::WPE))) void k_apply_h(
-    KArgs a, const _Float16 *pool_h, const double *pool_sc, const int *seg_list, double lo_mult, double hi_mult,
-    double single_mult, int cls_counter) {
+struct SplitSmem {
+    alignas(16) _Float16 xt[2 * SplitShape<N>::BUFH];  // X of items q, q - 1 (by parity)
+    alignas(16) _Float16 yt[2 * SplitShape<N>::BUFH];  // Y of items q - 1, q - 2
+    SplitItem wl[kSplitCap];
+    float2 wsig[kSplitCap];  // products of the item's operator scales, z of gt and of gt | 4
+    int ltab[64];
+    long long btab[64];
+};
+
+// One pair segment with FP16 split products (the passes described above);
+// nrm_s as for apply_segment.
+template <int N>
+__device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *pool_h, const double *pool_sc, int seg,
+                                              double lo_mult, double hi_mult, double single_mult, int cls_counter,
+                                              SplitSmem<N> &sm, double *nrm_s) {
     using S = SplitShape<N>;
     constexpr int N2 = S::N2, N3 = S::N3, G = S::G, NG = S::NG, NTZ = S::NTZ;
     constexpr int TWG = S::TWG, TWZ = S::TWZ, NTH = S::T, RH = S::RH, RL = S::RL, RE = S::RE, NW = S::NW;
     constexpr int ZROW = N2, DROW = N2 + 1;  // all-zero row, scratch row
-    constexpr int EXT0 = (N2 + 2) * RL;        // start of the extra chunks
-    constexpr int BUFH = (N2 + 2) * (RL + RE);  // halfs per intermediate buffer
-    __shared__ __align__(16) _Float16 xt[2 * BUFH];  // X of items q, q - 1 (by parity)
-    __shared__ __align__(16) _Float16 yt[2 * BUFH];  // Y of items q - 1, q - 2
-    __shared__ SplitItem wl[kSplitCap];
-    __shared__ float2 wsig[kSplitCap];  // products of the item's operator scales, z of gt and of gt | 4
-    __shared__ int ltab[64];
-    __shared__ long long btab[64];
-    extern __shared__ double nrm_s[];  // [term][c][l+W], 64 x nrm_stride(2 Wmax + 1)
+    constexpr int EXT0 = S::EXT0;              // start of the extra chunks
+    constexpr int BUFH = S::BUFH;              // halfs per intermediate buffer
+    _Float16 *xt = sm.xt, *yt = sm.yt;
+    SplitItem *wl = sm.wl;
+    float2 *wsig = sm.wsig;
+    int *ltab = sm.ltab;
+    long long *btab = sm.btab;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
 
-    const int seg = seg_list ? seg_list[blockIdx.x] : (int)blockIdx.x;
     const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
     const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
     const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
@@ -974,7 +1003,7 @@ __global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_e
     // acc[tt][r] = Z[iz = 4 lq + r][col = 16 t + lc] with col = ix N + iy:
     // put in (iz, iy, ix) order in LDS, then added in FP64 to the output
     // with consecutive lanes on consecutive elements
-    static_assert(N3 * sizeof(float) <= sizeof(xt), "output staging does not fit");
+    static_assert(N3 * sizeof(float) <= sizeof(sm.xt), "output staging does not fit");
     float *zs = reinterpret_cast<float *>(xt);
     auto write_out = [&](const mwg_float4 (&acc)[TWZ], int gt) {
         __syncthreads();
@@ -995,6 +1024,55 @@ __global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_e
     };
     if (napps0) write_out(acc0, j.gt);
     if (napps1) write_out(acc1, j.gt | 4);
+}
+
+// ------------------------------------------------------ persistent launches
+//
+// Each launch is a persistent grid over a queue of work entries: a workgroup
+// takes the next entry from an atomic head (zeroed before the launch), runs
+// it and takes another until the queue is empty, so every workgroup reaches
+// the exit.  The body's LDS is a static struct (compile-time LDS addresses);
+// the norm table is the dynamic LDS.
+
+// Next queue entry for the workgroup (the same value in every thread).
+__device__ __forceinline__ int next_entry(unsigned long long *head, int *s_next) {
+    // every thread is done with the previous entry (its LDS) before any
+    // thread starts the next one
+    __syncthreads();
+    if (threadIdx.x == 0) *s_next = (int)atomicAdd(head, 1ull);
+    __syncthreads();
+    // uniform: the job context derived from it stays in scalar registers
+    return __builtin_amdgcn_readfirstlane(*s_next);
+}
+
+// FP64 or FP32 per-node calculator over segments seg_list[0, n) (or segments
+// 0..n-1 without a list) in the class (lo_mult thr_cls, hi_mult thr_cls].
+template <int N, typename T>
+__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const int *seg_list, int n,
+                                                          double lo_mult, double hi_mult, int cls_counter,
+                                                          unsigned long long *head) {
+    __shared__ ApplySmem<N, T> sm;
+    extern __shared__ double nrm_s[];
+    __shared__ int s_next;
+    while (true) {
+        const int e = next_entry(head, &s_next);
+        if (e >= n) break;
+        apply_segment<N, T>(a, pool, seg_list ? seg_list[e] : e, lo_mult, hi_mult, cls_counter, sm, nrm_s);
+    }
+}
+
+// The split class of the split mode: one workgroup per pair segment of
+// seg_list (terms with bound <= tau thr_cls, FP16 split products).  It runs
+// beside the persistent FP64-class launch on the second stream.  A launch of
+// one grid per entry keeps this kernel at 128 VGPRs (4 waves per SIMD at
+// N <= 10); a persistent loop around the body, or the FP64 body in the same
+// kernel, costs registers and was measured slower.
+template <int N>
+__global__ __launch_bounds__(SplitShape<N>::T) __attribute__((amdgpu_waves_per_eu(SplitShape<N>::WPE))) void k_apply_h(
+    KArgs a, const _Float16 *pool_h, const double *pool_sc, const int *seg_list, double tau, double single_mult) {
+    __shared__ SplitSmem<N> sm;
+    extern __shared__ double nrm_s[];
+    split_segment<N>(a, pool_h, pool_sc, seg_list[blockIdx.x], 0.0, tau, single_mult, 1, sm, nrm_s);
 }
 
 // FP16 split rows of FP64 blocks (rows x N values each, row-major), data or
@@ -1052,13 +1130,20 @@ struct DevOper {
 
 struct DeviceCtx {
     int device = 0;
+    int n_cu = 0;       // compute units
+    int lds_max = 0;    // LDS bytes a workgroup may use
+    // resident workgroups per CU of a persistent launch, by (kernel, LDS bytes)
+    std::map<std::pair<const void *, size_t>, int> occupancy;
     hipStream_t stream = nullptr;
     // second stream (default priority): the FP64-class launch of the mixed
     // modes runs on it beside the low-precision launch
     hipStream_t stream2 = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr, evz = nullptr, evl = nullptr, ev2s = nullptr,
-               ev2e = nullptr;
+               ev2e = nullptr, evp = nullptr;
+    // resident operator tables by operator handle; the map is guarded by
+    // opers_mtx (an operator's destroy reaches every context from its thread)
     std::map<const void *, DevOper> opers;
+    std::mutex opers_mtx;
     // pinned staging buffers of the large device-to-host copies (allocated
     // on first use) and the events of their DMAs
     void *stage[2] = {nullptr, nullptr};
@@ -1088,10 +1173,28 @@ struct DevicePlan {
     // segment lists of the two classes, for list_tau and list_mode (the FP64
     // class lists fine segments, the low class of the split mode pair ones)
     double list_tau = -1.0;
-    int list_mode = -1;
+    int list_mode = -1, list_order = -1;
+    // tunable knobs (dev_plan_set_knobs): the split class's single-rounding
+    // cut and the order of the work queue (0: segment order, 1: largest
+    // passing-term count first)
+    double single_ratio = 16.0;
+    int order = 0;
+    std::vector<int> seg_apps, pseg_apps, fseg_apps;  // passing terms per segment
     int *list_hi = nullptr, *list_lo = nullptr;
     int n_hi = 0, n_lo = 0;
+    // post transforms in place (dev_plan_set_post): (branch, first child)
+    // pairs by depth, the filter and the node norms
+    PostPair *post_pairs = nullptr;
+    std::vector<long long> post_off;
+    double *post_F = nullptr, *post_norms = nullptr;
+    // the captured step (step_graph) and what it was captured for
+    hipGraphExec_t graph = nullptr;
+    int graph_mode = -1;
+    double graph_tau = -1.0;
+    bool graph_post = false;
+    unsigned ran_eager = 0;  // bit 2 mode + post: an eager run of that step happened
     ~DevicePlan() {
+        if (graph) (void)hipGraphExecDestroy(graph);
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (pool32) (void)hipFree(pool32);
@@ -1112,6 +1215,12 @@ int dev_create(int device, DeviceCtx **out, std::string &err) {
     HIP_TRY(hipSetDevice(device));
     auto *c = new DeviceCtx;
     c->device = device;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+        err = "device attribute query failed";
+        delete c;
+        return -2;
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -1121,6 +1230,7 @@ int dev_create(int device, DeviceCtx **out, std::string &err) {
     if (e == hipSuccess) e = hipEventCreate(&c->evl);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2s);
     if (e == hipSuccess) e = hipEventCreate(&c->ev2e);
+    if (e == hipSuccess) e = hipEventCreate(&c->evp);
     if (e != hipSuccess) {
         err = std::string("stream/event creation: ") + hipGetErrorString(e);
         delete c;
@@ -1154,6 +1264,7 @@ void dev_destroy(DeviceCtx *c) {
     if (c->evl) (void)hipEventDestroy(c->evl);
     if (c->ev2s) (void)hipEventDestroy(c->ev2s);
     if (c->ev2e) (void)hipEventDestroy(c->ev2e);
+    if (c->evp) (void)hipEventDestroy(c->evp);
     for (int b = 0; b < 2; b++) {
         if (c->stage[b]) (void)hipHostFree(c->stage[b]);
         if (c->stage_ev[b]) (void)hipEventDestroy(c->stage_ev[b]);
@@ -1226,6 +1337,7 @@ int dev_copy_d2h(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::st
 }
 
 void dev_forget_operator(DeviceCtx *c, const void *key) {
+    std::lock_guard<std::mutex> g(c->opers_mtx);
     auto it = c->opers.find(key);
     if (it == c->opers.end()) return;
     (void)hipSetDevice(c->device);
@@ -1252,6 +1364,7 @@ static int to_f32(const double *src, float **dst, long long n, hipStream_t s, st
 }
 
 static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **out, std::string &err) {
+    std::lock_guard<std::mutex> g(c->opers_mtx);
     auto it = c->opers.find(key);
     if (it != c->opers.end()) {
         *out = &it->second;
@@ -1296,6 +1409,7 @@ int dev_alloc_oper_tables(DeviceCtx *c, const OperFlat &f, DevOperTables &t, std
 
 int dev_register_operator(DeviceCtx *c, const void *key, const OperFlat &f, const DevOperTables &t, std::string &err) {
     (void)err;
+    std::lock_guard<std::mutex> g(c->opers_mtx);
     auto it = c->opers.find(key);
     if (it != c->opers.end()) {
         (void)hipStreamSynchronize(c->stream);
@@ -1317,6 +1431,7 @@ int dev_register_operator(DeviceCtx *c, const void *key, const OperFlat &f, cons
 
 int dev_download_operator_blocks(DeviceCtx *c, const void *key, double *out, long long n, std::string &err) {
     HIP_TRY(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(c->opers_mtx);
     auto it = c->opers.find(key);
     if (it == c->opers.end()) {
         err = "operator tables are not resident on this context";
@@ -1507,6 +1622,9 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         if (!rc) rc = upload_at(d_sj + o_seg, keep_i.back(), s, err);
         if (!rc) rc = upload_at(d_s0 + o_seg, pl.seg_p0, s, err);
         if (!rc) rc = upload_at(d_s1 + o_seg, pl.seg_p1, s, err);
+        p->seg_apps.insert(p->seg_apps.end(), pl.seg_apps.begin(), pl.seg_apps.end());
+        p->pseg_apps.insert(p->pseg_apps.end(), pl.pseg_apps.begin(), pl.pseg_apps.end());
+        p->fseg_apps.insert(p->fseg_apps.end(), pl.fseg_apps.begin(), pl.fseg_apps.end());
         for (size_t i = 0; i < pl.seg_job.size(); i++) {
             const double t = pl.out_thr[pl.seg_job[i] >> 3];
             p->seg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);  // as the kernel forms thr_cls
@@ -1578,6 +1696,52 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     return 0;
 }
 
+int dev_plan_set_post(DevicePlan *p, const MRA &mra,
+                      const std::vector<std::vector<std::pair<long long, long long>>> &levels, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    if (p->post_pairs) {
+        err = "post transforms already set";
+        return -1;
+    }
+    const int N = p->kp1;
+    const size_t blk = (size_t)8 * N * N * N;
+    std::vector<PostPair> pairs;
+    p->post_off.assign(1, 0);
+    for (const auto &l : levels) {
+        for (const auto &pr : l) {
+            if (pr.first < 0 || pr.first >= p->n_out || pr.second < 0 || pr.second + 8 > p->n_out) {
+                err = "post transform pair outside the plan's output nodes";
+                return -1;
+            }
+            pairs.push_back(PostPair{p->args.out_ns + (size_t)pr.first * blk, p->args.out_ns + (size_t)pr.second * blk});
+        }
+        p->post_off.push_back((long long)pairs.size());
+    }
+    int rc = plan_alloc(p, pairs.size(), &p->post_pairs, err);
+    if (!rc) rc = plan_alloc(p, mra.basis.F.size(), &p->post_F, err);
+    if (!rc) rc = plan_alloc(p, 2 * (size_t)std::max(p->n_out, 1), &p->post_norms, err);
+    if (!rc) rc = upload_at(p->post_pairs, pairs, c->stream, err);
+    if (!rc) rc = upload_at(p->post_F, mra.basis.F, c->stream, err);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dev_plan_download_norms(DevicePlan *p, double *s2, double *w2, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    if (!p->post_norms) {
+        err = "the plan has no post transforms";
+        return -1;
+    }
+    if (p->n_out == 0) return 0;
+    HIP_TRY(hipMemcpyAsync(s2, p->post_norms, sizeof(double) * p->n_out, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(w2, p->post_norms + p->n_out, sizeof(double) * p->n_out, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 void dev_plan_destroy(DevicePlan *p) {
     if (!p) return;
     (void)hipSetDevice(p->ctx->device);
@@ -1585,22 +1749,56 @@ void dev_plan_destroy(DevicePlan *p) {
     delete p;
 }
 
+// Norm table bytes at the start of a job's LDS (16-byte aligned end).
+static int nrm_bytes(int wmax) { return (int)((sizeof(double) * 64 * nrm_stride(2 * wmax + 1) + 15) / 16 * 16); }
+
+// Grid of a persistent launch: the resident workgroups of `kernel` with
+// `static_lds` bytes of static and `shmem` bytes of dynamic LDS (occupancy
+// per CU x CUs), at most n.
+static int persistent_grid(DeviceCtx *c, const void *kernel, int threads, size_t static_lds, size_t shmem, int n,
+                           int *grid, std::string &err) {
+    if (static_lds + shmem + 16 > (size_t)c->lds_max) {
+        err = "a job needs " + std::to_string(static_lds + shmem) + " bytes of LDS, more than the " +
+              std::to_string(c->lds_max) + " a workgroup may use (band too wide for this basis size)";
+        return -1;
+    }
+    auto key = std::make_pair(kernel, shmem);
+    auto it = c->occupancy.find(key);
+    int per_cu = 0;
+    if (it == c->occupancy.end()) {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, shmem));
+        c->occupancy[key] = per_cu;
+    } else {
+        per_cu = it->second;
+    }
+    if (per_cu < 1) {
+        err = "the apply kernel does not fit on a compute unit";
+        return -1;
+    }
+    *grid = std::min(n, per_cu * c->n_cu);
+    return 0;
+}
+
 template <int N, typename T>
-static int launch_nt(const KArgs &a, const T *pool, const int *list, int n_seg, int wmax, double lo, double hi, int cls,
-                     hipStream_t s, std::string &err) {
-    if (n_seg == 0) return 0;
-    const size_t shmem = sizeof(double) * 64 * nrm_stride(2 * wmax + 1);
-    hipLaunchKernelGGL((k_apply<N, T>), dim3(n_seg), dim3(JobShape<N>::T), shmem, s, a, pool, list, lo, hi, cls);
+static int launch_nt(DeviceCtx *c, const KArgs &a, const T *pool, const int *list, int n, int wmax, double lo,
+                     double hi, int cls, unsigned long long *head, hipStream_t s, std::string &err) {
+    if (n == 0) return 0;
+    const size_t shmem = nrm_bytes(wmax);
+    int grid = 0;
+    int rc = persistent_grid(c, reinterpret_cast<const void *>(&k_apply<N, T>), JobShape<N>::T,
+                             sizeof(ApplySmem<N, T>), shmem, n, &grid, err);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_apply<N, T>), dim3(grid), dim3(JobShape<N>::T), shmem, s, a, pool, list, n, lo, hi, cls, head);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
 template <typename T>
-static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n_seg, double lo, double hi, int cls,
-                    std::string &err, hipStream_t s = nullptr) {
+static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n, double lo, double hi, int cls,
+                    unsigned long long *head, std::string &err, hipStream_t s = nullptr) {
     const KArgs &a = p->args;
     if (!s) s = p->ctx->stream;
-#define MWG_LAUNCH(NN) launch_nt<NN, T>(a, pool, list, n_seg, p->wmax, lo, hi, cls, s, err)
+#define MWG_LAUNCH(NN) launch_nt<NN, T>(p->ctx, a, pool, list, n, p->wmax, lo, hi, cls, head, s, err)
     switch (p->kp1) {
         case 2: return MWG_LAUNCH(2);
         case 3: return MWG_LAUNCH(3);
@@ -1622,8 +1820,10 @@ static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n_s
 // tau * thr_cls (the FP64 class) and segments holding one at or below it.  The
 // lists are widened by a relative 1e-9 so that rounding can only add segments;
 // the kernel's own class test decides each term.
+static int drop_graph(DevicePlan *p, std::string &err);
+
 static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
-    if (p->list_tau == tau && p->list_mode == mode) return 0;
+    if (p->list_tau == tau && p->list_mode == mode && p->list_order == p->order) return 0;
     std::vector<int> hi, lo;
     for (int i = 0; i < p->n_fseg; i++)
         if (p->fseg_bmax[i] > tau * p->fseg_thr[i] * (1.0 - 1e-9)) hi.push_back(p->n_seg + p->n_pseg + i);
@@ -1633,6 +1833,20 @@ static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
     if (mode == 2)
         for (int i = 0; i < p->n_pseg; i++)
             if (p->pseg_bmin[i] <= tau * p->pseg_thr[i] * (1.0 + 1e-9)) lo.push_back(p->n_seg + i);
+    if (p->order == 1) {
+        // largest first (segment index -> its passing-term count)
+        auto apps = [&](int sg) {
+            if (sg < p->n_seg) return p->seg_apps[sg];
+            if (sg < p->n_seg + p->n_pseg) return p->pseg_apps[sg - p->n_seg];
+            return p->fseg_apps[sg - p->n_seg - p->n_pseg];
+        };
+        auto by_cost = [&](int x, int y) { return apps(x) > apps(y); };
+        std::stable_sort(hi.begin(), hi.end(), by_cost);
+        std::stable_sort(lo.begin(), lo.end(), by_cost);
+    }
+    // a captured step holds the old lists
+    int rc0 = drop_graph(p, err);
+    if (rc0) return rc0;
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
     if (p->list_hi) HIP_TRY(hipFree(p->list_hi));
     if (p->list_lo) HIP_TRY(hipFree(p->list_lo));
@@ -1645,6 +1859,7 @@ static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
     p->n_lo = (int)lo.size();
     p->list_tau = tau;
     p->list_mode = mode;
+    p->list_order = p->order;
     return 0;
 }
 
@@ -1652,6 +1867,7 @@ static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
 static int ensure_f32(DevicePlan *p, std::string &err) {
     DeviceCtx *c = p->ctx;
     bool changed = false;
+    std::lock_guard<std::mutex> g(c->opers_mtx);
     for (size_t i = 0; i < p->op_keys.size(); i++) {
         auto it = c->opers.find(p->op_keys[i]);
         if (it == c->opers.end()) {
@@ -1718,7 +1934,8 @@ static int split_rows(int N, const double *src, long long nblocks, int rows, boo
     }
 }
 
-// Split-mode items whose bound is at most kSingleRatio x thr_cls keep their
+// Split-mode items whose bound is at most single_ratio x thr_cls (default
+// kSingleRatio; a tunable knob of the plan) keep their
 // intermediates rounded once to FP16 (relative error ~1e-3 of a term bounded
 // by 16 thresholds); larger ones use hi/lo intermediates.
 constexpr double kSingleRatio = 16.0;
@@ -1728,6 +1945,7 @@ static int ensure_h16(DevicePlan *p, std::string &err) {
     DeviceCtx *c = p->ctx;
     const int N = p->kp1;
     bool changed = false;
+    std::lock_guard<std::mutex> g(c->opers_mtx);
     for (size_t i = 0; i < p->op_keys.size(); i++) {
         auto it = c->opers.find(p->op_keys[i]);
         if (it == c->opers.end()) {
@@ -1758,46 +1976,230 @@ static int ensure_h16(DevicePlan *p, std::string &err) {
 }
 
 template <int N>
-static int launch_h_n(const DevicePlan *p, const int *list, int n_seg, double lo, double hi, int cls,
-                      std::string &err) {
-    if (n_seg == 0) return 0;
-    const size_t shmem = sizeof(double) * 64 * nrm_stride(2 * p->wmax + 1);
-    hipFuncAttributes fa;
-    HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_apply_h<N>)));
-    int lds_max = 0;
-    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->ctx->device));
-    if (fa.sharedSizeBytes + shmem > (size_t)lds_max) {
+static int launch_h_n(const DevicePlan *p, double tau, std::string &err) {
+    if (p->n_lo == 0) return 0;
+    const size_t shmem = nrm_bytes(p->wmax);
+    if (sizeof(SplitSmem<N>) + shmem + 16 > (size_t)p->ctx->lds_max) {
         err = "band half-width " + std::to_string(p->wmax) + " needs more LDS than the split kernel has at k+1 = " +
               std::to_string(N) + "; use the FP64 or mixed mode";
         return -1;
     }
-    hipLaunchKernelGGL((k_apply_h<N>), dim3(n_seg), dim3(SplitShape<N>::T), shmem, p->ctx->stream, p->args,
-                       p->pool_h, p->pool_hsc, list, lo, hi, kSingleRatio, cls);
+    hipLaunchKernelGGL((k_apply_h<N>), dim3(p->n_lo), dim3(SplitShape<N>::T), shmem, p->ctx->stream, p->args,
+                       p->pool_h, p->pool_hsc, p->list_lo, tau, p->single_ratio);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-static int launch_h(const DevicePlan *p, const int *list, int n_seg, double lo, double hi, int cls, std::string &err) {
+static int launch_h(const DevicePlan *p, double tau, std::string &err) {
     switch (p->kp1) {
-        case 2: return launch_h_n<2>(p, list, n_seg, lo, hi, cls, err);
-        case 3: return launch_h_n<3>(p, list, n_seg, lo, hi, cls, err);
-        case 4: return launch_h_n<4>(p, list, n_seg, lo, hi, cls, err);
-        case 5: return launch_h_n<5>(p, list, n_seg, lo, hi, cls, err);
-        case 6: return launch_h_n<6>(p, list, n_seg, lo, hi, cls, err);
-        case 7: return launch_h_n<7>(p, list, n_seg, lo, hi, cls, err);
-        case 8: return launch_h_n<8>(p, list, n_seg, lo, hi, cls, err);
-        case 9: return launch_h_n<9>(p, list, n_seg, lo, hi, cls, err);
-        case 10: return launch_h_n<10>(p, list, n_seg, lo, hi, cls, err);
-        case 11: return launch_h_n<11>(p, list, n_seg, lo, hi, cls, err);
-        case 12: return launch_h_n<12>(p, list, n_seg, lo, hi, cls, err);
+        case 2: return launch_h_n<2>(p, tau, err);
+        case 3: return launch_h_n<3>(p, tau, err);
+        case 4: return launch_h_n<4>(p, tau, err);
+        case 5: return launch_h_n<5>(p, tau, err);
+        case 6: return launch_h_n<6>(p, tau, err);
+        case 7: return launch_h_n<7>(p, tau, err);
+        case 8: return launch_h_n<8>(p, tau, err);
+        case 9: return launch_h_n<9>(p, tau, err);
+        case 10: return launch_h_n<10>(p, tau, err);
+        case 11: return launch_h_n<11>(p, tau, err);
+        case 12: return launch_h_n<12>(p, tau, err);
         default: err = "unsupported basis size"; return -1;
     }
 }
 
-int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
-                 unsigned long long *lp_apps, std::string &err, double *split_ms) {
+// The launch sequence of one step on the context's stream(s): counters and
+// output cleared by kernels (no memset nodes, so the sequence can be
+// captured into a graph), the class launches of the mode, and with post the
+// TopDown/BottomUp transforms and the node norms.  With timed the events
+// of the per-class and post times are recorded between the launches (not
+// during a capture, where only the fork/join events of the second stream
+// are recorded).
+static int enqueue_step(DevicePlan *p, int mode, double tau, bool post, bool timed, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, p->args.counters, 8);
+    HIP_TRY(hipGetLastError());
+    {
+        const long long nz = (long long)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1;
+        const int bs = 256;
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((nz + bs - 1) / bs)), dim3(bs), 0, c->stream, p->args.out_ns, nz);
+        HIP_TRY(hipGetLastError());
+    }
+    int rc = 0;
+    const double inf = HUGE_VAL;
+    // queue heads of the persistent launches: counters[2] (first stream),
+    // counters[3] (second stream)
+    unsigned long long *head1 = p->args.counters + 2, *head2 = p->args.counters + 3;
+    if (mode == 0) {
+        // lo = 0: every term that passes screening
+        rc = launch_t<double>(p, p->pool, nullptr, p->n_seg, 0.0, inf, 0, head1, err);
+        if (rc) return rc;
+    } else if (mode == 1) {
+        // The FP64 class runs on the second stream while the FP32 class runs
+        // on the first; both only add into the zeroed output.
+        HIP_TRY(hipEventRecord(c->evz, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream2, c->evz, 0));
+        if (timed) HIP_TRY(hipEventRecord(c->ev2s, c->stream2));
+        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, head2, err, c->stream2);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
+        if (timed) HIP_TRY(hipEventRecord(c->evl, c->stream));
+        rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, head1, err);
+        if (rc) return rc;
+        if (timed) HIP_TRY(hipEventRecord(c->evm, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev2e, 0));
+    } else {
+        // the FP64 class (persistent, fine segments) on the second stream
+        // beside the split class on the first, as in the mixed mode
+        HIP_TRY(hipEventRecord(c->evz, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream2, c->evz, 0));
+        if (timed) HIP_TRY(hipEventRecord(c->ev2s, c->stream2));
+        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, head2, err, c->stream2);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
+        if (timed) HIP_TRY(hipEventRecord(c->evl, c->stream));
+        rc = launch_h(p, tau, err);
+        if (rc) return rc;
+        if (timed) HIP_TRY(hipEventRecord(c->evm, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev2e, 0));
+    }
+    // end of the calculator (both classes joined)
+    if (timed) HIP_TRY(hipEventRecord(c->evp, c->stream));
+    if (post) {
+        // TopDown/BottomUp in place on the output blocks, then the node norms
+        rc = dev_post_launch(c->stream, p->kp1, p->post_pairs, p->post_off, p->post_F, err);
+        if (!rc)
+            rc = dev_node_norms_launch(c->stream, p->kp1, p->args.out_ns, p->n_out, p->post_norms,
+                                       p->post_norms + p->n_out, err);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+static int drop_graph(DevicePlan *p, std::string &err) {
+    if (p->graph) {
+        HIP_TRY(hipGraphExecDestroy(p->graph));
+        p->graph = nullptr;
+        p->graph_mode = -1;
+    }
+    return 0;
+}
+
+int dev_plan_set_knobs(DevicePlan *p, double single_ratio, int order, std::string &err) {
+    if (!(single_ratio >= 0.0) || !std::isfinite(single_ratio) || (order != 0 && order != 1)) {
+        err = "knobs: single-rounding ratio must be finite and >= 0, order 0 or 1";
+        return -1;
+    }
+    if (single_ratio != p->single_ratio || order != p->order) {
+        HIP_TRY(hipSetDevice(p->ctx->device));
+        int rc = drop_graph(p, err);
+        if (rc) return rc;
+        p->single_ratio = single_ratio;
+        p->order = order;
+    }
+    return 0;
+}
+
+void dev_plan_knobs(const DevicePlan *p, double *single_ratio, int *order) {
+    *single_ratio = p->single_ratio;
+    *order = p->order;
+}
+
+int dev_plan_snapshot(DevicePlan *p, double **copy, std::string &err) { return dev_plan_clone_output(p, copy, err); }
+
+// Largest |a - b| and largest |b| over n doubles: per-workgroup maxima
+// (grid-stride), then one workgroup over those.
+__global__ __launch_bounds__(256) void k_max_diff(const double *a, const double *b, long long n, double *part) {
+    __shared__ double sd[256], sb[256];
+    double md = 0.0, mb = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        md = fmax(md, fabs(a[i] - b[i]));
+        mb = fmax(mb, fabs(b[i]));
+    }
+    sd[threadIdx.x] = md;
+    sb[threadIdx.x] = mb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + o]);
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sd[0];
+        part[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+int dev_plan_max_diff(DevicePlan *p, const double *ref, double *max_diff, double *max_ref, std::string &err) {
     DeviceCtx *c = p->ctx;
     HIP_TRY(hipSetDevice(c->device));
+    const long long n = (long long)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1;
+    constexpr int nb = 1024;
+    double *part = nullptr;
+    HIP_TRY(hipMalloc((void **)&part, sizeof(double) * 2 * nb));
+    hipLaunchKernelGGL(k_max_diff, dim3(nb), dim3(256), 0, c->stream, p->args.out_ns, ref, n, part);
+    std::vector<double> h(2 * nb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) {
+        err = std::string("max difference: ") + hipGetErrorString(e);
+        return -2;
+    }
+    *max_diff = *max_ref = 0.0;
+    for (int i = 0; i < nb; i++) {
+        *max_diff = std::max(*max_diff, h[2 * i]);
+        *max_ref = std::max(*max_ref, h[2 * i + 1]);
+    }
+    return 0;
+}
+
+// The step of (mode, tau, post) captured into one executable graph (made on
+// first use and kept until the plan is destroyed or another step is asked
+// for).  The first run of a step is eager, so that every lazily made table
+// and launch shape exists before the capture.
+static int step_graph(DevicePlan *p, int mode, double tau, bool post, std::string &err) {
+    DeviceCtx *c = p->ctx;
+    if (p->graph && p->graph_mode == mode && p->graph_tau == tau && p->graph_post == post) return 0;
+    int rc0 = drop_graph(p, err);
+    if (rc0) return rc0;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(p, mode, tau, post, false, err);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        err = std::string("graph capture: ") + hipGetErrorString(e);
+        return -2;
+    }
+    e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        p->graph = nullptr;
+        err = std::string("graph instantiation: ") + hipGetErrorString(e);
+        return -2;
+    }
+    p->graph_mode = mode;
+    p->graph_tau = tau;
+    p->graph_post = post;
+    return 0;
+}
+
+int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
+                 unsigned long long *lp_apps, std::string &err, double *split_ms, bool post, double *post_ms,
+                 bool graph) {
+    DeviceCtx *c = p->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    if (post && !p->post_pairs) {
+        err = "the plan has no post transforms";
+        return -1;
+    }
+    if (post_ms) *post_ms = 0.0;
     if (split_ms) split_ms[0] = split_ms[1] = 0.0;
     if (kernel_ms) *kernel_ms = 0.0;
     if (term_apps) *term_apps = 0;
@@ -1816,38 +2218,18 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
         if (!rc) rc = make_lists(p, mode, tau, err);
         if (rc) return rc;
     }
-    // counters cleared by a kernel (not a memset node, so the launch sequence
-    // can be captured into a graph)
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, p->args.counters, 8);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    {
-        const long long nz = (long long)p->n_out * 8 * p->kp1 * p->kp1 * p->kp1;
-        const int bs = 256;
-        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((nz + bs - 1) / bs)), dim3(bs), 0, c->stream, p->args.out_ns, nz);
-        HIP_TRY(hipGetLastError());
-    }
-    int rc = 0;
-    const double inf = HUGE_VAL;
-    if (mode == 0) {
-        // lo = 0: every term that passes screening
-        rc = launch_t<double>(p, p->pool, nullptr, p->n_seg, 0.0, inf, 0, err);
+    const unsigned bit = 1u << (2 * mode + (post ? 1 : 0));
+    const bool replay = graph && (p->ran_eager & bit);
+    if (replay) {
+        int rc = step_graph(p, mode, tau, post, err);
         if (rc) return rc;
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        HIP_TRY(hipGraphLaunch(p->graph, c->stream));
     } else {
-        // The FP64 class runs on the second stream while the low-precision
-        // class runs on the first; both only add into the zeroed output.
-        HIP_TRY(hipEventRecord(c->evz, c->stream));
-        HIP_TRY(hipStreamWaitEvent(c->stream2, c->evz, 0));
-        HIP_TRY(hipEventRecord(c->ev2s, c->stream2));
-        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, err, c->stream2);
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        int rc = enqueue_step(p, mode, tau, post, true, err);
         if (rc) return rc;
-        HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
-        HIP_TRY(hipEventRecord(c->evl, c->stream));
-        if (mode == 1) rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, err);
-        else rc = launch_h(p, p->list_lo, p->n_lo, 0.0, tau, 1, err);
-        if (rc) return rc;
-        HIP_TRY(hipEventRecord(c->evm, c->stream));
-        HIP_TRY(hipStreamWaitEvent(c->stream, c->ev2e, 0));
+        p->ran_eager |= bit;
     }
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     unsigned long long cnt[8];
@@ -1856,9 +2238,12 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (kernel_ms) *kernel_ms = ms;
-    if (split_ms) {
+    // per-launch times of an eager run (a replayed graph gives the whole only)
+    if (split_ms && !replay) {
         if (mode == 0) {
-            split_ms[0] = ms;
+            float mc = 0.f;
+            HIP_TRY(hipEventElapsedTime(&mc, c->ev0, c->evp));
+            split_ms[0] = mc;
         } else {
             // device time of each class launch (they overlap)
             float m1 = 0.f, m2 = 0.f;
@@ -1867,6 +2252,11 @@ int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigne
             split_ms[0] = m1;
             split_ms[1] = m2;
         }
+    }
+    if (post && post_ms && !replay) {
+        float mp = 0.f;
+        HIP_TRY(hipEventElapsedTime(&mp, c->evp, c->ev1));
+        *post_ms = mp;
     }
     if (term_apps) *term_apps = cnt[0];
     if (lp_apps) *lp_apps = cnt[1];

@@ -37,8 +37,31 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
 // receives the number of FP32 term-applications.
 // split_ms (optional, 2 values): device time of the FP64-class launch and of
 // the low-precision launch (they run concurrently, on two streams).
+// With post (after dev_plan_set_post) the post transforms and the node
+// norms run in place on the output blocks after the calculator, inside the
+// timed region; post_ms receives their device time.
+// With graph the step (zeroing, class launches, post) replays a hipGraph
+// captured on the plan for (mode, tau, post) after one eager run; a replay
+// reports the whole device time only (split_ms and post_ms stay 0).
 int dev_plan_run(DevicePlan *p, int mode, double tau, double *kernel_ms, unsigned long long *term_apps,
-                 unsigned long long *lp_apps, std::string &err, double *split_ms = nullptr);
+                 unsigned long long *lp_apps, std::string &err, double *split_ms = nullptr, bool post = false,
+                 double *post_ms = nullptr, bool graph = false);
+// Post transforms of the plan's output grid: levels[d] holds the (branch
+// node, first child) pairs at depth d as indices into the plan's output
+// nodes.  Uploads the pairs with their device addresses and the filter.
+int dev_plan_set_post(DevicePlan *p, const MRA &mra, const std::vector<std::vector<std::pair<long long, long long>>> &levels,
+                      std::string &err);
+// Tunable knobs of a plan: the split class's single-rounding cut (items with
+// bound <= single_ratio x thr_cls round their intermediates once) and the
+// work-queue order (0: segment order, 1: largest passing-term count first).
+int dev_plan_set_knobs(DevicePlan *p, double single_ratio, int order, std::string &err);
+void dev_plan_knobs(const DevicePlan *p, double *single_ratio, int *order);
+// A device copy of the plan's output blocks (freed with dev_free), and the
+// largest |output - ref| and |ref| over all output values.
+int dev_plan_snapshot(DevicePlan *p, double **copy, std::string &err);
+int dev_plan_max_diff(DevicePlan *p, const double *ref, double *max_diff, double *max_ref, std::string &err);
+// Squared norms (scaling, wavelet) of every output node after a run with post.
+int dev_plan_download_norms(DevicePlan *p, double *s2, double *w2, std::string &err);
 // Copy the raw non-standard blocks of every part, in part order
 // (sum of n_out x 8(k+1)^3), to the host.
 int dev_plan_download(DevicePlan *p, double *out_ns, std::string &err);
@@ -100,6 +123,15 @@ struct PostPair {
 // parts, deep to shallow).
 int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
                        std::string &err);
+// The same TopDown/BottomUp sequence over pairs already on the device:
+// level d's pairs are pairs[lvl_off[d], lvl_off[d + 1]); F is the two-scale
+// filter on the device.  Launches only, on `stream`.
+int dev_post_launch(void *stream, int kp1, const PostPair *pairs, const std::vector<long long> &lvl_off,
+                    const double *F, std::string &err);
+// Squared norms of the scaling and wavelet parts of blocks[0, n) into the
+// device arrays s2, w2 (a launch on `stream`).
+int dev_node_norms_launch(void *stream, int kp1, const double *blocks, long long n, double *s2, double *w2,
+                          std::string &err);
 int dev_download(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err);
 // Device-to-host copy on the context's stream, finished on return; large
 // copies go through pinned staging buffers (mw_device.hip).

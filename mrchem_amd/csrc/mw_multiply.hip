@@ -255,41 +255,125 @@ __global__ __launch_bounds__(64) void k_node_norms(const double *blocks, long lo
     }
 }
 
+// Post transforms (TopDown(add) and BottomUp of the apply's output tree).
+// One workgroup per branch node: its block (TopDown) or its 8 children's
+// scaling blocks (BottomUp) are staged in LDS, 8 N^3 doubles, and each of
+// the three directions is one in-place pass over the 4 N^2 lines of the
+// block: a thread takes a whole line (the N values of component t0 and the
+// N of t1 = t0 | bit along the direction, bit = 1 << dim), multiplies the
+// 2N-vector by the 2N x 2N filter (F for compress, F^T for reconstruct) in
+// registers and writes it back.  Every lane of a wave is at the same filter
+// element at the same time: the filter sits in LDS, row r of the pass's
+// matrix contiguous, and its reads are broadcasts.
+constexpr int kPostThreads = 256;
+
+template <int N, int DIM>
+__device__ __forceinline__ int post_index(int a, int c, int i) {
+    if constexpr (DIM == 0) return a * N + c * N * N + i;
+    else if constexpr (DIM == 1) return a + c * N * N + i * N;
+    else return a + c * N + i * N * N;
+}
+
+template <int N, int DIM>
+__device__ __forceinline__ void post_line_pass(double *blk, const double *M, int tid) {
+    constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N, bit = 1 << DIM, low = bit - 1;
+    for (int L = tid; L < 4 * NN; L += kPostThreads) {
+        const int pr = L / NN, ac = L % NN, a = ac % N, c = ac / N;
+        const int t0 = ((pr & ~low) << 1) | (pr & low), t1 = t0 | bit;
+        double v[N2];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            v[i] = blk[t0 * N3 + post_index<N, DIM>(a, c, i)];
+            v[N + i] = blk[t1 * N3 + post_index<N, DIM>(a, c, i)];
+        }
+        // the line's inputs are all in registers, so its outputs go straight
+        // back into its own places
+#pragma unroll 2
+        for (int r = 0; r < N2; r++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < N2; j++) acc = fma(M[r * N2 + j], v[j], acc);
+            const int i = r < N ? r : r - N;
+            blk[(r < N ? t0 : t1) * N3 + post_index<N, DIM>(a, c, i)] = acc;
+        }
+    }
+}
+
 // TopDown step for one level: each branch's block reconstructed into its 8
-// children's scaling parts, added to theirs.  One workgroup per branch.
-__global__ __launch_bounds__(kMulThreads) void k_topdown(int n, const PostPair *pairs, int n_pairs, const double *F,
-                                                         double *scratch) {
-    const int tid = threadIdx.x, n3 = n * n * n;
-    double *X = scratch + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
+// children's scaling parts (directions z, y, x), added to theirs.
+template <int N>
+__global__ __launch_bounds__(kPostThreads) void k_topdown(const PostPair *pairs, int n_pairs,
+                                                          const double *__restrict__ F) {
+    constexpr int N3 = N * N * N, N2 = 2 * N;
+    __shared__ double blk[8 * N3];
+    __shared__ double M[N2 * N2];  // F^T
+    const int tid = threadIdx.x;
+    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[(e % N2) * N2 + e / N2];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
-        pair_pass(n, F, pr.node, Y, 2, true, tid);
+        for (int e = tid; e < 8 * N3; e += kPostThreads) blk[e] = pr.node[e];
         __syncthreads();
-        pair_pass(n, F, Y, X, 1, true, tid);
+        post_line_pass<N, 2>(blk, M, tid);
         __syncthreads();
-        pair_pass(n, F, X, Y, 0, true, tid);
+        post_line_pass<N, 1>(blk, M, tid);
         __syncthreads();
-        for (int e = tid; e < 8 * n3; e += kMulThreads) pr.child0[(size_t)(e / n3) * 8 * n3 + e % n3] += Y[e];
+        post_line_pass<N, 0>(blk, M, tid);
+        __syncthreads();
+        for (int e = tid; e < 8 * N3; e += kPostThreads) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] += blk[e];
         __syncthreads();
     }
 }
 
 // BottomUp step for one level: each branch's [s, d1..d7] from its children's
-// scaling parts.
-__global__ __launch_bounds__(kMulThreads) void k_compress(int n, const PostPair *pairs, int n_pairs, const double *F,
-                                                          double *scratch) {
-    const int tid = threadIdx.x, n3 = n * n * n;
-    double *X = scratch + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
+// scaling parts (directions x, y, z).
+template <int N>
+__global__ __launch_bounds__(kPostThreads) void k_compress(const PostPair *pairs, int n_pairs,
+                                                           const double *__restrict__ F) {
+    constexpr int N3 = N * N * N, N2 = 2 * N;
+    __shared__ double blk[8 * N3];
+    __shared__ double M[N2 * N2];  // F
+    const int tid = threadIdx.x;
+    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[e];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
-        for (int e = tid; e < 8 * n3; e += kMulThreads) X[e] = pr.child0[(size_t)(e / n3) * 8 * n3 + e % n3];
+        for (int e = tid; e < 8 * N3; e += kPostThreads) blk[e] = pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3];
         __syncthreads();
-        pair_pass(n, F, X, Y, 0, false, tid);
+        post_line_pass<N, 0>(blk, M, tid);
         __syncthreads();
-        pair_pass(n, F, Y, X, 1, false, tid);
+        post_line_pass<N, 1>(blk, M, tid);
         __syncthreads();
-        pair_pass(n, F, X, pr.node, 2, false, tid);
+        post_line_pass<N, 2>(blk, M, tid);
         __syncthreads();
+        for (int e = tid; e < 8 * N3; e += kPostThreads) pr.node[e] = blk[e];
+        __syncthreads();
+    }
+}
+
+// One level of the TopDown (down) or BottomUp sequence.
+template <int N>
+static hipError_t post_level_n(hipStream_t s, bool down, const PostPair *pairs, int cnt, const double *F) {
+    const int g = std::min(cnt, kMulBlocks);
+    if (down)
+        hipLaunchKernelGGL((k_topdown<N>), dim3(g), dim3(kPostThreads), 0, s, pairs, cnt, F);
+    else
+        hipLaunchKernelGGL((k_compress<N>), dim3(g), dim3(kPostThreads), 0, s, pairs, cnt, F);
+    return hipGetLastError();
+}
+
+static hipError_t post_level(hipStream_t s, int n, bool down, const PostPair *pairs, int cnt, const double *F) {
+    switch (n) {
+        case 2: return post_level_n<2>(s, down, pairs, cnt, F);
+        case 3: return post_level_n<3>(s, down, pairs, cnt, F);
+        case 4: return post_level_n<4>(s, down, pairs, cnt, F);
+        case 5: return post_level_n<5>(s, down, pairs, cnt, F);
+        case 6: return post_level_n<6>(s, down, pairs, cnt, F);
+        case 7: return post_level_n<7>(s, down, pairs, cnt, F);
+        case 8: return post_level_n<8>(s, down, pairs, cnt, F);
+        case 9: return post_level_n<9>(s, down, pairs, cnt, F);
+        case 10: return post_level_n<10>(s, down, pairs, cnt, F);
+        case 11: return post_level_n<11>(s, down, pairs, cnt, F);
+        case 12: return post_level_n<12>(s, down, pairs, cnt, F);
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -315,45 +399,67 @@ int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, dou
 
 int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
                        std::string &err) {
-    size_t most = 0;
-    for (const auto &l : levels) most = std::max(most, l.size());
+    size_t most = 0, total = 0;
+    for (const auto &l : levels) {
+        most = std::max(most, l.size());
+        total += l.size();
+    }
     if (most == 0) return 0;
+    if (mra.kp1() > 12) {
+        err = "post transforms: unsupported basis size";
+        return -1;
+    }
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
-    const int n = mra.kp1(), n3 = n * n * n;
-    const int grid = (int)std::min<size_t>(most, kMulBlocks);
-    double *F = nullptr, *scratch = nullptr;
+    double *F = nullptr;
     PostPair *pairs = nullptr;
+    std::vector<PostPair> all;
+    std::vector<long long> off(1, 0);
+    all.reserve(total);
+    for (const auto &l : levels) {
+        all.insert(all.end(), l.begin(), l.end());
+        off.push_back((long long)all.size());
+    }
     hipError_t e = hipMalloc((void **)&F, sizeof(double) * mra.basis.F.size());
-    if (e == hipSuccess) e = hipMalloc((void **)&scratch, sizeof(double) * (size_t)grid * 16 * n3);
-    if (e == hipSuccess) e = hipMalloc((void **)&pairs, sizeof(PostPair) * most);
+    if (e == hipSuccess) e = hipMalloc((void **)&pairs, sizeof(PostPair) * all.size());
     if (e == hipSuccess)
         e = hipMemcpyAsync(F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size(), hipMemcpyHostToDevice, s);
-    // TopDown, shallow to deep, then BottomUp, deep to shallow; each level's
-    // pairs are uploaded before its launch, in stream order, so the buffer
-    // is rewritten only after the previous launch has read it
-    const int nl = (int)levels.size();
-    for (int step = 0; step < 2 * nl && e == hipSuccess; step++) {
-        const bool down = step < nl;
-        const int d = down ? step : 2 * nl - 1 - step;
-        const std::vector<PostPair> &l = levels[d];
-        if (l.empty()) continue;
-        e = hipMemcpyAsync(pairs, l.data(), sizeof(PostPair) * l.size(), hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) break;
-        const int g = (int)std::min<size_t>(l.size(), kMulBlocks);
-        if (down)
-            hipLaunchKernelGGL(k_topdown, dim3(g), dim3(kMulThreads), 0, s, n, pairs, (int)l.size(), F, scratch);
-        else
-            hipLaunchKernelGGL(k_compress, dim3(g), dim3(kMulThreads), 0, s, n, pairs, (int)l.size(), F, scratch);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    for (void *p : {(void *)F, (void *)scratch, (void *)pairs})
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pairs, all.data(), sizeof(PostPair) * all.size(), hipMemcpyHostToDevice, s);
+    int rc = 0;
+    if (e == hipSuccess) rc = dev_post_launch(s, mra.kp1(), pairs, off, F, err);
+    if (e == hipSuccess && rc == 0) e = hipStreamSynchronize(s);
+    for (void *p : {(void *)F, (void *)pairs})
         if (p) (void)hipFree(p);
+    if (rc) return rc;
     if (e != hipSuccess) {
         err = std::string("post transforms: ") + hipGetErrorString(e);
         return -2;
     }
+    return 0;
+}
+
+int dev_post_launch(void *stream, int kp1, const PostPair *pairs, const std::vector<long long> &lvl_off,
+                    const double *F, std::string &err) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nl = (int)lvl_off.size() - 1;
+    // TopDown shallow to deep, then BottomUp deep to shallow
+    for (int step = 0; step < 2 * nl; step++) {
+        const bool down = step < nl;
+        const int d = down ? step : 2 * nl - 1 - step;
+        const long long cnt = lvl_off[d + 1] - lvl_off[d];
+        if (cnt == 0) continue;
+        HIP_TRY(post_level(s, kp1, down, pairs + lvl_off[d], (int)cnt, F));
+    }
+    return 0;
+}
+
+int dev_node_norms_launch(void *stream, int kp1, const double *blocks, long long n, double *s2, double *w2,
+                          std::string &err) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_node_norms, dim3((unsigned)n), dim3(64), 0, static_cast<hipStream_t>(stream), blocks, n,
+                       kp1 * kp1 * kp1, s2, w2);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

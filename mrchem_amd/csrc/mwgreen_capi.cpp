@@ -9,13 +9,16 @@
 // computed by one HIP launch, and the result is finished by the TopDown(add)
 // and BottomUp transforms.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 #include <set>
 #include <string>
 #include <vector>
@@ -26,34 +29,70 @@
 
 using namespace mwg;
 
+// Knobs of a frozen-grid plan chosen by mwg_plan_tune.
+struct TunedKnobs {
+    double tau = 65536.0, single_ratio = 16.0;
+    int order = 0;
+};
 struct mwg_context_s {
     DeviceCtx *dev = nullptr;
+    // tuned knobs by (k + 1, terms of the first operator, widest band, mode):
+    // new plans with the same key start from them
+    std::map<std::tuple<int, int, int, int>, TunedKnobs> tuned;
     std::vector<double> last_stats = std::vector<double>(8, 0.0);
     int mode = MWG_MODE_FP64;
     double tau = 65536.0;    // mixed mode: FP64 for terms with bound > tau * thr
-};
-struct mwg_plan_s {
-    mwg_context ctx = nullptr;
-    DevicePlan *dev = nullptr;
-    std::vector<mwg_tree> outs;           // per apply of the batch
-    std::vector<std::vector<double>> thr;
-    std::vector<int> n_out;
-    int total_out = 0;
-    int mode = MWG_MODE_FP64;
-    double plan_ms = 0.0, slots = 0.0, gen = 0.0, host_apps = 0.0, segs = 0.0, band_w = 0.0;
-};
-struct mwg_mra_s {
-    MRA mra;
+    std::atomic<int> plan_refs{0};  // live plans on this context
+    std::string last_err;           // message of the last failed call on this context
 };
 struct mwg_operator_s {
     Operator op;
     OperFlat flat;
     std::shared_ptr<MRA> mra_keepalive;
     mwg_context dev_ctx = nullptr;  // device-built: the context holding the tables
+    std::atomic<int> plan_refs{0};  // live plans holding pointers to its device tables
 };
 struct mwg_tree_s {
     Tree tree;
     std::vector<double> apply_thr;
+    std::atomic<int> plan_refs{0};  // live plans that write their result into this tree
+};
+struct mwg_plan_s {
+    mwg_context ctx = nullptr;
+    DevicePlan *dev = nullptr;
+    std::vector<mwg_tree> outs;           // per apply of the batch
+    std::vector<mwg_operator> ops;        // per apply of the batch
+    std::vector<std::vector<NodeKey>> keys;  // output grids at plan creation
+    std::vector<std::vector<double>> thr;
+    std::vector<int> n_out;
+    int total_out = 0;
+    int mode = MWG_MODE_FP64;
+    double tau = 65536.0;  // FP64-class cut of this plan (the context's at creation, or tuned)
+    std::tuple<int, int, int, int> tune_key;
+    double plan_ms = 0.0, slots = 0.0, gen = 0.0, host_apps = 0.0, segs = 0.0, band_w = 0.0;
+    // the last execution ran the post transforms in place (mwg_plan_apply):
+    // the device holds the finished blocks and their norms
+    bool post_done = false;
+    // executions replay a captured hipGraph of the step (mwg_plan_use_graph)
+    bool use_graph = false;
+    // The plan holds device pointers into its operators' resident tables and
+    // writes into its output trees: each of them, and the context, counts the
+    // plan until it is destroyed, and refuses to be destroyed meanwhile.
+    void hold() {
+        ctx->plan_refs++;
+        for (auto *o : outs) o->plan_refs++;
+        for (auto *o : ops) o->plan_refs++;
+    }
+    ~mwg_plan_s() {
+        if (!held) return;
+        ctx->plan_refs--;
+        for (auto *o : outs) o->plan_refs--;
+        for (auto *o : ops) o->plan_refs--;
+    }
+    bool held = false;
+};
+struct mwg_mra_s {
+    MRA mra;
 };
 
 namespace {
@@ -75,6 +114,18 @@ int guarded(F &&f) {
     } catch (const std::exception &e) {
         return fail(MWG_ERR_RUNTIME, e.what());
     }
+}
+
+// Calls on a context (or a plan of it) also keep their failure message in the
+// context: mwg_context_last_error.
+int on_ctx(mwg_context ctx, int rc) {
+    if (rc != MWG_OK && ctx) ctx->last_err = g_err;
+    return rc;
+}
+
+template <typename F>
+int guarded_ctx(mwg_context ctx, F &&f) {
+    return on_ctx(ctx, guarded(std::forward<F>(f)));
 }
 
 double now_ms() {
@@ -447,6 +498,8 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
         p->segs += (double)ap.seg_job.size();
         for (int w : ap.band_w) p->band_w = std::max(p->band_w, (double)w);
         p->outs.push_back(out[i]);
+        p->ops.push_back(op[i]);
+        p->keys.push_back(out[i]->tree.keys);
         p->n_out.push_back((int)nodes[i].size());
         p->total_out += (int)nodes[i].size();
     }
@@ -454,10 +507,50 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
     std::string err;
     int rc = dev_plan_create(ctx->dev, parts, &p->dev, err);
     if (rc) return dev_fail(rc, err);
+    {
+        // (branch, first child) pairs of every output grid, by depth, as
+        // indices into the plan's concatenated output nodes
+        int maxd = 0;
+        for (int i = 0; i < n; i++) maxd = std::max(maxd, out[i]->tree.max_depth());
+        std::vector<std::vector<std::pair<long long, long long>>> levels(maxd + 1);
+        long long off = 0;
+        for (int i = 0; i < n; i++) {
+            const Tree &o = out[i]->tree;
+            for (int g = 0; g < o.size(); g++)
+                if (o.child0[g] >= 0) levels[o.keys[g].depth].emplace_back(off + g, off + o.child0[g]);
+            off += o.size();
+        }
+        rc = dev_plan_set_post(p->dev, *out[0]->tree.mra, levels, err);
+        if (rc) {
+            dev_plan_destroy(p->dev);
+            return dev_fail(rc, err);
+        }
+    }
     p->ctx = ctx;
     p->mode = mode;
+    p->tau = ctx->tau;
+    p->tune_key = std::make_tuple(op[0]->flat.kp1, op[0]->flat.M, (int)p->band_w, mode);
+    auto it = ctx->tuned.find(p->tune_key);
+    if (it != ctx->tuned.end()) {
+        p->tau = it->second.tau;
+        rc = dev_plan_set_knobs(p->dev, it->second.single_ratio, it->second.order, err);
+        if (rc) {
+            dev_plan_destroy(p->dev);
+            return dev_fail(rc, err);
+        }
+    }
+    p->hold();
+    p->held = true;
     *plan = p.release();
     return MWG_OK;
+}
+
+bool same_keys(const std::vector<NodeKey> &a, const std::vector<NodeKey> &b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); i++)
+        if (a[i].depth != b[i].depth || a[i].l[0] != b[i].l[0] || a[i].l[1] != b[i].l[1] || a[i].l[2] != b[i].l[2])
+            return false;
+    return true;
 }
 
 // A plan's result into its output trees, finished: the post transforms run
@@ -465,12 +558,32 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
 // for further executions and downloads).
 int plan_finish(mwg_plan plan) {
     for (size_t i = 0; i < plan->outs.size(); i++)
-        if (plan->outs[i]->tree.size() != plan->n_out[i])
-            return fail(MWG_ERR_ARG, "output tree changed since the plan was made");
+        if (!same_keys(plan->outs[i]->tree.keys, plan->keys[i]))
+            return fail(MWG_ERR_ARG, "output tree grid changed since the plan was made");
     const MRA &mra = *plan->outs[0]->tree.mra;
     const int blk = mra.block();
     std::string err;
     DeviceCtx *dc = plan->ctx->dev;
+    if (plan->post_done) {
+        // the blocks were transformed on the device by mwg_plan_apply
+        std::vector<double> s2(plan->total_out), w2(plan->total_out);
+        int rc = dev_plan_download_norms(plan->dev, s2.data(), w2.data(), err);
+        long long base = 0;
+        for (size_t i = 0; i < plan->outs.size() && !rc; i++) {
+            Tree &o = plan->outs[i]->tree;
+            rc = dev_plan_download_range(plan->dev, base, plan->n_out[i], o.coefs.data(), err);
+            double sq = 0.0;
+            for (int g = 0; g < o.size(); g++) {
+                sq += w2[base + g];
+                if (o.parent[g] < 0) sq += s2[base + g];
+            }
+            o.sqnorm = sq;
+            plan->outs[i]->apply_thr = plan->thr[i];
+            base += plan->n_out[i];
+        }
+        if (rc) return dev_fail(rc, err);
+        return MWG_OK;
+    }
     double *tmp = nullptr;
     int rc = dev_plan_clone_output(plan->dev, &tmp, err);
     if (rc) return dev_fail(rc, err);
@@ -517,6 +630,8 @@ extern "C" {
 
 const char *mwg_last_error(void) { return g_err.c_str(); }
 
+const char *mwg_context_last_error(mwg_context ctx) { return ctx ? ctx->last_err.c_str() : ""; }
+
 int mwg_context_create(int device, mwg_context *ctx) {
     return guarded([&] {
         if (!ctx) return fail(MWG_ERR_ARG, "null output pointer");
@@ -537,6 +652,7 @@ int mwg_context_create(int device, mwg_context *ctx) {
 
 int mwg_context_destroy(mwg_context ctx) {
     if (!ctx) return MWG_OK;
+    if (ctx->plan_refs > 0) return fail(MWG_ERR_ARG, "context destroy: a plan on this context is still alive");
     {
         std::lock_guard<std::mutex> g(g_ctx_mtx);
         g_contexts.erase(ctx);
@@ -634,7 +750,7 @@ int mwg_gaussian_operator_create(mwg_mra mra, int n_terms, const double *alpha, 
 
 int mwg_operators_create_device(mwg_context ctx, mwg_mra mra, int n, const double *mu, double build_prec,
                                 mwg_operator *ops) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         if (!ctx || !mra || !mu || !ops) return fail(MWG_ERR_ARG, "null pointer argument");
         if (n < 1) return fail(MWG_ERR_ARG, "need at least one operator");
         if (!(build_prec > 0.0 && build_prec < 1.0)) return fail(MWG_ERR_ARG, "build precision must be in (0, 1)");
@@ -822,7 +938,7 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tre
 
 int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
                  int abs_prec) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         if (!ctx || !out || !a || !b) return fail(MWG_ERR_ARG, "null pointer argument");
         if (out == a || out == b) return fail(MWG_ERR_ARG, "the output tree must differ from the inputs");
         const MRA &mra = *out->tree.mra;
@@ -911,7 +1027,7 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
 
 int mwg_add(mwg_context ctx, double prec, mwg_tree out, int n, const double *coefs, const mwg_tree *inputs,
             int max_iter, int abs_prec) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         if (!ctx || !out || !coefs || !inputs) return fail(MWG_ERR_ARG, "null pointer argument");
         if (n < 1) return fail(MWG_ERR_ARG, "at least one input tree is required");
         if (std::isnan(prec)) return fail(MWG_ERR_ARG, "bad precision");
@@ -929,7 +1045,7 @@ int mwg_add(mwg_context ctx, double prec, mwg_tree out, int n, const double *coe
 }
 
 int mwg_tree_add(mwg_context ctx, mwg_tree tree, double coef, mwg_tree inp) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         if (!ctx || !tree || !inp) return fail(MWG_ERR_ARG, "null pointer argument");
         if (tree == inp) {
             // f + c f on f's own grid
@@ -1012,6 +1128,7 @@ int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec) {
 
 int mwg_operator_destroy(mwg_operator op) {
     if (!op) return MWG_OK;
+    if (op->plan_refs > 0) return fail(MWG_ERR_ARG, "operator destroy: a plan still uses this operator");
     {
         std::lock_guard<std::mutex> g(g_ctx_mtx);
         for (auto *c : g_contexts) dev_forget_operator(c->dev, op);
@@ -1080,6 +1197,7 @@ int mwg_tree_create(mwg_mra mra, mwg_tree *out) {
 }
 
 int mwg_tree_destroy(mwg_tree t) {
+    if (t && t->plan_refs > 0) return fail(MWG_ERR_ARG, "tree destroy: a plan still writes into this tree");
     delete t;
     return MWG_OK;
 }
@@ -1355,7 +1473,7 @@ int mwg_apply_prec_trees(mwg_context ctx, double prec, mwg_tree out, mwg_operato
 }
 
 int mwg_apply_batch(mwg_context ctx, int n, const mwg_apply_desc *desc) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         if (!ctx || n < 0 || (n > 0 && !desc)) return fail(MWG_ERR_ARG, "bad arguments");
         std::vector<Job> jobs(n);
         for (int i = 0; i < n; i++) {
@@ -1380,7 +1498,7 @@ int mwg_apply_batch(mwg_context ctx, int n, const mwg_apply_desc *desc) {
 }
 
 int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         mwg_plan plan = nullptr;
         int rc = make_plan(ctx, 1, &out, &op, &inp, &thr, mode, &plan);
         if (rc) return rc;
@@ -1388,7 +1506,7 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
         std::string err;
         double ms = 0.0;
         unsigned long long apps = 0;
-        rc = dev_plan_run(plan->dev, mode, ctx->tau, &ms, &apps, nullptr, err, nullptr);
+        rc = dev_plan_run(plan->dev, mode, plan->tau, &ms, &apps, nullptr, err, nullptr);
         if (rc) return dev_fail(rc, err);
         rc = plan_finish(plan);
         if (rc) return rc;
@@ -1407,7 +1525,7 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
 
 int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp, const double *thr, int mode,
                     mwg_plan *plan) {
-    return guarded([&] {
+    return guarded_ctx(ctx, [&] {
         if (out == inp) return fail(MWG_ERR_ARG, "apply: output and input must be different trees");
         return make_plan(ctx, 1, &out, &op, &inp, &thr, mode, plan);
     });
@@ -1415,16 +1533,18 @@ int mwg_plan_create(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
 
 int mwg_plan_create_batch(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *op, const mwg_tree *inp,
                           const double *const *thr, int mode, mwg_plan *plan) {
-    return guarded([&] { return make_plan(ctx, n, out, op, inp, thr, mode, plan); });
+    return guarded_ctx(ctx, [&] { return make_plan(ctx, n, out, op, inp, thr, mode, plan); });
 }
 
 int mwg_plan_execute(mwg_plan plan, double *stats, int n) {
-    return guarded([&] {
+    return guarded_ctx(plan ? plan->ctx : nullptr, [&] {
         if (!plan) return fail(MWG_ERR_ARG, "null plan");
         std::string err;
         double ms = 0.0, split[2] = {0.0, 0.0};
         unsigned long long apps = 0, lp = 0;
-        int rc = dev_plan_run(plan->dev, plan->mode, plan->ctx->tau, &ms, &apps, &lp, err, split);
+        plan->post_done = false;
+        int rc = dev_plan_run(plan->dev, plan->mode, plan->tau, &ms, &apps, &lp, err, split, false, nullptr,
+                              plan->use_graph);
         if (rc) return dev_fail(rc, err);
         const double v[5] = {ms, (double)apps, (double)lp, split[0], split[1]};
         for (int i = 0; stats && i < n && i < 5; i++) stats[i] = v[i];
@@ -1432,16 +1552,148 @@ int mwg_plan_execute(mwg_plan plan, double *stats, int n) {
     });
 }
 
+int mwg_plan_apply(mwg_plan plan, double *stats, int n) {
+    return guarded_ctx(plan ? plan->ctx : nullptr, [&] {
+        if (!plan) return fail(MWG_ERR_ARG, "null plan");
+        std::string err;
+        double ms = 0.0, split[2] = {0.0, 0.0}, pms = 0.0;
+        unsigned long long apps = 0, lp = 0;
+        plan->post_done = false;
+        int rc = dev_plan_run(plan->dev, plan->mode, plan->tau, &ms, &apps, &lp, err, split, true, &pms,
+                              plan->use_graph);
+        if (rc) return dev_fail(rc, err);
+        plan->post_done = true;
+        const double v[6] = {ms, (double)apps, (double)lp, split[0], split[1], pms};
+        for (int i = 0; stats && i < n && i < 6; i++) stats[i] = v[i];
+        return MWG_OK;
+    });
+}
+
+int mwg_plan_tune(mwg_plan plan, double max_rel_err, int reps, double *report, int n) {
+    return guarded_ctx(plan ? plan->ctx : nullptr, [&] {
+        if (!plan) return fail(MWG_ERR_ARG, "null plan");
+        if (!(max_rel_err > 0.0) || !std::isfinite(max_rel_err) || reps < 1)
+            return fail(MWG_ERR_ARG, "tune: max_rel_err must be > 0 and reps >= 1");
+        std::string err;
+        DevicePlan *dp = plan->dev;
+        const bool lowp = plan->mode != MWG_MODE_FP64;
+        // FP64 reference of the plan on the device
+        double ms = 0.0;
+        unsigned long long apps = 0;
+        int rc = dev_plan_run(dp, MWG_MODE_FP64, plan->tau, &ms, &apps, nullptr, err);
+        double *ref = nullptr;
+        if (!rc) rc = dev_plan_snapshot(dp, &ref, err);
+        if (rc) return dev_fail(rc, err);
+        struct Free {
+            DeviceCtx *c;
+            double *p;
+            ~Free() { dev_free(c, p); }
+        } hold{plan->ctx->dev, ref};
+        // one configuration: its error against the reference (largest
+        // difference over the largest value) and its mean device time
+        int tried = 0;
+        auto measure = [&](const TunedKnobs &k, double &t_ms, double &rel) -> int {
+            int r = dev_plan_set_knobs(dp, k.single_ratio, k.order, err);
+            if (!r) r = dev_plan_run(dp, plan->mode, k.tau, &ms, &apps, nullptr, err);
+            double md = 0.0, mr = 0.0;
+            if (!r) r = dev_plan_max_diff(dp, ref, &md, &mr, err);
+            if (r) return r;
+            rel = mr > 0.0 ? md / mr : 0.0;
+            t_ms = 0.0;
+            for (int q = 0; q < reps && !r; q++) {
+                r = dev_plan_run(dp, plan->mode, k.tau, &ms, &apps, nullptr, err);
+                t_ms += ms / reps;
+            }
+            tried++;
+            return r;
+        };
+        TunedKnobs start;
+        start.tau = plan->tau;
+        dev_plan_knobs(dp, &start.single_ratio, &start.order);
+        TunedKnobs best = start;
+        double best_ms = 0.0, best_err = 0.0;
+        rc = measure(best, best_ms, best_err);
+        if (rc) return dev_fail(rc, err);
+        const double base_ms = best_ms;
+        // a starting setting outside the bound is replaced by any that meets it
+        bool found = best_err <= max_rel_err;
+        if (!found) best_ms = HUGE_VAL;
+        // coordinate search: tau (low-precision modes), then the single-
+        // rounding cut (split mode), then the queue order; a configuration
+        // counts only if its error stays within max_rel_err
+        auto consider = [&](const TunedKnobs &k) -> int {
+            double t = 0.0, e = 0.0;
+            int r = measure(k, t, e);
+            if (r) return r;
+            if (e <= max_rel_err && t < best_ms) {
+                best = k;
+                best_ms = t;
+                best_err = e;
+                found = true;
+            }
+            return 0;
+        };
+        if (lowp) {
+            for (double tau : {4096.0, 16384.0, 65536.0, 262144.0, 1048576.0, 4194304.0}) {
+                if (tau == best.tau) continue;
+                TunedKnobs k = best;
+                k.tau = tau;
+                if ((rc = consider(k))) return dev_fail(rc, err);
+            }
+        }
+        if (plan->mode == MWG_MODE_SPLIT16) {
+            for (double sr : {0.0, 4.0, 16.0, 64.0, 256.0}) {
+                if (sr == best.single_ratio) continue;
+                TunedKnobs k = best;
+                k.single_ratio = sr;
+                if ((rc = consider(k))) return dev_fail(rc, err);
+            }
+        }
+        {
+            TunedKnobs k = best;
+            k.order = 1 - best.order;
+            if ((rc = consider(k))) return dev_fail(rc, err);
+        }
+        if (!found) {
+            rc = dev_plan_set_knobs(dp, start.single_ratio, start.order, err);
+            if (rc) return dev_fail(rc, err);
+            return fail(MWG_ERR_ARG, "tune: no setting keeps the error within max_rel_err");
+        }
+        rc = dev_plan_set_knobs(dp, best.single_ratio, best.order, err);
+        if (rc) return dev_fail(rc, err);
+        plan->tau = best.tau;
+        plan->post_done = false;
+        plan->ctx->tuned[plan->tune_key] = best;
+        const double v[8] = {best.tau, best.single_ratio, (double)best.order, best_ms, base_ms, best_err,
+                             (double)tried, best_ms > 0.0 ? base_ms / best_ms : 0.0};
+        for (int i = 0; report && i < n && i < 8; i++) report[i] = v[i];
+        return MWG_OK;
+    });
+}
+
+int mwg_plan_use_graph(mwg_plan plan, int on) {
+    if (!plan) return fail(MWG_ERR_ARG, "null plan");
+    plan->use_graph = on != 0;
+    return MWG_OK;
+}
+
 int mwg_plan_finish(mwg_plan plan) {
-    return guarded([&] {
+    return guarded_ctx(plan ? plan->ctx : nullptr, [&] {
         if (!plan) return fail(MWG_ERR_ARG, "null plan");
         return plan_finish(plan);
     });
 }
 
-int mwg_plan_download_ns(mwg_plan plan, double *ns) {
-    return guarded([&] {
+int mwg_plan_download_ns(mwg_plan plan, double *ns, long long capacity) {
+    return guarded_ctx(plan ? plan->ctx : nullptr, [&] {
         if (!plan || !ns) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (plan->post_done)
+            return fail(MWG_ERR_ARG, "download_ns: the last execution ran the post transforms (mwg_plan_apply); "
+                                     "the raw non-standard blocks come from mwg_plan_execute");
+        const long long need = (long long)plan->total_out * plan->outs[0]->tree.mra->block();
+        if (capacity < need)
+            return fail(MWG_ERR_ARG, "download_ns: the buffer holds " + std::to_string(capacity) + " values, the plan has " +
+                                         std::to_string(need));
         std::string err;
         int rc = dev_plan_download(plan->dev, ns, err);
         if (rc) return dev_fail(rc, err);
@@ -1451,9 +1703,12 @@ int mwg_plan_download_ns(mwg_plan plan, double *ns) {
 
 int mwg_plan_info(mwg_plan plan, double *info, int n) {
     if (!plan || !info) return fail(MWG_ERR_ARG, "null pointer argument");
-    double v[8] = {(double)plan->total_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs,
-                   (double)plan->outs.size(), plan->band_w};
-    for (int i = 0; i < n && i < 8; i++) info[i] = v[i];
+    double sr = 0.0;
+    int order = 0;
+    dev_plan_knobs(plan->dev, &sr, &order);
+    double v[11] = {(double)plan->total_out, plan->slots, plan->gen, plan->plan_ms, plan->host_apps, plan->segs,
+                    (double)plan->outs.size(), plan->band_w, plan->tau, sr, (double)order};
+    for (int i = 0; i < n && i < 11; i++) info[i] = v[i];
     return MWG_OK;
 }
 
@@ -1467,8 +1722,9 @@ int mwg_plan_destroy(mwg_plan plan) {
 int mwg_context_set_precision(mwg_context ctx, int mode, double tau) {
     if (!ctx) return fail(MWG_ERR_ARG, "null context");
     if (mode != MWG_MODE_FP64 && mode != MWG_MODE_MIXED && mode != MWG_MODE_SPLIT16)
-        return fail(MWG_ERR_ARG, "unknown arithmetic mode");
-    if (!(tau >= 1.0) || !std::isfinite(tau)) return fail(MWG_ERR_ARG, "mixed-precision ratio must be finite and >= 1");
+        return on_ctx(ctx, fail(MWG_ERR_ARG, "unknown arithmetic mode"));
+    if (!(tau >= 1.0) || !std::isfinite(tau))
+        return on_ctx(ctx, fail(MWG_ERR_ARG, "mixed-precision ratio must be finite and >= 1"));
     ctx->mode = mode;
     ctx->tau = tau;
     return MWG_OK;

@@ -94,3 +94,15 @@ def test_operator_tables_layout():
         blk = t["blocks"][t["boff"][i]:t["boff"][i] + nb * kp1 * kp1].reshape(nb, kp1 * kp1)
         nr = t["norms"][t["noff"][i]:t["noff"][i] + nb]
         assert np.allclose(np.linalg.norm(blk, axis=1), nr, rtol=1e-12, atol=0)
+
+
+def test_cxx_caller_links():
+    """The C++ caller (tests/cxx/mwg_caller.cpp, built by build()) includes
+    include/mwgreen.hpp and links libmwgreen.so; --link-only resolves the
+    ABI entry points it calls without touching a device."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cxx", "mwg_caller")
+    assert os.path.exists(exe), "tests/cxx/mwg_caller is not built; run __graft_entry__.build()"
+    r = subprocess.run([exe, "--link-only"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "mwg_apply_batch" in r.stdout
