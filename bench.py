@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--mode", default="split16", help="the mode the headline value is quoted for")
     ap.add_argument("--tau", type=float, default=None, help="FP64 class cut (bound > tau x threshold)")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default knobs of the low-precision plans")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -188,10 +189,23 @@ def main():
     t0 = time.time()
     plans = {}
     trees = {}
+    prec = cfg["prec"]
     for m in modes:
         trees[m] = outs if m == "fp64" else [mw.FunctionTree(mra).set_nodes(*g) for g in grids]
         plans[m] = mw.ApplyPlan(trees[m], ops, inps, thr, mode=m, ctx=ctx)
     t_plan = time.time() - t0
+    # the autotuner: FP64-class cut, single-rounding cut and queue order of
+    # each low-precision plan, within 0.1 prec of the FP64 result
+    tuning = {}
+    if not args.no_tune:
+        for m in modes[1:]:
+            t0 = time.time()
+            tuning[m] = plans[m].tune(0.1 * prec, reps=3)
+            tuning[m]["tune_s"] = time.time() - t0
+            log("tuned %s: tau %g, single-rounding ratio %g, queue order %d: %.3f -> %.3f ms (%.2fx), error %.2e "
+                "(%d settings, %.1fs)" % (m, tuning[m]["tau"], tuning[m]["single_ratio"], tuning[m]["order"],
+                                         tuning[m]["start_ms"], tuning[m]["ms"], tuning[m]["speedup"],
+                                         tuning[m]["rel_err"], tuning[m]["tried"], tuning[m]["tune_s"]))
     pinfo = plans["fp64"].info()
     log("frozen plans: %d out nodes, %d input slots (%d generated), %d segments, band half-width <= %d, %.2fs" %
         (pinfo["out_nodes"], pinfo["in_slots"], pinfo["generated"], pinfo["segments"], pinfo["band_w"], t_plan))
@@ -291,7 +305,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("f16x3+f64 (FP16 hi/lo split products, FP32 accumulate; FP64 for bound > %g thr)" if head == "split16"
-                  else "f32+f64 (mixed: FP64 for bound > %g thr)") % ctx.tau,
+                  else "f32+f64 (mixed: FP64 for bound > %g thr)") % plans[head].info()["tau"],
         "data": "synthetic (SplitMix64-seeded Gaussian sums)",
         "config": {"workload": cfg["name"], "order": k, "prec": prec, "applies": n_app, "out_nodes": n_out,
                    "in_slots": pinfo["in_slots"], "terms": ops[0].size(), "path": head},
@@ -311,6 +325,7 @@ def main():
         "term_apps": apps,
         "lp_term_apps": hr["lp_term_apps"],
         "check": check,
+        "tuning": tuning,
         "setup": dict(setup, adaptive_apply_s=t_adapt, plan_s=t_plan,
                       adaptive_apply_ms={k: ast[k] for k in ("kernel_ms", "plan_ms", "device_ms", "split_ms",
                                                              "post_ms", "wall_ms")}),

@@ -35,7 +35,7 @@ namespace mwg {
 namespace {
 
 constexpr int kMulThreads = 256;
-constexpr int kMulBlocks = 1024;  // resident workgroups, each with its own scratch
+constexpr int kMulBlocks = 1024;  // resident workgroups, each with its own work area
 
 // Index of the element at position i along dim of the line (a, c) in an
 // n^3 block (x fastest): the two other indices are a (the lower direction)
@@ -116,7 +116,7 @@ __device__ void child_pass(int n, const double *F, const double *src, const int 
 
 // Scaling block of the node `dep` >= 1 levels below the source node block
 // `src` (8 components) along `path` (3 bits per level, first level lowest)
-// into X[0, n^3); X holds 4 and Y 2 blocks of scratch.
+// into X[0, n^3); X holds 4 and Y 2 blocks of work area.
 __device__ void descend(int n, const double *F, const double *src, int dep, unsigned long long path, double *X,
                         double *Y, int tid) {
     const int n3 = n * n * n;
@@ -152,7 +152,7 @@ __device__ void descend(int n, const double *F, const double *src, int dep, unsi
 
 // Scaling blocks of the 8 children of the work node, from the source node
 // block `src` (8 components) `dep` levels above it along `path`, into X
-// (Y: scratch of 8 blocks).
+// (Y: work area of 8 blocks).
 __device__ void children_of(int n, const double *F, const double *src, int dep, unsigned path, double *X, double *Y,
                             int tid) {
     const int n3 = n * n * n;
@@ -179,10 +179,10 @@ __device__ void children_of(int n, const double *F, const double *src, int dep, 
 __global__ __launch_bounds__(kMulThreads) void k_mul_nodes(int n, const MulNode *nodes, int n_nodes,
                                                            const double *pool_a, const double *pool_b,
                                                            const double *F, const double *wsc, double coef,
-                                                           double *scratch, double *out) {
+                                                           double *ws, double *out) {
     const int tid = threadIdx.x;
     const int n3 = n * n * n;
-    double *X = scratch + (size_t)blockIdx.x * 24 * n3, *Y = X + 8 * n3, *P = Y + 8 * n3;
+    double *X = ws + (size_t)blockIdx.x * 24 * n3, *Y = X + 8 * n3, *P = Y + 8 * n3;
     for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
         const MulNode nd = nodes[w];
         children_of(n, F, pool_a + nd.src_a, nd.dep_a, nd.path_a, X, Y, tid);
@@ -212,10 +212,10 @@ __global__ __launch_bounds__(kMulThreads) void k_mul_nodes(int n, const MulNode 
 // input.
 __global__ __launch_bounds__(kMulThreads) void k_add_nodes(int n, const AddSrc *src, int n_in, int n_nodes,
                                                            const double *pool, const double *coefs, const double *F,
-                                                           double *scratch, double *out) {
+                                                           double *ws, double *out) {
     const int tid = threadIdx.x;
     const int n3 = n * n * n;
-    double *X = scratch + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
+    double *X = ws + (size_t)blockIdx.x * 16 * n3, *Y = X + 8 * n3;
     for (int w = blockIdx.x; w < n_nodes; w += gridDim.x) {
         double *dst = out + (size_t)w * 8 * n3;
         for (int e = tid; e < 8 * n3; e += kMulThreads) dst[e] = 0.0;
@@ -263,8 +263,10 @@ __global__ __launch_bounds__(64) void k_node_norms(const double *blocks, long lo
 // N of t1 = t0 | bit along the direction, bit = 1 << dim), multiplies the
 // 2N-vector by the 2N x 2N filter (F for compress, F^T for reconstruct) in
 // registers and writes it back.  Every lane of a wave is at the same filter
-// element at the same time: the filter sits in LDS, row r of the pass's
-// matrix contiguous, and its reads are broadcasts.
+// element at the same time: the filter sits in LDS with column j of the
+// pass's matrix contiguous (MT[j][r] = M[r][j]), its reads are broadcasts,
+// and the 2N outputs of a line accumulate side by side (2N independent
+// chains) while the inputs v[j] are taken in turn.
 constexpr int kPostThreads = 256;
 
 template <int N, int DIM>
@@ -275,26 +277,27 @@ __device__ __forceinline__ int post_index(int a, int c, int i) {
 }
 
 template <int N, int DIM>
-__device__ __forceinline__ void post_line_pass(double *blk, const double *M, int tid) {
+__device__ __forceinline__ void post_line_pass(double *blk, const double *MT, int tid) {
     constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N, bit = 1 << DIM, low = bit - 1;
     for (int L = tid; L < 4 * NN; L += kPostThreads) {
         const int pr = L / NN, ac = L % NN, a = ac % N, c = ac / N;
         const int t0 = ((pr & ~low) << 1) | (pr & low), t1 = t0 | bit;
-        double v[N2];
+        double o[N2];
+#pragma unroll
+        for (int r = 0; r < N2; r++) o[r] = 0.0;
+        // one input at a time (read from LDS), into all 2N accumulators
+#pragma unroll 1
+        for (int j = 0; j < N2; j++) {
+            const double vj = blk[(j < N ? t0 : t1) * N3 + post_index<N, DIM>(a, c, j < N ? j : j - N)];
+#pragma unroll
+            for (int r = 0; r < N2; r++) o[r] = fma(MT[j * N2 + r], vj, o[r]);
+        }
+        // the line's inputs were all read first, so its outputs go straight
+        // back into its own places
 #pragma unroll
         for (int i = 0; i < N; i++) {
-            v[i] = blk[t0 * N3 + post_index<N, DIM>(a, c, i)];
-            v[N + i] = blk[t1 * N3 + post_index<N, DIM>(a, c, i)];
-        }
-        // the line's inputs are all in registers, so its outputs go straight
-        // back into its own places
-#pragma unroll 2
-        for (int r = 0; r < N2; r++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < N2; j++) acc = fma(M[r * N2 + j], v[j], acc);
-            const int i = r < N ? r : r - N;
-            blk[(r < N ? t0 : t1) * N3 + post_index<N, DIM>(a, c, i)] = acc;
+            blk[t0 * N3 + post_index<N, DIM>(a, c, i)] = o[i];
+            blk[t1 * N3 + post_index<N, DIM>(a, c, i)] = o[N + i];
         }
     }
 }
@@ -306,11 +309,13 @@ __global__ __launch_bounds__(kPostThreads) void k_topdown(const PostPair *pairs,
                                                           const double *__restrict__ F) {
     constexpr int N3 = N * N * N, N2 = 2 * N;
     __shared__ double blk[8 * N3];
-    __shared__ double M[N2 * N2];  // F^T
+    __shared__ double M[N2 * N2];  // transposed matrix of F^T: F itself
     const int tid = threadIdx.x;
-    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[(e % N2) * N2 + e / N2];
+    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[e];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
+        // unrolled, so that several global loads are in flight per thread
+#pragma unroll 8
         for (int e = tid; e < 8 * N3; e += kPostThreads) blk[e] = pr.node[e];
         __syncthreads();
         post_line_pass<N, 2>(blk, M, tid);
@@ -319,6 +324,7 @@ __global__ __launch_bounds__(kPostThreads) void k_topdown(const PostPair *pairs,
         __syncthreads();
         post_line_pass<N, 0>(blk, M, tid);
         __syncthreads();
+#pragma unroll 8
         for (int e = tid; e < 8 * N3; e += kPostThreads) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] += blk[e];
         __syncthreads();
     }
@@ -331,11 +337,12 @@ __global__ __launch_bounds__(kPostThreads) void k_compress(const PostPair *pairs
                                                            const double *__restrict__ F) {
     constexpr int N3 = N * N * N, N2 = 2 * N;
     __shared__ double blk[8 * N3];
-    __shared__ double M[N2 * N2];  // F
+    __shared__ double M[N2 * N2];  // transposed matrix of F: F^T
     const int tid = threadIdx.x;
-    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[e];
+    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[(e % N2) * N2 + e / N2];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
+#pragma unroll 8
         for (int e = tid; e < 8 * N3; e += kPostThreads) blk[e] = pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3];
         __syncthreads();
         post_line_pass<N, 0>(blk, M, tid);
@@ -344,6 +351,7 @@ __global__ __launch_bounds__(kPostThreads) void k_compress(const PostPair *pairs
         __syncthreads();
         post_line_pass<N, 2>(blk, M, tid);
         __syncthreads();
+#pragma unroll 8
         for (int e = tid; e < 8 * N3; e += kPostThreads) pr.node[e] = blk[e];
         __syncthreads();
     }
@@ -477,9 +485,9 @@ void dev_free(DeviceCtx *c, void *p) {
 struct DevMulSession {
     DeviceCtx *ctx = nullptr;
     int n = 0;
-    double *pool_a = nullptr, *pool_b = nullptr, *F = nullptr, *wsc = nullptr, *scratch = nullptr;
+    double *pool_a = nullptr, *pool_b = nullptr, *F = nullptr, *wsc = nullptr, *ws = nullptr;
     ~DevMulSession() {
-        for (double *p : {pool_a, pool_b, F, wsc, scratch})
+        for (double *p : {pool_a, pool_b, F, wsc, ws})
             if (p) (void)hipFree(p);
     }
 };
@@ -492,16 +500,16 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a
         err = "multiplication needs the interpolating basis";
         return -1;
     }
-    auto *m = new DevMulSession;
-    m->ctx = c;
     const int n = mra.kp1(), n3 = n * n * n;
-    m->n = n;
     std::vector<double> x(n), w(n), wsc(n3);
     const double *qx = nullptr, *qw = nullptr;
     quad_rule01(n, &qx, &qw);
     for (int k = 0; k < n; k++)
         for (int j = 0; j < n; j++)
             for (int i = 0; i < n; i++) wsc[(k * n + j) * n + i] = 1.0 / std::sqrt(qw[i] * qw[j] * qw[k]);
+    auto *m = new DevMulSession;
+    m->ctx = c;
+    m->n = n;
     auto up = [&](const double *v, size_t nv, double **d) -> int {
         HIP_TRY(hipMalloc((void **)d, sizeof(double) * std::max<size_t>(nv, 1)));
         if (nv) HIP_TRY(hipMemcpyAsync(*d, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
@@ -512,9 +520,9 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a
     if (!rc) rc = up(mra.basis.F.data(), mra.basis.F.size(), &m->F);
     if (!rc) rc = up(wsc.data(), wsc.size(), &m->wsc);
     if (!rc) {
-        hipError_t e = hipMalloc((void **)&m->scratch, sizeof(double) * (size_t)kMulBlocks * 24 * n3);
+        hipError_t e = hipMalloc((void **)&m->ws, sizeof(double) * (size_t)kMulBlocks * 24 * n3);
         if (e != hipSuccess) {
-            err = std::string("hipMalloc(scratch): ") + hipGetErrorString(e);
+            err = std::string("hipMalloc(work area): ") + hipGetErrorString(e);
             rc = -2;
         }
     }
@@ -546,7 +554,7 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
     if (e == hipSuccess) {
         const int grid = (int)std::min<size_t>(nodes.size(), kMulBlocks);
         hipLaunchKernelGGL(k_mul_nodes, dim3(grid), dim3(kMulThreads), 0, s, m->n, dn, (int)nodes.size(), m->pool_a,
-                           m->pool_b, m->F, m->wsc, coef, m->scratch, dout);
+                           m->pool_b, m->F, m->wsc, coef, m->ws, dout);
         e = hipGetLastError();
     }
     int crc = 0;
@@ -571,9 +579,9 @@ void dev_mul_end(DevMulSession *m) {
 struct DevAddSession {
     DeviceCtx *ctx = nullptr;
     int n = 0, n_in = 0;
-    double *pool = nullptr, *coefs = nullptr, *F = nullptr, *scratch = nullptr;
+    double *pool = nullptr, *coefs = nullptr, *F = nullptr, *ws = nullptr;
     ~DevAddSession() {
-        for (double *p : {pool, coefs, F, scratch})
+        for (double *p : {pool, coefs, F, ws})
             if (p) (void)hipFree(p);
     }
 };
@@ -596,9 +604,9 @@ int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool,
     if (!rc) rc = up(coefs, &m->coefs);
     if (!rc) rc = up(mra.basis.F, &m->F);
     if (!rc) {
-        hipError_t e = hipMalloc((void **)&m->scratch, sizeof(double) * (size_t)kMulBlocks * 16 * n3);
+        hipError_t e = hipMalloc((void **)&m->ws, sizeof(double) * (size_t)kMulBlocks * 16 * n3);
         if (e != hipSuccess) {
-            err = std::string("hipMalloc(scratch): ") + hipGetErrorString(e);
+            err = std::string("hipMalloc(work area): ") + hipGetErrorString(e);
             rc = -2;
         }
     }
@@ -634,7 +642,7 @@ int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_
     if (e == hipSuccess) {
         const int grid = (int)std::min<size_t>(n_nodes, kMulBlocks);
         hipLaunchKernelGGL(k_add_nodes, dim3(grid), dim3(kMulThreads), 0, s, m->n, dsrc, m->n_in, (int)n_nodes,
-                           m->pool, m->coefs, m->F, m->scratch, dout);
+                           m->pool, m->coefs, m->F, m->ws, dout);
         e = hipGetLastError();
     }
     int crc = 0;
