@@ -137,6 +137,19 @@ int mwg_tree_rescale(mwg_tree tree, double factor);
  * Interpolating basis only; out must differ from a and b. */
 int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
                  int abs_prec);
+/* mwg_multiply that allocates no device memory: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight per-pair loops, harnesses that forbid hipMalloc).  Every device
+ * buffer (both inputs' blocks, the filters, the workgroups' work areas and
+ * chunks of output nodes) is carved from one 128 MiB array in device memory
+ * that the library holds on each GPU it runs on until the process ends; no
+ * hipMalloc, hipFree or pinned host allocation is made.  Inputs whose blocks
+ * do not fit beside the work areas (about 100 MiB at k = 7, about 85 MiB at
+ * k = 11) are MWG_ERR_ARG.  The result equals mwg_multiply's bit for bit.
+ * Calls on one GPU share the array: calls that overlap on one GPU (from
+ * several threads or contexts) race and must not be made. */
+int mwg_multiply_noalloc(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b,
+                         int max_iter, int abs_prec);
 /* out = sum_i coefs[i] inputs[i]: mrcpp::add(prec, out, FunctionTreeVector,
  * max_iter, abs_prec), which MRChem calls through
  * mrcpp::cplxfunc::linear_combination for the exchange sums

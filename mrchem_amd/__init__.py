@@ -58,6 +58,7 @@ _sig("mwg_gaussian_operator_create", _p, _i, _dp, _dp, _i, ctypes.POINTER(_p))
 _sig("mwg_operator_destroy", _p)
 _sig("mwg_operators_create_device", _p, _p, _i, _dp, _d, ctypes.POINTER(_p))
 _sig("mwg_multiply", _p, _d, _p, _d, _p, _p, _i, _i)
+_sig("mwg_multiply_noalloc", _p, _d, _p, _d, _p, _p, _i, _i)
 _sig("mwg_add", _p, _d, _p, _i, _dp, ctypes.POINTER(_p), _i, _i)
 _sig("mwg_tree_add", _p, _p, _d, _p)
 _sig("mwg_tree_crop", _p, _d, _d, _i)
@@ -419,11 +420,14 @@ def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False, prec_trees=None, ct
     return out
 
 
-def multiply(prec, out, a, b, coef=1.0, max_iter=-1, abs_prec=False, ctx=None):
+def multiply(prec, out, a, b, coef=1.0, max_iter=-1, abs_prec=False, ctx=None, noalloc=False):
     """out = coef * a * b to prec on the device (mrcpp::multiply; interpolating
-    basis)."""
+    basis).  noalloc: the variant that allocates no device memory (its
+    buffers come from the library's fixed device array; inputs that do not
+    fit are an error)."""
     ctx = ctx or Context.default()
-    _check(lib.mwg_multiply(ctx._h, prec, out._h, coef, a._h, b._h, max_iter, 1 if abs_prec else 0))
+    f = lib.mwg_multiply_noalloc if noalloc else lib.mwg_multiply
+    _check(f(ctx._h, prec, out._h, coef, a._h, b._h, max_iter, 1 if abs_prec else 0))
     return out
 
 

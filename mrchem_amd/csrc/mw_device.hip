@@ -1144,6 +1144,10 @@ struct DeviceCtx {
     // opers_mtx (an operator's destroy reaches every context from its thread)
     std::map<const void *, DevOper> opers;
     std::mutex opers_mtx;
+    // device address and size of the scratch array of the allocation-free
+    // entry points (dev_noalloc_init)
+    void *noalloc = nullptr;
+    size_t noalloc_bytes = 0;
     // pinned staging buffers of the large device-to-host copies (allocated
     // on first use) and the events of their DMAs
     void *stage[2] = {nullptr, nullptr};
@@ -1234,6 +1238,10 @@ int dev_create(int device, DeviceCtx **out, std::string &err) {
     if (e != hipSuccess) {
         err = std::string("stream/event creation: ") + hipGetErrorString(e);
         delete c;
+        return -2;
+    }
+    if (dev_noalloc_init(c, err)) {
+        dev_destroy(c);
         return -2;
     }
     *out = c;
@@ -1393,6 +1401,12 @@ static int get_oper(DeviceCtx *c, const void *key, const OperFlat &f, DevOper **
 }
 
 int dev_device(DeviceCtx *c) { return c->device; }
+void dev_set_noalloc(DeviceCtx *c, void *base, size_t bytes) {
+    c->noalloc = base;
+    c->noalloc_bytes = bytes;
+}
+void *dev_noalloc_base(DeviceCtx *c) { return c->noalloc; }
+size_t dev_noalloc_bytes(DeviceCtx *c) { return c->noalloc_bytes; }
 void *dev_stream(DeviceCtx *c) { return c->stream; }
 
 int dev_alloc_oper_tables(DeviceCtx *c, const OperFlat &f, DevOperTables &t, std::string &err) {

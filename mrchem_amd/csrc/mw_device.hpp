@@ -101,8 +101,11 @@ struct MulNode {
     double hfac;
 };
 struct DevMulSession;
+// noalloc: every device buffer of the session is carved from the device's
+// scratch array (dev_noalloc_init); no device memory is allocated, and
+// inputs that do not fit are an argument error.
 int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a, const double *pool_b, size_t n_b,
-                  DevMulSession **out, std::string &err);
+                  DevMulSession **out, std::string &err, bool noalloc = false);
 // out_blocks: nodes.size() x 8(k+1)^3 on the host.
 int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,
                   std::string &err);
@@ -156,6 +159,14 @@ int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool,
 // 8(k+1)^3 on the host.
 int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err);
 void dev_add_end(DevAddSession *m);
+
+// The device's scratch array of the allocation-free entry points
+// (mw_multiply.hip): its device address, taken once per context with the
+// context's device current, and stored in the context.
+int dev_noalloc_init(DeviceCtx *c, std::string &err);
+void dev_set_noalloc(DeviceCtx *c, void *base, size_t bytes);
+void *dev_noalloc_base(DeviceCtx *c);
+size_t dev_noalloc_bytes(DeviceCtx *c);
 
 // Helpers shared by the HIP sources.
 struct DevOperTables {

@@ -482,18 +482,42 @@ void dev_free(DeviceCtx *c, void *p) {
     (void)hipFree(p);
 }
 
+// Device memory of the allocation-free product (mwg_multiply_noalloc): one
+// array on every GPU the library runs on, held until the process ends.  In
+// host code its name is a host-side stand-in: kernels and copies get its
+// device address, taken once per device by dev_noalloc_init.
+constexpr size_t kNoAllocBytes = (size_t)128 << 20;
+constexpr int kNoAllocBlocks = 64;  // workgroups of an allocation-free launch
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+int dev_noalloc_init(DeviceCtx *c, std::string &err) {
+    // the device is current (dev_create sets it)
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    dev_set_noalloc(c, ptr, kNoAllocBytes);
+    return 0;
+}
+
 struct DevMulSession {
     DeviceCtx *ctx = nullptr;
     int n = 0;
     double *pool_a = nullptr, *pool_b = nullptr, *F = nullptr, *wsc = nullptr, *ws = nullptr;
+    // allocation-free: every buffer is carved from the device's scratch
+    // array (nothing is freed); output nodes go in chunks of `chunk` nodes
+    // through dn / dout
+    bool noalloc = false;
+    size_t chunk = 0;
+    MulNode *dn = nullptr;
+    double *dout = nullptr;
     ~DevMulSession() {
+        if (noalloc) return;
         for (double *p : {pool_a, pool_b, F, wsc, ws})
             if (p) (void)hipFree(p);
     }
 };
 
 int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a, const double *pool_b, size_t n_b,
-                  DevMulSession **out, std::string &err) {
+                  DevMulSession **out, std::string &err, bool noalloc) {
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     if (mra.basis.type != BASIS_INTERPOLATING) {
@@ -507,6 +531,58 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a
     for (int k = 0; k < n; k++)
         for (int j = 0; j < n; j++)
             for (int i = 0; i < n; i++) wsc[(k * n + j) * n + i] = 1.0 / std::sqrt(qw[i] * qw[j] * qw[k]);
+    if (noalloc) {
+        // layout in the scratch array (256-byte aligned pieces): filter,
+        // point weights, both pools, the workgroups' work areas, then node
+        // descriptors and output blocks of one chunk
+        unsigned char *base = static_cast<unsigned char *>(dev_noalloc_base(c));
+        const size_t cap = dev_noalloc_bytes(c);
+        if (!base) {
+            err = "no device scratch array on this context";
+            return -1;
+        }
+        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t fixed = al(sizeof(double) * mra.basis.F.size()) + al(sizeof(double) * n3) +
+                             al(sizeof(double) * std::max<size_t>(n_a, 1)) + al(sizeof(double) * std::max<size_t>(n_b, 1)) +
+                             al(sizeof(double) * (size_t)kNoAllocBlocks * 24 * n3);
+        const size_t per_node = sizeof(MulNode) + sizeof(double) * 8 * (size_t)n3;
+        if (fixed + al(per_node) + 256 > cap) {
+            err = "mwg_multiply_noalloc: the inputs need " + std::to_string(fixed + per_node) +
+                  " bytes of device memory, more than its " + std::to_string(cap) + "-byte scratch array";
+            return -1;
+        }
+        auto *m = new DevMulSession;
+        m->ctx = c;
+        m->n = n;
+        m->noalloc = true;
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            unsigned char *p = base + off;
+            off += al(bytes);
+            return p;
+        };
+        m->F = reinterpret_cast<double *>(take(sizeof(double) * mra.basis.F.size()));
+        m->wsc = reinterpret_cast<double *>(take(sizeof(double) * n3));
+        m->pool_a = reinterpret_cast<double *>(take(sizeof(double) * std::max<size_t>(n_a, 1)));
+        m->pool_b = reinterpret_cast<double *>(take(sizeof(double) * std::max<size_t>(n_b, 1)));
+        m->ws = reinterpret_cast<double *>(take(sizeof(double) * (size_t)kNoAllocBlocks * 24 * n3));
+        m->chunk = (cap - off - 256) / per_node;
+        m->dn = reinterpret_cast<MulNode *>(take(sizeof(MulNode) * m->chunk));
+        m->dout = reinterpret_cast<double *>(base + off);
+        hipError_t e = hipMemcpyAsync(m->F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size(),
+                                      hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(m->wsc, wsc.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && n_a) e = hipMemcpyAsync(m->pool_a, pool_a, sizeof(double) * n_a, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && n_b) e = hipMemcpyAsync(m->pool_b, pool_b, sizeof(double) * n_b, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(s);
+            delete m;
+            err = std::string("upload: ") + hipGetErrorString(e);
+            return -2;
+        }
+        *out = m;
+        return 0;
+    }
     auto *m = new DevMulSession;
     m->ctx = c;
     m->n = n;
@@ -541,6 +617,22 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
     HIP_TRY(hipSetDevice(dev_device(m->ctx)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(m->ctx));
     const int n3 = m->n * m->n * m->n;
+    if (m->noalloc) {
+        // chunk by chunk through the scratch array; each output node is one
+        // workgroup's, so the blocks equal those of the allocating path
+        for (size_t c0 = 0; c0 < nodes.size(); c0 += m->chunk) {
+            const size_t cn = std::min(m->chunk, nodes.size() - c0);
+            HIP_TRY(hipMemcpyAsync(m->dn, nodes.data() + c0, sizeof(MulNode) * cn, hipMemcpyHostToDevice, s));
+            const int grid = (int)std::min<size_t>(cn, kNoAllocBlocks);
+            hipLaunchKernelGGL(k_mul_nodes, dim3(grid), dim3(kMulThreads), 0, s, m->n, m->dn, (int)cn, m->pool_a,
+                               m->pool_b, m->F, m->wsc, coef, m->ws, m->dout);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(out_blocks + c0 * 8 * n3, m->dout, sizeof(double) * cn * 8 * n3,
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        return 0;
+    }
     MulNode *dn = nullptr;
     double *dout = nullptr;
     HIP_TRY(hipMalloc((void **)&dn, sizeof(MulNode) * nodes.size()));

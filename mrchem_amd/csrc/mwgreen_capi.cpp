@@ -936,8 +936,12 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tre
 }
 }  // namespace
 
-int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
-                 int abs_prec) {
+namespace {
+// mrcpp::multiply on the device: the output grid from the union of the
+// inputs' grids, refined level by level with the split rule.  noalloc: the
+// device buffers come from the device's scratch array.
+int multiply_trees(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
+                   int abs_prec, bool noalloc) {
     return guarded_ctx(ctx, [&] {
         if (!ctx || !out || !a || !b) return fail(MWG_ERR_ARG, "null pointer argument");
         if (out == a || out == b) return fail(MWG_ERR_ARG, "the output tree must differ from the inputs");
@@ -971,7 +975,7 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
         std::string err;
         DevMulSession *m = nullptr;
         int rc = dev_mul_begin(ctx->dev, mra, a->tree.coefs.data(), a->tree.coefs.size(), b->tree.coefs.data(),
-                               b->tree.coefs.size(), &m, err);
+                               b->tree.coefs.size(), &m, err, noalloc);
         if (rc) return dev_fail(rc, err);
         const int blk = mra.block(), n3 = mra.kp1_3();
         for (int iter = 0;; iter++) {
@@ -1023,6 +1027,17 @@ int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tr
         T.sqnorm = tree_square_norm(T);
         return MWG_OK;
     });
+}
+}  // namespace
+
+int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
+                 int abs_prec) {
+    return multiply_trees(ctx, prec, out, coef, a, b, max_iter, abs_prec, false);
+}
+
+int mwg_multiply_noalloc(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b,
+                         int max_iter, int abs_prec) {
+    return multiply_trees(ctx, prec, out, coef, a, b, max_iter, abs_prec, true);
 }
 
 int mwg_add(mwg_context ctx, double prec, mwg_tree out, int n, const double *coefs, const mwg_tree *inputs,
