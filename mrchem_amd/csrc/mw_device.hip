@@ -1518,7 +1518,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         n_out += (long long)pl.out_node.size();
         n_band += (long long)pl.band.size();
         n_slot += (long long)pl.slot_off.size();
-        n_pool += (long long)pl.in_coefs.size();
+        n_pool += pl.device_pool ? pl.pool_values : (long long)pl.in_coefs.size();
         n_bw += (long long)pl.bw.size();
         n_actoff += (long long)pl.active_off.size();
         n_act += (long long)pl.active_terms.size();
@@ -1622,7 +1622,23 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         keep_l.emplace_back(pl.slot_off);
         for (auto &v : keep_l.back()) v += o_pool;
         if (!rc) rc = upload_at(d_slot_off + o_slot, keep_l.back(), s, err);
-        if (!rc) rc = upload_at(p->pool + o_pool, pl.in_coefs, s, err);
+        if (!rc) {
+            if (pl.device_pool) {
+                if (!pp.dev_pool) {
+                    err = "plan part without its device pool";
+                    rc = -1;
+                } else if (pl.pool_values > 0) {
+                    hipError_t e = hipMemcpyAsync(p->pool + o_pool, pp.dev_pool, sizeof(double) * pl.pool_values,
+                                                  hipMemcpyDeviceToDevice, s);
+                    if (e != hipSuccess) {
+                        err = std::string("pool copy: ") + hipGetErrorString(e);
+                        rc = -2;
+                    }
+                }
+            } else {
+                rc = upload_at(p->pool + o_pool, pl.in_coefs, s, err);
+            }
+        }
         if (!rc) rc = upload_at(d_norm + o_slot * 8, pl.in_norms, s, err);
         // screening tables
         if (!rc) rc = upload_at(d_bw + o_bw, pl.bw, s, err);
@@ -1672,7 +1688,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         o_out += (long long)no;
         o_band += (long long)pl.band.size();
         o_slot += (long long)pl.slot_off.size();
-        o_pool += (long long)pl.in_coefs.size();
+        o_pool += pl.device_pool ? pl.pool_values : (long long)pl.in_coefs.size();
         o_bw += (long long)pl.bw.size();
         o_actoff += (long long)pl.active_off.size();
         o_act += (long long)pl.active_terms.size();

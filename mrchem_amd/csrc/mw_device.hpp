@@ -23,6 +23,9 @@ struct PlanPart {
     const void *op_key;
     const OperFlat *flat;
     const ApplyPlan *plan;
+    // the plan's pool already on the device (dev_pool_fill), copied into the
+    // device plan; otherwise plan->in_coefs is uploaded
+    const double *dev_pool = nullptr;
 };
 
 // A plan resident in device memory: band tables, input slots and output
@@ -111,6 +114,22 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
                   std::string &err);
 void dev_mul_end(DevMulSession *m);
 
+// The pool of a plan built with device_pool (mw_multiply.hip): generated
+// nodes reconstructed from their parents and every slot's block copied from
+// the input tree's blocks on the device (inp: the input tree's coefficients,
+// 8 (k+1)^3 per node in tree order); pl.in_norms receives the slots'
+// component norms.  *pool (pl.pool_values doubles) is freed with dev_free.
+int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double **pool, std::string &err);
+// A device copy of n doubles (freed with dev_free).
+int dev_upload(DeviceCtx *c, const double *src, size_t n, double **dst, std::string &err);
+// n doubles of device memory (freed with dev_free).
+int dev_alloc(DeviceCtx *c, size_t n, double **dst, std::string &err);
+// Block i (blk doubles) of src to block dst_idx[i] of dst, on the device.
+int dev_scatter_blocks(DeviceCtx *c, const double *src, const std::vector<int> &dst_idx, int blk, double *dst,
+                       std::string &err);
+// n doubles from device memory to device memory, finished on return.
+int dev_d2d(DeviceCtx *c, double *dst, const double *src, size_t n, std::string &err);
+
 // Node blocks resident on the device (mw_multiply.hip).  Per node of
 // blocks[0, n): its scaling part's and wavelet parts' squared norms.
 int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host_s2, double *host_w2,
@@ -124,13 +143,15 @@ struct PostPair {
 // reconstruction added into its children's scaling parts, shallow to deep),
 // then BottomUp (each branch's block recomputed from its children's scaling
 // parts, deep to shallow).
+// Without topdown only the BottomUp steps run (a compressed tree from its
+// leaves' scaling parts).
 int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
-                       std::string &err);
+                       std::string &err, bool topdown = true);
 // The same TopDown/BottomUp sequence over pairs already on the device:
 // level d's pairs are pairs[lvl_off[d], lvl_off[d + 1]); F is the two-scale
 // filter on the device.  Launches only, on `stream`.
 int dev_post_launch(void *stream, int kp1, const PostPair *pairs, const std::vector<long long> &lvl_off,
-                    const double *F, std::string &err);
+                    const double *F, std::string &err, bool topdown = true);
 // Squared norms of the scaling and wavelet parts of blocks[0, n) into the
 // device arrays s2, w2 (a launch on `stream`).
 int dev_node_norms_launch(void *stream, int kp1, const double *blocks, long long n, double *s2, double *w2,

@@ -165,7 +165,9 @@ struct BigAlloc {
 };
 template <typename T>
 using UninitAlloc = BigAlloc<T, false>;
-using CoefVec = std::vector<double, BigAlloc<double, true>>;
+// Tree coefficients: a resize without a value leaves new elements
+// uninitialised (add_node fills its block with zeros explicitly).
+using CoefVec = std::vector<double, BigAlloc<double, false>>;
 
 struct Tree {
     const MRA *mra = nullptr;
@@ -173,6 +175,9 @@ struct Tree {
     std::vector<int> parent;   // -1 for roots
     std::vector<int> child0;   // first of 8 contiguous children, -1 for leaves
     CoefVec coefs;             // nodes x block; compressed [s, d1..d7] at every node
+    // structure only: add_node leaves coefs alone (an apply grows its output
+    // grid this way and sizes coefs once, for the blocks it then writes)
+    bool structure_only = false;
     std::unordered_map<std::pair<int, uint64_t>, int, KeyHash> index;
     double sqnorm = 0.0;
 
@@ -264,6 +269,15 @@ struct ApplyPlan {
     std::vector<double> fseg_bmin, fseg_bmax;
     long long term_apps = 0;   // exact count of term-applications in the plan
     double thr_floor = 0.0;    // smallest positive threshold of the plan
+    // A pool made on the device (build_plan with device_pool): in_coefs
+    // stays empty; slot_src[s] is the slot's input-tree node (>= 0) or
+    // generated node -(g + 1); generated node g is child gen_child[g] of
+    // generated node gen_parent[g] (>= 0) or of input node -(gen_parent[g]
+    // + 1), in order of depth, [gen_level_off[i], gen_level_off[i + 1]) one
+    // depth; slot_off[s] is the slot's offset in the pool as before.
+    bool device_pool = false;
+    std::vector<int> slot_src, gen_parent, gen_child, gen_level_off;
+    long long pool_values = 0;  // doubles of the pool
 };
 
 struct ApplyStats {
@@ -276,9 +290,13 @@ struct ApplyStats {
 
 // Build the plan for output nodes `nodes` of `out` with screening parameter
 // thr_param (thr_g = thr_param * prec_fac[g]).
+// With device_pool the plan stops after the slots and their sources: the
+// pool and in_norms are made on the device (dev_pool_fill), then
+// finish_plan_counts adds the exact counts and the work segments.
 void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
                 const std::vector<int> &nodes, const std::vector<double> &thr,
-                int max_band_w, int seg_target = 256);
+                int max_band_w, int seg_target = 256, bool device_pool = false);
+void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target = 256);
 double oper_norm(const Operator &op, int m, int d, int c, long long l);
 
 // Operator tables flattened for the device: per (m, d) the half-width L, the

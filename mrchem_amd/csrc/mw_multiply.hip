@@ -235,6 +235,66 @@ __global__ __launch_bounds__(kMulThreads) void k_add_nodes(int n, const AddSrc *
     }
 }
 
+// Generated input nodes of an apply plan's pool, one depth [g0, g1): node g
+// is child gen_child[g] of gen_parent[g], a generated node of a shallower
+// depth (its scaling block, in gen_s) or, for gen_parent[g] < 0, input-tree
+// node -(gen_parent[g] + 1) (its full block: scaling and wavelet parts); its
+// scaling block is the parent's reconstruction towards that child
+// (directions z, y, x).  One workgroup per node.
+__global__ __launch_bounds__(kMulThreads) void k_gen_fill(int n, const double *F, const double *inp,
+                                                          const int *gen_parent, const int *gen_child, int g0, int g1,
+                                                          double *gen_s) {
+    extern __shared__ double lds[];  // 6 n^3
+    const int tid = threadIdx.x, n3 = n * n * n;
+    double *X = lds, *Y = lds + 4 * n3;
+    for (int g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
+        const int par = gen_parent[g], t = gen_child[g];
+        double *out = gen_s + (size_t)g * n3;
+        if (par < 0) {
+            const double *src = inp + (size_t)(-par - 1) * 8 * n3;
+            const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
+            child_pass(n, F, src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
+            __syncthreads();
+            const int lo2[2] = {0, 1}, hi2[2] = {2, 3};
+            child_pass(n, F, X, lo2, hi2, 2, Y, 1, (t >> 1) & 1, tid);
+            __syncthreads();
+            const int lo1[1] = {0}, hi1[1] = {1};
+            child_pass(n, F, Y, lo1, hi1, 1, out, 0, t & 1, tid);
+        } else {
+            const double *src = gen_s + (size_t)par * n3;
+            const int lo[1] = {0}, none[1] = {-1};
+            child_pass(n, F, src, lo, none, 1, X, 2, (t >> 2) & 1, tid);
+            __syncthreads();
+            child_pass(n, F, X, lo, none, 1, Y, 1, (t >> 1) & 1, tid);
+            __syncthreads();
+            child_pass(n, F, Y, lo, none, 1, out, 0, t & 1, tid);
+        }
+        __syncthreads();
+    }
+}
+
+// The pool of an apply plan: slot s gets input-tree node slot_src[s]'s full
+// block, or generated node -(slot_src[s] + 1)'s scaling block, at
+// slot_off[s], and its 8 component norms (wavelet norms 0 for a generated
+// slot), each summed by one lane in index order.  One wave per slot.
+__global__ __launch_bounds__(64) void k_pool_fill(int n, const double *inp, const double *gen_s, const int *slot_src,
+                                                  const long long *slot_off, int n_slots, double *pool, double *norms) {
+    const int lane = threadIdx.x, n3 = n * n * n;
+    for (int s = blockIdx.x; s < n_slots; s += gridDim.x) {
+        const int src = slot_src[s];
+        double *dst = pool + slot_off[s];
+        const int len = src >= 0 ? 8 * n3 : n3;
+        const double *from = src >= 0 ? inp + (size_t)src * 8 * n3 : gen_s + (size_t)(-src - 1) * n3;
+        for (int e = lane; e < len; e += 64) dst[e] = from[e];
+        if (lane < 8) {
+            double q = 0.0;
+            if (lane * n3 < len)
+                for (int e = 0; e < n3; e++) q += from[lane * n3 + e] * from[lane * n3 + e];
+            norms[(size_t)s * 8 + lane] = sqrt(q);
+        }
+    }
+}
+
 // Squared norms of each node's scaling part and wavelet parts: one wave
 // per node.
 __global__ __launch_bounds__(64) void k_node_norms(const double *blocks, long long n, int n3, double *s2, double *w2) {
@@ -385,6 +445,120 @@ static hipError_t post_level(hipStream_t s, int n, bool down, const PostPair *pa
     }
 }
 
+int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double **pool, std::string &err) {
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    const int n = mra.kp1(), n3 = n * n * n;
+    const int ng = (int)pl.gen_parent.size(), ns = (int)pl.slot_src.size();
+    *pool = nullptr;
+    double *dpool = nullptr, *gen_s = nullptr, *F = nullptr, *norms = nullptr;
+    int *gpar = nullptr, *gch = nullptr, *ssrc = nullptr;
+    long long *soff = nullptr;
+    auto up = [&](const void *src, size_t bytes, void **dst) -> hipError_t {
+        hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 8));
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s);
+        return e;
+    };
+    hipError_t e = hipMalloc((void **)&dpool, sizeof(double) * std::max<long long>(pl.pool_values, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&gen_s, sizeof(double) * std::max<size_t>((size_t)ng * n3, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&norms, sizeof(double) * std::max<size_t>((size_t)ns * 8, 1));
+    if (e == hipSuccess) e = up(mra.basis.F.data(), sizeof(double) * mra.basis.F.size(), (void **)&F);
+    if (e == hipSuccess) e = up(pl.gen_parent.data(), sizeof(int) * ng, (void **)&gpar);
+    if (e == hipSuccess) e = up(pl.gen_child.data(), sizeof(int) * ng, (void **)&gch);
+    if (e == hipSuccess) e = up(pl.slot_src.data(), sizeof(int) * ns, (void **)&ssrc);
+    if (e == hipSuccess) e = up(pl.slot_off.data(), sizeof(long long) * ns, (void **)&soff);
+    // generated nodes depth by depth (each after its parent)
+    for (size_t lv = 0; e == hipSuccess && lv + 1 < pl.gen_level_off.size(); lv++) {
+        const int g0 = pl.gen_level_off[lv], g1 = pl.gen_level_off[lv + 1];
+        if (g1 <= g0) continue;
+        hipLaunchKernelGGL(k_gen_fill, dim3(std::min(g1 - g0, kMulBlocks)), dim3(kMulThreads),
+                           sizeof(double) * 6 * n3, s, n, F, inp, gpar, gch, g0, g1, gen_s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && ns > 0) {
+        hipLaunchKernelGGL(k_pool_fill, dim3(std::min(ns, 8192)), dim3(64), 0, s, n, inp, gen_s, ssrc, soff, ns, dpool,
+                           norms);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pl.in_norms.data(), norms, sizeof(double) * (size_t)ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void *q : {(void *)gen_s, (void *)F, (void *)norms, (void *)gpar, (void *)gch, (void *)ssrc, (void *)soff})
+        if (q) (void)hipFree(q);
+    if (e != hipSuccess) {
+        if (dpool) (void)hipFree(dpool);
+        err = std::string("pool fill: ") + hipGetErrorString(e);
+        return -2;
+    }
+    *pool = dpool;
+    return 0;
+}
+
+// Block i of src (blk doubles) to block dst_idx[i] of dst: one workgroup per
+// block.
+__global__ __launch_bounds__(256) void k_scatter_blocks(const double *src, const int *dst_idx, int n, int blk,
+                                                        double *dst) {
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const double *a = src + (size_t)i * blk;
+        double *b = dst + (size_t)dst_idx[i] * blk;
+        for (int e = threadIdx.x; e < blk; e += 256) b[e] = a[e];
+    }
+}
+
+int dev_alloc(DeviceCtx *c, size_t n, double **dst, std::string &err) {
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    *dst = nullptr;
+    HIP_TRY(hipMalloc((void **)dst, sizeof(double) * std::max<size_t>(n, 1)));
+    return 0;
+}
+
+int dev_scatter_blocks(DeviceCtx *c, const double *src, const std::vector<int> &dst_idx, int blk, double *dst,
+                       std::string &err) {
+    if (dst_idx.empty()) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    int *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(int) * dst_idx.size()));
+    hipError_t e = hipMemcpyAsync(d, dst_idx.data(), sizeof(int) * dst_idx.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_scatter_blocks, dim3((unsigned)std::min<size_t>(dst_idx.size(), 4096)), dim3(256), 0, s,
+                           src, d, (int)dst_idx.size(), blk, dst);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        err = std::string("block scatter: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+int dev_d2d(DeviceCtx *c, double *dst, const double *src, size_t n, std::string &err) {
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+int dev_upload(DeviceCtx *c, const double *src, size_t n, double **dst, std::string &err) {
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    *dst = nullptr;
+    HIP_TRY(hipMalloc((void **)dst, sizeof(double) * std::max<size_t>(n, 1)));
+    hipError_t e = n ? hipMemcpyAsync(*dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s) : hipSuccess;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipFree(*dst);
+        *dst = nullptr;
+        err = std::string("upload: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
 int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host_s2, double *host_w2,
                    std::string &err) {
     if (n <= 0) return 0;
@@ -406,7 +580,7 @@ int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, dou
 }
 
 int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vector<PostPair>> &levels,
-                       std::string &err) {
+                       std::string &err, bool topdown) {
     size_t most = 0, total = 0;
     for (const auto &l : levels) {
         most = std::max(most, l.size());
@@ -435,7 +609,7 @@ int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vect
     if (e == hipSuccess)
         e = hipMemcpyAsync(pairs, all.data(), sizeof(PostPair) * all.size(), hipMemcpyHostToDevice, s);
     int rc = 0;
-    if (e == hipSuccess) rc = dev_post_launch(s, mra.kp1(), pairs, off, F, err);
+    if (e == hipSuccess) rc = dev_post_launch(s, mra.kp1(), pairs, off, F, err, topdown);
     if (e == hipSuccess && rc == 0) e = hipStreamSynchronize(s);
     for (void *p : {(void *)F, (void *)pairs})
         if (p) (void)hipFree(p);
@@ -448,11 +622,11 @@ int dev_post_transform(DeviceCtx *c, const MRA &mra, const std::vector<std::vect
 }
 
 int dev_post_launch(void *stream, int kp1, const PostPair *pairs, const std::vector<long long> &lvl_off,
-                    const double *F, std::string &err) {
+                    const double *F, std::string &err, bool topdown) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int nl = (int)lvl_off.size() - 1;
     // TopDown shallow to deep, then BottomUp deep to shallow
-    for (int step = 0; step < 2 * nl; step++) {
+    for (int step = topdown ? 0 : nl; step < 2 * nl; step++) {
         const bool down = step < nl;
         const int d = down ? step : 2 * nl - 1 - step;
         const long long cnt = lvl_off[d + 1] - lvl_off[d];

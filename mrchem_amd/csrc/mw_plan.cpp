@@ -48,7 +48,8 @@ double oper_norm(const Operator &op, int m, int d, int c, long long l) {
 }
 
 void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
-                const std::vector<int> &nodes, const std::vector<double> &thr, int max_band_w, int seg_target) {
+                const std::vector<int> &nodes, const std::vector<double> &thr, int max_band_w, int seg_target,
+                bool device_pool) {
     const MRA &mra = *out.mra;
     const int n3 = mra.kp1_3(), blk = mra.block();
     const int M = op.size(), ndepth = op.max_depth + 1;
@@ -176,7 +177,8 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         plan.slot_off[s] = pool;
         pool += (slot_real[s] >= 0) ? blk : n3;
     }
-    plan.in_coefs.resize(pool);  // every element is written below
+    plan.pool_values = pool;
+    if (!device_pool) plan.in_coefs.resize(pool);  // every element is written below
     plan.in_norms.assign((size_t)ns * 8, 0.0);
     plan.in_depth.resize(ns);
     // Generated nodes (and the generated ancestors they descend from) are
@@ -225,6 +227,33 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     for (int s = 0; s < ns; s++)
         if (slot_real[s] < 0) slot_gen[s] = need(slot_keys[s]);
     const int ng = (int)gen_keys.size();
+    for (int s = 0; s < ns; s++) plan.n_generated += (slot_real[s] < 0);
+    if (device_pool) {
+        // the pool is made on the device (dev_pool_fill): the sources of the
+        // slots and the generated nodes in order of depth, each after its
+        // parent; in_norms and the counts follow (finish_plan_counts)
+        plan.device_pool = true;
+        for (int s = 0; s < ns; s++) plan.in_depth[s] = slot_keys[s].d;
+        std::vector<int> order(ng), rank(ng);
+        for (int i = 0; i < ng; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gen_keys[x].d < gen_keys[y].d; });
+        for (int i = 0; i < ng; i++) rank[order[i]] = i;
+        plan.gen_parent.resize(ng);
+        plan.gen_child.resize(ng);
+        plan.gen_level_off.assign(1, 0);
+        for (int i = 0; i < ng; i++) {
+            const int gi = order[i];
+            const GenKey &ch = gen_keys[gi];
+            const int par = gen_parent[gi];
+            plan.gen_parent[i] = par < 0 ? par : rank[par];
+            plan.gen_child[i] = (int)((ch.x & 1) | ((ch.y & 1) << 1) | ((ch.z & 1) << 2));
+            if (i > 0 && ch.d != gen_keys[order[i - 1]].d) plan.gen_level_off.push_back(i);
+        }
+        plan.gen_level_off.push_back(ng);
+        plan.slot_src.resize(ns);
+        for (int s = 0; s < ns; s++) plan.slot_src[s] = slot_real[s] >= 0 ? slot_real[s] : -(rank[slot_gen[s]] + 1);
+        return;
+    }
     std::vector<double, UninitAlloc<double>> gen_s((size_t)ng * n3);  // written depth by depth below
     {
         std::vector<int> order(ng);
@@ -261,7 +290,12 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         for (int e = 0; e < n3; e++) q += dst[e] * dst[e];
         plan.in_norms[(size_t)s * 8] = std::sqrt(q);
     }
-    for (int s = 0; s < ns; s++) plan.n_generated += (slot_real[s] < 0);
+    finish_plan_counts(plan, op, seg_target);
+}
+
+void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
+    const int M = op.size();
+    const std::vector<int> &nodes = plan.out_node;
 
     // Exact term-application counts per (job, band position) with the
     // kernel's rule: a candidate (term m, input component ft) of band node f
@@ -269,7 +303,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     // direction and ((N_x N_y) N_z) ||f_ft|| > thr.  Jobs are cut into
     // segments of about seg_target passing terms.
     plan.thr_floor = 0.0;
-    for (double t : thr)
+    for (double t : plan.out_thr)
         if (t > 0.0 && (plan.thr_floor == 0.0 || t < plan.thr_floor)) plan.thr_floor = t;
     const int nq = (int)nodes.size();
     std::vector<std::vector<int>> counts((size_t)nq * 8);

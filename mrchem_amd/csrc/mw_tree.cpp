@@ -67,7 +67,7 @@ int Tree::add_node(const NodeKey &k, int par) {
     keys.push_back(k);
     parent.push_back(par);
     child0.push_back(-1);
-    coefs.resize(coefs.size() + mra->block(), 0.0);
+    if (!structure_only) coefs.resize(coefs.size() + mra->block(), 0.0);
     index[{k.depth, morton(k)}] = id;
     return id;
 }

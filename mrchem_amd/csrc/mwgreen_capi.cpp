@@ -221,16 +221,51 @@ void validate_jobs(const std::vector<Job> &jobs) {
     }
 }
 
-// Build the host plans of `active`, in parallel across applies when there are
-// several (build_plan parallelises internally otherwise).
-void build_plans(std::vector<Job *> &active, const std::vector<std::vector<int>> &nodes,
-                 const std::vector<std::vector<double>> &thr) {
+// Device copies of the input trees' blocks of a batch, one per distinct
+// tree, freed with the batch.
+struct InputBlocks {
+    DeviceCtx *c = nullptr;
+    std::map<const Tree *, double *> dev;
+    ~InputBlocks() {
+        for (auto &kv : dev) dev_free(c, kv.second);
+    }
+};
+
+// Host plans of `active` (in parallel across applies when there are several;
+// build_plan parallelises internally otherwise), their pools made on the
+// device from the input trees' blocks, then the exact counts and segments.
+// parts receives the device plan parts, whose pools (pools) the caller frees
+// after dev_plan_create.
+void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<std::vector<int>> &nodes,
+                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, std::vector<double *> &pools) {
     const int n = (int)active.size();
 #pragma omp parallel for schedule(dynamic, 1) if (n > 1)
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
         try {
-            build_plan(j.plan, j.op->op, *j.inp, j.out->tree, nodes[i], thr[i], 64);
+            build_plan(j.plan, j.op->op, *j.inp, j.out->tree, nodes[i], thr[i], 64, 256, true);
+        } catch (const std::exception &e) {
+            j.err = e.what();
+        }
+    }
+    for (Job *j : active)
+        if (!j->err.empty()) throw std::runtime_error(j->err);
+    pools.assign(n, nullptr);
+    const MRA &mra = *active[0]->out->tree.mra;
+    for (int i = 0; i < n; i++) {
+        Job &j = *active[i];
+        std::string err;
+        double *&blocks = inputs.dev[j.inp];
+        int rc = 0;
+        if (!blocks) rc = dev_upload(ctx->dev, j.inp->coefs.data(), j.inp->coefs.size(), &blocks, err);
+        if (!rc) rc = dev_pool_fill(ctx->dev, mra, blocks, j.plan, &pools[i], err);
+        if (rc) throw std::runtime_error(err);
+    }
+#pragma omp parallel for schedule(dynamic, 1) if (n > 1)
+    for (int i = 0; i < n; i++) {
+        Job &j = *active[i];
+        try {
+            finish_plan_counts(j.plan, j.op->op);
         } catch (const std::exception &e) {
             j.err = e.what();
         }
@@ -238,6 +273,16 @@ void build_plans(std::vector<Job *> &active, const std::vector<std::vector<int>>
     for (Job *j : active)
         if (!j->err.empty()) throw std::runtime_error(j->err);
 }
+
+// Frees the device pools of a level's plans.
+struct LevelPools {
+    DeviceCtx *c;
+    std::vector<double *> &p;
+    ~LevelPools() {
+        for (double *q : p) dev_free(c, q);
+        p.clear();
+    }
+};
 
 // Adaptive applies with mrcpp::apply semantics per job; every refinement
 // level of every job still refining is computed in one shared device plan.
@@ -247,17 +292,38 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
     validate_jobs(jobs);
     const MRA &mra = *jobs[0].out->tree.mra;
     const int blk = mra.block();
+    // The output grids grow as structure only; their coefficients are sized
+    // once at the end and filled by the download.  A failure leaves each
+    // output as the zero function on its roots.
+    struct OutputGuard {
+        std::vector<Job> &jobs;
+        bool done = false;
+        ~OutputGuard() {
+            if (done) return;
+            for (Job &j : jobs) {
+                Tree &o = j.out->tree;
+                o.structure_only = false;
+                o.clear();
+                o.make_roots();
+                j.out->apply_thr.clear();
+            }
+        }
+    } guard{jobs};
     for (Job &j : jobs) {
         Tree &o = j.out->tree;
+        o.structure_only = false;
         o.clear();
         // room for an output grid up to twice the input's size without
-        // regrowing (untouched pages cost nothing; larger grids still grow)
+        // regrowing the structure (larger grids still grow)
         reserve_nodes(o, std::max<size_t>(64, 2 * (size_t)j.inp->size()));
+        o.structure_only = true;
         o.make_roots();
         j.work.clear();
         for (int i = 0; i < o.size(); i++) j.work.push_back(i);
         j.thr_all.assign(o.size(), 0.0);
     }
+    InputBlocks inputs;
+    inputs.c = ctx->dev;
     double kms = 0.0, pms = 0.0, slots = 0.0, gen = 0.0, computed = 0.0, apps_total = 0.0;
     double t_dev = 0.0, t_split = 0.0;  // device plan create + run + download; host split checks
     const double t_all = now_ms();
@@ -290,12 +356,15 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
                 thr[a][q] = j.prec * prec_factor(j.prec_trees, j.out->tree.keys[j.work[q]]) * std::sqrt(Nsq / M);
         }
         const double t0 = now_ms();
-        build_plans(active, nodes, thr);
+        std::vector<double *> pools;
+        LevelPools free_pools{ctx->dev, pools};
+        build_plans(ctx, active, nodes, thr, inputs, pools);
         pms += now_ms() - t0;
         std::vector<PlanPart> parts;
         size_t total = 0;
-        for (Job *j : active) {
-            parts.push_back(PlanPart{j->op, &j->op->flat, &j->plan});
+        for (size_t a = 0; a < active.size(); a++) {
+            Job *j = active[a];
+            parts.push_back(PlanPart{j->op, &j->op->flat, &j->plan, pools[a]});
             slots += (double)j->plan.slot_off.size();
             gen += (double)j->plan.n_generated;
             total += j->work.size();
@@ -362,11 +431,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
                             splits.push_back(i);
                     }
                 }
-                // at most one reallocation of the tree per level, growing
-                // geometrically (an exact reserve would copy the tree every level)
                 const size_t grown = (size_t)o.size() + 8 * splits.size();
-                if (grown * (size_t)blk > o.coefs.capacity())
-                    o.coefs.reserve(std::max(grown * (size_t)blk, 2 * o.coefs.capacity()));
                 o.keys.reserve(grown);
                 o.parent.reserve(grown);
                 o.child0.reserve(grown);
@@ -420,6 +485,8 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         // the blocks into the trees, in runs of nodes adjacent in both
         for (size_t ji = 0; ji < jobs.size() && !rc; ji++) {
             Tree &o = jobs[ji].out->tree;
+            o.structure_only = false;
+            o.coefs.resize((size_t)o.size() * blk);  // every block is written below
             const std::vector<NodeLoc> &loc = locs[ji];
             int i = 0;
             while (i < o.size() && !rc) {
@@ -439,6 +506,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         }
         if (rc) return dev_fail(rc, err);
     }
+    guard.done = true;
     auto &st = ctx->last_stats;
     st.assign(12, 0.0);
     st[0] = kms;
@@ -486,12 +554,16 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
         active.push_back(&jobs[i]);
     }
     const double t0 = now_ms();
-    build_plans(active, nodes, th);
+    InputBlocks inputs;
+    inputs.c = ctx->dev;
+    std::vector<double *> pools;
+    LevelPools free_pools{ctx->dev, pools};
+    build_plans(ctx, active, nodes, th, inputs, pools);
     p->plan_ms = now_ms() - t0;
     std::vector<PlanPart> parts;
     for (int i = 0; i < n; i++) {
         const ApplyPlan &ap = jobs[i].plan;
-        parts.push_back(PlanPart{op[i], &op[i]->flat, &ap});
+        parts.push_back(PlanPart{op[i], &op[i]->flat, &ap, pools[i]});
         p->slots += (double)ap.slot_off.size();
         p->gen += (double)ap.n_generated;
         p->host_apps += (double)ap.term_apps;
