@@ -112,7 +112,33 @@ int mwg_operator_tables(mwg_operator op, int *L, long long *boff, long long *nof
 
 /* FunctionTree<3>. */
 int mwg_tree_create(mwg_mra mra, mwg_tree *tree);
+/* Refused (MWG_ERR_ARG) while a plan writes into the tree; frees its device
+ * copy. */
 int mwg_tree_destroy(mwg_tree tree);
+/* Device-resident trees (MRCPP trees live in host memory; this keeps a
+ * tree's blocks in HBM across calls, as the exchange chain of
+ * src/qmoperators/two_electron/ExchangePotential.cpp:161-234 and the sums
+ * and crops of ExchangePotentialD1.cpp:317-323,365-371 use them).
+ * on = 1: the tree's blocks move to ctx's GPU at the first call on ctx that
+ * reads them and stay there.  Calls on ctx read them in place (mwg_apply*,
+ * mwg_multiply, mwg_add,
+ * mwg_tree_add, plans) and, for a resident output, leave the result on the
+ * device (built, transformed and normed there; only node norms and the grid
+ * come to the host).  mwg_tree_crop and mwg_tree_rescale of a tree that only
+ * the device holds run there.  Host reads (mwg_tree_get with coefficients,
+ * pack, save, dot, eval) download the blocks once, lazily; host writes
+ * (project, set, unpack, load) make the device copy stale until the next
+ * device call uploads it.  on = 0: the host copy is made current and the
+ * device copy freed.  Refused while a plan writes into the tree; a context
+ * with resident trees refuses to be destroyed. */
+int mwg_tree_set_resident(mwg_tree tree, mwg_context ctx, int on);
+/* 1 when the tree is resident on some context, else 0; *on_host / *on_device
+ * (may be NULL) say which copies are current. */
+int mwg_tree_is_resident(mwg_tree tree, int *on_host, int *on_device);
+/* Bytes of tree blocks moved host-to-device and device-to-host by calls on
+ * ctx (inputs uploaded, results and lazy host copies downloaded) since the
+ * context was made or last reset (reset = 1 zeroes them after reading). */
+int mwg_context_transfer_bytes(mwg_context ctx, long long *h2d, long long *d2h, int reset);
 /* Adaptive projection of sum_i c_i (a_i/pi)^{3/2} exp(-a_i |r - R_i|^2). */
 int mwg_tree_project_gaussians(mwg_tree tree, double prec, int abs_prec, int max_depth, int n, const double *coef,
                                const double *exponent, const double *center);
@@ -137,6 +163,17 @@ int mwg_tree_rescale(mwg_tree tree, double factor);
  * Interpolating basis only; out must differ from a and b. */
 int mwg_multiply(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
                  int abs_prec);
+/* mwg_multiply with options.  MWG_MUL_MAX_NORMS: the useMaxNorms refinement
+ * that every cplxfunc::multiply call of the exchange passes
+ * (ExchangePotential.cpp:175,213,223, last argument true).  The grid grows
+ * from the roots; a node splits when maxW_a maxS_b + maxW_b maxS_a > prec,
+ * with maxS and maxW the square roots of the largest 2^{3n} ||block||^2 and
+ * 2^{3n} ||wavelet part||^2 over the input's node and its descendants (n the
+ * node's scale; below an input's finest node its scaling value with maxW = 0).
+ * abs_prec is unused with it (the rule is absolute). */
+#define MWG_MUL_MAX_NORMS 1
+int mwg_multiply_ex(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
+                    int abs_prec, int flags);
 /* mwg_multiply that allocates no device memory: no counterpart in the
  * reference; for callers that cannot allocate (code under HIP graph capture,
  * tight per-pair loops, harnesses that forbid hipMalloc).  Every device
@@ -229,7 +266,10 @@ int mwg_apply_fixed(mwg_context ctx, mwg_tree out, mwg_operator op, mwg_tree inp
  * that passed screening (each 6(k+1)^4 flops of algorithmic work),
  * wall milliseconds of [7] device plan creation, runs and downloads,
  * [8] the host split checks, [9] the post transforms (TopDown, BottomUp,
- * norms), [10] the whole apply. */
+ * norms), [10] the whole apply, of the parts of [7]: [11] device plan
+ * creation, [12] runs, and of the parts of [4]: [13] bands and slots,
+ * [14] input pools on the device, [15] exact counts and segments (adaptive
+ * applies; frozen-grid calls leave [11..15] 0). */
 int mwg_last_apply_stats(mwg_context ctx, double *stats, int n);
 
 /* Frozen-grid apply split into a resident device plan (bands, input slots,

@@ -79,6 +79,13 @@ class Context : public Handle<mwg_context, mwg_context_destroy> {
         h_ = c;
     }
     void set_precision(int mode, double tau = 65536.0) { check(mwg_context_set_precision(h_, mode, tau)); }
+    // (host-to-device, device-to-host) bytes of tree blocks moved by calls on
+    // this context; reset zeroes the counters after reading them.
+    std::pair<long long, long long> transfer_bytes(bool reset = false) const {
+        long long up = 0, down = 0;
+        check(mwg_context_transfer_bytes(h_, &up, &down, reset ? 1 : 0));
+        return {up, down};
+    }
 };
 
 // MultiResolutionAnalysis<3> (src/mrenv.cpp:99-127).
@@ -159,6 +166,9 @@ class Tree : public Handle<mwg_tree, mwg_tree_destroy> {
         return s;
     }
     void rescale(double f) { check(mwg_tree_rescale(h_, f)); }
+    // Keep the blocks in HBM of ctx's GPU across calls (false: back to host
+    // memory); the context must outlive the residency.
+    void set_resident(const Context &ctx, bool on = true) { check(mwg_tree_set_resident(h_, ctx.get(), on ? 1 : 0)); }
     void project_gaussians(double prec, bool abs_prec, const std::vector<double> &coef,
                            const std::vector<double> &exponent, const std::vector<double> &center) {
         check(mwg_tree_project_gaussians(h_, prec, abs_prec ? 1 : 0, -1, (int)coef.size(), coef.data(),

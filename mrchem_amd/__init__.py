@@ -59,6 +59,7 @@ _sig("mwg_operator_destroy", _p)
 _sig("mwg_operators_create_device", _p, _p, _i, _dp, _d, ctypes.POINTER(_p))
 _sig("mwg_multiply", _p, _d, _p, _d, _p, _p, _i, _i)
 _sig("mwg_multiply_noalloc", _p, _d, _p, _d, _p, _p, _i, _i)
+_sig("mwg_multiply_ex", _p, _d, _p, _d, _p, _p, _i, _i, _i)
 _sig("mwg_add", _p, _d, _p, _i, _dp, ctypes.POINTER(_p), _i, _i)
 _sig("mwg_tree_add", _p, _p, _d, _p)
 _sig("mwg_tree_crop", _p, _d, _d, _i)
@@ -73,6 +74,9 @@ _sig("mwg_operator_table_sizes", _p, _llp, _llp)
 _sig("mwg_operator_tables", _p, _ip, _llp, _llp, _dp, _dp)
 _sig("mwg_tree_create", _p, ctypes.POINTER(_p))
 _sig("mwg_tree_destroy", _p)
+_sig("mwg_tree_set_resident", _p, _p, _i)
+_sig("mwg_tree_is_resident", _p, _ip, _ip)
+_sig("mwg_context_transfer_bytes", _p, _llp, _llp, _i)
 _sig("mwg_tree_project_gaussians", _p, _d, _i, _i, _i, _dp, _dp, _dp)
 _sig("mwg_tree_size", _p, _ip)
 _sig("mwg_tree_get", _p, _ip, _llp, _dp)
@@ -154,12 +158,20 @@ class Context:
         """Message of the last failed call on this context ("" if none)."""
         return lib.mwg_context_last_error(self._h).decode()
 
+    def transfer_bytes(self, reset=False):
+        """(host-to-device, device-to-host) bytes of tree blocks moved by calls
+        on this context since it was made or last reset."""
+        a, b = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        _check(lib.mwg_context_transfer_bytes(self._h, ctypes.byref(a), ctypes.byref(b), int(reset)))
+        return a.value, b.value
+
     def last_stats(self):
-        s = np.zeros(11)
-        _check(lib.mwg_last_apply_stats(self._h, _ptr(s, _d), 11))
+        s = np.zeros(16)
+        _check(lib.mwg_last_apply_stats(self._h, _ptr(s, _d), 16))
         return {"kernel_ms": s[0], "out_nodes": int(s[1]), "in_slots": int(s[2]), "generated": int(s[3]),
                 "plan_ms": s[4], "launches": int(s[5]), "term_apps": int(s[6]), "device_ms": s[7],
-                "split_ms": s[8], "post_ms": s[9], "wall_ms": s[10]}
+                "split_ms": s[8], "post_ms": s[9], "wall_ms": s[10], "device_create_ms": s[11],
+                "device_run_ms": s[12], "plan_grid_ms": s[13], "plan_pool_ms": s[14], "plan_counts_ms": s[15]}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -361,6 +373,22 @@ class FunctionTree:
         _check(lib.mwg_tree_rescale(self._h, f))
         return self
 
+    def set_resident(self, ctx=None, on=True):
+        """Keep the tree's blocks in HBM of ctx's GPU across calls (on=False:
+        back to host memory).  Device calls on ctx read and write them there;
+        host reads download them lazily."""
+        ctx = ctx or Context.default()
+        _check(lib.mwg_tree_set_resident(self._h, ctx._h, int(on)))
+        # the context must outlive the device copy
+        self._ctx = ctx if on else None
+        return self
+
+    def residency(self):
+        """(resident, host copy current, device copy current)."""
+        a, b = _i(), _i()
+        r = lib.mwg_tree_is_resident(self._h, ctypes.byref(a), ctypes.byref(b))
+        return bool(r), bool(a.value), bool(b.value)
+
     def add(self, c, inp, ctx=None):
         """self += c * inp on the union of the grids (FunctionTree::add; the
         node sums run on the device)."""
@@ -420,12 +448,18 @@ def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False, prec_trees=None, ct
     return out
 
 
-def multiply(prec, out, a, b, coef=1.0, max_iter=-1, abs_prec=False, ctx=None, noalloc=False):
+def multiply(prec, out, a, b, coef=1.0, max_iter=-1, abs_prec=False, ctx=None, noalloc=False, use_max_norms=False):
     """out = coef * a * b to prec on the device (mrcpp::multiply; interpolating
     basis).  noalloc: the variant that allocates no device memory (its
     buffers come from the library's fixed device array; inputs that do not
-    fit are an error)."""
+    fit are an error).  use_max_norms: the max-norm refinement from the
+    roots (useMaxNorms)."""
     ctx = ctx or Context.default()
+    if use_max_norms:
+        if noalloc:
+            raise ValueError("use_max_norms has no allocation-free form")
+        _check(lib.mwg_multiply_ex(ctx._h, prec, out._h, coef, a._h, b._h, max_iter, 1 if abs_prec else 0, 1))
+        return out
     f = lib.mwg_multiply_noalloc if noalloc else lib.mwg_multiply
     _check(f(ctx._h, prec, out._h, coef, a._h, b._h, max_iter, 1 if abs_prec else 0))
     return out

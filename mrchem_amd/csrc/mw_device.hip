@@ -233,10 +233,10 @@ __device__ __forceinline__ void stage_terms(const KArgs &a, const AppDesc &ad, c
 // (bound > thr) and fall in the launch's class (bound > thr_lo and not
 // bound > thr_hi).  With PAIR the job stands for the output components gt
 // and gt | 4 (gt < 4), which share N_x and N_y: both bounds are formed and a
-// term is appended when either passes.  emit(pos, p, ft, lx, ly, lz, b0, b1,
-// t0, t1) is called by each appending lane with its worklist position, the
-// bounds for gt and gt | 4 and which of them pass (b1, t1 unused without
-// PAIR).  Stops before a pair that could overflow the worklist (cap items)
+// term is appended when either passes.  emit(pos, p, ft, s, lx, ly, lz, b0,
+// b1, t0, t1) is called by each appending lane with its worklist position,
+// the band node's input slot s, the bounds for gt and gt | 4 and which of
+// them pass (b1, t1 unused without PAIR).  Stops before a pair that could overflow the worklist (cap items)
 // and leaves pi there.  Returns the item count; apps0, apps1 count the
 // passing term-applications of gt and gt | 4.
 template <bool PAIR, typename Emit>
@@ -300,7 +300,7 @@ __device__ __forceinline__ int screen_scan(const KArgs &a, const AppDesc &ad, co
                 apps0 += __popcll(mask);
             }
             if (t0 || t1)
-                emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, ll[0], ll[1], ll[2], b0, b1, t0, t1);
+                emit(n + __popcll(mask & ((1ull << lane) - 1ull)), p, ft, s, ll[0], ll[1], ll[2], b0, b1, t0, t1);
             n += __popcll(mask);
         }
         pi += 64;
@@ -406,7 +406,8 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
             // screening into the worklist (wave 0 writes the items)
             unsigned long long n1 = 0;
             const int n = screen_scan<false>(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane, napps, n1,
-                                             [&](int pos, int p, int ft, int, int, int, double, double, bool, bool) {
+                                             [&](int pos, int p, int ft, int, int, int, int, double, double, bool,
+                                                 bool) {
                                                  if (wv == 0) wl[pos] = (p << 9) | (ft << 6) | lane;
                                              });
             __syncthreads();
@@ -646,7 +647,7 @@ __device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const
 // passes for gt, gt | 4; kSingleBit when the item's intermediates are
 // rounded once (no lo part).
 struct SplitItem {
-    int bx, by, bz0, bz1, grp;
+    int bx, by, bz0, bz1, grp, cb;
 };
 constexpr int kZ0Bit = 28, kZ1Bit = 29, kSingleBit = 30;
 
@@ -816,12 +817,9 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
     mwg_h4 fe[TWG];
 #pragma unroll
     for (int tt = 0; tt < TWG; tt++) fe[tt] = zero4;
-    float sF = 0.f;
-    auto load_f = [&](int grp) {
-        const int s = a.band[j.boff + (grp >> 3)];
-        const long long cb = a.slot_off[s] / N3 + (grp & 7);  // component block of the pool
-        sF = (float)pool_sc[cb];
-        const __attribute__((address_space(1))) _Float16 *fb = poolg + cb * (long long)(N2 * RH);
+    // cb: the component block of the pool (F rows of the x pass)
+    auto load_f = [&](int cb) {
+        const __attribute__((address_space(1))) _Float16 *fb = poolg + (long long)cb * (N2 * RH);
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
             fa[tt] = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(fb + fl[tt]);
@@ -906,7 +904,7 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
             // item's three operator block ids; wave 0 writes the items
             const int n = screen_scan<true>(
                 a, ad, j, pi, 8 * p_end, cnt, nrm_s, kSplitCap, lane, napps0, napps1,
-                [&](int pos, int p, int ft, int lx, int ly, int lz, double b0, double b1, bool t0, bool t1) {
+                [&](int pos, int p, int ft, int slot, int lx, int ly, int lz, double b0, double b1, bool t0, bool t1) {
                     if (wv != 0) return;
                     // a passing term has |l| <= L in every direction, so all
                     // four block ids exist
@@ -919,8 +917,12 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
                     const double bmax = fmax(t0 ? b0 : 0.0, t1 ? b1 : 0.0);
                     itm.grp = (p << 3) | ft | ((bmax <= thr_single ? 1 : 0) << kSingleBit) | ((t0 ? 1 : 0) << kZ0Bit) |
                               ((t1 ? 1 : 0) << kZ1Bit);
+                    // the input component block (its F rows in the split
+                    // pool) and its scale, resolved here so that a phase
+                    // starts the F load without a chain of dependent loads
+                    itm.cb = (int)(a.slot_off[slot] / N3) + ft;
                     wl[pos] = itm;
-                    const double sxy = opsc[itm.bx] * opsc[itm.by];
+                    const double sxy = pool_sc[itm.cb] * opsc[itm.bx] * opsc[itm.by];
                     wsig[pos] = make_float2(t0 ? (float)(sxy * opsc[itm.bz0]) : 0.f,
                                             t1 ? (float)(sxy * opsc[itm.bz1]) : 0.f);
                 });
@@ -958,20 +960,21 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
                 float2 sig0 = make_float2(0.f, 0.f);
                 unsigned msk0 = 0u;
                 int zm0 = 0;
+                // the F rows of item q load (when its group changes) while
+                // the y and z passes run; the x pass comes last
                 if (hx) {
                     const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
                     if ((g & mask_grp) != cur_grp) {
                         cur_grp = g & mask_grp;
-                        load_f(cur_grp);
+                        load_f(__builtin_amdgcn_readfirstlane(wl[q].cb));
                     }
                     msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
                     zm0 = (g >> kZ0Bit) & 3;
-                    const float2 w2 = wsig[q];
-                    sig0 = make_float2(sF * w2.x, sF * w2.y);
-                    x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
+                    sig0 = wsig[q];
                 }
                 if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
                 if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], oc[3], ec[3], sig2, zm2);
+                if (hx) x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
                 __syncthreads();
                 sig2 = sig1;
                 sig1 = sig0;
@@ -1543,44 +1546,70 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     KArgs &a = p->args;
     hipStream_t s = c->stream;
     int rc = 0;
-    int *d_out_depth, *d_out_app, *d_band_w, *d_band, *d_bw, *d_act_off, *d_act, *d_sj, *d_s0, *d_s1;
-    double *d_thr, *d_norm;
-    long long *d_band_off, *d_slot_off;
-    rc = plan_alloc(p, n_out, &d_out_depth, err);
-    if (!rc) rc = plan_alloc(p, n_out, &d_out_app, err);
-    if (!rc) rc = plan_alloc(p, n_out, &d_thr, err);
-    if (!rc) rc = plan_alloc(p, n_out, &d_band_off, err);
-    if (!rc) rc = plan_alloc(p, n_out, &d_band_w, err);
-    if (!rc) rc = plan_alloc(p, n_band, &d_band, err);
-    if (!rc) rc = plan_alloc(p, n_slot, &d_slot_off, err);
-    if (!rc) rc = plan_alloc(p, n_pool, &p->pool, err);
-    if (!rc) rc = plan_alloc(p, n_slot * 8, &d_norm, err);
-    if (!rc) rc = plan_alloc(p, n_bw, &d_bw, err);
-    if (!rc) rc = plan_alloc(p, n_actoff, &d_act_off, err);
-    if (!rc) rc = plan_alloc(p, n_act, &d_act, err);
-    if (!rc) rc = plan_alloc(p, n_all, &d_sj, err);
-    if (!rc) rc = plan_alloc(p, n_all, &d_s0, err);
-    if (!rc) rc = plan_alloc(p, n_all, &d_s1, err);
-    if (!rc) rc = plan_alloc(p, parts.size(), &p->d_apps, err);
+    // Every table of the plan but the pool and the output lives in one
+    // device arena, filled from one host staging buffer with one copy: a
+    // refinement level's plan costs one allocation and one upload however
+    // many applies it holds.
+    std::vector<DevOper *> opers(parts.size(), nullptr);
+    std::vector<std::vector<double>> mtabs(parts.size());
+    std::vector<std::pair<int, int>> mt_shape(parts.size());
+    for (size_t ai = 0; ai < parts.size() && !rc; ai++) {
+        rc = get_oper(c, parts[ai].op_key, *parts[ai].flat, &opers[ai], err);
+        if (!rc) build_prescreen(*parts[ai].plan, *parts[ai].flat, mtabs[ai], mt_shape[ai].first, mt_shape[ai].second);
+    }
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    size_t arena = 0;
+    auto place = [&](size_t bytes) {
+        const size_t off = arena;
+        arena += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return off;
+    };
+    const size_t f_out_depth = place(sizeof(int) * n_out), f_out_app = place(sizeof(int) * n_out);
+    const size_t f_thr = place(sizeof(double) * n_out), f_band_off = place(sizeof(long long) * n_out);
+    const size_t f_band_w = place(sizeof(int) * n_out), f_band = place(sizeof(int) * n_band);
+    const size_t f_slot_off = place(sizeof(long long) * n_slot), f_norm = place(sizeof(double) * 8 * n_slot);
+    const size_t f_bw = place(sizeof(int) * n_bw), f_act_off = place(sizeof(int) * n_actoff);
+    const size_t f_act = place(sizeof(int) * n_act);
+    const size_t f_sj = place(sizeof(int) * n_all), f_s0 = place(sizeof(int) * n_all), f_s1 = place(sizeof(int) * n_all);
+    const size_t f_apps = place(sizeof(AppDesc) * parts.size());
+    const size_t f_cnt = place(sizeof(unsigned long long) * 8);
+    std::vector<size_t> f_mt(parts.size());
+    for (size_t ai = 0; ai < parts.size(); ai++) f_mt[ai] = place(sizeof(double) * mtabs[ai].size());
+    unsigned char *dev = nullptr;
+    rc = plan_alloc(p, arena, &dev, err);
+    if (!rc) rc = plan_alloc(p, (size_t)std::max<long long>(n_pool, 1), &p->pool, err);
     p->out_bytes = sizeof(double) * (size_t)std::max<long long>(n_out, 1) * 8 * N * N * N;
     if (!rc) rc = plan_alloc(p, p->out_bytes / sizeof(double), &a.out_ns, err);
-    if (!rc) rc = plan_alloc(p, 8, &a.counters, err);
-    // host staging of the arrays that need index fix-ups (kept alive until
-    // the stream has consumed them)
-    std::vector<std::vector<int>> keep_i;
-    std::vector<std::vector<long long>> keep_l;
-    std::vector<std::vector<double>> keep_d;
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    std::vector<unsigned char, UninitAlloc<unsigned char>> stage(arena);
+    auto at = [&](size_t f) { return stage.data() + f; };
+    auto put = [&](size_t f, long long off, const auto &v) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        if (!v.empty()) std::memcpy(at(f) + sizeof(T) * off, v.data(), sizeof(T) * v.size());
+    };
     long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0, o_seg = 0;
     long long o_pseg = n_seg, o_fseg = n_seg + n_pseg;
     p->seg_thr.reserve(n_seg);
     p->seg_bmin.reserve(n_seg);
     p->seg_bmax.reserve(n_seg);
+    // a segment list's jobs, shifted to the batch's output numbering
+    auto put_jobs = [&](long long o_dst, const std::vector<int> &jobs, const std::vector<int> &p0,
+                        const std::vector<int> &p1, long long o_o) {
+        int *sj = reinterpret_cast<int *>(at(f_sj)) + o_dst;
+        for (size_t i = 0; i < jobs.size(); i++) sj[i] = jobs[i] + (int)(8 * o_o);
+        put(f_s0, o_dst, p0);
+        put(f_s1, o_dst, p1);
+    };
     for (size_t ai = 0; ai < parts.size() && !rc; ai++) {
         const PlanPart &pp = parts[ai];
         const ApplyPlan &pl = *pp.plan;
-        DevOper *o = nullptr;
-        rc = get_oper(c, pp.op_key, *pp.flat, &o, err);
-        if (rc) break;
+        DevOper *o = opers[ai];
         p->op_keys.push_back(pp.op_key);
         AppDesc ad{};
         ad.opL = o->L;
@@ -1596,62 +1625,53 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         ad.bw_off = o_bw;
         ad.act_base = (int)o_actoff;
         ad.thr_floor = pl.thr_floor;
-        keep_d.emplace_back();
-        build_prescreen(pl, *pp.flat, keep_d.back(), ad.mt_d0, ad.mt_w);
-        double *d_mt = nullptr;
-        rc = plan_alloc(p, keep_d.back().size(), &d_mt, err);
-        if (!rc) rc = upload_at(d_mt, keep_d.back(), s, err);
-        if (rc) break;
-        ad.mtab = d_mt;
+        ad.mt_d0 = mt_shape[ai].first;
+        ad.mt_w = mt_shape[ai].second;
+        put(f_mt[ai], 0, mtabs[ai]);
+        ad.mtab = reinterpret_cast<const double *>(dev + f_mt[ai]);
         p->apps.push_back(ad);
         const size_t no = pl.out_node.size();
         // outputs
-        rc = upload_at(d_out_depth + o_out, pl.out_depth, s, err);
-        keep_i.emplace_back(no, (int)ai);
-        if (!rc) rc = upload_at(d_out_app + o_out, keep_i.back(), s, err);
-        if (!rc) rc = upload_at(d_thr + o_out, pl.out_thr, s, err);
-        keep_l.emplace_back(pl.band_off);
-        for (auto &v : keep_l.back()) v += o_band;
-        if (!rc) rc = upload_at(d_band_off + o_out, keep_l.back(), s, err);
-        if (!rc) rc = upload_at(d_band_w + o_out, pl.band_w, s, err);
-        keep_i.emplace_back(pl.band);
-        for (auto &v : keep_i.back())
-            if (v >= 0) v += (int)o_slot;
-        if (!rc) rc = upload_at(d_band + o_band, keep_i.back(), s, err);
-        // inputs
-        keep_l.emplace_back(pl.slot_off);
-        for (auto &v : keep_l.back()) v += o_pool;
-        if (!rc) rc = upload_at(d_slot_off + o_slot, keep_l.back(), s, err);
-        if (!rc) {
-            if (pl.device_pool) {
-                if (!pp.dev_pool) {
-                    err = "plan part without its device pool";
-                    rc = -1;
-                } else if (pl.pool_values > 0) {
-                    hipError_t e = hipMemcpyAsync(p->pool + o_pool, pp.dev_pool, sizeof(double) * pl.pool_values,
-                                                  hipMemcpyDeviceToDevice, s);
-                    if (e != hipSuccess) {
-                        err = std::string("pool copy: ") + hipGetErrorString(e);
-                        rc = -2;
-                    }
-                }
-            } else {
-                rc = upload_at(p->pool + o_pool, pl.in_coefs, s, err);
-            }
+        put(f_out_depth, o_out, pl.out_depth);
+        std::fill_n(reinterpret_cast<int *>(at(f_out_app)) + o_out, no, (int)ai);
+        put(f_thr, o_out, pl.out_thr);
+        {
+            long long *bo = reinterpret_cast<long long *>(at(f_band_off)) + o_out;
+            for (size_t i = 0; i < no; i++) bo[i] = pl.band_off[i] + o_band;
+            int *bd = reinterpret_cast<int *>(at(f_band)) + o_band;
+            for (size_t i = 0; i < pl.band.size(); i++) bd[i] = pl.band[i] >= 0 ? pl.band[i] + (int)o_slot : pl.band[i];
         }
-        if (!rc) rc = upload_at(d_norm + o_slot * 8, pl.in_norms, s, err);
+        put(f_band_w, o_out, pl.band_w);
+        // inputs
+        {
+            long long *so = reinterpret_cast<long long *>(at(f_slot_off)) + o_slot;
+            for (size_t i = 0; i < pl.slot_off.size(); i++) so[i] = pl.slot_off[i] + o_pool;
+        }
+        if (pl.device_pool) {
+            if (!pp.dev_pool) {
+                err = "plan part without its device pool";
+                rc = -1;
+            } else if (pl.pool_values > 0) {
+                hipError_t e = hipMemcpyAsync(p->pool + o_pool, pp.dev_pool, sizeof(double) * pl.pool_values,
+                                              hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) {
+                    err = std::string("pool copy: ") + hipGetErrorString(e);
+                    rc = -2;
+                }
+            }
+        } else {
+            rc = upload_at(p->pool + o_pool, pl.in_coefs, s, err);
+        }
+        put(f_norm, o_slot * 8, pl.in_norms);
         // screening tables
-        if (!rc) rc = upload_at(d_bw + o_bw, pl.bw, s, err);
-        keep_i.emplace_back(pl.active_off);
-        for (auto &v : keep_i.back()) v += (int)o_act;
-        if (!rc) rc = upload_at(d_act_off + o_actoff, keep_i.back(), s, err);
-        if (!rc) rc = upload_at(d_act + o_act, pl.active_terms, s, err);
+        put(f_bw, o_bw, pl.bw);
+        {
+            int *ao = reinterpret_cast<int *>(at(f_act_off)) + o_actoff;
+            for (size_t i = 0; i < pl.active_off.size(); i++) ao[i] = pl.active_off[i] + (int)o_act;
+        }
+        put(f_act, o_act, pl.active_terms);
         // segments
-        keep_i.emplace_back(pl.seg_job);
-        for (auto &v : keep_i.back()) v += (int)(8 * o_out);
-        if (!rc) rc = upload_at(d_sj + o_seg, keep_i.back(), s, err);
-        if (!rc) rc = upload_at(d_s0 + o_seg, pl.seg_p0, s, err);
-        if (!rc) rc = upload_at(d_s1 + o_seg, pl.seg_p1, s, err);
+        put_jobs(o_seg, pl.seg_job, pl.seg_p0, pl.seg_p1, o_out);
         p->seg_apps.insert(p->seg_apps.end(), pl.seg_apps.begin(), pl.seg_apps.end());
         p->pseg_apps.insert(p->pseg_apps.end(), pl.pseg_apps.begin(), pl.pseg_apps.end());
         p->fseg_apps.insert(p->fseg_apps.end(), pl.fseg_apps.begin(), pl.fseg_apps.end());
@@ -1661,11 +1681,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
             p->seg_bmin.push_back(pl.seg_bmin[i]);
             p->seg_bmax.push_back(pl.seg_bmax[i]);
         }
-        keep_i.emplace_back(pl.pseg_job);
-        for (auto &v : keep_i.back()) v += (int)(8 * o_out);
-        if (!rc) rc = upload_at(d_sj + o_pseg, keep_i.back(), s, err);
-        if (!rc) rc = upload_at(d_s0 + o_pseg, pl.pseg_p0, s, err);
-        if (!rc) rc = upload_at(d_s1 + o_pseg, pl.pseg_p1, s, err);
+        put_jobs(o_pseg, pl.pseg_job, pl.pseg_p0, pl.pseg_p1, o_out);
         for (size_t i = 0; i < pl.pseg_job.size(); i++) {
             const double t = pl.out_thr[pl.pseg_job[i] >> 3];
             p->pseg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);
@@ -1673,11 +1689,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
             p->pseg_bmax.push_back(pl.pseg_bmax[i]);
         }
         o_pseg += (long long)pl.pseg_job.size();
-        keep_i.emplace_back(pl.fseg_job);
-        for (auto &v : keep_i.back()) v += (int)(8 * o_out);
-        if (!rc) rc = upload_at(d_sj + o_fseg, keep_i.back(), s, err);
-        if (!rc) rc = upload_at(d_s0 + o_fseg, pl.fseg_p0, s, err);
-        if (!rc) rc = upload_at(d_s1 + o_fseg, pl.fseg_p1, s, err);
+        put_jobs(o_fseg, pl.fseg_job, pl.fseg_p0, pl.fseg_p1, o_out);
         for (size_t i = 0; i < pl.fseg_job.size(); i++) {
             const double t = pl.out_thr[pl.fseg_job[i] >> 3];
             p->fseg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);
@@ -1694,9 +1706,11 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         o_act += (long long)pl.active_terms.size();
         o_seg += (long long)pl.seg_job.size();
     }
-    if (!rc) rc = upload_at(p->d_apps, p->apps, s, err);
     if (!rc) {
-        hipError_t e = hipStreamSynchronize(s);
+        std::memcpy(at(f_apps), p->apps.data(), sizeof(AppDesc) * p->apps.size());
+        std::memset(at(f_cnt), 0, sizeof(unsigned long long) * 8);
+        hipError_t e = hipMemcpyAsync(dev, stage.data(), arena, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
             err = std::string("upload: ") + hipGetErrorString(e);
             rc = -2;
@@ -1707,21 +1721,23 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         delete p;
         return rc;
     }
-    a.out_depth = d_out_depth;
-    a.out_app = d_out_app;
-    a.out_thr = d_thr;
-    a.band_off = d_band_off;
-    a.band_w = d_band_w;
-    a.band = d_band;
-    a.slot_off = d_slot_off;
-    a.slot_norm = d_norm;
-    a.bw = d_bw;
-    a.act_off = d_act_off;
-    a.act_terms = d_act;
+    p->d_apps = reinterpret_cast<AppDesc *>(dev + f_apps);
+    a.counters = reinterpret_cast<unsigned long long *>(dev + f_cnt);
+    a.out_depth = reinterpret_cast<const int *>(dev + f_out_depth);
+    a.out_app = reinterpret_cast<const int *>(dev + f_out_app);
+    a.out_thr = reinterpret_cast<const double *>(dev + f_thr);
+    a.band_off = reinterpret_cast<const long long *>(dev + f_band_off);
+    a.band_w = reinterpret_cast<const int *>(dev + f_band_w);
+    a.band = reinterpret_cast<const int *>(dev + f_band);
+    a.slot_off = reinterpret_cast<const long long *>(dev + f_slot_off);
+    a.slot_norm = reinterpret_cast<const double *>(dev + f_norm);
+    a.bw = reinterpret_cast<const int *>(dev + f_bw);
+    a.act_off = reinterpret_cast<const int *>(dev + f_act_off);
+    a.act_terms = reinterpret_cast<const int *>(dev + f_act);
     a.apps = p->d_apps;
-    a.seg_job = d_sj;
-    a.seg_p0 = d_s0;
-    a.seg_p1 = d_s1;
+    a.seg_job = reinterpret_cast<const int *>(dev + f_sj);
+    a.seg_p0 = reinterpret_cast<const int *>(dev + f_s0);
+    a.seg_p1 = reinterpret_cast<const int *>(dev + f_s1);
     *out = p;
     return 0;
 }

@@ -107,11 +107,16 @@ struct DevMulSession;
 // noalloc: every device buffer of the session is carved from the device's
 // scratch array (dev_noalloc_init); no device memory is allocated, and
 // inputs that do not fit are an argument error.
+// a_on_device / b_on_device: that pool is device memory, used in place (or,
+// with noalloc, copied into the scratch array); otherwise it is host memory
+// and uploaded.
 int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a, const double *pool_b, size_t n_b,
-                  DevMulSession **out, std::string &err, bool noalloc = false);
-// out_blocks: nodes.size() x 8(k+1)^3 on the host.
+                  DevMulSession **out, std::string &err, bool noalloc = false, bool a_on_device = false,
+                  bool b_on_device = false);
+// out_blocks: nodes.size() x 8(k+1)^3, on the host, or on the device with
+// out_on_device.
 int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,
-                  std::string &err);
+                  std::string &err, bool out_on_device = false);
 void dev_mul_end(DevMulSession *m);
 
 // The pool of a plan built with device_pool (mw_multiply.hip): generated
@@ -127,6 +132,13 @@ int dev_alloc(DeviceCtx *c, size_t n, double **dst, std::string &err);
 // Block i (blk doubles) of src to block dst_idx[i] of dst, on the device.
 int dev_scatter_blocks(DeviceCtx *c, const double *src, const std::vector<int> &dst_idx, int blk, double *dst,
                        std::string &err);
+// Block src_idx[i] of src to block i of dst, on the device.
+int dev_gather_blocks(DeviceCtx *c, const double *src, const std::vector<int> &src_idx, int blk, double *dst,
+                      std::string &err);
+// p[0, n) *= f on the device, finished on return.
+int dev_scale(DeviceCtx *c, double *p, size_t n, double f, std::string &err);
+// n doubles from host memory into device memory at dst, finished on return.
+int dev_upload_into(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err);
 // n doubles from device memory to device memory, finished on return.
 int dev_d2d(DeviceCtx *c, double *dst, const double *src, size_t n, std::string &err);
 
@@ -134,6 +146,8 @@ int dev_d2d(DeviceCtx *c, double *dst, const double *src, size_t n, std::string 
 // blocks[0, n): its scaling part's and wavelet parts' squared norms.
 int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host_s2, double *host_w2,
                    std::string &err);
+// Per node of blocks[0, n): the norms of its 8 components (8 per node).
+int dev_component_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host8, std::string &err);
 // A branch node's block and its 8 children's blocks (consecutive).
 struct PostPair {
     double *node, *child0;
@@ -174,11 +188,19 @@ struct AddSrc {
 };
 constexpr int kAddMaxLevels = 21;  // levels a 64-bit path holds
 struct DevAddSession;
-int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool, const std::vector<double> &coefs,
+// One input's blocks: n doubles at p, in host or device memory.
+struct BlockSource {
+    const double *p;
+    size_t n;
+    bool on_device;
+};
+// The inputs' blocks are concatenated in order into the session's pool.
+int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<BlockSource> &pool, const std::vector<double> &coefs,
                   DevAddSession **out, std::string &err);
 // src: n_nodes x n_inputs entries (node-major); out_blocks: n_nodes x
-// 8(k+1)^3 on the host.
-int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err);
+// 8(k+1)^3, on the host, or on the device with out_on_device.
+int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err,
+                  bool out_on_device = false);
 void dev_add_end(DevAddSession *m);
 
 // The device's scratch array of the allocation-free entry points

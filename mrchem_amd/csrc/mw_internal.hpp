@@ -293,9 +293,12 @@ struct ApplyStats {
 // With device_pool the plan stops after the slots and their sources: the
 // pool and in_norms are made on the device (dev_pool_fill), then
 // finish_plan_counts adds the exact counts and the work segments.
+// in_cnorm8: the input tree's 8 component norms per node (tree order), when
+// the caller has them; otherwise computed here.
 void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
                 const std::vector<int> &nodes, const std::vector<double> &thr,
-                int max_band_w, int seg_target = 256, bool device_pool = false);
+                int max_band_w, int seg_target = 256, bool device_pool = false,
+                const double *in_cnorm8 = nullptr);
 void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target = 256);
 double oper_norm(const Operator &op, int m, int d, int c, long long l);
 

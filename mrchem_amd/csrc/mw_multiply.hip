@@ -315,6 +315,21 @@ __global__ __launch_bounds__(64) void k_node_norms(const double *blocks, long lo
     }
 }
 
+// The 8 component norms of each node (scaling part, then the 7 wavelet
+// parts): one wave per node.
+__global__ __launch_bounds__(64) void k_component_norms(const double *blocks, long long n, int n3, double *out8) {
+    const long long b = blockIdx.x;
+    if (b >= n) return;
+    const double *x = blocks + b * 8 * n3;
+    for (int t = 0; t < 8; t++) {
+        double s = 0.0;
+        for (int e = threadIdx.x; e < n3; e += 64) s += x[t * n3 + e] * x[t * n3 + e];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (threadIdx.x == 0) out8[b * 8 + t] = sqrt(s);
+    }
+}
+
 // Post transforms (TopDown(add) and BottomUp of the apply's output tree).
 // One workgroup per branch node: its block (TopDown) or its 8 children's
 // scaling blocks (BottomUp) are staged in LDS, 8 N^3 doubles, and each of
@@ -494,15 +509,30 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl
     return 0;
 }
 
-// Block i of src (blk doubles) to block dst_idx[i] of dst: one workgroup per
-// block.
-__global__ __launch_bounds__(256) void k_scatter_blocks(const double *src, const int *dst_idx, int n, int blk,
-                                                        double *dst) {
+// Block src_idx[i] of src (blk doubles; i itself without src_idx) to block
+// dst_idx[i] of dst (i without dst_idx): one workgroup per block.
+__global__ __launch_bounds__(256) void k_move_blocks(const double *src, const int *src_idx, const int *dst_idx, int n,
+                                                     int blk, double *dst) {
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const double *a = src + (size_t)i * blk;
-        double *b = dst + (size_t)dst_idx[i] * blk;
+        const double *a = src + (size_t)(src_idx ? src_idx[i] : i) * blk;
+        double *b = dst + (size_t)(dst_idx ? dst_idx[i] : i) * blk;
         for (int e = threadIdx.x; e < blk; e += 256) b[e] = a[e];
     }
+}
+
+__global__ __launch_bounds__(256) void k_scale(double *p, long long n, double f) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] *= f;
+}
+
+int dev_scale(DeviceCtx *c, double *p, size_t n, double f, std::string &err) {
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, s, p,
+                       (long long)n, f);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
 }
 
 int dev_alloc(DeviceCtx *c, size_t n, double **dst, std::string &err) {
@@ -512,25 +542,45 @@ int dev_alloc(DeviceCtx *c, size_t n, double **dst, std::string &err) {
     return 0;
 }
 
-int dev_scatter_blocks(DeviceCtx *c, const double *src, const std::vector<int> &dst_idx, int blk, double *dst,
-                       std::string &err) {
-    if (dst_idx.empty()) return 0;
+static int move_blocks(DeviceCtx *c, const double *src, const std::vector<int> *src_idx,
+                       const std::vector<int> *dst_idx, size_t n, int blk, double *dst, std::string &err) {
+    if (n == 0) return 0;
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     int *d = nullptr;
-    HIP_TRY(hipMalloc((void **)&d, sizeof(int) * dst_idx.size()));
-    hipError_t e = hipMemcpyAsync(d, dst_idx.data(), sizeof(int) * dst_idx.size(), hipMemcpyHostToDevice, s);
+    HIP_TRY(hipMalloc((void **)&d, sizeof(int) * n));
+    const std::vector<int> &idx = src_idx ? *src_idx : *dst_idx;
+    hipError_t e = hipMemcpyAsync(d, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_scatter_blocks, dim3((unsigned)std::min<size_t>(dst_idx.size(), 4096)), dim3(256), 0, s,
-                           src, d, (int)dst_idx.size(), blk, dst);
+        hipLaunchKernelGGL(k_move_blocks, dim3((unsigned)std::min<size_t>(n, 4096)), dim3(256), 0, s, src,
+                           src_idx ? d : nullptr, src_idx ? nullptr : d, (int)n, blk, dst);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
     if (e != hipSuccess) {
-        err = std::string("block scatter: ") + hipGetErrorString(e);
+        err = std::string("block move: ") + hipGetErrorString(e);
         return -2;
     }
+    return 0;
+}
+
+int dev_scatter_blocks(DeviceCtx *c, const double *src, const std::vector<int> &dst_idx, int blk, double *dst,
+                       std::string &err) {
+    return move_blocks(c, src, nullptr, &dst_idx, dst_idx.size(), blk, dst, err);
+}
+
+int dev_gather_blocks(DeviceCtx *c, const double *src, const std::vector<int> &src_idx, int blk, double *dst,
+                      std::string &err) {
+    return move_blocks(c, src, &src_idx, nullptr, src_idx.size(), blk, dst, err);
+}
+
+int dev_upload_into(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err) {
+    if (n == 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return 0;
 }
 
@@ -574,6 +624,24 @@ int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, dou
     (void)hipFree(d);
     if (e != hipSuccess) {
         err = std::string("node norms: ") + hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+int dev_component_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host8, std::string &err) {
+    if (n <= 0) return 0;
+    HIP_TRY(hipSetDevice(dev_device(c)));
+    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * 8 * (size_t)n));
+    hipLaunchKernelGGL(k_component_norms, dim3((unsigned)n), dim3(64), 0, s, blocks, n, kp1 * kp1 * kp1, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host8, d, sizeof(double) * 8 * (size_t)n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        err = std::string("component norms: ") + hipGetErrorString(e);
         return -2;
     }
     return 0;
@@ -683,15 +751,19 @@ struct DevMulSession {
     size_t chunk = 0;
     MulNode *dn = nullptr;
     double *dout = nullptr;
+    // the pools are the caller's device blocks (not freed here)
+    bool borrowed_a = false, borrowed_b = false;
     ~DevMulSession() {
         if (noalloc) return;
-        for (double *p : {pool_a, pool_b, F, wsc, ws})
+        for (double *p : {F, wsc, ws})
             if (p) (void)hipFree(p);
+        if (pool_a && !borrowed_a) (void)hipFree(pool_a);
+        if (pool_b && !borrowed_b) (void)hipFree(pool_b);
     }
 };
 
 int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a, const double *pool_b, size_t n_b,
-                  DevMulSession **out, std::string &err, bool noalloc) {
+                  DevMulSession **out, std::string &err, bool noalloc, bool a_on_device, bool b_on_device) {
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     if (mra.basis.type != BASIS_INTERPOLATING) {
@@ -746,8 +818,12 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a
         hipError_t e = hipMemcpyAsync(m->F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size(),
                                       hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipMemcpyAsync(m->wsc, wsc.data(), sizeof(double) * n3, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess && n_a) e = hipMemcpyAsync(m->pool_a, pool_a, sizeof(double) * n_a, hipMemcpyHostToDevice, s);
-        if (e == hipSuccess && n_b) e = hipMemcpyAsync(m->pool_b, pool_b, sizeof(double) * n_b, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && n_a)
+            e = hipMemcpyAsync(m->pool_a, pool_a, sizeof(double) * n_a,
+                               a_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && n_b)
+            e = hipMemcpyAsync(m->pool_b, pool_b, sizeof(double) * n_b,
+                               b_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
         if (e != hipSuccess) {
             (void)hipStreamSynchronize(s);
             delete m;
@@ -765,8 +841,19 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a
         if (nv) HIP_TRY(hipMemcpyAsync(*d, v, sizeof(double) * nv, hipMemcpyHostToDevice, s));
         return 0;
     };
-    int rc = up(pool_a, n_a, &m->pool_a);
-    if (!rc) rc = up(pool_b, n_b, &m->pool_b);
+    int rc = 0;
+    if (a_on_device) {
+        m->pool_a = const_cast<double *>(pool_a);
+        m->borrowed_a = true;
+    } else {
+        rc = up(pool_a, n_a, &m->pool_a);
+    }
+    if (!rc && b_on_device) {
+        m->pool_b = const_cast<double *>(pool_b);
+        m->borrowed_b = true;
+    } else if (!rc) {
+        rc = up(pool_b, n_b, &m->pool_b);
+    }
     if (!rc) rc = up(mra.basis.F.data(), mra.basis.F.size(), &m->F);
     if (!rc) rc = up(wsc.data(), wsc.size(), &m->wsc);
     if (!rc) {
@@ -786,7 +873,7 @@ int dev_mul_begin(DeviceCtx *c, const MRA &mra, const double *pool_a, size_t n_a
 }
 
 int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double coef, double *out_blocks,
-                  std::string &err) {
+                  std::string &err, bool out_on_device) {
     if (nodes.empty()) return 0;
     HIP_TRY(hipSetDevice(dev_device(m->ctx)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(m->ctx));
@@ -802,15 +889,15 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
                                m->pool_b, m->F, m->wsc, coef, m->ws, m->dout);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(out_blocks + c0 * 8 * n3, m->dout, sizeof(double) * cn * 8 * n3,
-                                   hipMemcpyDeviceToHost, s));
+                                   out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
         }
         return 0;
     }
     MulNode *dn = nullptr;
-    double *dout = nullptr;
+    double *dout = out_on_device ? out_blocks : nullptr;
     HIP_TRY(hipMalloc((void **)&dn, sizeof(MulNode) * nodes.size()));
-    hipError_t e = hipMalloc((void **)&dout, sizeof(double) * nodes.size() * 8 * n3);
+    hipError_t e = out_on_device ? hipSuccess : hipMalloc((void **)&dout, sizeof(double) * nodes.size() * 8 * n3);
     if (e != hipSuccess) {
         (void)hipFree(dn);
         err = std::string("hipMalloc(out): ") + hipGetErrorString(e);
@@ -824,9 +911,11 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
         e = hipGetLastError();
     }
     int crc = 0;
-    if (e == hipSuccess) crc = dev_copy_d2h(m->ctx, out_blocks, dout, sizeof(double) * nodes.size() * 8 * n3, err);
+    if (e == hipSuccess && !out_on_device)
+        crc = dev_copy_d2h(m->ctx, out_blocks, dout, sizeof(double) * nodes.size() * 8 * n3, err);
+    if (e == hipSuccess && out_on_device) e = hipStreamSynchronize(s);
     (void)hipFree(dn);
-    (void)hipFree(dout);
+    if (!out_on_device) (void)hipFree(dout);
     if (crc) return crc;
     if (e != hipSuccess) {
         err = std::string("multiply launch: ") + hipGetErrorString(e);
@@ -852,7 +941,7 @@ struct DevAddSession {
     }
 };
 
-int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool, const std::vector<double> &coefs,
+int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<BlockSource> &pool, const std::vector<double> &coefs,
                   DevAddSession **out, std::string &err) {
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
@@ -866,7 +955,25 @@ int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool,
         if (!v.empty()) HIP_TRY(hipMemcpyAsync(*d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, s));
         return 0;
     };
-    int rc = up(pool, &m->pool);
+    // the pool: the sources one after another, copied from host or device
+    size_t total = 0;
+    for (const BlockSource &b : pool) total += b.n;
+    int rc = 0;
+    {
+        hipError_t e = hipMalloc((void **)&m->pool, sizeof(double) * std::max<size_t>(total, 1));
+        size_t off = 0;
+        for (const BlockSource &b : pool) {
+            if (e != hipSuccess) break;
+            if (b.n)
+                e = hipMemcpyAsync(m->pool + off, b.p, sizeof(double) * b.n,
+                                   b.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s);
+            off += b.n;
+        }
+        if (e != hipSuccess) {
+            err = std::string("add pool: ") + hipGetErrorString(e);
+            rc = -2;
+        }
+    }
     if (!rc) rc = up(coefs, &m->coefs);
     if (!rc) rc = up(mra.basis.F, &m->F);
     if (!rc) {
@@ -885,7 +992,8 @@ int dev_add_begin(DeviceCtx *c, const MRA &mra, const std::vector<double> &pool,
     return 0;
 }
 
-int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err) {
+int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_blocks, std::string &err,
+                  bool out_on_device) {
     const size_t n_nodes = m->n_in > 0 ? src.size() / (size_t)m->n_in : 0;
     if (n_nodes == 0) return 0;
     if (src.size() != n_nodes * (size_t)m->n_in) {
@@ -896,9 +1004,9 @@ int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_
     hipStream_t s = static_cast<hipStream_t>(dev_stream(m->ctx));
     const int n3 = m->n * m->n * m->n;
     AddSrc *dsrc = nullptr;
-    double *dout = nullptr;
+    double *dout = out_on_device ? out_blocks : nullptr;
     HIP_TRY(hipMalloc((void **)&dsrc, sizeof(AddSrc) * src.size()));
-    hipError_t e = hipMalloc((void **)&dout, sizeof(double) * n_nodes * 8 * n3);
+    hipError_t e = out_on_device ? hipSuccess : hipMalloc((void **)&dout, sizeof(double) * n_nodes * 8 * n3);
     if (e != hipSuccess) {
         (void)hipFree(dsrc);
         err = std::string("hipMalloc(out): ") + hipGetErrorString(e);
@@ -912,9 +1020,11 @@ int dev_add_nodes(DevAddSession *m, const std::vector<AddSrc> &src, double *out_
         e = hipGetLastError();
     }
     int crc = 0;
-    if (e == hipSuccess) crc = dev_copy_d2h(m->ctx, out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, err);
+    if (e == hipSuccess && !out_on_device)
+        crc = dev_copy_d2h(m->ctx, out_blocks, dout, sizeof(double) * n_nodes * 8 * n3, err);
+    if (e == hipSuccess && out_on_device) e = hipStreamSynchronize(s);
     (void)hipFree(dsrc);
-    (void)hipFree(dout);
+    if (!out_on_device) (void)hipFree(dout);
     if (crc) return crc;
     if (e != hipSuccess) {
         err = std::string("add launch: ") + hipGetErrorString(e);

@@ -49,7 +49,7 @@ double oper_norm(const Operator &op, int m, int d, int c, long long l) {
 
 void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree &out,
                 const std::vector<int> &nodes, const std::vector<double> &thr, int max_band_w, int seg_target,
-                bool device_pool) {
+                bool device_pool, const double *in_cnorm8) {
     const MRA &mra = *out.mra;
     const int n3 = mra.kp1_3(), blk = mra.block();
     const int M = op.size(), ndepth = op.max_depth + 1;
@@ -79,7 +79,11 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     for (int i = 0; i < inp.size(); i++) {
         int d = inp.keys[i].depth;
         double nrm[8], tot = 0.0;
-        component_norms(mra, inp.node(i), nrm);
+        if (in_cnorm8) {
+            for (int t = 0; t < 8; t++) nrm[t] = in_cnorm8[(size_t)i * 8 + t];
+        } else {
+            component_norms(mra, inp.node(i), nrm);
+        }
         for (int t = 0; t < 8; t++) {
             tot += nrm[t] * nrm[t];
             if (d <= maxd) fmax[d] = std::max(fmax[d], nrm[t]);

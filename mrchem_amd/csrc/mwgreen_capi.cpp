@@ -44,6 +44,10 @@ struct mwg_context_s {
     double tau = 65536.0;    // mixed mode: FP64 for terms with bound > tau * thr
     std::atomic<int> plan_refs{0};  // live plans on this context
     std::string last_err;           // message of the last failed call on this context
+    std::atomic<int> resident_refs{0};  // trees whose blocks are resident on this context's GPU
+    // bytes of tree blocks moved between host and device by calls on this
+    // context (mwg_context_transfer_bytes)
+    std::atomic<long long> h2d_bytes{0}, d2h_bytes{0};
 };
 struct mwg_operator_s {
     Operator op;
@@ -56,6 +60,22 @@ struct mwg_tree_s {
     Tree tree;
     std::vector<double> apply_thr;
     std::atomic<int> plan_refs{0};  // live plans that write their result into this tree
+    // Residency (mwg_tree_set_resident): the blocks in HBM of dctx's GPU,
+    // tree.size() x 8 (k+1)^3 doubles in tree order at dblocks (dcap
+    // allocated).  host_valid: tree.coefs holds the blocks; dev_valid:
+    // dblocks does.  At least one of them holds; a resident tree written by a
+    // device call on dctx holds only the device copy until host code reads it.
+    mwg_context dctx = nullptr;
+    double *dblocks = nullptr;
+    size_t dcap = 0;
+    bool host_valid = true, dev_valid = false;
+    // per-node squared norms of the scaling part (s2) and of the wavelet
+    // parts (w2) of the current blocks, when norms_valid
+    std::vector<double> s2, w2;
+    bool norms_valid = false;
+    // the 8 component norms of each node, when cn8_valid (apply inputs)
+    std::vector<double> cn8;
+    bool cn8_valid = false;
 };
 struct mwg_plan_s {
     mwg_context ctx = nullptr;
@@ -105,12 +125,25 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+void ensure_norms(mwg_tree t);
+// A failed device step inside a call (rc -1: an argument the device code
+// refused, else a HIP error).
+struct DeviceFailure : std::runtime_error {
+    int rc;
+    DeviceFailure(int r, const std::string &m) : std::runtime_error(m), rc(r) {}
+};
+void dev_check(int rc, const std::string &err) {
+    if (rc) throw DeviceFailure(rc, err);
+}
+
 template <typename F>
 int guarded(F &&f) {
     try {
         return f();
     } catch (const std::invalid_argument &e) {
         return fail(MWG_ERR_ARG, e.what());
+    } catch (const DeviceFailure &e) {
+        return fail(e.rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, e.what());
     } catch (const std::exception &e) {
         return fail(MWG_ERR_RUNTIME, e.what());
     }
@@ -132,29 +165,32 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A prec tree: its grid and the squared norms of each node's scaling and
+// wavelet parts.
+struct PrecTree {
+    const Tree *t;
+    const std::vector<double> *s2, *w2;
+};
+
 // Norm of the node at key in a prec tree, or of its deepest ancestor scaled
 // by the volume ratio when the tree is coarser there.
-double prec_tree_norm(const Tree &t, const NodeKey &k) {
+double prec_tree_norm(const PrecTree &p, const NodeKey &k) {
+    const Tree &t = *p.t;
     NodeKey a = k;
     int up = 0;
     while (true) {
         int i = t.find(a.depth, a.l);
-        if (i >= 0) {
-            const double *b = t.node(i);
-            double s = 0.0;
-            for (int e = 0; e < t.mra->block(); e++) s += b[e] * b[e];
-            return std::sqrt(s) * std::pow(2.0, -1.5 * up);
-        }
+        if (i >= 0) return std::sqrt((*p.s2)[i] + (*p.w2)[i]) * std::pow(2.0, -1.5 * up);
         if (a.depth == 0) return 0.0;
         a = NodeKey{a.depth - 1, {a.l[0] >> 1, a.l[1] >> 1, a.l[2] >> 1}};
         up++;
     }
 }
 
-double prec_factor(const std::vector<const Tree *> &pt, const NodeKey &k) {
+double prec_factor(const std::vector<PrecTree> &pt, const NodeKey &k) {
     if (pt.empty()) return 1.0;
     double pn = 1.0;
-    for (const Tree *t : pt) pn = std::max(pn, prec_tree_norm(*t, k));
+    for (const PrecTree &t : pt) pn = std::max(pn, prec_tree_norm(t, k));
     return 1.0 / pn;
 }
 
@@ -166,11 +202,173 @@ void check_compatible(const Tree &out, const Operator &op, const Tree &inp) {
         throw std::invalid_argument("apply: the device kernels take k + 1 <= 12 (order <= 11), this MRA has k + 1 = " +
                                     std::to_string(op.mra->kp1()));
     if (inp.size() == 0) throw std::invalid_argument("apply: input tree is empty");
-    for (double v : inp.coefs)
-        if (!std::isfinite(v)) throw std::invalid_argument("apply: input tree holds NaN or Inf");
 }
 
 int dev_fail(int rc, const std::string &err) { return fail(rc == -1 ? MWG_ERR_ARG : MWG_ERR_DEVICE, err); }
+
+// ---- Device-resident trees ----
+
+size_t tree_values(const mwg_tree_s *t) { return (size_t)t->tree.size() * t->tree.mra->block(); }
+
+// The host copy of t's blocks, downloaded when only the device holds them.
+void ensure_host(mwg_tree t) {
+    if (t->host_valid) return;
+    const size_t n = tree_values(t);
+    t->tree.structure_only = false;
+    t->tree.coefs.resize(n);
+    std::string err;
+    dev_check(dev_download(t->dctx->dev, t->dblocks, n, t->tree.coefs.data(), err), err);
+    t->dctx->d2h_bytes += (long long)(sizeof(double) * n);
+    t->host_valid = true;
+}
+
+// Host code changed t's blocks or grid: the device copy and the norms are
+// stale.
+void host_changed(mwg_tree t) {
+    t->host_valid = true;
+    t->dev_valid = false;
+    t->norms_valid = false;
+    t->cn8_valid = false;
+}
+
+// The 8 component norms of every node of t (on the device for a tree whose
+// blocks only the device holds).
+const double *ensure_cnorms(mwg_tree t) {
+    if (t->cn8_valid) return t->cn8.data();
+    const Tree &T = t->tree;
+    const int n = T.size();
+    t->cn8.assign((size_t)8 * n, 0.0);
+    if (!t->host_valid) {
+        std::string err;
+        dev_check(dev_component_norms(t->dctx->dev, T.mra->kp1(), t->dblocks, n, t->cn8.data(), err), err);
+    } else {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++) component_norms(*T.mra, T.node(i), t->cn8.data() + (size_t)8 * i);
+    }
+    t->cn8_valid = true;
+    return t->cn8.data();
+}
+
+// t's blocks in device memory of ctx: the resident copy (uploaded first when
+// stale), or else an upload into *tmp, which the caller frees (dev_free).
+const double *device_blocks(mwg_context ctx, mwg_tree t, double **tmp) {
+    *tmp = nullptr;
+    const size_t n = tree_values(t);
+    std::string err;
+    if (t->dctx == ctx) {
+        if (!t->dev_valid) {
+            if (t->dcap < n) {
+                dev_free(ctx->dev, t->dblocks);
+                t->dblocks = nullptr;
+                t->dcap = 0;
+                dev_check(dev_alloc(ctx->dev, n, &t->dblocks, err), err);
+                t->dcap = n;
+            }
+            dev_check(dev_upload_into(ctx->dev, t->tree.coefs.data(), n, t->dblocks, err), err);
+            ctx->h2d_bytes += (long long)(sizeof(double) * n);
+            t->dev_valid = true;
+        }
+        return t->dblocks;
+    }
+    ensure_host(t);
+    dev_check(dev_upload(ctx->dev, t->tree.coefs.data(), n, tmp, err), err);
+    ctx->h2d_bytes += (long long)(sizeof(double) * n);
+    return *tmp;
+}
+
+// Frees the temporary device copies of a call's input trees.
+struct TempBlocks {
+    DeviceCtx *c;
+    std::vector<double *> p;
+    ~TempBlocks() {
+        for (double *q : p) dev_free(c, q);
+    }
+    const double *of(mwg_context ctx, mwg_tree t) {
+        double *tmp = nullptr;
+        const double *b = device_blocks(ctx, t, &tmp);
+        if (tmp) p.push_back(tmp);
+        return b;
+    }
+};
+
+// The per-node norms of t's blocks (computed on the device for a tree whose
+// blocks only the device holds).
+void ensure_norms(mwg_tree t) {
+    if (t->norms_valid) return;
+    const Tree &T = t->tree;
+    const int n = T.size(), n3 = T.mra->kp1_3(), blk = T.mra->block();
+    t->s2.assign(n, 0.0);
+    t->w2.assign(n, 0.0);
+    if (!t->host_valid) {
+        std::string err;
+        dev_check(dev_node_norms(t->dctx->dev, T.mra->kp1(), t->dblocks, n, t->s2.data(), t->w2.data(), err), err);
+    } else {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; i++) {
+            const double *b = T.node(i);
+            double a = 0.0, w = 0.0;
+            for (int e = 0; e < n3; e++) a += b[e] * b[e];
+            for (int e = n3; e < blk; e++) w += b[e] * b[e];
+            t->s2[i] = a;
+            t->w2[i] = w;
+        }
+    }
+    t->norms_valid = true;
+}
+
+// A result computed on the device by a call on ctx: t's blocks (tree order,
+// tree_values(t) doubles, owned by the call until handed over here) and their
+// norms.  Kept as t's resident copy when t is resident on ctx, else
+// downloaded into the host tree and freed.
+void take_device_result(mwg_context ctx, mwg_tree t, double *blocks, std::vector<double> &&s2,
+                        std::vector<double> &&w2) {
+    const size_t n = tree_values(t);
+    Tree &T = t->tree;
+    T.structure_only = false;
+    if (t->dctx == ctx) {
+        if (t->dblocks && t->dblocks != blocks) dev_free(ctx->dev, t->dblocks);
+        t->dblocks = blocks;
+        t->dcap = n;
+        t->dev_valid = true;
+        t->host_valid = false;
+        T.coefs.clear();
+    } else {
+        T.coefs.resize(n);
+        std::string err;
+        const int rc = dev_download(ctx->dev, blocks, n, T.coefs.data(), err);
+        dev_free(ctx->dev, blocks);
+        dev_check(rc, err);
+        ctx->d2h_bytes += (long long)(sizeof(double) * n);
+        t->host_valid = true;
+        t->dev_valid = false;
+    }
+    double sq = 0.0;
+    for (int i = 0; i < T.size(); i++) sq += w2[i] + (T.parent[i] < 0 ? s2[i] : 0.0);
+    T.sqnorm = sq;
+    t->s2 = std::move(s2);
+    t->w2 = std::move(w2);
+    t->norms_valid = true;
+    t->cn8_valid = false;
+}
+
+// An input must hold finite values (checked on its node norms).
+void check_finite(mwg_tree t, const char *what) {
+    ensure_norms(t);
+    for (size_t i = 0; i < t->s2.size(); i++)
+        if (!std::isfinite(t->s2[i] + t->w2[i])) throw std::invalid_argument(std::string(what) + " holds NaN or Inf");
+}
+
+// Device memory for a result of n doubles, freed on failure.
+struct DeviceResult {
+    DeviceCtx *c = nullptr;
+    double *p = nullptr;
+    ~DeviceResult() { dev_free(c, p); }
+    double *release() {
+        double *q = p;
+        p = nullptr;
+        return q;
+    }
+};
 
 // Room for n nodes in t (one allocation instead of repeated growth).
 void reserve_nodes(Tree &t, size_t n) {
@@ -193,7 +391,10 @@ struct Job {
     mwg_tree_s *out = nullptr;
     mwg_operator_s *op = nullptr;
     const Tree *inp = nullptr;
-    std::vector<const Tree *> prec_trees;
+    mwg_tree inp_h = nullptr;
+    const double *inp_cn8 = nullptr;  // the input's component norms (set by validate_jobs)
+    std::vector<mwg_tree> prec_h;
+    std::vector<PrecTree> prec_trees;  // filled by validate_jobs
     double prec = 0.0;
     int max_iter = -1;
     bool abs_prec = false;
@@ -205,10 +406,10 @@ struct Job {
     std::string err;
 };
 
-void validate_jobs(const std::vector<Job> &jobs) {
+void validate_jobs(std::vector<Job> &jobs) {
     std::set<const void *> outs, ins;
     for (const Job &j : jobs) ins.insert(j.inp);
-    for (const Job &j : jobs) {
+    for (Job &j : jobs) {
         if (!(j.prec >= 0.0) || !std::isfinite(j.prec))
             throw std::invalid_argument("apply: precision must be finite and >= 0");
         check_compatible(j.out->tree, j.op->op, *j.inp);
@@ -216,18 +417,31 @@ void validate_jobs(const std::vector<Job> &jobs) {
             throw std::invalid_argument("apply: all applies of a batch must share one MRA");
         if (!outs.insert(&j.out->tree).second) throw std::invalid_argument("apply: an output tree appears twice");
         if (ins.count(&j.out->tree)) throw std::invalid_argument("apply: a tree is both output and input");
-        for (const Tree *t : j.prec_trees)
-            if (t->mra != j.op->op.mra) throw std::invalid_argument("apply: prec tree on another MRA");
+        for (mwg_tree t : j.prec_h)
+            if (t->tree.mra != j.op->op.mra) throw std::invalid_argument("apply: prec tree on another MRA");
+    }
+    for (Job &j : jobs) {
+        check_finite(j.inp_h, "apply: input tree");
+        j.inp_cn8 = ensure_cnorms(j.inp_h);
+        j.prec_trees.clear();
+        for (mwg_tree t : j.prec_h) {
+            if (outs.count(&t->tree)) throw std::invalid_argument("apply: a prec tree is an output of the batch");
+            ensure_norms(t);
+            j.prec_trees.push_back(PrecTree{&t->tree, &t->s2, &t->w2});
+        }
     }
 }
 
-// Device copies of the input trees' blocks of a batch, one per distinct
-// tree, freed with the batch.
+// The input trees' blocks of a batch on the device, one copy per distinct
+// tree: a resident tree's own, else a temporary upload freed with the batch.
 struct InputBlocks {
-    DeviceCtx *c = nullptr;
-    std::map<const Tree *, double *> dev;
-    ~InputBlocks() {
-        for (auto &kv : dev) dev_free(c, kv.second);
+    TempBlocks tmp;
+    std::map<mwg_tree, const double *> dev;
+    explicit InputBlocks(mwg_context ctx) : tmp{ctx->dev, {}} {}
+    const double *of(mwg_context ctx, mwg_tree t) {
+        auto it = dev.find(t);
+        if (it != dev.end()) return it->second;
+        return dev[t] = tmp.of(ctx, t);
     }
 };
 
@@ -237,30 +451,31 @@ struct InputBlocks {
 // parts receives the device plan parts, whose pools (pools) the caller frees
 // after dev_plan_create.
 void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<std::vector<int>> &nodes,
-                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, std::vector<double *> &pools) {
+                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, std::vector<double *> &pools,
+                 double *ms3 = nullptr) {
     const int n = (int)active.size();
+    const double t0 = now_ms();
 #pragma omp parallel for schedule(dynamic, 1) if (n > 1)
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
         try {
-            build_plan(j.plan, j.op->op, *j.inp, j.out->tree, nodes[i], thr[i], 64, 256, true);
+            build_plan(j.plan, j.op->op, *j.inp, j.out->tree, nodes[i], thr[i], 64, 256, true, j.inp_cn8);
         } catch (const std::exception &e) {
             j.err = e.what();
         }
     }
     for (Job *j : active)
         if (!j->err.empty()) throw std::runtime_error(j->err);
+    const double t1 = now_ms();
     pools.assign(n, nullptr);
     const MRA &mra = *active[0]->out->tree.mra;
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
         std::string err;
-        double *&blocks = inputs.dev[j.inp];
-        int rc = 0;
-        if (!blocks) rc = dev_upload(ctx->dev, j.inp->coefs.data(), j.inp->coefs.size(), &blocks, err);
-        if (!rc) rc = dev_pool_fill(ctx->dev, mra, blocks, j.plan, &pools[i], err);
-        if (rc) throw std::runtime_error(err);
+        const double *blocks = inputs.of(ctx, j.inp_h);
+        dev_check(dev_pool_fill(ctx->dev, mra, blocks, j.plan, &pools[i], err), err);
     }
+    const double t2 = now_ms();
 #pragma omp parallel for schedule(dynamic, 1) if (n > 1)
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
@@ -272,6 +487,11 @@ void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<
     }
     for (Job *j : active)
         if (!j->err.empty()) throw std::runtime_error(j->err);
+    if (ms3) {
+        ms3[0] += t1 - t0;
+        ms3[1] += t2 - t1;
+        ms3[2] += now_ms() - t2;
+    }
 }
 
 // Frees the device pools of a level's plans.
@@ -305,6 +525,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
                 o.structure_only = false;
                 o.clear();
                 o.make_roots();
+                host_changed(j.out);
                 j.out->apply_thr.clear();
             }
         }
@@ -313,6 +534,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         Tree &o = j.out->tree;
         o.structure_only = false;
         o.clear();
+        host_changed(j.out);
         // room for an output grid up to twice the input's size without
         // regrowing the structure (larger grids still grow)
         reserve_nodes(o, std::max<size_t>(64, 2 * (size_t)j.inp->size()));
@@ -322,10 +544,11 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         for (int i = 0; i < o.size(); i++) j.work.push_back(i);
         j.thr_all.assign(o.size(), 0.0);
     }
-    InputBlocks inputs;
-    inputs.c = ctx->dev;
+    InputBlocks inputs(ctx);
     double kms = 0.0, pms = 0.0, slots = 0.0, gen = 0.0, computed = 0.0, apps_total = 0.0;
     double t_dev = 0.0, t_split = 0.0;  // device plan create + run + download; host split checks
+    double t_create = 0.0, t_run = 0.0;  // parts of t_dev: plan creation, run (both host wall time)
+    double plan3[3] = {0.0, 0.0, 0.0};   // parts of the plan time: grids and bands, device pools, counts
     const double t_all = now_ms();
     int launches = 0;
     // Every level's output blocks stay on the device until the post
@@ -358,7 +581,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         const double t0 = now_ms();
         std::vector<double *> pools;
         LevelPools free_pools{ctx->dev, pools};
-        build_plans(ctx, active, nodes, thr, inputs, pools);
+        build_plans(ctx, active, nodes, thr, inputs, pools, plan3);
         pms += now_ms() - t0;
         std::vector<PlanPart> parts;
         size_t total = 0;
@@ -374,9 +597,12 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         const double t1 = now_ms();
         int rc = dev_plan_create(ctx->dev, parts, &dp, err);
         if (rc) return dev_fail(rc, err);
+        const double t1r = now_ms();
+        t_create += t1r - t1;
         double ms = 0.0;
         unsigned long long apps = 0;
         rc = dev_plan_run(dp, ctx->mode, ctx->tau, &ms, &apps, nullptr, err, nullptr);
+        t_run += now_ms() - t1r;
         // the level's blocks stay on the device; the host takes their norms
         double *lvl = rc ? nullptr : dev_plan_take_output(dp);
         dev_plan_destroy(dp);
@@ -482,33 +708,59 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
             w2[l].resize(keep.count[l]);
             rc = dev_node_norms(ctx->dev, mra.kp1(), keep.bufs[l], keep.count[l], s2[l].data(), w2[l].data(), err);
         }
-        // the blocks into the trees, in runs of nodes adjacent in both
+        // the blocks into the trees, in runs of nodes adjacent in both: into
+        // the device copy of an output resident on this context, else
+        // downloaded into the host tree
         for (size_t ji = 0; ji < jobs.size() && !rc; ji++) {
-            Tree &o = jobs[ji].out->tree;
-            o.structure_only = false;
-            o.coefs.resize((size_t)o.size() * blk);  // every block is written below
+            mwg_tree out = jobs[ji].out;
+            Tree &o = out->tree;
             const std::vector<NodeLoc> &loc = locs[ji];
+            const bool resident = out->dctx == ctx;
+            DeviceResult dres{ctx->dev, nullptr};
+            if (resident) {
+                rc = dev_alloc(ctx->dev, (size_t)o.size() * blk, &dres.p, err);
+            } else {
+                o.structure_only = false;
+                o.coefs.resize((size_t)o.size() * blk);  // every block is written below
+            }
             int i = 0;
             while (i < o.size() && !rc) {
                 int e = i + 1;
                 while (e < o.size() && loc[e].level == loc[i].level && loc[e].pos == loc[i].pos + (e - i)) e++;
-                rc = dev_download(ctx->dev, addr(loc[i]), (size_t)(e - i) * blk, o.node(i), err);
+                const size_t nv = (size_t)(e - i) * blk;
+                if (resident) {
+                    rc = dev_d2d(ctx->dev, dres.p + (size_t)i * blk, addr(loc[i]), nv, err);
+                } else {
+                    rc = dev_download(ctx->dev, addr(loc[i]), nv, o.node(i), err);
+                    ctx->d2h_bytes += (long long)(sizeof(double) * nv);
+                }
                 i = e;
             }
-            double sq = 0.0;
+            if (rc) break;
+            std::vector<double> ns2(o.size()), nw2(o.size());
             for (int n = 0; n < o.size(); n++) {
                 const NodeLoc &l = loc[n];
-                sq += w2[l.level][l.pos];
-                if (o.parent[n] < 0) sq += s2[l.level][l.pos];
+                ns2[n] = s2[l.level][l.pos];
+                nw2[n] = w2[l.level][l.pos];
             }
-            o.sqnorm = sq;
-            jobs[ji].out->apply_thr = jobs[ji].thr_all;
+            if (resident) {
+                take_device_result(ctx, out, dres.release(), std::move(ns2), std::move(nw2));
+            } else {
+                double sq = 0.0;
+                for (int n = 0; n < o.size(); n++) sq += nw2[n] + (o.parent[n] < 0 ? ns2[n] : 0.0);
+                o.sqnorm = sq;
+                host_changed(out);
+                out->s2 = std::move(ns2);
+                out->w2 = std::move(nw2);
+                out->norms_valid = true;
+            }
+            out->apply_thr = jobs[ji].thr_all;
         }
         if (rc) return dev_fail(rc, err);
     }
     guard.done = true;
     auto &st = ctx->last_stats;
-    st.assign(12, 0.0);
+    st.assign(16, 0.0);
     st[0] = kms;
     st[1] = computed;
     st[2] = slots;
@@ -520,6 +772,11 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
     st[8] = t_split;
     st[9] = now_ms() - t3;
     st[10] = now_ms() - t_all;
+    st[11] = t_create;
+    st[12] = t_run;
+    st[13] = plan3[0];
+    st[14] = plan3[1];
+    st[15] = plan3[2];
     return MWG_OK;
 }
 
@@ -536,6 +793,7 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
         jobs[i].out = out[i];
         jobs[i].op = op[i];
         jobs[i].inp = &inp[i]->tree;
+        jobs[i].inp_h = inp[i];
     }
     validate_jobs(jobs);
     auto p = std::make_unique<mwg_plan_s>();
@@ -554,8 +812,7 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
         active.push_back(&jobs[i]);
     }
     const double t0 = now_ms();
-    InputBlocks inputs;
-    inputs.c = ctx->dev;
+    InputBlocks inputs(ctx);
     std::vector<double *> pools;
     LevelPools free_pools{ctx->dev, pools};
     build_plans(ctx, active, nodes, th, inputs, pools);
@@ -636,6 +893,14 @@ int plan_finish(mwg_plan plan) {
     const int blk = mra.block();
     std::string err;
     DeviceCtx *dc = plan->ctx->dev;
+    // the results go into the host trees (a resident output's device copy
+    // becomes stale)
+    for (mwg_tree o : plan->outs) {
+        o->tree.structure_only = false;
+        o->tree.coefs.resize(tree_values(o));
+        host_changed(o);
+        plan->ctx->d2h_bytes += (long long)(sizeof(double) * tree_values(o));
+    }
     if (plan->post_done) {
         // the blocks were transformed on the device by mwg_plan_apply
         std::vector<double> s2(plan->total_out), w2(plan->total_out);
@@ -725,6 +990,8 @@ int mwg_context_create(int device, mwg_context *ctx) {
 int mwg_context_destroy(mwg_context ctx) {
     if (!ctx) return MWG_OK;
     if (ctx->plan_refs > 0) return fail(MWG_ERR_ARG, "context destroy: a plan on this context is still alive");
+    if (ctx->resident_refs > 0)
+        return fail(MWG_ERR_ARG, "context destroy: trees are still resident on this context (mwg_tree_set_resident)");
     {
         std::lock_guard<std::mutex> g(g_ctx_mtx);
         g_contexts.erase(ctx);
@@ -917,38 +1184,138 @@ std::vector<int> by_depth(const Tree &f) {
     return order;
 }
 
+// Blocks of a tree grown level by level on the device (products and sums
+// whose output stays on the device): each level's blocks in a buffer of its
+// own, their norms on the host for the split decisions, and at the end all
+// blocks in one array in tree order.
+struct DeviceLevels {
+    DeviceCtx *c;
+    int blk, kp1;
+    std::vector<double *> bufs;
+    std::vector<std::vector<int>> nodes;  // tree nodes of each level, in buffer order
+    std::vector<double> s2, w2;           // per tree node, of its last computed block
+    ~DeviceLevels() {
+        for (double *p : bufs) dev_free(c, p);
+    }
+    double *add_level(const std::vector<int> &work) {
+        double *p = nullptr;
+        std::string err;
+        dev_check(dev_alloc(c, work.size() * (size_t)blk, &p, err), err);
+        bufs.push_back(p);
+        nodes.push_back(work);
+        return p;
+    }
+    // the norms of the last level's blocks, tree_size nodes in the tree now
+    void level_norms(int tree_size) {
+        s2.resize(tree_size, 0.0);
+        w2.resize(tree_size, 0.0);
+        const std::vector<int> &w = nodes.back();
+        std::vector<double> a(w.size()), b(w.size());
+        std::string err;
+        dev_check(dev_node_norms(c, kp1, bufs.back(), (long long)w.size(), a.data(), b.data(), err), err);
+        for (size_t q = 0; q < w.size(); q++) {
+            s2[w[q]] = a[q];
+            w2[w[q]] = b[q];
+        }
+    }
+    // every level's blocks into one array of tree_size blocks in tree order
+    double *assemble(int tree_size) {
+        DeviceResult r{c, nullptr};
+        std::string err;
+        dev_check(dev_alloc(c, (size_t)tree_size * blk, &r.p, err), err);
+        for (size_t l = 0; l < bufs.size(); l++) {
+            dev_check(dev_scatter_blocks(c, bufs[l], nodes[l], blk, r.p, err), err);
+            dev_free(c, bufs[l]);
+            bufs[l] = nullptr;
+        }
+        return r.release();
+    }
+};
+
+// A failed product or sum leaves its output as the zero function on its
+// roots.
+struct ResetOnFailure {
+    mwg_tree t;
+    bool done = false;
+    ~ResetOnFailure() {
+        if (done) return;
+        t->tree.structure_only = false;
+        t->tree.clear();
+        t->tree.make_roots();
+        host_changed(t);
+    }
+};
+
+// An input tree's blocks for a product or sum on ctx: in device memory when
+// the tree is resident on ctx (on_dev), else the host copy.  noalloc: never
+// an upload into a resident copy (that could allocate).
+const double *input_blocks(mwg_context ctx, mwg_tree t, TempBlocks &tb, bool noalloc, bool &on_dev) {
+    if (t->dctx == ctx && (t->dev_valid || !noalloc)) {
+        on_dev = true;
+        return noalloc ? t->dblocks : tb.of(ctx, t);
+    }
+    ensure_host(t);
+    on_dev = false;
+    ctx->h2d_bytes += (long long)(sizeof(double) * tree_values(t));
+    return t->tree.coefs.data();
+}
+
+// The union of the grids of ins in T (which holds its roots).
+int union_grid(Tree &T, const std::vector<const Tree *> &ins) {
+    for (const Tree *f : ins) {
+        std::vector<int> order(f->size());
+        for (int i = 0; i < f->size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return f->keys[x].depth < f->keys[y].depth; });
+        for (int i : order) {
+            const NodeKey &k = f->keys[i];
+            if (k.depth == 0 || T.find(k.depth, k.l) >= 0) continue;
+            long long lp[3] = {k.l[0] >> 1, k.l[1] >> 1, k.l[2] >> 1};
+            const int par = T.find(k.depth - 1, lp);
+            if (par < 0) return fail(MWG_ERR_ARG, "input tree is not complete from its roots");
+            T.split(par);
+        }
+    }
+    return MWG_OK;
+}
+
 // T = sum_i coefs[i] ins[i] (mrcpp::add).  prec < 0: on the union of the
 // input grids, every node computed in one launch.  prec >= 0: built from the
 // roots, level by level, splitting a node when the sum's wavelet norm fails
 // the split check (absolute, or relative to the sum's norm estimated from the
 // leaves), for at most max_iter levels (< 0: no limit) and the MRA's depth.
-int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tree *> &ins,
-              const std::vector<double> &coefs, int max_iter, bool abs_prec) {
+// With dres the sum is built on the device and its blocks (tree order) and
+// node norms are returned there; otherwise each level's blocks are
+// downloaded into T.
+struct DeviceSum {
+    double *blocks = nullptr;
+    std::vector<double> s2, w2;
+};
+int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<mwg_tree> &ins,
+              const std::vector<double> &coefs, int max_iter, bool abs_prec, DeviceSum *dres) {
     const MRA &mra = *T.mra;
     const int blk = mra.block(), n3 = mra.kp1_3();
-    std::vector<double> pool;
+    const bool on_device = dres != nullptr;
+    TempBlocks tb{ctx->dev, {}};
+    std::vector<BlockSource> pool;
     std::vector<long long> base(ins.size());
     size_t total = 0;
-    for (const Tree *f : ins) total += f->coefs.size();
-    pool.reserve(total);
     for (size_t i = 0; i < ins.size(); i++) {
-        base[i] = (long long)pool.size();
-        pool.insert(pool.end(), ins[i]->coefs.begin(), ins[i]->coefs.end());
+        base[i] = (long long)total;
+        bool on_dev = false;
+        const double *p = input_blocks(ctx, ins[i], tb, false, on_dev);
+        pool.push_back(BlockSource{p, tree_values(ins[i]), on_dev});
+        total += tree_values(ins[i]);
     }
+    T.structure_only = false;
     T.clear();
     // the union grid holds at most all input nodes
     if (prec < 0.0) reserve_nodes(T, total / (size_t)blk);
+    T.structure_only = on_device;
     T.make_roots();
     if (prec < 0.0) {
-        for (const Tree *f : ins)
-            for (int i : by_depth(*f)) {
-                const NodeKey &k = f->keys[i];
-                if (k.depth == 0 || T.find(k.depth, k.l) >= 0) continue;
-                long long lp[3] = {k.l[0] >> 1, k.l[1] >> 1, k.l[2] >> 1};
-                const int par = T.find(k.depth - 1, lp);
-                if (par < 0) return fail(MWG_ERR_ARG, "input tree is not complete from its roots");
-                T.split(par);
-            }
+        std::vector<const Tree *> grids;
+        for (mwg_tree f : ins) grids.push_back(&f->tree);
+        if (int rc = union_grid(T, grids)) return rc;
     }
     std::vector<int> work;
     if (prec < 0.0) {
@@ -959,41 +1326,52 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tre
     }
     std::string err;
     DevAddSession *m = nullptr;
-    int rc = dev_add_begin(ctx->dev, mra, pool, coefs, &m, err);
-    if (rc) return dev_fail(rc, err);
+    dev_check(dev_add_begin(ctx->dev, mra, pool, coefs, &m, err), err);
+    std::unique_ptr<DevAddSession, void (*)(DevAddSession *)> session(m, dev_add_end);
+    DeviceLevels lv{ctx->dev, blk, mra.kp1(), {}, {}, {}, {}};
     const size_t n_in = ins.size();
     for (int iter = 0;; iter++) {
         std::vector<AddSrc> src(work.size() * n_in);
         for (size_t w = 0; w < work.size(); w++)
             for (size_t i = 0; i < n_in; i++)
-                if (!add_source(*ins[i], T.keys[work[w]], base[i], src[w * n_in + i])) {
-                    dev_add_end(m);
+                if (!add_source(ins[i]->tree, T.keys[work[w]], base[i], src[w * n_in + i]))
                     return fail(MWG_ERR_ARG, "output node more than " + std::to_string(kAddMaxLevels) +
                                                  " levels below an input's finest node");
-                }
-        std::vector<double> blocks(work.size() * (size_t)blk);
-        rc = dev_add_nodes(m, src, blocks.data(), err);
-        if (rc) {
-            dev_add_end(m);
-            return dev_fail(rc, err);
-        }
+        if (on_device) {
+            double *buf = lv.add_level(work);
+            dev_check(dev_add_nodes(m, src, buf, err, true), err);
+            lv.level_norms(T.size());
+        } else {
+            std::vector<double> blocks(work.size() * (size_t)blk);
+            dev_check(dev_add_nodes(m, src, blocks.data(), err), err);
+            ctx->d2h_bytes += (long long)(sizeof(double) * blocks.size());
 #pragma omp parallel for schedule(static)
-        for (long long w = 0; w < (long long)work.size(); w++)
-            std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
+            for (long long w = 0; w < (long long)work.size(); w++)
+                std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
+        }
         if (prec < 0.0 || (max_iter >= 0 && iter >= max_iter)) break;
         double tnorm = 1.0;
         if (!abs_prec) {
             double sq = 0.0;
             for (int i = 0; i < T.size(); i++)
-                if (T.child0[i] < 0)
-                    for (int e = 0; e < n3; e++) sq += T.node(i)[e] * T.node(i)[e];
+                if (T.child0[i] < 0) {
+                    if (on_device) {
+                        sq += lv.s2[i];
+                    } else {
+                        for (int e = 0; e < n3; e++) sq += T.node(i)[e] * T.node(i)[e];
+                    }
+                }
             tnorm = std::sqrt(sq);
         }
         std::vector<int> next;
         for (int wi : work) {
             const int d = T.keys[wi].depth;
             double wsq = 0.0;
-            for (int e = n3; e < blk; e++) wsq += T.node(wi)[e] * T.node(wi)[e];
+            if (on_device) {
+                wsq = lv.w2[wi];
+            } else {
+                for (int e = n3; e < blk; e++) wsq += T.node(wi)[e] * T.node(wi)[e];
+            }
             if (d < mra.max_depth && split_check(mra, d, std::sqrt(wsq), prec, tnorm)) {
                 const int c0 = T.split(wi);
                 for (int t = 0; t < 8; t++) next.push_back(c0 + t);
@@ -1002,18 +1380,74 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<const Tre
         if (next.empty()) break;
         work.swap(next);
     }
-    dev_add_end(m);
-    T.sqnorm = tree_square_norm(T);
+    session.reset();
+    if (on_device) {
+        // every node was computed once: its level's norms are its own
+        dres->blocks = lv.assemble(T.size());
+        dres->s2 = std::move(lv.s2);
+        dres->w2 = std::move(lv.w2);
+        dres->s2.resize(T.size(), 0.0);
+        dres->w2.resize(T.size(), 0.0);
+    } else {
+        T.sqnorm = tree_square_norm(T);
+    }
     return MWG_OK;
 }
 }  // namespace
 
 namespace {
+// Max norms of a tree for the product's max-norm refinement: per node, the
+// largest 2^{3n} ||block||^2 (ms) and 2^{3n} ||wavelet part||^2 (mw) over the
+// node and its descendants, n = the node's scale (root scale + depth): the
+// scaled squared norm estimates the squared sup-norm over the node's box.
+void max_norms(mwg_tree t, std::vector<double> &ms, std::vector<double> &mw) {
+    ensure_norms(t);
+    const Tree &T = t->tree;
+    const int n = T.size();
+    ms.assign(n, 0.0);
+    mw.assign(n, 0.0);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return T.keys[x].depth > T.keys[y].depth; });
+    for (int i : order) {
+        const double f = std::pow(2.0, 3.0 * (T.mra->n0 + T.keys[i].depth));
+        ms[i] = std::max(ms[i], f * (t->s2[i] + t->w2[i]));
+        mw[i] = std::max(mw[i], f * t->w2[i]);
+        if (T.parent[i] >= 0) {
+            ms[T.parent[i]] = std::max(ms[T.parent[i]], ms[i]);
+            mw[T.parent[i]] = std::max(mw[T.parent[i]], mw[i]);
+        }
+    }
+}
+
+// The max norms of input f at output node k: f's node at k, or below f's
+// finest node on the path that node's scaling max with no wavelet part
+// (f is a polynomial there).
+void max_norms_at(const Tree &f, const std::vector<double> &ms, const std::vector<double> &mw, const NodeKey &k,
+                  double &s, double &w) {
+    s = w = 0.0;
+    for (int d = k.depth; d >= 0; d--) {
+        long long l[3];
+        for (int q = 0; q < 3; q++) l[q] = k.l[q] >> (k.depth - d);
+        const int idx = f.find(d, l);
+        if (idx < 0) continue;
+        s = std::sqrt(ms[idx]);
+        w = d == k.depth ? std::sqrt(mw[idx]) : 0.0;
+        return;
+    }
+}
+
 // mrcpp::multiply on the device: the output grid from the union of the
-// inputs' grids, refined level by level with the split rule.  noalloc: the
-// device buffers come from the device's scratch array.
+// inputs' grids, refined level by level with the split rule.  With
+// max_norms (useMaxNorms) the grid instead grows from the roots with the
+// product rule of the reference's MultiplicationAdaptor: a node splits when
+// maxW_a maxS_b + maxW_b maxS_a > prec, from the inputs' max norms.  noalloc: the
+// device buffers come from the device's scratch array.  An output resident
+// on ctx is built on the device (levels, BottomUp transform and norms there)
+// and only the node norms come to the host; otherwise each level's blocks are
+// downloaded and the tree is compressed on the host.
 int multiply_trees(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
-                   int abs_prec, bool noalloc) {
+                   int abs_prec, bool noalloc, bool use_max_norms = false) {
     return guarded_ctx(ctx, [&] {
         if (!ctx || !out || !a || !b) return fail(MWG_ERR_ARG, "null pointer argument");
         if (out == a || out == b) return fail(MWG_ERR_ARG, "the output tree must differ from the inputs");
@@ -1021,71 +1455,100 @@ int multiply_trees(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_
         if (a->tree.mra != &mra || b->tree.mra != &mra) return fail(MWG_ERR_ARG, "trees of different MRAs");
         if (mra.basis.type != BASIS_INTERPOLATING) return fail(MWG_ERR_ARG, "multiplication needs the interpolating basis");
         if (!(prec >= 0.0) || !std::isfinite(coef)) return fail(MWG_ERR_ARG, "bad precision or coefficient");
+        const bool on_device = out->dctx == ctx && !noalloc;
         Tree &T = out->tree;
+        T.structure_only = false;
         T.clear();
+        host_changed(out);
+        ResetOnFailure reset{out};
         // the union grid holds at most all input nodes
         reserve_nodes(T, (size_t)a->tree.size() + (size_t)b->tree.size());
+        T.structure_only = on_device;
         T.make_roots();
-        // the union of the input grids
-        for (const Tree *f : {&a->tree, &b->tree}) {
-            std::vector<int> order(f->size());
-            for (int i = 0; i < f->size(); i++) order[i] = i;
-            std::stable_sort(order.begin(), order.end(),
-                             [&](int x, int y) { return f->keys[x].depth < f->keys[y].depth; });
-            for (int i : order) {
-                const NodeKey &k = f->keys[i];
-                if (k.depth == 0 || T.find(k.depth, k.l) >= 0) continue;
-                long long lp[3] = {k.l[0] >> 1, k.l[1] >> 1, k.l[2] >> 1};
-                const int par = T.find(k.depth - 1, lp);
-                if (par < 0) return fail(MWG_ERR_ARG, "input tree is not complete from its roots");
-                T.split(par);
-            }
+        std::vector<double> ms_a, mw_a, ms_b, mw_b;
+        if (use_max_norms) {
+            max_norms(a, ms_a, mw_a);
+            max_norms(b, ms_b, mw_b);
+        } else if (int rc = union_grid(T, {&a->tree, &b->tree})) {
+            return rc;
         }
         std::vector<int> work;
         for (int i = 0; i < T.size(); i++)
             if (T.child0[i] < 0) work.push_back(i);
         std::string err;
+        TempBlocks tb{ctx->dev, {}};
+        bool a_dev = false, b_dev = false;
+        const double *pa = input_blocks(ctx, a, tb, noalloc, a_dev);
+        const double *pb = input_blocks(ctx, b, tb, noalloc, b_dev);
         DevMulSession *m = nullptr;
-        int rc = dev_mul_begin(ctx->dev, mra, a->tree.coefs.data(), a->tree.coefs.size(), b->tree.coefs.data(),
-                               b->tree.coefs.size(), &m, err, noalloc);
-        if (rc) return dev_fail(rc, err);
+        dev_check(dev_mul_begin(ctx->dev, mra, pa, tree_values(a), pb, tree_values(b), &m, err, noalloc, a_dev, b_dev),
+                  err);
+        std::unique_ptr<DevMulSession, void (*)(DevMulSession *)> session(m, dev_mul_end);
         const int blk = mra.block(), n3 = mra.kp1_3();
+        DeviceLevels lv{ctx->dev, blk, mra.kp1(), {}, {}, {}, {}};
         for (int iter = 0;; iter++) {
             std::vector<MulNode> nodes(work.size());
             for (size_t w = 0; w < work.size(); w++) {
                 const NodeKey &k = T.keys[work[w]];
                 MulNode &nd = nodes[w];
                 if (!mul_source(a->tree, k, nd.src_a, nd.dep_a, nd.path_a) ||
-                    !mul_source(b->tree, k, nd.src_b, nd.dep_b, nd.path_b)) {
-                    dev_mul_end(m);
+                    !mul_source(b->tree, k, nd.src_b, nd.dep_b, nd.path_b))
                     return fail(MWG_ERR_ARG, "output node more than 10 levels below an input's finest node");
-                }
                 nd.hfac = std::pow(mra.box_size(k.depth + 1), -1.5);
             }
-            std::vector<double> blocks(work.size() * (size_t)blk);
-            rc = dev_mul_nodes(m, nodes, coef, blocks.data(), err);
-            if (rc) {
-                dev_mul_end(m);
-                return dev_fail(rc, err);
-            }
+            if (on_device) {
+                double *buf = lv.add_level(work);
+                dev_check(dev_mul_nodes(m, nodes, coef, buf, err, true), err);
+                lv.level_norms(T.size());
+            } else {
+                std::vector<double> blocks(work.size() * (size_t)blk);
+                dev_check(dev_mul_nodes(m, nodes, coef, blocks.data(), err), err);
+                ctx->d2h_bytes += (long long)(sizeof(double) * blocks.size());
 #pragma omp parallel for schedule(static)
-            for (long long w = 0; w < (long long)work.size(); w++)
-                std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
+                for (long long w = 0; w < (long long)work.size(); w++)
+                    std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
+            }
             if (max_iter >= 0 && iter >= max_iter) break;
+            if (use_max_norms) {
+                std::vector<int> next;
+                for (int wi : work) {
+                    const NodeKey k = T.keys[wi];
+                    if (k.depth >= mra.max_depth) continue;
+                    double sa, wa, sb, wb;
+                    max_norms_at(a->tree, ms_a, mw_a, k, sa, wa);
+                    max_norms_at(b->tree, ms_b, mw_b, k, sb, wb);
+                    if (wa * sb + wb * sa > prec) {
+                        const int c0 = T.split(wi);
+                        for (int t = 0; t < 8; t++) next.push_back(c0 + t);
+                    }
+                }
+                if (next.empty()) break;
+                work.swap(next);
+                continue;
+            }
             // norm estimate of the product from the leaves' scaling parts
             double tnorm = 1.0;
             if (!abs_prec) {
                 double sq = 0.0;
                 for (int i = 0; i < T.size(); i++)
-                    if (T.child0[i] < 0)
-                        for (int e = 0; e < n3; e++) sq += T.node(i)[e] * T.node(i)[e];
+                    if (T.child0[i] < 0) {
+                        if (on_device) {
+                            sq += lv.s2[i];
+                        } else {
+                            for (int e = 0; e < n3; e++) sq += T.node(i)[e] * T.node(i)[e];
+                        }
+                    }
                 tnorm = std::sqrt(sq);
             }
             std::vector<int> next;
             for (int wi : work) {
                 const int d = T.keys[wi].depth;
                 double wsq = 0.0;
-                for (int e = n3; e < blk; e++) wsq += T.node(wi)[e] * T.node(wi)[e];
+                if (on_device) {
+                    wsq = lv.w2[wi];
+                } else {
+                    for (int e = n3; e < blk; e++) wsq += T.node(wi)[e] * T.node(wi)[e];
+                }
                 if (d < mra.max_depth && split_check(mra, d, std::sqrt(wsq), prec, tnorm)) {
                     const int c0 = T.split(wi);
                     for (int t = 0; t < 8; t++) next.push_back(c0 + t);
@@ -1094,10 +1557,25 @@ int multiply_trees(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_
             if (next.empty()) break;
             work.swap(next);
         }
-        dev_mul_end(m);
-        tree_compress(T);
-        T.sqnorm = tree_square_norm(T);
-        return MWG_OK;
+        session.reset();
+        if (on_device) {
+            DeviceResult res{ctx->dev, lv.assemble(T.size())};
+            // BottomUp: every branch's block from its children's scaling parts
+            std::vector<std::vector<PostPair>> levels(T.max_depth() + 1);
+            for (int i = 0; i < T.size(); i++)
+                if (T.child0[i] >= 0)
+                    levels[T.keys[i].depth].push_back(
+                        PostPair{res.p + (size_t)i * blk, res.p + (size_t)T.child0[i] * blk});
+            dev_check(dev_post_transform(ctx->dev, mra, levels, err, false), err);
+            std::vector<double> s2(T.size()), w2(T.size());
+            dev_check(dev_node_norms(ctx->dev, mra.kp1(), res.p, T.size(), s2.data(), w2.data(), err), err);
+            take_device_result(ctx, out, res.release(), std::move(s2), std::move(w2));
+        } else {
+            tree_compress(T);
+            T.sqnorm = tree_square_norm(T);
+        }
+        reset.done = true;
+        return (int)MWG_OK;
     });
 }
 }  // namespace
@@ -1112,22 +1590,39 @@ int mwg_multiply_noalloc(mwg_context ctx, double prec, mwg_tree out, double coef
     return multiply_trees(ctx, prec, out, coef, a, b, max_iter, abs_prec, true);
 }
 
+int mwg_multiply_ex(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_tree a, mwg_tree b, int max_iter,
+                    int abs_prec, int flags) {
+    if (flags & ~MWG_MUL_MAX_NORMS) return fail(MWG_ERR_ARG, "unknown multiply flags");
+    return multiply_trees(ctx, prec, out, coef, a, b, max_iter, abs_prec, false, (flags & MWG_MUL_MAX_NORMS) != 0);
+}
+
 int mwg_add(mwg_context ctx, double prec, mwg_tree out, int n, const double *coefs, const mwg_tree *inputs,
             int max_iter, int abs_prec) {
     return guarded_ctx(ctx, [&] {
         if (!ctx || !out || !coefs || !inputs) return fail(MWG_ERR_ARG, "null pointer argument");
         if (n < 1) return fail(MWG_ERR_ARG, "at least one input tree is required");
         if (std::isnan(prec)) return fail(MWG_ERR_ARG, "bad precision");
-        std::vector<const Tree *> ins(n);
+        std::vector<mwg_tree> ins(n);
         std::vector<double> cs(coefs, coefs + n);
         for (int i = 0; i < n; i++) {
             if (!inputs[i]) return fail(MWG_ERR_ARG, "null input tree");
             if (inputs[i] == out) return fail(MWG_ERR_ARG, "the output tree must differ from the inputs");
             if (inputs[i]->tree.mra != out->tree.mra) return fail(MWG_ERR_ARG, "trees of different MRAs");
             if (!std::isfinite(cs[i])) return fail(MWG_ERR_ARG, "non-finite coefficient");
-            ins[i] = &inputs[i]->tree;
+            ins[i] = inputs[i];
         }
-        return add_trees(ctx, prec, out->tree, ins, cs, max_iter, abs_prec != 0);
+        host_changed(out);
+        ResetOnFailure reset{out};
+        const bool on_device = out->dctx == ctx;
+        DeviceSum ds;
+        const int rc = add_trees(ctx, prec, out->tree, ins, cs, max_iter, abs_prec != 0, on_device ? &ds : nullptr);
+        if (rc != MWG_OK) {
+            dev_free(ctx->dev, ds.blocks);
+            return rc;
+        }
+        if (on_device) take_device_result(ctx, out, ds.blocks, std::move(ds.s2), std::move(ds.w2));
+        reset.done = true;
+        return (int)MWG_OK;
     });
 }
 
@@ -1137,18 +1632,25 @@ int mwg_tree_add(mwg_context ctx, mwg_tree tree, double coef, mwg_tree inp) {
         if (tree == inp) {
             // f + c f on f's own grid
             if (!std::isfinite(coef)) return fail(MWG_ERR_ARG, "non-finite coefficient");
-            for (double &v : tree->tree.coefs) v *= 1.0 + coef;
-            tree->tree.sqnorm = tree_square_norm(tree->tree);
-            return (int)MWG_OK;
+            return mwg_tree_rescale(tree, 1.0 + coef);
         }
         if (inp->tree.mra != tree->tree.mra) return fail(MWG_ERR_ARG, "trees of different MRAs");
         if (!std::isfinite(coef)) return fail(MWG_ERR_ARG, "non-finite coefficient");
         Tree sum;
         sum.mra = tree->tree.mra;
-        const std::vector<const Tree *> ins = {&tree->tree, &inp->tree};
-        const int rc = add_trees(ctx, -1.0, sum, ins, {1.0, coef}, -1, true);
-        if (rc != MWG_OK) return rc;
+        const bool on_device = tree->dctx == ctx;
+        DeviceSum ds;
+        const int rc = add_trees(ctx, -1.0, sum, {tree, inp}, {1.0, coef}, -1, true, on_device ? &ds : nullptr);
+        if (rc != MWG_OK) {
+            dev_free(ctx->dev, ds.blocks);
+            return rc;
+        }
         tree->tree = std::move(sum);
+        if (on_device) {
+            take_device_result(ctx, tree, ds.blocks, std::move(ds.s2), std::move(ds.w2));
+        } else {
+            host_changed(tree);
+        }
         return (int)MWG_OK;
     });
 }
@@ -1161,7 +1663,18 @@ int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec) {
         if (T.size() == 0 || !(prec > 0.0)) return (int)MWG_OK;
         const MRA &mra = *T.mra;
         const int blk = mra.block(), n3 = mra.kp1_3();
-        const double tnorm = abs_prec ? 1.0 : std::sqrt(std::max(tree_square_norm(T), 0.0));
+        // a tree whose blocks only the device holds is cropped there: the
+        // decisions from its node norms, the kept blocks gathered on the
+        // device
+        const bool on_device = !tree->host_valid;
+        if (on_device) ensure_norms(tree);
+        auto sqnorm = [&]() {
+            if (!on_device) return tree_square_norm(T);
+            double sq = 0.0;
+            for (int i = 0; i < T.size(); i++) sq += tree->w2[i] + (T.parent[i] < 0 ? tree->s2[i] : 0.0);
+            return sq;
+        };
+        const double tnorm = abs_prec ? 1.0 : std::sqrt(std::max(sqnorm(), 0.0));
         // post-order: a branch whose children are all leaves (after their own
         // crop) and whose wavelet norm fails the split check loses them
         std::vector<char> keep(T.size(), 1), leaf(T.size(), 0);
@@ -1176,7 +1689,11 @@ int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec) {
             for (int t = 0; t < 8; t++) all_leaves = all_leaves && leaf[T.child0[i] + t];
             if (!all_leaves) continue;
             double wsq = 0.0;
-            for (int e = n3; e < blk; e++) wsq += T.node(i)[e] * T.node(i)[e];
+            if (on_device) {
+                wsq = tree->w2[i];
+            } else {
+                for (int e = n3; e < blk; e++) wsq += T.node(i)[e] * T.node(i)[e];
+            }
             const int d = T.keys[i].depth;
             // the split check with the wavelet threshold's scale exponent
             // multiplied by split_fac (1: the WaveletAdaptor rule)
@@ -1193,7 +1710,9 @@ int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec) {
             if (T.parent[i] >= 0 && !keep[T.parent[i]]) keep[i] = 0;
         Tree C;
         C.mra = T.mra;
+        C.structure_only = on_device;
         C.make_roots();
+        std::vector<int> src_of;  // node of T behind each node of C
         for (int i : order) {
             if (!keep[i]) continue;
             const NodeKey &k = T.keys[i];
@@ -1205,10 +1724,30 @@ int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec) {
                 C.split(par);
                 j = C.find(k.depth, k.l);
             }
-            std::copy(T.node(i), T.node(i) + blk, C.node(j));
+            if ((int)src_of.size() < C.size()) src_of.resize(C.size(), -1);
+            src_of[j] = i;
+            if (!on_device) std::copy(T.node(i), T.node(i) + blk, C.node(j));
         }
-        C.sqnorm = tree_square_norm(C);
+        if (!on_device) {
+            C.sqnorm = tree_square_norm(C);
+            T = std::move(C);
+            host_changed(tree);
+            return (int)MWG_OK;
+        }
+        for (int v : src_of)
+            if (v < 0) return fail(MWG_ERR_RUNTIME, "crop: a node of the cropped tree has no source");
+        mwg_context dc = tree->dctx;
+        DeviceResult res{dc->dev, nullptr};
+        std::string err;
+        dev_check(dev_alloc(dc->dev, (size_t)C.size() * blk, &res.p, err), err);
+        dev_check(dev_gather_blocks(dc->dev, tree->dblocks, src_of, blk, res.p, err), err);
+        std::vector<double> s2(C.size()), w2(C.size());
+        for (int j = 0; j < C.size(); j++) {
+            s2[j] = tree->s2[src_of[j]];
+            w2[j] = tree->w2[src_of[j]];
+        }
         T = std::move(C);
+        take_device_result(dc, tree, res.release(), std::move(s2), std::move(w2));
         return (int)MWG_OK;
     });
 }
@@ -1285,7 +1824,53 @@ int mwg_tree_create(mwg_mra mra, mwg_tree *out) {
 
 int mwg_tree_destroy(mwg_tree t) {
     if (t && t->plan_refs > 0) return fail(MWG_ERR_ARG, "tree destroy: a plan still writes into this tree");
+    if (t && t->dctx) {
+        dev_free(t->dctx->dev, t->dblocks);
+        t->dctx->resident_refs--;
+    }
     delete t;
+    return MWG_OK;
+}
+
+int mwg_tree_set_resident(mwg_tree t, mwg_context ctx, int on) {
+    return guarded_ctx(ctx, [&] {
+        if (!t || (on && !ctx)) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (t->plan_refs > 0) return fail(MWG_ERR_ARG, "set_resident: a plan still writes into this tree");
+        if (on && t->dctx == ctx) return (int)MWG_OK;
+        if (t->dctx) {
+            // leave the current device: the host copy first
+            ensure_host(t);
+            dev_free(t->dctx->dev, t->dblocks);
+            t->dctx->resident_refs--;
+            t->dctx = nullptr;
+            t->dblocks = nullptr;
+            t->dcap = 0;
+            t->dev_valid = false;
+        }
+        if (!on) return (int)MWG_OK;
+        // the blocks move to the device at the first call on ctx that reads
+        // them (an output never uploads its old blocks)
+        t->dctx = ctx;
+        ctx->resident_refs++;
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_tree_is_resident(mwg_tree t, int *on_host, int *on_device) {
+    if (!t) return fail(MWG_ERR_ARG, "null tree");
+    if (on_host) *on_host = t->host_valid ? 1 : 0;
+    if (on_device) *on_device = t->dev_valid ? 1 : 0;
+    return t->dctx ? 1 : 0;
+}
+
+int mwg_context_transfer_bytes(mwg_context ctx, long long *h2d, long long *d2h, int reset) {
+    if (!ctx) return fail(MWG_ERR_ARG, "null context");
+    if (h2d) *h2d = ctx->h2d_bytes.load();
+    if (d2h) *d2h = ctx->d2h_bytes.load();
+    if (reset) {
+        ctx->h2d_bytes = 0;
+        ctx->d2h_bytes = 0;
+    }
     return MWG_OK;
 }
 
@@ -1299,7 +1884,9 @@ int mwg_tree_project_gaussians(mwg_tree t, double prec, int abs_prec, int max_de
         f.pos.assign(center, center + 3 * n);
         for (int i = 0; i < n; i++)
             if (!(exponent[i] > 0.0)) return fail(MWG_ERR_ARG, "Gaussian exponents must be > 0");
+        t->tree.structure_only = false;
         project_adaptive(t->tree, f, prec, abs_prec != 0, max_depth);
+        host_changed(t);
         t->apply_thr.clear();
         return MWG_OK;
     });
@@ -1312,15 +1899,20 @@ int mwg_tree_size(mwg_tree t, int *n) {
 }
 
 int mwg_tree_get(mwg_tree t, int *depth, long long *l, double *coefs) {
-    if (!t) return fail(MWG_ERR_ARG, "null tree");
-    const Tree &tr = t->tree;
-    for (int i = 0; i < tr.size(); i++) {
-        if (depth) depth[i] = tr.keys[i].depth;
-        if (l)
-            for (int d = 0; d < 3; d++) l[3 * i + d] = tr.keys[i].l[d];
-    }
-    if (coefs) std::memcpy(coefs, tr.coefs.data(), sizeof(double) * tr.coefs.size());
-    return MWG_OK;
+    return guarded([&] {
+        if (!t) return fail(MWG_ERR_ARG, "null tree");
+        const Tree &tr = t->tree;
+        for (int i = 0; i < tr.size(); i++) {
+            if (depth) depth[i] = tr.keys[i].depth;
+            if (l)
+                for (int d = 0; d < 3; d++) l[3 * i + d] = tr.keys[i].l[d];
+        }
+        if (coefs) {
+            ensure_host(t);
+            std::memcpy(coefs, tr.coefs.data(), sizeof(double) * tr.coefs.size());
+        }
+        return (int)MWG_OK;
+    });
 }
 
 int mwg_tree_set(mwg_tree t, int n, const int *depth, const long long *l, const double *coefs) {
@@ -1363,6 +1955,7 @@ int mwg_tree_set(mwg_tree t, int n, const int *depth, const long long *l, const 
         dst.sqnorm = tree_square_norm(dst);
         t->tree = std::move(dst);
         t->tree.mra = &mra;
+        host_changed(t);
         t->apply_thr.clear();
         return MWG_OK;
     });
@@ -1404,6 +1997,7 @@ int mwg_tree_pack_size(mwg_tree t, long long *bytes) {
 int mwg_tree_pack(mwg_tree t, void *buffer, long long bytes) {
     return guarded([&] {
         if (!t || !buffer) return fail(MWG_ERR_ARG, "null pointer argument");
+        ensure_host(t);
         const Tree &T = t->tree;
         const MRA &mra = *T.mra;
         const long long n = T.size();
@@ -1510,28 +2104,48 @@ int mwg_tree_load(mwg_tree t, const char *path) {
 
 int mwg_tree_square_norm(mwg_tree t, double *s) {
     if (!t || !s) return fail(MWG_ERR_ARG, "null pointer argument");
-    *s = tree_square_norm(t->tree);
+    // a tree whose blocks only the device holds: the norm its last device
+    // call computed
+    *s = t->host_valid ? tree_square_norm(t->tree) : t->tree.sqnorm;
     return MWG_OK;
 }
 
 int mwg_tree_dot(mwg_tree a, mwg_tree b, double *s) {
-    if (!a || !b || !s) return fail(MWG_ERR_ARG, "null pointer argument");
-    if (a->tree.mra != b->tree.mra) return fail(MWG_ERR_ARG, "trees on different MRAs");
-    *s = tree_dot(a->tree, b->tree);
-    return MWG_OK;
+    return guarded([&] {
+        if (!a || !b || !s) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (a->tree.mra != b->tree.mra) return fail(MWG_ERR_ARG, "trees on different MRAs");
+        ensure_host(a);
+        ensure_host(b);
+        *s = tree_dot(a->tree, b->tree);
+        return (int)MWG_OK;
+    });
 }
 
 int mwg_tree_eval(mwg_tree t, int n, const double *r, double *v) {
-    if (!t || n < 0 || (n > 0 && (!r || !v))) return fail(MWG_ERR_ARG, "bad arguments");
-    for (int i = 0; i < n; i++) v[i] = tree_eval(t->tree, r + 3 * i);
-    return MWG_OK;
+    return guarded([&] {
+        if (!t || n < 0 || (n > 0 && (!r || !v))) return fail(MWG_ERR_ARG, "bad arguments");
+        ensure_host(t);
+        for (int i = 0; i < n; i++) v[i] = tree_eval(t->tree, r + 3 * i);
+        return (int)MWG_OK;
+    });
 }
 
 int mwg_tree_rescale(mwg_tree t, double f) {
-    if (!t) return fail(MWG_ERR_ARG, "null tree");
-    for (double &x : t->tree.coefs) x *= f;
-    t->tree.sqnorm *= f * f;
-    return MWG_OK;
+    return guarded([&] {
+        if (!t) return fail(MWG_ERR_ARG, "null tree");
+        if (!std::isfinite(f)) return fail(MWG_ERR_ARG, "non-finite factor");
+        if (t->host_valid)
+            for (double &x : t->tree.coefs) x *= f;
+        if (t->dev_valid) {
+            std::string err;
+            dev_check(dev_scale(t->dctx->dev, t->dblocks, tree_values(t), f, err), err);
+        }
+        t->tree.sqnorm *= f * f;
+        for (double &x : t->s2) x *= f * f;
+        for (double &x : t->w2) x *= f * f;
+        for (double &x : t->cn8) x *= std::fabs(f);
+        return (int)MWG_OK;
+    });
 }
 
 int mwg_tree_thresholds(mwg_tree t, double *thr) {
@@ -1572,12 +2186,13 @@ int mwg_apply_batch(mwg_context ctx, int n, const mwg_apply_desc *desc) {
             j.out = d.out;
             j.op = d.op;
             j.inp = &d.inp->tree;
+            j.inp_h = d.inp;
             j.prec = d.prec;
             j.max_iter = d.max_iter;
             j.abs_prec = d.abs_prec != 0;
             for (int t = 0; t < d.n_prec; t++) {
                 if (!d.prec_trees[t]) return fail(MWG_ERR_ARG, "null prec tree");
-                j.prec_trees.push_back(&d.prec_trees[t]->tree);
+                j.prec_h.push_back(d.prec_trees[t]);
             }
         }
         return apply_batch(ctx, jobs);

@@ -118,3 +118,80 @@ def test_multiply_argument_errors(ctx):
     g = mw.FunctionTree(leg).project_gaussians(1e-3, [1.0], [2.0], [[0.0, 0.0, 0.0]])
     with pytest.raises(mw.MWGError, match="interpolating"):
         mw.multiply(1e-3, mw.FunctionTree(leg), g, g, ctx=ctx)
+
+
+def _max_norm_grid(mra, a, b, prec):
+    """The output grid of the max-norm product rule, restated from the
+    inputs' blocks: per node the largest 2^{3n} ||block||^2 and
+    2^{3n} ||wavelet part||^2 over its subtree (n = min_scale + depth); from
+    the roots, a node splits while maxW_a maxS_b + maxW_b maxS_a > prec
+    (below an input's finest node: its scaling maximum, wavelet 0)."""
+    n3 = mra.kp1 ** 3
+
+    def tables(t):
+        d, l, c = t
+        keys = {(int(d[i]),) + tuple(int(x) for x in l[i]): i for i in range(len(d))}
+        f = 2.0 ** (3 * (mra.min_scale + d.astype(np.float64)))
+        w2 = (c[:, n3:] ** 2).sum(1)
+        ms, mw = f * ((c[:, :n3] ** 2).sum(1) + w2), f * w2
+        for i in np.argsort(-d, kind="stable"):
+            if d[i] > 0:
+                p = keys[(int(d[i]) - 1,) + tuple(int(x) >> 1 for x in l[i])]
+                ms[p], mw[p] = max(ms[p], ms[i]), max(mw[p], mw[i])
+        return keys, ms, mw
+
+    def at(tab, k):
+        keys, ms, mw = tab
+        kk = k
+        while kk not in keys:
+            kk = (kk[0] - 1,) + tuple(x >> 1 for x in kk[1:])
+        return np.sqrt(ms[keys[kk]]), (np.sqrt(mw[keys[kk]]) if kk == k else 0.0)
+
+    A, B = tables(a), tables(b)
+    work = [k for k in A[0] if k[0] == 0]
+    grid = set(work)
+    while work:
+        nxt = []
+        for k in work:
+            if k[0] >= mra.max_depth:
+                continue
+            sa, wa = at(A, k)
+            sb, wb = at(B, k)
+            if wa * sb + wb * sa > prec:
+                nxt += [(k[0] + 1, 2 * k[1] + (t & 1), 2 * k[2] + ((t >> 1) & 1), 2 * k[3] + (t >> 2)) for t in range(8)]
+        grid |= set(nxt)
+        work = nxt
+    return grid
+
+
+def test_max_norm_refinement(ctx):
+    """useMaxNorms (mwg_multiply_ex with MWG_MUL_MAX_NORMS): the grid equals
+    the restated max-norm rule exactly, every leaf equals the oracle's product
+    on that grid, and the product agrees with the exact one (Gaussian product
+    theorem) to a small multiple of prec.  The reference's own max-norm grid
+    cannot be compared (MRCPP is absent): parity of the rule itself is
+    unpinned beyond this restatement."""
+    import oracle
+    mra = _mra(7)
+    prec = 1e-5
+    fi, fj = _orbitals(777, 2, 3, 0.5, 20.0, 2.0)
+    a = mw.FunctionTree(mra).project_gaussians(prec / 10, *fi)
+    b = mw.FunctionTree(mra).project_gaussians(prec / 10, *fj)
+    out = mw.multiply(prec, mw.FunctionTree(mra), a, b, abs_prec=True, use_max_norms=True, ctx=ctx)
+    od, ol, oc = out.nodes()
+    keys = {(int(d),) + tuple(int(x) for x in l) for d, l in zip(od, ol)}
+    want = _max_norm_grid(mra, a.nodes(), b.nodes(), prec)
+    lv = np.array([not np.any((od == od[i] + 1) & np.all((ol >> 1) == ol[i], axis=1)) for i in range(len(od))])
+    ref = oracle.multiply(mra, a.nodes(), b.nodes(), od[lv], ol[lv])
+    err = np.max(np.abs(oc[lv] - ref)) / np.max(np.abs(ref))
+    exact = mw.FunctionTree(mra).project_gaussians(prec / 10, *workloads.pair_density(fi, fj))
+    plain = mw.multiply(prec, mw.FunctionTree(mra), a, b, abs_prec=True, ctx=ctx)
+    e_max, e_plain = _diff_norm(out, exact), _diff_norm(plain, exact)
+    print("max norms: %d nodes (rule %d; union-grid product %d), leaves vs oracle %.2e, |out - exact| %.2e "
+          "(union-grid product %.2e)" % (len(od), len(want), plain.size(), err, e_max, e_plain))
+    assert keys == want
+    assert err < 1e-13
+    assert e_max < 20 * prec * max(1.0, exact.norm())
+    # an unknown flag is an argument error
+    with pytest.raises(mw.MWGError):
+        mw._check(mw.lib.mwg_multiply_ex(ctx._h, prec, out._h, 1.0, a._h, b._h, -1, 1, 2))

@@ -1,0 +1,211 @@
+"""Device-resident trees (mwg_tree_set_resident) through one exchange step.
+
+The chain of ExchangePotential::calcExchange_kij
+(src/qmoperators/two_electron/ExchangePotential.cpp:161-234) followed by the
+sum and crop of ExchangePotentialD1 (ExchangePotentialD1.cpp:317-323,365-371),
+for real orbitals (phi^dagger = phi):
+
+    rho_ij = phi_i phi_j                       multiply, prec / 10, absolute
+    V_ij   = P[rho_ij]                         apply, 10 prec, absolute, precTrees {phi_k, phi_j, phi_i}
+    x_kij  = phi_k V_ij,  x_jji = phi_j V_ij   multiply, prec / 100, absolute
+    ex     = x_kij + 0.5 x_jji                 linear combination (mrcpp::add) to prec
+    ex    += x_kij                             FunctionTree::add on the union grid
+    ex.crop(prec)
+
+With every tree resident on the GPU, the only block traffic between host and
+device is the upload of the three orbitals, until the results are read.  The
+same chain on host trees moves every intermediate both ways; both counts are
+printed.  Pins, each on the resident chain's own grids after reading it back:
+- rho_ij and x_kij at every leaf against oracle.multiply (to 1e-13 of the
+  largest value), so the device BottomUp transform of a resident product
+  agrees with the host compression;
+- V_ij at every node against oracle.apply_fixed with its own thresholds (to
+  1e-12), the precTrees factors taken from device-computed node norms;
+- the sums through inner products: ||ex - (2 x_kij + 0.5 x_jji)|| within the
+  sum's own precision;
+- the resident chain equals the host chain (the same grids and blocks to
+  1e-12 of the largest, up to split decisions at the margin, which are
+  checked to change the function by less than 0.1 of their precision).
+"""
+import time
+
+import numpy as np
+import pytest
+
+import mrchem_amd as mw
+from mrchem_amd import workloads
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _mra(order):
+    return mw.MultiResolutionAnalysis(order=order, basis="interpolating", max_depth=14, **workloads.WORLD)
+
+
+def _diff_norm(u, v):
+    return np.sqrt(max(u.square_norm() + v.square_norm() - 2.0 * u.dot(v), 0.0))
+
+
+def _coef_diff(a, b):
+    """Largest coefficient difference relative to b's largest, when a and b
+    have the same grid (else None).  The inner-product form of the
+    difference cannot resolve below about 1e-7 of the norm (it subtracts
+    squared norms), so equal grids are compared block by block."""
+    da, la, ca = a.nodes()
+    db, lb, cb = b.nodes()
+    if len(da) != len(db) or not (np.array_equal(da, db) and np.array_equal(la, lb)):
+        return None
+    return np.max(np.abs(ca - cb)) / np.max(np.abs(cb))
+
+
+def _leaves(d, l):
+    return np.array([not np.any((d == d[i] + 1) & np.all((l >> 1) == l[i], axis=1)) for i in range(len(d))])
+
+
+def _chain(ctx, mra, P, phi, prec, resident):
+    """calcExchange_kij for (k, i, j) = (0, 1, 2), then the D1 sum and crop."""
+    pk, pi, pj = phi
+    t = {}
+    mk = lambda: mw.FunctionTree(mra).set_resident(ctx) if resident else mw.FunctionTree(mra)
+    t["rho"] = mw.multiply(prec / 10, mk(), pi, pj, abs_prec=True, ctx=ctx)
+    assert t["rho"].norm() > prec
+    t["V"] = mk()
+    mw.apply(prec * 10, t["V"], P, t["rho"], abs_prec=True, prec_trees=[pk, pj, pi], ctx=ctx)
+    t["xk"] = mw.multiply(prec / 100, mk(), pk, t["V"], abs_prec=True, ctx=ctx)
+    t["xj"] = mw.multiply(prec / 100, mk(), pj, t["V"], abs_prec=True, ctx=ctx)
+    t["ex"] = mw.add(prec, mk(), [1.0, 0.5], [t["xk"], t["xj"]], abs_prec=True, ctx=ctx)
+    t["ex"].add(1.0, t["xk"], ctx=ctx)
+    t["ex"].crop(prec, 1.0, True)
+    return t
+
+
+def test_exchange_chain_stays_on_the_device(ctx):
+    order, prec = 7, 1e-4
+    mra = _mra(order)
+    P = mw.PoissonOperator(mra, prec / 10)
+    rng = workloads.SplitMix64(2718)
+    orbs = [workloads.gaussian_sum(rng, 4, 0.5, 20.0, 2.0) for _ in range(3)]
+    host_phi = [mw.FunctionTree(mra).project_gaussians(prec / 10, *f) for f in orbs]
+    dev_phi = [mw.FunctionTree(mra).project_gaussians(prec / 10, *f) for f in orbs]
+    blk_bytes = 8 * mra.block
+    ctx.transfer_bytes(reset=True)
+    for f in dev_phi:
+        f.set_resident(ctx)
+    assert ctx.transfer_bytes(reset=True) == (0, 0)  # uploaded when first read
+    _chain(ctx, mra, P, dev_phi, prec, True)  # warm-up (operator tables, kernels)
+    orb_bytes = sum(f.size() for f in dev_phi) * blk_bytes
+    assert ctx.transfer_bytes(reset=True) == (orb_bytes, 0)
+    t0 = time.perf_counter()
+    dev = _chain(ctx, mra, P, dev_phi, prec, True)
+    t_dev = time.perf_counter() - t0
+    up, down = ctx.transfer_bytes(reset=True)
+    inter = sum(dev[k].size() for k in ("rho", "V", "xk", "xj", "ex")) * blk_bytes
+    print("resident chain: %.1f ms, %d B up, %d B down (orbitals %d B, intermediates and results %d B)" %
+          (1e3 * t_dev, up, down, orb_bytes, inter))
+    for k in ("rho", "V", "xk", "xj", "ex"):
+        assert dev[k].residency() == (True, False, True), k
+    assert (up, down) == (0, 0)
+    t0 = time.perf_counter()
+    host = _chain(ctx, mra, P, host_phi, prec, False)
+    t_host = time.perf_counter() - t0
+    hup, hdown = ctx.transfer_bytes(reset=True)
+    print("host chain: %.1f ms, %d B up, %d B down" % (1e3 * t_host, hup, hdown))
+    assert hup + hdown > 2 * inter
+
+    # rho and x_kij: every leaf against the oracle's product
+    for name, a, b, p in (("rho", host_phi[1], host_phi[2], None), ("xk", host_phi[0], None, "V")):
+        d, l, c = dev[name].nodes()
+        bt = b.nodes() if b is not None else dev[p].nodes()
+        lv = _leaves(d, l)
+        ref = oracle.multiply(mra, a.nodes(), bt, d[lv], l[lv])
+        err = np.max(np.abs(c[lv] - ref)) / np.max(np.abs(ref))
+        print("%s: %d nodes, %d leaves, resident vs oracle %.2e" % (name, len(d), lv.sum(), err))
+        assert lv.sum() > 50 and err < 1e-13
+    # V_ij at every node against the oracle's apply with its thresholds
+    d, l, c = dev["V"].nodes()
+    ref, _ = oracle.apply_fixed(mra, P.tables(), dev["rho"].nodes(), d, l, dev["V"].thresholds())
+    err = np.max(np.abs(ref - c)) / np.max(np.abs(ref))
+    print("V: %d nodes, resident vs oracle %.2e" % (len(d), err))
+    assert err < 1e-12
+    # the sums: ex (cropped) against 2 x_kij + 0.5 x_jji
+    want = mw.add(-1.0, mw.FunctionTree(mra), [2.0, 0.5], [dev["xk"], dev["xj"]], ctx=ctx)
+    e = _diff_norm(dev["ex"], want)
+    print("ex: %d nodes after crop, |ex - (2 x_kij + 0.5 x_jji)| = %.2e (|ex| %.3e)" %
+          (dev["ex"].size(), e, want.norm()))
+    assert e < 10 * prec
+    # resident chain == host chain
+    for k in ("rho", "V", "xk", "xj", "ex"):
+        a, b = dev[k], host[k]
+        same = a.size() == b.size() and all(np.array_equal(x, y) for x, y in zip(a.nodes()[:2], b.nodes()[:2]))
+        diff = _diff_norm(a, b)
+        if same:
+            ca, cb = a.nodes()[2], b.nodes()[2]
+            rel = np.max(np.abs(ca - cb)) / np.max(np.abs(cb))
+            print("%s: same grid (%d nodes), max diff %.2e" % (k, a.size(), rel))
+            assert rel < 1e-12
+        else:
+            print("%s: grids differ (%d vs %d nodes), |diff| %.2e" % (k, a.size(), b.size(), diff))
+        assert diff < 0.1 * prec * max(1.0, b.norm())
+
+
+def test_residency_follows_host_reads_and_writes(ctx):
+    mra = _mra(5)
+    prec = 1e-4
+    f = mw.FunctionTree(mra).project_gaussians(prec, [1.0, -0.5], [4.0, 20.0], [[0.1, 0.0, 0.2], [-0.3, 0.1, 0.0]])
+    g = mw.FunctionTree(mra).project_gaussians(prec, [0.8], [6.0], [[0.0, 0.2, -0.1]])
+    ref = mw.multiply(prec, mw.FunctionTree(mra), f, g, abs_prec=True, ctx=ctx)
+    f.set_resident(ctx)
+    g.set_resident(ctx)
+    out = mw.multiply(prec, mw.FunctionTree(mra).set_resident(ctx), f, g, abs_prec=True, ctx=ctx)
+    assert out.residency() == (True, False, True)
+    n2 = out.square_norm()  # from the device's norms, no download
+    assert out.residency() == (True, False, True)
+    # the device path splits on device-summed norms: equal up to decisions
+    # at the margin of the split threshold
+    assert abs(n2 - ref.square_norm()) <= 1e-8 * n2
+    # device rescale of a device-only tree, then a lazy host read
+    out.rescale(-2.0)
+    assert out.residency() == (True, False, True)
+    _, _, c = out.nodes()
+    assert out.residency() == (True, True, True)
+    cd = _coef_diff(out, ref.rescale(-2.0))
+    assert (cd is not None and cd < 1e-12) or _diff_norm(out, ref) <= 0.1 * prec
+    # a host write makes the device copy stale; the next device call uploads it
+    f.project_gaussians(prec, [1.0], [5.0], [[0.0, 0.0, 0.0]])
+    assert f.residency() == (True, True, False)
+    ctx.transfer_bytes(reset=True)
+    mw.multiply(prec, out, f, g, abs_prec=True, ctx=ctx)
+    up, _ = ctx.transfer_bytes()
+    assert up == f.size() * 8 * mra.block and f.residency() == (True, True, True)
+    ref2 = mw.multiply(prec, mw.FunctionTree(mra), f, g, abs_prec=True, ctx=ctx)
+    # grids may differ where a wavelet norm sits at the split threshold
+    cd = _coef_diff(out, ref2)
+    print("after a host write: %d nodes (host path %d), same grid: %s, max coef diff %s, |diff| %.2e" %
+          (out.size(), ref2.size(), cd is not None, cd, _diff_norm(out, ref2)))
+    assert (cd is not None and cd < 1e-12) or _diff_norm(out, ref2) <= 0.1 * prec
+    # crop of a device-only tree runs on the device, as the host crop does
+    host_copy = mw.FunctionTree(mra).set_nodes(*out.nodes())
+    out.set_resident(ctx, False)
+    out.set_resident(ctx)
+    mw.multiply(prec, out, f, g, abs_prec=True, ctx=ctx)
+    assert out.residency() == (True, False, True)
+    out.crop(10 * prec)
+    host_copy.crop(10 * prec)
+    print("crop: %d nodes on the device, %d on the host" % (out.size(), host_copy.size()))
+    # the same decisions up to the rounding of the wavelet norms (a node at
+    # the threshold may go either way)
+    cd = _coef_diff(out, host_copy)
+    if cd is not None:
+        assert cd == 0.0  # the kept blocks are moved, not recomputed
+    else:
+        assert abs(out.size() - host_copy.size()) <= 8
+        assert _diff_norm(out, host_copy) <= 10 * prec
+    # leaving residency brings the host copy back
+    out.set_resident(ctx, False)
+    assert out.residency() == (False, True, False)
+    # a context holding resident trees is not destroyed
+    c2 = mw.Context(0)
+    h = mw.FunctionTree(mra).project_gaussians(prec, [1.0], [3.0], [[0.0, 0.0, 0.0]]).set_resident(c2)
+    assert mw.lib.mwg_context_destroy(c2._h) != 0
+    h.set_resident(c2, False)
