@@ -222,6 +222,24 @@ int mwg_tree_crop(mwg_tree tree, double prec, double split_fac, int abs_prec);
 int mwg_tree_pack_size(mwg_tree tree, long long *bytes);
 int mwg_tree_pack(mwg_tree tree, void *buffer, long long bytes);
 int mwg_tree_unpack(mwg_tree tree, const void *buffer, long long bytes);
+/* The packed form of mwg_tree_pack written into device memory of ctx's GPU
+ * (dst: a device pointer to at least mwg_tree_pack_size bytes), for a
+ * device-to-device send of a tree, e.g. over RCCL, in the chunks
+ * mrcpp::send_tree sends (src/parallel.cpp:323-337): the header and keys are
+ * uploaded and the blocks copied on the device (from a resident tree's
+ * device copy, else uploaded once). */
+int mwg_tree_pack_device(mwg_tree tree, mwg_context ctx, void *dst, long long bytes);
+/* Replace the tree by a packed tree held in device memory of ctx's GPU (as
+ * received from a device-to-device send).  The tree becomes resident on ctx
+ * (mwg_tree_set_resident) and its blocks are gathered on the device; only
+ * the header and keys come to the host. */
+int mwg_tree_unpack_device(mwg_tree tree, mwg_context ctx, const void *src, long long bytes);
+/* Device memory on ctx's GPU for packed trees (a send or receive buffer),
+ * and a copy of it to host memory: for callers without an allocator of
+ * their own on that device.  No counterpart in the reference. */
+int mwg_device_buffer_create(mwg_context ctx, long long bytes, void **dev);
+int mwg_device_buffer_destroy(mwg_context ctx, void *dev);
+int mwg_device_buffer_read(mwg_context ctx, const void *dev, void *host, long long bytes);
 int mwg_tree_save(mwg_tree tree, const char *path);
 int mwg_tree_load(mwg_tree tree, const char *path);
 /* Screening threshold of every node used by the last apply into this tree. */

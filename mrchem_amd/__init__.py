@@ -22,7 +22,7 @@ import os
 import numpy as np
 
 __all__ = ["MultiResolutionAnalysis", "PoissonOperator", "HelmholtzOperator", "GaussianOperator",
-           "FunctionTree", "Context", "ApplyPlan", "apply", "apply_fixed", "MWGError", "lib", "LIB_PATH"]
+           "FunctionTree", "Context", "ApplyPlan", "DeviceBuffer", "apply", "apply_fixed", "MWGError", "lib", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmwgreen.so")
 if not os.path.exists(LIB_PATH):
@@ -66,6 +66,11 @@ _sig("mwg_tree_crop", _p, _d, _d, _i)
 _sig("mwg_tree_pack_size", _p, _llp)
 _sig("mwg_tree_pack", _p, ctypes.c_void_p, ctypes.c_longlong)
 _sig("mwg_tree_unpack", _p, ctypes.c_void_p, ctypes.c_longlong)
+_sig("mwg_tree_pack_device", _p, _p, ctypes.c_void_p, ctypes.c_longlong)
+_sig("mwg_tree_unpack_device", _p, _p, ctypes.c_void_p, ctypes.c_longlong)
+_sig("mwg_device_buffer_create", _p, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p))
+_sig("mwg_device_buffer_destroy", _p, ctypes.c_void_p)
+_sig("mwg_device_buffer_read", _p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong)
 _sig("mwg_tree_save", _p, ctypes.c_char_p)
 _sig("mwg_tree_load", _p, ctypes.c_char_p)
 _sig("mwg_operator_info", _p, _ip, _ip, _ip, _dp, _dp, _dp)
@@ -178,6 +183,28 @@ class Context:
         if h:
             lib.mwg_context_destroy(h)
             self._h = None
+
+
+class DeviceBuffer:
+    """Device memory on a context's GPU (a packed-tree send or receive
+    buffer); ptr is its device address."""
+
+    def __init__(self, nbytes, ctx=None):
+        self.ctx = ctx or Context.default()
+        p = ctypes.c_void_p()
+        _check(lib.mwg_device_buffer_create(self.ctx._h, nbytes, ctypes.byref(p)))
+        self.ptr, self.nbytes = p.value, nbytes
+
+    def read(self):
+        """The buffer's bytes, copied to the host."""
+        out = ctypes.create_string_buffer(self.nbytes)
+        _check(lib.mwg_device_buffer_read(self.ctx._h, ctypes.c_void_p(self.ptr), out, self.nbytes))
+        return out.raw
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib.mwg_device_buffer_destroy(self.ctx._h, ctypes.c_void_p(self.ptr))
+            self.ptr = None
 
 
 class MultiResolutionAnalysis:
@@ -403,6 +430,26 @@ class FunctionTree:
         buf = ctypes.create_string_buffer(n.value)
         _check(lib.mwg_tree_pack(self._h, buf, n.value))
         return buf.raw
+
+    def pack_size(self):
+        n = ctypes.c_longlong(0)
+        _check(lib.mwg_tree_pack_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def pack_device(self, ptr, nbytes, ctx=None):
+        """Write the packed form into device memory at ptr (an address on
+        ctx's GPU of at least pack_size() bytes)."""
+        ctx = ctx or Context.default()
+        _check(lib.mwg_tree_pack_device(self._h, ctx._h, ctypes.c_void_p(ptr), nbytes))
+        return self
+
+    def unpack_device(self, ptr, nbytes, ctx=None):
+        """Replace the tree by the packed tree at device address ptr; the tree
+        becomes resident on ctx."""
+        ctx = ctx or Context.default()
+        _check(lib.mwg_tree_unpack_device(self._h, ctx._h, ctypes.c_void_p(ptr), nbytes))
+        self._ctx = ctx
+        return self
 
     def unpack(self, data):
         """Replace the tree by a packed one written on the same MRA."""

@@ -534,6 +534,9 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
 typedef _Float16 mwg_h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 mwg_h4 __attribute__((ext_vector_type(4)));
 
+#ifndef MWG_SPLIT_MASK_ALL
+#define MWG_SPLIT_MASK_ALL 0
+#endif
 template <int N>
 struct SplitShape {
     static constexpr int N2 = N * N, N3 = N2 * N;
@@ -553,6 +556,12 @@ struct SplitShape {
     // per lane spill); no limit below, where a cap of 4 spills at N = 5, 7, 8
     // or schedules worse
     static constexpr int WPE = N > 10 ? 3 : 1;
+    // N <= 8: every phase issues its six loads unconditionally (exact wait
+    // counts; 136 exchange pair applies at k = 7: -25 %); above, the extra
+    // live registers cost occupancy or spill and the loads stay conditional
+    static constexpr bool UNC = N <= 8;
+    // lanes without a result row store nothing (else they write a scratch row)
+    static constexpr bool MASK = MWG_SPLIT_MASK_ALL || N <= 8;
     // 16-byte chunks per row: hi, lo, tail (+ the extra chunk for N > 10).
     // For N <= 10 the odd number of chunks makes the row pitch spread 16
     // consecutive rows over distinct LDS banks; for N > 10 the four chunks
@@ -674,8 +683,10 @@ constexpr int kZ0Bit = 28, kZ1Bit = 29, kSingleBit = 30;
 // Lanes whose operand row lies past the last line read a line that exists
 // (line j - 2 or N - 1, or row 0 of the block): their products meet zero
 // operand slots or land in output rows and columns that are not stored.
-// Lanes without a row to write write a scratch row, and the single-rounding
-// choice is a bit mask, so no pass branches on the lane or the item.
+// Lanes without a row to write store nothing at N <= 8 (an exec mask: all
+// of them writing one scratch row serialises the store), a scratch row above,
+// and the single-rounding choice is a bit mask, so no pass branches on the
+// item.
 constexpr int kSplitCap = 384;  // worklist items per batch (>= 64 + 128)
 
 // LDS of one job of split_segment: the double-buffered intermediates, the
@@ -739,15 +750,21 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
     // extra chunk) and operator-block reads (row min(lc, N - 1)).
     int xs1[TWG], xs2[TWG], xse[TWG], ys1[TWG], ys2[TWG], yse[TWG];
     int yl[TWG], yle[TWG], fl[TWG], fle[TWG], zl[TWZ], zle[TWZ];
+    // lanes with a result row to store, and those whose row is a tail row
+    // (the extra chunk, N > 10): the others store nothing (no two lanes of a
+    // store meet at one scratch address)
+    bool sok[TWG], xtl[TWG], ytl[TWG];
 #pragma unroll
     for (int tt = 0; tt < TWG; tt++) {
         const int t = wv + NW * tt;
         const int g4 = 4 * t + lq;  // group of this lane's four results
         const bool okx = g4 < NG && lc < N;
+        sok[tt] = okx;
         // x: group g4 = (jz, jy / 4) = (g4 / G, g4 % G), lane ix = lc -> Xt row jz N + ix
         {
             const int ri = okx ? (g4 / G) * N + lc : DROW, row = ri * RL, j0 = 4 * (g4 % G);
             const bool tail = N > 8 && j0 >= 8;
+            xtl[tt] = tail;
             xs1[tt] = row + (tail ? 16 : j0);
             xs2[tt] = row + (tail ? 20 : 8 + j0);
             xse[tt] = EXT0 + (tail ? ri : DROW) * RE;
@@ -756,6 +773,7 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
         {
             const int ri = okx ? (g4 % N) * N + lc : DROW, row = ri * RL, j0 = 4 * (g4 / N);
             const bool tail = N > 8 && j0 >= 8;
+            ytl[tt] = tail;
             ys1[tt] = row + (tail ? 16 : j0);
             ys2[tt] = row + (tail ? 20 : 8 + j0);
             yse[tt] = EXT0 + (tail ? ri : DROW) * RE;
@@ -794,7 +812,7 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
     // four results of one tile into a data-side split row: hi keeps the bits
     // of msk (0xffffe000: an 11-bit mantissa, so hi converts exactly and
     // lo = v - hi is the exact remainder; all ones: single rounding, lo = 0)
-    auto store4 = [&](_Float16 *base, int o1, int o2, int oe, const mwg_float4 &v, unsigned msk) {
+    auto store4 = [&](_Float16 *base, int o1, int o2, int oe, const mwg_float4 &v, unsigned msk, bool ok, bool ext) {
         float h[4], l[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
@@ -803,9 +821,11 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
         }
         const mwg_h4 hi = mwg_h4{(_Float16)h[0], (_Float16)h[1], (_Float16)h[2], (_Float16)h[3]};
         const mwg_h4 lo = mwg_h4{(_Float16)l[0], (_Float16)l[1], (_Float16)l[2], (_Float16)l[3]};
-        *reinterpret_cast<mwg_h4 *>(base + o1) = hi;
-        *reinterpret_cast<mwg_h4 *>(base + o2) = lo;
-        if (S::TWO) *reinterpret_cast<mwg_h4 *>(base + oe) = hi;
+        if (!S::MASK || ok) {
+            *reinterpret_cast<mwg_h4 *>(base + o1) = hi;
+            *reinterpret_cast<mwg_h4 *>(base + o2) = lo;
+            if (S::TWO && (!S::MASK || ext)) *reinterpret_cast<mwg_h4 *>(base + oe) = hi;
+        }
     };
     auto lds_operand = [&](const _Float16 *base, int off, int offe, mwg_h8 &v, mwg_h4 &e) {
         v = *reinterpret_cast<const mwg_h8 *>(base + off);
@@ -846,7 +866,7 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) d[tt] = split_mfma<N>(fa[tt], fe[tt], o, oe);
 #pragma unroll
-        for (int tt = 0; tt < TWG; tt++) store4(X, xs1[tt], xs2[tt], xse[tt], d[tt], msk);
+        for (int tt = 0; tt < TWG; tt++) store4(X, xs1[tt], xs2[tt], xse[tt], d[tt], msk, sok[tt], xtl[tt]);
     };
     auto y_pass = [&](const _Float16 *X, _Float16 *Y, const mwg_h8 &o, const mwg_h4 &oe, unsigned msk) {
         mwg_h8 va[TWG];
@@ -860,7 +880,7 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) d[tt] = split_mfma<N>(va[tt], ve[tt], o, oe);
 #pragma unroll
-        for (int tt = 0; tt < TWG; tt++) store4(Y, ys1[tt], ys2[tt], yse[tt], d[tt], msk);
+        for (int tt = 0; tt < TWG; tt++) store4(Y, ys1[tt], ys2[tt], yse[tt], d[tt], msk, sok[tt], ytl[tt]);
     };
     // z of gt (zm bit 0, operator o0) and of gt | 4 (bit 1, operator o1)
     auto z_pass = [&](const _Float16 *Y, const mwg_h8 &o0, const mwg_h4 &oe0, const mwg_h8 &o1, const mwg_h4 &oe1,
@@ -933,11 +953,11 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
             // operators load into the other set; phases are unrolled in pairs
             // so that the set and the LDS buffers are fixed in each copy
             const int mask_grp = (1 << kZ0Bit) - 1;
+            int cur_grp = -1;  // the group whose F rows are loaded (N > 8)
             mwg_h8 o0[4], o1[4];  // operator sets: Ox, Oy, Oz of gt, Oz of gt | 4
             mwg_h4 e0[4], e1[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) e0[u] = e1[u] = zero4;
-            int cur_grp = -1;
             float2 sig1 = make_float2(0.f, 0.f), sig2 = make_float2(0.f, 0.f);  // scales of items q - 1, q - 2
             int zm1 = 0, zm2 = 0;  // z masks of items q - 1, q - 2
             unsigned msk1 = 0u;    // hi mask of item q - 1
@@ -951,30 +971,57 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
                 auto &on = pick<1 - P>(o0, o1);
                 auto &en = pick<1 - P>(e0, e1);
                 const bool hx = FULL || q < n, hy = FULL || (q >= 1 && q <= n), hz = FULL || q >= 2;
-                // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1) of the
-                // components item q - 1 passes for
-                if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), on[0], en[0]);
-                if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), on[1], en[1]);
-                if (hy && (zm1 & 1)) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz0), on[2], en[2]);
-                if (hy && (zm1 & 2)) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz1), on[3], en[3]);
                 float2 sig0 = make_float2(0.f, 0.f);
                 unsigned msk0 = 0u;
                 int zm0 = 0;
-                // the F rows of item q load (when its group changes) while
-                // the y and z passes run; the x pass comes last
-                if (hx) {
-                    const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
-                    if ((g & mask_grp) != cur_grp) {
-                        cur_grp = g & mask_grp;
-                        load_f(__builtin_amdgcn_readfirstlane(wl[q].cb));
+                if constexpr (S::UNC) {
+                    // The F rows of item q first, then the next phase's
+                    // operators Ox(q + 1), Oy(q), Oz(q - 1) of gt and gt | 4.
+                    // All six loads are issued in every phase, their items
+                    // clamped into the batch (a repeated F block or an unused
+                    // operator block is a cache hit), so the loads complete
+                    // in a fixed order and each pass waits only for its own
+                    // operands: y and z for the previous phase's loads, x
+                    // (run last) for F.
+                    const int qx = q < n ? q : n - 1, qn = q + 1 < n ? q + 1 : n - 1;
+                    const int qz = q >= 1 && q <= n ? q - 1 : 0;
+                    const int g = __builtin_amdgcn_readfirstlane(wl[qx].grp);
+                    load_f(__builtin_amdgcn_readfirstlane(wl[qx].cb));
+                    load_op(__builtin_amdgcn_readfirstlane(wl[qn].bx), on[0], en[0]);
+                    load_op(__builtin_amdgcn_readfirstlane(wl[qx].by), on[1], en[1]);
+                    load_op(__builtin_amdgcn_readfirstlane(wl[qz].bz0), on[2], en[2]);
+                    load_op(__builtin_amdgcn_readfirstlane(wl[qz].bz1), on[3], en[3]);
+                    if (hx) {
+                        msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
+                        zm0 = (g >> kZ0Bit) & 3;
+                        sig0 = wsig[q];
                     }
-                    msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
-                    zm0 = (g >> kZ0Bit) & 3;
-                    sig0 = wsig[q];
+                    if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
+                    if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], oc[3], ec[3], sig2, zm2);
+                    if (hx) x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
+                } else {
+                    // next phase's operators: Ox(q + 1), Oy(q), Oz(q - 1) of
+                    // the components item q - 1 passes for; the F rows of
+                    // item q only when its group changes (registers are
+                    // short at N > 8)
+                    if (q + 1 < n) load_op(__builtin_amdgcn_readfirstlane(wl[q + 1].bx), on[0], en[0]);
+                    if (hx) load_op(__builtin_amdgcn_readfirstlane(wl[q].by), on[1], en[1]);
+                    if (hy && (zm1 & 1)) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz0), on[2], en[2]);
+                    if (hy && (zm1 & 2)) load_op(__builtin_amdgcn_readfirstlane(wl[q - 1].bz1), on[3], en[3]);
+                    if (hx) {
+                        const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
+                        if ((g & mask_grp) != cur_grp) {
+                            cur_grp = g & mask_grp;
+                            load_f(__builtin_amdgcn_readfirstlane(wl[q].cb));
+                        }
+                        msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
+                        zm0 = (g >> kZ0Bit) & 3;
+                        sig0 = wsig[q];
+                        x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
+                    }
+                    if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
+                    if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], oc[3], ec[3], sig2, zm2);
                 }
-                if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
-                if (hz) z_pass(yt + P * BUFH, oc[2], ec[2], oc[3], ec[3], sig2, zm2);
-                if (hx) x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
                 __syncthreads();
                 sig2 = sig1;
                 sig1 = sig0;
@@ -1481,7 +1528,8 @@ static int upload_at(T *dst, const std::vector<T, A> &v, hipStream_t s, std::str
     return 0;
 }
 
-int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan **out, std::string &err) {
+int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan **out, std::string &err,
+                    double *adopt_pool) {
     HIP_TRY(hipSetDevice(c->device));
     if (parts.empty()) {
         err = "empty plan";
@@ -1578,9 +1626,25 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     const size_t f_cnt = place(sizeof(unsigned long long) * 8);
     std::vector<size_t> f_mt(parts.size());
     for (size_t ai = 0; ai < parts.size(); ai++) f_mt[ai] = place(sizeof(double) * mtabs[ai].size());
+    if (adopt_pool) {
+        // the parts' pools must sit one after another in it
+        long long off = 0;
+        for (const PlanPart &pp : parts) {
+            if (!pp.plan->device_pool || pp.dev_pool != adopt_pool + off) {
+                err = "adopted pool does not hold the parts' pools in order";
+                delete p;
+                return -1;
+            }
+            off += pp.plan->pool_values;
+        }
+    }
     unsigned char *dev = nullptr;
     rc = plan_alloc(p, arena, &dev, err);
-    if (!rc) rc = plan_alloc(p, (size_t)std::max<long long>(n_pool, 1), &p->pool, err);
+    if (!rc && adopt_pool) {
+        p->pool = adopt_pool;  // owned (in bufs) only once the plan is made
+    } else if (!rc) {
+        rc = plan_alloc(p, (size_t)std::max<long long>(n_pool, 1), &p->pool, err);
+    }
     p->out_bytes = sizeof(double) * (size_t)std::max<long long>(n_out, 1) * 8 * N * N * N;
     if (!rc) rc = plan_alloc(p, p->out_bytes / sizeof(double), &a.out_ns, err);
     if (rc) {
@@ -1651,7 +1715,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
             if (!pp.dev_pool) {
                 err = "plan part without its device pool";
                 rc = -1;
-            } else if (pl.pool_values > 0) {
+            } else if (pl.pool_values > 0 && !adopt_pool) {
                 hipError_t e = hipMemcpyAsync(p->pool + o_pool, pp.dev_pool, sizeof(double) * pl.pool_values,
                                               hipMemcpyDeviceToDevice, s);
                 if (e != hipSuccess) {
@@ -1738,6 +1802,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     a.seg_job = reinterpret_cast<const int *>(dev + f_sj);
     a.seg_p0 = reinterpret_cast<const int *>(dev + f_s0);
     a.seg_p1 = reinterpret_cast<const int *>(dev + f_s1);
+    if (adopt_pool) p->bufs.push_back(adopt_pool);
     *out = p;
     return 0;
 }

@@ -31,7 +31,11 @@ struct PlanPart {
 // A plan resident in device memory: band tables, input slots and output
 // blocks of one or more applies, run together in each launch.
 struct DevicePlan;
-int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan **out, std::string &err);
+// adopt_pool: the parts' device pools lie one after another in this
+// allocation, in part order; the plan takes it as its pool (no copy) and
+// frees it.  Without it each part's pool is copied (or uploaded).
+int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan **out, std::string &err,
+                    double *adopt_pool = nullptr);
 // Run the per-node calculator over the resident plan.  kernel_ms receives the
 // device time of the launches, term_apps the number of term-applications that
 // passed screening.
@@ -122,9 +126,9 @@ void dev_mul_end(DevMulSession *m);
 // The pool of a plan built with device_pool (mw_multiply.hip): generated
 // nodes reconstructed from their parents and every slot's block copied from
 // the input tree's blocks on the device (inp: the input tree's coefficients,
-// 8 (k+1)^3 per node in tree order); pl.in_norms receives the slots'
-// component norms.  *pool (pl.pool_values doubles) is freed with dev_free.
-int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double **pool, std::string &err);
+// 8 (k+1)^3 per node in tree order) into pool (pl.pool_values doubles, the
+// caller's); pl.in_norms receives the slots' component norms.
+int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double *pool, std::string &err);
 // A device copy of n doubles (freed with dev_free).
 int dev_upload(DeviceCtx *c, const double *src, size_t n, double **dst, std::string &err);
 // n doubles of device memory (freed with dev_free).

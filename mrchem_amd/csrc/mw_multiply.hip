@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -460,28 +461,42 @@ static hipError_t post_level(hipStream_t s, int n, bool down, const PostPair *pa
     }
 }
 
-int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double **pool, std::string &err) {
+int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double *pool, std::string &err) {
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     const int n = mra.kp1(), n3 = n * n * n;
     const int ng = (int)pl.gen_parent.size(), ns = (int)pl.slot_src.size();
-    *pool = nullptr;
-    double *dpool = nullptr, *gen_s = nullptr, *F = nullptr, *norms = nullptr;
-    int *gpar = nullptr, *gch = nullptr, *ssrc = nullptr;
-    long long *soff = nullptr;
-    auto up = [&](const void *src, size_t bytes, void **dst) -> hipError_t {
-        hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 8));
-        if (e == hipSuccess && bytes) e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s);
-        return e;
+    // one device allocation for the temporaries, filled by one upload:
+    // [generated scaling blocks | slot norms | filter | gen_parent | gen_child
+    //  | slot_src | slot_off]
+    size_t bytes = 0;
+    auto place = [&](size_t b) {
+        const size_t off = bytes;
+        bytes += (std::max<size_t>(b, 8) + 255) / 256 * 256;
+        return off;
     };
-    hipError_t e = hipMalloc((void **)&dpool, sizeof(double) * std::max<long long>(pl.pool_values, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&gen_s, sizeof(double) * std::max<size_t>((size_t)ng * n3, 1));
-    if (e == hipSuccess) e = hipMalloc((void **)&norms, sizeof(double) * std::max<size_t>((size_t)ns * 8, 1));
-    if (e == hipSuccess) e = up(mra.basis.F.data(), sizeof(double) * mra.basis.F.size(), (void **)&F);
-    if (e == hipSuccess) e = up(pl.gen_parent.data(), sizeof(int) * ng, (void **)&gpar);
-    if (e == hipSuccess) e = up(pl.gen_child.data(), sizeof(int) * ng, (void **)&gch);
-    if (e == hipSuccess) e = up(pl.slot_src.data(), sizeof(int) * ns, (void **)&ssrc);
-    if (e == hipSuccess) e = up(pl.slot_off.data(), sizeof(long long) * ns, (void **)&soff);
+    const size_t f_gen = place(sizeof(double) * (size_t)ng * n3), f_nrm = place(sizeof(double) * (size_t)ns * 8);
+    const size_t f_F = place(sizeof(double) * mra.basis.F.size()), f_gp = place(sizeof(int) * ng);
+    const size_t f_gc = place(sizeof(int) * ng), f_ss = place(sizeof(int) * ns);
+    const size_t f_so = place(sizeof(long long) * ns);
+    const size_t up0 = f_F;  // the uploaded part starts at the filter
+    std::vector<unsigned char, UninitAlloc<unsigned char>> stage(bytes - up0);
+    auto put = [&](size_t f, const void *src, size_t b) {
+        if (b) std::memcpy(stage.data() + (f - up0), src, b);
+    };
+    put(f_F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size());
+    put(f_gp, pl.gen_parent.data(), sizeof(int) * ng);
+    put(f_gc, pl.gen_child.data(), sizeof(int) * ng);
+    put(f_ss, pl.slot_src.data(), sizeof(int) * ns);
+    put(f_so, pl.slot_off.data(), sizeof(long long) * ns);
+    unsigned char *dev = nullptr;
+    HIP_TRY(hipMalloc((void **)&dev, bytes));
+    double *gen_s = reinterpret_cast<double *>(dev + f_gen), *norms = reinterpret_cast<double *>(dev + f_nrm);
+    const double *F = reinterpret_cast<const double *>(dev + f_F);
+    const int *gpar = reinterpret_cast<const int *>(dev + f_gp), *gch = reinterpret_cast<const int *>(dev + f_gc);
+    const int *ssrc = reinterpret_cast<const int *>(dev + f_ss);
+    const long long *soff = reinterpret_cast<const long long *>(dev + f_so);
+    hipError_t e = hipMemcpyAsync(dev + up0, stage.data(), bytes - up0, hipMemcpyHostToDevice, s);
     // generated nodes depth by depth (each after its parent)
     for (size_t lv = 0; e == hipSuccess && lv + 1 < pl.gen_level_off.size(); lv++) {
         const int g0 = pl.gen_level_off[lv], g1 = pl.gen_level_off[lv + 1];
@@ -491,21 +506,18 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl
         e = hipGetLastError();
     }
     if (e == hipSuccess && ns > 0) {
-        hipLaunchKernelGGL(k_pool_fill, dim3(std::min(ns, 8192)), dim3(64), 0, s, n, inp, gen_s, ssrc, soff, ns, dpool,
+        hipLaunchKernelGGL(k_pool_fill, dim3(std::min(ns, 8192)), dim3(64), 0, s, n, inp, gen_s, ssrc, soff, ns, pool,
                            norms);
         e = hipGetLastError();
     }
-    if (e == hipSuccess)
+    if (e == hipSuccess && ns > 0)
         e = hipMemcpyAsync(pl.in_norms.data(), norms, sizeof(double) * (size_t)ns * 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    for (void *q : {(void *)gen_s, (void *)F, (void *)norms, (void *)gpar, (void *)gch, (void *)ssrc, (void *)soff})
-        if (q) (void)hipFree(q);
+    (void)hipFree(dev);
     if (e != hipSuccess) {
-        if (dpool) (void)hipFree(dpool);
         err = std::string("pool fill: ") + hipGetErrorString(e);
         return -2;
     }
-    *pool = dpool;
     return 0;
 }
 

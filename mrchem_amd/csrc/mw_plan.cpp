@@ -310,18 +310,29 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
     for (double t : plan.out_thr)
         if (t > 0.0 && (plan.thr_floor == 0.0 || t < plan.thr_floor)) plan.thr_floor = t;
     const int nq = (int)nodes.size();
-    std::vector<std::vector<int>> counts((size_t)nq * 8);
-    std::vector<std::vector<double>> bmin((size_t)nq * 8), bmax((size_t)nq * 8);
+    // counts and bounds per (job q * 8 + gt, band position), flat: job jq's
+    // band starts at cof[q] + gt * nb_q
+    std::vector<long long> cof(nq + 1, 0);
+    for (int q = 0; q < nq; q++) {
+        const int side = 2 * plan.band_w[q] + 1;
+        cof[q + 1] = cof[q] + 8LL * side * side * side;
+    }
+    std::vector<int> counts(cof[nq]);
+    std::vector<double> bmin(cof[nq]), bmax(cof[nq]);
     // Per output depth (band half-width w): the norm the kernel would use per
-    // active term, component and translation (tn), and the largest
-    // ((N_x N_y) N_z) over the terms per component combination cmb = cx |
-    // cy << 2 | cz << 4 and translation (pre); a (band position, ft) pair
-    // whose fl(pre ||f_ft||) does not pass holds no passing term (rounding is
-    // monotone) and is skipped, so the counts are those of the full scan.
+    // active term, component and translation (tn), and for each component
+    // combination cmb = cx | cy << 2 | cz << 4 and translation the products
+    // ((N_x N_y) N_z) of the active terms, formed as the kernel forms them,
+    // in ascending order (srt).  For a (band position, ft) pair the passing
+    // terms are those with fl(b ||f_ft||) > thr; rounding is monotone in b, so
+    // they are a suffix of the sorted products, found by bisection: the
+    // counts and bounds are those of the full scan.
     std::vector<int> wd(plan.active_off.size(), -1);
     for (int q = 0; q < nq; q++) wd[plan.out_depth[q]] = plan.band_w[q];
-    std::vector<std::vector<double>> tn_d(wd.size()), pre_d(wd.size());
-#pragma omp parallel for schedule(dynamic, 1)
+    std::vector<std::vector<double>> tn_d(wd.size()), srt_d(wd.size());
+    // norms per depth (small), then the sorted products in parallel over
+    // (depth, cmb, translation): a refinement level has one output depth
+    std::vector<std::pair<int, int>> items;  // (depth, cmb * nl3 + translation)
     for (int d = 0; d < (int)wd.size(); d++) {
         const int w = wd[d];
         if (w < 0 || d + 1 >= (int)plan.active_off.size()) continue;
@@ -338,24 +349,23 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
                         tn[((size_t)(ai - a0) * 4 + c) * nl + (l + w)] = oper_norm(op, m, d, c, l);
             }
         }
-        const size_t nl3 = (size_t)nl * nl * nl;
-        std::vector<double> &pre = pre_d[d];
-        pre.assign(64 * nl3, 0.0);
-        for (int cmb = 0; cmb < 64; cmb++) {
-            const int cx = cmb & 3, cy = (cmb >> 2) & 3, cz = (cmb >> 4) & 3;
-            double *dst = pre.data() + cmb * nl3;
-            for (int ai = 0; ai < a1 - a0; ai++) {
-                const double *tx = &tn[((size_t)ai * 4 + cx) * nl], *ty = &tn[((size_t)ai * 4 + cy) * nl];
-                const double *tz = &tn[((size_t)ai * 4 + cz) * nl];
-                for (int lz = 0; lz < nl; lz++)
-                    for (int ly = 0; ly < nl; ly++)
-                        for (int lx = 0; lx < nl; lx++) {
-                            const double b = (tx[lx] * ty[ly]) * tz[lz];
-                            double &o = dst[((size_t)lz * nl + ly) * nl + lx];
-                            if (b > o) o = b;
-                        }
-            }
-        }
+        const int nl3 = nl * nl * nl;
+        srt_d[d].assign((size_t)64 * nl3 * (a1 - a0), 0.0);
+        for (int e = 0; e < 64 * nl3; e++) items.emplace_back(d, e);
+    }
+#pragma omp parallel for schedule(static)
+    for (long long it = 0; it < (long long)items.size(); it++) {
+        const int d = items[it].first, e = items[it].second;
+        const int w = wd[d], nl = 2 * w + 1, nl3 = nl * nl * nl;
+        const size_t na = (size_t)(plan.active_off[d + 1] - plan.active_off[d]);
+        const int cmb = e / nl3, pos = e % nl3;
+        const int lx = pos % nl, ly = (pos / nl) % nl, lz = pos / (nl * nl);
+        const int cx = cmb & 3, cy = (cmb >> 2) & 3, cz = (cmb >> 4) & 3;
+        const std::vector<double> &tn = tn_d[d];
+        double *dst = srt_d[d].data() + (size_t)e * na;
+        for (size_t ai = 0; ai < na; ai++)
+            dst[ai] = (tn[(ai * 4 + cx) * nl + lx] * tn[(ai * 4 + cy) * nl + ly]) * tn[(ai * 4 + cz) * nl + lz];
+        std::sort(dst, dst + na);
     }
     long long napps = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : napps)
@@ -365,15 +375,14 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
         const double th = plan.out_thr[q];
         const int a0 = plan.active_off[d], a1 = plan.active_off[d + 1];
         const int nl = 2 * w + 1;
-        const std::vector<double> &tn = tn_d[d];
-        const double *pre = pre_d[d].data();
-        const size_t nl3 = (size_t)nl * nl * nl;
+        const double *srt = srt_d[d].data();
+        const size_t nl3 = (size_t)nl * nl * nl, na = (size_t)(a1 - a0);
         for (int gt = 0; gt < 8; gt++) {
-            std::vector<int> &cp = counts[(size_t)q * 8 + gt];
-            std::vector<double> &bl = bmin[(size_t)q * 8 + gt], &bh = bmax[(size_t)q * 8 + gt];
-            cp.assign(nb, 0);
-            bl.assign(nb, HUGE_VAL);
-            bh.assign(nb, 0.0);
+            int *cp = counts.data() + cof[q] + (long long)gt * nb;
+            double *bl = bmin.data() + cof[q] + (long long)gt * nb, *bh = bmax.data() + cof[q] + (long long)gt * nb;
+            std::fill_n(cp, nb, 0);
+            std::fill_n(bl, nb, HUGE_VAL);
+            std::fill_n(bh, nb, 0.0);
             for (int p = 0; p < nb; p++) {
                 const int sl = plan.band[plan.band_off[q] + p];
                 if (sl < 0) continue;
@@ -385,19 +394,20 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
                     int cxyz[3];
                     for (int dim = 0; dim < 3; dim++) cxyz[dim] = (((gt >> dim) & 1) << 1) | ((ft >> dim) & 1);
                     const int cmb = cxyz[0] | (cxyz[1] << 2) | (cxyz[2] << 4);
-                    if (!(pre[cmb * nl3 + ((size_t)(ll[2] + w) * nl + (ll[1] + w)) * nl + (ll[0] + w)] * fn > th))
-                        continue;
-                    for (int ai = 0; ai < a1 - a0; ai++) {
-                        const double nx = tn[((size_t)ai * 4 + cxyz[0]) * nl + (ll[0] + w)];
-                        const double ny = tn[((size_t)ai * 4 + cxyz[1]) * nl + (ll[1] + w)];
-                        const double nz = tn[((size_t)ai * 4 + cxyz[2]) * nl + (ll[2] + w)];
-                        const double b = ((nx * ny) * nz) * fn;
-                        if (b > th) {
-                            cp[p]++;
-                            bl[p] = std::min(bl[p], b);
-                            bh[p] = std::max(bh[p], b);
-                        }
+                    const double *b = srt + (cmb * nl3 + ((size_t)(ll[2] + w) * nl + (ll[1] + w)) * nl + (ll[0] + w)) * na;
+                    if (na == 0 || !(b[na - 1] * fn > th)) continue;
+                    // the first product that passes
+                    size_t lo = 0, hi = na - 1;
+                    while (lo < hi) {
+                        const size_t mid = (lo + hi) / 2;
+                        if (b[mid] * fn > th)
+                            hi = mid;
+                        else
+                            lo = mid + 1;
                     }
+                    cp[p] += (int)(na - lo);
+                    bl[p] = std::min(bl[p], b[lo] * fn);
+                    bh[p] = std::max(bh[p], b[na - 1] * fn);
                 }
                 napps += cp[p];
             }
@@ -405,29 +415,37 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
     }
     plan.term_apps = napps;
     // Cut the band positions of one job (components gts[0..ng)) into
-    // segments of about `target` passing terms.
-    auto cut = [&](int q, const int *gts, int ng, int target, std::vector<int> &sj, std::vector<int> &s0,
-                   std::vector<int> &s1, std::vector<int> &sa, std::vector<double> &sl, std::vector<double> &sh) {
+    // segments of about `target` passing terms; with emit false only counts
+    // them.  Jobs are cut in parallel, each into its place in job order.
+    struct SegOut {
+        std::vector<int> *job, *p0, *p1, *apps;
+        std::vector<double> *lo, *hi;
+    };
+    auto cut = [&](int q, const int *gts, int ng, int target, const SegOut *o, size_t at) {
         const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
-        int p0 = 0, acc = 0;
+        int p0 = 0, acc = 0, nseg = 0;
         double lo = HUGE_VAL, hi = 0.0;
         for (int p = 0; p < nb; p++) {
             for (int e = 0; e < ng; e++) {
-                const size_t jq = (size_t)q * 8 + gts[e];
-                acc += counts[jq][p];
-                lo = std::min(lo, bmin[jq][p]);
-                hi = std::max(hi, bmax[jq][p]);
+                const long long jq = cof[q] + (long long)gts[e] * nb + p;
+                acc += counts[jq];
+                lo = std::min(lo, bmin[jq]);
+                hi = std::max(hi, bmax[jq]);
             }
             if ((acc >= target && p + 1 < nb) || p + 1 == nb) {
                 // segments without passing terms are dropped: the output
                 // is zeroed before the launches
                 if (acc > 0) {
-                    sj.push_back(q * 8 + gts[0]);
-                    s0.push_back(p0);
-                    s1.push_back(p + 1);
-                    sa.push_back(acc);
-                    sl.push_back(lo);
-                    sh.push_back(hi);
+                    if (o) {
+                        const size_t k = at + nseg;
+                        (*o->job)[k] = q * 8 + gts[0];
+                        (*o->p0)[k] = p0;
+                        (*o->p1)[k] = p + 1;
+                        (*o->apps)[k] = acc;
+                        (*o->lo)[k] = lo;
+                        (*o->hi)[k] = hi;
+                    }
+                    nseg++;
                 }
                 p0 = p + 1;
                 acc = 0;
@@ -435,20 +453,33 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
                 hi = 0.0;
             }
         }
+        return nseg;
     };
     const int fine_target = std::max(1, seg_target / 8);
-    for (int q = 0; q < nq; q++) {
-        for (int gt = 0; gt < 8; gt++) {
-            cut(q, &gt, 1, seg_target, plan.seg_job, plan.seg_p0, plan.seg_p1, plan.seg_apps, plan.seg_bmin,
-                plan.seg_bmax);
-            cut(q, &gt, 1, fine_target, plan.fseg_job, plan.fseg_p0, plan.fseg_p1, plan.fseg_apps, plan.fseg_bmin,
-                plan.fseg_bmax);
+    // kind 0: single-component segments, 1: fine ones, 2: pair segments
+    auto cut_job = [&](int kind, int q, const SegOut *o, size_t at) {
+        int n = 0;
+        for (int gt = 0; gt < (kind == 2 ? 4 : 8); gt++) {
+            const int comps[2] = {gt, gt | 4};
+            n += cut(q, comps, kind == 2 ? 2 : 1, kind == 1 ? fine_target : seg_target, o, o ? at + n : 0);
         }
-        for (int gt = 0; gt < 4; gt++) {
-            const int pair[2] = {gt, gt | 4};
-            cut(q, pair, 2, seg_target, plan.pseg_job, plan.pseg_p0, plan.pseg_p1, plan.pseg_apps, plan.pseg_bmin,
-                plan.pseg_bmax);
-        }
+        return n;
+    };
+    const SegOut outs[3] = {
+        {&plan.seg_job, &plan.seg_p0, &plan.seg_p1, &plan.seg_apps, &plan.seg_bmin, &plan.seg_bmax},
+        {&plan.fseg_job, &plan.fseg_p0, &plan.fseg_p1, &plan.fseg_apps, &plan.fseg_bmin, &plan.fseg_bmax},
+        {&plan.pseg_job, &plan.pseg_p0, &plan.pseg_p1, &plan.pseg_apps, &plan.pseg_bmin, &plan.pseg_bmax}};
+    for (int kind = 0; kind < 3; kind++) {
+        std::vector<size_t> at(nq + 1, 0);
+#pragma omp parallel for schedule(static)
+        for (int q = 0; q < nq; q++) at[q + 1] = (size_t)cut_job(kind, q, nullptr, 0);
+        for (int q = 0; q < nq; q++) at[q + 1] += at[q];
+        const SegOut &o = outs[kind];
+        for (auto *v : {o.job, o.p0, o.p1, o.apps}) v->resize(at[nq]);
+        o.lo->resize(at[nq]);
+        o.hi->resize(at[nq]);
+#pragma omp parallel for schedule(static)
+        for (int q = 0; q < nq; q++) cut_job(kind, q, &o, at[q]);
     }
 }
 

@@ -445,13 +445,26 @@ struct InputBlocks {
     }
 };
 
+// The device pools of one plan's parts, one after another in one
+// allocation (the device plan adopts it; freed here otherwise).
+struct LevelPool {
+    DeviceCtx *c;
+    double *base = nullptr;
+    std::vector<double *> parts;
+    ~LevelPool() { dev_free(c, base); }
+    double *release() {
+        double *b = base;
+        base = nullptr;
+        return b;
+    }
+};
+
 // Host plans of `active` (in parallel across applies when there are several;
 // build_plan parallelises internally otherwise), their pools made on the
-// device from the input trees' blocks, then the exact counts and segments.
-// parts receives the device plan parts, whose pools (pools) the caller frees
-// after dev_plan_create.
+// device from the input trees' blocks into one allocation, then the exact
+// counts and segments.
 void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<std::vector<int>> &nodes,
-                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, std::vector<double *> &pools,
+                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, LevelPool &pools,
                  double *ms3 = nullptr) {
     const int n = (int)active.size();
     const double t0 = now_ms();
@@ -467,13 +480,23 @@ void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<
     for (Job *j : active)
         if (!j->err.empty()) throw std::runtime_error(j->err);
     const double t1 = now_ms();
-    pools.assign(n, nullptr);
     const MRA &mra = *active[0]->out->tree.mra;
-    for (int i = 0; i < n; i++) {
-        Job &j = *active[i];
+    {
+        size_t total = 0;
+        for (Job *j : active) total += (size_t)j->plan.pool_values;
         std::string err;
-        const double *blocks = inputs.of(ctx, j.inp_h);
-        dev_check(dev_pool_fill(ctx->dev, mra, blocks, j.plan, &pools[i], err), err);
+        dev_free(pools.c, pools.base);
+        pools.base = nullptr;
+        dev_check(dev_alloc(pools.c, total, &pools.base, err), err);
+        pools.parts.assign(n, nullptr);
+        size_t off = 0;
+        for (int i = 0; i < n; i++) {
+            Job &j = *active[i];
+            pools.parts[i] = pools.base + off;
+            off += (size_t)j.plan.pool_values;
+            const double *blocks = inputs.of(ctx, j.inp_h);
+            dev_check(dev_pool_fill(ctx->dev, mra, blocks, j.plan, pools.parts[i], err), err);
+        }
     }
     const double t2 = now_ms();
 #pragma omp parallel for schedule(dynamic, 1) if (n > 1)
@@ -493,16 +516,6 @@ void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<
         ms3[2] += now_ms() - t2;
     }
 }
-
-// Frees the device pools of a level's plans.
-struct LevelPools {
-    DeviceCtx *c;
-    std::vector<double *> &p;
-    ~LevelPools() {
-        for (double *q : p) dev_free(c, q);
-        p.clear();
-    }
-};
 
 // Adaptive applies with mrcpp::apply semantics per job; every refinement
 // level of every job still refining is computed in one shared device plan.
@@ -579,15 +592,14 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
                 thr[a][q] = j.prec * prec_factor(j.prec_trees, j.out->tree.keys[j.work[q]]) * std::sqrt(Nsq / M);
         }
         const double t0 = now_ms();
-        std::vector<double *> pools;
-        LevelPools free_pools{ctx->dev, pools};
+        LevelPool pools{ctx->dev};
         build_plans(ctx, active, nodes, thr, inputs, pools, plan3);
         pms += now_ms() - t0;
         std::vector<PlanPart> parts;
         size_t total = 0;
         for (size_t a = 0; a < active.size(); a++) {
             Job *j = active[a];
-            parts.push_back(PlanPart{j->op, &j->op->flat, &j->plan, pools[a]});
+            parts.push_back(PlanPart{j->op, &j->op->flat, &j->plan, pools.parts[a]});
             slots += (double)j->plan.slot_off.size();
             gen += (double)j->plan.n_generated;
             total += j->work.size();
@@ -595,8 +607,9 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         std::string err;
         DevicePlan *dp = nullptr;
         const double t1 = now_ms();
-        int rc = dev_plan_create(ctx->dev, parts, &dp, err);
+        int rc = dev_plan_create(ctx->dev, parts, &dp, err, pools.base);
         if (rc) return dev_fail(rc, err);
+        pools.release();  // the plan owns the pool now
         const double t1r = now_ms();
         t_create += t1r - t1;
         double ms = 0.0;
@@ -813,14 +826,13 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
     }
     const double t0 = now_ms();
     InputBlocks inputs(ctx);
-    std::vector<double *> pools;
-    LevelPools free_pools{ctx->dev, pools};
+    LevelPool pools{ctx->dev};
     build_plans(ctx, active, nodes, th, inputs, pools);
     p->plan_ms = now_ms() - t0;
     std::vector<PlanPart> parts;
     for (int i = 0; i < n; i++) {
         const ApplyPlan &ap = jobs[i].plan;
-        parts.push_back(PlanPart{op[i], &op[i]->flat, &ap, pools[i]});
+        parts.push_back(PlanPart{op[i], &op[i]->flat, &ap, pools.parts[i]});
         p->slots += (double)ap.slot_off.size();
         p->gen += (double)ap.n_generated;
         p->host_apps += (double)ap.term_apps;
@@ -834,8 +846,9 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
     }
     p->thr = std::move(th);
     std::string err;
-    int rc = dev_plan_create(ctx->dev, parts, &p->dev, err);
+    int rc = dev_plan_create(ctx->dev, parts, &p->dev, err, pools.base);
     if (rc) return dev_fail(rc, err);
+    pools.release();  // the plan owns the pool now
     {
         // (branch, first child) pairs of every output grid, by depth, as
         // indices into the plan's concatenated output nodes
@@ -1915,43 +1928,64 @@ int mwg_tree_get(mwg_tree t, int *depth, long long *l, double *coefs) {
     });
 }
 
+namespace {
+// A tree from a complete set of node keys (every parent present) in dst's
+// construction order (roots, then each split's 8 children); src_of[j] is
+// the position in the key list of dst's node j.  dst is structure only.
+int tree_from_keys(const MRA &mra, int n, const int *depth, const long long *l, Tree &dst, std::vector<int> &src_of) {
+    Tree src;
+    src.mra = &mra;
+    src.structure_only = true;
+    for (int i = 0; i < n; i++) {
+        NodeKey k{depth[i], {l[3 * i], l[3 * i + 1], l[3 * i + 2]}};
+        if (k.depth < 0 || k.depth > mra.max_depth || !mra.in_world(k.depth, k.l))
+            return fail(MWG_ERR_ARG, "node key outside the world box");
+        if (src.find(k.depth, k.l) >= 0) return fail(MWG_ERR_ARG, "duplicate node key");
+        src.add_node(k, -1);
+    }
+    dst.mra = &mra;
+    dst.structure_only = true;
+    dst.make_roots();
+    src_of.assign(dst.size(), -1);
+    for (int i = 0; i < dst.size(); i++) {
+        int j = src.find(dst.keys[i].depth, dst.keys[i].l);
+        if (j < 0) return fail(MWG_ERR_ARG, "root node missing");
+        src_of[i] = j;
+    }
+    for (int i = 0; i < dst.size(); i++) {
+        const NodeKey k = dst.keys[i];
+        long long c0[3] = {2 * k.l[0], 2 * k.l[1], 2 * k.l[2]};
+        if (src.find(k.depth + 1, c0) < 0) continue;
+        int first = dst.split(i);
+        src_of.resize(dst.size(), -1);
+        for (int c = 0; c < 8; c++) {
+            int j = src.find(dst.keys[first + c].depth, dst.keys[first + c].l);
+            if (j < 0) return fail(MWG_ERR_ARG, "incomplete node family (a child is missing)");
+            src_of[first + c] = j;
+        }
+    }
+    if (dst.size() != n) return fail(MWG_ERR_ARG, "nodes unreachable from the roots (missing parent)");
+    dst.structure_only = false;
+    return MWG_OK;
+}
+}  // namespace
+
 int mwg_tree_set(mwg_tree t, int n, const int *depth, const long long *l, const double *coefs) {
     return guarded([&] {
         if (!t || n < 1 || !depth || !l) return fail(MWG_ERR_ARG, "bad arguments");
         const MRA &mra = *t->tree.mra;
         const int blk = mra.block();
-        Tree src;
-        src.mra = &mra;
-        for (int i = 0; i < n; i++) {
-            NodeKey k{depth[i], {l[3 * i], l[3 * i + 1], l[3 * i + 2]}};
-            if (k.depth < 0 || k.depth > mra.max_depth || !mra.in_world(k.depth, k.l))
-                return fail(MWG_ERR_ARG, "node key outside the world box");
-            if (src.find(k.depth, k.l) >= 0) return fail(MWG_ERR_ARG, "duplicate node key");
-            src.add_node(k, -1);
-            if (coefs) std::memcpy(src.node(i), coefs + (size_t)i * blk, sizeof(double) * blk);
-        }
         Tree dst;
-        dst.mra = &mra;
-        dst.make_roots();
-        for (int i = 0; i < dst.size(); i++) {
-            int j = src.find(dst.keys[i].depth, dst.keys[i].l);
-            if (j < 0) return fail(MWG_ERR_ARG, "root node missing");
-            std::memcpy(dst.node(i), src.node(j), sizeof(double) * blk);
+        std::vector<int> src_of;
+        if (int rc = tree_from_keys(mra, n, depth, l, dst, src_of)) return rc;
+        dst.coefs.resize((size_t)n * blk);  // every block is written below
+#pragma omp parallel for schedule(static) if (n > 4096)
+        for (int i = 0; i < n; i++) {
+            if (coefs)
+                std::memcpy(dst.node(i), coefs + (size_t)src_of[i] * blk, sizeof(double) * blk);
+            else
+                std::fill_n(dst.node(i), blk, 0.0);
         }
-        int placed = dst.size();
-        for (int i = 0; i < dst.size(); i++) {
-            const NodeKey k = dst.keys[i];
-            long long c0[3] = {2 * k.l[0], 2 * k.l[1], 2 * k.l[2]};
-            if (src.find(k.depth + 1, c0) < 0) continue;
-            int first = dst.split(i);
-            for (int c = 0; c < 8; c++) {
-                int j = src.find(dst.keys[first + c].depth, dst.keys[first + c].l);
-                if (j < 0) return fail(MWG_ERR_ARG, "incomplete node family (a child is missing)");
-                std::memcpy(dst.node(first + c), src.node(j), sizeof(double) * blk);
-            }
-            placed += 8;
-        }
-        if (placed != n) return fail(MWG_ERR_ARG, "nodes unreachable from the roots (missing parent)");
         dst.sqnorm = tree_square_norm(dst);
         t->tree = std::move(dst);
         t->tree.mra = &mra;
@@ -1994,15 +2028,12 @@ int mwg_tree_pack_size(mwg_tree t, long long *bytes) {
     return MWG_OK;
 }
 
-int mwg_tree_pack(mwg_tree t, void *buffer, long long bytes) {
-    return guarded([&] {
-        if (!t || !buffer) return fail(MWG_ERR_ARG, "null pointer argument");
-        ensure_host(t);
-        const Tree &T = t->tree;
-        const MRA &mra = *T.mra;
-        const long long n = T.size();
-        if (bytes < 0 || (size_t)bytes < packed_bytes(n, mra.block())) return fail(MWG_ERR_ARG, "buffer too small");
-        PackedHeader h{};
+namespace {
+// The packed form's header and keys (everything before the blocks) into p.
+void pack_head(const Tree &T, char *p) {
+    const MRA &mra = *T.mra;
+    const long long n = T.size();
+    PackedHeader h{};
         std::memcpy(h.magic, kPackMagic, 8);
         h.version = 1;
         h.kp1 = mra.kp1();
@@ -2015,7 +2046,6 @@ int mwg_tree_pack(mwg_tree t, void *buffer, long long bytes) {
         h.sfac = mra.sfac;
         h.n_nodes = n;
         h.sqnorm = T.sqnorm;
-        char *p = static_cast<char *>(buffer);
         std::memcpy(p, &h, sizeof(h));
         p += sizeof(h);
         std::vector<int32_t> dep(n);
@@ -2029,8 +2059,131 @@ int mwg_tree_pack(mwg_tree t, void *buffer, long long bytes) {
         std::memset(p + db, 0, pad8(db) - db);
         p += pad8(db);
         std::memcpy(p, l.data(), sizeof(int64_t) * 3 * (size_t)n);
-        p += sizeof(int64_t) * 3 * (size_t)n;
-        std::memcpy(p, T.coefs.data(), sizeof(double) * T.coefs.size());
+}
+size_t head_bytes(long long n) { return sizeof(PackedHeader) + pad8(sizeof(int32_t) * (size_t)n) + sizeof(int64_t) * 3 * (size_t)n; }
+
+// Validates a packed header against mra; the node count on success.
+long long check_head(const MRA &mra, const PackedHeader &h, long long bytes) {
+    if (std::memcmp(h.magic, kPackMagic, 8) != 0) throw std::invalid_argument("packed tree: bad magic");
+    if (h.version != 1) throw std::invalid_argument("packed tree: unknown version");
+    bool same = h.kp1 == mra.kp1() && h.basis == mra.basis.type && h.n0 == mra.n0 &&
+                std::memcmp(&h.sfac, &mra.sfac, sizeof(double)) == 0;
+    for (int d = 0; d < 3; d++) same = same && h.corner[d] == mra.corner[d] && h.boxes[d] == mra.boxes[d];
+    if (!same) throw std::invalid_argument("packed tree: written on a different MRA");
+    if (h.n_nodes < 1 || h.n_nodes >= (1LL << 31)) throw std::invalid_argument("packed tree: bad node count");
+    if ((size_t)bytes != packed_bytes(h.n_nodes, mra.block())) throw std::invalid_argument("packed tree: size mismatch");
+    return h.n_nodes;
+}
+}  // namespace
+
+int mwg_tree_pack(mwg_tree t, void *buffer, long long bytes) {
+    return guarded([&] {
+        if (!t || !buffer) return fail(MWG_ERR_ARG, "null pointer argument");
+        ensure_host(t);
+        const Tree &T = t->tree;
+        const long long n = T.size();
+        if (bytes < 0 || (size_t)bytes < packed_bytes(n, T.mra->block())) return fail(MWG_ERR_ARG, "buffer too small");
+        char *p = static_cast<char *>(buffer);
+        pack_head(T, p);
+        std::memcpy(p + head_bytes(n), T.coefs.data(), sizeof(double) * T.coefs.size());
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_tree_pack_device(mwg_tree t, mwg_context ctx, void *dst, long long bytes) {
+    return guarded_ctx(ctx, [&] {
+        if (!t || !ctx || !dst) return fail(MWG_ERR_ARG, "null pointer argument");
+        const Tree &T = t->tree;
+        const long long n = T.size();
+        if (bytes < 0 || (size_t)bytes < packed_bytes(n, T.mra->block())) return fail(MWG_ERR_ARG, "buffer too small");
+        std::vector<char> head(head_bytes(n));
+        pack_head(T, head.data());
+        std::string err;
+        // the head as doubles: its size is a multiple of 8 bytes
+        dev_check(dev_upload_into(ctx->dev, reinterpret_cast<const double *>(head.data()), head.size() / 8,
+                                  static_cast<double *>(dst), err),
+                  err);
+        TempBlocks tb{ctx->dev, {}};
+        const double *blocks = tb.of(ctx, t);
+        dev_check(dev_d2d(ctx->dev, reinterpret_cast<double *>(static_cast<char *>(dst) + head.size()), blocks,
+                          tree_values(t), err),
+                  err);
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_device_buffer_create(mwg_context ctx, long long bytes, void **dev) {
+    return guarded_ctx(ctx, [&] {
+        if (!ctx || !dev || bytes < 0) return fail(MWG_ERR_ARG, "bad arguments");
+        std::string err;
+        double *p = nullptr;
+        dev_check(dev_alloc(ctx->dev, ((size_t)bytes + 7) / 8, &p, err), err);
+        *dev = p;
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_device_buffer_destroy(mwg_context ctx, void *dev) {
+    if (!ctx) return fail(MWG_ERR_ARG, "null context");
+    dev_free(ctx->dev, dev);
+    return MWG_OK;
+}
+
+int mwg_device_buffer_read(mwg_context ctx, const void *dev, void *host, long long bytes) {
+    return guarded_ctx(ctx, [&] {
+        if (!ctx || !dev || !host || bytes < 0) return fail(MWG_ERR_ARG, "bad arguments");
+        std::string err;
+        dev_check(dev_copy_d2h(ctx->dev, host, dev, (size_t)bytes, err), err);
+        return (int)MWG_OK;
+    });
+}
+
+int mwg_tree_unpack_device(mwg_tree t, mwg_context ctx, const void *src, long long bytes) {
+    return guarded_ctx(ctx, [&] {
+        if (!t || !ctx || !src) return fail(MWG_ERR_ARG, "null pointer argument");
+        if (t->plan_refs > 0) return fail(MWG_ERR_ARG, "unpack: a plan still writes into this tree");
+        const MRA &mra = *t->tree.mra;
+        if (bytes < (long long)sizeof(PackedHeader)) return fail(MWG_ERR_ARG, "packed tree: truncated header");
+        std::string err;
+        PackedHeader h;
+        dev_check(dev_download(ctx->dev, static_cast<const double *>(src), sizeof(h) / 8, reinterpret_cast<double *>(&h),
+                               err),
+                  err);
+        const long long n = check_head(mra, h, bytes);
+        std::vector<char> head(head_bytes(n));
+        dev_check(dev_download(ctx->dev, static_cast<const double *>(src), head.size() / 8,
+                               reinterpret_cast<double *>(head.data()), err),
+                  err);
+        std::vector<int> dep(n);
+        std::vector<long long> l(3 * n);
+        {
+            const char *p = head.data() + sizeof(h);
+            std::vector<int32_t> d32(n);
+            std::memcpy(d32.data(), p, sizeof(int32_t) * (size_t)n);
+            for (long long i = 0; i < n; i++) dep[i] = d32[i];
+            p += pad8(sizeof(int32_t) * (size_t)n);
+            std::vector<int64_t> l64(3 * n);
+            std::memcpy(l64.data(), p, sizeof(int64_t) * 3 * (size_t)n);
+            for (long long i = 0; i < 3 * n; i++) l[i] = l64[i];
+        }
+        Tree dst;
+        std::vector<int> src_of;
+        if (int rc = tree_from_keys(mra, (int)n, dep.data(), l.data(), dst, src_of)) return rc;
+        // the tree becomes resident on ctx, its blocks gathered on the device
+        if (t->dctx != ctx) {
+            if (int rc = mwg_tree_set_resident(t, ctx, 1)) return rc;
+        }
+        const int blk = mra.block();
+        DeviceResult res{ctx->dev, nullptr};
+        dev_check(dev_alloc(ctx->dev, (size_t)n * blk, &res.p, err), err);
+        const double *sb = reinterpret_cast<const double *>(static_cast<const char *>(src) + head.size());
+        dev_check(dev_gather_blocks(ctx->dev, sb, src_of, blk, res.p, err), err);
+        std::vector<double> s2(n), w2(n);
+        dev_check(dev_node_norms(ctx->dev, mra.kp1(), res.p, n, s2.data(), w2.data(), err), err);
+        t->tree = std::move(dst);
+        t->tree.mra = &mra;
+        t->apply_thr.clear();
+        take_device_result(ctx, t, res.release(), std::move(s2), std::move(w2));
         return (int)MWG_OK;
     });
 }
@@ -2042,15 +2195,7 @@ int mwg_tree_unpack(mwg_tree t, const void *buffer, long long bytes) {
         if (bytes < (long long)sizeof(PackedHeader)) return fail(MWG_ERR_ARG, "packed tree: truncated header");
         PackedHeader h;
         std::memcpy(&h, buffer, sizeof(h));
-        if (std::memcmp(h.magic, kPackMagic, 8) != 0) return fail(MWG_ERR_ARG, "packed tree: bad magic");
-        if (h.version != 1) return fail(MWG_ERR_ARG, "packed tree: unknown version");
-        bool same = h.kp1 == mra.kp1() && h.basis == mra.basis.type && h.n0 == mra.n0 &&
-                    std::memcmp(&h.sfac, &mra.sfac, sizeof(double)) == 0;
-        for (int d = 0; d < 3; d++) same = same && h.corner[d] == mra.corner[d] && h.boxes[d] == mra.boxes[d];
-        if (!same) return fail(MWG_ERR_ARG, "packed tree: written on a different MRA");
-        if (h.n_nodes < 1 || h.n_nodes >= (1LL << 31)) return fail(MWG_ERR_ARG, "packed tree: bad node count");
-        const long long n = h.n_nodes;
-        if ((size_t)bytes != packed_bytes(n, mra.block())) return fail(MWG_ERR_ARG, "packed tree: size mismatch");
+        const long long n = check_head(mra, h, bytes);
         const char *p = static_cast<const char *>(buffer) + sizeof(h);
         std::vector<int> dep(n);
         std::vector<int32_t> d32(n);

@@ -209,3 +209,37 @@ def test_residency_follows_host_reads_and_writes(ctx):
     h = mw.FunctionTree(mra).project_gaussians(prec, [1.0], [3.0], [[0.0, 0.0, 0.0]]).set_resident(c2)
     assert mw.lib.mwg_context_destroy(c2._h) != 0
     h.set_resident(c2, False)
+
+
+def test_device_pack_round_trip(ctx):
+    """mwg_tree_pack_device writes the same bytes as the host pack into
+    device memory, from a host tree and from a device-only tree, and
+    mwg_tree_unpack_device rebuilds the tree from them bit for bit, resident
+    on the device (the tree wire format of a device-to-device send)."""
+    mra = _mra(5)
+    prec = 1e-4
+    f = mw.FunctionTree(mra).project_gaussians(prec, [1.0, -0.4], [3.0, 30.0], [[0.1, 0.0, 0.2], [-0.3, 0.2, 0.1]])
+    g = mw.FunctionTree(mra).project_gaussians(prec, [0.7], [8.0], [[0.0, -0.1, 0.1]])
+    prod = mw.multiply(prec, mw.FunctionTree(mra).set_resident(ctx), f, g, abs_prec=True, ctx=ctx)
+    assert prod.residency() == (True, False, True)
+    for name, t in (("host tree", f), ("device-only tree", prod)):
+        nb = t.pack_size()
+        buf = mw.DeviceBuffer(nb, ctx=ctx)
+        t.pack_device(buf.ptr, nb, ctx=ctx)
+        assert buf.read() == t.pack(), name
+        u = mw.FunctionTree(mra).unpack_device(buf.ptr, nb, ctx=ctx)
+        assert u.residency() == (True, False, True)
+        # the same nodes and blocks (the node order is the tree's own
+        # construction order, which may differ from the sender's)
+        ud, ul, uc = u.nodes()
+        td, tl, tc = t.nodes()
+        um = {(int(d),) + tuple(int(x) for x in l): i for i, (d, l) in enumerate(zip(ud, ul))}
+        tm = {(int(d),) + tuple(int(x) for x in l): i for i, (d, l) in enumerate(zip(td, tl))}
+        assert um.keys() == tm.keys(), name
+        assert all(np.array_equal(uc[um[k]], tc[tm[k]]) for k in tm), name
+        assert abs(u.square_norm() - t.square_norm()) <= 1e-13 * t.square_norm()
+        print("%s: %d nodes, %d packed bytes, device pack == host pack, unpacked bit-identical" % (name, t.size(), nb))
+        with pytest.raises(mw.MWGError):
+            mw.FunctionTree(mra).unpack_device(buf.ptr, nb - 8, ctx=ctx)
+        with pytest.raises(mw.MWGError):
+            t.pack_device(buf.ptr, nb - 8, ctx=ctx)
