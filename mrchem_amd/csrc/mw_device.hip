@@ -560,6 +560,10 @@ struct SplitShape {
     // counts; 136 exchange pair applies at k = 7: -25 %); above, the extra
     // live registers cost occupancy or spill and the loads stay conditional
     static constexpr bool UNC = N <= 8;
+    // N <= 10: screening resolves each item's F block (address and scale), so
+    // a phase loads F without a chain of dependent loads; at N > 10 the extra
+    // screening work costs more than it saves (64 applies at k = 11: +6 %)
+    static constexpr bool CB = N <= 10;
     // lanes without a result row store nothing (else they write a scratch row)
     static constexpr bool MASK = MWG_SPLIT_MASK_ALL || N <= 8;
     // 16-byte chunks per row: hi, lo, tail (+ the extra chunk for N > 10).
@@ -837,9 +841,17 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
     mwg_h4 fe[TWG];
 #pragma unroll
     for (int tt = 0; tt < TWG; tt++) fe[tt] = zero4;
-    // cb: the component block of the pool (F rows of the x pass)
-    auto load_f = [&](int cb) {
-        const __attribute__((address_space(1))) _Float16 *fb = poolg + (long long)cb * (N2 * RH);
+    // the F rows of the x pass: component block cb of the pool, or (without
+    // CB) that of group grp = band position << 3 | ft, with its scale sF
+    float sF = 1.f;
+    auto load_f = [&](int cbg) {
+        long long cb = cbg;
+        if constexpr (!S::CB) {
+            const int s = a.band[j.boff + (cbg >> 3)];
+            cb = a.slot_off[s] / N3 + (cbg & 7);
+            sF = (float)pool_sc[cb];
+        }
+        const __attribute__((address_space(1))) _Float16 *fb = poolg + cb * (N2 * RH);
 #pragma unroll
         for (int tt = 0; tt < TWG; tt++) {
             fa[tt] = *reinterpret_cast<const __attribute__((address_space(1))) mwg_h8 *>(fb + fl[tt]);
@@ -940,9 +952,12 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
                     // the input component block (its F rows in the split
                     // pool) and its scale, resolved here so that a phase
                     // starts the F load without a chain of dependent loads
-                    itm.cb = (int)(a.slot_off[slot] / N3) + ft;
+                    double sxy = opsc[itm.bx] * opsc[itm.by];
+                    if constexpr (SplitShape<N>::CB) {
+                        itm.cb = (int)(a.slot_off[slot] / N3) + ft;
+                        sxy *= pool_sc[itm.cb];
+                    }
                     wl[pos] = itm;
-                    const double sxy = pool_sc[itm.cb] * opsc[itm.bx] * opsc[itm.by];
                     wsig[pos] = make_float2(t0 ? (float)(sxy * opsc[itm.bz0]) : 0.f,
                                             t1 ? (float)(sxy * opsc[itm.bz1]) : 0.f);
                 });
@@ -1012,11 +1027,12 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
                         const int g = __builtin_amdgcn_readfirstlane(wl[q].grp);
                         if ((g & mask_grp) != cur_grp) {
                             cur_grp = g & mask_grp;
-                            load_f(__builtin_amdgcn_readfirstlane(wl[q].cb));
+                            load_f(S::CB ? __builtin_amdgcn_readfirstlane(wl[q].cb) : cur_grp);
                         }
                         msk0 = ((g >> kSingleBit) & 1) ? 0xffffffffu : 0xffffe000u;
                         zm0 = (g >> kZ0Bit) & 3;
-                        sig0 = wsig[q];
+                        const float2 w2 = wsig[q];
+                        sig0 = make_float2(sF * w2.x, sF * w2.y);
                         x_pass(xt + P * BUFH, oc[0], ec[0], msk0);
                     }
                     if (hy) y_pass(xt + (1 - P) * BUFH, yt + (1 - P) * BUFH, oc[1], ec[1], msk1);
