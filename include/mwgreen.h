@@ -325,11 +325,12 @@ int mwg_plan_apply(mwg_plan plan, double *stats, int n);
  * time) and counts only if its largest output difference to the reference is
  * at most max_rel_err of the reference's largest value.  The fastest is kept
  * in the plan and cached in the context under (k + 1, operator terms, widest
- * band, mode): later plans with that key start from it.  report (n <= 8):
- * [0] tau, [1] single-rounding ratio, [2] order, [3] its device ms,
- * [4] device ms of the starting setting, [5] its relative error,
- * [6] configurations timed, [7] speedup [4]/[3].  No counterpart in MRChem's
- * calls. */
+ * band, mode): later plans with that key start from it, and the pick is
+ * never slower than the starting setting when that one meets the bound.
+ * report (n <= 9): [0] tau, [1] single-rounding ratio, [2] order, [3] its
+ * device ms, [4] device ms of the starting setting, [5] its relative error,
+ * [6] configurations timed, [7] speedup [4]/[3], [8] relative error of the
+ * starting setting.  No counterpart in MRChem's calls. */
 int mwg_plan_tune(mwg_plan plan, double max_rel_err, int reps, double *report, int n);
 /* With on != 0, later mwg_plan_execute / mwg_plan_apply calls replay one
  * hipGraph of the whole step (output and counters cleared by kernels, the

@@ -597,10 +597,10 @@ class ApplyPlan:
         queue order) against an FP64 run of the same plan; the choice stays
         in the plan and is cached in the context for plans of the same
         (k + 1, terms, band width, mode).  Returns the picks and times."""
-        v = np.zeros(8)
-        _check(lib.mwg_plan_tune(self._h, float(max_rel_err), int(reps), _ptr(v, _d), 8))
+        v = np.zeros(9)
+        _check(lib.mwg_plan_tune(self._h, float(max_rel_err), int(reps), _ptr(v, _d), 9))
         return {"tau": v[0], "single_ratio": v[1], "order": int(v[2]), "ms": v[3], "start_ms": v[4],
-                "rel_err": v[5], "tried": int(v[6]), "speedup": v[7]}
+                "rel_err": v[5], "tried": int(v[6]), "speedup": v[7], "start_rel_err": v[8]}
 
     def use_graph(self, on=True):
         """Later execute()/apply() calls replay one captured hipGraph of the

@@ -87,8 +87,9 @@ __device__ __forceinline__ const float *op_blocks<float>(const AppDesc &ad) {
     return ad.blk32;
 }
 
-// One workgroup per job (output node g, output component gt); its NW waves
-// share the job's 16-wide tiles (TW tiles each).
+// One workgroup per job (output node g, output component gt, or the pair of
+// components gt and gt | 4 of a pair job); its NW waves share the job's
+// 16-wide tiles (TW tiles each).
 //
 // Screening: the active terms of the job's depth are taken in chunks of 64
 // (one term per lane).  For the chunk, the 1D norms of every term, component
@@ -107,11 +108,16 @@ __device__ __forceinline__ const float *op_blocks<float>(const AppDesc &ad) {
 //   x: X[(jz,jy)][ix]  = F[(jz,jy)][jx] . Ox^T[jx][ix]   F tiles in registers
 //   y: Y[iy][(jz,ix)]  = Oy[iy][jy] . X'[jy][(jz,ix)]
 //   z: acc[iz][(iy,ix)] += Oz[iz][jz] . Y''[jz][(iy,ix)]  acc tiles persist
+// A pair job's components share Ox and Oy, so its items run the x and y
+// chains once and the z chain for each component whose bound passes (two
+// accumulators): 4 chains instead of 6 when both pass.
 // X and Y change layout through one padded LDS buffer whose lane addresses
 // are a base plus a compile-time offset (padding entries stay zero or finite
 // and only meet zero operands).  While an item computes, the operator blocks
 // of the next item are loaded (registers, then a double-buffered LDS stage)
-// and, when the next item starts a new (f, ft) group, its F tiles too.
+// and, when the next item starts a new (f, ft) group, its F tiles too (into
+// the F registers, which the x chain no longer needs, during the y and z
+// chains).
 // MFMA lane layouts (checked on gfx950): A[i][k] in lane i + 16k, B[k][j] in
 // lane j + 16k; FP64 D[i][j] in lane j + 16 (i mod 4), register i / 4; FP32
 // D[i][j] in lane j + 16 (i / 4), register i mod 4.
@@ -142,10 +148,13 @@ struct JobShape {
     static constexpr int N2 = N * N, N3 = N2 * N;
     static constexpr int NT = (N2 + 15) / 16;  // 16-wide tiles over N^2
     static constexpr int KS = (N + 3) / 4;     // 4-deep k steps over N
-    static constexpr int TW = 4;               // tiles per wave
-    static constexpr int NW = (NT + TW - 1) / TW;
+    // waves per job, at most 4 tiles each (fewer tiles per wave measured
+    // slower at N = 8, 10, 12: the operator operands are read from LDS once
+    // per wave and item)
+    static constexpr int NW = (NT + 3) / 4;
+    static constexpr int TW = (NT + NW - 1) / NW;  // tiles per wave
     static constexpr int T = 64 * NW;          // threads per job
-    static constexpr int NTP = NW * TW;        // tiles including padding tiles
+    static constexpr int NTP = NW * TW;        // tiles including padding tiles (N = 10: 8 for 7)
     // Xp[row][16], row = (jz,jy) flattened; Yp[16][CP], [iy][(jz,ix)]
     static constexpr int XROWS_W = 16 * NTP;
     static constexpr int XROWS_R = ((16 * NTP - 1) / N) * N + 4 * KS;
@@ -153,8 +162,7 @@ struct JobShape {
     static constexpr int CP = 16 * NTP > 4 * KS * N ? 16 * NTP : 4 * KS * N;
     static constexpr int XSZ = XROWS * 16, YSZ = 16 * CP;
     static constexpr int BUF = XSZ > YSZ ? XSZ : YSZ;
-    static constexpr int OB = 3 * N2;          // operator values per stage
-    static constexpr int OPL = (OB + T - 1) / T;
+    static constexpr int OB = 4 * N2;          // operator values per stage (Ox, Oy, Oz and Oz of gt | 4)
 };
 
 constexpr int kMaxBand = 9;     // band half-width limit
@@ -322,16 +330,23 @@ struct ApplySmem {
 // One work segment: the terms of output job seg_job[seg] over band positions
 // [seg_p0, seg_p1) in the launch's class, added into the FP64 output.  nrm_s
 // is the norm table [term][c][l+W], 64 x nrm_stride(2 Wmax + 1) doubles.
-template <int N, typename T>
+// With PAIR the segment is a pair segment: its job stands for the output
+// components gt and gt | 4 (gt < 4), whose operators differ only along z.
+// An item then carries which of the two passes; its x and y passes are
+// computed once and its z pass once per passing component, into two
+// accumulators.  Worklist items are (p << 11) | (t1 << 10) | (t0 << 9) |
+// (ft << 6) | lane-term.
+template <int N, typename T, bool PAIR>
 __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int seg, double lo_mult, double hi_mult,
                                               int cls_counter, ApplySmem<N, T> &sm, double *nrm_s) {
     using S = JobShape<N>;
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
-    constexpr int N2 = S::N2, N3 = S::N3, KS = S::KS, TW = S::TW, NTH = S::T;
-    constexpr int CP = S::CP, BUF = S::BUF, OB = S::OB, OPL = S::OPL;
+    constexpr int N2 = S::N2, N3 = S::N3, NT = S::NT, KS = S::KS, TW = S::TW, NTH = S::T;
+    constexpr int CP = S::CP, BUF = S::BUF;
+    constexpr int OB = (PAIR ? 4 : 3) * N2, OPL = (OB + NTH - 1) / NTH;
     T *buf = sm.buf;
-    T(*ops)[OB] = sm.ops;
+    T(*ops)[S::OB] = sm.ops;
     int *wl = sm.wl;
     int *ltab = sm.ltab;
     long long *btab = sm.btab;
@@ -347,7 +362,9 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
     const int *gb = j.gb;
     const T *opblk = op_blocks<T>(ad);
 
-    // this wave's tiles: t = wv*TW + tt
+    // this wave's tiles: t = wv*TW + tt; tiles t >= NT (N = 10: the eighth)
+    // are skipped (wave-uniform), their LDS entries keep finite values that
+    // only meet zero operator entries
     int ybase[TW], zbase[TW];
 #pragma unroll
     for (int tt = 0; tt < TW; tt++) {
@@ -356,10 +373,13 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
         ybase[tt] = (q * N + lq) * 16 + r;  // Xp[(jz*N + 4k + lq)][ix]
         zbase[tt] = q * CP + r + lq * N;    // Yp[iy][(4k + lq)*N + ix]
     }
-    acc_t acc[TW];
+    // (in FP64 only: in FP32 the branches cost more registers than the
+    // padding tile's MFMAs)
+    auto live = [&](int tt) { return S::NTP == NT || !std::is_same<T, double>::value || wv * TW + tt < NT; };
+    acc_t acc[TW], acc1[TW];  // acc1: gt | 4 (PAIR only)
 #pragma unroll
-    for (int tt = 0; tt < TW; tt++) acc[tt] = acc_t{T(0), T(0), T(0), T(0)};
-    unsigned long long napps = 0;
+    for (int tt = 0; tt < TW; tt++) acc[tt] = acc1[tt] = acc_t{T(0), T(0), T(0), T(0)};
+    unsigned long long napps = 0, napps1 = 0;
     // F tiles of band node p, component ft: x-pass A operand F[16t + lc][4k + lq]
     auto load_f = [&](int p, int ft, T (*dst)[KS]) {
         const int s = a.band[boff + p];
@@ -374,9 +394,12 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
             }
         }
     };
-    // operator blocks Ox, Oy, Oz of worklist item `it` into this thread's share
+    // operator blocks Ox, Oy, Oz (and Oz of gt | 4) of worklist item `it`
+    // into this thread's share; z blocks of a component that does not pass
+    // are left out
     auto fetch = [&](int it, T *regs) {
-        const int p = it >> 9, ft = (it >> 6) & 7, b = it & 63;
+        const int p = it >> 11, ft = (it >> 6) & 7, b = it & 63;
+        const bool t0 = !PAIR || ((it >> 9) & 1), t1 = PAIR && ((it >> 10) & 1);
         const int lx = p % side - w, ly = (p / side) % side - w, lz = p / (side * side) - w;
         const int Lb = ltab[b];
         const long long bb = btab[b];
@@ -384,17 +407,19 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
         for (int u = 0; u < OPL; u++) {
             const int e = tid + NTH * u;
             regs[u] = T(0);
-            if (e < OB) {
-                const int dim = e / N2, r = e % N2;
+            const int dim = e / N2, r = e % N2;
+            if (e < OB && (dim < 2 || (dim == 2 ? t0 : t1))) {
                 // selects, not a runtime index into a small array (which would
-                // live in scratch)
-                const int gbd = dim == 0 ? gb[0] : (dim == 1 ? gb[1] : gb[2]);
+                // live in scratch); dim 3 is z of gt | 4 (gb[2] = 1)
+                const int gbd = dim == 0 ? gb[0] : (dim == 1 ? gb[1] : (dim == 2 ? gb[2] : 1));
                 const int ld = dim == 0 ? lx : (dim == 1 ? ly : lz);
-                const int c = (gbd << 1) | ((ft >> dim) & 1);
+                const int fb = dim == 0 ? (ft & 1) : (dim == 1 ? ((ft >> 1) & 1) : ((ft >> 2) & 1));
+                const int c = (gbd << 1) | fb;
                 regs[u] = opblk[bb + ((long long)c * (2 * Lb + 1) + (ld + Lb)) * N2 + r];
             }
         }
     };
+    auto group = [](int it) { return ((it >> 11) << 3) | ((it >> 6) & 7); };
 
     for (int base = 0; base < j.nact; base += 64) {
         const int cnt = min(64, j.nact - base);
@@ -404,19 +429,20 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
         int pi = 8 * p_begin;
         while (pi < 8 * p_end) {
             // screening into the worklist (wave 0 writes the items)
-            unsigned long long n1 = 0;
-            const int n = screen_scan<false>(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane, napps, n1,
-                                             [&](int pos, int p, int ft, int, int, int, int, double, double, bool,
-                                                 bool) {
-                                                 if (wv == 0) wl[pos] = (p << 9) | (ft << 6) | lane;
-                                             });
+            const int n = screen_scan<PAIR>(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane, napps, napps1,
+                                            [&](int pos, int p, int ft, int, int, int, int, double, double, bool t0,
+                                                bool t1) {
+                                                if (wv == 0)
+                                                    wl[pos] = (p << 11) | ((int)t1 << 10) | ((int)t0 << 9) |
+                                                              (ft << 6) | lane;
+                                            });
             __syncthreads();
             if (n == 0) continue;
             // pipelined application of the worklist
-            T fa[TW][KS], fnext[TW][KS];
+            T fa[TW][KS];
             T pre[OPL];
             int it = wl[0];
-            load_f(it >> 9, (it >> 6) & 7, fa);
+            load_f(it >> 11, (it >> 6) & 7, fa);
             fetch(it, pre);
             int stage = 0;
             for (int i = 0; i < n; i++) {
@@ -427,20 +453,21 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
                 }
                 const bool more = i + 1 < n;
                 const int nx = more ? wl[i + 1] : it;
-                const bool newgroup = more && ((nx >> 6) != (it >> 6));
+                const bool newgroup = more && group(nx) != group(it);
+                const bool t0 = !PAIR || ((it >> 9) & 1), t1 = PAIR && ((it >> 10) & 1);
                 if (more) fetch(nx, pre);
-                if (newgroup) load_f(nx >> 9, (nx >> 6) & 7, fnext);
                 __syncthreads();
                 // operator operands: Ox[lc][4k+lq] (B of x as Ox^T), Oy, Oz (A of y, z)
-                T ox[KS], oy[KS], oz[KS];
+                T ox[KS], oy[KS], oz[KS], oz1[PAIR ? KS : 1];
                 const T *O = ops[stage];
 #pragma unroll
                 for (int k = 0; k < KS; k++) {
-                    const int j = 4 * k + lq;
-                    const bool v = lc < N && j < N;
-                    ox[k] = v ? O[lc * N + j] : T(0);
-                    oy[k] = v ? O[N2 + lc * N + j] : T(0);
-                    oz[k] = v ? O[2 * N2 + lc * N + j] : T(0);
+                    const int jj = 4 * k + lq;
+                    const bool v = lc < N && jj < N;
+                    ox[k] = v ? O[lc * N + jj] : T(0);
+                    oy[k] = v ? O[N2 + lc * N + jj] : T(0);
+                    oz[k] = v ? O[2 * N2 + lc * N + jj] : T(0);
+                    if (PAIR) oz1[k] = v ? O[3 * N2 + lc * N + jj] : T(0);
                 }
                 // x pass -> Xp[row][ix]
                 {
@@ -450,13 +477,18 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
 #pragma unroll
                     for (int k = 0; k < KS; k++)
 #pragma unroll
-                        for (int tt = 0; tt < TW; tt++) dx[tt] = MF::run(fa[tt][k], ox[k], dx[tt]);
+                        for (int tt = 0; tt < TW; tt++)
+                            if (live(tt)) dx[tt] = MF::run(fa[tt][k], ox[k], dx[tt]);
 #pragma unroll
                     for (int tt = 0; tt < TW; tt++)
+                        if (live(tt))
 #pragma unroll
-                        for (int r = 0; r < 4; r++)
-                            buf[(16 * (wv * TW + tt) + MF::drow(lq, r)) * 16 + lc] = dx[tt][r];
+                            for (int r = 0; r < 4; r++)
+                                buf[(16 * (wv * TW + tt) + MF::drow(lq, r)) * 16 + lc] = dx[tt][r];
                 }
+                // the F tiles are dead after the x pass: those of the next
+                // item's group load during its y and z passes
+                if (newgroup) load_f(nx >> 11, (nx >> 6) & 7, fa);
                 __syncthreads();
                 // y pass: B[jy][col] = X[jz*N + jy][ix] -> Yp[iy][col]
                 {
@@ -466,24 +498,41 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
 #pragma unroll
                     for (int k = 0; k < KS; k++)
 #pragma unroll
-                        for (int tt = 0; tt < TW; tt++) dy[tt] = MF::run(oy[k], buf[ybase[tt] + 64 * k], dy[tt]);
+                        for (int tt = 0; tt < TW; tt++)
+                            if (live(tt)) dy[tt] = MF::run(oy[k], buf[ybase[tt] + 64 * k], dy[tt]);
                     __syncthreads();
 #pragma unroll
                     for (int tt = 0; tt < TW; tt++)
+                        if (live(tt))
 #pragma unroll
-                        for (int r = 0; r < 4; r++) buf[MF::drow(lq, r) * CP + 16 * (wv * TW + tt) + lc] = dy[tt][r];
+                            for (int r = 0; r < 4; r++)
+                                buf[MF::drow(lq, r) * CP + 16 * (wv * TW + tt) + lc] = dy[tt][r];
                 }
                 __syncthreads();
-                // z pass: B[jz][col'] = Yp[iy][jz*N + ix], col' = iy*N + ix
+                // z pass: B[jz][col'] = Yp[iy][jz*N + ix], col' = iy*N + ix,
+                // once per passing component (item-uniform branches)
+                if (!PAIR || (t0 && t1)) {
 #pragma unroll
-                for (int k = 0; k < KS; k++)
+                    for (int k = 0; k < KS; k++)
 #pragma unroll
-                    for (int tt = 0; tt < TW; tt++) acc[tt] = MF::run(oz[k], buf[zbase[tt] + 4 * k * N], acc[tt]);
-                if (newgroup) {
+                        for (int tt = 0; tt < TW; tt++)
+                            if (live(tt)) {
+                                const T yv = buf[zbase[tt] + 4 * k * N];
+                                acc[tt] = MF::run(oz[k], yv, acc[tt]);
+                                if (PAIR) acc1[tt] = MF::run(oz1[k], yv, acc1[tt]);
+                            }
+                } else if (t0) {
 #pragma unroll
-                    for (int tt = 0; tt < TW; tt++)
+                    for (int k = 0; k < KS; k++)
 #pragma unroll
-                        for (int k = 0; k < KS; k++) fa[tt][k] = fnext[tt][k];
+                        for (int tt = 0; tt < TW; tt++)
+                            if (live(tt)) acc[tt] = MF::run(oz[k], buf[zbase[tt] + 4 * k * N], acc[tt]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < KS; k++)
+#pragma unroll
+                        for (int tt = 0; tt < TW; tt++)
+                            if (live(tt)) acc1[tt] = MF::run(oz1[k], buf[zbase[tt] + 4 * k * N], acc1[tt]);
                 }
                 it = nx;
                 stage ^= 1;
@@ -491,23 +540,26 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
             __syncthreads();
         }
     }
-    if (tid == 0 && napps) {
-        atomicAdd(&a.counters[0], napps);
-        if (cls_counter > 0) atomicAdd(&a.counters[cls_counter], napps);
+    if (tid == 0 && napps + napps1) {
+        atomicAdd(&a.counters[0], napps + napps1);
+        if (cls_counter > 0) atomicAdd(&a.counters[cls_counter], napps + napps1);
     }
     // acc[tt][r] = out[iz = drow(lq, r)][col = 16t + lc], added in FP64 to the
     // (zeroed) output: several segments and launches may share a job
-    if (napps == 0) return;
-    double *dst = a.out_ns + (long long)g * 8 * N3 + (long long)gt * N3;
+    auto store = [&](const acc_t *ac, int comp) {
+        double *dst = a.out_ns + (long long)g * 8 * N3 + (long long)comp * N3;
 #pragma unroll
-    for (int tt = 0; tt < TW; tt++) {
-        const int col = 16 * (wv * TW + tt) + lc;
+        for (int tt = 0; tt < TW; tt++) {
+            const int col = 16 * (wv * TW + tt) + lc;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int iz = MF::drow(lq, r);
-            if (col < N2 && iz < N) atomicAdd(&dst[iz * N2 + col], (double)acc[tt][r]);
+            for (int r = 0; r < 4; r++) {
+                const int iz = MF::drow(lq, r);
+                if (col < N2 && iz < N) atomicAdd(&dst[iz * N2 + col], (double)ac[tt][r]);
+            }
         }
-    }
+    };
+    if (napps) store(acc, gt);
+    if (PAIR && napps1) store(acc1, gt | 4);
 }
 
 // ------------------------------------------------------ FP16 split products
@@ -1112,18 +1164,23 @@ __device__ __forceinline__ int next_entry(unsigned long long *head, int *s_next)
 }
 
 // FP64 or FP32 per-node calculator over segments seg_list[0, n) (or segments
-// 0..n-1 without a list) in the class (lo_mult thr_cls, hi_mult thr_cls].
-template <int N, typename T>
-__global__ __launch_bounds__(JobShape<N>::T) void k_apply(KArgs a, const T *pool, const int *seg_list, int n,
-                                                          double lo_mult, double hi_mult, int cls_counter,
-                                                          unsigned long long *head) {
+// seg0 .. seg0 + n - 1 without a list) in the class (lo_mult thr_cls,
+// hi_mult thr_cls]; PAIR: the segments are pair segments.  Held to two waves
+// per SIMD: the FP64 pair jobs at N = 7, 10, 11 would otherwise take more
+// than 256 registers and run one wave per SIMD (N = 10: 8.4 ms against
+// 5.8 ms with 72 bytes per lane spilled, 116 Helmholtz terms at k = 9).
+template <int N, typename T, bool PAIR>
+__global__ __launch_bounds__(JobShape<N>::T) __attribute__((amdgpu_waves_per_eu(2))) void k_apply(
+    KArgs a, const T *pool, const int *seg_list, int seg0, int n, double lo_mult, double hi_mult, int cls_counter,
+    unsigned long long *head) {
     __shared__ ApplySmem<N, T> sm;
     extern __shared__ double nrm_s[];
     __shared__ int s_next;
     while (true) {
         const int e = next_entry(head, &s_next);
         if (e >= n) break;
-        apply_segment<N, T>(a, pool, seg_list ? seg_list[e] : e, lo_mult, hi_mult, cls_counter, sm, nrm_s);
+        apply_segment<N, T, PAIR>(a, pool, seg_list ? seg_list[e] : seg0 + e, lo_mult, hi_mult, cls_counter, sm,
+                                  nrm_s);
     }
 }
 
@@ -1223,9 +1280,9 @@ struct DeviceCtx {
 struct DevicePlan {
     DeviceCtx *ctx = nullptr;
     std::vector<const void *> op_keys;  // per part
-    // segments: n_seg single-component ones, n_pseg pair ones, n_fseg fine
+    // segments: n_pseg pair ones (indices 0 .. n_pseg - 1), then n_fseg fine
     // single-component ones
-    int n_out = 0, kp1 = 0, wmax = 0, n_seg = 0, n_pseg = 0, n_fseg = 0;
+    int n_out = 0, kp1 = 0, wmax = 0, n_pseg = 0, n_fseg = 0;
     long long npool = 0;
     double *pool = nullptr;
     float *pool32 = nullptr;
@@ -1236,10 +1293,10 @@ struct DevicePlan {
     AppDesc *d_apps = nullptr;
     std::vector<void *> bufs;
     size_t out_bytes = 0;
-    // per segment: class cut base thr_cls and the range of passing bounds
-    std::vector<double> seg_thr, seg_bmin, seg_bmax;
-    std::vector<double> pseg_thr, pseg_bmin, pseg_bmax;  // the same for pair segments
-    std::vector<double> fseg_thr, fseg_bmin, fseg_bmax;  // and for fine segments
+    // per pair segment: class cut base thr_cls and the range of passing
+    // bounds; the same for the fine segments
+    std::vector<double> pseg_thr, pseg_bmin, pseg_bmax;
+    std::vector<double> fseg_thr, fseg_bmin, fseg_bmax;
     // segment lists of the two classes, for list_tau and list_mode (the FP64
     // class lists fine segments, the low class of the split mode pair ones)
     double list_tau = -1.0;
@@ -1249,7 +1306,7 @@ struct DevicePlan {
     // passing-term count first)
     double single_ratio = 16.0;
     int order = 0;
-    std::vector<int> seg_apps, pseg_apps, fseg_apps;  // passing terms per segment
+    std::vector<int> pseg_apps, fseg_apps;  // passing terms per segment
     int *list_hi = nullptr, *list_lo = nullptr;
     int n_hi = 0, n_lo = 0;
     // post transforms in place (dev_plan_set_post): (branch, first child)
@@ -1370,26 +1427,62 @@ static void par_memcpy(void *dst, const void *src, size_t n) {
     for (auto &x : th) x.join();
 }
 
+// The context's two pinned staging buffers and their events, allocated on
+// first use; false when pinned memory is refused (the copy then goes
+// straight between the device and the pageable host memory).
+static bool stage_ready(DeviceCtx *c, std::string &err) {
+    for (int b = 0; b < 2; b++) {
+        if (!c->stage[b] && hipHostMalloc(&c->stage[b], kStageChunk, hipHostMallocDefault) != hipSuccess) {
+            c->stage[b] = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+    }
+    for (int b = 0; b < 2; b++)
+        if (!c->stage_ev[b] && hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming) != hipSuccess) {
+            c->stage_ev[b] = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+    return true;
+}
+
+// Host-to-device copies of kStageMin bytes or more go through the same
+// staging buffers: the host copy of chunk i into buffer i & 1 overlaps the
+// DMA of chunk i - 1 and waits for the DMA of chunk i - 2 (the buffer's
+// previous chunk).
+int dev_copy_h2d(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err) {
+    if (bytes == 0) return 0;
+    HIP_TRY(hipSetDevice(c->device));
+    if (bytes < kStageMin || !stage_ready(c, err)) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
+    auto len = [&](size_t i) { return std::min(kStageChunk, bytes - i * kStageChunk); };
+    for (size_t i = 0; i < nch; i++) {
+        const int b = (int)(i & 1);
+        if (i >= 2) HIP_TRY(hipEventSynchronize(c->stage_ev[b]));
+        par_memcpy(c->stage[b], static_cast<const char *>(src) + i * kStageChunk, len(i));
+        HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + i * kStageChunk, c->stage[b], len(i),
+                               hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipEventRecord(c->stage_ev[b], c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int dev_copy_d2h(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err) {
     if (bytes == 0) return 0;
     HIP_TRY(hipSetDevice(c->device));
     // without pinned staging memory (allocation refused) the copy goes
     // straight from the device into the pageable destination
-    bool staged = bytes >= kStageMin;
-    for (int b = 0; b < 2 && staged; b++) {
-        if (!c->stage[b] && hipHostMalloc(&c->stage[b], kStageChunk, hipHostMallocDefault) != hipSuccess) {
-            c->stage[b] = nullptr;
-            (void)hipGetLastError();
-            staged = false;
-        }
-    }
-    if (!staged) {
+    if (bytes < kStageMin || !stage_ready(c, err)) {
         HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         return 0;
     }
-    for (int b = 0; b < 2; b++)
-        if (!c->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming));
     const size_t nch = (bytes + kStageChunk - 1) / kStageChunk;
     auto len = [&](size_t i) { return std::min(kStageChunk, bytes - i * kStageChunk); };
     // chunk i goes through buffer i & 1; its DMA is queued only after the
@@ -1562,7 +1655,7 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     }
     // sizes and validation
     long long n_out = 0, n_band = 0, n_slot = 0, n_pool = 0, n_bw = 0, n_actoff = 0, n_act = 0;
-    long long n_seg = 0, n_pseg = 0, n_fseg = 0;
+    long long n_pseg = 0, n_fseg = 0;
     int wmax = 0;
     for (const PlanPart &pp : parts) {
         const ApplyPlan &pl = *pp.plan;
@@ -1589,11 +1682,10 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         n_bw += (long long)pl.bw.size();
         n_actoff += (long long)pl.active_off.size();
         n_act += (long long)pl.active_terms.size();
-        n_seg += (long long)pl.seg_job.size();
         n_pseg += (long long)pl.pseg_job.size();
         n_fseg += (long long)pl.fseg_job.size();
     }
-    const long long n_all = n_seg + n_pseg + n_fseg;
+    const long long n_all = n_pseg + n_fseg;
     if (n_out >= (1LL << 27) || n_all >= (1LL << 31) || n_slot >= (1LL << 31) || n_band >= (1LL << 31)) {
         err = "batch too large for 32-bit node, segment or slot indices";
         return -1;
@@ -1604,7 +1696,6 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     p->kp1 = N;
     p->wmax = wmax;
     p->npool = n_pool;
-    p->n_seg = (int)n_seg;
     p->n_pseg = (int)n_pseg;
     p->n_fseg = (int)n_fseg;
     KArgs &a = p->args;
@@ -1673,11 +1764,8 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         using T = typename std::decay_t<decltype(v)>::value_type;
         if (!v.empty()) std::memcpy(at(f) + sizeof(T) * off, v.data(), sizeof(T) * v.size());
     };
-    long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0, o_seg = 0;
-    long long o_pseg = n_seg, o_fseg = n_seg + n_pseg;
-    p->seg_thr.reserve(n_seg);
-    p->seg_bmin.reserve(n_seg);
-    p->seg_bmax.reserve(n_seg);
+    long long o_out = 0, o_band = 0, o_slot = 0, o_pool = 0, o_bw = 0, o_actoff = 0, o_act = 0;
+    long long o_pseg = 0, o_fseg = n_pseg;
     // a segment list's jobs, shifted to the batch's output numbering
     auto put_jobs = [&](long long o_dst, const std::vector<int> &jobs, const std::vector<int> &p0,
                         const std::vector<int> &p1, long long o_o) {
@@ -1751,20 +1839,12 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         }
         put(f_act, o_act, pl.active_terms);
         // segments
-        put_jobs(o_seg, pl.seg_job, pl.seg_p0, pl.seg_p1, o_out);
-        p->seg_apps.insert(p->seg_apps.end(), pl.seg_apps.begin(), pl.seg_apps.end());
         p->pseg_apps.insert(p->pseg_apps.end(), pl.pseg_apps.begin(), pl.pseg_apps.end());
         p->fseg_apps.insert(p->fseg_apps.end(), pl.fseg_apps.begin(), pl.fseg_apps.end());
-        for (size_t i = 0; i < pl.seg_job.size(); i++) {
-            const double t = pl.out_thr[pl.seg_job[i] >> 3];
-            p->seg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);  // as the kernel forms thr_cls
-            p->seg_bmin.push_back(pl.seg_bmin[i]);
-            p->seg_bmax.push_back(pl.seg_bmax[i]);
-        }
         put_jobs(o_pseg, pl.pseg_job, pl.pseg_p0, pl.pseg_p1, o_out);
         for (size_t i = 0; i < pl.pseg_job.size(); i++) {
             const double t = pl.out_thr[pl.pseg_job[i] >> 3];
-            p->pseg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);
+            p->pseg_thr.push_back(t > pl.thr_floor ? t : pl.thr_floor);  // as the kernels form thr_cls
             p->pseg_bmin.push_back(pl.pseg_bmin[i]);
             p->pseg_bmax.push_back(pl.pseg_bmax[i]);
         }
@@ -1784,7 +1864,6 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
         o_bw += (long long)pl.bw.size();
         o_actoff += (long long)pl.active_off.size();
         o_act += (long long)pl.active_terms.size();
-        o_seg += (long long)pl.seg_job.size();
     }
     if (!rc) {
         std::memcpy(at(f_apps), p->apps.data(), sizeof(AppDesc) * p->apps.size());
@@ -1906,26 +1985,31 @@ static int persistent_grid(DeviceCtx *c, const void *kernel, int threads, size_t
     return 0;
 }
 
-template <int N, typename T>
-static int launch_nt(DeviceCtx *c, const KArgs &a, const T *pool, const int *list, int n, int wmax, double lo,
-                     double hi, int cls, unsigned long long *head, hipStream_t s, std::string &err) {
+template <int N, typename T, bool PAIR>
+static int launch_nt(DeviceCtx *c, const KArgs &a, const T *pool, const int *list, int seg0, int n, int wmax,
+                     double lo, double hi, int cls, unsigned long long *head, hipStream_t s, std::string &err) {
     if (n == 0) return 0;
     const size_t shmem = nrm_bytes(wmax);
     int grid = 0;
-    int rc = persistent_grid(c, reinterpret_cast<const void *>(&k_apply<N, T>), JobShape<N>::T,
+    int rc = persistent_grid(c, reinterpret_cast<const void *>(&k_apply<N, T, PAIR>), JobShape<N>::T,
                              sizeof(ApplySmem<N, T>), shmem, n, &grid, err);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_apply<N, T>), dim3(grid), dim3(JobShape<N>::T), shmem, s, a, pool, list, n, lo, hi, cls, head);
+    hipLaunchKernelGGL((k_apply<N, T, PAIR>), dim3(grid), dim3(JobShape<N>::T), shmem, s, a, pool, list, seg0, n, lo,
+                       hi, cls, head);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
+// Persistent launch of k_apply over seg_list[0, n) or, without a list,
+// segments seg0 .. seg0 + n - 1; pair: the segments are pair segments.
 template <typename T>
-static int launch_t(const DevicePlan *p, const T *pool, const int *list, int n, double lo, double hi, int cls,
-                    unsigned long long *head, std::string &err, hipStream_t s = nullptr) {
+static int launch_t(const DevicePlan *p, const T *pool, bool pair, const int *list, int seg0, int n, double lo,
+                    double hi, int cls, unsigned long long *head, std::string &err, hipStream_t s = nullptr) {
     const KArgs &a = p->args;
     if (!s) s = p->ctx->stream;
-#define MWG_LAUNCH(NN) launch_nt<NN, T>(p->ctx, a, pool, list, n, p->wmax, lo, hi, cls, head, s, err)
+#define MWG_LAUNCH(NN)                                                                                           \
+    (pair ? launch_nt<NN, T, true>(p->ctx, a, pool, list, seg0, n, p->wmax, lo, hi, cls, head, s, err)           \
+          : launch_nt<NN, T, false>(p->ctx, a, pool, list, seg0, n, p->wmax, lo, hi, cls, head, s, err))
     switch (p->kp1) {
         case 2: return MWG_LAUNCH(2);
         case 3: return MWG_LAUNCH(3);
@@ -1953,19 +2037,15 @@ static int make_lists(DevicePlan *p, int mode, double tau, std::string &err) {
     if (p->list_tau == tau && p->list_mode == mode && p->list_order == p->order) return 0;
     std::vector<int> hi, lo;
     for (int i = 0; i < p->n_fseg; i++)
-        if (p->fseg_bmax[i] > tau * p->fseg_thr[i] * (1.0 - 1e-9)) hi.push_back(p->n_seg + p->n_pseg + i);
-    if (mode == 1)
-        for (int i = 0; i < p->n_seg; i++)
-            if (p->seg_bmin[i] <= tau * p->seg_thr[i] * (1.0 + 1e-9)) lo.push_back(i);
-    if (mode == 2)
+        if (p->fseg_bmax[i] > tau * p->fseg_thr[i] * (1.0 - 1e-9)) hi.push_back(p->n_pseg + i);
+    if (mode == 1 || mode == 2)
         for (int i = 0; i < p->n_pseg; i++)
-            if (p->pseg_bmin[i] <= tau * p->pseg_thr[i] * (1.0 + 1e-9)) lo.push_back(p->n_seg + i);
+            if (p->pseg_bmin[i] <= tau * p->pseg_thr[i] * (1.0 + 1e-9)) lo.push_back(i);
     if (p->order == 1) {
         // largest first (segment index -> its passing-term count)
         auto apps = [&](int sg) {
-            if (sg < p->n_seg) return p->seg_apps[sg];
-            if (sg < p->n_seg + p->n_pseg) return p->pseg_apps[sg - p->n_seg];
-            return p->fseg_apps[sg - p->n_seg - p->n_pseg];
+            if (sg < p->n_pseg) return p->pseg_apps[sg];
+            return p->fseg_apps[sg - p->n_pseg];
         };
         auto by_cost = [&](int x, int y) { return apps(x) > apps(y); };
         std::stable_sort(hi.begin(), hi.end(), by_cost);
@@ -2157,8 +2237,8 @@ static int enqueue_step(DevicePlan *p, int mode, double tau, bool post, bool tim
     // counters[3] (second stream)
     unsigned long long *head1 = p->args.counters + 2, *head2 = p->args.counters + 3;
     if (mode == 0) {
-        // lo = 0: every term that passes screening
-        rc = launch_t<double>(p, p->pool, nullptr, p->n_seg, 0.0, inf, 0, head1, err);
+        // lo = 0: every term that passes screening, over the pair segments
+        rc = launch_t<double>(p, p->pool, true, nullptr, 0, p->n_pseg, 0.0, inf, 0, head1, err);
         if (rc) return rc;
     } else if (mode == 1) {
         // The FP64 class runs on the second stream while the FP32 class runs
@@ -2166,11 +2246,11 @@ static int enqueue_step(DevicePlan *p, int mode, double tau, bool post, bool tim
         HIP_TRY(hipEventRecord(c->evz, c->stream));
         HIP_TRY(hipStreamWaitEvent(c->stream2, c->evz, 0));
         if (timed) HIP_TRY(hipEventRecord(c->ev2s, c->stream2));
-        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, head2, err, c->stream2);
+        rc = launch_t<double>(p, p->pool, false, p->list_hi, 0, p->n_hi, tau, inf, 0, head2, err, c->stream2);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
         if (timed) HIP_TRY(hipEventRecord(c->evl, c->stream));
-        rc = launch_t<float>(p, p->pool32, p->list_lo, p->n_lo, 0.0, tau, 1, head1, err);
+        rc = launch_t<float>(p, p->pool32, true, p->list_lo, 0, p->n_lo, 0.0, tau, 1, head1, err);
         if (rc) return rc;
         if (timed) HIP_TRY(hipEventRecord(c->evm, c->stream));
         HIP_TRY(hipStreamWaitEvent(c->stream, c->ev2e, 0));
@@ -2180,7 +2260,7 @@ static int enqueue_step(DevicePlan *p, int mode, double tau, bool post, bool tim
         HIP_TRY(hipEventRecord(c->evz, c->stream));
         HIP_TRY(hipStreamWaitEvent(c->stream2, c->evz, 0));
         if (timed) HIP_TRY(hipEventRecord(c->ev2s, c->stream2));
-        rc = launch_t<double>(p, p->pool, p->list_hi, p->n_hi, tau, inf, 0, head2, err, c->stream2);
+        rc = launch_t<double>(p, p->pool, false, p->list_hi, 0, p->n_hi, tau, inf, 0, head2, err, c->stream2);
         if (rc) return rc;
         HIP_TRY(hipEventRecord(c->ev2e, c->stream2));
         if (timed) HIP_TRY(hipEventRecord(c->evl, c->stream));

@@ -123,12 +123,19 @@ int dev_mul_nodes(DevMulSession *m, const std::vector<MulNode> &nodes, double co
                   std::string &err, bool out_on_device = false);
 void dev_mul_end(DevMulSession *m);
 
-// The pool of a plan built with device_pool (mw_multiply.hip): generated
-// nodes reconstructed from their parents and every slot's block copied from
-// the input tree's blocks on the device (inp: the input tree's coefficients,
-// 8 (k+1)^3 per node in tree order) into pool (pl.pool_values doubles, the
-// caller's); pl.in_norms receives the slots' component norms.
-int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double *pool, std::string &err);
+// The pools of plans built with device_pool (mw_multiply.hip), of every
+// job in one pass (one temporary allocation, one upload, one wait):
+// generated nodes reconstructed from their parents and every slot's block
+// copied from the input tree's blocks on the device (inp: the input tree's
+// coefficients, 8 (k+1)^3 per node in tree order) into pool
+// (pl->pool_values doubles, the caller's); pl->in_norms receives the slots'
+// component norms.
+struct PoolFill {
+    const double *inp;
+    ApplyPlan *pl;
+    double *pool;
+};
+int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &jobs, std::string &err);
 // A device copy of n doubles (freed with dev_free).
 int dev_upload(DeviceCtx *c, const double *src, size_t n, double **dst, std::string &err);
 // n doubles of device memory (freed with dev_free).
@@ -178,6 +185,9 @@ int dev_download(DeviceCtx *c, const double *src, size_t n, double *dst, std::st
 // Device-to-host copy on the context's stream, finished on return; large
 // copies go through pinned staging buffers (mw_device.hip).
 int dev_copy_d2h(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err);
+// Host-to-device copy on the context's stream, finished on return; large
+// copies go through the same pinned staging buffers.
+int dev_copy_h2d(DeviceCtx *c, void *dst, const void *src, size_t bytes, std::string &err);
 void dev_free(DeviceCtx *c, void *p);
 
 // Node-wise sums (mw_multiply.hip): for one output node and one input, the

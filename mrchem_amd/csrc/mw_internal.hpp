@@ -18,6 +18,16 @@
 
 namespace mwg {
 
+// ------------------------------------------------------------------- threads
+// Threads of the library's parallel loops: MWG_NUM_THREADS when it is a
+// positive count; else half the CPUs the process may use (affinity mask and
+// cgroup CPU quota), at least 1 and at most OpenMP's own count.  The loops
+// are short (milliseconds) and each waits for its slowest thread, so a team
+// as large as the CPU count loses whenever the GPU runtime's or the caller's
+// threads take a CPU: on a 16-CPU host a 2688-node adaptive apply took
+// 174 ms with 16 threads, 107 ms with 12 and 77 ms with 8.
+int team_size();
+
 // ---------------------------------------------------------------- math utils
 // n-point Gauss-Legendre rule mapped to [0,1] (nodes ascending, weights sum to 1).
 void gauss_legendre01(int n, double *x, double *w);
@@ -248,18 +258,14 @@ struct ApplyPlan {
     std::vector<int> bw;
     std::vector<int> active_off, active_terms;  // per depth, terms with any w >= 0
     long long n_generated = 0;
-    // Work segments: job = 8*out + gt; a segment covers band positions
-    // [seg_p0, seg_p1) of one job.  Jobs are cut so that no segment holds
-    // many more than `seg_target` term-applications (counted exactly here with
-    // the kernel's screening rule); jobs without any are left out.
-    std::vector<int> seg_job, seg_p0, seg_p1;
-    // per segment: passing terms and the smallest / largest screening bound
-    // among them (for choosing the segments of each precision class)
-    std::vector<int> seg_apps;
-    std::vector<double> seg_bmin, seg_bmax;
-    // Pair segments: the same for the output component pairs (gt, gt | 4),
-    // gt < 4 (job = 8*out + gt), cut by the passing terms of both; the FP16
-    // split kernel shares the x and y passes of a pair.
+    // Work segments of the output component pairs (gt, gt | 4), gt < 4:
+    // job = 8*out + gt; a segment covers band positions [pseg_p0, pseg_p1)
+    // of one job.  Jobs are cut so that no segment holds many more than
+    // `seg_target` term-applications of the two components (counted exactly
+    // here with the kernels' screening rule); jobs without any are left out.
+    // The kernels share the x and y passes of a pair.  Per segment: passing
+    // terms and the smallest / largest screening bound among them (for
+    // choosing the segments of each precision class).
     std::vector<int> pseg_job, pseg_p0, pseg_p1, pseg_apps;
     std::vector<double> pseg_bmin, pseg_bmax;
     // Fine segments: single-component jobs cut at about seg_target / 8

@@ -9,13 +9,51 @@
 // quadrature instead of read from filter files.
 #include "mw_internal.hpp"
 
+#include <omp.h>
+#include <sched.h>
+
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <stdexcept>
 
 namespace mwg {
+
+namespace {
+int cpus_usable() {
+    int n = 0;
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+    // cgroup v2 quota "max" or "<quota> <period>"
+    if (FILE *f = std::fopen("/sys/fs/cgroup/cpu.max", "r")) {
+        char q[32] = {0};
+        long long period = 0;
+        if (std::fscanf(f, "%31s %lld", q, &period) == 2 && period > 0 && q[0] >= '0' && q[0] <= '9') {
+            const long long quota = std::atoll(q);
+            const int qc = (int)std::max<long long>(1, quota / period);
+            n = n > 0 ? std::min(n, qc) : qc;
+        }
+        std::fclose(f);
+    }
+    return n;
+}
+}  // namespace
+
+int team_size() {
+    static const int n = [] {
+        if (const char *s = std::getenv("MWG_NUM_THREADS")) {
+            const int v = std::atoi(s);
+            if (v > 0) return v;
+        }
+        const int omp = std::max(1, omp_get_max_threads());
+        const int cpus = cpus_usable();
+        return cpus > 0 ? std::max(1, std::min(omp, cpus / 2)) : omp;
+    }();
+    return n;
+}
 
 void gauss_legendre01(int n, double *x, double *w) {
     if (n < 1) throw std::invalid_argument("gauss_legendre01: n < 1");

@@ -461,62 +461,96 @@ static hipError_t post_level(hipStream_t s, int n, bool down, const PostPair *pa
     }
 }
 
-int dev_pool_fill(DeviceCtx *c, const MRA &mra, const double *inp, ApplyPlan &pl, double *pool, std::string &err) {
+int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &jobs, std::string &err) {
+    if (jobs.empty()) return 0;
     HIP_TRY(hipSetDevice(dev_device(c)));
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     const int n = mra.kp1(), n3 = n * n * n;
-    const int ng = (int)pl.gen_parent.size(), ns = (int)pl.slot_src.size();
-    // one device allocation for the temporaries, filled by one upload:
-    // [generated scaling blocks | slot norms | filter | gen_parent | gen_child
-    //  | slot_src | slot_off]
+    const size_t nj = jobs.size();
+    // one device allocation for the temporaries of every job, filled by one
+    // upload: [filter | per job: gen_parent gen_child slot_src slot_off]
+    // [per job: generated scaling blocks] [every job's slot norms]
     size_t bytes = 0;
     auto place = [&](size_t b) {
         const size_t off = bytes;
         bytes += (std::max<size_t>(b, 8) + 255) / 256 * 256;
         return off;
     };
-    const size_t f_gen = place(sizeof(double) * (size_t)ng * n3), f_nrm = place(sizeof(double) * (size_t)ns * 8);
-    const size_t f_F = place(sizeof(double) * mra.basis.F.size()), f_gp = place(sizeof(int) * ng);
-    const size_t f_gc = place(sizeof(int) * ng), f_ss = place(sizeof(int) * ns);
-    const size_t f_so = place(sizeof(long long) * ns);
-    const size_t up0 = f_F;  // the uploaded part starts at the filter
-    std::vector<unsigned char, UninitAlloc<unsigned char>> stage(bytes - up0);
+    struct Off {
+        size_t gp, gc, ss, so, gen;
+        long long nrm;  // doubles into the norm table
+    };
+    std::vector<Off> off(nj);
+    const size_t f_F = place(sizeof(double) * mra.basis.F.size());
+    for (size_t i = 0; i < nj; i++) {
+        const ApplyPlan &pl = *jobs[i].pl;
+        const size_t ng = pl.gen_parent.size(), ns = pl.slot_src.size();
+        off[i].gp = place(sizeof(int) * ng);
+        off[i].gc = place(sizeof(int) * ng);
+        off[i].ss = place(sizeof(int) * ns);
+        off[i].so = place(sizeof(long long) * ns);
+    }
+    const size_t up_bytes = bytes;
+    long long n_norm = 0;
+    for (size_t i = 0; i < nj; i++) {
+        const ApplyPlan &pl = *jobs[i].pl;
+        off[i].gen = place(sizeof(double) * pl.gen_parent.size() * (size_t)n3);
+        off[i].nrm = n_norm;
+        n_norm += (long long)pl.slot_src.size() * 8;
+    }
+    const size_t f_nrm = place(sizeof(double) * (size_t)n_norm);
+    std::vector<unsigned char, UninitAlloc<unsigned char>> stage(up_bytes);
     auto put = [&](size_t f, const void *src, size_t b) {
-        if (b) std::memcpy(stage.data() + (f - up0), src, b);
+        if (b) std::memcpy(stage.data() + f, src, b);
     };
     put(f_F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size());
-    put(f_gp, pl.gen_parent.data(), sizeof(int) * ng);
-    put(f_gc, pl.gen_child.data(), sizeof(int) * ng);
-    put(f_ss, pl.slot_src.data(), sizeof(int) * ns);
-    put(f_so, pl.slot_off.data(), sizeof(long long) * ns);
+    for (size_t i = 0; i < nj; i++) {
+        const ApplyPlan &pl = *jobs[i].pl;
+        put(off[i].gp, pl.gen_parent.data(), sizeof(int) * pl.gen_parent.size());
+        put(off[i].gc, pl.gen_child.data(), sizeof(int) * pl.gen_child.size());
+        put(off[i].ss, pl.slot_src.data(), sizeof(int) * pl.slot_src.size());
+        put(off[i].so, pl.slot_off.data(), sizeof(long long) * pl.slot_off.size());
+    }
+    std::vector<double, UninitAlloc<double>> norms_h((size_t)n_norm);
     unsigned char *dev = nullptr;
     HIP_TRY(hipMalloc((void **)&dev, bytes));
-    double *gen_s = reinterpret_cast<double *>(dev + f_gen), *norms = reinterpret_cast<double *>(dev + f_nrm);
+    double *norms = reinterpret_cast<double *>(dev + f_nrm);
     const double *F = reinterpret_cast<const double *>(dev + f_F);
-    const int *gpar = reinterpret_cast<const int *>(dev + f_gp), *gch = reinterpret_cast<const int *>(dev + f_gc);
-    const int *ssrc = reinterpret_cast<const int *>(dev + f_ss);
-    const long long *soff = reinterpret_cast<const long long *>(dev + f_so);
-    hipError_t e = hipMemcpyAsync(dev + up0, stage.data(), bytes - up0, hipMemcpyHostToDevice, s);
-    // generated nodes depth by depth (each after its parent)
-    for (size_t lv = 0; e == hipSuccess && lv + 1 < pl.gen_level_off.size(); lv++) {
-        const int g0 = pl.gen_level_off[lv], g1 = pl.gen_level_off[lv + 1];
-        if (g1 <= g0) continue;
-        hipLaunchKernelGGL(k_gen_fill, dim3(std::min(g1 - g0, kMulBlocks)), dim3(kMulThreads),
-                           sizeof(double) * 6 * n3, s, n, F, inp, gpar, gch, g0, g1, gen_s);
-        e = hipGetLastError();
+    hipError_t e = hipMemcpyAsync(dev, stage.data(), up_bytes, hipMemcpyHostToDevice, s);
+    for (size_t i = 0; e == hipSuccess && i < nj; i++) {
+        const ApplyPlan &pl = *jobs[i].pl;
+        const int ns = (int)pl.slot_src.size();
+        double *gen_s = reinterpret_cast<double *>(dev + off[i].gen);
+        const int *gpar = reinterpret_cast<const int *>(dev + off[i].gp);
+        const int *gch = reinterpret_cast<const int *>(dev + off[i].gc);
+        const int *ssrc = reinterpret_cast<const int *>(dev + off[i].ss);
+        const long long *soff = reinterpret_cast<const long long *>(dev + off[i].so);
+        // generated nodes depth by depth (each after its parent)
+        for (size_t lv = 0; e == hipSuccess && lv + 1 < pl.gen_level_off.size(); lv++) {
+            const int g0 = pl.gen_level_off[lv], g1 = pl.gen_level_off[lv + 1];
+            if (g1 <= g0) continue;
+            hipLaunchKernelGGL(k_gen_fill, dim3(std::min(g1 - g0, kMulBlocks)), dim3(kMulThreads),
+                               sizeof(double) * 6 * n3, s, n, F, jobs[i].inp, gpar, gch, g0, g1, gen_s);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && ns > 0) {
+            hipLaunchKernelGGL(k_pool_fill, dim3(std::min(ns, 8192)), dim3(64), 0, s, n, jobs[i].inp, gen_s, ssrc,
+                               soff, ns, jobs[i].pool, norms + off[i].nrm);
+            e = hipGetLastError();
+        }
     }
-    if (e == hipSuccess && ns > 0) {
-        hipLaunchKernelGGL(k_pool_fill, dim3(std::min(ns, 8192)), dim3(64), 0, s, n, inp, gen_s, ssrc, soff, ns, pool,
-                           norms);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess && ns > 0)
-        e = hipMemcpyAsync(pl.in_norms.data(), norms, sizeof(double) * (size_t)ns * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && n_norm > 0)
+        e = hipMemcpyAsync(norms_h.data(), norms, sizeof(double) * (size_t)n_norm, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(dev);
     if (e != hipSuccess) {
         err = std::string("pool fill: ") + hipGetErrorString(e);
         return -2;
+    }
+    for (size_t i = 0; i < nj; i++) {
+        ApplyPlan &pl = *jobs[i].pl;
+        const size_t cnt = pl.slot_src.size() * 8;
+        if (cnt) std::memcpy(pl.in_norms.data(), norms_h.data() + off[i].nrm, sizeof(double) * cnt);
     }
     return 0;
 }
@@ -588,12 +622,7 @@ int dev_gather_blocks(DeviceCtx *c, const double *src, const std::vector<int> &s
 }
 
 int dev_upload_into(DeviceCtx *c, const double *src, size_t n, double *dst, std::string &err) {
-    if (n == 0) return 0;
-    HIP_TRY(hipSetDevice(dev_device(c)));
-    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
-    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    return 0;
+    return dev_copy_h2d(c, dst, src, sizeof(double) * n, err);
 }
 
 int dev_d2d(DeviceCtx *c, double *dst, const double *src, size_t n, std::string &err) {
@@ -607,18 +636,15 @@ int dev_d2d(DeviceCtx *c, double *dst, const double *src, size_t n, std::string 
 
 int dev_upload(DeviceCtx *c, const double *src, size_t n, double **dst, std::string &err) {
     HIP_TRY(hipSetDevice(dev_device(c)));
-    hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     *dst = nullptr;
     HIP_TRY(hipMalloc((void **)dst, sizeof(double) * std::max<size_t>(n, 1)));
-    hipError_t e = n ? hipMemcpyAsync(*dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s) : hipSuccess;
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
+    const int rc = dev_copy_h2d(c, *dst, src, sizeof(double) * n, err);
+    if (rc) {
         (void)hipFree(*dst);
         *dst = nullptr;
-        err = std::string("upload: ") + hipGetErrorString(e);
-        return -2;
+        err = "upload: " + err;
     }
-    return 0;
+    return rc;
 }
 
 int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, double *host_s2, double *host_w2,

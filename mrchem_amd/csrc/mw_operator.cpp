@@ -266,7 +266,7 @@ void build_operator_terms(Operator &op, const MRA &mra, const std::vector<GaussT
     }
     int nm = (int)g.size();
     int total = nm * (max_depth + 1);
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 1)
     for (int t = 0; t < total; t++) {
         int m = t / (max_depth + 1), d = t % (max_depth + 1);
         build_term_depth(mra, op.terms[m], d, tau);

@@ -267,7 +267,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         while (i0 < order.size()) {
             size_t i1 = i0;
             while (i1 < order.size() && gen_keys[order[i1]].d == gen_keys[order[i0]].d) i1++;
-#pragma omp parallel for schedule(dynamic, 8)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 8)
             for (long long ii = (long long)i0; ii < (long long)i1; ii++) {
                 const int gi = order[ii];
                 const GenKey &ch = gen_keys[gi];
@@ -280,7 +280,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
             i0 = i1;
         }
     }
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 16)
     for (int s = 0; s < ns; s++) {
         plan.in_depth[s] = slot_keys[s].d;
         double *dst = plan.in_coefs.data() + plan.slot_off[s];
@@ -353,7 +353,7 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
         srt_d[d].assign((size_t)64 * nl3 * (a1 - a0), 0.0);
         for (int e = 0; e < 64 * nl3; e++) items.emplace_back(d, e);
     }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
     for (long long it = 0; it < (long long)items.size(); it++) {
         const int d = items[it].first, e = items[it].second;
         const int w = wd[d], nl = 2 * w + 1, nl3 = nl * nl * nl;
@@ -368,7 +368,7 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
         std::sort(dst, dst + na);
     }
     long long napps = 0;
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : napps)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 4) reduction(+ : napps)
     for (int q = 0; q < nq; q++) {
         const int d = plan.out_depth[q], w = plan.band_w[q], side = 2 * w + 1;
         const int nb = side * side * side;
@@ -456,29 +456,28 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
         return nseg;
     };
     const int fine_target = std::max(1, seg_target / 8);
-    // kind 0: single-component segments, 1: fine ones, 2: pair segments
+    // kind 0: fine single-component segments, 1: pair segments
     auto cut_job = [&](int kind, int q, const SegOut *o, size_t at) {
         int n = 0;
-        for (int gt = 0; gt < (kind == 2 ? 4 : 8); gt++) {
+        for (int gt = 0; gt < (kind == 1 ? 4 : 8); gt++) {
             const int comps[2] = {gt, gt | 4};
-            n += cut(q, comps, kind == 2 ? 2 : 1, kind == 1 ? fine_target : seg_target, o, o ? at + n : 0);
+            n += cut(q, comps, kind == 1 ? 2 : 1, kind == 0 ? fine_target : seg_target, o, o ? at + n : 0);
         }
         return n;
     };
-    const SegOut outs[3] = {
-        {&plan.seg_job, &plan.seg_p0, &plan.seg_p1, &plan.seg_apps, &plan.seg_bmin, &plan.seg_bmax},
+    const SegOut outs[2] = {
         {&plan.fseg_job, &plan.fseg_p0, &plan.fseg_p1, &plan.fseg_apps, &plan.fseg_bmin, &plan.fseg_bmax},
         {&plan.pseg_job, &plan.pseg_p0, &plan.pseg_p1, &plan.pseg_apps, &plan.pseg_bmin, &plan.pseg_bmax}};
-    for (int kind = 0; kind < 3; kind++) {
+    for (int kind = 0; kind < 2; kind++) {
         std::vector<size_t> at(nq + 1, 0);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
         for (int q = 0; q < nq; q++) at[q + 1] = (size_t)cut_job(kind, q, nullptr, 0);
         for (int q = 0; q < nq; q++) at[q + 1] += at[q];
         const SegOut &o = outs[kind];
         for (auto *v : {o.job, o.p0, o.p1, o.apps}) v->resize(at[nq]);
         o.lo->resize(at[nq]);
         o.hi->resize(at[nq]);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
         for (int q = 0; q < nq; q++) cut_job(kind, q, &o, at[q]);
     }
 }
@@ -503,7 +502,7 @@ void build_prescreen(const ApplyPlan &pl, const OperFlat &f, std::vector<double>
     }
     const int S = 2 * w + 1, S3 = S * S * S, nd = d1 - d0 + 1;
     mt.assign((size_t)nd * 64 * S3, 0.0);
-#pragma omp parallel for collapse(2) schedule(dynamic)
+#pragma omp parallel for num_threads(team_size()) collapse(2) schedule(dynamic)
     for (int dd = 0; dd < nd; dd++) {
         for (int cmb = 0; cmb < 64; cmb++) {
             const int d = d0 + dd;

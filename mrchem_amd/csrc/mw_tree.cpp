@@ -289,7 +289,7 @@ void tree_compress(Tree &t) {
         if (t.child0[i] >= 0) by_depth[t.keys[i].depth].push_back(i);
     for (int d = maxd; d >= 0; d--) {
         auto &v = by_depth[d];
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
         for (int q = 0; q < (int)v.size(); q++) {
             int i = v[q];
             std::vector<double> in(blk);
@@ -423,7 +423,7 @@ void project_adaptive(Tree &t, const GaussSum &f, double prec, bool abs_prec, in
             for (int e = 0; e < d; e++) base += span[e];
             return (base + (l - lo[d])) * ng + g;
         };
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 16)
         for (long long e = 0; e < ntab; e++) {
             const int g = (int)(e % ng);
             long long r = e / ng;
@@ -432,7 +432,7 @@ void project_adaptive(Tree &t, const GaussSum &f, double prec, bool abs_prec, in
             nz[e] = project_1d(mra, f, cd, d, lo[d] + r, g, tab.data() + e * n) ? 1 : 0;
         }
         std::vector<char> do_split(work.size(), 0);
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 4)
         for (int q = 0; q < (int)work.size(); q++) {
             int i = work[q];
             const NodeKey k = t.keys[i];

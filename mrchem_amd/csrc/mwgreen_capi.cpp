@@ -242,7 +242,7 @@ const double *ensure_cnorms(mwg_tree t) {
         std::string err;
         dev_check(dev_component_norms(t->dctx->dev, T.mra->kp1(), t->dblocks, n, t->cn8.data(), err), err);
     } else {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
         for (int i = 0; i < n; i++) component_norms(*T.mra, T.node(i), t->cn8.data() + (size_t)8 * i);
     }
     t->cn8_valid = true;
@@ -303,7 +303,7 @@ void ensure_norms(mwg_tree t) {
         std::string err;
         dev_check(dev_node_norms(t->dctx->dev, T.mra->kp1(), t->dblocks, n, t->s2.data(), t->w2.data(), err), err);
     } else {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
         for (int i = 0; i < n; i++) {
             const double *b = T.node(i);
             double a = 0.0, w = 0.0;
@@ -468,7 +468,7 @@ void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<
                  double *ms3 = nullptr) {
     const int n = (int)active.size();
     const double t0 = now_ms();
-#pragma omp parallel for schedule(dynamic, 1) if (n > 1)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 1) if (n > 1)
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
         try {
@@ -489,17 +489,18 @@ void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<
         pools.base = nullptr;
         dev_check(dev_alloc(pools.c, total, &pools.base, err), err);
         pools.parts.assign(n, nullptr);
+        std::vector<PoolFill> fills(n);
         size_t off = 0;
         for (int i = 0; i < n; i++) {
             Job &j = *active[i];
             pools.parts[i] = pools.base + off;
             off += (size_t)j.plan.pool_values;
-            const double *blocks = inputs.of(ctx, j.inp_h);
-            dev_check(dev_pool_fill(ctx->dev, mra, blocks, j.plan, pools.parts[i], err), err);
+            fills[i] = PoolFill{inputs.of(ctx, j.inp_h), &j.plan, pools.parts[i]};
         }
+        dev_check(dev_pool_fill(ctx->dev, mra, fills, err), err);
     }
     const double t2 = now_ms();
-#pragma omp parallel for schedule(dynamic, 1) if (n > 1)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 1) if (n > 1)
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
         try {
@@ -641,7 +642,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
             off += active[a]->work.size();
         }
         const int na = (int)active.size();
-#pragma omp parallel for schedule(dynamic, 1) if (na > 1)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 1) if (na > 1)
         for (int a = 0; a < na; a++) {
             Job &j = *active[a];
             try {
@@ -836,7 +837,7 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
         p->slots += (double)ap.slot_off.size();
         p->gen += (double)ap.n_generated;
         p->host_apps += (double)ap.term_apps;
-        p->segs += (double)ap.seg_job.size();
+        p->segs += (double)ap.pseg_job.size();
         for (int w : ap.band_w) p->band_w = std::max(p->band_w, (double)w);
         p->outs.push_back(out[i]);
         p->ops.push_back(op[i]);
@@ -1112,7 +1113,7 @@ int mwg_operators_create_device(mwg_context ctx, mwg_mra mra, int n, const doubl
         for (int i = 0; i < n; i++) made[i].reset(new mwg_operator_s);
         // kernel fits, one per operator, in parallel
         std::vector<std::string> fit_err(n);
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 1)
         for (int i = 0; i < n; i++) {
             try {
                 prepare_operator(made[i]->op, mra->mra, mu[i], build_prec, mra->mra.max_depth);
@@ -1358,7 +1359,7 @@ int add_trees(mwg_context ctx, double prec, Tree &T, const std::vector<mwg_tree>
             std::vector<double> blocks(work.size() * (size_t)blk);
             dev_check(dev_add_nodes(m, src, blocks.data(), err), err);
             ctx->d2h_bytes += (long long)(sizeof(double) * blocks.size());
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
             for (long long w = 0; w < (long long)work.size(); w++)
                 std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
         }
@@ -1517,7 +1518,7 @@ int multiply_trees(mwg_context ctx, double prec, mwg_tree out, double coef, mwg_
                 std::vector<double> blocks(work.size() * (size_t)blk);
                 dev_check(dev_mul_nodes(m, nodes, coef, blocks.data(), err), err);
                 ctx->d2h_bytes += (long long)(sizeof(double) * blocks.size());
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for num_threads(team_size()) schedule(static)
                 for (long long w = 0; w < (long long)work.size(); w++)
                     std::copy(blocks.begin() + w * blk, blocks.begin() + (w + 1) * blk, T.node(work[w]));
             }
@@ -1979,7 +1980,7 @@ int mwg_tree_set(mwg_tree t, int n, const int *depth, const long long *l, const 
         std::vector<int> src_of;
         if (int rc = tree_from_keys(mra, n, depth, l, dst, src_of)) return rc;
         dst.coefs.resize((size_t)n * blk);  // every block is written below
-#pragma omp parallel for schedule(static) if (n > 4096)
+#pragma omp parallel for num_threads(team_size()) schedule(static) if (n > 4096)
         for (int i = 0; i < n; i++) {
             if (coefs)
                 std::memcpy(dst.node(i), coefs + (size_t)src_of[i] * blk, sizeof(double) * blk);
@@ -2461,7 +2462,7 @@ int mwg_plan_tune(mwg_plan plan, double max_rel_err, int reps, double *report, i
         double best_ms = 0.0, best_err = 0.0;
         rc = measure(best, best_ms, best_err);
         if (rc) return dev_fail(rc, err);
-        const double base_ms = best_ms;
+        const double base_ms = best_ms, base_err = best_err;
         // a starting setting outside the bound is replaced by any that meets it
         bool found = best_err <= max_rel_err;
         if (!found) best_ms = HUGE_VAL;
@@ -2511,9 +2512,9 @@ int mwg_plan_tune(mwg_plan plan, double max_rel_err, int reps, double *report, i
         plan->tau = best.tau;
         plan->post_done = false;
         plan->ctx->tuned[plan->tune_key] = best;
-        const double v[8] = {best.tau, best.single_ratio, (double)best.order, best_ms, base_ms, best_err,
-                             (double)tried, best_ms > 0.0 ? base_ms / best_ms : 0.0};
-        for (int i = 0; report && i < n && i < 8; i++) report[i] = v[i];
+        const double v[9] = {best.tau, best.single_ratio, (double)best.order, best_ms, base_ms, best_err,
+                             (double)tried, best_ms > 0.0 ? base_ms / best_ms : 0.0, base_err};
+        for (int i = 0; report && i < n && i < 9; i++) report[i] = v[i];
         return MWG_OK;
     });
 }

@@ -3,7 +3,8 @@
 For a split16 and a mixed plan: the tuner's pick keeps the largest output
 difference to an FP64 run of the same plan within the requested bound, the
 term set is unchanged, the reported time of the pick is the fastest it saw
-(never slower than the starting setting), and a new plan of the same
+among the settings within the bound (never slower than the starting setting
+when that one is within it), and a new plan of the same
 (k + 1, terms, band width, mode) starts from the cached pick.
 """
 import numpy as np
@@ -36,7 +37,8 @@ def test_tune_keeps_error_bound_and_caches(mode):
     rep = plan.tune(bound, reps=3)
     print("%s tune: %s" % (mode, rep))
     assert rep["rel_err"] <= bound and rep["tried"] >= 3
-    assert rep["ms"] <= rep["start_ms"] and rep["speedup"] >= 1.0
+    if rep["start_rel_err"] <= bound:
+        assert rep["ms"] <= rep["start_ms"] and rep["speedup"] >= 1.0
     r = plan.execute()
     assert r["term_apps"] == r64["term_apps"]
     err = np.max(np.abs(plan.download_ns() - ns64)) / scale
