@@ -236,64 +236,77 @@ __global__ __launch_bounds__(kMulThreads) void k_add_nodes(int n, const AddSrc *
     }
 }
 
-// Generated input nodes of an apply plan's pool, one depth [g0, g1): node g
-// is child gen_child[g] of gen_parent[g], a generated node of a shallower
-// depth (its scaling block, in gen_s) or, for gen_parent[g] < 0, input-tree
-// node -(gen_parent[g] + 1) (its full block: scaling and wavelet parts); its
-// scaling block is the parent's reconstruction towards that child
-// (directions z, y, x).  One workgroup per node.
-__global__ __launch_bounds__(kMulThreads) void k_gen_fill(int n, const double *F, const double *inp,
-                                                          const int *gen_parent, const int *gen_child, int g0, int g1,
-                                                          double *gen_s) {
+// One generated node of an apply plan's pool: the scaling block of child t
+// of a parent block, the parent being an input-tree node (full: its 8
+// components) or a generated node (its scaling block only).
+struct GenItem {
+    const double *src;
+    double *out;
+    int t, full;
+};
+
+// Generated nodes items[0, n_items) of one depth (their parents are input
+// nodes or generated nodes of the depth above): the z, y and x passes of
+// the two-scale filter through LDS, one workgroup per node.
+__global__ __launch_bounds__(kMulThreads) void k_gen_fill(int n, const double *F, const GenItem *items, int n_items) {
     extern __shared__ double lds[];  // 6 n^3
     const int tid = threadIdx.x, n3 = n * n * n;
     double *X = lds, *Y = lds + 4 * n3;
-    for (int g = g0 + blockIdx.x; g < g1; g += gridDim.x) {
-        const int par = gen_parent[g], t = gen_child[g];
-        double *out = gen_s + (size_t)g * n3;
-        if (par < 0) {
-            const double *src = inp + (size_t)(-par - 1) * 8 * n3;
+    for (int g = blockIdx.x; g < n_items; g += gridDim.x) {
+        const GenItem it = items[g];
+        const int t = it.t;
+        if (it.full) {
             const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
-            child_pass(n, F, src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
+            child_pass(n, F, it.src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
             __syncthreads();
             const int lo2[2] = {0, 1}, hi2[2] = {2, 3};
             child_pass(n, F, X, lo2, hi2, 2, Y, 1, (t >> 1) & 1, tid);
             __syncthreads();
             const int lo1[1] = {0}, hi1[1] = {1};
-            child_pass(n, F, Y, lo1, hi1, 1, out, 0, t & 1, tid);
+            child_pass(n, F, Y, lo1, hi1, 1, it.out, 0, t & 1, tid);
         } else {
-            const double *src = gen_s + (size_t)par * n3;
             const int lo[1] = {0}, none[1] = {-1};
-            child_pass(n, F, src, lo, none, 1, X, 2, (t >> 2) & 1, tid);
+            child_pass(n, F, it.src, lo, none, 1, X, 2, (t >> 2) & 1, tid);
             __syncthreads();
             child_pass(n, F, X, lo, none, 1, Y, 1, (t >> 1) & 1, tid);
             __syncthreads();
-            child_pass(n, F, Y, lo, none, 1, out, 0, t & 1, tid);
+            child_pass(n, F, Y, lo, none, 1, it.out, 0, t & 1, tid);
         }
         __syncthreads();
     }
 }
 
-// The pool of an apply plan: slot s gets input-tree node slot_src[s]'s full
-// block, or generated node -(slot_src[s] + 1)'s scaling block, at
-// slot_off[s], and its 8 component norms (wavelet norms 0 for a generated
-// slot), each summed by one lane in index order.  One wave per slot.
-__global__ __launch_bounds__(64) void k_pool_fill(int n, const double *inp, const double *gen_s, const int *slot_src,
-                                                  const long long *slot_off, int n_slots, double *pool, double *norms) {
-    const int lane = threadIdx.x, n3 = n * n * n;
-    for (int s = blockIdx.x; s < n_slots; s += gridDim.x) {
-        const int src = slot_src[s];
-        double *dst = pool + slot_off[s];
-        const int len = src >= 0 ? 8 * n3 : n3;
-        const double *from = src >= 0 ? inp + (size_t)src * 8 * n3 : gen_s + (size_t)(-src - 1) * n3;
-        for (int e = lane; e < len; e += 64) dst[e] = from[e];
-        if (lane < 8) {
-            double q = 0.0;
-            if (lane * n3 < len)
-                for (int e = 0; e < n3; e++) q += from[lane * n3 + e] * from[lane * n3 + e];
-            norms[(size_t)s * 8 + lane] = sqrt(q);
-        }
+// One slot of an apply plan's pool: len doubles from src (an input node's
+// full block, or a generated node's scaling block) to dst, and the slot's 8
+// component norms to norms (wavelet norms 0 for a generated slot).
+struct SlotItem {
+    const double *src;
+    double *dst, *norms;
+    int len;
+};
+
+// The copies: one wave per slot.
+__global__ __launch_bounds__(64) void k_pool_fill(const SlotItem *items, int n_items) {
+    const int lane = threadIdx.x;
+    for (int s = blockIdx.x; s < n_items; s += gridDim.x) {
+        const SlotItem it = items[s];
+        for (int e = lane; e < it.len; e += 64) it.dst[e] = it.src[e];
     }
+}
+
+// The norms: one lane per (slot, component), each summed in index order.
+__global__ __launch_bounds__(256) void k_slot_norms(int n, const SlotItem *items, int n_items) {
+    const int n3 = n * n * n;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= 8LL * n_items) return;
+    const int s = (int)(i >> 3), comp = (int)(i & 7);
+    const SlotItem it = items[s];
+    double q = 0.0;
+    if (comp * n3 < it.len) {
+        const double *x = it.src + (size_t)comp * n3;
+        for (int e = 0; e < n3; e++) q += x[e] * x[e];
+    }
+    it.norms[comp] = sqrt(q);
 }
 
 // Squared norms of each node's scaling part and wavelet parts: one wave
@@ -467,80 +480,102 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     const int n = mra.kp1(), n3 = n * n * n;
     const size_t nj = jobs.size();
-    // one device allocation for the temporaries of every job, filled by one
-    // upload: [filter | per job: gen_parent gen_child slot_src slot_off]
-    // [per job: generated scaling blocks] [every job's slot norms]
+    // generated nodes of all jobs by depth (a job's depth lv after its depth
+    // lv - 1), then every job's slots
+    size_t nlv = 0;
+    std::vector<size_t> gen_at(nj + 1, 0), slot_at(nj + 1, 0);
+    for (size_t i = 0; i < nj; i++) {
+        const ApplyPlan &pl = *jobs[i].pl;
+        nlv = std::max(nlv, pl.gen_level_off.empty() ? (size_t)0 : pl.gen_level_off.size() - 1);
+        gen_at[i + 1] = gen_at[i] + pl.gen_parent.size();
+        slot_at[i + 1] = slot_at[i] + pl.slot_src.size();
+    }
+    const size_t n_gen = gen_at[nj], n_slot = slot_at[nj];
+    if (n_gen >= (1u << 31) || n_slot >= (1u << 31)) {
+        err = "pool fill: too many slots";
+        return -1;
+    }
+    std::vector<size_t> lv_at(nlv + 1, 0);  // items of depth lv: [lv_at[lv], lv_at[lv + 1])
+    for (size_t lv = 0; lv < nlv; lv++) {
+        size_t k = 0;
+        for (size_t i = 0; i < nj; i++) {
+            const std::vector<int> &go = jobs[i].pl->gen_level_off;
+            if (lv + 1 < go.size()) k += (size_t)(go[lv + 1] - go[lv]);
+        }
+        lv_at[lv + 1] = lv_at[lv] + k;
+    }
+    // one device allocation: [filter | gen items | slot items] (uploaded)
+    // [generated scaling blocks of every job] [every slot's norms]
     size_t bytes = 0;
     auto place = [&](size_t b) {
         const size_t off = bytes;
         bytes += (std::max<size_t>(b, 8) + 255) / 256 * 256;
         return off;
     };
-    struct Off {
-        size_t gp, gc, ss, so, gen;
-        long long nrm;  // doubles into the norm table
-    };
-    std::vector<Off> off(nj);
     const size_t f_F = place(sizeof(double) * mra.basis.F.size());
-    for (size_t i = 0; i < nj; i++) {
-        const ApplyPlan &pl = *jobs[i].pl;
-        const size_t ng = pl.gen_parent.size(), ns = pl.slot_src.size();
-        off[i].gp = place(sizeof(int) * ng);
-        off[i].gc = place(sizeof(int) * ng);
-        off[i].ss = place(sizeof(int) * ns);
-        off[i].so = place(sizeof(long long) * ns);
-    }
+    const size_t f_gi = place(sizeof(GenItem) * n_gen), f_si = place(sizeof(SlotItem) * n_slot);
     const size_t up_bytes = bytes;
-    long long n_norm = 0;
-    for (size_t i = 0; i < nj; i++) {
-        const ApplyPlan &pl = *jobs[i].pl;
-        off[i].gen = place(sizeof(double) * pl.gen_parent.size() * (size_t)n3);
-        off[i].nrm = n_norm;
-        n_norm += (long long)pl.slot_src.size() * 8;
-    }
-    const size_t f_nrm = place(sizeof(double) * (size_t)n_norm);
-    std::vector<unsigned char, UninitAlloc<unsigned char>> stage(up_bytes);
-    auto put = [&](size_t f, const void *src, size_t b) {
-        if (b) std::memcpy(stage.data() + f, src, b);
-    };
-    put(f_F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size());
-    for (size_t i = 0; i < nj; i++) {
-        const ApplyPlan &pl = *jobs[i].pl;
-        put(off[i].gp, pl.gen_parent.data(), sizeof(int) * pl.gen_parent.size());
-        put(off[i].gc, pl.gen_child.data(), sizeof(int) * pl.gen_child.size());
-        put(off[i].ss, pl.slot_src.data(), sizeof(int) * pl.slot_src.size());
-        put(off[i].so, pl.slot_off.data(), sizeof(long long) * pl.slot_off.size());
-    }
-    std::vector<double, UninitAlloc<double>> norms_h((size_t)n_norm);
+    const size_t f_gen = place(sizeof(double) * n_gen * (size_t)n3), f_nrm = place(sizeof(double) * 8 * n_slot);
     unsigned char *dev = nullptr;
     HIP_TRY(hipMalloc((void **)&dev, bytes));
-    double *norms = reinterpret_cast<double *>(dev + f_nrm);
-    const double *F = reinterpret_cast<const double *>(dev + f_F);
-    hipError_t e = hipMemcpyAsync(dev, stage.data(), up_bytes, hipMemcpyHostToDevice, s);
-    for (size_t i = 0; e == hipSuccess && i < nj; i++) {
+    double *gen_s = reinterpret_cast<double *>(dev + f_gen), *norms = reinterpret_cast<double *>(dev + f_nrm);
+    std::vector<unsigned char, UninitAlloc<unsigned char>> stage(up_bytes);
+    std::memcpy(stage.data() + f_F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size());
+    GenItem *gi = reinterpret_cast<GenItem *>(stage.data() + f_gi);
+    SlotItem *si = reinterpret_cast<SlotItem *>(stage.data() + f_si);
+    // job i's generated node g is gen_s block gen_at[i] + g
+    std::vector<size_t> lv_fill(lv_at.begin(), lv_at.end() - (nlv ? 1 : 0));
+    for (size_t i = 0; i < nj; i++) {
         const ApplyPlan &pl = *jobs[i].pl;
-        const int ns = (int)pl.slot_src.size();
-        double *gen_s = reinterpret_cast<double *>(dev + off[i].gen);
-        const int *gpar = reinterpret_cast<const int *>(dev + off[i].gp);
-        const int *gch = reinterpret_cast<const int *>(dev + off[i].gc);
-        const int *ssrc = reinterpret_cast<const int *>(dev + off[i].ss);
-        const long long *soff = reinterpret_cast<const long long *>(dev + off[i].so);
-        // generated nodes depth by depth (each after its parent)
-        for (size_t lv = 0; e == hipSuccess && lv + 1 < pl.gen_level_off.size(); lv++) {
-            const int g0 = pl.gen_level_off[lv], g1 = pl.gen_level_off[lv + 1];
-            if (g1 <= g0) continue;
-            hipLaunchKernelGGL(k_gen_fill, dim3(std::min(g1 - g0, kMulBlocks)), dim3(kMulThreads),
-                               sizeof(double) * 6 * n3, s, n, F, jobs[i].inp, gpar, gch, g0, g1, gen_s);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess && ns > 0) {
-            hipLaunchKernelGGL(k_pool_fill, dim3(std::min(ns, 8192)), dim3(64), 0, s, n, jobs[i].inp, gen_s, ssrc,
-                               soff, ns, jobs[i].pool, norms + off[i].nrm);
-            e = hipGetLastError();
+        const double *inp = jobs[i].inp;
+        double *gs = gen_s + gen_at[i] * (size_t)n3;
+        for (size_t lv = 0; lv + 1 < pl.gen_level_off.size(); lv++)
+            for (int g = pl.gen_level_off[lv]; g < pl.gen_level_off[lv + 1]; g++) {
+                const int par = pl.gen_parent[g];
+                GenItem &it = gi[lv_fill[lv]++];
+                it.src = par < 0 ? inp + (size_t)(-par - 1) * 8 * n3 : gs + (size_t)par * n3;
+                it.out = gs + (size_t)g * n3;
+                it.t = pl.gen_child[g];
+                it.full = par < 0;
+            }
+    }
+    for (size_t i = 0; i < nj; i++) {
+        const ApplyPlan &pl = *jobs[i].pl;
+        const double *inp = jobs[i].inp;
+        const double *gs = gen_s + gen_at[i] * (size_t)n3;
+        for (size_t q = 0; q < pl.slot_src.size(); q++) {
+            const int sr = pl.slot_src[q];
+            SlotItem &it = si[slot_at[i] + q];
+            it.src = sr >= 0 ? inp + (size_t)sr * 8 * n3 : gs + (size_t)(-sr - 1) * n3;
+            it.dst = jobs[i].pool + pl.slot_off[q];
+            it.norms = norms + (slot_at[i] + q) * 8;
+            it.len = sr >= 0 ? 8 * n3 : n3;
         }
     }
-    if (e == hipSuccess && n_norm > 0)
-        e = hipMemcpyAsync(norms_h.data(), norms, sizeof(double) * (size_t)n_norm, hipMemcpyDeviceToHost, s);
+    std::vector<double, UninitAlloc<double>> norms_h(8 * n_slot);
+    const double *F = reinterpret_cast<const double *>(dev + f_F);
+    const GenItem *dgi = reinterpret_cast<const GenItem *>(dev + f_gi);
+    const SlotItem *dsi = reinterpret_cast<const SlotItem *>(dev + f_si);
+    hipError_t e = hipMemcpyAsync(dev, stage.data(), up_bytes, hipMemcpyHostToDevice, s);
+    for (size_t lv = 0; e == hipSuccess && lv < nlv; lv++) {
+        const int cnt = (int)(lv_at[lv + 1] - lv_at[lv]);
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_gen_fill, dim3(std::min(cnt, kMulBlocks)), dim3(kMulThreads), sizeof(double) * 6 * n3, s,
+                           n, F, dgi + lv_at[lv], cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && n_slot > 0) {
+        hipLaunchKernelGGL(k_pool_fill, dim3((unsigned)std::min<size_t>(n_slot, 16384)), dim3(64), 0, s, dsi,
+                           (int)n_slot);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && n_slot > 0) {
+        hipLaunchKernelGGL(k_slot_norms, dim3((unsigned)((8 * n_slot + 255) / 256)), dim3(256), 0, s, n, dsi,
+                           (int)n_slot);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && n_slot > 0)
+        e = hipMemcpyAsync(norms_h.data(), norms, sizeof(double) * 8 * n_slot, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(dev);
     if (e != hipSuccess) {
@@ -550,7 +585,7 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
     for (size_t i = 0; i < nj; i++) {
         ApplyPlan &pl = *jobs[i].pl;
         const size_t cnt = pl.slot_src.size() * 8;
-        if (cnt) std::memcpy(pl.in_norms.data(), norms_h.data() + off[i].nrm, sizeof(double) * cnt);
+        if (cnt) std::memcpy(pl.in_norms.data(), norms_h.data() + slot_at[i] * 8, sizeof(double) * cnt);
     }
     return 0;
 }

@@ -414,6 +414,11 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
         }
     }
     plan.term_apps = napps;
+    // A small plan (an adaptive apply's level) is cut finer, into about 2048
+    // segments of at least 32 terms, so that its launch still has work for
+    // every compute unit's workgroups; plans with 2048 x seg_target passing
+    // terms or more keep seg_target.
+    seg_target = (int)std::min<long long>(seg_target, std::max<long long>(32, napps / 2048));
     // Cut the band positions of one job (components gts[0..ng)) into
     // segments of about `target` passing terms; with emit false only counts
     // them.  Jobs are cut in parallel, each into its place in job order.
