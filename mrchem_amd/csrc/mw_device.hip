@@ -72,7 +72,8 @@ struct KArgs {
     const AppDesc *apps;
     // [0] term-applications passing screening (algorithmic work)
     unsigned long long *counters;
-    // work segments: job = 8*out + gt, band positions [p0, p1)
+    // work segments: job = 8*out + gt, (band position p, input component ft)
+    // pairs 8 p + ft in [p0, p1)
     const int *seg_job, *seg_p0, *seg_p1;
 };
 
@@ -327,8 +328,9 @@ struct ApplySmem {
     long long btab[64];
 };
 
-// One work segment: the terms of output job seg_job[seg] over band positions
-// [seg_p0, seg_p1) in the launch's class, added into the FP64 output.  nrm_s
+// One work segment: the terms of output job seg_job[seg] over the (band
+// position, input component) pairs [seg_p0, seg_p1) in the launch's class,
+// added into the FP64 output.  nrm_s
 // is the norm table [term][c][l+W], 64 x nrm_stride(2 Wmax + 1) doubles.
 // With PAIR the segment is a pair segment: its job stands for the output
 // components gt and gt | 4 (gt < 4), whose operators differ only along z.
@@ -354,7 +356,7 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
     const int lc = lane & 15, lq = lane >> 4;
     for (int e = tid; e < BUF; e += NTH) buf[e] = T(0);
 
-    const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
+    const int pi_begin = a.seg_p0[seg], pi_end = a.seg_p1[seg];
     const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
     const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
     const int g = j.g, gt = j.gt, w = j.w, side = j.side;
@@ -426,10 +428,10 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
         __syncthreads();
         stage_terms(a, ad, j, base, cnt, ltab, btab, nrm_s, tid, NTH);
         __syncthreads();
-        int pi = 8 * p_begin;
-        while (pi < 8 * p_end) {
+        int pi = pi_begin;
+        while (pi < pi_end) {
             // screening into the worklist (wave 0 writes the items)
-            const int n = screen_scan<PAIR>(a, ad, j, pi, 8 * p_end, cnt, nrm_s, kWorkCap, lane, napps, napps1,
+            const int n = screen_scan<PAIR>(a, ad, j, pi, pi_end, cnt, nrm_s, kWorkCap, lane, napps, napps1,
                                             [&](int pos, int p, int ft, int, int, int, int, double, double, bool t0,
                                                 bool t1) {
                                                 if (wv == 0)
@@ -777,7 +779,7 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
 
-    const int p_begin = a.seg_p0[seg], p_end = a.seg_p1[seg];
+    const int pi_begin = a.seg_p0[seg], pi_end = a.seg_p1[seg];
     const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
     const JobCtx j = job_ctx(a, ad, a.seg_job[seg], lo_mult, hi_mult);
     const double thr_single = (j.thr > ad.thr_floor ? j.thr : ad.thr_floor) * single_mult;
@@ -982,12 +984,12 @@ __device__ __forceinline__ void split_segment(const KArgs &a, const _Float16 *po
         // this lane's term: operator block base and half-width
         const int my_L = ltab[lane];
         const int my_b0 = (int)(btab[lane] / N2) + my_L;
-        int pi = 8 * p_begin;
-        while (pi < 8 * p_end) {
+        int pi = pi_begin;
+        while (pi < pi_end) {
             // screening into SplitItems: the lane-term also resolves the
             // item's three operator block ids; wave 0 writes the items
             const int n = screen_scan<true>(
-                a, ad, j, pi, 8 * p_end, cnt, nrm_s, kSplitCap, lane, napps0, napps1,
+                a, ad, j, pi, pi_end, cnt, nrm_s, kSplitCap, lane, napps0, napps1,
                 [&](int pos, int p, int ft, int slot, int lx, int ly, int lz, double b0, double b1, bool t0, bool t1) {
                     if (wv != 0) return;
                     // a passing term has |l| <= L in every direction, so all

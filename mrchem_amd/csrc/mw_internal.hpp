@@ -305,7 +305,12 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
                 const std::vector<int> &nodes, const std::vector<double> &thr,
                 int max_band_w, int seg_target = 256, bool device_pool = false,
                 const double *in_cnorm8 = nullptr);
-void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target = 256);
+// fine_segments: also cut the fine single-component segments (the FP64 class
+// of the mixed and split modes; plain FP64 runs over the pair segments only).
+// n_share: the number of plans of about this size launched together (the
+// applies of a batch level), which decides whether the launch is small.
+void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target = 256, bool fine_segments = true,
+                        int n_share = 1);
 double oper_norm(const Operator &op, int m, int d, int c, long long l);
 
 // Operator tables flattened for the device: per (m, d) the half-width L, the

@@ -16,6 +16,8 @@
 // of the term at that depth and Fmax bounds every input component norm there.
 #include "mw_internal.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -297,7 +299,7 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
     finish_plan_counts(plan, op, seg_target);
 }
 
-void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
+void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target, bool fine_segments, int n_share) {
     const int M = op.size();
     const std::vector<int> &nodes = plan.out_node;
 
@@ -317,8 +319,9 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
         const int side = 2 * plan.band_w[q] + 1;
         cof[q + 1] = cof[q] + 8LL * side * side * side;
     }
-    std::vector<int> counts(cof[nq]);
-    std::vector<double> bmin(cof[nq]), bmax(cof[nq]);
+    // every entry is written by the count loop below
+    std::vector<int, UninitAlloc<int>> counts(cof[nq]);
+    std::vector<double, UninitAlloc<double>> bmin(cof[nq]), bmax(cof[nq]);
     // Per output depth (band half-width w): the norm the kernel would use per
     // active term, component and translation (tn), and for each component
     // combination cmb = cx | cy << 2 | cz << 4 and translation the products
@@ -367,123 +370,198 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target) {
             dst[ai] = (tn[(ai * 4 + cx) * nl + lx] * tn[(ai * 4 + cy) * nl + ly]) * tn[(ai * 4 + cz) * nl + lz];
         std::sort(dst, dst + na);
     }
+    // Passing terms of output component gt of a job (its view jv) from input
+    // component ft of band position p, and the smallest / largest passing
+    // bound.
+    struct JobView {
+        int d, w, side, nl;
+        size_t nl3, na;
+        double th;
+        const int *band;
+        const double *srt;
+    };
+    auto view = [&](int q) {
+        JobView v;
+        v.d = plan.out_depth[q];
+        v.w = plan.band_w[q];
+        v.side = 2 * v.w + 1;
+        v.nl = v.side;
+        v.nl3 = (size_t)v.nl * v.nl * v.nl;
+        v.na = (size_t)(plan.active_off[v.d + 1] - plan.active_off[v.d]);
+        v.th = plan.out_thr[q];
+        v.band = plan.band.data() + plan.band_off[q];
+        v.srt = srt_d[v.d].data();
+        return v;
+    };
+    const double *in_norms = plan.in_norms.data();
+    auto pass_range = [in_norms](const JobView &v, int gt, int p, int ft, int &cnt, double &blo, double &bhi) {
+        cnt = 0;
+        const int sl = v.band[p];
+        if (sl < 0 || (v.d > 0 && ft == 0 && gt == 0)) return;
+        const double fn = in_norms[(size_t)sl * 8 + ft];
+        if (!(fn > 0.0) || v.na == 0) return;
+        const int w = v.w, nl = v.nl, side = v.side;
+        const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+        int cxyz[3];
+        for (int dim = 0; dim < 3; dim++) cxyz[dim] = (((gt >> dim) & 1) << 1) | ((ft >> dim) & 1);
+        const int cmb = cxyz[0] | (cxyz[1] << 2) | (cxyz[2] << 4);
+        const size_t na = v.na;
+        const double *b = v.srt + (cmb * v.nl3 + ((size_t)(ll[2] + w) * nl + (ll[1] + w)) * nl + (ll[0] + w)) * na;
+        if (!(b[na - 1] * fn > v.th)) return;
+        // the first product that passes
+        size_t lo = 0, hi = na - 1;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            if (b[mid] * fn > v.th)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        cnt = (int)(na - lo);
+        blo = b[lo] * fn;
+        bhi = b[na - 1] * fn;
+    };
     long long napps = 0;
 #pragma omp parallel for num_threads(team_size()) schedule(dynamic, 4) reduction(+ : napps)
     for (int q = 0; q < nq; q++) {
-        const int d = plan.out_depth[q], w = plan.band_w[q], side = 2 * w + 1;
-        const int nb = side * side * side;
-        const double th = plan.out_thr[q];
-        const int a0 = plan.active_off[d], a1 = plan.active_off[d + 1];
-        const int nl = 2 * w + 1;
-        const double *srt = srt_d[d].data();
-        const size_t nl3 = (size_t)nl * nl * nl, na = (size_t)(a1 - a0);
+        const JobView v = view(q);
+        const int nb = v.side * v.side * v.side;
         for (int gt = 0; gt < 8; gt++) {
             int *cp = counts.data() + cof[q] + (long long)gt * nb;
             double *bl = bmin.data() + cof[q] + (long long)gt * nb, *bh = bmax.data() + cof[q] + (long long)gt * nb;
-            std::fill_n(cp, nb, 0);
-            std::fill_n(bl, nb, HUGE_VAL);
-            std::fill_n(bh, nb, 0.0);
             for (int p = 0; p < nb; p++) {
-                const int sl = plan.band[plan.band_off[q] + p];
-                if (sl < 0) continue;
-                const int ll[3] = {p % side - w, (p / side) % side - w, p / (side * side) - w};
+                cp[p] = 0;
+                bl[p] = HUGE_VAL;
+                bh[p] = 0.0;
+                if (v.band[p] < 0) continue;
                 for (int ft = 0; ft < 8; ft++) {
-                    if (d > 0 && ft == 0 && gt == 0) continue;
-                    const double fn = plan.in_norms[(size_t)sl * 8 + ft];
-                    if (!(fn > 0.0)) continue;
-                    int cxyz[3];
-                    for (int dim = 0; dim < 3; dim++) cxyz[dim] = (((gt >> dim) & 1) << 1) | ((ft >> dim) & 1);
-                    const int cmb = cxyz[0] | (cxyz[1] << 2) | (cxyz[2] << 4);
-                    const double *b = srt + (cmb * nl3 + ((size_t)(ll[2] + w) * nl + (ll[1] + w)) * nl + (ll[0] + w)) * na;
-                    if (na == 0 || !(b[na - 1] * fn > th)) continue;
-                    // the first product that passes
-                    size_t lo = 0, hi = na - 1;
-                    while (lo < hi) {
-                        const size_t mid = (lo + hi) / 2;
-                        if (b[mid] * fn > th)
-                            hi = mid;
-                        else
-                            lo = mid + 1;
-                    }
-                    cp[p] += (int)(na - lo);
-                    bl[p] = std::min(bl[p], b[lo] * fn);
-                    bh[p] = std::max(bh[p], b[na - 1] * fn);
+                    int c = 0;
+                    double l = 0.0, h = 0.0;
+                    pass_range(v, gt, p, ft, c, l, h);
+                    if (c == 0) continue;
+                    cp[p] += c;
+                    bl[p] = std::min(bl[p], l);
+                    bh[p] = std::max(bh[p], h);
                 }
                 napps += cp[p];
             }
         }
     }
     plan.term_apps = napps;
-    // A small plan (an adaptive apply's level) is cut finer, into about 2048
-    // segments of at least 32 terms, so that its launch still has work for
-    // every compute unit's workgroups; plans with 2048 x seg_target passing
-    // terms or more keep seg_target.
-    seg_target = (int)std::min<long long>(seg_target, std::max<long long>(32, napps / 2048));
-    // Cut the band positions of one job (components gts[0..ng)) into
-    // segments of about `target` passing terms; with emit false only counts
-    // them.  Jobs are cut in parallel, each into its place in job order.
+    // A small launch (an adaptive apply's level) is cut finer, into about
+    // 2048 segments of at least 32 terms, so that it still has work for every
+    // compute unit's workgroups; launches with 2048 x seg_target passing terms
+    // or more (this plan's count times n_share) keep seg_target.
+    const int target_in = seg_target;
+    const long long launch_apps = napps * std::max(1, n_share);
+    seg_target = (int)std::min<long long>(seg_target, std::max<long long>(32, launch_apps / 2048));
+    // In such a small plan a band position holding more passing terms than a
+    // segment is cut inside, at input-component boundaries (an adaptive
+    // apply's root level: a few output nodes with every term).
+    const bool split_positions = seg_target < target_in;
+    // Cut the (band position p, input component ft) pairs u = 8 p + ft of one
+    // job (components gts[0..ng)) into segments of about `target` passing
+    // terms, appended to out.  Jobs are cut in parallel, each thread over a
+    // contiguous range of jobs into a buffer of its own, so that the buffers
+    // in thread order hold the segments in job order.
+    struct Seg {
+        int job, p0, p1, apps;
+        double lo, hi;
+    };
+    auto cut = [&](int q, const int *gts, int ng, int target, std::vector<Seg> &out) {
+        const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
+        int u0 = 0, acc = 0;
+        double lo = HUGE_VAL, hi = 0.0;
+        // close the segment [u0, u1); segments without passing terms are
+        // dropped (the output is zeroed before the launches)
+        auto close = [&](int u1) {
+            if (acc > 0) out.push_back(Seg{q * 8 + gts[0], u0, u1, acc, lo, hi});
+            u0 = u1;
+            acc = 0;
+            lo = HUGE_VAL;
+            hi = 0.0;
+        };
+        JobView v{};
+        bool have_view = false;
+        for (int p = 0; p < nb; p++) {
+            int pc = 0;
+            double pl = HUGE_VAL, ph = 0.0;
+            for (int e = 0; e < ng; e++) {
+                const long long jq = cof[q] + (long long)gts[e] * nb + p;
+                pc += counts[jq];
+                pl = std::min(pl, bmin[jq]);
+                ph = std::max(ph, bmax[jq]);
+            }
+            if (split_positions && pc > target) {
+                if (!have_view) {
+                    v = view(q);
+                    have_view = true;
+                }
+                for (int ft = 0; ft < 8; ft++) {
+                    for (int e = 0; e < ng; e++) {
+                        int c = 0;
+                        double l = 0.0, h = 0.0;
+                        pass_range(v, gts[e], p, ft, c, l, h);
+                        if (c == 0) continue;
+                        acc += c;
+                        lo = std::min(lo, l);
+                        hi = std::max(hi, h);
+                    }
+                    if (acc >= target) close(8 * p + ft + 1);
+                }
+            } else {
+                acc += pc;
+                lo = std::min(lo, pl);
+                hi = std::max(hi, ph);
+                if (acc >= target) close(8 * (p + 1));
+            }
+        }
+        close(8 * nb);
+    };
+    const int fine_target = std::max(1, seg_target / 8);
     struct SegOut {
         std::vector<int> *job, *p0, *p1, *apps;
         std::vector<double> *lo, *hi;
     };
-    auto cut = [&](int q, const int *gts, int ng, int target, const SegOut *o, size_t at) {
-        const int w = plan.band_w[q], side = 2 * w + 1, nb = side * side * side;
-        int p0 = 0, acc = 0, nseg = 0;
-        double lo = HUGE_VAL, hi = 0.0;
-        for (int p = 0; p < nb; p++) {
-            for (int e = 0; e < ng; e++) {
-                const long long jq = cof[q] + (long long)gts[e] * nb + p;
-                acc += counts[jq];
-                lo = std::min(lo, bmin[jq]);
-                hi = std::max(hi, bmax[jq]);
-            }
-            if ((acc >= target && p + 1 < nb) || p + 1 == nb) {
-                // segments without passing terms are dropped: the output
-                // is zeroed before the launches
-                if (acc > 0) {
-                    if (o) {
-                        const size_t k = at + nseg;
-                        (*o->job)[k] = q * 8 + gts[0];
-                        (*o->p0)[k] = p0;
-                        (*o->p1)[k] = p + 1;
-                        (*o->apps)[k] = acc;
-                        (*o->lo)[k] = lo;
-                        (*o->hi)[k] = hi;
-                    }
-                    nseg++;
-                }
-                p0 = p + 1;
-                acc = 0;
-                lo = HUGE_VAL;
-                hi = 0.0;
-            }
-        }
-        return nseg;
-    };
-    const int fine_target = std::max(1, seg_target / 8);
     // kind 0: fine single-component segments, 1: pair segments
-    auto cut_job = [&](int kind, int q, const SegOut *o, size_t at) {
-        int n = 0;
-        for (int gt = 0; gt < (kind == 1 ? 4 : 8); gt++) {
-            const int comps[2] = {gt, gt | 4};
-            n += cut(q, comps, kind == 1 ? 2 : 1, kind == 0 ? fine_target : seg_target, o, o ? at + n : 0);
-        }
-        return n;
-    };
     const SegOut outs[2] = {
         {&plan.fseg_job, &plan.fseg_p0, &plan.fseg_p1, &plan.fseg_apps, &plan.fseg_bmin, &plan.fseg_bmax},
         {&plan.pseg_job, &plan.pseg_p0, &plan.pseg_p1, &plan.pseg_apps, &plan.pseg_bmin, &plan.pseg_bmax}};
-    for (int kind = 0; kind < 2; kind++) {
-        std::vector<size_t> at(nq + 1, 0);
-#pragma omp parallel for num_threads(team_size()) schedule(static)
-        for (int q = 0; q < nq; q++) at[q + 1] = (size_t)cut_job(kind, q, nullptr, 0);
-        for (int q = 0; q < nq; q++) at[q + 1] += at[q];
+    const int nt = team_size();
+    std::vector<std::vector<Seg>> buf(nt);
+    for (int kind = fine_segments ? 0 : 1; kind < 2; kind++) {
+        int used = 1;
+#pragma omp parallel num_threads(nt)
+        {
+            const int t = omp_get_thread_num(), T = omp_get_num_threads();
+#pragma omp single
+            used = T;
+            std::vector<Seg> &mine = buf[t];
+            mine.clear();
+            const int q0 = (int)((long long)nq * t / T), q1 = (int)((long long)nq * (t + 1) / T);
+            for (int q = q0; q < q1; q++)
+                for (int gt = 0; gt < (kind == 1 ? 4 : 8); gt++) {
+                    const int comps[2] = {gt, gt | 4};
+                    cut(q, comps, kind == 1 ? 2 : 1, kind == 0 ? fine_target : seg_target, mine);
+                }
+        }
+        size_t total = 0;
+        for (int t = 0; t < used; t++) total += buf[t].size();
         const SegOut &o = outs[kind];
-        for (auto *v : {o.job, o.p0, o.p1, o.apps}) v->resize(at[nq]);
-        o.lo->resize(at[nq]);
-        o.hi->resize(at[nq]);
-#pragma omp parallel for num_threads(team_size()) schedule(static)
-        for (int q = 0; q < nq; q++) cut_job(kind, q, &o, at[q]);
+        for (auto *vv : {o.job, o.p0, o.p1, o.apps}) vv->resize(total);
+        o.lo->resize(total);
+        o.hi->resize(total);
+        size_t k = 0;
+        for (int t = 0; t < used; t++)
+            for (const Seg &s : buf[t]) {
+                (*o.job)[k] = s.job;
+                (*o.p0)[k] = s.p0;
+                (*o.p1)[k] = s.p1;
+                (*o.apps)[k] = s.apps;
+                (*o.lo)[k] = s.lo;
+                (*o.hi)[k] = s.hi;
+                k++;
+            }
     }
 }
 

@@ -462,9 +462,10 @@ struct LevelPool {
 // Host plans of `active` (in parallel across applies when there are several;
 // build_plan parallelises internally otherwise), their pools made on the
 // device from the input trees' blocks into one allocation, then the exact
-// counts and segments.
+// counts and segments (the fine single-component ones only for a mode with
+// an FP64 class beside a low-precision one).
 void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<std::vector<int>> &nodes,
-                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, LevelPool &pools,
+                 const std::vector<std::vector<double>> &thr, InputBlocks &inputs, LevelPool &pools, int mode,
                  double *ms3 = nullptr) {
     const int n = (int)active.size();
     const double t0 = now_ms();
@@ -504,7 +505,7 @@ void build_plans(mwg_context ctx, std::vector<Job *> &active, const std::vector<
     for (int i = 0; i < n; i++) {
         Job &j = *active[i];
         try {
-            finish_plan_counts(j.plan, j.op->op);
+            finish_plan_counts(j.plan, j.op->op, 256, mode != MWG_MODE_FP64, n);
         } catch (const std::exception &e) {
             j.err = e.what();
         }
@@ -594,7 +595,7 @@ int apply_batch(mwg_context ctx, std::vector<Job> &jobs) {
         }
         const double t0 = now_ms();
         LevelPool pools{ctx->dev};
-        build_plans(ctx, active, nodes, thr, inputs, pools, plan3);
+        build_plans(ctx, active, nodes, thr, inputs, pools, ctx->mode, plan3);
         pms += now_ms() - t0;
         std::vector<PlanPart> parts;
         size_t total = 0;
@@ -828,7 +829,7 @@ int make_plan(mwg_context ctx, int n, const mwg_tree *out, const mwg_operator *o
     const double t0 = now_ms();
     InputBlocks inputs(ctx);
     LevelPool pools{ctx->dev};
-    build_plans(ctx, active, nodes, th, inputs, pools);
+    build_plans(ctx, active, nodes, th, inputs, pools, mode);
     p->plan_ms = now_ms() - t0;
     std::vector<PlanPart> parts;
     for (int i = 0; i < n; i++) {
