@@ -1708,15 +1708,21 @@ int dev_plan_create(DeviceCtx *c, const std::vector<PlanPart> &parts, DevicePlan
     // refinement level's plan costs one allocation and one upload however
     // many applies it holds.
     std::vector<DevOper *> opers(parts.size(), nullptr);
-    std::vector<std::vector<double>> mtabs(parts.size());
-    std::vector<std::pair<int, int>> mt_shape(parts.size());
-    for (size_t ai = 0; ai < parts.size() && !rc; ai++) {
-        rc = get_oper(c, parts[ai].op_key, *parts[ai].flat, &opers[ai], err);
-        if (!rc) build_prescreen(*parts[ai].plan, *parts[ai].flat, mtabs[ai], mt_shape[ai].first, mt_shape[ai].second);
-    }
+    std::vector<std::vector<double>> mtabs;
+    std::vector<std::pair<int, int>> mt_shape;
+    for (size_t ai = 0; ai < parts.size() && !rc; ai++) rc = get_oper(c, parts[ai].op_key, *parts[ai].flat, &opers[ai], err);
     if (rc) {
         delete p;
         return rc;
+    }
+    {
+        std::vector<const ApplyPlan *> pls;
+        std::vector<const OperFlat *> fs;
+        for (const PlanPart &pp : parts) {
+            pls.push_back(pp.plan);
+            fs.push_back(pp.flat);
+        }
+        build_prescreens(pls, fs, mtabs, mt_shape);
     }
     size_t arena = 0;
     auto place = [&](size_t bytes) {

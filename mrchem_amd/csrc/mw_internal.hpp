@@ -339,7 +339,9 @@ void quad_rule01(int n, const double **x, const double **w);
 // Pre-screen table of an apply plan for the device screening: for each
 // output depth d in [d0, d0 + nd), component combination cmb = cx | cy << 2 |
 // cz << 4 and translation |l| <= w (the plan's widest band), the largest
-// ((N_x N_y) N_z) over the depth's active terms.
-void build_prescreen(const ApplyPlan &pl, const OperFlat &f, std::vector<double> &mt, int &d0, int &w);
+// ((N_x N_y) N_z) over the depth's active terms.  The tables of all plans of
+// a batch are built in one parallel pass over (plan, depth, cmb).
+void build_prescreens(const std::vector<const ApplyPlan *> &pls, const std::vector<const OperFlat *> &fs,
+                      std::vector<std::vector<double>> &mts, std::vector<std::pair<int, int>> &shapes);
 
 }  // namespace mwg

@@ -570,48 +570,64 @@ void finish_plan_counts(ApplyPlan &plan, const Operator &op, int seg_target, boo
 // |l| <= w (w: the apply's widest band), the largest ((N_x N_y) N_z) over the
 // depth's active terms, with the norms as the kernels stage them (zero outside
 // a term's bandwidth).  Layout [d - d0][cmb][lz + w][ly + w][lx + w].
-void build_prescreen(const ApplyPlan &pl, const OperFlat &f, std::vector<double> &mt, int &d0, int &w) {
-    w = 0;
-    d0 = 0;
-    int d1 = -1;
-    for (size_t q = 0; q < pl.out_depth.size(); q++) {
-        w = std::max(w, pl.band_w[q]);
-        if (d1 < 0 || pl.out_depth[q] < d0) d0 = pl.out_depth[q];
-        d1 = std::max(d1, pl.out_depth[q]);
+void build_prescreens(const std::vector<const ApplyPlan *> &pls, const std::vector<const OperFlat *> &fs,
+                      std::vector<std::vector<double>> &mts, std::vector<std::pair<int, int>> &shapes) {
+    const size_t np = pls.size();
+    mts.assign(np, {});
+    shapes.assign(np, {0, 0});
+    // shapes and the flat task list (plan, depth offset, cmb)
+    struct Task {
+        int part, dd, cmb;
+    };
+    std::vector<Task> tasks;
+    for (size_t ip = 0; ip < np; ip++) {
+        const ApplyPlan &pl = *pls[ip];
+        int w = 0, d0 = 0, d1 = -1;
+        for (size_t q = 0; q < pl.out_depth.size(); q++) {
+            w = std::max(w, pl.band_w[q]);
+            if (d1 < 0 || pl.out_depth[q] < d0) d0 = pl.out_depth[q];
+            d1 = std::max(d1, pl.out_depth[q]);
+        }
+        shapes[ip] = {d0, w};
+        if (d1 < 0) {
+            mts[ip].assign(1, 0.0);
+            continue;
+        }
+        const int S = 2 * w + 1, nd = d1 - d0 + 1;
+        mts[ip].assign((size_t)nd * 64 * S * S * S, 0.0);
+        for (int dd = 0; dd < nd; dd++)
+            for (int cmb = 0; cmb < 64; cmb++) tasks.push_back(Task{(int)ip, dd, cmb});
     }
-    if (d1 < 0) {
-        mt.assign(1, 0.0);
-        return;
-    }
-    const int S = 2 * w + 1, S3 = S * S * S, nd = d1 - d0 + 1;
-    mt.assign((size_t)nd * 64 * S3, 0.0);
-#pragma omp parallel for num_threads(team_size()) collapse(2) schedule(dynamic)
-    for (int dd = 0; dd < nd; dd++) {
-        for (int cmb = 0; cmb < 64; cmb++) {
-            const int d = d0 + dd;
-            if (d + 1 >= (int)pl.active_off.size()) continue;
-            const int cc[3] = {cmb & 3, (cmb >> 2) & 3, (cmb >> 4) & 3};
-            double *dst = mt.data() + ((size_t)dd * 64 + cmb) * S3;
-            std::vector<double> v(3 * (size_t)S);
-            for (int ai = pl.active_off[d]; ai < pl.active_off[d + 1]; ai++) {
-                const int m = pl.active_terms[ai];
-                const int L = f.L[(size_t)m * f.ndepth + d];
-                for (int dim = 0; dim < 3; dim++) {
-                    const int bwc = pl.bw[((size_t)d * pl.M + m) * 4 + cc[dim]];
-                    for (int l = -w; l <= w; l++)
-                        v[dim * S + (l + w)] =
-                            (std::abs(l) <= bwc && std::abs(l) <= L)
-                                ? f.norms[f.noff[(size_t)m * f.ndepth + d] + (long long)cc[dim] * (2 * L + 1) + (l + L)]
-                                : 0.0;
-                }
-                for (int lz = 0; lz < S; lz++)
-                    for (int ly = 0; ly < S; ly++)
-                        for (int lx = 0; lx < S; lx++) {
-                            const double b = (v[lx] * v[S + ly]) * v[2 * S + lz];
-                            double &o = dst[((size_t)lz * S + ly) * S + lx];
-                            if (b > o) o = b;
-                        }
+#pragma omp parallel for num_threads(team_size()) schedule(dynamic, 8)
+    for (long long ti = 0; ti < (long long)tasks.size(); ti++) {
+        const Task tk = tasks[ti];
+        const ApplyPlan &pl = *pls[tk.part];
+        const OperFlat &f = *fs[tk.part];
+        const int d0 = shapes[tk.part].first, w = shapes[tk.part].second;
+        const int S = 2 * w + 1, S3 = S * S * S;
+        const int d = d0 + tk.dd, cmb = tk.cmb;
+        if (d + 1 >= (int)pl.active_off.size()) continue;
+        const int cc[3] = {cmb & 3, (cmb >> 2) & 3, (cmb >> 4) & 3};
+        double *dst = mts[tk.part].data() + ((size_t)tk.dd * 64 + cmb) * S3;
+        std::vector<double> v(3 * (size_t)S);
+        for (int ai = pl.active_off[d]; ai < pl.active_off[d + 1]; ai++) {
+            const int m = pl.active_terms[ai];
+            const int L = f.L[(size_t)m * f.ndepth + d];
+            for (int dim = 0; dim < 3; dim++) {
+                const int bwc = pl.bw[((size_t)d * pl.M + m) * 4 + cc[dim]];
+                for (int l = -w; l <= w; l++)
+                    v[dim * S + (l + w)] =
+                        (std::abs(l) <= bwc && std::abs(l) <= L)
+                            ? f.norms[f.noff[(size_t)m * f.ndepth + d] + (long long)cc[dim] * (2 * L + 1) + (l + L)]
+                            : 0.0;
             }
+            for (int lz = 0; lz < S; lz++)
+                for (int ly = 0; ly < S; ly++)
+                    for (int lx = 0; lx < S; lx++) {
+                        const double b = (v[lx] * v[S + ly]) * v[2 * S + lz];
+                        double &o = dst[((size_t)lz * S + ly) * S + lx];
+                        if (b > o) o = b;
+                    }
         }
     }
 }
