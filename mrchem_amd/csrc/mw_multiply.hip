@@ -356,7 +356,13 @@ __global__ __launch_bounds__(64) void k_component_norms(const double *blocks, lo
 // pass's matrix contiguous (MT[j][r] = M[r][j]), its reads are broadcasts,
 // and the 2N outputs of a line accumulate side by side (2N independent
 // chains) while the inputs v[j] are taken in turn.
-constexpr int kPostThreads = 256;
+// Threads per workgroup: one per line of a pass (4 N^2 lines, in whole
+// waves), so that each pass is one round (the C2 step's post transforms:
+// 0.62 ms with 256 threads, 0.42 ms with 448 at k = 9).
+template <int N>
+struct PostShape {
+    static constexpr int T = 64 * ((4 * N * N + 63) / 64);
+};
 
 template <int N, int DIM>
 __device__ __forceinline__ int post_index(int a, int c, int i) {
@@ -368,7 +374,7 @@ __device__ __forceinline__ int post_index(int a, int c, int i) {
 template <int N, int DIM>
 __device__ __forceinline__ void post_line_pass(double *blk, const double *MT, int tid) {
     constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N, bit = 1 << DIM, low = bit - 1;
-    for (int L = tid; L < 4 * NN; L += kPostThreads) {
+    for (int L = tid; L < 4 * NN; L += PostShape<N>::T) {
         const int pr = L / NN, ac = L % NN, a = ac % N, c = ac / N;
         const int t0 = ((pr & ~low) << 1) | (pr & low), t1 = t0 | bit;
         double o[N2];
@@ -394,18 +400,18 @@ __device__ __forceinline__ void post_line_pass(double *blk, const double *MT, in
 // TopDown step for one level: each branch's block reconstructed into its 8
 // children's scaling parts (directions z, y, x), added to theirs.
 template <int N>
-__global__ __launch_bounds__(kPostThreads) void k_topdown(const PostPair *pairs, int n_pairs,
+__global__ __launch_bounds__(PostShape<N>::T) void k_topdown(const PostPair *pairs, int n_pairs,
                                                           const double *__restrict__ F) {
     constexpr int N3 = N * N * N, N2 = 2 * N;
     __shared__ double blk[8 * N3];
     __shared__ double M[N2 * N2];  // transposed matrix of F^T: F itself
     const int tid = threadIdx.x;
-    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[e];
+    for (int e = tid; e < N2 * N2; e += PostShape<N>::T) M[e] = F[e];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
         // unrolled, so that several global loads are in flight per thread
 #pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += kPostThreads) blk[e] = pr.node[e];
+        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) blk[e] = pr.node[e];
         __syncthreads();
         post_line_pass<N, 2>(blk, M, tid);
         __syncthreads();
@@ -414,7 +420,7 @@ __global__ __launch_bounds__(kPostThreads) void k_topdown(const PostPair *pairs,
         post_line_pass<N, 0>(blk, M, tid);
         __syncthreads();
 #pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += kPostThreads) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] += blk[e];
+        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] += blk[e];
         __syncthreads();
     }
 }
@@ -422,17 +428,17 @@ __global__ __launch_bounds__(kPostThreads) void k_topdown(const PostPair *pairs,
 // BottomUp step for one level: each branch's [s, d1..d7] from its children's
 // scaling parts (directions x, y, z).
 template <int N>
-__global__ __launch_bounds__(kPostThreads) void k_compress(const PostPair *pairs, int n_pairs,
+__global__ __launch_bounds__(PostShape<N>::T) void k_compress(const PostPair *pairs, int n_pairs,
                                                            const double *__restrict__ F) {
     constexpr int N3 = N * N * N, N2 = 2 * N;
     __shared__ double blk[8 * N3];
     __shared__ double M[N2 * N2];  // transposed matrix of F: F^T
     const int tid = threadIdx.x;
-    for (int e = tid; e < N2 * N2; e += kPostThreads) M[e] = F[(e % N2) * N2 + e / N2];
+    for (int e = tid; e < N2 * N2; e += PostShape<N>::T) M[e] = F[(e % N2) * N2 + e / N2];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
 #pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += kPostThreads) blk[e] = pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3];
+        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) blk[e] = pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3];
         __syncthreads();
         post_line_pass<N, 0>(blk, M, tid);
         __syncthreads();
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(kPostThreads) void k_compress(const PostPair *pairs
         post_line_pass<N, 2>(blk, M, tid);
         __syncthreads();
 #pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += kPostThreads) pr.node[e] = blk[e];
+        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) pr.node[e] = blk[e];
         __syncthreads();
     }
 }
@@ -451,9 +457,9 @@ template <int N>
 static hipError_t post_level_n(hipStream_t s, bool down, const PostPair *pairs, int cnt, const double *F) {
     const int g = std::min(cnt, kMulBlocks);
     if (down)
-        hipLaunchKernelGGL((k_topdown<N>), dim3(g), dim3(kPostThreads), 0, s, pairs, cnt, F);
+        hipLaunchKernelGGL((k_topdown<N>), dim3(g), dim3(PostShape<N>::T), 0, s, pairs, cnt, F);
     else
-        hipLaunchKernelGGL((k_compress<N>), dim3(g), dim3(kPostThreads), 0, s, pairs, cnt, F);
+        hipLaunchKernelGGL((k_compress<N>), dim3(g), dim3(PostShape<N>::T), 0, s, pairs, cnt, F);
     return hipGetLastError();
 }
 
