@@ -121,7 +121,8 @@ def test_split16_term_set_and_error(ctx, order, prec, basis):
     """FP16 hi/lo split products: same term set as FP64 and within prec of
     the FP64 result on the identical grid (N <= 8, N = 9..10 and N = 11..12
     take different operand layouts; N = 9 and 11 have partly empty tails), in
-    both bases."""
+    both bases; and the FP64 plan equal to the oracle on that grid at every
+    N (the FP64 kernel's tile split differs by N)."""
     mra = _mra(order, basis)
     P = mw.PoissonOperator(mra, prec)
     rng = np.random.default_rng(23)
@@ -151,3 +152,9 @@ def test_split16_term_set_and_error(ctx, order, prec, basis):
     psp.finish()
     E64, Esp = rho.dot(V), rho.dot(V2)
     assert abs(Esp - E64) < prec * max(1.0, abs(E64))
+    # the FP64 plan (pair jobs; each N has its own tile split) against the
+    # oracle on this grid
+    ref, st = oracle.apply_fixed(mra, P.tables(), rho.nodes(), od, ol, thr, want_ns=False)
+    c64 = V.nodes()[2]
+    assert np.max(np.abs(ref - c64)) / np.max(np.abs(ref)) < TOL_FP64
+    assert st["term_apps"] == r64["term_apps"]
