@@ -149,10 +149,12 @@ struct JobShape {
     static constexpr int N2 = N * N, N3 = N2 * N;
     static constexpr int NT = (N2 + 15) / 16;  // 16-wide tiles over N^2
     static constexpr int KS = (N + 3) / 4;     // 4-deep k steps over N
-    // waves per job, at most 4 tiles each (fewer tiles per wave measured
-    // slower at N = 8, 10, 12: the operator operands are read from LDS once
-    // per wave and item)
-    static constexpr int NW = (NT + 3) / 4;
+    // waves per job: at most 4 tiles each at N >= 9 (fewer tiles per wave
+    // measured slower at N = 10, 12: each wave reads the item's operator
+    // operands from LDS once), 2 of 2 tiles at N = 7, 8 (one wave per job
+    // left LDS-bound occupancy at 5 waves per CU: 136 exchange applies at
+    // k = 7, plain FP64 105.9 -> 97.6 ms)
+    static constexpr int NW = N == 7 || N == 8 ? (NT + 1) / 2 : (NT + 3) / 4;
     static constexpr int TW = (NT + NW - 1) / NW;  // tiles per wave
     static constexpr int T = 64 * NW;          // threads per job
     static constexpr int NTP = NW * TW;        // tiles including padding tiles (N = 10: 8 for 7)

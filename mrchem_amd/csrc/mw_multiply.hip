@@ -85,9 +85,9 @@ __device__ void pair_pass(int n, const double *F, const double *src, double *dst
 // pairs: dst[k] = sum_j H_bit[j][i] lo_k[j] + G_bit[j][i] hi_k[j] along dim,
 // with lo_k = src + lo_off[k] blocks, hi_k likewise (hi_off < 0: none).
 __device__ void child_pass(int n, const double *F, const double *src, const int *lo_blk, const int *hi_blk, int nb,
-                           double *dst, int dim, int bit, int tid) {
+                           double *dst, int dim, int bit, int tid, int nth = kMulThreads) {
     const int n2 = 2 * n, n3 = n * n * n;
-    for (int e = tid; e < nb * n3; e += kMulThreads) {
+    for (int e = tid; e < nb * n3; e += nth) {
         const int k = e / n3, rem = e % n3;
         int i, a, c;
         if (dim == 0) {
@@ -248,7 +248,13 @@ struct GenItem {
 // Generated nodes items[0, n_items) of one depth (their parents are input
 // nodes or generated nodes of the depth above): the z, y and x passes of
 // the two-scale filter through LDS, one workgroup per node.
-__global__ __launch_bounds__(kMulThreads) void k_gen_fill(int n, const double *F, const GenItem *items, int n_items) {
+// Threads of a k_gen_fill workgroup: its LDS (6 n^3 doubles, 83 KB at
+// n = 12) allows one or two workgroups per compute unit, so a workgroup
+// brings 16 waves (64 Helmholtz applies at k = 11, adaptive on resident
+// trees: 1099 -> 1023 ms against 256 threads).
+constexpr int kGenThreads = 1024;
+
+__global__ __launch_bounds__(kGenThreads) void k_gen_fill(int n, const double *F, const GenItem *items, int n_items) {
     extern __shared__ double lds[];  // 6 n^3
     const int tid = threadIdx.x, n3 = n * n * n;
     double *X = lds, *Y = lds + 4 * n3;
@@ -257,20 +263,20 @@ __global__ __launch_bounds__(kMulThreads) void k_gen_fill(int n, const double *F
         const int t = it.t;
         if (it.full) {
             const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
-            child_pass(n, F, it.src, lo, hi, 4, X, 2, (t >> 2) & 1, tid);
+            child_pass(n, F, it.src, lo, hi, 4, X, 2, (t >> 2) & 1, tid, kGenThreads);
             __syncthreads();
             const int lo2[2] = {0, 1}, hi2[2] = {2, 3};
-            child_pass(n, F, X, lo2, hi2, 2, Y, 1, (t >> 1) & 1, tid);
+            child_pass(n, F, X, lo2, hi2, 2, Y, 1, (t >> 1) & 1, tid, kGenThreads);
             __syncthreads();
             const int lo1[1] = {0}, hi1[1] = {1};
-            child_pass(n, F, Y, lo1, hi1, 1, it.out, 0, t & 1, tid);
+            child_pass(n, F, Y, lo1, hi1, 1, it.out, 0, t & 1, tid, kGenThreads);
         } else {
             const int lo[1] = {0}, none[1] = {-1};
-            child_pass(n, F, it.src, lo, none, 1, X, 2, (t >> 2) & 1, tid);
+            child_pass(n, F, it.src, lo, none, 1, X, 2, (t >> 2) & 1, tid, kGenThreads);
             __syncthreads();
-            child_pass(n, F, X, lo, none, 1, Y, 1, (t >> 1) & 1, tid);
+            child_pass(n, F, X, lo, none, 1, Y, 1, (t >> 1) & 1, tid, kGenThreads);
             __syncthreads();
-            child_pass(n, F, Y, lo, none, 1, it.out, 0, t & 1, tid);
+            child_pass(n, F, Y, lo, none, 1, it.out, 0, t & 1, tid, kGenThreads);
         }
         __syncthreads();
     }
@@ -566,7 +572,7 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
     for (size_t lv = 0; e == hipSuccess && lv < nlv; lv++) {
         const int cnt = (int)(lv_at[lv + 1] - lv_at[lv]);
         if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_gen_fill, dim3(std::min(cnt, kMulBlocks)), dim3(kMulThreads), sizeof(double) * 6 * n3, s,
+        hipLaunchKernelGGL(k_gen_fill, dim3(std::min(cnt, kMulBlocks)), dim3(kGenThreads), sizeof(double) * 6 * n3, s,
                            n, F, dgi + lv_at[lv], cnt);
         e = hipGetLastError();
     }
