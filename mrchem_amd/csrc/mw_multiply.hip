@@ -236,49 +236,70 @@ __global__ __launch_bounds__(kMulThreads) void k_add_nodes(int n, const AddSrc *
     }
 }
 
-// One generated node of an apply plan's pool: the scaling block of child t
-// of a parent block, the parent being an input-tree node (full: its 8
-// components) or a generated node (its scaling block only).
-struct GenItem {
+// The generated children of one parent in an apply plan's pool: out[t] is
+// the scaling block of child t (null when child t is not generated), the
+// parent an input-tree node (full: its 8 components) or a generated node
+// (its scaling block only).
+struct GenGroup {
     const double *src;
-    double *out;
-    int t, full;
+    double *out[8];
+    int full, mask;
 };
 
-// Generated nodes items[0, n_items) of one depth (their parents are input
-// nodes or generated nodes of the depth above): the z, y and x passes of
-// the two-scale filter through LDS, one workgroup per node.
 // Threads of a k_gen_fill workgroup: its LDS (6 n^3 doubles, 83 KB at
 // n = 12) allows one or two workgroups per compute unit, so a workgroup
 // brings 16 waves (64 Helmholtz applies at k = 11, adaptive on resident
 // trees: 1099 -> 1023 ms against 256 threads).
 constexpr int kGenThreads = 1024;
 
-__global__ __launch_bounds__(kGenThreads) void k_gen_fill(int n, const double *F, const GenItem *items, int n_items) {
+// Generated nodes of one depth, grouped by parent (their parents are input
+// nodes or generated nodes of the depth above): one workgroup per parent
+// runs the z pass once per z half of the children it has, the y pass once
+// per (z, y) quarter and the x pass per child, through LDS (X: 4 blocks,
+// Y: 2); each output gets the operations it gets taken alone.
+__global__ __launch_bounds__(kGenThreads) void k_gen_fill(int n, const double *F, const GenGroup *groups,
+                                                          int n_groups) {
     extern __shared__ double lds[];  // 6 n^3
     const int tid = threadIdx.x, n3 = n * n * n;
     double *X = lds, *Y = lds + 4 * n3;
-    for (int g = blockIdx.x; g < n_items; g += gridDim.x) {
-        const GenItem it = items[g];
-        const int t = it.t;
-        if (it.full) {
-            const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
-            child_pass(n, F, it.src, lo, hi, 4, X, 2, (t >> 2) & 1, tid, kGenThreads);
+    for (int gi = blockIdx.x; gi < n_groups; gi += gridDim.x) {
+        const GenGroup &gr = groups[gi];
+        const int mask = gr.mask, full = gr.full;
+        const double *src = gr.src;
+        for (int zb = 0; zb < 2; zb++) {
+            if (!((mask >> (4 * zb)) & 0xf)) continue;
+            if (full) {
+                const int lo[4] = {0, 1, 2, 3}, hi[4] = {4, 5, 6, 7};
+                child_pass(n, F, src, lo, hi, 4, X, 2, zb, tid, kGenThreads);
+            } else {
+                const int lo[1] = {0}, none[1] = {-1};
+                child_pass(n, F, src, lo, none, 1, X, 2, zb, tid, kGenThreads);
+            }
             __syncthreads();
-            const int lo2[2] = {0, 1}, hi2[2] = {2, 3};
-            child_pass(n, F, X, lo2, hi2, 2, Y, 1, (t >> 1) & 1, tid, kGenThreads);
-            __syncthreads();
-            const int lo1[1] = {0}, hi1[1] = {1};
-            child_pass(n, F, Y, lo1, hi1, 1, it.out, 0, t & 1, tid, kGenThreads);
-        } else {
-            const int lo[1] = {0}, none[1] = {-1};
-            child_pass(n, F, it.src, lo, none, 1, X, 2, (t >> 2) & 1, tid, kGenThreads);
-            __syncthreads();
-            child_pass(n, F, X, lo, none, 1, Y, 1, (t >> 1) & 1, tid, kGenThreads);
-            __syncthreads();
-            child_pass(n, F, Y, lo, none, 1, it.out, 0, t & 1, tid, kGenThreads);
+            for (int yb = 0; yb < 2; yb++) {
+                if (!((mask >> (4 * zb + 2 * yb)) & 0x3)) continue;
+                if (full) {
+                    const int lo2[2] = {0, 1}, hi2[2] = {2, 3};
+                    child_pass(n, F, X, lo2, hi2, 2, Y, 1, yb, tid, kGenThreads);
+                } else {
+                    const int lo[1] = {0}, none[1] = {-1};
+                    child_pass(n, F, X, lo, none, 1, Y, 1, yb, tid, kGenThreads);
+                }
+                __syncthreads();
+                for (int xb = 0; xb < 2; xb++) {
+                    const int t = xb | (yb << 1) | (zb << 2);
+                    if (!((mask >> t) & 1)) continue;
+                    if (full) {
+                        const int lo1[1] = {0}, hi1[1] = {1};
+                        child_pass(n, F, Y, lo1, hi1, 1, gr.out[t], 0, xb, tid, kGenThreads);
+                    } else {
+                        const int lo[1] = {0}, none[1] = {-1};
+                        child_pass(n, F, Y, lo, none, 1, gr.out[t], 0, xb, tid, kGenThreads);
+                    }
+                }
+                __syncthreads();
+            }
         }
-        __syncthreads();
     }
 }
 
@@ -507,15 +528,23 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
         err = "pool fill: too many slots";
         return -1;
     }
-    std::vector<size_t> lv_at(nlv + 1, 0);  // items of depth lv: [lv_at[lv], lv_at[lv + 1])
+    // parent groups (runs of one parent within a depth: the plan orders the
+    // children of a parent next to each other) of depth lv:
+    // [lv_at[lv], lv_at[lv + 1])
+    auto n_groups = [&](const ApplyPlan &pl, size_t lv) {
+        size_t k = 0;
+        for (int g = pl.gen_level_off[lv]; g < pl.gen_level_off[lv + 1]; g++)
+            k += (g == pl.gen_level_off[lv] || pl.gen_parent[g] != pl.gen_parent[g - 1]);
+        return k;
+    };
+    std::vector<size_t> lv_at(nlv + 1, 0);
     for (size_t lv = 0; lv < nlv; lv++) {
         size_t k = 0;
-        for (size_t i = 0; i < nj; i++) {
-            const std::vector<int> &go = jobs[i].pl->gen_level_off;
-            if (lv + 1 < go.size()) k += (size_t)(go[lv + 1] - go[lv]);
-        }
+        for (size_t i = 0; i < nj; i++)
+            if (lv + 1 < jobs[i].pl->gen_level_off.size()) k += n_groups(*jobs[i].pl, lv);
         lv_at[lv + 1] = lv_at[lv] + k;
     }
+    const size_t n_grp = lv_at[nlv];
     // one device allocation: [filter | gen items | slot items] (uploaded)
     // [generated scaling blocks of every job] [every slot's norms]
     size_t bytes = 0;
@@ -525,7 +554,7 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
         return off;
     };
     const size_t f_F = place(sizeof(double) * mra.basis.F.size());
-    const size_t f_gi = place(sizeof(GenItem) * n_gen), f_si = place(sizeof(SlotItem) * n_slot);
+    const size_t f_gi = place(sizeof(GenGroup) * n_grp), f_si = place(sizeof(SlotItem) * n_slot);
     const size_t up_bytes = bytes;
     const size_t f_gen = place(sizeof(double) * n_gen * (size_t)n3), f_nrm = place(sizeof(double) * 8 * n_slot);
     unsigned char *dev = nullptr;
@@ -533,7 +562,7 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
     double *gen_s = reinterpret_cast<double *>(dev + f_gen), *norms = reinterpret_cast<double *>(dev + f_nrm);
     std::vector<unsigned char, UninitAlloc<unsigned char>> stage(up_bytes);
     std::memcpy(stage.data() + f_F, mra.basis.F.data(), sizeof(double) * mra.basis.F.size());
-    GenItem *gi = reinterpret_cast<GenItem *>(stage.data() + f_gi);
+    GenGroup *gi = reinterpret_cast<GenGroup *>(stage.data() + f_gi);
     SlotItem *si = reinterpret_cast<SlotItem *>(stage.data() + f_si);
     // job i's generated node g is gen_s block gen_at[i] + g
     std::vector<size_t> lv_fill(lv_at.begin(), lv_at.end() - (nlv ? 1 : 0));
@@ -541,15 +570,27 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
         const ApplyPlan &pl = *jobs[i].pl;
         const double *inp = jobs[i].inp;
         double *gs = gen_s + gen_at[i] * (size_t)n3;
-        for (size_t lv = 0; lv + 1 < pl.gen_level_off.size(); lv++)
+        for (size_t lv = 0; lv + 1 < pl.gen_level_off.size(); lv++) {
+            GenGroup *cur = nullptr;
             for (int g = pl.gen_level_off[lv]; g < pl.gen_level_off[lv + 1]; g++) {
                 const int par = pl.gen_parent[g];
-                GenItem &it = gi[lv_fill[lv]++];
-                it.src = par < 0 ? inp + (size_t)(-par - 1) * 8 * n3 : gs + (size_t)par * n3;
-                it.out = gs + (size_t)g * n3;
-                it.t = pl.gen_child[g];
-                it.full = par < 0;
+                if (g == pl.gen_level_off[lv] || par != pl.gen_parent[g - 1]) {
+                    cur = &gi[lv_fill[lv]++];
+                    cur->src = par < 0 ? inp + (size_t)(-par - 1) * 8 * n3 : gs + (size_t)par * n3;
+                    for (int t = 0; t < 8; t++) cur->out[t] = nullptr;
+                    cur->full = par < 0;
+                    cur->mask = 0;
+                }
+                const int t = pl.gen_child[g];
+                if ((cur->mask >> t) & 1) {
+                    (void)hipFree(dev);
+                    err = "pool fill: a generated node twice";
+                    return -1;
+                }
+                cur->out[t] = gs + (size_t)g * n3;
+                cur->mask |= 1 << t;
             }
+        }
     }
     for (size_t i = 0; i < nj; i++) {
         const ApplyPlan &pl = *jobs[i].pl;
@@ -566,7 +607,7 @@ int dev_pool_fill(DeviceCtx *c, const MRA &mra, const std::vector<PoolFill> &job
     }
     std::vector<double, UninitAlloc<double>> norms_h(8 * n_slot);
     const double *F = reinterpret_cast<const double *>(dev + f_F);
-    const GenItem *dgi = reinterpret_cast<const GenItem *>(dev + f_gi);
+    const GenGroup *dgi = reinterpret_cast<const GenGroup *>(dev + f_gi);
     const SlotItem *dsi = reinterpret_cast<const SlotItem *>(dev + f_si);
     hipError_t e = hipMemcpyAsync(dev, stage.data(), up_bytes, hipMemcpyHostToDevice, s);
     for (size_t lv = 0; e == hipSuccess && lv < nlv; lv++) {

@@ -240,10 +240,24 @@ void build_plan(ApplyPlan &plan, const Operator &op, const Tree &inp, const Tree
         // parent; in_norms and the counts follow (finish_plan_counts)
         plan.device_pool = true;
         for (int s = 0; s < ns; s++) plan.in_depth[s] = slot_keys[s].d;
+        // depth by depth, and within a depth the children of one parent
+        // next to each other (the pool fill reconstructs them together)
         std::vector<int> order(ng), rank(ng);
         for (int i = 0; i < ng; i++) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gen_keys[x].d < gen_keys[y].d; });
-        for (int i = 0; i < ng; i++) rank[order[i]] = i;
+        for (int i0 = 0; i0 < ng;) {
+            int i1 = i0 + 1;
+            while (i1 < ng && gen_keys[order[i1]].d == gen_keys[order[i0]].d) i1++;
+            // parents of this depth are real nodes (key < 0) or generated
+            // nodes of the depth above, already ranked
+            auto pkey = [&](int g) {
+                const int par = gen_parent[g];
+                return par < 0 ? (long long)par : (long long)rank[par];
+            };
+            std::stable_sort(order.begin() + i0, order.begin() + i1, [&](int x, int y) { return pkey(x) < pkey(y); });
+            for (int i = i0; i < i1; i++) rank[order[i]] = i;
+            i0 = i1;
+        }
         plan.gen_parent.resize(ng);
         plan.gen_child.resize(ng);
         plan.gen_level_off.assign(1, 0);
