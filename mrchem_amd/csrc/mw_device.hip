@@ -112,9 +112,9 @@ __device__ __forceinline__ const float *op_blocks<float>(const AppDesc &ad) {
 // A pair job's components share Ox and Oy, so its items run the x and y
 // chains once and the z chain for each component whose bound passes (two
 // accumulators): 4 chains instead of 6 when both pass.
-// X and Y change layout through one padded LDS buffer whose lane addresses
-// are a base plus a compile-time offset (padding entries stay zero or finite
-// and only meet zero operands).  While an item computes, the operator blocks
+// X and Y change layout through one padded LDS buffer (two at N >= 11)
+// whose lane addresses are a base plus a compile-time offset (padding
+// entries stay zero or finite and only meet zero operands).  While an item computes, the operator blocks
 // of the next item are loaded (registers, then a double-buffered LDS stage)
 // and, when the next item starts a new (f, ft) group, its F tiles too (into
 // the F registers, which the x chain no longer needs, during the y and z
@@ -164,7 +164,12 @@ struct JobShape {
     static constexpr int XROWS = XROWS_W > XROWS_R ? XROWS_W : XROWS_R;
     static constexpr int CP = 16 * NTP > 4 * KS * N ? 16 * NTP : 4 * KS * N;
     static constexpr int XSZ = XROWS * 16, YSZ = 16 * CP;
-    static constexpr int BUF = XSZ > YSZ ? XSZ : YSZ;
+    // N >= 11: X and Y in buffers of their own, so that an item needs two
+    // barriers instead of four (64 Helmholtz applies at k = 11, plain FP64:
+    // 703 -> 679 ms); at N <= 10 the larger LDS footprint costs a workgroup
+    // per CU and one shared buffer is faster
+    static constexpr bool SEP = N >= 11;
+    static constexpr int BUF = SEP ? XSZ + YSZ : (XSZ > YSZ ? XSZ : YSZ);
     static constexpr int OB = 4 * N2;          // operator values per stage (Ox, Oy, Oz and Oz of gt | 4)
 };
 
@@ -349,14 +354,15 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
     constexpr int N2 = S::N2, N3 = S::N3, NT = S::NT, KS = S::KS, TW = S::TW, NTH = S::T;
     constexpr int CP = S::CP, BUF = S::BUF;
     constexpr int OB = (PAIR ? 4 : 3) * N2, OPL = (OB + NTH - 1) / NTH;
-    T *buf = sm.buf;
+    T *const xb = sm.buf;
+    T *const yb = sm.buf + (S::SEP ? S::XSZ : 0);
     T(*ops)[S::OB] = sm.ops;
     int *wl = sm.wl;
     int *ltab = sm.ltab;
     long long *btab = sm.btab;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lc = lane & 15, lq = lane >> 4;
-    for (int e = tid; e < BUF; e += NTH) buf[e] = T(0);
+    for (int e = tid; e < BUF; e += NTH) xb[e] = T(0);
 
     const int pi_begin = a.seg_p0[seg], pi_end = a.seg_p1[seg];
     const AppDesc ad = a.apps[a.out_app[a.seg_job[seg] >> 3]];
@@ -448,19 +454,46 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
             int it = wl[0];
             load_f(it >> 11, (it >> 6) & 7, fa);
             fetch(it, pre);
-            int stage = 0;
-            for (int i = 0; i < n; i++) {
+            if constexpr (S::SEP) {
+                // separate buffers: item i + 1's operator blocks go into the
+                // other stage at the start of item i (fetched during item
+                // i - 1), so no barrier is needed before an item's operands
 #pragma unroll
                 for (int u = 0; u < OPL; u++) {
                     const int e = tid + NTH * u;
-                    if (e < OB) ops[stage][e] = pre[u];
+                    if (e < OB) ops[0][e] = pre[u];
+                }
+                if (n > 1) fetch(wl[1], pre);
+                __syncthreads();
+            }
+            int stage = 0;
+            for (int i = 0; i < n; i++) {
+                if constexpr (!S::SEP) {
+#pragma unroll
+                    for (int u = 0; u < OPL; u++) {
+                        const int e = tid + NTH * u;
+                        if (e < OB) ops[stage][e] = pre[u];
+                    }
                 }
                 const bool more = i + 1 < n;
                 const int nx = more ? wl[i + 1] : it;
                 const bool newgroup = more && group(nx) != group(it);
                 const bool t0 = !PAIR || ((it >> 9) & 1), t1 = PAIR && ((it >> 10) & 1);
-                if (more) fetch(nx, pre);
-                __syncthreads();
+                if constexpr (S::SEP) {
+                    if (more) {
+#pragma unroll
+                        for (int u = 0; u < OPL; u++) {
+                            const int e = tid + NTH * u;
+                            if (e < OB) ops[stage ^ 1][e] = pre[u];
+                        }
+                        if (i + 2 < n) fetch(wl[i + 2], pre);
+                    }
+                } else {
+                    if (more) fetch(nx, pre);
+                    // (also: the previous item's z pass has read the shared
+                    // buffer)
+                    __syncthreads();
+                }
                 // operator operands: Ox[lc][4k+lq] (B of x as Ox^T), Oy, Oz (A of y, z)
                 T ox[KS], oy[KS], oz[KS], oz1[PAIR ? KS : 1];
                 const T *O = ops[stage];
@@ -488,7 +521,7 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
                         if (live(tt))
 #pragma unroll
                             for (int r = 0; r < 4; r++)
-                                buf[(16 * (wv * TW + tt) + MF::drow(lq, r)) * 16 + lc] = dx[tt][r];
+                                xb[(16 * (wv * TW + tt) + MF::drow(lq, r)) * 16 + lc] = dx[tt][r];
                 }
                 // the F tiles are dead after the x pass: those of the next
                 // item's group load during its y and z passes
@@ -503,14 +536,16 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
                     for (int k = 0; k < KS; k++)
 #pragma unroll
                         for (int tt = 0; tt < TW; tt++)
-                            if (live(tt)) dy[tt] = MF::run(oy[k], buf[ybase[tt] + 64 * k], dy[tt]);
-                    __syncthreads();
+                            if (live(tt)) dy[tt] = MF::run(oy[k], xb[ybase[tt] + 64 * k], dy[tt]);
+                    // one shared buffer: every wave has read X before Y
+                    // overwrites it
+                    if constexpr (!S::SEP) __syncthreads();
 #pragma unroll
                     for (int tt = 0; tt < TW; tt++)
                         if (live(tt))
 #pragma unroll
                             for (int r = 0; r < 4; r++)
-                                buf[MF::drow(lq, r) * CP + 16 * (wv * TW + tt) + lc] = dy[tt][r];
+                                yb[MF::drow(lq, r) * CP + 16 * (wv * TW + tt) + lc] = dy[tt][r];
                 }
                 __syncthreads();
                 // z pass: B[jz][col'] = Yp[iy][jz*N + ix], col' = iy*N + ix,
@@ -521,7 +556,7 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
 #pragma unroll
                         for (int tt = 0; tt < TW; tt++)
                             if (live(tt)) {
-                                const T yv = buf[zbase[tt] + 4 * k * N];
+                                const T yv = yb[zbase[tt] + 4 * k * N];
                                 acc[tt] = MF::run(oz[k], yv, acc[tt]);
                                 if (PAIR) acc1[tt] = MF::run(oz1[k], yv, acc1[tt]);
                             }
@@ -530,13 +565,13 @@ __device__ __forceinline__ void apply_segment(const KArgs &a, const T *pool, int
                     for (int k = 0; k < KS; k++)
 #pragma unroll
                         for (int tt = 0; tt < TW; tt++)
-                            if (live(tt)) acc[tt] = MF::run(oz[k], buf[zbase[tt] + 4 * k * N], acc[tt]);
+                            if (live(tt)) acc[tt] = MF::run(oz[k], yb[zbase[tt] + 4 * k * N], acc[tt]);
                 } else {
 #pragma unroll
                     for (int k = 0; k < KS; k++)
 #pragma unroll
                         for (int tt = 0; tt < TW; tt++)
-                            if (live(tt)) acc1[tt] = MF::run(oz1[k], buf[zbase[tt] + 4 * k * N], acc1[tt]);
+                            if (live(tt)) acc1[tt] = MF::run(oz1[k], yb[zbase[tt] + 4 * k * N], acc1[tt]);
                 }
                 it = nx;
                 stage ^= 1;
