@@ -259,8 +259,11 @@ struct ApplyPlan {
     std::vector<int> active_off, active_terms;  // per depth, terms with any w >= 0
     long long n_generated = 0;
     // Work segments of the output component pairs (gt, gt | 4), gt < 4:
-    // job = 8*out + gt; a segment covers band positions [pseg_p0, pseg_p1)
-    // of one job.  Jobs are cut so that no segment holds many more than
+    // job = 8*out + gt; a segment covers the (band position p, input
+    // component ft) pairs 8 p + ft in [pseg_p0, pseg_p1) of one job (whole
+    // band positions, except in small launches, where a band position with
+    // more terms than a segment is cut by ft).  Jobs are cut so that no
+    // segment holds many more than
     // `seg_target` term-applications of the two components (counted exactly
     // here with the kernels' screening rule); jobs without any are left out.
     // The kernels share the x and y passes of a pair.  Per segment: passing
