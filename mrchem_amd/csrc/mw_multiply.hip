@@ -336,23 +336,31 @@ __global__ __launch_bounds__(256) void k_slot_norms(int n, const SlotItem *items
     it.norms[comp] = sqrt(q);
 }
 
-// Squared norms of each node's scaling part and wavelet parts: one wave
-// per node.
-__global__ __launch_bounds__(64) void k_node_norms(const double *blocks, long long n, int n3, double *s2, double *w2) {
+// Squared norms of each node's scaling part and wavelet parts: one
+// workgroup of four waves per node; the four wave sums are added in wave
+// order, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_node_norms(const double *blocks, long long n, int n3, double *s2, double *w2) {
+    __shared__ double part[2][4];
     const long long b = blockIdx.x;
     if (b >= n) return;
     const double *x = blocks + b * 8 * n3;
+    const int tid = threadIdx.x, wave = tid >> 6;
     double s = 0.0, w = 0.0;
-    for (int e = threadIdx.x; e < n3; e += 64) s += x[e] * x[e];
-    for (int e = n3 + threadIdx.x; e < 8 * n3; e += 64) w += x[e] * x[e];
+    for (int e = tid; e < n3; e += 256) s += x[e] * x[e];
+    for (int e = n3 + tid; e < 8 * n3; e += 256) w += x[e] * x[e];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o);
         w += __shfl_xor(w, o);
     }
-    if (threadIdx.x == 0) {
-        s2[b] = s;
-        w2[b] = w;
+    if ((tid & 63) == 0) {
+        part[0][wave] = s;
+        part[1][wave] = w;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s2[b] = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
+        w2[b] = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
     }
 }
 
@@ -383,12 +391,18 @@ __global__ __launch_bounds__(64) void k_component_norms(const double *blocks, lo
 // pass's matrix contiguous (MT[j][r] = M[r][j]), its reads are broadcasts,
 // and the 2N outputs of a line accumulate side by side (2N independent
 // chains) while the inputs v[j] are taken in turn.
-// Threads per workgroup: one per line of a pass (4 N^2 lines, in whole
-// waves), so that each pass is one round (the C2 step's post transforms:
-// 0.62 ms with 256 threads, 0.42 ms with 448 at k = 9).
+// Threads per workgroup: two per line of a pass where the 8 N^2 of them fit
+// a workgroup (N <= 11), each computing the N outputs of one of the line's
+// two components, else one per line (N = 12), in whole waves, so that each
+// pass is one round.  The levels of a step hold fewer branch nodes than the
+// chip has CUs, so each launch lasts as long as one workgroup's pass
+// pipeline: shorter per-thread chains shorten it (the C2 step's post
+// transforms at k = 9: 0.62 ms with 256 threads, 0.42 ms with one thread
+// per line).
 template <int N>
 struct PostShape {
-    static constexpr int T = 64 * ((4 * N * N + 63) / 64);
+    static constexpr int H = (8 * N * N <= 1024) ? 2 : 1;  // threads per line
+    static constexpr int T = 64 * ((H * 4 * N * N + 63) / 64);
 };
 
 template <int N, int DIM>
@@ -401,25 +415,54 @@ __device__ __forceinline__ int post_index(int a, int c, int i) {
 template <int N, int DIM>
 __device__ __forceinline__ void post_line_pass(double *blk, const double *MT, int tid) {
     constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N, bit = 1 << DIM, low = bit - 1;
-    for (int L = tid; L < 4 * NN; L += PostShape<N>::T) {
-        const int pr = L / NN, ac = L % NN, a = ac % N, c = ac / N;
-        const int t0 = ((pr & ~low) << 1) | (pr & low), t1 = t0 | bit;
-        double o[N2];
+    constexpr int H = PostShape<N>::H, NO = N2 / H;  // outputs per thread
+    static_assert(PostShape<N>::T >= H * 4 * NN && PostShape<N>::T <= 1024, "one line (half) per thread");
+    // lanes 2i and 2i + 1 share line i (their input reads are one address)
+    const bool live = tid < H * 4 * NN;
+    const int L = live ? tid / H : 0, h = live ? tid % H : 0;
+    const int pr = L / NN, ac = L % NN, a = ac % N, c = ac / N;
+    const int t0 = ((pr & ~low) << 1) | (pr & low), t1 = t0 | bit;
+    double o[NO];
+    if (live) {
+        // each output a sum over j = 0 .. 2N - 1 in turn
 #pragma unroll
-        for (int r = 0; r < N2; r++) o[r] = 0.0;
-        // one input at a time (read from LDS), into all 2N accumulators
+        for (int r = 0; r < NO; r++) o[r] = 0.0;
+        if constexpr (H > 1) {
+            // the line's 2N inputs first (all loads in flight at once)
+            double v[N2];
+#pragma unroll
+            for (int j = 0; j < N2; j++)
+                v[j] = blk[(j < N ? t0 : t1) * N3 + post_index<N, DIM>(a, c, j < N ? j : j - N)];
+#pragma unroll
+            for (int j = 0; j < N2; j++) {
+#pragma unroll
+                for (int r = 0; r < NO; r++) o[r] = fma(MT[j * N2 + h * NO + r], v[j], o[r]);
+            }
+        } else {
+            // N = 12: one input at a time into the 2N accumulators, which
+            // keeps the kernel free of spills
 #pragma unroll 1
-        for (int j = 0; j < N2; j++) {
-            const double vj = blk[(j < N ? t0 : t1) * N3 + post_index<N, DIM>(a, c, j < N ? j : j - N)];
+            for (int j = 0; j < N2; j++) {
+                const double vj = blk[(j < N ? t0 : t1) * N3 + post_index<N, DIM>(a, c, j < N ? j : j - N)];
 #pragma unroll
-            for (int r = 0; r < N2; r++) o[r] = fma(MT[j * N2 + r], vj, o[r]);
+                for (int r = 0; r < NO; r++) o[r] = fma(MT[j * N2 + r], vj, o[r]);
+            }
         }
-        // the line's inputs were all read first, so its outputs go straight
-        // back into its own places
+    }
+    // with two threads per line, every input of the pass is read before
+    // any output is written back in its place
+    if constexpr (H > 1) __syncthreads();
+    if (live) {
+        if constexpr (H == 1) {
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            blk[t0 * N3 + post_index<N, DIM>(a, c, i)] = o[i];
-            blk[t1 * N3 + post_index<N, DIM>(a, c, i)] = o[N + i];
+            for (int i = 0; i < N; i++) {
+                blk[t0 * N3 + post_index<N, DIM>(a, c, i)] = o[i];
+                blk[t1 * N3 + post_index<N, DIM>(a, c, i)] = o[N + i];
+            }
+        } else {
+            const int t = h ? t1 : t0;
+#pragma unroll
+            for (int i = 0; i < N; i++) blk[t * N3 + post_index<N, DIM>(a, c, i)] = o[i];
         }
     }
 }
@@ -742,7 +785,7 @@ int dev_node_norms(DeviceCtx *c, int kp1, const double *blocks, long long n, dou
     hipStream_t s = static_cast<hipStream_t>(dev_stream(c));
     double *d = nullptr;
     HIP_TRY(hipMalloc((void **)&d, sizeof(double) * 2 * (size_t)n));
-    hipLaunchKernelGGL(k_node_norms, dim3((unsigned)n), dim3(64), 0, s, blocks, n, kp1 * kp1 * kp1, d, d + n);
+    hipLaunchKernelGGL(k_node_norms, dim3((unsigned)n), dim3(256), 0, s, blocks, n, kp1 * kp1 * kp1, d, d + n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host_s2, d, sizeof(double) * n, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(host_w2, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, s);
@@ -833,7 +876,7 @@ int dev_post_launch(void *stream, int kp1, const PostPair *pairs, const std::vec
 int dev_node_norms_launch(void *stream, int kp1, const double *blocks, long long n, double *s2, double *w2,
                           std::string &err) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_node_norms, dim3((unsigned)n), dim3(64), 0, static_cast<hipStream_t>(stream), blocks, n,
+    hipLaunchKernelGGL(k_node_norms, dim3((unsigned)n), dim3(256), 0, static_cast<hipStream_t>(stream), blocks, n,
                        kp1 * kp1 * kp1, s2, w2);
     HIP_TRY(hipGetLastError());
     return 0;
