@@ -467,6 +467,38 @@ __device__ __forceinline__ void post_line_pass(double *blk, const double *MT, in
     }
 }
 
+// Block element e of a branch's 8 N^3 values, for the thread's e in turn:
+// e = tid + k T.  Where a thread has at most 10 of them (N <= 10) the loop is
+// unrolled whole, so that all of a thread's global loads are in flight at
+// once (the loads of one level launch wait on memory only once).
+template <int N>
+struct PostBlock {
+    static constexpr int E = 8 * N * N * N, T = PostShape<N>::T, K = (E + T - 1) / T;
+    static constexpr bool whole = K <= 10;
+};
+
+// blk[e] = src[map(e)] for the block's elements
+template <int N, class Map>
+__device__ __forceinline__ void post_gather(double *blk, const double *src, int tid, Map map) {
+    using B = PostBlock<N>;
+    if constexpr (B::whole) {
+        double v[B::K];
+#pragma unroll
+        for (int k = 0; k < B::K; k++) {
+            const int e = tid + k * B::T;
+            v[k] = e < B::E ? src[map(e)] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < B::K; k++) {
+            const int e = tid + k * B::T;
+            if (e < B::E) blk[e] = v[k];
+        }
+    } else {
+#pragma unroll 8
+        for (int e = tid; e < B::E; e += B::T) blk[e] = src[map(e)];
+    }
+}
+
 // TopDown step for one level: each branch's block reconstructed into its 8
 // children's scaling parts (directions z, y, x), added to theirs.
 template <int N>
@@ -480,8 +512,7 @@ __global__ __launch_bounds__(PostShape<N>::T) void k_topdown(const PostPair *pai
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
         // unrolled, so that several global loads are in flight per thread
-#pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) blk[e] = pr.node[e];
+        post_gather<N>(blk, pr.node, tid, [](int e) { return (size_t)e; });
         __syncthreads();
         post_line_pass<N, 2>(blk, M, tid);
         __syncthreads();
@@ -489,8 +520,25 @@ __global__ __launch_bounds__(PostShape<N>::T) void k_topdown(const PostPair *pai
         __syncthreads();
         post_line_pass<N, 0>(blk, M, tid);
         __syncthreads();
+        // the children's scaling parts += the reconstruction: every load
+        // of the thread first, then the stores
+        if constexpr (PostBlock<N>::whole) {
+            using B = PostBlock<N>;
+            double v[B::K];
+#pragma unroll
+            for (int k = 0; k < B::K; k++) {
+                const int e = tid + k * B::T;
+                v[k] = e < B::E ? pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < B::K; k++) {
+                const int e = tid + k * B::T;
+                if (e < B::E) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] = v[k] + blk[e];
+            }
+        } else {
 #pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] += blk[e];
+            for (int e = tid; e < 8 * N3; e += PostShape<N>::T) pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3] += blk[e];
+        }
         __syncthreads();
     }
 }
@@ -507,8 +555,7 @@ __global__ __launch_bounds__(PostShape<N>::T) void k_compress(const PostPair *pa
     for (int e = tid; e < N2 * N2; e += PostShape<N>::T) M[e] = F[(e % N2) * N2 + e / N2];
     for (int w = blockIdx.x; w < n_pairs; w += gridDim.x) {
         const PostPair pr = pairs[w];
-#pragma unroll 8
-        for (int e = tid; e < 8 * N3; e += PostShape<N>::T) blk[e] = pr.child0[(size_t)(e / N3) * 8 * N3 + e % N3];
+        post_gather<N>(blk, pr.child0, tid, [](int e) { return (size_t)(e / N3) * 8 * N3 + e % N3; });
         __syncthreads();
         post_line_pass<N, 0>(blk, M, tid);
         __syncthreads();
