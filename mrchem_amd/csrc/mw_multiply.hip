@@ -569,9 +569,121 @@ __global__ __launch_bounds__(PostShape<N>::T) void k_compress(const PostPair *pa
     }
 }
 
+// Levels with few branches: one workgroup per (branch, output block), so
+// that a branch's three passes spread over 8 CUs.  TopDown: output block ch
+// is child ch's reconstructed scaling part (added into it); BottomUp: output
+// block t is component t of the branch's [s, d1..d7].  Each pass keeps only
+// the half of its outputs that leads to the workgroup's block: the first
+// pass reads its lines from global memory and leaves 4 N^3 values in LDS,
+// the second 2 N^3, the third N^3, which go straight to global memory.
+// Every value kept is the same FMA sequence as in k_topdown / k_compress,
+// so the results are the same to the bit.  The 8 workgroups of a branch
+// have equal blockIdx modulo 8, so they run on one XCD and share its L2:
+// blockIdx = 64 (p / 8) + 8 ch + p % 8 for branch p and block ch.
+template <int N>
+struct PostOneShape {
+    static constexpr int T = 64 * ((4 * N * N + 63) / 64);  // one thread per line of the first pass
+    static constexpr int LDS = (4 * N * N * N + 4 * N * N) * 8;
+    // workgroups resident per CU by LDS (at most 4), and the largest level
+    // that then fits the 256 CUs in one round
+    static constexpr int PER_CU = (160 * 1024 / LDS) < 4 ? (160 * 1024 / LDS) : 4;
+    static constexpr int MAX_PAIRS = 256 * PER_CU / 8;
+};
+
+// o[i] = sum over j of MT[j][h N + i] v[j], j = 0 .. 2N - 1 in turn
+template <int N>
+__device__ __forceinline__ void post_half_line(const double *v, const double *MT, int h, double *o) {
+    constexpr int N2 = 2 * N;
+#pragma unroll
+    for (int i = 0; i < N; i++) o[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < N2; j++) {
+#pragma unroll
+        for (int i = 0; i < N; i++) o[i] = fma(MT[j * N2 + h * N + i], v[j], o[i]);
+    }
+}
+
+template <int N, bool DOWN>
+__global__ __launch_bounds__(PostOneShape<N>::T) void k_post_one(const PostPair *pairs, int n_pairs,
+                                                              const double *__restrict__ F) {
+    constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N;
+    __shared__ double X[4 * N3];  // slots of the first pass's outputs
+    __shared__ double M[N2 * N2];
+    const int tid = threadIdx.x, b = blockIdx.x;
+    const int p = (b >> 6) * 8 + (b & 7), ob = (b >> 3) & 7;  // branch, output block
+    if (p >= n_pairs) return;
+    // TopDown reads F as is, BottomUp its transpose (as k_topdown / k_compress)
+    for (int e = tid; e < N2 * N2; e += PostOneShape<N>::T) M[e] = DOWN ? F[e] : F[(e % N2) * N2 + e / N2];
+    const PostPair pr = pairs[p];
+    const int hx = ob & 1, hy = (ob >> 1) & 1, hz = ob >> 2;
+    double v[N2], o[N];
+    // pass 1 from global memory: TopDown along z over the branch's 8 blocks
+    // (slot s = (ty, tx), blocks s and s | 4), BottomUp along x over the
+    // children's scaling parts (slot s = (cz, cy), children 2s and 2s + 1)
+    const bool l1 = tid < 4 * NN;
+    const int s1 = tid / NN, ac = tid % NN, a = ac % N, c = ac / N;
+    if (l1) {
+#pragma unroll
+        for (int j = 0; j < N2; j++) {
+            const int jj = j < N ? j : j - N;
+            if constexpr (DOWN) v[j] = pr.node[(j < N ? s1 : s1 | 4) * N3 + post_index<N, 2>(a, c, jj)];
+            else v[j] = pr.child0[(size_t)(j < N ? 2 * s1 : 2 * s1 + 1) * 8 * N3 + post_index<N, 0>(a, c, jj)];
+        }
+    }
+    __syncthreads();  // M in LDS
+    if (l1) {
+        post_half_line<N>(v, M, DOWN ? hz : hx, o);
+#pragma unroll
+        for (int i = 0; i < N; i++) X[s1 * N3 + (DOWN ? post_index<N, 2>(a, c, i) : post_index<N, 0>(a, c, i))] = o[i];
+    }
+    __syncthreads();
+    // pass 2 along y: TopDown over slots (tx, tx | 2) into tx | 2 hy,
+    // BottomUp over slots (2 cz, 2 cz + 1) into 2 cz + hy
+    if (tid < 2 * NN) {
+        const int q = tid / NN;
+        const int s0 = DOWN ? q : 2 * q, s1b = DOWN ? (q | 2) : (2 * q + 1);
+#pragma unroll
+        for (int j = 0; j < N2; j++) v[j] = X[(j < N ? s0 : s1b) * N3 + post_index<N, 1>(a, c, j < N ? j : j - N)];
+        post_half_line<N>(v, M, hy, o);
+        const int so = DOWN ? (q | (hy << 1)) : (2 * q + hy);
+#pragma unroll
+        for (int i = 0; i < N; i++) X[so * N3 + post_index<N, 1>(a, c, i)] = o[i];
+    }
+    __syncthreads();
+    // pass 3: TopDown along x over slots (2 hy, 2 hy + 1), added into child
+    // ob's scaling part; BottomUp along z over slots (hy, hy | 2), stored as
+    // block ob of the branch's node
+    if (tid < NN) {
+        const int s0 = DOWN ? 2 * hy : hy, s1b = DOWN ? 2 * hy + 1 : (hy | 2);
+        constexpr int D3 = DOWN ? 0 : 2;
+#pragma unroll
+        for (int j = 0; j < N2; j++) v[j] = X[(j < N ? s0 : s1b) * N3 + post_index<N, D3>(a, c, j < N ? j : j - N)];
+        post_half_line<N>(v, M, DOWN ? hx : hz, o);
+        if constexpr (DOWN) {
+            double *dst = pr.child0 + (size_t)ob * 8 * N3;
+            double old[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) old[i] = dst[post_index<N, 0>(a, c, i)];
+#pragma unroll
+            for (int i = 0; i < N; i++) dst[post_index<N, 0>(a, c, i)] = old[i] + o[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; i++) pr.node[ob * N3 + post_index<N, 2>(a, c, i)] = o[i];
+        }
+    }
+}
+
 // One level of the TopDown (down) or BottomUp sequence.
 template <int N>
 static hipError_t post_level_n(hipStream_t s, bool down, const PostPair *pairs, int cnt, const double *F) {
+    if (cnt <= PostOneShape<N>::MAX_PAIRS) {
+        const unsigned g = (unsigned)((cnt + 7) / 8) * 64;
+        if (down)
+            hipLaunchKernelGGL((k_post_one<N, true>), dim3(g), dim3(PostOneShape<N>::T), 0, s, pairs, cnt, F);
+        else
+            hipLaunchKernelGGL((k_post_one<N, false>), dim3(g), dim3(PostOneShape<N>::T), 0, s, pairs, cnt, F);
+        return hipGetLastError();
+    }
     const int g = std::min(cnt, kMulBlocks);
     if (down)
         hipLaunchKernelGGL((k_topdown<N>), dim3(g), dim3(PostShape<N>::T), 0, s, pairs, cnt, F);
