@@ -2291,8 +2291,8 @@ static int enqueue_step(DevicePlan *p, int mode, double tau, bool post, bool tim
         // the other take less wall time than the FP64 class on the second
         // stream beside the FP32 class: run side by side, the two grids share
         // the CUs and the FP64 class (a few percent of the term-apps) ends
-        // with the FP32 class instead of before it (C5, k + 1 = 10: 316 ms
-        // against 394 ms a step).
+        // with the FP32 class instead of before it (64 Helmholtz applies at
+        // k = 11, 119,968 output nodes: 316 ms against 394 ms a step).
         if (timed) HIP_TRY(hipEventRecord(c->ev2s, c->stream));
         rc = launch_t<double>(p, p->pool, false, p->list_hi, 0, p->n_hi, tau, inf, 0, head2, err, c->stream);
         if (rc) return rc;

@@ -394,11 +394,11 @@ __global__ __launch_bounds__(64) void k_component_norms(const double *blocks, lo
 // Threads per workgroup: two per line of a pass where the 8 N^2 of them fit
 // a workgroup (N <= 11), each computing the N outputs of one of the line's
 // two components, else one per line (N = 12), in whole waves, so that each
-// pass is one round.  The levels of a step hold fewer branch nodes than the
-// chip has CUs, so each launch lasts as long as one workgroup's pass
-// pipeline: shorter per-thread chains shorten it (the C2 step's post
-// transforms at k = 9: 0.62 ms with 256 threads, 0.42 ms with one thread
-// per line).
+// pass is one round and each thread's chain of FMAs is short (one Helmholtz
+// apply at k = 9, 2688 output nodes, when every level ran this kernel: post
+// transforms 0.62 ms with 256 threads, 0.42 ms with one thread per line,
+// 0.36 ms with two).  These kernels take the levels with more branches than
+// k_post_one below is used for.
 template <int N>
 struct PostShape {
     static constexpr int H = (8 * N * N <= 1024) ? 2 : 1;  // threads per line
