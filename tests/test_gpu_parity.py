@@ -158,3 +158,45 @@ def test_split16_term_set_and_error(ctx, order, prec, basis):
     c64 = V.nodes()[2]
     assert np.max(np.abs(ref - c64)) / np.max(np.abs(ref)) < TOL_FP64
     assert st["term_apps"] == r64["term_apps"]
+
+
+@pytest.mark.parametrize("order", [1, 9])
+def test_post_transforms_on_wide_levels(ctx, order):
+    """A frozen grid whose root box around the density is complete to depth
+    4 (the other root boxes are leaves): its levels hold 1, 8, 64 and 512
+    branches.  Levels of up to 128 branches run the post transforms as one
+    workgroup per (branch, output block), the 512-branch level as one
+    workgroup per branch, so both TopDown/BottomUp kernels meet in one tree;
+    the post-transformed FP64 result equals the oracle's on the same grid and
+    thresholds."""
+    mra = _mra(order)
+    prec = 1e-4
+    P = mw.PoissonOperator(mra, prec)
+    cen = np.array([0.3, -0.2, 0.1])
+    rho = mw.FunctionTree(mra).project_gaussians(prec, [1.0], [10.0], cen[None, :])
+    d0, l0, _ = rho.nodes()
+    roots = l0[d0 == 0]
+    # the root box holding the centre (root boxes are 2^4 x 1.25 = 20 wide
+    # here); any root would do, this one carries the largest values
+    want = np.floor(cen / 20.0).astype(np.int64)
+    hit = np.where(np.all(roots == want, axis=1))[0]
+    box = roots[hit[0]] if len(hit) else roots[0]
+    depth, keys = [0] * len(roots), [np.asarray(roots)]
+    for d in range(1, 5):
+        g = np.stack(np.meshgrid(*[np.arange(2 ** d)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        keys.append(box[None, :] * 2 ** d + g)
+        depth += [d] * len(g)
+    od, ol = np.array(depth, dtype=np.int32), np.concatenate(keys).astype(np.int64)
+    assert len(od) == len(roots) + 8 + 64 + 512 + 4096
+    V = mw.FunctionTree(mra).set_nodes(od, ol)
+    t = 1e-3 * rho.norm()
+    plan = mw.ApplyPlan(V, P, rho, np.full(len(od), t), mode="fp64", ctx=ctx)
+    r = plan.execute()
+    plan.finish()
+    vd, vl, vc = V.nodes()
+    ref, st = oracle.apply_fixed(mra, P.tables(), rho.nodes(), vd, vl, np.full(len(vd), t))
+    assert st["term_apps"] == r["term_apps"] > 0
+    err = np.max(np.abs(ref - vc)) / np.max(np.abs(ref))
+    print("order %d: %d nodes, %d term-apps, post-transformed FP64 vs oracle %.2e" %
+          (order, len(vd), r["term_apps"], err))
+    assert err < TOL_FP64
