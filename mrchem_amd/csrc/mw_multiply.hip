@@ -398,7 +398,8 @@ __global__ __launch_bounds__(64) void k_component_norms(const double *blocks, lo
 // apply at k = 9, 2688 output nodes, when every level ran this kernel: post
 // transforms 0.62 ms with 256 threads, 0.42 ms with one thread per line,
 // 0.36 ms with two).  These kernels take the levels with more branches than
-// k_post_one below is used for.
+// k_post_one below is used for at N <= 10; at N = 11, 12 k_post_two takes
+// them.
 template <int N>
 struct PostShape {
     static constexpr int H = (8 * N * N <= 1024) ? 2 : 1;  // threads per line
@@ -673,6 +674,97 @@ __global__ __launch_bounds__(PostOneShape<N>::T) void k_post_one(const PostPair 
     }
 }
 
+// Wide levels at N >= 11, where k_topdown / k_compress hold 8 N^3 values in
+// LDS and so run one workgroup per CU: one workgroup per (branch, half h of
+// the first pass's outputs).  The first pass reads its lines from global
+// memory and keeps half h (4 N^3 values in LDS: half the LDS, two or three
+// workgroups per CU), the second and third passes run whole on those, two
+// threads per line, and the workgroup's 4 output blocks go to global memory
+// (TopDown: the children with bit h along z, added into their scaling parts;
+// BottomUp: the node's components with bit h along x).  No FMA is added, and
+// every value is the same FMA sequence as in k_topdown / k_compress.  The two
+// workgroups of a branch share an XCD: blockIdx = 16 (p / 8) + 8 h + p % 8.
+template <int N>
+struct PostTwoShape {
+    static constexpr int T = 64 * ((4 * N * N + 63) / 64);
+};
+
+// A whole pass along DIM over slot pairs (s0, s0 | sbit), s0 = q << qshift
+// for q = 0, 1: two threads per line, each the N outputs of one slot.
+template <int N, int DIM>
+__device__ __forceinline__ void post_slot_pass(double *X, const double *MT, int tid, int sbit, int qshift) {
+    constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N;
+    const bool live = tid < 4 * NN;
+    const int L = live ? tid >> 1 : 0, h = tid & 1;
+    const int q = L / NN, ac = L % NN, a = ac % N, c = ac / N;
+    const int s0 = q << qshift, s1 = s0 | sbit;
+    double v[N2], o[N];
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < N2; j++) v[j] = X[(j < N ? s0 : s1) * N3 + post_index<N, DIM>(a, c, j < N ? j : j - N)];
+        post_half_line<N>(v, MT, h, o);
+    }
+    __syncthreads();  // both slots of every line read before any is written
+    if (live) {
+        const int so = h ? s1 : s0;
+#pragma unroll
+        for (int i = 0; i < N; i++) X[so * N3 + post_index<N, DIM>(a, c, i)] = o[i];
+    }
+}
+
+template <int N, bool DOWN>
+__global__ __launch_bounds__(PostTwoShape<N>::T) void k_post_two(const PostPair *pairs, int n_pairs,
+                                                              const double *__restrict__ F) {
+    constexpr int N3 = N * N * N, NN = N * N, N2 = 2 * N;
+    __shared__ double X[4 * N3];
+    __shared__ double M[N2 * N2];
+    const int tid = threadIdx.x, b = blockIdx.x;
+    const int p = (b >> 4) * 8 + (b & 7), h0 = (b >> 3) & 1;
+    if (p >= n_pairs) return;
+    for (int e = tid; e < N2 * N2; e += PostTwoShape<N>::T) M[e] = DOWN ? F[e] : F[(e % N2) * N2 + e / N2];
+    const PostPair pr = pairs[p];
+    // pass 1 from global memory, as in k_post_one: TopDown along z (slot
+    // s = (ty, tx), blocks s and s | 4), BottomUp along x (slot s = (cz, cy),
+    // children 2s and 2s + 1); half h0 kept
+    double v[N2], o[N];
+    const bool l1 = tid < 4 * NN;
+    const int s1 = tid / NN, ac = tid % NN, a = ac % N, c = ac / N;
+    if (l1) {
+#pragma unroll
+        for (int j = 0; j < N2; j++) {
+            const int jj = j < N ? j : j - N;
+            if constexpr (DOWN) v[j] = pr.node[(j < N ? s1 : s1 | 4) * N3 + post_index<N, 2>(a, c, jj)];
+            else v[j] = pr.child0[(size_t)(j < N ? 2 * s1 : 2 * s1 + 1) * 8 * N3 + post_index<N, 0>(a, c, jj)];
+        }
+    }
+    __syncthreads();  // M in LDS
+    if (l1) {
+        post_half_line<N>(v, M, h0, o);
+#pragma unroll
+        for (int i = 0; i < N; i++) X[s1 * N3 + (DOWN ? post_index<N, 2>(a, c, i) : post_index<N, 0>(a, c, i))] = o[i];
+    }
+    __syncthreads();
+    if constexpr (DOWN) {
+        post_slot_pass<N, 1>(X, M, tid, 2, 0);  // y: slots (tx, tx | 2)
+        __syncthreads();
+        post_slot_pass<N, 0>(X, M, tid, 1, 1);  // x: slots (2 ty, 2 ty + 1)
+        __syncthreads();
+        // slot s = (ty, tx) is child s | 4 h0: added into its scaling part
+        for (int e = tid; e < 4 * N3; e += PostTwoShape<N>::T) {
+            double *dst = pr.child0 + (size_t)((e / N3) | (h0 << 2)) * 8 * N3 + e % N3;
+            *dst = *dst + X[e];
+        }
+    } else {
+        post_slot_pass<N, 1>(X, M, tid, 1, 1);  // y: slots (2 cz, 2 cz + 1)
+        __syncthreads();
+        post_slot_pass<N, 2>(X, M, tid, 2, 0);  // z: slots (ty, ty | 2)
+        __syncthreads();
+        // slot s = (tz, ty) is component 2 s + h0 of the node
+        for (int e = tid; e < 4 * N3; e += PostTwoShape<N>::T)
+            pr.node[(size_t)(2 * (e / N3) + h0) * N3 + e % N3] = X[e];
+    }
+}
+
 // One level of the TopDown (down) or BottomUp sequence.
 template <int N>
 static hipError_t post_level_n(hipStream_t s, bool down, const PostPair *pairs, int cnt, const double *F) {
@@ -682,6 +774,14 @@ static hipError_t post_level_n(hipStream_t s, bool down, const PostPair *pairs, 
             hipLaunchKernelGGL((k_post_one<N, true>), dim3(g), dim3(PostOneShape<N>::T), 0, s, pairs, cnt, F);
         else
             hipLaunchKernelGGL((k_post_one<N, false>), dim3(g), dim3(PostOneShape<N>::T), 0, s, pairs, cnt, F);
+        return hipGetLastError();
+    }
+    if constexpr (N >= 11) {
+        const unsigned g = (unsigned)((cnt + 7) / 8) * 16;
+        if (down)
+            hipLaunchKernelGGL((k_post_two<N, true>), dim3(g), dim3(PostTwoShape<N>::T), 0, s, pairs, cnt, F);
+        else
+            hipLaunchKernelGGL((k_post_two<N, false>), dim3(g), dim3(PostTwoShape<N>::T), 0, s, pairs, cnt, F);
         return hipGetLastError();
     }
     const int g = std::min(cnt, kMulBlocks);

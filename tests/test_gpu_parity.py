@@ -160,14 +160,15 @@ def test_split16_term_set_and_error(ctx, order, prec, basis):
     assert st["term_apps"] == r64["term_apps"]
 
 
-@pytest.mark.parametrize("order", [1, 9])
+@pytest.mark.parametrize("order", [1, 9, 10, 11])
 def test_post_transforms_on_wide_levels(ctx, order):
     """A frozen grid whose root box around the density is complete to depth
     4 (the other root boxes are leaves): its levels hold 1, 8, 64 and 512
-    branches.  Levels of up to 128 branches run the post transforms as one
-    workgroup per (branch, output block), the 512-branch level as one
-    workgroup per branch, so both TopDown/BottomUp kernels meet in one tree;
-    the post-transformed FP64 result equals the oracle's on the same grid and
+    branches.  Levels of up to 128 branches (64 at k = 11) run the post
+    transforms as one workgroup per (branch, output block), the 512-branch
+    level as one workgroup per branch (k <= 9) or per (branch, first-pass
+    half) (k = 10, 11), so two TopDown/BottomUp kernels meet in one tree; the
+    post-transformed FP64 result equals the oracle's on the same grid and
     thresholds."""
     mra = _mra(order)
     prec = 1e-4
